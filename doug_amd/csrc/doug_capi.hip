@@ -1,0 +1,911 @@
+// C-ABI of the MI355X Schwarz-PCG solve path (include/doug.h).
+//
+// Host side: ingest, aggregation, index sets and the per-subdomain scatter
+// maps (doug_setup.cpp).  Device side: every factorisation and every
+// per-iteration operator (doug_kernels.hip).  There is no CPU fallback: a
+// missing GPU is an error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/doug.h"
+#include "doug_kernels.hpp"
+#include "doug_setup.hpp"
+
+using namespace doug;
+
+namespace {
+
+enum { S_RHO = 0, S_RHO_PREV = 1, S_BETA = 2, S_PQ = 3, S_ALPHA = 4, S_RES = 5, S_N = 8 };
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct doug_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Params p;
+  doug_info info{};
+
+  // host problem
+  bool have_matrix = false, ready = false, host_ready = false;
+  Csr A;                // values after the scale/unscale round trip once set up
+  std::vector<double> A_input_val;
+  std::vector<int> part_starts, part_nodes;
+  bool have_partition = false;
+  ProcessState ps;
+  Aggregates fin, ffull, cin, cfull;
+  Csr R, Rt, AC;
+  int ol = 0, smoothers = 0;
+  std::vector<int> sstarts, sinds;
+  bool has_coarse = false;
+
+  // device: matrix
+  std::vector<void*> allocs;
+  int32_t *d_ptr = nullptr, *d_col = nullptr, *d_rowblk = nullptr;
+  double* d_val = nullptr;
+  int nrb = 0;
+  // restriction (CSR rows = aggregates), prolongation (CSR rows = nodes)
+  int32_t *d_Rptr = nullptr, *d_Rcol = nullptr, *d_Rrowblk = nullptr;
+  double* d_Rval = nullptr;
+  int nrbR = 0;
+  int32_t *d_Rtptr = nullptr, *d_Rtcol = nullptr;
+  double* d_Rtval = nullptr;
+  // subdomains
+  int nsub = 0;
+  BlockDesc* d_sub = nullptr;
+  std::vector<BlockDesc> h_sub;
+  double* d_tiles = nullptr;
+  int32_t *d_inds = nullptr, *d_occ_ptr = nullptr, *d_occ_pos = nullptr;
+  double* d_subsol = nullptr;
+  // coarse
+  int nc = 0, cnt = 0;
+  BlockDesc* d_cdesc = nullptr;
+  double *d_ctiles = nullptr, *d_cW = nullptr;
+  double *d_crhs = nullptr, *d_cu = nullptr, *d_csol = nullptr;
+  // PCG
+  int n = 0;
+  double *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr;
+  double *d_b = nullptr, *d_scal = nullptr, *d_partial = nullptr, *d_tmp = nullptr;
+  unsigned* d_counter = nullptr;
+  int* d_flag = nullptr;
+  double* h_scal = nullptr;  // pinned
+};
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+      return DOUG_E_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+static int fail(doug_ctx* ctx, int code, const std::string& msg) {
+  ctx->err = msg;
+  return code;
+}
+
+template <class T>
+static int dalloc(doug_ctx* ctx, T** out, size_t count) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
+    return DOUG_E_NOMEM;
+  }
+  ctx->allocs.push_back(p);
+  *out = (T*)p;
+  return DOUG_OK;
+}
+
+template <class T>
+static int upload(doug_ctx* ctx, T** out, const std::vector<T>& h) {
+  int rc = dalloc(ctx, out, h.size());
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(hipMemcpyAsync(*out, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return DOUG_OK;
+}
+
+static void free_device(doug_ctx* ctx) {
+  for (void* p : ctx->allocs) (void)hipFree(p);
+  ctx->allocs.clear();
+  ctx->ready = false;
+}
+
+// CSR-stream row blocks: consecutive rows with at most kSpmvThreads rows and
+// kSpmvChunk nonzeros; a longer row gets a block of its own.
+static std::vector<int32_t> row_blocks(const Csr& M) {
+  std::vector<int32_t> rb{0};
+  int r = 0;
+  while (r < M.nrows) {
+    int r1 = r;
+    int64_t nz = 0;
+    while (r1 < M.nrows && r1 - r < kSpmvThreads) {
+      int64_t rn = M.ptr[r1 + 1] - M.ptr[r1];
+      if (nz + rn > kSpmvChunk && r1 > r) break;
+      nz += rn;
+      r1++;
+      if (nz > kSpmvChunk) break;
+    }
+    rb.push_back(r1);
+    r = r1;
+  }
+  return rb;
+}
+
+static int64_t tile_count(int nt) { return (int64_t)nt * (nt + 1) / 2; }
+static int64_t tile_pos(int li, int lj) {  // element offset of (li, lj), li >= lj, in a block
+  int ti = li / kTile, tj = lj / kTile;
+  return (tile_count(ti) + tj) * kTileElems + (int64_t)(li % kTile) * kTile + (lj % kTile);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void doug_params_default(doug_params* p) {
+  Params d;
+  p->radius1 = d.radius1;
+  p->radius2 = d.radius2;
+  p->minasize1 = d.minasize1;
+  p->minasize2 = d.minasize2;
+  p->maxasize1 = d.maxasize1;
+  p->maxasize2 = d.maxasize2;
+  p->strong1 = d.strong1;
+  p->strong2 = d.strong2;
+  p->smoothers = d.smoothers;
+  p->overlap = d.overlap;
+  p->levels = d.levels;
+  p->method = d.method;
+  p->fine_method = d.fine_method;
+  p->symmstruct = d.symmstruct;
+  p->symmnumeric = d.symmnumeric;
+  p->omega = d.omega;
+}
+
+int doug_ctx_create(int device, doug_ctx** out) {
+  if (!out) return DOUG_E_ARG;
+  *out = nullptr;
+  if (device == -1) {  // host-only context: setup products, no GPU work
+    doug_ctx* ctx = new doug_ctx();
+    ctx->device = -1;
+    *out = ctx;
+    return DOUG_OK;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return DOUG_E_HIP;  // no GPU: there is no CPU path
+  }
+  if (device < 0 || device >= ndev) return DOUG_E_ARG;
+  doug_ctx* ctx = new doug_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    delete ctx;
+    return DOUG_E_HIP;
+  }
+  *out = ctx;
+  return DOUG_OK;
+}
+
+int doug_ctx_destroy(doug_ctx* ctx) {
+  if (!ctx) return DOUG_OK;
+  if (ctx->device < 0) {
+    delete ctx;
+    return DOUG_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_device(ctx);
+  if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DOUG_OK;
+}
+
+const char* doug_last_error(const doug_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int doug_set_params(doug_ctx* ctx, const doug_params* q) {
+  if (!ctx || !q) return DOUG_E_ARG;
+  Params& p = ctx->p;
+  p.radius1 = q->radius1;
+  p.radius2 = q->radius2;
+  p.minasize1 = q->minasize1;
+  p.minasize2 = q->minasize2;
+  p.maxasize1 = q->maxasize1;
+  p.maxasize2 = q->maxasize2;
+  p.strong1 = q->strong1;
+  p.strong2 = q->strong2;
+  p.smoothers = q->smoothers;
+  p.overlap = q->overlap;
+  p.levels = q->levels;
+  p.method = q->method;
+  p.fine_method = q->fine_method;
+  p.symmstruct = q->symmstruct;
+  p.symmnumeric = q->symmnumeric;
+  p.omega = q->omega;
+  ctx->ready = false;
+  return DOUG_OK;
+}
+
+int doug_set_matrix_coo(doug_ctx* ctx, int32_t n, int64_t nnz, const int32_t* indi,
+                        const int32_t* indj, const double* val, int32_t base) {
+  if (!ctx) return DOUG_E_ARG;
+  if (n <= 0 || nnz <= 0 || !indi || !indj || !val || (base != 0 && base != 1))
+    return fail(ctx, DOUG_E_ARG, "doug_set_matrix_coo: bad arguments");
+  if (nnz > INT32_MAX) return fail(ctx, DOUG_E_UNSUPPORTED, "nnz above 2^31-1 is not supported");
+  std::string err;
+  Csr A = coo_to_csr(n, nnz, indi, indj, val, base, err);
+  if (!err.empty()) return fail(ctx, DOUG_E_ARG, err);
+  ctx->A = std::move(A);
+  ctx->A_input_val = ctx->A.val;
+  ctx->n = n;
+  ctx->have_matrix = true;
+  ctx->ready = false;
+  return DOUG_OK;
+}
+
+int doug_set_partition(doug_ctx* ctx, int32_t nparts, const int32_t* starts, const int32_t* nodes,
+                       int32_t base) {
+  if (!ctx) return DOUG_E_ARG;
+  if (nparts <= 0) {
+    ctx->have_partition = false;
+    ctx->part_starts.clear();
+    ctx->part_nodes.clear();
+    return DOUG_OK;
+  }
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "doug_set_partition before doug_set_matrix_coo");
+  if (!starts || !nodes || starts[0] != base) return fail(ctx, DOUG_E_ARG, "bad partition");
+  int64_t tot = (int64_t)starts[nparts] - base;
+  if (tot != ctx->n) return fail(ctx, DOUG_E_ARG, "partition must list every unknown once");
+  std::vector<char> seen((size_t)ctx->n, 0);
+  ctx->part_starts.assign((size_t)nparts + 1, 0);
+  ctx->part_nodes.resize((size_t)tot);
+  for (int i = 0; i <= nparts; i++) {
+    ctx->part_starts[i] = starts[i] - base;
+    if (i > 0 && ctx->part_starts[i] < ctx->part_starts[i - 1]) return fail(ctx, DOUG_E_ARG, "partition starts must be non-decreasing");
+  }
+  for (int64_t t = 0; t < tot; t++) {
+    int v = nodes[t] - base;
+    if (v < 0 || v >= ctx->n || seen[v]) return fail(ctx, DOUG_E_ARG, "partition must list every unknown once");
+    seen[v] = 1;
+    ctx->part_nodes[t] = v;
+  }
+  ctx->have_partition = true;
+  ctx->ready = false;
+  return DOUG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Dense tile storage for a set of index blocks: descriptors, scatter map and
+// factorisation on the device.
+static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& starts,
+                        const std::vector<int>& inds, bool one_block_solve,
+                        std::vector<BlockDesc>& desc, BlockDesc** d_desc, double** d_tiles,
+                        int64_t* total_elems) {
+  const int nb = (int)starts.size() - 1;
+  desc.resize((size_t)nb);
+  int64_t base = 0;
+  for (int b = 0; b < nb; b++) {
+    int s = starts[b + 1] - starts[b];
+    int nt = (s + kTile - 1) / kTile;
+    if (nt > kMaxTrsvTiles && one_block_solve)
+      return fail(ctx, DOUG_E_UNSUPPORTED,
+                  "subdomain of " + std::to_string(s) + " unknowns exceeds the one-block solve limit " +
+                      std::to_string(kMaxTrsvTiles * kTile));
+    desc[b] = BlockDesc{base, (int64_t)starts[b], nt, s};
+    base += tile_count(nt) * kTileElems;
+  }
+  *total_elems = base;
+  // scatter map: lower-triangle entries of M(inds, inds)
+  std::vector<int64_t> dst;
+  std::vector<int32_t> src;
+  std::vector<int> floc((size_t)M.nrows, -1);
+  for (int b = 0; b < nb; b++) {
+    const int s0 = starts[b], s = starts[b + 1] - s0;
+    for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
+    for (int li = 0; li < s; li++) {
+      int g = inds[s0 + li];
+      size_t row_begin = dst.size();
+      for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
+        int lj = floc[M.col[p]];
+        if (lj < 0 || lj > li) continue;
+        dst.push_back(desc[b].base + tile_pos(li, lj));
+        src.push_back(p);
+      }
+      // duplicate (i,j) pairs would race in the scatter
+      if (dst.size() - row_begin > 1) {
+        std::vector<int64_t> chk(dst.begin() + row_begin, dst.end());
+        std::sort(chk.begin(), chk.end());
+        if (std::adjacent_find(chk.begin(), chk.end()) != chk.end())
+          return fail(ctx, DOUG_E_UNSUPPORTED, "duplicate matrix entries inside a subdomain are not supported");
+      }
+    }
+    for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
+  }
+  int rc;
+  if ((rc = dalloc(ctx, d_tiles, (size_t)base))) return rc;
+  if ((rc = upload(ctx, d_desc, desc))) return rc;
+  int64_t* d_dst;
+  int32_t* d_src;
+  if ((rc = upload(ctx, &d_dst, dst))) return rc;
+  if ((rc = upload(ctx, &d_src, src))) return rc;
+  double* d_vals;
+  if ((rc = upload(ctx, &d_vals, M.val))) return rc;
+  HIPCHK(hipMemsetAsync(*d_tiles, 0, (size_t)base * sizeof(double), ctx->stream));
+  int maxpad = 0;
+  for (auto& d : desc) maxpad = std::max(maxpad, d.nt * kTile - d.s);
+  launch_pad_identity(*d_tiles, *d_desc, nb, maxpad, ctx->stream);
+  launch_scatter_entries(*d_tiles, d_dst, d_src, d_vals, (int64_t)dst.size(), ctx->stream);
+  HIPCHK(hipGetLastError());
+  return DOUG_OK;
+}
+
+static int factor_blocks(doug_ctx* ctx, double* tiles, const BlockDesc* d_desc,
+                         const std::vector<BlockDesc>& desc) {
+  int ntmax = 0;
+  for (auto& d : desc) ntmax = std::max(ntmax, d.nt);
+  HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
+  for (int j = 0; j < ntmax; j++) {
+    launch_chol_diag(tiles, d_desc, (int)desc.size(), j, ctx->d_flag, ctx->stream);
+    launch_chol_offdiag(tiles, d_desc, (int)desc.size(), j, ntmax - j - 1, ctx->stream);
+  }
+  HIPCHK(hipGetLastError());
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (flag)
+    return fail(ctx, DOUG_E_NUMERIC,
+                "matrix block " + std::to_string(flag - 1) + " is not positive definite");
+  return DOUG_OK;
+}
+
+static int setup_host(doug_ctx* ctx) {
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "setup before doug_set_matrix_coo");
+  const Params& p = ctx->p;
+  if (p.fine_method != 0 && p.fine_method != 1)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "fine_method must be 0 or 1 on this path");
+  if (p.method != 0 && p.method != 1)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none) or 1 (additive Schwarz)");
+  ctx->ready = false;
+  ctx->host_ready = false;
+  double t0 = now();
+  doug_info& info = ctx->info;
+  info = doug_info{};
+  Csr& A = ctx->A;
+  A.val = ctx->A_input_val;
+  const int n = A.nrows;
+  ctx->ps = ProcessState{};
+
+  // ---- Partitionings_aggr_InitFine (Partitioning_aggr.F90:33-117) ----
+  const bool symm = p.symmstruct && p.symmnumeric;
+  const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
+  const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
+  const int min_asize1 = p.minasize1 > 0 ? p.minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
+  const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
+  std::vector<double> scaled, diag;
+  scale_values(A, symm, scaled, diag);
+  std::vector<uint8_t> strong = find_strong(A, scaled, diag, strong_conn1);
+  aggregate(A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
+            ctx->fin, ctx->ffull);
+  A.val = scaled;
+  unscale_values(A, symm, diag, A.val);  // the reference continues with these values
+  info.t_aggregate = now() - t0;
+
+  // ---- CoarsePreconditioner_smooth_Init (CoarsePreconditioner_smooth.F90:47-89) ----
+  double t1 = now();
+  int smoothers = p.smoothers;
+  if (smoothers > p.radius1) smoothers = p.radius1;  // CoarseMtx.F90:75-78
+  if (smoothers != 0 && smoothers != 1)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "smoothers must resolve to 0 or 1 (got " + std::to_string(smoothers) + ")");
+  ctx->smoothers = smoothers;
+  ctx->R = restriction(A, strong, ctx->fin, smoothers, p.omega);
+  ctx->Rt = transpose(ctx->R);
+  ctx->AC = galerkin(A, ctx->R);
+  ctx->has_coarse = p.levels > 1;
+  info.t_coarse_build = now() - t1;
+
+  // ---- subdomains ----
+  double t2 = now();
+  std::vector<int> pstarts, pnodes;
+  ctx->cin = Aggregates{};
+  ctx->cfull = Aggregates{};
+  if (ctx->have_partition) {
+    pstarts = ctx->part_starts;
+    pnodes = ctx->part_nodes;
+  } else {
+    // Partitionings_aggr_InitCoarse (Partitioning_aggr.F90:120-193)
+    const double strong_conn2 = p.strong2 > 0 ? p.strong2 : strong_conn1 / 2.0;
+    Csr& AC = ctx->AC;
+    std::vector<double> cscaled, cdiag;
+    scale_values(AC, false, cscaled, cdiag);  // SpMtx_AB results are not flagged symmetric
+    std::vector<uint8_t> cstrong = find_strong(AC, cscaled, cdiag, strong_conn2);
+    int r2;
+    if (p.radius2 > 0) {
+      r2 = p.radius2;
+    } else {
+      int nn = (int)std::sqrt(1.0 * (double)n);
+      r2 = (int)std::lround(3 * std::sqrt((double)nn) / (2 * radius1 + 1) - 1);
+    }
+    int min2 = p.minasize2 > 0 ? p.minasize2
+                               : (p.radius2 > 0 ? 2 * p.radius2 + 1 : (int)(0.5 * (double)((2 * r2 + 1) * (2 * r2 + 1))));
+    int max2 = p.maxasize2 > 0 ? p.maxasize2 : (2 * r2 + 1) * (2 * r2 + 1);
+    aggregate(AC, cscaled, cstrong, r2, min2, max2, strong_conn2, true, ctx->ps, ctx->cin, ctx->cfull);
+    AC.val = cscaled;
+    unscale_values(AC, false, cdiag, AC.val);
+    coarse_partition(ctx->cfull, ctx->ffull, n, pstarts, pnodes);
+  }
+  ctx->ol = p.overlap < 0 ? std::max(smoothers, 0) : p.overlap;  // aggr.F90:207-212
+  add_layers_all(A, pstarts, pnodes, ctx->ol, ctx->sstarts, ctx->sinds);
+  ctx->nsub = (int)ctx->sstarts.size() - 1;
+  info.t_subdomains = now() - t2;
+  info.n = n;
+  info.nnz = A.nnz();
+  info.nagr_fine = ctx->fin.nagr;
+  info.nagr_coarse = ctx->cfull.nagr;
+  info.nsub = ctx->nsub;
+  info.ncoarse = ctx->R.nrows;
+  info.overlap = ctx->ol;
+  info.smoothers = ctx->smoothers;
+  info.sub_total = (int64_t)ctx->sinds.size();
+  info.t_total = now() - t0;
+  ctx->host_ready = true;
+  return DOUG_OK;
+}
+
+static int setup_device(doug_ctx* ctx) {
+  HIPCHK(hipSetDevice(ctx->device));
+  free_device(ctx);
+  double t0d = now();
+  const Params& p = ctx->p;
+  doug_info& info = ctx->info;
+  Csr& A = ctx->A;
+  const int n = A.nrows;
+  // ---- device data ----
+  double t3 = now();
+  int rc;
+  std::vector<int32_t> ptr(A.ptr.begin(), A.ptr.end()), col(A.col.begin(), A.col.end());
+  if ((rc = upload(ctx, &ctx->d_ptr, ptr)) || (rc = upload(ctx, &ctx->d_col, col)) ||
+      (rc = upload(ctx, &ctx->d_val, A.val)))
+    return rc;
+  std::vector<int32_t> rb = row_blocks(A);
+  ctx->nrb = (int)rb.size() - 1;
+  if ((rc = upload(ctx, &ctx->d_rowblk, rb))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_flag, 1))) return rc;
+  ctx->nc = ctx->R.nrows;
+  if (ctx->has_coarse) {
+    const Csr& R = ctx->R;
+    if ((rc = upload(ctx, &ctx->d_Rptr, R.ptr)) || (rc = upload(ctx, &ctx->d_Rcol, R.col)) ||
+        (rc = upload(ctx, &ctx->d_Rval, R.val)))
+      return rc;
+    std::vector<int32_t> rbR = row_blocks(R);
+    ctx->nrbR = (int)rbR.size() - 1;
+    if ((rc = upload(ctx, &ctx->d_Rrowblk, rbR))) return rc;
+    const Csr& Rt = ctx->Rt;
+    if ((rc = upload(ctx, &ctx->d_Rtptr, Rt.ptr)) || (rc = upload(ctx, &ctx->d_Rtcol, Rt.col)) ||
+        (rc = upload(ctx, &ctx->d_Rtval, Rt.val)))
+      return rc;
+  }
+  // occurrence lists for the deterministic overlap sum
+  {
+    std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(ctx->sinds.size());
+    if (p.fine_method == 1) {
+      for (int v : ctx->sinds) occ_ptr[v + 1]++;
+      for (int i = 0; i < n; i++) occ_ptr[i + 1] += occ_ptr[i];
+      std::vector<int32_t> fill(occ_ptr.begin(), occ_ptr.end() - 1);
+      for (size_t q = 0; q < ctx->sinds.size(); q++) occ_pos[fill[ctx->sinds[q]]++] = (int32_t)q;
+    } else {  // no fine solves: z starts as r (FinePreconditioner_Apply, type NONE)
+      occ_pos.resize((size_t)n);
+      for (int i = 0; i < n; i++) {
+        occ_ptr[i + 1] = i + 1;
+        occ_pos[i] = i;
+      }
+    }
+    if ((rc = upload(ctx, &ctx->d_occ_ptr, occ_ptr)) || (rc = upload(ctx, &ctx->d_occ_pos, occ_pos)))
+      return rc;
+    std::vector<int32_t> inds(ctx->sinds.begin(), ctx->sinds.end());
+    if ((rc = upload(ctx, &ctx->d_inds, inds))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
+  }
+  for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
+    if ((rc = dalloc(ctx, v, (size_t)n))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) || (rc = dalloc(ctx, &ctx->d_partial, kRedBlocks)) ||
+      (rc = dalloc(ctx, &ctx->d_counter, 1)))
+    return rc;
+  HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_scal, 0, S_N * sizeof(double), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  info.t_upload = now() - t3;
+
+  // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
+  double t4 = now();
+  int64_t elems = 0;
+  if (p.fine_method == 1) {
+    if ((rc = build_blocks(ctx, A, ctx->sstarts, ctx->sinds, true, ctx->h_sub, &ctx->d_sub, &ctx->d_tiles, &elems)))
+      return rc;
+    if ((rc = factor_blocks(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub))) return rc;
+  }
+  info.factor_bytes = elems * (int64_t)sizeof(double);
+  info.t_factor = now() - t4;
+
+  // ---- coarse factorisation and explicit inverse factor ----
+  double t5 = now();
+  if (ctx->has_coarse) {
+    std::vector<int> cst{0, ctx->nc}, cinds((size_t)ctx->nc);
+    for (int i = 0; i < ctx->nc; i++) cinds[i] = i;
+    std::vector<BlockDesc> cdesc;
+    int64_t celems = 0;
+    if ((rc = build_blocks(ctx, ctx->AC, cst, cinds, false, cdesc, &ctx->d_cdesc, &ctx->d_ctiles, &celems)))
+      return rc;
+    if ((rc = factor_blocks(ctx, ctx->d_ctiles, ctx->d_cdesc, cdesc))) return rc;
+    ctx->cnt = cdesc[0].nt;
+    if ((rc = dalloc(ctx, &ctx->d_cW, (size_t)celems))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_cW, ctx->d_ctiles, (size_t)celems * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    for (int dd = 1; dd < ctx->cnt; dd++)
+      launch_trinv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, ctx->cnt, dd, ctx->stream);
+    HIPCHK(hipGetLastError());
+    size_t cpad = (size_t)ctx->cnt * kTile;
+    for (double** v : {&ctx->d_crhs, &ctx->d_cu, &ctx->d_csol}) {
+      if ((rc = dalloc(ctx, v, cpad))) return rc;
+      HIPCHK(hipMemsetAsync(*v, 0, cpad * sizeof(double), ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    info.coarse_factor_bytes = 2 * celems * (int64_t)sizeof(double);
+  }
+  info.t_coarse_factor = now() - t5;
+  info.t_total += now() - t0d;
+  ctx->ready = true;
+  return DOUG_OK;
+}
+
+int doug_setup_host(doug_ctx* ctx) {
+  if (!ctx) return DOUG_E_ARG;
+  return setup_host(ctx);
+}
+
+int doug_setup(doug_ctx* ctx, int32_t precision) {
+  if (!ctx) return DOUG_E_ARG;
+  if (ctx->device < 0) return fail(ctx, DOUG_E_STATE, "host-only context: no GPU work");
+  if (precision != DOUG_PREC_FP64) return fail(ctx, DOUG_E_UNSUPPORTED, "only DOUG_PREC_FP64 is available");
+  int rc = setup_host(ctx);
+  if (rc) return rc;
+  return setup_device(ctx);
+}
+
+int doug_get_info(const doug_ctx* ctx, doug_info* info) {
+  if (!ctx || !info) return DOUG_E_ARG;
+  *info = ctx->info;
+  return DOUG_OK;
+}
+
+static void put_aggr(const Aggregates& a, int32_t* num, int32_t* starts, int32_t* nodes) {
+  if (num)
+    for (size_t i = 0; i < a.num.size(); i++) num[i] = a.num[i];
+  if (starts)
+    for (size_t i = 0; i < a.starts.size(); i++) starts[i] = a.starts[i] + 1;
+  if (nodes)
+    for (size_t i = 0; i < a.nodes.size(); i++) nodes[i] = a.nodes[i] + 1;
+}
+
+int doug_get_fine_aggregates(const doug_ctx* ctx, int32_t* num, int32_t* starts, int32_t* nodes) {
+  if (!ctx || !ctx->host_ready) return DOUG_E_STATE;
+  put_aggr(ctx->fin, num, starts, nodes);
+  return DOUG_OK;
+}
+
+int doug_get_coarse_aggregates(const doug_ctx* ctx, int32_t* num, int32_t* starts, int32_t* nodes) {
+  if (!ctx || !ctx->host_ready) return DOUG_E_STATE;
+  put_aggr(ctx->cfull, num, starts, nodes);
+  return DOUG_OK;
+}
+
+int doug_get_subdomains(const doug_ctx* ctx, int32_t* starts, int32_t* inds) {
+  if (!ctx || !ctx->host_ready) return DOUG_E_STATE;
+  for (size_t i = 0; i < ctx->sstarts.size(); i++) starts[i] = ctx->sstarts[i] + 1;
+  for (size_t i = 0; i < ctx->sinds.size(); i++) inds[i] = ctx->sinds[i] + 1;
+  return DOUG_OK;
+}
+
+static void put_csr(const Csr& M, int32_t* rowptr, int32_t* cols, double* vals) {
+  for (size_t i = 0; i < M.ptr.size(); i++) rowptr[i] = M.ptr[i];
+  for (size_t i = 0; i < M.col.size(); i++) {
+    cols[i] = M.col[i];
+    vals[i] = M.val[i];
+  }
+}
+
+int64_t doug_restriction_nnz(const doug_ctx* ctx) { return ctx ? ctx->R.nnz() : -1; }
+int64_t doug_coarse_matrix_nnz(const doug_ctx* ctx) { return ctx ? ctx->AC.nnz() : -1; }
+
+int doug_get_restriction(const doug_ctx* ctx, int32_t* rowptr, int32_t* cols, double* vals) {
+  if (!ctx || !ctx->host_ready) return DOUG_E_STATE;
+  put_csr(ctx->R, rowptr, cols, vals);
+  return DOUG_OK;
+}
+
+int doug_get_coarse_matrix(const doug_ctx* ctx, int32_t* rowptr, int32_t* cols, double* vals) {
+  if (!ctx || !ctx->host_ready) return DOUG_E_STATE;
+  put_csr(ctx->AC, rowptr, cols, vals);
+  return DOUG_OK;
+}
+
+// ---------------------------------------------------------------------------
+static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
+  launch_spmv(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, x, y, ctx->stream);
+}
+
+// preconditioner (pcg.F90:216-270): fine solves, then coarse correction added
+static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
+  const Params& p = ctx->p;
+  if (p.method == 0) {
+    launch_copy(z, r, ctx->n, ctx->stream);
+    return;
+  }
+  const bool coarse = ctx->has_coarse;
+  if (coarse) {
+    launch_spmv(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, ctx->d_crhs, ctx->stream);
+    launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_crhs, ctx->d_cu, ctx->stream);
+    launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cu, ctx->d_csol, ctx->stream);
+  }
+  const double* fine_src = r;
+  if (p.fine_method == 1) {
+    launch_trsv(ctx->d_tiles, ctx->d_sub, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    fine_src = ctx->d_subsol;
+  }
+  launch_overlap_sum(ctx->n, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
+                     ctx->d_Rtcol, ctx->d_Rtval, ctx->d_csol, z, ctx->stream);
+}
+
+int doug_spmv_device(doug_ctx* ctx, const double* dx, double* dy) {
+  if (!ctx) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_spmv before doug_setup");
+  enqueue_spmv(ctx, dx, dy);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+int doug_spmv(doug_ctx* ctx, const double* x, double* y) {
+  if (!ctx || !x || !y) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_spmv before doug_setup");
+  size_t bytes = (size_t)ctx->n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(ctx->d_p, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+  enqueue_spmv(ctx, ctx->d_p, ctx->d_q);
+  HIPCHK(hipMemcpyAsync(y, ctx->d_q, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+int doug_prec_apply_device(doug_ctx* ctx, const double* dr, double* dz) {
+  if (!ctx) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_prec_apply before doug_setup");
+  enqueue_prec(ctx, dr, dz);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+int doug_prec_apply(doug_ctx* ctx, const double* r, double* z) {
+  if (!ctx || !r || !z) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_prec_apply before doug_setup");
+  size_t bytes = (size_t)ctx->n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(ctx->d_r, r, bytes, hipMemcpyHostToDevice, ctx->stream));
+  enqueue_prec(ctx, ctx->d_r, ctx->d_z);
+  HIPCHK(hipMemcpyAsync(z, ctx->d_z, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos condition estimate (pcg.F90:465-497) with EISPACK tql1
+// (Bowdler, Martin, Reinsch, Wilkinson, Num. Math. 11 (1968) 293-306),
+// 1-based arrays.
+static double pythag(double a, double b) {
+  double p = std::max(std::fabs(a), std::fabs(b));
+  if (p == 0.0) return p;
+  double r = std::min(std::fabs(a), std::fabs(b)) / p;
+  r = r * r;
+  for (;;) {
+    double t = 4.0 + r;
+    if (t == 4.0) break;
+    double s = r / t, u = 1.0 + 2.0 * s;
+    p = u * p;
+    r = (s / u) * (s / u) * r;
+  }
+  return p;
+}
+
+static int tql1(int n, std::vector<double>& d, std::vector<double>& e) {
+  if (n == 1) return 0;
+  for (int i = 2; i <= n; i++) e[i - 1] = e[i];
+  double f = 0.0, tst1 = 0.0;
+  e[n] = 0.0;
+  for (int l = 1; l <= n; l++) {
+    int j = 0, m = l;
+    double h = std::fabs(d[l]) + std::fabs(e[l]);
+    tst1 = std::max(tst1, h);
+    for (m = l; m <= n; m++)
+      if (tst1 + std::fabs(e[m]) == tst1) break;
+    if (m != l) {
+      for (;;) {
+        if (j == 30) return l;
+        j++;
+        int l1 = l + 1, l2 = l1 + 1;
+        double g = d[l];
+        double pp = (d[l1] - g) / (2.0 * e[l]);
+        double r = pythag(pp, 1.0);
+        double sr = pp >= 0 ? std::fabs(r) : -std::fabs(r);
+        d[l] = e[l] / (pp + sr);
+        d[l1] = e[l] * (pp + sr);
+        double dl1 = d[l1];
+        h = g - d[l];
+        for (int i = l2; i <= n; i++) d[i] = d[i] - h;
+        f = f + h;
+        pp = d[m];
+        double c = 1.0, c2 = c, c3 = c, el1 = e[l1], s = 0.0, s2 = 0.0;
+        for (int ii = 1; ii <= m - l; ii++) {
+          c3 = c2;
+          c2 = c;
+          s2 = s;
+          int i = m - ii;
+          g = c * e[i];
+          h = c * pp;
+          r = pythag(pp, e[i]);
+          e[i + 1] = s * r;
+          s = e[i] / r;
+          c = pp / r;
+          pp = c * d[i] - s * g;
+          d[i + 1] = h + s * (c * g + s * d[i]);
+        }
+        pp = -s * s2 * c3 * el1 * e[l] / dl1;
+        e[l] = s * pp;
+        d[l] = c * pp;
+        if (!(tst1 + std::fabs(e[l]) > tst1)) break;
+      }
+    }
+    double pp = d[l] + f;
+    int i = 1;
+    for (int ii = 2; ii <= l; ii++) {
+      int k = l + 2 - ii;
+      if (pp >= d[k - 1]) {
+        i = k;
+        break;
+      }
+      d[k] = d[k - 1];
+      i = 1;
+    }
+    d[i] = pp;
+  }
+  return 0;
+}
+
+static double lanczos_cond(int it, const std::vector<double>& alpha, const std::vector<double>& beta) {
+  if (it < 1) return 0.0;
+  std::vector<double> d((size_t)it + 1, 0.0), e((size_t)it + 1, 0.0);
+  d[1] = 1.0 / alpha[1];
+  for (int i = 2; i <= it; i++) {
+    if (alpha[i] == 0 || beta[i] < 0) return 0.0;
+    d[i] = 1.0 / alpha[i] + beta[i] / alpha[i - 1];
+    e[i] = -std::sqrt(beta[i]) / alpha[i - 1];
+  }
+  tql1(it, d, e);
+  return d[it] / d[1];
+}
+
+static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, double* cond,
+                    double* res_hist) {
+  if (maxit == -1) maxit = 100000;
+  const int n = ctx->n;
+  hipStream_t st = ctx->stream;
+  double* scal = ctx->h_scal;
+  // r = b - A x
+  enqueue_spmv(ctx, ctx->d_x, ctx->d_tmp);
+  launch_residual(n, ctx->d_b, ctx->d_tmp, ctx->d_r, st);
+  launch_dot(ctx->d_r, ctx->d_r, n, ctx->d_partial, ctx->d_counter, nullptr, 3, ctx->d_scal, st);
+  HIPCHK(hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double init_norm = scal[S_RES];
+  if (init_norm == 0.0) init_norm = 1.0;
+  std::vector<double> alpha((size_t)std::min(maxit, 1 << 20) + 2, 0.0), beta(alpha.size(), 0.0);
+  int it = 0;
+  double ratio = 1.0;
+  while (ratio > tol * tol && it < maxit) {
+    it++;
+    enqueue_prec(ctx, ctx->d_r, ctx->d_z);
+    launch_dot(ctx->d_r, ctx->d_z, n, ctx->d_partial, ctx->d_counter, nullptr, 1, ctx->d_scal, st);
+    launch_p_update(n, ctx->d_z, ctx->d_p, ctx->d_scal, it == 1, st);
+    enqueue_spmv(ctx, ctx->d_p, ctx->d_q);
+    launch_dot(ctx->d_p, ctx->d_q, n, ctx->d_partial, ctx->d_counter, nullptr, 2, ctx->d_scal, st);
+    launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_counter, ctx->d_scal, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if ((size_t)it < alpha.size()) {
+      alpha[it] = scal[S_ALPHA];
+      beta[it] = scal[S_BETA];
+    }
+    double res = scal[S_RES];
+    ratio = res / init_norm;
+    if (res_hist) res_hist[it - 1] = std::sqrt(res);
+  }
+  if (it_out) *it_out = it;
+  if (cond) *cond = lanczos_cond(std::min<int>(it, (int)alpha.size() - 2), alpha, beta);
+  return DOUG_OK;
+}
+
+int doug_pcg_device(doug_ctx* ctx, const double* db, double* dx, double tol, int32_t maxit,
+                    int32_t* it, double* cond, double* res_hist) {
+  if (!ctx || !db || !dx) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_pcg before doug_setup");
+  size_t bytes = (size_t)ctx->n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(ctx->d_b, db, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_x, dx, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  int rc = pcg_core(ctx, tol, maxit, it, cond, res_hist);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(dx, ctx->d_x, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+int doug_pcg(doug_ctx* ctx, const double* b, double* x, double tol, int32_t maxit, int32_t* it,
+             double* cond, double* res_hist) {
+  if (!ctx || !b || !x) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_pcg before doug_setup");
+  size_t bytes = (size_t)ctx->n * sizeof(double);
+  HIPCHK(hipMemcpyAsync(ctx->d_b, b, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_x, x, bytes, hipMemcpyHostToDevice, ctx->stream));
+  int rc = pcg_core(ctx, tol, maxit, it, cond, res_hist);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(x, ctx->d_x, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
+  if (!ctx || !ms || reps <= 0) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_time_operator before doug_setup");
+  hipStream_t st = ctx->stream;
+  auto run = [&]() {
+    switch (which) {
+      case 0: enqueue_spmv(ctx, ctx->d_p, ctx->d_q); break;
+      case 1:
+        if (ctx->p.fine_method == 1)
+          launch_trsv(ctx->d_tiles, ctx->d_sub, ctx->nsub, ctx->d_inds, ctx->d_r, ctx->d_subsol, st);
+        break;
+      case 2: enqueue_prec(ctx, ctx->d_r, ctx->d_z); break;
+      default: break;
+    }
+  };
+  if (which < 0 || which > 2) return fail(ctx, DOUG_E_ARG, "unknown operator");
+  run();  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; r++) run();
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  *ms = (double)t / reps;
+  return DOUG_OK;
+}
+
+}  // extern "C"

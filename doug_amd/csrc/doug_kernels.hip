@@ -1,0 +1,660 @@
+// gfx950 kernels of the two-level Schwarz PCG solve path.
+//
+// Dense work (subdomain and coarse factorisation) runs on the FP64 matrix
+// cores (v_mfma_f64_16x16x4f64) over 64x64 tiles staged in LDS; every
+// per-iteration operator is HBM-bound and written as streaming kernels with
+// coalesced 512-byte row segments and wavefront reductions.  Kernels whose
+// results are compared bit-for-bit with a sequential CPU loop (SpMV,
+// restriction, overlap sum, vector updates) keep products and sums as
+// separate roundings.
+#include "doug_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace doug {
+
+namespace {
+
+constexpr int LD = kTile + 1;  // padded LDS row stride (doubles)
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int64_t toff(int i, int k) {
+  return ((int64_t)i * (i + 1) / 2 + k) * kTileElems;
+}
+
+// 64x64 row-major global tile -> LDS (stride LD); 256 threads.
+__device__ __forceinline__ void tile_to_lds(double* __restrict__ S, const double* __restrict__ g) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    int e = (t + 256 * q) * 2;
+    int r = e >> 6, c = e & 63;
+    double2 v = *reinterpret_cast<const double2*>(g + e);
+    S[r * LD + c] = v.x;
+    S[r * LD + c + 1] = v.y;
+  }
+}
+
+// Wave w of 4 owns the 32x32 quadrant (wr, wc) = (w/2, w%2) of the 64x64
+// output as 2x2 blocks of 16x16.  MFMA f64 16x16x4 operand layout: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; result element r of lane l is
+// D[4*(l/16)+r][l%16].
+// acc += A * B^T  (both operands row-major tiles in LDS)
+__device__ __forceinline__ void mma_abt(d4 (&acc)[2][2], const double* As, const double* Bs,
+                                        int lane, int wr, int wc) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 16; kk++) {
+    int k = 4 * kk + lk;
+    double a0 = As[(wr * 32 + li) * LD + k];
+    double a1 = As[(wr * 32 + 16 + li) * LD + k];
+    double b0 = Bs[(wc * 32 + li) * LD + k];
+    double b1 = Bs[(wc * 32 + 16 + li) * LD + k];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// acc += A * B  (B row-major [k][c])
+__device__ __forceinline__ void mma_ab(d4 (&acc)[2][2], const double* As, const double* Bs,
+                                       int lane, int wr, int wc) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int kk = 0; kk < 16; kk++) {
+    int k = 4 * kk + lk;
+    double a0 = As[(wr * 32 + li) * LD + k];
+    double a1 = As[(wr * 32 + 16 + li) * LD + k];
+    double b0 = Bs[k * LD + wc * 32 + li];
+    double b1 = Bs[k * LD + wc * 32 + 16 + li];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void zero_acc(d4 (&acc)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// element (row, col) held by accumulator (bi, bj, r) of this lane
+__device__ __forceinline__ int acc_row(int wr, int bi, int lane, int r) {
+  return wr * 32 + bi * 16 + 4 * (lane >> 4) + r;
+}
+__device__ __forceinline__ int acc_col(int wc, int bj, int lane) {
+  return wc * 32 + bj * 16 + (lane & 15);
+}
+
+// ------------------------------------------------------------------------
+// Setup kernels
+// ------------------------------------------------------------------------
+__global__ void k_scatter(double* __restrict__ tiles, const int64_t* __restrict__ dst,
+                          const int32_t* __restrict__ src, const double* __restrict__ val,
+                          int64_t ne) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) tiles[dst[e]] = val[src[e]];
+}
+
+__global__ void k_pad_identity(double* __restrict__ tiles, const BlockDesc* __restrict__ d,
+                               int maxpad) {
+  const BlockDesc b = d[blockIdx.y];
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int npad = b.nt * kTile - b.s;
+  if (p >= npad || p >= maxpad) return;
+  int lr = kTile - npad + p;  // local row inside the last diagonal tile
+  tiles[b.base + toff(b.nt - 1, b.nt - 1) + lr * kTile + lr] = 1.0;
+}
+
+__global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ tiles,
+                                                   const BlockDesc* __restrict__ d, int j,
+                                                   int* __restrict__ flag) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * LD];
+  const BlockDesc b = d[blockIdx.x];
+  if (j >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  double* Cg = tiles + b.base + toff(j, j);
+  d4 acc[2][2];
+  zero_acc(acc);
+  for (int k = 0; k < j; k++) {
+    tile_to_lds(As, tiles + b.base + toff(j, k));
+    __syncthreads();
+    mma_abt(acc, As, As, lane, wr, wc);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int row = acc_row(wr, bi, lane, r), col = acc_col(wc, bj, lane);
+        As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
+      }
+  __syncthreads();
+  // unblocked Cholesky of the 64x64 tile (lower triangle)
+  for (int c = 0; c < kTile; c++) {
+    if (t == 0) {
+      double dv = As[c * LD + c];
+      if (!(dv > 0.0)) {
+        atomicCAS(flag, 0, (int)blockIdx.x + 1);
+        dv = 1.0;
+      }
+      As[c * LD + c] = sqrt(dv);
+    }
+    __syncthreads();
+    const double piv = As[c * LD + c];
+    for (int r = c + 1 + t; r < kTile; r += 256) As[r * LD + c] = As[r * LD + c] / piv;
+    __syncthreads();
+    const int m = kTile - 1 - c;
+    for (int idx = t; idx < m * m; idx += 256) {
+      int rr = idx / m, qq = idx - rr * m;
+      if (qq <= rr) {
+        int r = c + 1 + rr, q = c + 1 + qq;
+        As[r * LD + q] -= As[r * LD + c] * As[q * LD + c];
+      }
+    }
+    __syncthreads();
+  }
+  // inverse of the lower factor, one column per thread of the first wave
+  for (int idx = t; idx < kTileElems; idx += 256) Bs[(idx >> 6) * LD + (idx & 63)] = 0.0;
+  __syncthreads();
+  if (t < kTile) {
+    const int col = t;
+    for (int r = col; r < kTile; r++) {
+      double s = (r == col) ? 1.0 : 0.0;
+      for (int k = col; k < r; k++) s -= As[r * LD + k] * Bs[k * LD + col];
+      Bs[r * LD + col] = s / As[r * LD + r];
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < kTileElems; idx += 256) Cg[idx] = Bs[(idx >> 6) * LD + (idx & 63)];
+}
+
+__global__ __launch_bounds__(256) void k_chol_offdiag(double* __restrict__ tiles,
+                                                      const BlockDesc* __restrict__ d, int j) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * LD];
+  const BlockDesc b = d[blockIdx.y];
+  const int i = j + 1 + blockIdx.x;
+  if (i >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  double* Cg = tiles + b.base + toff(i, j);
+  d4 acc[2][2];
+  zero_acc(acc);
+  for (int k = 0; k < j; k++) {
+    tile_to_lds(As, tiles + b.base + toff(i, k));
+    tile_to_lds(Bs, tiles + b.base + toff(j, k));
+    __syncthreads();
+    mma_abt(acc, As, Bs, lane, wr, wc);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int row = acc_row(wr, bi, lane, r), col = acc_col(wc, bj, lane);
+        As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
+      }
+  tile_to_lds(Bs, tiles + b.base + toff(j, j));  // inv(L_jj)
+  __syncthreads();
+  zero_acc(acc);
+  mma_abt(acc, As, Bs, lane, wr, wc);  // L_ij = C inv(L_jj)^T
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        Cg[acc_row(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = acc[bi][bj][r];
+}
+
+// W_{i,j} = -inv(L_ii) * sum_{k=j}^{i-1} L_ik W_kj, i = j + dd
+__global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
+                                               double* __restrict__ W,
+                                               const BlockDesc* __restrict__ d, int dd) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * LD];
+  const BlockDesc b = d[0];
+  const int j = blockIdx.x, i = j + dd;
+  if (i >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  d4 acc[2][2];
+  zero_acc(acc);
+  for (int k = j; k < i; k++) {
+    tile_to_lds(As, L + b.base + toff(i, k));
+    tile_to_lds(Bs, W + b.base + toff(k, j));
+    __syncthreads();
+    mma_ab(acc, As, Bs, lane, wr, wc);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        Bs[acc_row(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
+  tile_to_lds(As, L + b.base + toff(i, i));  // inv(L_ii)
+  __syncthreads();
+  zero_acc(acc);
+  mma_ab(acc, As, Bs, lane, wr, wc);
+  double* Wg = W + b.base + toff(i, j);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        Wg[acc_row(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = -acc[bi][bj][r];
+}
+
+// ------------------------------------------------------------------------
+// Wavefront reductions
+// ------------------------------------------------------------------------
+// 16 row partial sums spread over the 64 lanes (lane = column) -> lane l
+// returns the full sum of row (l >> 2); four reduce-scatter halvings then
+// two butterflies.
+__device__ __forceinline__ double reduce_scatter16(double (&v)[16], int lane) {
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    bool hi = lane & 32;
+    double keep = hi ? v[q + 8] : v[q], send = hi ? v[q] : v[q + 8];
+    v[q] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    bool hi = lane & 16;
+    double keep = hi ? v[q + 4] : v[q], send = hi ? v[q] : v[q + 4];
+    v[q] = keep + __shfl_xor(send, 16);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    bool hi = lane & 8;
+    double keep = hi ? v[q + 2] : v[q], send = hi ? v[q] : v[q + 2];
+    v[q] = keep + __shfl_xor(send, 8);
+  }
+  {
+    bool hi = lane & 4;
+    double keep = hi ? v[1] : v[0], send = hi ? v[0] : v[1];
+    v[0] = keep + __shfl_xor(send, 4);
+  }
+  v[0] += __shfl_xor(v[0], 2);
+  v[0] += __shfl_xor(v[0], 1);
+  return v[0];
+}
+
+// ------------------------------------------------------------------------
+// Per-iteration kernels
+// ------------------------------------------------------------------------
+// Forward then backward substitution for one subdomain per block.
+__global__ __launch_bounds__(256) void k_trsv(const double* __restrict__ tiles,
+                                              const BlockDesc* __restrict__ d,
+                                              const int32_t* __restrict__ inds,
+                                              const double* __restrict__ rhs,
+                                              double* __restrict__ sol) {
+  __shared__ double y[kMaxTrsvTiles * kTile];
+  __shared__ double tv[kTile];
+  __shared__ double red[4 * kTile];
+  const BlockDesc b = d[blockIdx.x];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nt = b.nt, npad = nt * kTile;
+  const double* T0 = tiles + b.base;
+  for (int p = t; p < npad; p += 256) y[p] = p < b.s ? rhs[inds[b.vec_off + p]] : 0.0;
+  __syncthreads();
+  // forward: y_i = inv(L_ii) (r_i - sum_{k<i} L_ik y_k)
+  for (int i = 0; i < nt; i++) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = 0.0;
+    for (int k = 0; k < i; k++) {
+      const double* Tk = T0 + toff(i, k) + (16 * w) * kTile + lane;
+      const double yk = y[k * kTile + lane];
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * yk;
+    }
+    double s = reduce_scatter16(acc, lane);
+    if ((lane & 3) == 0) {
+      int row = 16 * w + (lane >> 2);
+      tv[row] = y[i * kTile + row] - s;
+    }
+    __syncthreads();
+    const double* Ti = T0 + toff(i, i) + (16 * w) * kTile + lane;
+    const double tl = tv[lane];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] = Ti[q * kTile] * tl;
+    s = reduce_scatter16(acc, lane);
+    if ((lane & 3) == 0) y[i * kTile + 16 * w + (lane >> 2)] = s;
+    __syncthreads();
+  }
+  // backward: x_j = inv(L_jj)^T (y_j - sum_{i>j} L_ij^T x_i)
+  for (int j = nt - 1; j >= 0; j--) {
+    double acc = 0.0;
+    for (int i = j + 1; i < nt; i++) {
+      const double* Ti = T0 + toff(i, j) + (16 * w) * kTile + lane;
+      const double* xi = y + i * kTile + 16 * w;
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * xi[q];
+    }
+    red[w * kTile + lane] = acc;
+    __syncthreads();
+    if (w == 0)
+      tv[lane] = y[j * kTile + lane] -
+                 ((red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]));
+    __syncthreads();
+    const double* Tj = T0 + toff(j, j) + (16 * w) * kTile + lane;
+    acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc += Tj[q * kTile] * tv[16 * w + q];
+    red[w * kTile + lane] = acc;
+    __syncthreads();
+    if (w == 0)
+      y[j * kTile + lane] =
+          (red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]);
+    __syncthreads();
+  }
+  for (int p = t; p < b.s; p += 256) sol[b.vec_off + p] = y[p];
+}
+
+// u_i = sum_{k<=i} W_ik c_k, one block per tile row
+__global__ __launch_bounds__(256) void k_gemv_w(const double* __restrict__ W,
+                                                const BlockDesc* __restrict__ d,
+                                                const double* __restrict__ c,
+                                                double* __restrict__ u) {
+  const BlockDesc b = d[0];
+  const int i = blockIdx.x;
+  if (i >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) acc[q] = 0.0;
+  for (int k = 0; k <= i; k++) {
+    const double* Tk = W + b.base + toff(i, k) + (16 * w) * kTile + lane;
+    const double ck = c[k * kTile + lane];
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * ck;
+  }
+  double s = reduce_scatter16(acc, lane);
+  if ((lane & 3) == 0) u[i * kTile + 16 * w + (lane >> 2)] = s;
+}
+
+// c_j = sum_{i>=j} W_ij^T u_i, one block per tile column
+__global__ __launch_bounds__(256) void k_gemv_wt(const double* __restrict__ W,
+                                                 const BlockDesc* __restrict__ d,
+                                                 const double* __restrict__ u,
+                                                 double* __restrict__ c) {
+  __shared__ double red[4 * kTile];
+  const BlockDesc b = d[0];
+  const int j = blockIdx.x;
+  if (j >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double acc = 0.0;
+  for (int i = j; i < b.nt; i++) {
+    const double* Ti = W + b.base + toff(i, j) + (16 * w) * kTile + lane;
+    const double* ui = u + i * kTile + 16 * w;
+#pragma unroll
+    for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * ui[q];
+  }
+  red[w * kTile + lane] = acc;
+  __syncthreads();
+  if (w == 0)
+    c[j * kTile + lane] =
+        (red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]);
+}
+
+// CSR-stream SpMV: the block's nonzeros are multiplied with coalesced loads
+// into LDS, then each row is summed in stored order by one thread.
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(const int32_t* __restrict__ rowblk,
+                                                       const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ col,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ y) {
+  __shared__ double prod[kSpmvChunk];
+  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int e0 = ptr[r0], e1 = ptr[r1];
+  const int t = threadIdx.x;
+  if (e1 - e0 <= kSpmvChunk) {
+    for (int e = e0 + t; e < e1; e += kSpmvThreads)
+      prod[e - e0] = __builtin_nontemporal_load(val + e) * x[__builtin_nontemporal_load(col + e)];
+    __syncthreads();
+    const int row = r0 + t;
+    if (row < r1) {
+      double acc = 0.0;
+      for (int e = ptr[row]; e < ptr[row + 1]; e++) acc = acc + prod[e - e0];
+      y[row] = acc;
+    }
+  } else {
+    // a single long row, streamed through LDS chunk by chunk
+    double acc = 0.0;
+    for (int c0 = e0; c0 < e1; c0 += kSpmvChunk) {
+      int c1 = min(e1, c0 + kSpmvChunk);
+      for (int e = c0 + t; e < c1; e += kSpmvThreads) prod[e - c0] = val[e] * x[col[e]];
+      __syncthreads();
+      if (t == 0)
+        for (int e = c0; e < c1; e++) acc = acc + prod[e - c0];
+      __syncthreads();
+    }
+    if (t == 0) y[r0] = acc;
+  }
+}
+
+__global__ void k_overlap_sum(int n, const int32_t* __restrict__ occ_ptr,
+                              const int32_t* __restrict__ occ_pos, const double* __restrict__ sol,
+                              const int32_t* __restrict__ rt_ptr, const int32_t* __restrict__ rt_col,
+                              const double* __restrict__ rt_val, const double* __restrict__ csol,
+                              double* __restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int p = occ_ptr[i]; p < occ_ptr[i + 1]; p++) acc = acc + sol[occ_pos[p]];
+  if (rt_ptr) {
+    double tmp = 0.0;
+    for (int p = rt_ptr[i]; p < rt_ptr[i + 1]; p++) tmp = tmp + rt_val[p] * csol[rt_col[p]];
+    acc = acc + tmp;
+  }
+  z[i] = acc;
+}
+
+// ------------------------------------------------------------------------
+// Deterministic reductions: fixed grid, per-block tree, last block sums the
+// partials in a fixed tree and applies the scalar recurrence.
+// ------------------------------------------------------------------------
+enum { S_RHO = 0, S_RHO_PREV = 1, S_BETA = 2, S_PQ = 3, S_ALPHA = 4, S_RES = 5 };
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (t == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return r;
+}
+
+__device__ void finish(double* partial, unsigned* counter, double* out, int mode, double* scal,
+                       double* sh) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // partial[] was written by other blocks: atomic loads read it at device
+  // scope instead of a possibly stale L1 line
+  double v = 0.0;
+  for (int p = threadIdx.x; p < (int)gridDim.x; p += blockDim.x)
+    v += __hip_atomic_load(&partial[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  double s = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    if (mode == 0) {
+      out[0] = s;
+    } else if (mode == 1) {  // rho = r.z
+      scal[S_BETA] = s / scal[S_RHO];
+      scal[S_RHO_PREV] = scal[S_RHO];
+      scal[S_RHO] = s;
+    } else if (mode == 2) {  // p.q
+      scal[S_PQ] = s;
+      scal[S_ALPHA] = scal[S_RHO] / s;
+    } else {
+      scal[S_RES] = s;
+    }
+    *counter = 0u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dot(const double* __restrict__ a,
+                                             const double* __restrict__ b, int n,
+                                             double* partial, unsigned* counter, double* out,
+                                             int mode, double* scal) {
+  __shared__ double sh[4];
+  double v = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v = v + a[i] * b[i];
+  double s = block_sum(v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  __syncthreads();
+  finish(partial, counter, out, mode, scal, sh);
+}
+
+__global__ void k_p_update(int n, const double* __restrict__ z, double* __restrict__ p,
+                           const double* __restrict__ scal, int first) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (first)
+    p[i] = z[i];
+  else
+    p[i] = z[i] + scal[S_BETA] * p[i];
+}
+
+__global__ __launch_bounds__(256) void k_xr_update(int n, double* __restrict__ x,
+                                                   double* __restrict__ r,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* partial,
+                                                   unsigned* counter, double* scal) {
+  __shared__ double sh[4];
+  const double alpha = scal[S_ALPHA];
+  double v = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    x[i] = x[i] + alpha * p[i];
+    double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    v = v + ri * ri;
+  }
+  double s = block_sum(v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  __syncthreads();
+  finish(partial, counter, nullptr, 3, scal, sh);
+}
+
+__global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_residual(int n, const double* __restrict__ b, const double* __restrict__ ax,
+                           double* __restrict__ r) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - ax[i];
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+void launch_scatter_entries(double* tiles, const int64_t* dst, const int32_t* src,
+                            const double* val, int64_t ne, hipStream_t st) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(cdiv(ne, 256)), dim3(256), 0, st, tiles, dst, src, val, ne);
+}
+
+void launch_pad_identity(double* tiles, const BlockDesc* d, int nblocks, int maxpad,
+                         hipStream_t st) {
+  if (maxpad <= 0 || nblocks <= 0) return;
+  hipLaunchKernelGGL(k_pad_identity, dim3(cdiv(maxpad, 64), nblocks), dim3(64), 0, st, tiles, d,
+                     maxpad);
+}
+
+void launch_chol_diag(double* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_chol_diag, dim3(nblocks), dim3(256), 0, st, tiles, d, j, flag);
+}
+
+void launch_chol_offdiag(double* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
+                         hipStream_t st) {
+  if (maxcount <= 0) return;
+  hipLaunchKernelGGL(k_chol_offdiag, dim3(maxcount, nblocks), dim3(256), 0, st, tiles, d, j);
+}
+
+void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, int dd,
+                       hipStream_t st) {
+  if (nt - dd <= 0) return;
+  hipLaunchKernelGGL(k_trinv, dim3(nt - dd), dim3(256), 0, st, L, W, d, dd);
+}
+
+void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
+                 const double* val, const double* x, double* y, hipStream_t st) {
+  if (nrb <= 0) return;
+  hipLaunchKernelGGL(k_spmv, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x, y);
+}
+
+void launch_trsv(const double* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+                 const double* r, double* sol, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_trsv, dim3(nblocks), dim3(256), 0, st, tiles, d, inds, r, sol);
+}
+
+void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_w, dim3(nt), dim3(256), 0, st, W, d, c, u);
+}
+
+void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u, double* c,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_wt, dim3(nt), dim3(256), 0, st, W, d, u, c);
+}
+
+void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
+                        const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,
+                        const double* csol, double* z, hipStream_t st) {
+  hipLaunchKernelGGL(k_overlap_sum, dim3(cdiv(n, 256)), dim3(256), 0, st, n, occ_ptr, occ_pos,
+                     sol, rt_ptr, rt_col, rt_val, csol, z);
+}
+
+void launch_dot(const double* a, const double* b, int n, double* partial, unsigned* counter,
+                double* out, int mode, double* scal, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(256), 0, st, a, b, n, partial, counter, out,
+                     mode, scal);
+}
+
+void launch_p_update(int n, const double* z, double* p, const double* scal, int first,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_p_update, dim3(cdiv(n, 256)), dim3(256), 0, st, n, z, p, scal, first);
+}
+
+void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
+                      double* partial, unsigned* counter, double* scal, hipStream_t st) {
+  hipLaunchKernelGGL(k_xr_update, dim3(kRedBlocks), dim3(256), 0, st, n, x, r, p, q, partial,
+                     counter, scal);
+}
+
+void launch_copy(double* dst, const double* src, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), dim3(256), 0, st, dst, src, n);
+}
+
+void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st) {
+  hipLaunchKernelGGL(k_residual, dim3(cdiv(n, 256)), dim3(256), 0, st, n, b, ax, r);
+}
+
+}  // namespace doug

@@ -1,0 +1,76 @@
+// Launch wrappers for the gfx950 kernels of the Schwarz-PCG solve path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace doug {
+
+constexpr int kTile = 64;              // dense tile edge (one wavefront wide)
+constexpr int kTileElems = kTile * kTile;
+constexpr int kSpmvThreads = 256;      // CSR-stream block
+constexpr int kSpmvChunk = 2048;       // products staged in LDS per block
+constexpr int kMaxTrsvTiles = 128;     // subdomain size limit of the one-block solve (8192)
+constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reductions
+
+// Dense subdomain block in tile-packed lower storage: tile (i,k), i>=k, of
+// block b starts at base + (i*(i+1)/2 + k) * kTileElems; each tile is 64x64
+// row-major.  After factorisation the off-diagonal tiles hold L and the
+// diagonal tiles hold inv(L_ii).
+struct BlockDesc {
+  int64_t base;     // element offset of the block's tile storage
+  int64_t vec_off;  // offset into the concatenated subdomain vectors
+  int32_t nt;       // tiles per dimension
+  int32_t s;        // true size (rows), <= nt*64
+};
+
+// ---- dense batched factorisation ----
+// Zero the tile storage, put ones on the padded diagonal, scatter A entries.
+void launch_scatter_entries(double* tiles, const int64_t* dst, const int32_t* src,
+                            const double* val, int64_t nentries, hipStream_t st);
+void launch_pad_identity(double* tiles, const BlockDesc* d, int nblocks, int maxpad,
+                         hipStream_t st);
+// Left-looking tile Cholesky, step j: diagonal tiles (SYRK + POTRF + inverse)
+// then the tiles below them (GEMM + TRSM).  `flag` receives the first failing
+// block+1 on a non-positive pivot.
+void launch_chol_diag(double* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
+                      hipStream_t st);
+void launch_chol_offdiag(double* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
+                         hipStream_t st);
+// Explicit inverse of a tile-packed lower factor (single block d[0]):
+// W = L^{-1}, tiles at diagonal offset `dd` (W_{j+dd,j} for all j); W must
+// start as a copy of L so that its diagonal tiles hold inv(L_jj).
+void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, int dd,
+                       hipStream_t st);
+
+// ---- per-iteration operators ----
+// y = A x, CSR-stream: each row summed in stored order, products rounded
+// before the add (bit-identical to a sequential row loop without FMA).
+void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
+                 const double* val, const double* x, double* y, hipStream_t st);
+// Batched forward/backward substitution, one block per subdomain: gathers
+// r(inds), solves L L^T u = r, writes u to sol (concatenated layout).
+void launch_trsv(const double* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+                 const double* r, double* sol, hipStream_t st);
+// u = W c and c' = W^T u for the coarse inverse factor.
+void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
+                   hipStream_t st);
+void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u, double* c,
+                    hipStream_t st);
+// z[i] = sum over occurrences (ascending subdomain) of sol, then
+//        + sum_a Rt(i,a) csol[a]  (coarse prolongation), fine part first.
+void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
+                        const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,
+                        const double* csol, double* z, hipStream_t st);
+
+// ---- PCG vector kernels with deterministic two-level reductions ----
+// scal layout (device doubles): see doug_capi.hip
+void launch_dot(const double* a, const double* b, int n, double* partial, unsigned* counter,
+                double* out, int mode, double* scal, hipStream_t st);
+void launch_p_update(int n, const double* z, double* p, const double* scal, int first,
+                     hipStream_t st);
+void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
+                      double* partial, unsigned* counter, double* scal, hipStream_t st);
+void launch_copy(double* dst, const double* src, int n, hipStream_t st);
+void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st);
+
+}  // namespace doug

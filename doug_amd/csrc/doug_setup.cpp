@@ -1,0 +1,693 @@
+// Host setup of the two-level Schwarz preconditioner; see doug_setup.hpp.
+// Reference citations are relative to the DOUG source tree.
+#include "doug_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+
+namespace doug {
+
+static constexpr int kMaxInt = 2147483647;            // parameters.F90:99
+static constexpr int kAggregated = kMaxInt;           // parameters.F90:101
+static constexpr int kPending = kMaxInt / 2;          // parameters.F90:102
+
+Csr coo_to_csr(int n, int64_t nnz, const int32_t* indi, const int32_t* indj, const double* val,
+               int base, std::string& err) {
+  Csr A;
+  A.nrows = A.ncols = n;
+  A.ptr.assign((size_t)n + 1, 0);
+  for (int64_t t = 0; t < nnz; t++) {
+    int64_t i = (int64_t)indi[t] - base, j = (int64_t)indj[t] - base;
+    if (i < 0 || i >= n || j < 0 || j >= n) {
+      err = "matrix entry " + std::to_string(t) + " out of range";
+      return Csr{};
+    }
+    A.ptr[(size_t)i + 1]++;
+  }
+  for (int i = 0; i < n; i++) A.ptr[i + 1] += A.ptr[i];
+  A.col.resize((size_t)nnz);
+  A.val.resize((size_t)nnz);
+  std::vector<int> fill(A.ptr.begin(), A.ptr.end() - 1);
+  for (int64_t t = 0; t < nnz; t++) {
+    int i = indi[t] - base;
+    int p = fill[i]++;
+    A.col[p] = indj[t] - base;
+    A.val[p] = val[t];
+  }
+  return A;
+}
+
+void scale_values(const Csr& A, bool symm, std::vector<double>& scaled, std::vector<double>& diag) {
+  int n = std::max(A.nrows, A.ncols);
+  diag.assign((size_t)n, 0.0);
+  for (int i = 0; i < A.nrows; i++)
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+      if (A.col[p] == i) diag[i] = A.val[p];
+  scaled = A.val;
+  std::vector<double> sv((size_t)n);
+  if (symm) {
+    for (int i = 0; i < n; i++) sv[i] = std::sqrt(std::fabs(diag[i]));
+    for (int i = 0; i < A.nrows; i++)
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+        double v = scaled[p] / sv[i];
+        scaled[p] = v / sv[A.col[p]];
+      }
+  } else {
+    for (int i = 0; i < n; i++) sv[i] = std::fabs(diag[i]);
+    for (int i = 0; i < A.nrows; i++)
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) scaled[p] = scaled[p] / sv[i];
+  }
+}
+
+void unscale_values(const Csr& A, bool symm, const std::vector<double>& diag,
+                    std::vector<double>& vals) {
+  int n = (int)diag.size();
+  std::vector<double> sv((size_t)n);
+  if (symm) {
+    for (int i = 0; i < n; i++) sv[i] = std::sqrt(std::fabs(diag[i]));
+    for (int i = 0; i < A.nrows; i++)
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+        double v = vals[p] * sv[i];
+        vals[p] = v * sv[A.col[p]];
+      }
+  } else {
+    for (int i = 0; i < n; i++) sv[i] = std::fabs(diag[i]);
+    for (int i = 0; i < A.nrows; i++)
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) vals[p] = vals[p] * sv[i];
+  }
+}
+
+std::vector<uint8_t> find_strong(const Csr& A, const std::vector<double>& scaled,
+                                 const std::vector<double>& diag, double alpha) {
+  std::vector<uint8_t> strong((size_t)A.nnz(), 0);
+  for (int i = 0; i < A.nrows; i++) {
+    // single-precision literal in the reference (SpMtx_aggregation.F90:867)
+    double maxndiag = (double)(-1.0e15f);
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+      if (A.col[p] != i) maxndiag = std::max(maxndiag, std::fabs(scaled[p]));
+    maxndiag = maxndiag * alpha;
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      if (A.col[p] != i)
+        strong[p] = std::fabs(scaled[p]) > maxndiag;
+      else
+        strong[p] = diag[i] > maxndiag;
+    }
+  }
+  return strong;
+}
+
+// SpMtx_build_refs_symm with sortdown (SpMtx_arrangement.F90:461-568):
+// strong off-diagonal neighbours per row, |scaled| descending, ties keep
+// their row order.
+static void strong_refs(const Csr& A, const std::vector<double>& scaled,
+                        const std::vector<uint8_t>& strong, std::vector<int>& srow,
+                        std::vector<int>& scol) {
+  int n = A.nrows;
+  srow.assign((size_t)n + 1, 0);
+  for (int i = 0; i < n; i++)
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+      if (strong[p] && A.col[p] != i && A.col[p] < n) srow[i + 1]++;
+  for (int i = 0; i < n; i++) srow[i + 1] += srow[i];
+  scol.resize((size_t)srow[n]);
+  std::vector<int> tmp;
+  for (int i = 0; i < n; i++) {
+    tmp.clear();
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+      if (strong[p] && A.col[p] != i && A.col[p] < n) tmp.push_back(p);
+    std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int b) {
+      return std::fabs(scaled[a]) > std::fabs(scaled[b]);
+    });
+    for (size_t k = 0; k < tmp.size(); k++) scol[srow[i] + k] = A.col[tmp[k]];
+  }
+}
+
+namespace {
+
+// Growing layers around a seed: lets_colour / can_add_roundlayer
+// (Aggregate_mod.F90:287-497).  List positions are 0-based here; node ids are
+// 0-based; `stat` keeps the reference's encoding.
+struct Grower {
+  const std::vector<int>& rs;
+  const std::vector<int>& cn;
+  std::vector<int>& stat;
+  std::vector<int>& distance;
+  std::vector<int>& nodes;
+  int& allmind;
+  int neighood, maxasize;
+  std::vector<int> inlayer, newlayer, rounder;
+  int nneigs = 0;
+
+  Grower(const std::vector<int>& rs_, const std::vector<int>& cn_, std::vector<int>& st,
+         std::vector<int>& di, std::vector<int>& nd, int& am, int nh, int mx)
+      : rs(rs_), cn(cn_), stat(st), distance(di), nodes(nd), allmind(am), neighood(nh),
+        maxasize(mx) {}
+
+  void push(std::vector<int>& v, int& cnt, int x) {
+    if ((size_t)cnt >= v.size()) v.resize(v.size() * 2 + 16);
+    v[cnt++] = x;
+  }
+
+  int round_layer(int layer, int& ninnodes) {
+    const int nxtlayer = layer + 1;
+    bool layerfits = true;
+    int ntoadd = 0, nrounders = 0;
+    if (layer <= 2 * neighood) {
+      for (int k = 0; k < ninnodes; k++) {
+        int ln = inlayer[k];
+        for (int i = rs[ln]; i < rs[ln + 1]; i++) {
+          int c = cn[i];
+          if (stat[c] <= 0) {
+            if (stat[c] == 0) {
+              push(newlayer, ntoadd, c);
+              distance[c] = distance[ln] + 1;
+            }
+            stat[c] = stat[c] - 1;
+          }
+        }
+      }
+    }
+    if (layer <= neighood) {
+      int mind = kMaxInt, maxd = 0;
+      for (int k = 0; k < ntoadd; k++) {
+        int d = -stat[newlayer[k]];
+        mind = std::min(mind, d);
+        maxd = std::max(maxd, d);
+      }
+      if (mind < allmind) allmind = mind;
+      if (mind < maxd || mind > allmind) {
+        for (int k = 0; k < ntoadd; k++) {
+          if (stat[newlayer[k]] == -maxd) {
+            nrounders++;
+            if (nneigs + ninnodes + 1 > maxasize) {
+              layerfits = false;
+              break;
+            }
+            push(inlayer, ninnodes, newlayer[k]);
+            if ((int)rounder.size() < nrounders) rounder.resize(nrounders);
+            rounder[nrounders - 1] = newlayer[k];
+            newlayer[k] = -1;
+          }
+        }
+      }
+      if (!layerfits) nrounders = 0;
+      int j = 0;
+      for (int i = 0; i < ntoadd; i++)
+        if (newlayer[i] != -1) newlayer[j++] = newlayer[i];
+      ntoadd = j;
+      for (int k = 0; k < nrounders; k++) {
+        int ln = rounder[k];
+        for (int i = rs[ln]; i < rs[ln + 1]; i++) {
+          int c = cn[i];
+          if (stat[c] == 0) {
+            push(newlayer, ntoadd, c);
+            distance[c] = distance[ln] + 1;
+          }
+        }
+      }
+    }
+    if (layer <= neighood && nneigs + ninnodes > maxasize) layerfits = false;
+    if (layerfits) {
+      for (int k = 0; k < ninnodes; k++) {
+        stat[inlayer[k]] = layer + 1;
+        nodes[nneigs++] = inlayer[k];
+      }
+      if (layer <= 2 * neighood) {
+        for (int k = 0; k < ntoadd; k++) stat[newlayer[k]] = kPending - 1;
+        ninnodes = ntoadd;
+        if ((int)inlayer.size() < ntoadd) inlayer.resize(ntoadd);
+        for (int k = 0; k < ntoadd; k++) inlayer[k] = newlayer[k];
+        return round_layer(nxtlayer, ninnodes);
+      }
+      return 3;
+    }
+    for (int k = 0; k < ninnodes; k++) distance[inlayer[k]] = 0;
+    for (int k = 0; k < ntoadd; k++) distance[newlayer[k]] = 0;
+    for (int k = 0; k < nneigs; k++) distance[nodes[k]] = 0;
+    for (int k = 0; k < ninnodes; k++) nodes[nneigs++] = inlayer[k];
+    if (nneigs > 0) return layer > neighood ? 2 : 1;
+    return 0;
+  }
+
+  int colour(int seed) {
+    nneigs = 0;
+    stat[seed] = 1;
+    distance[seed] = 0;
+    int ninnodes = 1;
+    if (inlayer.empty()) inlayer.resize(64);
+    if (newlayer.empty()) newlayer.resize(64);
+    inlayer[0] = seed;
+    return round_layer(0, ninnodes);
+  }
+};
+
+void form_aggr(Aggregates& ag, int nagrs, int n, const std::vector<int>& num) {
+  ag.nagr = nagrs;
+  ag.num = num;
+  ag.starts.assign((size_t)nagrs + 1, 0);
+  for (int i = 0; i < n; i++)
+    if (num[i] > 0) ag.starts[num[i]]++;
+  for (int a = 0; a < nagrs; a++) ag.starts[a + 1] += ag.starts[a];
+  ag.nodes.resize((size_t)ag.starts[nagrs]);
+  std::vector<int> fill(ag.starts.begin(), ag.starts.end() - 1);
+  for (int i = 0; i < n; i++)
+    if (num[i] > 0) ag.nodes[fill[num[i] - 1]++] = i;
+}
+
+}  // namespace
+
+void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vector<uint8_t>& strong,
+               int neighood, int minasize, int maxasize, double alpha, bool coarse_level,
+               ProcessState& ps, Aggregates& inner, Aggregates& full) {
+  const int n = A.nrows;
+  std::vector<int> aggrnum((size_t)n, 0);
+  int nagrs = 0;
+  bool toosmall = false;
+  int maxasizelargest = 0;
+  int sn = (int)std::sqrt((float)n);  // single-precision sqrt (SpMtx_aggregation.F90:122)
+  if (alpha >= 0 || sn * sn != n) {
+    std::vector<int> srow, scol;
+    strong_refs(A, scaled, strong, srow, scol);
+    std::vector<int> stat((size_t)n, 0), nodes((size_t)n + 1, 0), distance((size_t)n, 0);
+    maxasizelargest = coarse_level ? 3 * maxasize : maxasize + 4 * (2 * neighood);
+    if (!coarse_level) ps.beta = alpha / 4.0;
+    int ngoodstarts = 0;
+    std::vector<int> nextgood((size_t)n, 0);
+    int firstwocol = 0;
+    std::vector<int64_t> stamp((size_t)n, 0);
+    int64_t stamp_id = 0;
+    Grower g(srow, scol, stat, distance, nodes, ps.allmind, neighood, maxasize);
+    while (firstwocol < n) {
+      if (ngoodstarts > 0) {
+        int j = 0;
+        for (int i = 0; i < ngoodstarts; i++)
+          if (stat[nextgood[i]] < kPending) nextgood[j++] = nextgood[i];
+        ngoodstarts = j;
+      }
+      int startnode = -1;
+      if (ngoodstarts == 0) {
+        bool done = false;
+        while (stat[firstwocol] >= kPending) {
+          firstwocol++;
+          if (firstwocol >= n) {
+            done = true;
+            break;
+          }
+        }
+        if (done) break;
+        startnode = firstwocol;
+      } else {
+        // the first entry that repeats an earlier one (SpMtx_aggregation.F90:
+        // 209-216 scans all pairs); a stamp per node finds it in one pass
+        stamp_id++;
+        for (int i = 0; i < ngoodstarts; i++) {
+          int v = nextgood[i];
+          if (stamp[v] == stamp_id) {
+            startnode = v;
+            break;
+          }
+          stamp[v] = stamp_id;
+        }
+        if (startnode < 0) startnode = nextgood[0];
+      }
+      int ok = g.colour(startnode);
+      int nneigs = g.nneigs;
+      int mindistance = kMaxInt;
+      if (ok > 0) {
+        nagrs++;
+        int nouters = 0, next_start_layer;
+        if (ok == 3) {
+          std::vector<int> layerlen((size_t)(2 * neighood + 3), 0);
+          int maxlayer = 0;
+          if (!toosmall && nneigs < minasize) toosmall = true;
+          for (int j = nneigs - 1; j >= 0; j--) {
+            int k = stat[nodes[j]];
+            if (k > maxlayer) maxlayer = k;
+            if (k <= neighood + 1) break;
+            layerlen[k]++;
+          }
+          next_start_layer = neighood + 2;
+          int k = layerlen[neighood + 2];
+          for (int j = neighood + 3; j <= maxlayer; j++)
+            if (k < layerlen[j]) {
+              k = layerlen[j];
+              next_start_layer = j;
+            }
+        } else if (ok == 2) {
+          next_start_layer = neighood + 2;
+          if (!toosmall && nneigs < minasize) toosmall = true;
+        } else {
+          next_start_layer = 0;
+        }
+        for (int j = 0; j < nneigs; j++) {
+          int node = nodes[j];
+          if (stat[node] <= neighood + 1 && j + 1 <= maxasize) {
+            stat[node] = kAggregated;
+            aggrnum[node] = nagrs;
+          } else if (next_start_layer > neighood + 1) {
+            if (stat[node] == next_start_layer) {
+              mindistance = std::min(mindistance, distance[node]);
+              nouters++;
+              nextgood[n - nouters] = node;
+            }
+          }
+        }
+        for (int j = 1; j <= nouters; j++)
+          if (distance[nextgood[n - j]] == mindistance) nextgood[ngoodstarts++] = nextgood[n - j];
+        for (int j = 0; j < nneigs; j++) {
+          int node = nodes[j];
+          if (stat[node] != kAggregated) {
+            distance[node] = 0;
+            stat[node] = 0;
+          }
+        }
+      }
+    }
+  } else {
+    // Cartesian aggregates for a perfect-square n (SpMtx_aggregation.F90:333-346)
+    int nsa = (sn + neighood - 1) / neighood;
+    int k = 0;
+    for (int i = 1; i <= sn; i++)
+      for (int j = 1; j <= sn; j++)
+        aggrnum[k++] = ((i + neighood - 1) / neighood - 1) * nsa + (j + neighood - 1) / neighood;
+    nagrs = *std::max_element(aggrnum.begin(), aggrnum.end());
+  }
+
+  std::vector<int> fullnum((size_t)n, 0);
+  int full_nagrs = 0;
+  if (!toosmall) {
+    fullnum = aggrnum;
+    for (int v : fullnum) full_nagrs = std::max(full_nagrs, v);
+    form_aggr(inner, nagrs, n, aggrnum);
+  } else {
+    // merge too-small aggregates (SpMtx_aggregation.F90:365-591)
+    std::vector<std::vector<int>> members((size_t)nagrs + 1);
+    for (int i = 0; i < n; i++) members[aggrnum[i]].push_back(i);
+    std::vector<int> colsaround;
+    std::vector<double> connw;
+    int nagrs_new = nagrs;
+    const double beta = ps.beta;
+    for (int a = 1; a <= nagrs; a++) {
+      std::vector<int>& mem = members[a];
+      if ((int)mem.size() >= minasize) continue;
+      colsaround.clear();
+      connw.clear();
+      for (int node : mem) {
+        for (int p = A.ptr[node]; p < A.ptr[node + 1]; p++) {
+          int c = A.col[p];
+          if (c >= n) continue;
+          int colr = aggrnum[c];
+          if (colr > 0 && colr != a) {
+            size_t kk = 0;
+            for (; kk < colsaround.size(); kk++)
+              if (colsaround[kk] == colr) {
+                connw[kk] = connw[kk] + std::fabs(scaled[p]);
+                break;
+              }
+            if (kk == colsaround.size()) {
+              colsaround.push_back(colr);
+              connw.push_back(std::fabs(scaled[p]));
+            }
+          }
+        }
+      }
+      double maxcw = 0.0;
+      int eater = 0;
+      for (size_t j = 0; j < colsaround.size(); j++)
+        if (maxcw < connw[j] &&
+            (int)members[colsaround[j]].size() + (int)mem.size() <= maxasize) {
+          maxcw = connw[j];
+          eater = colsaround[j];
+        }
+      if (maxcw >= alpha || (coarse_level && maxcw >= beta)) {
+        for (int node : mem) {
+          members[eater].push_back(node);
+          aggrnum[node] = eater;
+        }
+        mem.clear();
+        nagrs_new--;
+      } else {
+        // give the aggregate away node by node, from its last node back
+        int nleft = (int)mem.size();
+        int sz = nleft;
+        bool reduced = true;
+        while (nleft > 0 && reduced) {
+          reduced = false;
+          for (int k = sz - 1; k >= 0; k--) {
+            int node = mem[k];
+            if (node < 0) continue;
+            for (int p = A.ptr[node]; p < A.ptr[node + 1]; p++) {
+              int c = A.col[p];
+              if (c >= n) continue;
+              int colr = aggrnum[c];
+              if (colr != a && (int)members[colr].size() < maxasizelargest &&
+                  std::fabs(scaled[p]) >= beta) {
+                members[colr].push_back(node);
+                aggrnum[node] = colr;
+                mem[k] = -1;
+                nleft--;
+                reduced = true;
+                break;
+              }
+            }
+          }
+        }
+        if (nleft > 0 && nleft < sz) {
+          int k = 0;
+          for (int j = 0; j < sz; j++)
+            if (mem[j] >= 0) mem[k++] = mem[j];
+          mem.resize((size_t)nleft);
+        } else if (nleft == 0) {
+          mem.clear();
+          nagrs_new--;
+        }
+      }
+    }
+    int64_t naggregated = 0;
+    for (int a = 1; a <= nagrs; a++) naggregated += (int64_t)members[a].size();
+    full_nagrs = nagrs_new;
+    int cnt = *std::max_element(aggrnum.begin(), aggrnum.end());
+    if (cnt > nagrs_new) {
+      std::vector<int> st((size_t)cnt + 1, 0);
+      int col = 0;
+      for (int i = 0; i < n; i++)
+        if (aggrnum[i] > 0) {
+          if (st[aggrnum[i]] == 0) st[aggrnum[i]] = ++col;
+          aggrnum[i] = st[aggrnum[i]];
+        }
+    }
+    if (naggregated == n) {
+      fullnum = aggrnum;
+      full_nagrs = *std::max_element(fullnum.begin(), fullnum.end());
+    } else {
+      for (int i = 0; i < n; i++) fullnum[i] = aggrnum[i] > 0 ? aggrnum[i] : ++full_nagrs;
+    }
+    form_aggr(inner, nagrs_new, n, aggrnum);
+  }
+  form_aggr(full, full_nagrs, n, fullnum);
+}
+
+Csr transpose(const Csr& A) {
+  Csr T;
+  T.nrows = A.ncols;
+  T.ncols = A.nrows;
+  T.ptr.assign((size_t)T.nrows + 1, 0);
+  for (int c : A.col) T.ptr[c + 1]++;
+  for (int i = 0; i < T.nrows; i++) T.ptr[i + 1] += T.ptr[i];
+  T.col.resize(A.col.size());
+  T.val.resize(A.val.size());
+  std::vector<int> fill(T.ptr.begin(), T.ptr.end() - 1);
+  for (int i = 0; i < A.nrows; i++)
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      int q = fill[A.col[p]]++;
+      T.col[q] = i;
+      T.val[q] = A.val[p];
+    }
+  return T;
+}
+
+Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregates& ag,
+                int smoothers, double omega) {
+  const int n = A.nrows, nagr = ag.nagr;
+  Csr R;
+  R.nrows = nagr;
+  R.ncols = n;
+  if (smoothers == 0) {  // R = T, the aggregate indicator (CoarseMtx.F90:80-108)
+    R.ptr = ag.starts;
+    R.col = ag.nodes;
+    R.val.assign(R.col.size(), 1.0);
+    return R;
+  }
+  // filtered diagonal and the Jacobi smoother S (CoarseMtx.F90:152-233)
+  std::vector<double> diag((size_t)n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+      if (A.col[p] == i || !strong[p]) diag[i] = diag[i] + A.val[p];
+  Csr S;
+  S.nrows = S.ncols = n;
+  S.ptr = A.ptr;
+  S.col = A.col;
+  S.val.resize(A.val.size());
+  const double one_minus = 1.0 - omega;
+  for (int i = 0; i < n; i++)
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      if (A.col[p] == i)
+        S.val[p] = one_minus;
+      else if (strong[p])
+        S.val[p] = -omega / diag[i] * A.val[p];
+      else
+        S.val[p] = 0.0;  // weak entries stay as explicit zeros (CoarseMtx.F90:187-191)
+    }
+  // R = T S^T (SpMtx_op_AB.f90:125-176): row a of R visits the aggregate's
+  // nodes kk ascending and, for each, column kk of S in ascending row order;
+  // an entry is first assigned, then accumulated in that order.
+  Csr St = transpose(S);
+  std::vector<int> mark((size_t)n, -1), list;
+  std::vector<double> acc((size_t)n, 0.0);
+  R.ptr.assign((size_t)nagr + 1, 0);
+  for (int a = 0; a < nagr; a++) {
+    list.clear();
+    for (int q = ag.starts[a]; q < ag.starts[a + 1]; q++) {
+      int kk = ag.nodes[q];
+      for (int p = St.ptr[kk]; p < St.ptr[kk + 1]; p++) {
+        int j = St.col[p];
+        double prod = 1.0 * St.val[p];
+        if (mark[j] != a) {
+          mark[j] = a;
+          list.push_back(j);
+          acc[j] = prod;
+        } else {
+          acc[j] = acc[j] + prod;
+        }
+      }
+    }
+    std::sort(list.begin(), list.end());
+    for (int j : list) {
+      R.col.push_back(j);
+      R.val.push_back(acc[j]);
+    }
+    R.ptr[a + 1] = (int)R.col.size();
+  }
+  return R;
+}
+
+Csr galerkin(const Csr& A, const Csr& R) {
+  const int n = A.nrows, nc = R.nrows;
+  // T' = A R^T: row i visits A's row in stored order, then column k of R in
+  // ascending aggregate order (SpMtx_op_AB.f90 with BT).
+  Csr Rt = transpose(R);  // rows = fine nodes, aggregates ascending
+  Csr Tp;
+  Tp.nrows = n;
+  Tp.ncols = nc;
+  Tp.ptr.assign((size_t)n + 1, 0);
+  std::vector<int> mark((size_t)std::max(n, nc), -1), pos((size_t)std::max(n, nc), 0);
+  for (int i = 0; i < n; i++) {
+    int start = (int)Tp.col.size();
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      int k = A.col[p];
+      for (int q = Rt.ptr[k]; q < Rt.ptr[k + 1]; q++) {
+        int jj = Rt.col[q];
+        double prod = A.val[p] * Rt.val[q];
+        if (mark[jj] != i) {
+          mark[jj] = i;
+          pos[jj] = (int)Tp.col.size();
+          Tp.col.push_back(jj);
+          Tp.val.push_back(prod);
+        } else {
+          double& v = Tp.val[pos[jj]];
+          v = v + prod;
+        }
+      }
+    }
+    (void)start;
+    Tp.ptr[i + 1] = (int)Tp.col.size();
+  }
+  // A_c = R T' with R's rows in ascending column order.
+  Csr Ac;
+  Ac.nrows = Ac.ncols = nc;
+  Ac.ptr.assign((size_t)nc + 1, 0);
+  std::fill(mark.begin(), mark.end(), -1);
+  for (int i = 0; i < nc; i++) {
+    for (int p = R.ptr[i]; p < R.ptr[i + 1]; p++) {
+      int kk = R.col[p];
+      for (int q = Tp.ptr[kk]; q < Tp.ptr[kk + 1]; q++) {
+        int jj = Tp.col[q];
+        double prod = R.val[p] * Tp.val[q];
+        if (mark[jj] != i) {
+          mark[jj] = i;
+          pos[jj] = (int)Ac.col.size();
+          Ac.col.push_back(jj);
+          Ac.val.push_back(prod);
+        } else {
+          double& v = Ac.val[pos[jj]];
+          v = v + prod;
+        }
+      }
+    }
+    Ac.ptr[i + 1] = (int)Ac.col.size();
+  }
+  return Ac;
+}
+
+void coarse_partition(const Aggregates& cfull, const Aggregates& ffull, int n,
+                      std::vector<int>& starts, std::vector<int>& nodes) {
+  std::vector<char> seen((size_t)n, 0);
+  starts.assign((size_t)cfull.nagr + 1, 0);
+  nodes.clear();
+  for (int c = 0; c < cfull.nagr; c++) {
+    size_t first = nodes.size();
+    for (int q = cfull.starts[c]; q < cfull.starts[c + 1]; q++) {
+      int fa = cfull.nodes[q];  // fine aggregate (0-based)
+      for (int t = ffull.starts[fa]; t < ffull.starts[fa + 1]; t++) {
+        int node = ffull.nodes[t];
+        if (!seen[node]) {
+          seen[node] = 1;
+          nodes.push_back(node);
+        }
+      }
+    }
+    for (size_t t = first; t < nodes.size(); t++) seen[nodes[t]] = 0;
+    starts[c + 1] = (int)nodes.size();
+  }
+}
+
+void add_layers_all(const Csr& A, const std::vector<int>& pstarts, const std::vector<int>& pnodes,
+                    int ol, std::vector<int>& sstarts, std::vector<int>& sinds) {
+  const int n = A.nrows;
+  const int np = (int)pstarts.size() - 1;
+  std::vector<int> onfront((size_t)n, 0);
+  sstarts.assign((size_t)np + 1, 0);
+  sinds.clear();
+  std::vector<int> nodes;
+  for (int ip = 0; ip < np; ip++) {
+    nodes.assign(pnodes.begin() + pstarts[ip], pnodes.begin() + pstarts[ip + 1]);
+    int nn = (int)nodes.size();
+    for (int v : nodes) onfront[v] = -1;
+    // the reference re-scans the last node of the previous front at each
+    // layer (frontstart(layer)=nfront); that visit adds nothing, so a plain
+    // BFS over [begin, end) of the previous front gives the same order.
+    int fbeg = 0, fend = nn;
+    for (int layer = 1; layer <= ol; layer++) {
+      int nfront = (int)nodes.size();
+      for (int i = fbeg; i < fend; i++) {
+        int node = nodes[i];
+        for (int p = A.ptr[node]; p < A.ptr[node + 1]; p++) {
+          int nb = A.col[p];
+          if (onfront[nb] == 0) {
+            onfront[nb] = layer;
+            nodes.push_back(nb);
+          }
+        }
+      }
+      fbeg = nfront;
+      fend = (int)nodes.size();
+    }
+    for (int v : nodes) onfront[v] = 0;
+    sinds.insert(sinds.end(), nodes.begin(), nodes.end());
+    sstarts[ip + 1] = (int)sinds.size();
+  }
+}
+
+}  // namespace doug

@@ -1,0 +1,90 @@
+// Host-side setup of DOUG's two-level Schwarz preconditioner (numprocs == 1):
+// strength of connection, greedy aggregation, smoothed restriction, Galerkin
+// coarse matrix, subdomain index sets.  Sequential, order-dependent graph work
+// that decides *which* unknowns go together; every FP64 contraction that runs
+// per iteration lives in the HIP kernels.
+#pragma once
+#include <climits>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace doug {
+
+// CSR with 0-based indices; within a row the entries keep the order the
+// reference's stable arrangement gives them (src/datatypes/SpMtx/
+// SpMtx_arrangement.F90:58-252).
+struct Csr {
+  int nrows = 0, ncols = 0;
+  std::vector<int> ptr, col;
+  std::vector<double> val;
+  int64_t nnz() const { return (int64_t)col.size(); }
+};
+
+struct Params {
+  int radius1 = -1, radius2 = -1, minasize1 = -1, minasize2 = -1;
+  int maxasize1 = -1, maxasize2 = -1;
+  double strong1 = -1.0, strong2 = -1.0;
+  int smoothers = -1, overlap = -1, levels = 2, method = 1, fine_method = 1;
+  int symmstruct = 0, symmnumeric = 0;
+  double omega = 0.33333333333333;  // CoarseMtx.F90:64
+};
+
+// Aggregates (Aggregate_mod.F90:39-51): num is 1-based aggregate number per
+// node (0 = none); starts/nodes is the 0-based CSR listing nodes ascending.
+struct Aggregates {
+  int nagr = 0;
+  std::vector<int> num, starts, nodes;
+};
+
+// Variables the reference keeps alive across calls within one process:
+// `allmind` (Aggregate_mod.F90:348) and the initialised local `beta` of
+// SpMtx_aggregate (SpMtx_aggregation.F90:83).
+struct ProcessState {
+  int allmind = INT_MAX;
+  double beta = 1.0e-5;
+};
+
+// COO (1-based or 0-based) -> CSR by a stable counting sort on rows.
+Csr coo_to_csr(int n, int64_t nnz, const int32_t* indi, const int32_t* indj, const double* val,
+               int base, std::string& err);
+
+// SpMtx_scale (SpMtx_arrangement.F90:298-351): returns the scaled values and
+// the stored diagonal; symmetric scaling if `symm`, row scaling otherwise.
+void scale_values(const Csr& A, bool symm, std::vector<double>& scaled, std::vector<double>& diag);
+// SpMtx_unscale (SpMtx_arrangement.F90:355-388) applied to scaled values.
+void unscale_values(const Csr& A, bool symm, const std::vector<double>& diag,
+                    std::vector<double>& vals);
+
+// SpMtx_find_strong (SpMtx_aggregation.F90:829-899)
+std::vector<uint8_t> find_strong(const Csr& A, const std::vector<double>& scaled,
+                                 const std::vector<double>& diag, double alpha);
+
+// SpMtx_aggregate (SpMtx_aggregation.F90:49-641) and its helpers.
+void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vector<uint8_t>& strong,
+               int neighood, int minasize, int maxasize, double alpha, bool coarse_level,
+               ProcessState& ps, Aggregates& inner, Aggregates& full);
+
+// IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 and 1.  Returned as
+// CSR with each row sorted by column (the order prec2Level restricts in).
+Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregates& inner,
+                int smoothers, double omega);
+
+// CoarseMtxBuild (CoarseMtx.F90:476-516): A_c = R (A R^T), rows in the
+// reference's first-touch column order.
+Csr galerkin(const Csr& A, const Csr& R);
+
+// Transpose keeping ascending source-row order inside each output row.
+Csr transpose(const Csr& A);
+
+// Get_aggregate_nodes (Aggregate_mod.F90:162-190) for every coarse
+// aggregate: the np=1 subdomain partition (Partitioning_aggr.F90:176-189).
+void coarse_partition(const Aggregates& cfull, const Aggregates& ffull, int n,
+                      std::vector<int>& starts, std::vector<int>& nodes);
+
+// Add_layers (Decomposition.f90:56-99) for every part: appends `ol` BFS
+// layers over the row graph of A.
+void add_layers_all(const Csr& A, const std::vector<int>& pstarts, const std::vector<int>& pnodes,
+                    int ol, std::vector<int>& sstarts, std::vector<int>& sinds);
+
+}  // namespace doug

@@ -1,0 +1,146 @@
+/*
+ * doug.h -- C-ABI of the MI355X (gfx950) implementation of DOUG's solve path:
+ * PCG preconditioned by two-level additive Schwarz with a smoothed-aggregation
+ * coarse grid (doug_aggr with solver 2, method 1, levels 2, fine_method 1,
+ * coarse_method 1).  One context = one DOUG process (numprocs == 1) with all
+ * of its subdomains resident on one GPU.
+ *
+ * Conventions (kept from the reference, src/datatypes/SpMtx/SpMtx_class.F90
+ * and src/utils/globals.F90):
+ *   - scalars are int32 / double; index arrays are int32 with a caller-chosen
+ *     base (1 = Fortran numbering as in DOUG's files, 0 = C numbering);
+ *   - the caller owns every host array; the library copies what it keeps;
+ *   - every entry point returns DOUG_OK (0) or an error code; the message is
+ *     available from doug_last_error().  DOUG's own behaviour on any error is
+ *     DOUG_abort -> MPI_ABORT (src/utils/DOUG_utils.F90:254-286): a caller
+ *     wanting that behaviour aborts on a non-zero status;
+ *   - all GPU work runs on the context's stream (default priority); the
+ *     *_device entry points take device pointers and return after the work
+ *     has completed on that stream.
+ */
+#ifndef DOUG_H
+#define DOUG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DOUG_OK = 0,
+  DOUG_E_ARG = 1,      /* invalid argument */
+  DOUG_E_STATE = 2,    /* call out of order (e.g. solve before setup) */
+  DOUG_E_HIP = 3,      /* HIP runtime error */
+  DOUG_E_NUMERIC = 4,  /* non-positive pivot in a subdomain or coarse factor */
+  DOUG_E_NOMEM = 5,    /* device allocation failed */
+  DOUG_E_UNSUPPORTED = 6
+};
+
+enum { DOUG_PREC_FP64 = 0 };
+
+typedef struct doug_ctx doug_ctx;
+
+/* Control values on the path.  Field names and the meaning of -1 ("not set")
+ * follow DOUG.ctl (src/utils/globals.F90:85-116, src/utils/controls.F90). */
+typedef struct {
+  int32_t radius1, radius2, minasize1, minasize2, maxasize1, maxasize2;
+  double strong1, strong2;
+  int32_t smoothers, overlap, levels, method, fine_method;
+  int32_t symmstruct, symmnumeric;
+  /* prolongator damping: DOUG hard-codes 0.33333333333333
+   * (src/coarse/CoarseMtx.F90:64); other values are a documented extension */
+  double omega;
+} doug_params;
+
+/* Sizes and timings of the last doug_setup (seconds). */
+typedef struct {
+  int32_t n, nagr_fine, nagr_coarse, nsub, ncoarse, overlap, smoothers;
+  int64_t nnz, sub_total, factor_bytes, coarse_factor_bytes;
+  double t_aggregate, t_coarse_build, t_subdomains, t_upload, t_factor, t_coarse_factor;
+  double t_total;
+} doug_info;
+
+/* Context.  Replaces the process-wide state DOUG_Init sets up
+ * (src/utils/DOUG_utils.F90:367-429); `device` is a HIP device ordinal, or
+ * -1 for a host-only context that can run doug_setup_host and the getters
+ * but no GPU work. */
+int doug_ctx_create(int device, doug_ctx **out);
+int doug_ctx_destroy(doug_ctx *ctx);
+const char *doug_last_error(const doug_ctx *ctx);
+
+/* Defaults identical to an empty DOUG.ctl for the keys above, plus
+ * levels 2, method 1, fine_method 1, omega 0.33333333333333. */
+void doug_params_default(doug_params *p);
+int doug_set_params(doug_ctx *ctx, const doug_params *p);
+
+/* Assembled matrix as COO triples, duplicates allowed (SpMV adds them).
+ * Replaces ReadInSparseAssembled + Distribution_NewInit for one process
+ * (src/datatypes/SpMtx/SpMtx_class.F90:410-453,
+ *  src/components/Distribution.F90:50-75). */
+int doug_set_matrix_coo(doug_ctx *ctx, int32_t n, int64_t nnz, const int32_t *indi,
+                        const int32_t *indj, const double *val, int32_t index_base);
+
+/* Optional non-overlapping partition of the unknowns into subdomains; takes
+ * the place of METIS in Partitionings_metis_InitCoarse
+ * (src/components/Partitioning_metis.F90:34-86).  Without it the subdomains
+ * are the coarse aggregates expanded to fine nodes, DOUG's one-process rule
+ * (src/components/Partitioning_aggr.F90:120-193).  starts has nparts+1
+ * entries (starts[0] == index_base); nodes lists each part's unknowns. */
+int doug_set_partition(doug_ctx *ctx, int32_t nparts, const int32_t *starts,
+                       const int32_t *nodes, int32_t index_base);
+
+/* Builds the preconditioner: Partitionings_aggr_InitFine,
+ * CoarsePreconditioner_smooth_Init, Partitionings_aggr_InitCoarse,
+ * FinePreconditioner_Init and the factorisations of Factorise_subdomains and
+ * of the coarse matrix (src/main/aggr.F90:131-219,
+ * src/solvers/subsolvers.F90:78-163).  precision: DOUG_PREC_FP64. */
+int doug_setup(doug_ctx *ctx, int32_t precision);
+/* The host half of doug_setup alone: aggregates, restriction, coarse matrix
+ * and subdomain index sets, without any device allocation or factorisation. */
+int doug_setup_host(doug_ctx *ctx);
+int doug_get_info(const doug_ctx *ctx, doug_info *info);
+
+/* Setup products, 1-based as DOUG stores them (caller allocates):
+ * aggregates (Aggregate_mod.F90:39-51): num[n], starts[nagr+1], nodes[...]. */
+int doug_get_fine_aggregates(const doug_ctx *ctx, int32_t *num, int32_t *starts, int32_t *nodes);
+int doug_get_coarse_aggregates(const doug_ctx *ctx, int32_t *num, int32_t *starts,
+                               int32_t *nodes);
+/* Decomposition%subd(:)%inds (Preconditioner_base.F90:96-123) */
+int doug_get_subdomains(const doug_ctx *ctx, int32_t *starts, int32_t *inds);
+/* restriction R (CoarseMtx.F90:50-474) as CSR, rows ascending in column;
+ * coarse matrix A_c (CoarseMtx.F90:476-516) as CSR in DOUG's entry order */
+int doug_get_restriction(const doug_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals);
+int doug_get_coarse_matrix(const doug_ctx *ctx, int32_t *rowptr, int32_t *cols, double *vals);
+int64_t doug_restriction_nnz(const doug_ctx *ctx);
+int64_t doug_coarse_matrix_nnz(const doug_ctx *ctx);
+
+/* y = A x (SpMtx_Ax / Distribution_pmvm, SpMtx_op_Ax.f90:44-101) */
+int doug_spmv(doug_ctx *ctx, const double *x, double *y);
+int doug_spmv_device(doug_ctx *ctx, const double *dx, double *dy);
+
+/* z = M^{-1} r, the two-level additive Schwarz preconditioner
+ * (`preconditioner` + prec2Level, src/solvers/pcg.F90:54-270) */
+int doug_prec_apply(doug_ctx *ctx, const double *r, double *z);
+int doug_prec_apply_device(doug_ctx *ctx, const double *dr, double *dz);
+
+/* pcg_weigs (src/solvers/pcg.F90:289-462): x holds x0 on entry (DOUG's
+ * driver passes zero) and the solution on return.  Stops when
+ * ||r_k||^2 / ||r_0||^2 <= tol^2 or after maxit iterations (maxit == -1 means
+ * 100000).  cond receives the Lanczos condition estimate; res_hist (optional,
+ * maxit entries) the residual norm of every iteration. */
+int doug_pcg(doug_ctx *ctx, const double *b, double *x, double tol, int32_t maxit, int32_t *it,
+             double *cond, double *res_hist);
+int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int32_t maxit,
+                    int32_t *it, double *cond, double *res_hist);
+
+/* Timing of one operator over `reps` back-to-back launches on the context's
+ * stream (HIP events); ms receives the average milliseconds per launch.
+ * which: 0 = SpMV, 1 = subdomain forward/backward solves, 2 = whole
+ * preconditioner apply. */
+int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1701 @@
+/*
+ * doug_oracle.c -- CPU restatement of DOUG's solve path (PCG + two-level
+ * additive Schwarz + smoothed-aggregation coarse grid), numprocs == 1.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the MI355X library.  Nothing in
+ * the product links or loads this file.
+ *
+ * Each function cites the reference routine it restates (paths relative to
+ * the reference source tree).  Arrays are 1-based (element 0 unused) so that
+ * index arithmetic reads like the Fortran it follows.  Compiled with
+ * -ffp-contract=off: products and sums are rounded separately, as a default
+ * x86-64 build of the reference computes them.
+ *
+ * Pinning (what checks this restatement independently of itself):
+ *   SpMV, dots, axpy ............ closed forms + numpy brute force
+ *   PCG + Schwarz + coarse ...... reference fixtures tests/assembled/lap16x16
+ *                                 (csolution.xdr) and lap4x4
+ *                                 (solution_correct.dat); scipy/numpy dense
+ *                                 preconditioner built from the same index sets
+ *   Lanczos/tql1 ................ eigenvalues of the 5-point Laplacian (closed
+ *                                 form) and numpy.linalg.eigvalsh
+ *   strength mask ............... closed-form threshold test (numpy)
+ *   R, A_c given aggregates ..... scipy.sparse products of the stated formulas
+ *   Add_layers .................. BFS brute force
+ *   aggregation (fine, coarse) .. invariants + hand-traced small grids only;
+ *                                 the greedy order itself is PARITY UNPINNED
+ *                                 (the reference cannot be built here)
+ *   subdomains from coarse aggr . PARITY UNPINNED (follows from aggregation)
+ */
+#include "doug_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define D_MAXINT 2147483647                 /* src/utils/parameters.F90:99 */
+#define D_AGGREGATED D_MAXINT               /* parameters.F90:101 */
+#define D_PENDING (D_MAXINT / 2)            /* parameters.F90:102 */
+
+enum { ARR_NO = 0, ARR_ROWS = 1, ARR_COLS = 2 };
+
+static char g_err[512];
+static void seterr(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+const char *or_last_error(void) { return g_err; }
+
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz);
+  if (!p) {
+    fprintf(stderr, "doug_oracle: out of memory (%zu x %zu)\n", n, sz);
+    abort();
+  }
+  return p;
+}
+#define IALLOC(n) ((int *)xcalloc((size_t)(n) + 1, sizeof(int)))
+#define DALLOC(n) ((double *)xcalloc((size_t)(n) + 1, sizeof(double)))
+
+/* ------------------------------------------------------------------ */
+/* Saved state of one process                                           */
+/* ------------------------------------------------------------------ */
+static int g_allmind = D_MAXINT;   /* Aggregate_mod.F90:348 `save` */
+static double g_beta = 1.0E-5;     /* SpMtx_aggregation.F90:83 initialised => saved */
+
+void or_reset_process(void) {
+  g_allmind = D_MAXINT;
+  g_beta = 1.0E-5;
+}
+
+void or_params_default(or_params *p) {
+  p->radius1 = p->radius2 = -1;
+  p->minasize1 = p->minasize2 = -1;
+  p->maxasize1 = p->maxasize2 = -1;
+  p->strong1 = p->strong2 = -1.0;
+  p->smoothers = -1;
+  p->overlap = -1;
+  p->levels = 2;
+  p->method = 1;
+  p->fine_method = 1;
+  p->symmstruct = 0;
+  p->symmnumeric = 0;
+  p->omega = 0.33333333333333;
+}
+
+/* ------------------------------------------------------------------ */
+/* Sparse matrix in DOUG's COO + bounds form (SpMtx_class.F90:69-146)   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int nrows, ncols;
+  long nnz;
+  int *indi, *indj;
+  double *val;
+  int arr;
+  int nbound;      /* size of M_bound, 0 if none */
+  int *bound;      /* 1..nbound */
+  unsigned char *strong;
+  double *diag;    /* 1..max(nrows,ncols) */
+  int scaled;
+  int symmstruct, symmnumeric;
+  int *srow, *scol; /* strong_rowstart / strong_colnrs */
+} OM;
+
+static OM *om_new(long nnz, int nrows, int ncols) {
+  OM *m = (OM *)xcalloc(1, sizeof(OM));
+  m->nnz = nnz;
+  m->nrows = nrows;
+  m->ncols = ncols;
+  m->indi = IALLOC(nnz);
+  m->indj = IALLOC(nnz);
+  m->val = DALLOC(nnz);
+  return m;
+}
+
+static void om_free(OM *m) {
+  if (!m) return;
+  free(m->indi); free(m->indj); free(m->val); free(m->bound);
+  free(m->strong); free(m->diag); free(m->srow); free(m->scol);
+  free(m);
+}
+
+/* SpMtx_Copy (SpMtx_class.F90:680-699): entries, shape and arrangement;
+ * symmetry flags reset by newInit. */
+static OM *om_copy(const OM *a) {
+  OM *m = om_new(a->nnz, a->nrows, a->ncols);
+  memcpy(m->indi, a->indi, sizeof(int) * (a->nnz + 1));
+  memcpy(m->indj, a->indj, sizeof(int) * (a->nnz + 1));
+  memcpy(m->val, a->val, sizeof(double) * (a->nnz + 1));
+  m->arr = a->arr;
+  if (a->arr != ARR_NO) {
+    m->nbound = a->nbound;
+    m->bound = IALLOC(a->nbound);
+    memcpy(m->bound, a->bound, sizeof(int) * (a->nbound + 1));
+  }
+  return m;
+}
+
+/* SpMtx_arrange (SpMtx_arrangement.F90:58-252).  Stable counting sort by row
+ * or column; with `sort`, each row/column is insertion-sorted by the other
+ * index (strict <, so equal keys keep their order).  Returns early when the
+ * matrix already has the requested arrangement. Destroys the strong mask. */
+static void om_arrange(OM *m, int at, int sort, int nrows_ov, int ncols_ov) {
+  int Mnrows = nrows_ov > 0 ? nrows_ov : m->nrows;
+  int Mncols = ncols_ov > 0 ? ncols_ov : m->ncols;
+  long Mnnz = m->nnz;
+  int cols = (at == ARR_COLS);
+  int ok = (m->arr == at);
+  int size = cols ? Mncols : Mnrows;
+  if (ok && m->nbound != size + 1) ok = 0;
+  if (ok) return;
+  m->arr = at;
+  int *el = IALLOC(size);
+  free(m->bound);
+  m->nbound = size + 1;
+  m->bound = IALLOC(size + 1);
+  for (long i = 1; i <= Mnnz; i++) el[cols ? m->indj[i] : m->indi[i]]++;
+  m->bound[1] = 1;
+  for (int i = 1; i <= size; i++) m->bound[i + 1] = m->bound[i] + el[i];
+  int *ni = IALLOC(Mnnz), *nj = IALLOC(Mnnz);
+  double *nv = DALLOC(Mnnz);
+  if (sort) {
+    long *sortref = (long *)xcalloc((size_t)Mnnz + 1, sizeof(long));
+    memset(el, 0, sizeof(int) * (size + 1));
+    for (long i = 1; i <= Mnnz; i++) {
+      int k = cols ? m->indj[i] : m->indi[i];
+      long ind_beg = m->bound[k];
+      if (el[k] == 0) {
+        sortref[ind_beg] = i;
+        el[k] = 1;
+      } else {
+        long ind = m->bound[k] + el[k];
+        long j = ind_beg;
+        for (;;) {
+          int key_i = cols ? m->indi[i] : m->indj[i];
+          if (j == ind) {
+            sortref[j] = i;
+            el[k]++;
+            break;
+          } else if (key_i < (cols ? m->indi[sortref[j]] : m->indj[sortref[j]])) {
+            for (long kk = ind; kk >= j + 1; kk--) sortref[kk] = sortref[kk - 1];
+            sortref[j] = i;
+            el[k]++;
+            break;
+          } else {
+            j++;
+          }
+        }
+      }
+    }
+    for (long i = 1; i <= Mnnz; i++) {
+      long ind = sortref[i];
+      ni[i] = m->indi[ind];
+      nj[i] = m->indj[ind];
+      nv[i] = m->val[ind];
+    }
+    free(sortref);
+  } else {
+    memset(el, 0, sizeof(int) * (size + 1));
+    for (long i = 1; i <= Mnnz; i++) {
+      int k = cols ? m->indj[i] : m->indi[i];
+      long ind = m->bound[k] + el[k];
+      el[k]++;
+      ni[ind] = m->indi[i];
+      nj[ind] = m->indj[i];
+      nv[ind] = m->val[i];
+    }
+  }
+  memcpy(m->indi + 1, ni + 1, sizeof(int) * Mnnz);
+  memcpy(m->indj + 1, nj + 1, sizeof(int) * Mnnz);
+  memcpy(m->val + 1, nv + 1, sizeof(double) * Mnnz);
+  free(ni); free(nj); free(nv); free(el);
+  free(m->strong);
+  m->strong = NULL;
+}
+
+/* SpMtx_scale (SpMtx_arrangement.F90:298-351).  Symmetric scaling when the
+ * matrix is flagged symmetric in structure and values, else row scaling. */
+static void om_scale(OM *m) {
+  if (m->scaled) return;
+  int nd = m->nrows > m->ncols ? m->nrows : m->ncols;
+  if (!m->diag) m->diag = DALLOC(nd);
+  double *sv = DALLOC(nd);
+  for (long i = 1; i <= m->nnz; i++)
+    if (m->indi[i] == m->indj[i]) m->diag[m->indi[i]] = m->val[i];
+  if (m->symmstruct && m->symmnumeric) {
+    for (int i = 1; i <= nd; i++) sv[i] = sqrt(fabs(m->diag[i]));
+    for (long i = 1; i <= m->nnz; i++) {
+      m->val[i] = m->val[i] / sv[m->indi[i]];
+      m->val[i] = m->val[i] / sv[m->indj[i]];
+    }
+  } else {
+    for (int i = 1; i <= nd; i++) sv[i] = fabs(m->diag[i]);
+    for (long i = 1; i <= m->nnz; i++) m->val[i] = m->val[i] / sv[m->indi[i]];
+  }
+  free(sv);
+  m->scaled = 1;
+}
+
+/* SpMtx_unscale (SpMtx_arrangement.F90:355-388) */
+static void om_unscale(OM *m) {
+  if (!m->scaled) return;
+  int nd = m->nrows > m->ncols ? m->nrows : m->ncols;
+  double *sv = DALLOC(nd);
+  if (m->symmstruct && m->symmnumeric) {
+    for (int i = 1; i <= nd; i++) sv[i] = sqrt(fabs(m->diag[i]));
+    for (long i = 1; i <= m->nnz; i++) {
+      m->val[i] = m->val[i] * sv[m->indi[i]];
+      m->val[i] = m->val[i] * sv[m->indj[i]];
+    }
+  } else {
+    for (int i = 1; i <= nd; i++) sv[i] = fabs(m->diag[i]);
+    for (long i = 1; i <= m->nnz; i++) m->val[i] = m->val[i] * sv[m->indi[i]];
+  }
+  free(sv);
+  free(m->diag);
+  m->diag = NULL;
+  m->scaled = 0;
+}
+
+/* SpMtx_find_strong, non-simple branch (SpMtx_aggregation.F90:829-899):
+ * off-diagonal a_ij strong iff |a~_ij| > alpha * max_{k!=i} |a~_ik| on the
+ * scaled matrix; the diagonal entry compares the unscaled a_ii.
+ * Symmetrisation is off on the path (`symm` saved .false., :838). */
+static void om_find_strong(OM *a, double alpha) {
+  if (!a->scaled) om_scale(a);
+  om_arrange(a, ARR_ROWS, 0, 0, 0);
+  if (!a->strong) a->strong = (unsigned char *)xcalloc((size_t)a->nnz + 1, 1);
+  for (int i = 1; i <= a->nrows; i++) {
+    int st = a->bound[i], en = a->bound[i + 1] - 1;
+    double maxndiag = (double)(-1.0e15f);
+    for (int j = st; j <= en; j++)
+      if (a->indj[j] != i) {
+        double aa = fabs(a->val[j]);
+        if (maxndiag < aa) maxndiag = aa;
+      }
+    maxndiag = maxndiag * alpha;
+    for (int j = st; j <= en; j++) {
+      double aa = fabs(a->val[j]);
+      if (a->indj[j] != i)
+        a->strong[j] = aa > maxndiag;
+      else
+        a->strong[j] = a->diag[i] > maxndiag;
+    }
+  }
+}
+
+/* SpMtx_build_refs_symm with sortdown (SpMtx_arrangement.F90:461-568):
+ * strong off-diagonal adjacency, each row ordered by |a~| descending with a
+ * stable insertion (strict >). */
+static void om_build_refs_symm(OM *a) {
+  int nn = a->nrows;
+  long nnz = a->nnz;
+  int *cnt = IALLOC(nn);
+  for (long i = 1; i <= nnz; i++)
+    if (a->strong[i]) {
+      int ii = a->indi[i], jj = a->indj[i];
+      if (ii != jj && jj <= nn) cnt[ii]++;
+    }
+  a->srow = IALLOC(nn + 1);
+  a->srow[1] = 1;
+  for (int i = 1; i <= nn; i++) a->srow[i + 1] = a->srow[i] + cnt[i];
+  long noff = a->srow[nn + 1] - 1;
+  a->scol = IALLOC(noff);
+  memset(cnt, 0, sizeof(int) * (nn + 1));
+  long *sortref = (long *)xcalloc((size_t)noff + 1, sizeof(long));
+  for (long i = 1; i <= nnz; i++) {
+    if (!a->strong[i]) continue;
+    int ii = a->indi[i], jj = a->indj[i];
+    if (ii == jj || jj > nn) continue;
+    long ind_beg = a->srow[ii];
+    if (cnt[ii] == 0) {
+      sortref[ind_beg] = i;
+      cnt[ii] = 1;
+    } else {
+      long ind = a->srow[ii] + cnt[ii] - 1;
+      long j = ind_beg;
+      for (;;) {
+        if (j == ind + 1) {
+          sortref[j] = i;
+          cnt[ii]++;
+          break;
+        } else if (fabs(a->val[i]) > fabs(a->val[sortref[j]])) {
+          for (long k = ind + 1; k >= j + 1; k--) sortref[k] = sortref[k - 1];
+          sortref[j] = i;
+          cnt[ii]++;
+          break;
+        } else {
+          j++;
+        }
+      }
+    }
+  }
+  for (long i = 1; i <= noff; i++) a->scol[i] = a->indj[sortref[i]];
+  free(sortref);
+  free(cnt);
+}
+
+/* ------------------------------------------------------------------ */
+/* Aggregates                                                           */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int nagr, n;
+  int *num;    /* 1..n */
+  int *starts; /* 1..nagr+1 */
+  int *nodes;  /* 1..starts[nagr+1]-1 */
+} Aggrs;
+
+static void aggrs_free(Aggrs *a) {
+  free(a->num); free(a->starts); free(a->nodes);
+  memset(a, 0, sizeof *a);
+}
+
+/* Form_Aggr (Aggregate_mod.F90:82-125): CSR of the aggregates in node order. */
+static void form_aggr(Aggrs *ag, int nagrs, int n, const int *aggrnum) {
+  ag->nagr = nagrs;
+  ag->n = n;
+  ag->starts = IALLOC(nagrs + 1);
+  int *stat = IALLOC(nagrs);
+  for (int i = 1; i <= n; i++)
+    if (aggrnum[i] > 0) stat[aggrnum[i]]++;
+  ag->starts[1] = 1;
+  for (int i = 1; i <= nagrs; i++) {
+    ag->starts[i + 1] = ag->starts[i] + stat[i];
+    stat[i] = ag->starts[i];
+  }
+  ag->nodes = IALLOC(ag->starts[nagrs + 1] - 1);
+  for (int i = 1; i <= n; i++) {
+    int j = aggrnum[i];
+    if (j > 0) ag->nodes[stat[j]++] = i;
+  }
+  ag->num = IALLOC(n);
+  memcpy(ag->num + 1, aggrnum + 1, sizeof(int) * n);
+  free(stat);
+}
+
+/* work buffers of lets_colour (Aggregate_mod.F90:287-321) */
+typedef struct {
+  int *inlayer, *newlayer, *roundernodes;
+  int cap;
+} Lbuf;
+
+static void lcheck(const Lbuf *b, int idx) {
+  if (idx > b->cap) {
+    fprintf(stderr, "doug_oracle: aggregation layer buffer overflow (%d > %d)\n", idx, b->cap);
+    abort();
+  }
+}
+
+/* can_add_roundlayer (Aggregate_mod.F90:323-497) */
+static int can_add_roundlayer(int layer, int *ninnodes, Lbuf *b, int neighood,
+                              int minasize, int maxasize, int *stat,
+                              int *distance, int *nneigs, int *nodes,
+                              const int *rowstart, const int *colnrs) {
+  (void)minasize;
+  int *inlayer = b->inlayer, *newlayer = b->newlayer, *rounder = b->roundernodes;
+  int nxtlayer = layer + 1;
+  int layerfits = 1;
+  int ntoadd = 0, nrounders = 0;
+  if (layerfits && layer <= 2 * neighood) {
+    for (int k = 1; k <= *ninnodes; k++) {
+      int layernode = inlayer[k];
+      for (int i = rowstart[layernode]; i <= rowstart[layernode + 1] - 1; i++) {
+        int cn = colnrs[i];
+        if (stat[cn] <= 0) {
+          if (stat[cn] == 0) {
+            ntoadd++;
+            lcheck(b, ntoadd);
+            newlayer[ntoadd] = cn;
+            distance[cn] = distance[layernode] + 1;
+          }
+          stat[cn] = stat[cn] - 1;
+        }
+      }
+    }
+  }
+  if (layerfits && layer <= neighood) {
+    nrounders = 0;
+    int mind = D_MAXINT, maxd = 0;
+    for (int k = 1; k <= ntoadd; k++) {
+      int d = -stat[newlayer[k]];
+      if (d < mind) mind = d;
+      if (d > maxd) maxd = d;
+    }
+    if (mind < g_allmind) g_allmind = mind;
+    if (mind < maxd || mind > g_allmind) {
+      for (int k = 1; k <= ntoadd; k++) {
+        if (stat[newlayer[k]] == -maxd) {
+          nrounders++;
+          if (*nneigs + *ninnodes + 1 > maxasize) {
+            layerfits = 0;
+            break;
+          }
+          (*ninnodes)++;
+          lcheck(b, *ninnodes);
+          inlayer[*ninnodes] = newlayer[k];
+          if (nrounders > maxasize) {
+            fprintf(stderr, "doug_oracle: rounder buffer overflow\n");
+            abort();
+          }
+          rounder[nrounders] = newlayer[k];
+          newlayer[k] = -1;
+        }
+      }
+    }
+    if (!layerfits) nrounders = 0;
+    int j = 0;
+    for (int i = 1; i <= ntoadd; i++)
+      if (newlayer[i] != -1) {
+        j++;
+        if (j < i) newlayer[j] = newlayer[i];
+      }
+    ntoadd = j;
+    for (int k = 1; k <= nrounders; k++) {
+      int layernode = rounder[k];
+      for (int i = rowstart[layernode]; i <= rowstart[layernode + 1] - 1; i++) {
+        int cn = colnrs[i];
+        if (stat[cn] == 0) {
+          ntoadd++;
+          lcheck(b, ntoadd);
+          newlayer[ntoadd] = cn;
+          distance[cn] = distance[layernode] + 1;
+        }
+      }
+    }
+  }
+  if (layer <= neighood && *nneigs + *ninnodes > maxasize) layerfits = 0;
+  if (layerfits) {
+    for (int k = 1; k <= *ninnodes; k++) {
+      stat[inlayer[k]] = layer + 1;
+      (*nneigs)++;
+      nodes[*nneigs] = inlayer[k];
+    }
+    if (layer <= 2 * neighood) {
+      for (int k = 1; k <= ntoadd; k++) stat[newlayer[k]] = D_PENDING - 1;
+      *ninnodes = ntoadd;
+      for (int k = 1; k <= ntoadd; k++) inlayer[k] = newlayer[k];
+      return can_add_roundlayer(nxtlayer, ninnodes, b, neighood, minasize,
+                                maxasize, stat, distance, nneigs, nodes,
+                                rowstart, colnrs);
+    }
+    return 3;
+  }
+  for (int k = 1; k <= *ninnodes; k++) distance[inlayer[k]] = 0;
+  for (int k = 1; k <= ntoadd; k++) distance[newlayer[k]] = 0;
+  for (int k = 1; k <= *nneigs; k++) distance[nodes[k]] = 0;
+  for (int k = 1; k <= *ninnodes; k++) {
+    (*nneigs)++;
+    nodes[*nneigs] = inlayer[k];
+  }
+  if (*nneigs > 0) return layer > neighood ? 2 : 1;
+  return 0;
+}
+
+/* lets_colour (Aggregate_mod.F90:287-321) */
+static int lets_colour(int innode, int neighood, int minasize, int maxasize,
+                       int *nneigs, int *nodes, int *stat, int *distance,
+                       const int *rowstart, const int *colnrs, Lbuf *b) {
+  *nneigs = 0;
+  stat[innode] = 1;
+  distance[innode] = 0;
+  int ninnodes = 1;
+  b->inlayer[1] = innode;
+  return can_add_roundlayer(0, &ninnodes, b, neighood, minasize, maxasize,
+                            stat, distance, nneigs, nodes, rowstart, colnrs);
+}
+
+/* SpMtx_aggregate (SpMtx_aggregation.F90:49-641) for numprocs == 1.
+ * On entry A is the scaled matrix carrying its strong mask; `has_fine`
+ * selects the coarse-level behaviour (aggr_fine present). */
+static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
+                         int minasize, int maxasize, double alpha, int has_fine) {
+  int n = A->nrows;
+  int *aggrnum = IALLOC(n), *fullaggrnum = IALLOC(n);
+  int nagrs = 0, toosmall = 0, full_nagrs_new = 0;
+  int maxasizelargest;
+  int sn = (int)sqrtf((float)n);
+  if (alpha >= 0 || sn * sn != n) {
+    if (!A->srow) om_build_refs_symm(A);
+    int *stat = IALLOC(n), *nodes = IALLOC(n), *distance = IALLOC(n);
+    if (has_fine)
+      maxasizelargest = 3 * maxasize;
+    else
+      maxasizelargest = maxasize + 4 * (2 * neighood);
+    if (!has_fine) g_beta = alpha / 4.0;
+    int ngoodstarts = 0;
+    int *nextgoodstart = IALLOC(n);
+    int firstwocol = 1;
+    long *stamp = (long *)xcalloc((size_t)n + 1, sizeof(long));
+    long stamp_id = 0;
+    Lbuf lb;
+    long virt = 2L * maxasize * maxasize;
+    long capl = virt > n ? virt : n;
+    if (capl > 4L * n + 16) capl = 4L * n + 16;
+    lb.cap = (int)capl;
+    lb.inlayer = IALLOC(lb.cap);
+    lb.newlayer = IALLOC(lb.cap);
+    lb.roundernodes = IALLOC(maxasize);
+    for (;;) {
+      if (firstwocol > n) break;
+      if (ngoodstarts > 0) {
+        int j = 0;
+        for (int i = 1; i <= ngoodstarts; i++)
+          if (stat[nextgoodstart[i]] < D_PENDING) {
+            j++;
+            if (j < i) nextgoodstart[j] = nextgoodstart[i];
+          }
+        ngoodstarts = j;
+      }
+      int startnode;
+      if (ngoodstarts == 0) {
+        int done = 0;
+        while (stat[firstwocol] >= D_PENDING) {
+          firstwocol++;
+          if (firstwocol > n) { done = 1; break; }
+        }
+        if (done) break;
+        startnode = firstwocol;
+      } else {
+        /* ng4 (SpMtx_aggregation.F90:209-216): the first entry i whose value
+         * already occurred at some j < i; the pairwise scan is replaced by a
+         * per-node stamp, which selects the same entry in linear time */
+        startnode = 0;
+        stamp_id++;
+        for (int i = 1; i <= ngoodstarts; i++) {
+          int v = nextgoodstart[i];
+          if (stamp[v] == stamp_id) {
+            startnode = v;
+            break;
+          }
+          stamp[v] = stamp_id;
+        }
+        if (startnode == 0) startnode = nextgoodstart[1];
+      }
+      int nneigs = 0;
+      int ok = lets_colour(startnode, neighood, minasize, maxasize, &nneigs,
+                           nodes, stat, distance, A->srow, A->scol, &lb);
+      int mindistance = D_MAXINT;
+      if (ok > 0) {
+        int next_start_layer;
+        int nouters = 0;
+        nagrs++;
+        if (ok == 3) {
+          int nl = 2 * neighood + 2;
+          int *layerlen = IALLOC(nl);
+          int maxlayer = 0;
+          if (!toosmall && nneigs < minasize) toosmall = 1;
+          for (int j = nneigs; j >= 1; j--) {
+            int k = stat[nodes[j]];
+            if (k > maxlayer) maxlayer = k;
+            if (k <= neighood + 1) break;
+            layerlen[k]++;
+          }
+          next_start_layer = neighood + 2;
+          int k = layerlen[neighood + 2];
+          for (int j = neighood + 3; j <= maxlayer; j++)
+            if (k < layerlen[j]) {
+              k = layerlen[j];
+              next_start_layer = j;
+            }
+          free(layerlen);
+        } else if (ok == 2) {
+          next_start_layer = neighood + 2;
+          if (!toosmall && nneigs < minasize) toosmall = 1;
+        } else {
+          next_start_layer = 0;
+        }
+        for (int j = 1; j <= nneigs; j++) {
+          int node = nodes[j];
+          if (stat[node] <= neighood + 1 && j <= maxasize) {
+            stat[node] = D_AGGREGATED;
+            aggrnum[node] = nagrs;
+          } else if (next_start_layer > neighood + 1) {
+            if (stat[node] == next_start_layer) {
+              if (distance[node] < mindistance) mindistance = distance[node];
+              nouters++;
+              nextgoodstart[n - nouters + 1] = node;
+            }
+          }
+        }
+        for (int j = 1; j <= nouters; j++)
+          if (distance[nextgoodstart[n - j + 1]] == mindistance) {
+            ngoodstarts++;
+            nextgoodstart[ngoodstarts] = nextgoodstart[n - j + 1];
+          }
+        for (int j = 1; j <= nneigs; j++) {
+          int node = nodes[j];
+          if (stat[node] != D_AGGREGATED) {
+            distance[node] = 0;
+            stat[node] = 0;
+          }
+        }
+      } else {
+        fprintf(stderr, "doug_oracle: aggregation produced nothing\n");
+      }
+    }
+    free(nextgoodstart); free(distance); free(nodes); free(stat); free(stamp);
+    free(lb.inlayer); free(lb.newlayer); free(lb.roundernodes);
+  } else {
+    /* Cartesian aggregates (SpMtx_aggregation.F90:333-346) */
+    int nsa = (sn + neighood - 1) / neighood;
+    int k = 0;
+    for (int i = 1; i <= sn; i++)
+      for (int j = 1; j <= sn; j++) {
+        k++;
+        aggrnum[k] = ((i + neighood - 1) / neighood - 1) * nsa + (j + neighood - 1) / neighood;
+      }
+    nagrs = 0;
+    for (int i = 1; i <= n; i++)
+      if (aggrnum[i] > nagrs) nagrs = aggrnum[i];
+    maxasizelargest = 0;
+  }
+
+  if (!toosmall) {
+    memcpy(fullaggrnum + 1, aggrnum + 1, sizeof(int) * n);
+    full_nagrs_new = 0;
+    for (int i = 1; i <= n; i++)
+      if (fullaggrnum[i] > full_nagrs_new) full_nagrs_new = fullaggrnum[i];
+    form_aggr(inner, nagrs, n, aggrnum);
+  } else {
+    /* merge too-small aggregates (SpMtx_aggregation.F90:365-591) */
+    int *aggrsize = IALLOC(nagrs);
+    for (int i = 1; i <= n; i++) aggrsize[aggrnum[i]]++;
+    int maxsz = 0;
+    for (int i = 1; i <= nagrs; i++)
+      if (aggrsize[i] > maxsz) maxsz = aggrsize[i];
+    int maxstructlen = maxasizelargest > maxsz ? maxasizelargest : maxsz;
+    int *structnodes = (int *)xcalloc((size_t)(maxstructlen + 1) * (size_t)(nagrs + 1), sizeof(int));
+#define SN(r, c) structnodes[(size_t)(c) * (size_t)(maxstructlen + 1) + (size_t)(r)]
+    memset(aggrsize, 0, sizeof(int) * (nagrs + 1));
+    for (int i = 1; i <= n; i++) {
+      int j = aggrnum[i];
+      aggrsize[j]++;
+      SN(aggrsize[j], j) = i;
+    }
+    int mnstructneigs = 4 * maxasize;
+    om_arrange(A, ARR_ROWS, 0, 0, 0);
+    int *colsaround = IALLOC(mnstructneigs);
+    double *connw = DALLOC(mnstructneigs);
+    int nagrs_new = nagrs;
+    for (int i = 1; i <= nagrs; i++) {
+      if (aggrsize[i] >= minasize) continue;
+      int ncolsaround = 0;
+      memset(colsaround, 0, sizeof(int) * (mnstructneigs + 1));
+      for (int k = 1; k <= aggrsize[i]; k++) {
+        int kkk = SN(k, i);
+        for (int j = A->bound[kkk]; j <= A->bound[kkk + 1] - 1; j++) {
+          int cn = A->indj[j];
+          if (cn > n) continue;
+          int colr = aggrnum[cn];
+          if (colr > 0 && colr != i) {
+            int kk;
+            for (kk = 1; kk <= ncolsaround; kk++)
+              if (colr == colsaround[kk]) {
+                connw[kk] = connw[kk] + fabs(A->val[j]);
+                break;
+              }
+            if (kk > ncolsaround) {
+              ncolsaround++;
+              if (ncolsaround > mnstructneigs) {
+                fprintf(stderr, "doug_oracle: mnstructneigs too small\n");
+                abort();
+              }
+              connw[kk] = fabs(A->val[j]);
+              colsaround[kk] = colr;
+            }
+          }
+        }
+      }
+      double maxcw = 0.0;
+      int eater = 0;
+      for (int j = 1; j <= ncolsaround; j++)
+        if (maxcw < connw[j] && aggrsize[colsaround[j]] + aggrsize[i] <= maxasize) {
+          maxcw = connw[j];
+          eater = colsaround[j];
+        }
+      if (maxcw >= alpha || (has_fine && maxcw >= g_beta)) {
+        for (int j = 1; j <= aggrsize[i]; j++) {
+          aggrsize[eater]++;
+          if (aggrsize[eater] > maxstructlen) {
+            fprintf(stderr, "doug_oracle: structnodes overflow\n");
+            abort();
+          }
+          SN(aggrsize[eater], eater) = SN(j, i);
+          aggrnum[SN(j, i)] = eater;
+        }
+        aggrsize[i] = 0;
+        nagrs_new--;
+      } else {
+        int nleft = aggrsize[i];
+        int reduced = 1;
+        while (nleft > 0 && reduced) {
+          reduced = 0;
+          for (int k = aggrsize[i]; k >= 1; k--) {
+            int kkk = SN(k, i);
+            if (kkk <= 0) continue;
+            for (int j = A->bound[kkk]; j <= A->bound[kkk + 1] - 1; j++) {
+              int cn = A->indj[j];
+              if (cn > n) continue;
+              int colr = aggrnum[cn];
+              if (colr != i && aggrsize[colr] < maxasizelargest && fabs(A->val[j]) >= g_beta) {
+                aggrsize[colr]++;
+                if (aggrsize[colr] > maxstructlen) {
+                  fprintf(stderr, "doug_oracle: structnodes overflow\n");
+                  abort();
+                }
+                SN(aggrsize[colr], colr) = SN(k, i);
+                aggrnum[SN(k, i)] = colr;
+                SN(k, i) = -1;
+                nleft--;
+                reduced = 1;
+                break;
+              }
+            }
+          }
+        }
+        if (nleft > 0 && nleft < aggrsize[i]) {
+          int k = 0;
+          for (int j = 1; j <= aggrsize[i]; j++)
+            if (SN(j, i) > 0) {
+              k++;
+              if (k > 0 && k < j) SN(k, i) = SN(j, i);
+            }
+          aggrsize[i] = nleft;
+        } else if (nleft == 0) {
+          aggrsize[i] = 0;
+          nagrs_new--;
+        }
+      }
+    }
+    long naggregated = 0;
+    for (int i = 1; i <= nagrs; i++) naggregated += aggrsize[i];
+    full_nagrs_new = nagrs_new;
+    int cnt = 0;
+    for (int i = 1; i <= n; i++)
+      if (aggrnum[i] > cnt) cnt = aggrnum[i];
+    if (cnt > nagrs_new) {
+      int *st = IALLOC(cnt);
+      int col = 0;
+      for (int i = 1; i <= n; i++)
+        if (aggrnum[i] > 0) {
+          int thiscol;
+          if (st[aggrnum[i]] == 0) {
+            col++;
+            st[aggrnum[i]] = col;
+            thiscol = col;
+          } else {
+            thiscol = st[aggrnum[i]];
+          }
+          aggrnum[i] = thiscol;
+        }
+      free(st);
+    }
+    if (n == naggregated) {
+      memcpy(fullaggrnum + 1, aggrnum + 1, sizeof(int) * n);
+      full_nagrs_new = 0;
+      for (int i = 1; i <= n; i++)
+        if (fullaggrnum[i] > full_nagrs_new) full_nagrs_new = fullaggrnum[i];
+    } else {
+      for (int i = 1; i <= n; i++) {
+        if (aggrnum[i] > 0)
+          fullaggrnum[i] = aggrnum[i];
+        else
+          fullaggrnum[i] = ++full_nagrs_new;
+      }
+    }
+    form_aggr(inner, nagrs_new, n, aggrnum);
+    free(connw); free(colsaround); free(structnodes); free(aggrsize);
+#undef SN
+  }
+  form_aggr(full, full_nagrs_new, n, fullaggrnum);
+  free(fullaggrnum);
+  free(aggrnum);
+}
+
+/* ------------------------------------------------------------------ */
+/* Products                                                             */
+/* ------------------------------------------------------------------ */
+/* SpMtx_AB (SpMtx_op_AB.f90:39-200): Gustavson row loop, columns of each
+ * output row in first-touch order, partial products accumulated in
+ * traversal order.  Arranges A by rows (or columns if AT) and B likewise. */
+static OM *om_AB(OM *A, OM *B, int AT, int BT) {
+  int *Aindi, *Aindj, *Bindi, *Bindj;
+  int Anrows, Ancols, Bnrows, Bncols;
+  int mx;
+  if (AT) {
+    Aindi = A->indj; Aindj = A->indi;
+    mx = 0; for (long i = 1; i <= A->nnz; i++) if (A->indj[i] > mx) mx = A->indj[i];
+    Anrows = A->ncols > mx ? A->ncols : mx;
+    mx = 0; for (long i = 1; i <= A->nnz; i++) if (A->indi[i] > mx) mx = A->indi[i];
+    Ancols = A->nrows > mx ? A->nrows : mx;
+    om_arrange(A, ARR_COLS, 0, Ancols, Anrows);
+  } else {
+    mx = 0; for (long i = 1; i <= A->nnz; i++) if (A->indi[i] > mx) mx = A->indi[i];
+    Anrows = A->nrows > mx ? A->nrows : mx;
+    mx = 0; for (long i = 1; i <= A->nnz; i++) if (A->indj[i] > mx) mx = A->indj[i];
+    Ancols = A->ncols > mx ? A->ncols : mx;
+    om_arrange(A, ARR_ROWS, 0, Anrows, Ancols);
+  }
+  if (BT) {
+    mx = 0; for (long i = 1; i <= B->nnz; i++) if (B->indj[i] > mx) mx = B->indj[i];
+    Bnrows = B->ncols > mx ? B->ncols : mx;
+    mx = 0; for (long i = 1; i <= B->nnz; i++) if (B->indi[i] > mx) mx = B->indi[i];
+    Bncols = B->nrows > mx ? B->nrows : mx;
+    om_arrange(B, ARR_COLS, 0, Bncols, Bnrows);
+  } else {
+    mx = 0; for (long i = 1; i <= B->nnz; i++) if (B->indi[i] > mx) mx = B->indi[i];
+    Bnrows = B->nrows > mx ? B->nrows : mx;
+    mx = 0; for (long i = 1; i <= B->nnz; i++) if (B->indj[i] > mx) mx = B->indj[i];
+    Bncols = B->ncols > mx ? B->ncols : mx;
+    om_arrange(B, ARR_ROWS, 0, Bnrows, Bncols);
+  }
+  /* index pointers must be re-read: arrange replaced nothing but reordered */
+  if (AT) { Aindi = A->indj; Aindj = A->indi; } else { Aindi = A->indi; Aindj = A->indj; }
+  if (BT) { Bindi = B->indj; Bindj = B->indi; } else { Bindi = B->indi; Bindj = B->indj; }
+  (void)Aindi; (void)Bindi;
+  if (Ancols != Bnrows) {
+    fprintf(stderr, "doug_oracle: A*B impossible (%d vs %d)\n", Ancols, Bnrows);
+    abort();
+  }
+  /* exact count first (the reference grows an estimate and retries) */
+  int *coladder = IALLOC(Bncols), *indeces = IALLOC(Bncols);
+  long total = 0;
+  for (int i = 1; i <= Anrows; i++) {
+    for (int k = A->bound[i]; k <= A->bound[i + 1] - 1; k++) {
+      int kk = Aindj[k];
+      for (int j = B->bound[kk]; j <= B->bound[kk + 1] - 1; j++) {
+        int jj = Bindj[j];
+        if (coladder[jj] != i) {
+          coladder[jj] = i;
+          total++;
+        }
+      }
+    }
+  }
+  OM *C = om_new(total, Anrows, Bncols);
+  C->arr = ARR_ROWS;
+  C->nbound = Anrows + 1;
+  C->bound = IALLOC(Anrows + 1);
+  memset(coladder, 0, sizeof(int) * (Bncols + 1));
+  int *colind = IALLOC(Bncols);
+  double *indvals = DALLOC(Bncols);
+  long counter = 1, nnz = 0;
+  for (int i = 1; i <= Anrows; i++) {
+    C->bound[i] = (int)counter;
+    int nind = 0;
+    for (int k = A->bound[i]; k <= A->bound[i + 1] - 1; k++) {
+      int kk = Aindj[k];
+      for (int j = B->bound[kk]; j <= B->bound[kk + 1] - 1; j++) {
+        int jj = Bindj[j];
+        double prod = A->val[k] * B->val[j];
+        if (coladder[jj] != i) {
+          coladder[jj] = i;
+          nind++;
+          indeces[jj] = nind;
+          colind[nind] = jj;
+          indvals[nind] = prod;
+          nnz++;
+        } else {
+          int kkk = indeces[jj];
+          indvals[kkk] = indvals[kkk] + prod;
+        }
+      }
+    }
+    for (int t = 1; t <= nind; t++) {
+      C->indi[counter + t - 1] = i;
+      C->indj[counter + t - 1] = colind[t];
+      C->val[counter + t - 1] = indvals[t];
+    }
+    counter += nind;
+  }
+  C->bound[Anrows + 1] = (int)counter;
+  free(coladder); free(indeces); free(colind); free(indvals);
+  return C;
+}
+
+/* KeepGivenColumnIndeces with inds = 1..ninner, keepShape (SpMtx_util.f90:232-286) */
+static void om_keep_columns_upto(OM *A, int ninner) {
+  long nz = 0;
+  for (long i = 1; i <= A->nnz; i++)
+    if (A->indj[i] >= 1 && A->indj[i] <= ninner) {
+      nz++;
+      A->indi[nz] = A->indi[i];
+      A->indj[nz] = A->indj[i];
+      A->val[nz] = A->val[i];
+    }
+  A->nnz = nz;
+  if (A->arr != ARR_NO) {
+    free(A->bound);
+    A->bound = NULL;
+    A->nbound = 0;
+    A->arr = ARR_NO;
+  }
+}
+
+/* IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 and 1. */
+static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
+  int nagr = ag->nagr;
+  long nz = ag->starts[nagr + 1] - 1;
+  int Tnrows = A->nrows;
+  OM *T = om_new(nz, nagr, Tnrows);
+  for (int i = 1; i <= nagr; i++)
+    for (int j = ag->starts[i]; j <= ag->starts[i + 1] - 1; j++) T->indi[j] = i;
+  for (long j = 1; j <= nz; j++) {
+    T->indj[j] = ag->nodes[j];
+    T->val[j] = 1.0;
+  }
+  T->arr = ARR_NO;
+  T->nbound = nagr + 1;
+  T->bound = IALLOC(nagr + 1);
+  memcpy(T->bound + 1, ag->starts + 1, sizeof(int) * (nagr + 1));
+  if (smoothers == 0) return T;
+
+  int nd = A->nrows > A->ncols ? A->nrows : A->ncols;
+  double *diag = DALLOC(nd);
+  for (long i = 1; i <= A->nnz; i++) {
+    if (A->indi[i] == A->indj[i])
+      diag[A->indi[i]] = diag[A->indi[i]] + A->val[i];
+    else if (!A->strong[i])
+      diag[A->indi[i]] = diag[A->indi[i]] + A->val[i];
+  }
+  OM *S = om_new(A->nnz, Tnrows, Tnrows);
+  double one_minus = 1.0 - omega;
+  for (long i = 1; i <= A->nnz; i++) {
+    S->indi[i] = A->indi[i];
+    S->indj[i] = A->indj[i];
+    if (A->indi[i] == A->indj[i])
+      S->val[i] = one_minus;
+    else if (A->strong[i])
+      S->val[i] = -omega / diag[A->indi[i]] * A->val[i];
+    else
+      S->val[i] = 0.0;
+  }
+  S->arr = ARR_NO;
+  free(diag);
+  OM *R = om_AB(T, S, 0, 1);
+  om_free(S);
+  om_free(T);
+  return R;
+}
+
+/* CoarseMtxBuild (CoarseMtx.F90:476-516): A_c = R (A R^T). */
+static OM *coarse_mtx_build(OM *A, OM *R, int ninner) {
+  OM *TT = om_copy(A);
+  OM *T = om_AB(TT, R, 0, 1);
+  om_free(TT);
+  OM *RT = om_copy(R);
+  om_keep_columns_upto(RT, ninner);
+  OM *AC = om_AB(RT, T, 0, 0);
+  om_free(RT);
+  om_free(T);
+  return AC;
+}
+
+/* Get_aggregate_nodes (Aggregate_mod.F90:162-190) */
+static int get_aggregate_nodes(int cAggr, const Aggrs *c, const Aggrs *f,
+                               int *nodeLoc, int *nodes) {
+  int nnodes = 0;
+  for (int i = c->starts[cAggr]; i <= c->starts[cAggr + 1] - 1; i++) {
+    int fa = c->nodes[i];
+    for (int k = f->starts[fa]; k <= f->starts[fa + 1] - 1; k++) {
+      int node = f->nodes[k];
+      if (nodeLoc[node] == 0) {
+        nnodes++;
+        nodeLoc[node] = nnodes;
+        nodes[nnodes] = node;
+      }
+    }
+  }
+  for (int k = 1; k <= nnodes; k++) nodeLoc[nodes[k]] = 0;
+  return nnodes;
+}
+
+/* Add_layers (Decomposition.f90:56-99); onfront sized from the node count */
+static int add_layers(const int *bounds, const int *adj, int n, int *nodes,
+                      int nnodes, int nlayers, int *onfront) {
+  int *fs = IALLOC(nlayers + 1), *fe = IALLOC(nlayers + 1);
+  /* fs/fe indices 0..nlayers stored at +1 */
+  fs[1] = 1;
+  for (int i = 1; i <= nnodes; i++) onfront[nodes[i]] = -1;
+  int nfront = nnodes;
+  fe[1] = nfront;
+  for (int layer = 1; layer <= nlayers; layer++) {
+    fs[layer + 1] = nfront;
+    for (int i = fs[layer]; i <= fe[layer]; i++) {
+      int node = nodes[i];
+      for (int j = bounds[node]; j <= bounds[node + 1] - 1; j++) {
+        int neigh = adj[j];
+        if (onfront[neigh] == 0) {
+          onfront[neigh] = layer;
+          nfront++;
+          nodes[nfront] = neigh;
+        }
+      }
+    }
+    fe[layer + 1] = nfront;
+  }
+  int on = fe[nlayers + 1];
+  for (int i = 1; i <= on; i++) onfront[nodes[i]] = 0;
+  (void)n;
+  free(fs); free(fe);
+  return on;
+}
+
+int or_add_layers(const int *bounds, const int *adj, int n, int *nodes,
+                  int nnodes, int nlayers) {
+  /* public wrapper: bounds/adj/nodes are 1-based arrays passed as 0-offset
+   * C arrays holding Fortran element k at [k-1] */
+  int *onfront = IALLOC(n);
+  int nb = n + 1;
+  int *b1 = IALLOC(nb);
+  for (int i = 1; i <= nb; i++) b1[i] = bounds[i - 1];
+  int na = b1[nb] - 1;
+  int *a1 = IALLOC(na);
+  for (int i = 1; i <= na; i++) a1[i] = adj[i - 1];
+  int *nd1 = IALLOC(n);
+  for (int i = 1; i <= nnodes; i++) nd1[i] = nodes[i - 1];
+  int on = add_layers(b1, a1, n, nd1, nnodes, nlayers, onfront);
+  for (int i = 1; i <= on; i++) nodes[i - 1] = nd1[i];
+  free(onfront); free(b1); free(a1); free(nd1);
+  return on;
+}
+
+/* ------------------------------------------------------------------ */
+/* Exact sparse SPD solves: envelope Cholesky after reverse Cuthill-McKee */
+/* (stands in for UMFPACK's exact solve, src/solvers/Fact.F90:64-281)   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int n;
+  int *perm;     /* new -> old (0-based) */
+  long *rowptr;  /* row i stores columns first[i]..i */
+  int *first;
+  double *L;
+} Env;
+
+static void env_free(Env *e) {
+  if (!e) return;
+  free(e->perm); free(e->rowptr); free(e->first); free(e->L);
+  free(e);
+}
+
+/* local COO (0-based) -> envelope factor; returns NULL if not SPD */
+static Env *env_factor(int n, long nz, const int *ci, const int *cj, const double *cv) {
+  /* adjacency (symmetrised pattern) */
+  int *deg = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  for (long t = 0; t < nz; t++)
+    if (ci[t] != cj[t]) { deg[ci[t]]++; deg[cj[t]]++; }
+  long *ap = (long *)xcalloc((size_t)n + 1, sizeof(long));
+  for (int i = 0; i < n; i++) ap[i + 1] = ap[i] + deg[i];
+  int *adj = (int *)xcalloc((size_t)ap[n] + 1, sizeof(int));
+  long *fill = (long *)xcalloc((size_t)n + 1, sizeof(long));
+  for (int i = 0; i < n; i++) fill[i] = ap[i];
+  for (long t = 0; t < nz; t++)
+    if (ci[t] != cj[t]) { adj[fill[ci[t]]++] = cj[t]; adj[fill[cj[t]]++] = ci[t]; }
+  free(fill);
+  /* RCM over every component */
+  int *order = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  char *seen = (char *)xcalloc((size_t)n + 1, 1);
+  int head = 0, tail = 0;
+  int *tmp = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  for (int s0 = 0; s0 < n; s0++) {
+    if (seen[s0]) continue;
+    /* pick a low-degree start inside this component: BFS once, take the last
+     * node reached (a pseudo-peripheral node) */
+    int start = s0;
+    {
+      int h = tail, t2 = tail;
+      order[t2++] = s0; seen[s0] = 1;
+      while (h < t2) { int v = order[h++]; for (long k = ap[v]; k < ap[v + 1]; k++) if (!seen[adj[k]]) { seen[adj[k]] = 1; order[t2++] = adj[k]; } }
+      start = order[t2 - 1];
+      for (int k = tail; k < t2; k++) seen[order[k]] = 0;
+    }
+    head = tail;
+    order[tail++] = start;
+    seen[start] = 1;
+    while (head < tail) {
+      int v = order[head++];
+      int m = 0;
+      for (long k = ap[v]; k < ap[v + 1]; k++)
+        if (!seen[adj[k]]) { seen[adj[k]] = 1; tmp[m++] = adj[k]; }
+      /* insertion sort by degree */
+      for (int a = 1; a < m; a++) {
+        int x = tmp[a], b = a - 1;
+        while (b >= 0 && deg[tmp[b]] > deg[x]) { tmp[b + 1] = tmp[b]; b--; }
+        tmp[b + 1] = x;
+      }
+      for (int a = 0; a < m; a++) order[tail++] = tmp[a];
+    }
+  }
+  free(tmp); free(seen);
+  Env *e = (Env *)xcalloc(1, sizeof(Env));
+  e->n = n;
+  e->perm = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  int *inv = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  for (int i = 0; i < n; i++) { e->perm[i] = order[n - 1 - i]; }
+  for (int i = 0; i < n; i++) inv[e->perm[i]] = i;
+  free(order);
+  e->first = (int *)xcalloc((size_t)n + 1, sizeof(int));
+  for (int i = 0; i < n; i++) e->first[i] = i;
+  for (long t = 0; t < nz; t++) {
+    int a = inv[ci[t]], b = inv[cj[t]];
+    int r = a > b ? a : b, c = a < b ? a : b;
+    if (c < e->first[r]) e->first[r] = c;
+  }
+  e->rowptr = (long *)xcalloc((size_t)n + 1, sizeof(long));
+  for (int i = 0; i < n; i++) e->rowptr[i + 1] = e->rowptr[i] + (i - e->first[i] + 1);
+  e->L = (double *)xcalloc((size_t)e->rowptr[n] + 1, sizeof(double));
+#define LE(i, j) e->L[e->rowptr[i] + ((j) - e->first[i])]
+  for (long t = 0; t < nz; t++) {
+    int a = inv[ci[t]], b = inv[cj[t]];
+    if (a >= b) LE(a, b) += cv[t];  /* lower triangle; duplicates summed */
+  }
+  free(inv); free(adj); free(ap); free(deg);
+  for (int i = 0; i < n; i++) {
+    int fi = e->first[i];
+    for (int j = fi; j < i; j++) {
+      int fj = e->first[j];
+      int k0 = fi > fj ? fi : fj;
+      double s = LE(i, j);
+      for (int k = k0; k < j; k++) s -= LE(i, k) * LE(j, k);
+      LE(i, j) = s / LE(j, j);
+    }
+    double d = LE(i, i);
+    for (int k = fi; k < i; k++) d -= LE(i, k) * LE(i, k);
+    if (!(d > 0.0)) {
+      seterr("subdomain matrix not positive definite at pivot %d (%g)", i, d);
+      env_free(e);
+      return NULL;
+    }
+    LE(i, i) = sqrt(d);
+  }
+  return e;
+}
+
+static void env_solve(const Env *e, const double *b, double *x, double *w) {
+  int n = e->n;
+  for (int i = 0; i < n; i++) w[i] = b[e->perm[i]];
+  for (int i = 0; i < n; i++) {
+    double s = w[i];
+    for (int k = e->first[i]; k < i; k++) s -= LE(i, k) * w[k];
+    w[i] = s / LE(i, i);
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    w[i] = w[i] / LE(i, i);
+    double wi = w[i];
+    for (int k = e->first[i]; k < i; k++) w[k] -= LE(i, k) * wi;
+  }
+  for (int i = 0; i < n; i++) x[e->perm[i]] = w[i];
+}
+#undef LE
+
+/* ------------------------------------------------------------------ */
+/* State                                                                */
+/* ------------------------------------------------------------------ */
+struct or_state {
+  int n;
+  or_params p;
+  OM *A;            /* row arranged, after the scale/unscale round trip */
+  Aggrs fin, ffull; /* fine aggregates (inner/full) */
+  Aggrs cin, cfull; /* coarse aggregates (np=1 rule) */
+  OM *R, *AC;
+  int ol, smoothers;
+  int nsub;
+  int *sstarts;     /* 1..nsub+1 */
+  int *sinds;       /* 1..total */
+  Env **fact;       /* per subdomain */
+  Env *cfact;
+  int has_coarse;
+  double *w1, *w2, *w3, *w4;
+};
+
+static void state_free(or_state *s) {
+  if (!s) return;
+  om_free(s->A); om_free(s->R); om_free(s->AC);
+  aggrs_free(&s->fin); aggrs_free(&s->ffull); aggrs_free(&s->cin); aggrs_free(&s->cfull);
+  if (s->fact) for (int i = 0; i < s->nsub; i++) env_free(s->fact[i]);
+  free(s->fact);
+  env_free(s->cfact);
+  free(s->sstarts); free(s->sinds);
+  free(s->w1); free(s->w2); free(s->w3); free(s->w4);
+  free(s);
+}
+void or_free(or_state *s) { state_free(s); }
+
+/* Factorise_subdomain (subsolvers.F90:104-163): entries of A, in A's order,
+ * with both ends inside the subdomain, renumbered through the local map. */
+static Env *factor_subdomain(const OM *A, const int *inds, int nn, int *floc) {
+  for (int i = 1; i <= nn; i++) floc[inds[i]] = i;
+  long cnt = 0;
+  for (int t = 1; t <= nn; t++) {
+    int g = inds[t];
+    for (int j = A->bound[g]; j <= A->bound[g + 1] - 1; j++)
+      if (floc[A->indj[j]] > 0) cnt++;
+  }
+  int *ci = (int *)xcalloc((size_t)cnt + 1, sizeof(int));
+  int *cj = (int *)xcalloc((size_t)cnt + 1, sizeof(int));
+  double *cv = (double *)xcalloc((size_t)cnt + 1, sizeof(double));
+  long c = 0;
+  for (int t = 1; t <= nn; t++) {
+    int g = inds[t];
+    for (int j = A->bound[g]; j <= A->bound[g + 1] - 1; j++)
+      if (floc[A->indj[j]] > 0) {
+        ci[c] = floc[g] - 1;
+        cj[c] = floc[A->indj[j]] - 1;
+        cv[c] = A->val[j];
+        c++;
+      }
+  }
+  Env *e = env_factor(nn, cnt, ci, cj, cv);
+  free(ci); free(cj); free(cv);
+  for (int i = 1; i <= nn; i++) floc[inds[i]] = 0;
+  return e;
+}
+
+or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
+                   const double *val, const or_params *pp, int nparts,
+                   const int *pstarts, const int *pnodes) {
+  or_state *s = (or_state *)xcalloc(1, sizeof(or_state));
+  s->n = n;
+  s->p = *pp;
+  or_params *p = &s->p;
+  for (long i = 0; i < nnz; i++)
+    if (indi[i] < 1 || indi[i] > n || indj[i] < 1 || indj[i] > n) {
+      seterr("matrix entry %ld out of range (%d,%d), n=%d", i + 1, indi[i], indj[i], n);
+      state_free(s);
+      return NULL;
+    }
+  OM *A = om_new(nnz, n, n);
+  for (long i = 0; i < nnz; i++) {
+    A->indi[i + 1] = indi[i];
+    A->indj[i + 1] = indj[i];
+    A->val[i + 1] = val[i];
+  }
+  A->symmstruct = p->symmstruct;
+  A->symmnumeric = p->symmstruct && p->symmnumeric;
+  s->A = A;
+
+  /* aggr.F90:131: with one process the aggregates, the coarse matrix and the
+   * coarse-aggregate subdomains are built for levels 1 and 2 alike; only the
+   * coarse correction is skipped when levels == 1 */
+  int radius1 = 2;
+  double strong_conn1 = 0.67;
+  {
+    /* Partitionings_aggr_InitFine (Partitioning_aggr.F90:33-117) */
+    strong_conn1 = p->strong1 != 0.0 ? p->strong1 : 0.67;
+    radius1 = p->radius1 > 0 ? p->radius1 : 2;
+    int min_asize1 = p->minasize1 > 0 ? p->minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
+    int max_asize1 = p->maxasize1 > 0 ? p->maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
+    om_find_strong(A, strong_conn1);
+    om_aggregate(A, &s->fin, &s->ffull, radius1, min_asize1, max_asize1, strong_conn1, 0);
+    om_unscale(A);
+
+    /* CoarsePreconditioner_smooth_Init (CoarsePreconditioner_smooth.F90:47-89) */
+    int smoothers = p->smoothers;
+    if (smoothers > p->radius1) smoothers = p->radius1;  /* CoarseMtx.F90:75-78 */
+    s->smoothers = smoothers;
+    if (smoothers != 0 && smoothers != 1) {
+      seterr("smoothers=%d not supported on this path (0 or 1)", smoothers);
+      state_free(s);
+      return NULL;
+    }
+    s->R = int_rest_build(A, &s->fin, smoothers, p->omega);
+    s->AC = coarse_mtx_build(A, s->R, n);
+    s->has_coarse = 1;
+
+    if (nparts <= 0) {
+      /* Partitionings_aggr_InitCoarse (Partitioning_aggr.F90:120-193) */
+      double strong_conn2 = p->strong2 > 0 ? p->strong2 : strong_conn1 / 2.0;
+      om_find_strong(s->AC, strong_conn2);
+      int aggr_radius2;
+      if (p->radius2 > 0) {
+        aggr_radius2 = p->radius2;
+      } else {
+        int nn = (int)sqrt(1.0 * (double)A->nrows);
+        aggr_radius2 = (int)lround(3 * sqrt((double)nn) / (2 * radius1 + 1) - 1);
+      }
+      int min_asize2, max_asize2;
+      if (p->minasize2 > 0)
+        min_asize2 = p->minasize2;
+      else if (p->radius2 > 0)
+        min_asize2 = 2 * p->radius2 + 1;
+      else
+        min_asize2 = (int)(0.5 * (double)((2 * aggr_radius2 + 1) * (2 * aggr_radius2 + 1)));
+      max_asize2 = p->maxasize2 > 0 ? p->maxasize2 : (2 * aggr_radius2 + 1) * (2 * aggr_radius2 + 1);
+      om_aggregate(s->AC, &s->cin, &s->cfull, aggr_radius2, min_asize2, max_asize2, strong_conn2, 1);
+      om_unscale(s->AC);
+      /* cPart from coarse aggregates expanded to fine nodes */
+      s->nsub = s->cfull.nagr;
+      s->sstarts = IALLOC(s->nsub + 1);
+    }
+  }
+  /* overlap (aggr.F90:207-212) */
+  int ol = p->overlap < 0 ? (s->smoothers > 0 ? s->smoothers : 0) : p->overlap;
+  s->ol = ol;
+
+  /* partitions -> node lists */
+  int *partstarts, *partnodes, np;
+  if (nparts > 0) {
+    np = nparts;
+    partstarts = IALLOC(np + 1);
+    long tot = pstarts[np] - pstarts[0];
+    partnodes = IALLOC(tot);
+    for (int i = 0; i <= np; i++) partstarts[i + 1] = pstarts[i] - pstarts[0] + 1;
+    for (long i = 0; i < tot; i++) partnodes[i + 1] = pnodes[i];
+  } else if (s->has_coarse) {
+    np = s->cfull.nagr;
+    partstarts = IALLOC(np + 1);
+    partnodes = IALLOC(n);
+    int *nodeLoc = IALLOC(n), *nodes = IALLOC(n);
+    partstarts[1] = 1;
+    for (int c = 1; c <= np; c++) {
+      int nn = get_aggregate_nodes(c, &s->cfull, &s->ffull, nodeLoc, nodes);
+      int st = partstarts[c];
+      partstarts[c + 1] = st + nn;
+      for (int k = 1; k <= nn; k++) partnodes[st + k - 1] = nodes[k];
+    }
+    free(nodeLoc); free(nodes);
+  } else {
+    seterr("one-level setup needs an explicit partition");
+    state_free(s);
+    return NULL;
+  }
+
+  /* FinePreconditioner_Init (Preconditioner_base.F90:96-123) */
+  om_arrange(A, ARR_ROWS, 0, 0, 0);
+  s->nsub = np;
+  free(s->sstarts);
+  s->sstarts = IALLOC(np + 1);
+  long cap = 0;
+  int *onfront = IALLOC(n), *nodes = IALLOC(n);
+  /* first pass: sizes */
+  int *sizes = IALLOC(np);
+  for (int ip = 1; ip <= np; ip++) {
+    int st = partstarts[ip], nn = partstarts[ip + 1] - st;
+    for (int k = 1; k <= nn; k++) nodes[k] = partnodes[st + k - 1];
+    sizes[ip] = add_layers(A->bound, A->indj, n, nodes, nn, ol, onfront);
+    cap += sizes[ip];
+  }
+  s->sinds = IALLOC(cap);
+  s->sstarts[1] = 1;
+  for (int ip = 1; ip <= np; ip++) {
+    int st = partstarts[ip], nn = partstarts[ip + 1] - st;
+    for (int k = 1; k <= nn; k++) nodes[k] = partnodes[st + k - 1];
+    int on = add_layers(A->bound, A->indj, n, nodes, nn, ol, onfront);
+    s->sstarts[ip + 1] = s->sstarts[ip] + on;
+    memcpy(s->sinds + s->sstarts[ip], nodes + 1, sizeof(int) * on);
+  }
+  free(sizes); free(onfront); free(nodes); free(partstarts); free(partnodes);
+
+  /* factorisations (Factorise_subdomains, subsolvers.F90:78-102) */
+  if (p->fine_method == 1) {
+    s->fact = (Env **)xcalloc((size_t)np, sizeof(Env *));
+    int *floc = IALLOC(n);
+    for (int ip = 1; ip <= np; ip++) {
+      int st = s->sstarts[ip], nn = s->sstarts[ip + 1] - st;
+      s->fact[ip - 1] = factor_subdomain(A, s->sinds + st - 1, nn, floc);
+      if (!s->fact[ip - 1]) {
+        free(floc);
+        state_free(s);
+        return NULL;
+      }
+    }
+    free(floc);
+  }
+  if (s->has_coarse) {
+    OM *AC = s->AC;
+    int nc = AC->nrows;
+    int *ci = (int *)xcalloc((size_t)AC->nnz + 1, sizeof(int));
+    int *cj = (int *)xcalloc((size_t)AC->nnz + 1, sizeof(int));
+    for (long t = 1; t <= AC->nnz; t++) { ci[t - 1] = AC->indi[t] - 1; cj[t - 1] = AC->indj[t] - 1; }
+    s->cfact = env_factor(nc, AC->nnz, ci, cj, AC->val + 1);
+    free(ci); free(cj);
+    if (!s->cfact) { state_free(s); return NULL; }
+  }
+  s->w1 = DALLOC(n); s->w2 = DALLOC(n); s->w3 = DALLOC(n); s->w4 = DALLOC(n);
+  return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* getters                                                              */
+/* ------------------------------------------------------------------ */
+int or_n(const or_state *s) { return s->n; }
+int or_nagr_fine(const or_state *s) { return s->fin.nagr; }
+int or_nagr_coarse(const or_state *s) { return s->cfull.nagr; }
+int or_nsub(const or_state *s) { return s->nsub; }
+long or_sub_total(const or_state *s) { return s->sstarts[s->nsub + 1] - 1; }
+long or_R_nnz(const or_state *s) { return s->R ? s->R->nnz : 0; }
+long or_AC_nnz(const or_state *s) { return s->AC ? s->AC->nnz : 0; }
+int or_overlap(const or_state *s) { return s->ol; }
+int or_smoothers(const or_state *s) { return s->smoothers; }
+long or_A_nnz(const or_state *s) { return s->A->nnz; }
+
+static void get_aggr(const Aggrs *a, int *num, int *starts, int *nodes) {
+  if (num) memcpy(num, a->num + 1, sizeof(int) * a->n);
+  if (starts) memcpy(starts, a->starts + 1, sizeof(int) * (a->nagr + 1));
+  if (nodes) memcpy(nodes, a->nodes + 1, sizeof(int) * (a->starts[a->nagr + 1] - 1));
+}
+void or_get_fine_aggr(const or_state *s, int *num, int *starts, int *nodes) { get_aggr(&s->fin, num, starts, nodes); }
+void or_get_coarse_aggr(const or_state *s, int *num, int *starts, int *nodes) { get_aggr(&s->cfull, num, starts, nodes); }
+void or_get_subdomains(const or_state *s, int *starts, int *inds) {
+  memcpy(starts, s->sstarts + 1, sizeof(int) * (s->nsub + 1));
+  memcpy(inds, s->sinds + 1, sizeof(int) * (s->sstarts[s->nsub + 1] - 1));
+}
+static void get_om(const OM *m, int *i, int *j, double *v) {
+  memcpy(i, m->indi + 1, sizeof(int) * m->nnz);
+  memcpy(j, m->indj + 1, sizeof(int) * m->nnz);
+  memcpy(v, m->val + 1, sizeof(double) * m->nnz);
+}
+void or_get_R(const or_state *s, int *i, int *j, double *v) { get_om(s->R, i, j, v); }
+void or_get_AC(const or_state *s, int *i, int *j, double *v) { get_om(s->AC, i, j, v); }
+void or_get_A(const or_state *s, int *i, int *j, double *v) { get_om(s->A, i, j, v); }
+
+/* ------------------------------------------------------------------ */
+/* Operators                                                            */
+/* ------------------------------------------------------------------ */
+/* SpMtx_Ax, arrangement ROWS/COLS/NO (SpMtx_op_Ax.f90:44-101); y zeroed */
+static void om_Ax(const OM *m, const double *x1, double *y1, int transp, int ny) {
+  for (int i = 1; i <= ny; i++) y1[i] = 0.0;
+  int arr = m->arr;
+  const int *ii = transp ? m->indj : m->indi;
+  const int *jj = transp ? m->indi : m->indj;
+  if (transp) arr = arr == ARR_ROWS ? ARR_COLS : (arr == ARR_COLS ? ARR_ROWS : arr);
+  int nb = m->nbound - 1;
+  if (arr == ARR_NO) {
+    for (long t = 1; t <= m->nnz; t++) y1[ii[t]] = y1[ii[t]] + m->val[t] * x1[jj[t]];
+  } else if (arr == ARR_ROWS) {
+    for (int i = 1; i <= nb; i++)
+      for (int t = m->bound[i]; t <= m->bound[i + 1] - 1; t++) y1[i] = y1[i] + m->val[t] * x1[jj[t]];
+  } else {
+    for (int j = 1; j <= nb; j++)
+      for (int t = m->bound[j]; t <= m->bound[j + 1] - 1; t++) y1[ii[t]] = y1[ii[t]] + m->val[t] * x1[j];
+  }
+}
+
+void or_spmv(const or_state *s, const double *x, double *y) {
+  /* 0-based API over 1-based storage */
+  om_Ax(s->A, x - 1, y - 1, 0, s->n);
+}
+
+void or_spmv_csr(int n, const int *rowptr, const int *col, const double *val,
+                 const double *x, double *y) {
+  for (int i = 0; i < n; i++) {
+    double acc = 0.0;
+    for (int t = rowptr[i]; t < rowptr[i + 1]; t++) acc = acc + val[t] * x[col[t]];
+    y[i] = acc;
+  }
+}
+
+/* preconditioner (pcg.F90:216-270) with prec2Level (pcg.F90:54-211) and
+ * Solve_subdomains (subsolvers.F90:54-76), numprocs == 1 */
+void or_prec_apply(or_state *s, const double *r, double *z) {
+  int n = s->n;
+  const or_params *p = &s->p;
+  if (p->method == 0) {
+    memcpy(z, r, sizeof(double) * n);
+    return;
+  }
+  for (int i = 0; i < n; i++) z[i] = 0.0;
+  /* fine level */
+  if (p->fine_method == 0) {
+    memcpy(z, r, sizeof(double) * n);
+  } else {
+    int maxs = 0;
+    for (int ip = 1; ip <= s->nsub; ip++) {
+      int m = s->sstarts[ip + 1] - s->sstarts[ip];
+      if (m > maxs) maxs = m;
+    }
+    double *sr = DALLOC(maxs), *sx = DALLOC(maxs), *w = DALLOC(maxs);
+    for (int ip = 1; ip <= s->nsub; ip++) {
+      int st = s->sstarts[ip], m = s->sstarts[ip + 1] - st;
+      for (int k = 0; k < m; k++) sr[k] = r[s->sinds[st + k] - 1];
+      env_solve(s->fact[ip - 1], sr, sx, w);
+      for (int k = 0; k < m; k++) z[s->sinds[st + k] - 1] = z[s->sinds[st + k] - 1] + sx[k];
+    }
+    free(sr); free(sx); free(w);
+  }
+  if (p->levels > 1 && s->has_coarse) {
+    int nc = s->AC->nrows;
+    double *crhs = DALLOC(nc), *csol = DALLOC(nc), *w = DALLOC(nc);
+    om_Ax(s->R, r - 1, crhs, 0, nc);
+    env_solve(s->cfact, crhs + 1, csol + 1, w);
+    double *tmp = s->w1;
+    om_Ax(s->R, csol, tmp, 1, n);
+    for (int i = 0; i < n; i++) z[i] = z[i] + tmp[i + 1];
+    free(crhs); free(csol); free(w);
+  }
+}
+
+static double dot(const double *a, const double *b, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s = s + a[i] * b[i];
+  return s;
+}
+
+static double pythag(double a, double b) {
+  double p = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+  if (p == 0.0) return p;
+  double mn = fabs(a) < fabs(b) ? fabs(a) : fabs(b);
+  double r = (mn / p) * (mn / p);
+  for (;;) {
+    double t = 4.0 + r;
+    if (t == 4.0) break;
+    double s2 = r / t;
+    double u = 1.0 + 2.0 * s2;
+    p = u * p;
+    r = (s2 / u) * (s2 / u) * r;
+  }
+  return p;
+}
+
+/* tql1 (pcg.F90:499-623; EISPACK, Bowdler-Martin-Reinsch-Wilkinson 1968).
+ * d[1..n], e[1..n] with e[1] arbitrary (1-based). */
+static int tql1_1(int n, double *d, double *e) {
+  int ierr = 0;
+  if (n == 1) return 0;
+  for (int i = 2; i <= n; i++) e[i - 1] = e[i];
+  double f = 0.0, tst1 = 0.0;
+  e[n] = 0.0;
+  for (int l = 1; l <= n; l++) {
+    int j = 0, m;
+    double h = fabs(d[l]) + fabs(e[l]);
+    if (tst1 < h) tst1 = h;
+    for (m = l; m <= n; m++) {
+      double tst2 = tst1 + fabs(e[m]);
+      if (tst2 == tst1) break;
+    }
+    if (m != l) {
+      for (;;) {
+        if (j == 30) return l;
+        j++;
+        int l1 = l + 1, l2 = l1 + 1;
+        double g = d[l];
+        double p = (d[l1] - g) / (2.0 * e[l]);
+        double r = pythag(p, 1.0);
+        double sr = p >= 0 ? fabs(r) : -fabs(r);
+        d[l] = e[l] / (p + sr);
+        d[l1] = e[l] * (p + sr);
+        double dl1 = d[l1];
+        h = g - d[l];
+        if (l2 <= n)
+          for (int i = l2; i <= n; i++) d[i] = d[i] - h;
+        f = f + h;
+        p = d[m];
+        double c = 1.0, c2 = c, c3 = c;
+        double el1 = e[l1];
+        double s = 0.0, s2 = 0.0;
+        int mml = m - l;
+        for (int ii = 1; ii <= mml; ii++) {
+          c3 = c2;
+          c2 = c;
+          s2 = s;
+          int i = m - ii;
+          g = c * e[i];
+          h = c * p;
+          r = pythag(p, e[i]);
+          e[i + 1] = s * r;
+          s = e[i] / r;
+          c = p / r;
+          p = c * d[i] - s * g;
+          d[i + 1] = h + s * (c * g + s * d[i]);
+        }
+        p = -s * s2 * c3 * el1 * e[l] / dl1;
+        e[l] = s * p;
+        d[l] = c * p;
+        double tst2 = tst1 + fabs(e[l]);
+        if (!(tst2 > tst1)) break;
+      }
+    }
+    double p = d[l] + f;
+    int i;
+    if (l == 1) {
+      i = 1;
+    } else {
+      int placed = 0;
+      for (int ii = 2; ii <= l; ii++) {
+        i = l + 2 - ii;
+        if (p >= d[i - 1]) { placed = 1; break; }
+        d[i] = d[i - 1];
+      }
+      if (!placed) i = 1;
+    }
+    d[i] = p;
+  }
+  return ierr;
+}
+
+int or_tql1(int n, double *d, double *e) {
+  double *d1 = DALLOC(n), *e1 = DALLOC(n);
+  for (int i = 1; i <= n; i++) { d1[i] = d[i - 1]; e1[i] = e[i - 1]; }
+  int ierr = tql1_1(n, d1, e1);
+  for (int i = 1; i <= n; i++) d[i - 1] = d1[i];
+  free(d1); free(e1);
+  return ierr;
+}
+
+/* CalculateEigenvalues (pcg.F90:465-497) */
+int or_lanczos_eigs(int it, const double *alpha, const double *beta, double *eig) {
+  double *dd = DALLOC(it), *ee = DALLOC(it);
+  dd[1] = 1.0 / alpha[0];
+  ee[1] = 0.0;
+  for (int i = 2; i <= it; i++) {
+    if (alpha[i - 1] == 0 || beta[i - 1] < 0) {
+      free(dd); free(ee);
+      return -1;
+    }
+    dd[i] = 1.0 / alpha[i - 1] + beta[i - 1] / alpha[i - 2];
+    ee[i] = -sqrt(beta[i - 1]) / alpha[i - 2];
+  }
+  int ierr = tql1_1(it, dd, ee);
+  for (int i = 1; i <= it; i++) eig[i - 1] = dd[i];
+  free(dd); free(ee);
+  return ierr;
+}
+
+/* pcg_weigs (pcg.F90:289-462) */
+int or_pcg(or_state *s, const double *b, double *x, double tol, int maxit,
+           double *cond, double *res_hist) {
+  int n = s->n;
+  if (maxit == -1) maxit = 100000;
+  double *r = DALLOC(n), *pv = DALLOC(n), *q = DALLOC(n), *z = DALLOC(n);
+  double *alpha = DALLOC(maxit), *beta = DALLOC(maxit);
+  or_spmv(s, x, r);
+  for (int i = 0; i < n; i++) r[i] = b[i] - r[i];
+  double init_norm = dot(r, r, n);
+  if (init_norm == 0.0) init_norm = 1.0;
+  int it = 0;
+  double ratio = 1.0, rho_prev = 0.0;
+  while (ratio > tol * tol && it < maxit) {
+    it++;
+    or_prec_apply(s, r, z);
+    double rho = dot(r, z, n);
+    if (it == 1) {
+      memcpy(pv, z, sizeof(double) * n);
+    } else {
+      beta[it - 1] = rho / rho_prev;
+      for (int i = 0; i < n; i++) pv[i] = z[i] + beta[it - 1] * pv[i];
+    }
+    or_spmv(s, pv, q);
+    alpha[it - 1] = rho / dot(pv, q, n);
+    for (int i = 0; i < n; i++) x[i] = x[i] + alpha[it - 1] * pv[i];
+    for (int i = 0; i < n; i++) r[i] = r[i] - alpha[it - 1] * q[i];
+    rho_prev = rho;
+    double res = dot(r, r, n);
+    ratio = res / init_norm;
+    if (res_hist) res_hist[it - 1] = sqrt(res);
+  }
+  if (cond) {
+    *cond = 0.0;
+    if (it >= 1) {
+      double *eig = DALLOC(it);
+      if (or_lanczos_eigs(it, alpha, beta, eig) >= 0) *cond = eig[it - 1] / eig[0];
+      free(eig);
+    }
+  }
+  free(r); free(pv); free(q); free(z); free(alpha); free(beta);
+  return it;
+}

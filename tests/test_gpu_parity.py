@@ -1,0 +1,140 @@
+"""The HIP path (libdoug_hip.so on the GPU) against the oracle."""
+import numpy as np
+import pytest
+
+import doug_amd as D
+import oracle as O
+from conftest import LAP16_CTL
+from doug_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+CFG0_CTL = dict(radius1=2, strong1=0.08, smoothers=1, overlap=1, symmstruct=1, symmnumeric=1)
+
+
+def _pair(n, i, j, v, part=None, **ctl):
+    o = O.Oracle(n, i, j, v, O.params(**ctl), partition=part)
+    s = D.Solver(0)
+    s.set_params(**ctl)
+    s.set_matrix(n, i, j, v, 1)
+    if part is not None:
+        s.set_partition(*part)
+    s.setup()
+    return o, s
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.int64), np.asarray(b).view(np.int64))
+
+
+@pytest.mark.parametrize("which", ["cfg0", "3d"])
+def test_spmv_bit_exact(which):
+    if which == "cfg0":
+        n, i, j, v = P.laplace2d(128)
+        part = P.block_partition((128, 128), 64)
+    else:
+        n, i, j, v = P.diffusion3d(24, 1.0, 11)
+        part = P.block_partition((24, 24, 24), 12)
+    print(f"spmv {which} n={n}", flush=True)
+    o, s = _pair(n, i, j, v, part, **CFG0_CTL)
+    with s:
+        for seed in range(3):
+            x = np.random.default_rng(seed).uniform(-1, 1, n)
+            assert _bits_equal(s.spmv(x), o.spmv(x))
+
+
+def test_preconditioner_matches_oracle():
+    n, i, j, v = P.diffusion3d(20, 1.0, 3)
+    print(f"prec n={n}", flush=True)
+    o, s = _pair(n, i, j, v, P.block_partition((20, 20, 20), 10), **CFG0_CTL)
+    with s:
+        info = s.info()
+        assert info["nsub"] == 8 and info["sub_total"] > n
+        for seed in range(2):
+            r = np.random.default_rng(seed).uniform(-1, 1, n)
+            z, zo = s.prec(r), o.prec(r)
+            assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+@pytest.mark.parametrize("smoothers", [0, 1])
+def test_lap16x16_known_answer(lap16, smoothers):
+    print(f"lap16x16 smoothers={smoothers}", flush=True)
+    o, s = _pair(lap16["n"], lap16["i"], lap16["j"], lap16["v"], smoothers=smoothers, **LAP16_CTL)
+    with s:
+        x, it, cond, hist = s.pcg(lap16["b"], 1e-12, 300)
+        xo, ito, condo, _ = o.pcg(lap16["b"], 1e-12, 300)
+        assert np.linalg.norm(x - lap16["x"]) <= 1e-6  # scripts/doug/test.py:159-172
+        assert it == ito
+        assert abs(cond - condo) <= 1e-8 * condo
+
+
+def test_cfg0_four_subdomains():
+    n, i, j, v = P.laplace2d(128)
+    b = P.uniform_rhs(n, 2024)
+    print(f"cfg0 n={n}", flush=True)
+    o, s = _pair(n, i, j, v, P.block_partition((128, 128), 64), **CFG0_CTL)
+    with s:
+        x, it, cond, hist = s.pcg(b, 1e-8, 1000)
+        xo, ito, condo, histo = o.pcg(b, 1e-8, 1000)
+        assert it == ito
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        np.testing.assert_allclose(hist, histo, rtol=1e-6)
+        r = b - o.spmv(x)
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * (1 + 1e-6)
+
+
+def test_coarse_aggregate_subdomains_3d():
+    n, i, j, v = P.diffusion3d(16, 1.0, 7)
+    b = P.uniform_rhs(n, 5)
+    print(f"3d coarse-aggregate subdomains n={n}", flush=True)
+    o, s = _pair(n, i, j, v, None, radius2=2, **CFG0_CTL)
+    with s:
+        assert s.info()["nsub"] == o.subdomains()[0].size - 1
+        x, it, cond, hist = s.pcg(b, 1e-8, 500)
+        xo, ito, _, _ = o.pcg(b, 1e-8, 500)
+        assert it == ito
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_device_pointer_entry_points():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no torch GPU")
+    n, i, j, v = P.laplace2d(64)
+    print(f"device pointers n={n}", flush=True)
+    o, s = _pair(n, i, j, v, P.block_partition((64, 64), 32), **CFG0_CTL)
+    with s:
+        b = torch.tensor(P.uniform_rhs(n, 9), dtype=torch.float64, device="cuda:0")
+        x = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        it, cond, hist = s.pcg_device(b.data_ptr(), x.data_ptr(), 1e-8, 500)
+        xo, ito, _, _ = o.pcg(b.cpu().numpy(), 1e-8, 500)
+        assert it == ito
+        assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-8 * np.linalg.norm(xo)
+        y = torch.empty_like(b)
+        s.spmv_device(b.data_ptr(), y.data_ptr())
+        assert _bits_equal(y.cpu().numpy(), o.spmv(b.cpu().numpy()))
+
+
+def test_not_positive_definite_is_reported():
+    n, i, j, v = P.laplace2d(16)
+    v = v.copy()
+    v[(i == j) & (i == 5)] = -4.0  # one negative pivot
+    print("non-SPD", flush=True)
+    with D.Solver(0) as s:
+        s.set_params(**CFG0_CTL)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_partition(*P.block_partition((16, 16), 8))
+        with pytest.raises(D.DougError, match="NUMERIC"):
+            s.setup()
+
+
+def test_time_operator_runs():
+    n, i, j, v = P.laplace2d(64)
+    with D.Solver(0) as s:
+        s.set_params(**CFG0_CTL)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_partition(*P.block_partition((64, 64), 32))
+        s.setup()
+        for which in (0, 1, 2):
+            assert s.time_operator(which, 3) > 0
