@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 
-CTL = dict(radius1=3, minasize1=40, maxasize1=125, strong1=0.08, smoothers=1, overlap=1,
+CTL = dict(radius1=4, minasize1=100, maxasize1=343, strong1=0.08, smoothers=1, overlap=1,
            symmstruct=1, symmnumeric=1)
 
 
@@ -49,7 +49,7 @@ def cpu_baseline(size, block, sigma, seed):
     import oracle as O
     from doug_amd import problems as P
 
-    m = 32
+    m = 64
     n, i, j, v = P.diffusion3d(m, sigma, seed)
     b = P.uniform_rhs(n, seed + 1)
     t0 = time.perf_counter()

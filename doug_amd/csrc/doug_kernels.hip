@@ -37,8 +37,8 @@ __device__ __forceinline__ void tile_to_lds(double* __restrict__ S, const double
 
 // Wave w of 4 owns the 32x32 quadrant (wr, wc) = (w/2, w%2) of the 64x64
 // output as 2x2 blocks of 16x16.  MFMA f64 16x16x4 operand layout: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16]; result element r of lane l is
-// D[4*(l/16)+r][l%16].
+// supplies A[l%16][l/16] and B[l/16][l%16]; result register r of lane l is
+// D[l/16 + 4r][l%16] (single-row groups strided by 4, unlike the f32 form).
 // acc += A * B^T  (both operands row-major tiles in LDS)
 __device__ __forceinline__ void mma_abt(d4 (&acc)[2][2], const double* As, const double* Bs,
                                         int lane, int wr, int wc) {
@@ -84,7 +84,7 @@ __device__ __forceinline__ void zero_acc(d4 (&acc)[2][2]) {
 
 // element (row, col) held by accumulator (bi, bj, r) of this lane
 __device__ __forceinline__ int acc_row(int wr, int bi, int lane, int r) {
-  return wr * 32 + bi * 16 + 4 * (lane >> 4) + r;
+  return wr * 32 + bi * 16 + (lane >> 4) + 4 * r;
 }
 __device__ __forceinline__ int acc_col(int wc, int bj, int lane) {
   return wc * 32 + bj * 16 + (lane & 15);

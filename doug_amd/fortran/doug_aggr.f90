@@ -1,0 +1,166 @@
+!> doug_aggr: the DOUG driver (src/main/aggr.F90:67-295) for one process with
+!> all subdomains on one MI355X.  Reads DOUG.ctl and the assembled input,
+!> builds the two-level Schwarz preconditioner and runs PCG through
+!> libdoug_hip.so, then writes the solution in DOUG's formats.
+!>
+!> usage: doug_aggr [-f DOUG.ctl] [-p [prefix]] [-q [file]] [-h]
+program doug_aggr
+  use iso_c_binding
+  use doug_capi
+  use doug_ctl
+  use doug_io
+  implicit none
+
+  character(len=300) :: ctlfile, arg, profprefix
+  type(ctl_values) :: c
+  type(doug_params) :: p
+  type(doug_info) :: info
+  type(c_ptr) :: ctx
+  integer(c_int32_t), allocatable :: indi(:), indj(:), pstarts(:), pnodes(:)
+  real(c_double), allocatable :: val(:), b(:), x(:), hist(:)
+  integer :: n, nnz, k, nargs, maxit, pu
+  integer(c_int32_t) :: it
+  real(c_double) :: cond, tol, t0, t1
+  logical :: ok, prof
+  integer(c_int) :: rc
+
+  ctlfile = 'DOUG.ctl'
+  prof = .false.
+  profprefix = 'prof'
+  nargs = command_argument_count()
+  k = 1
+  do while (k <= nargs)
+    call get_command_argument(k, arg)
+    select case (trim(arg))
+    case ('-f')
+      k = k + 1
+      call get_command_argument(k, ctlfile)
+      if (len_trim(ctlfile) == 0) call fatal('No DOUG control file specified')
+    case ('-p')
+      prof = .true.
+      if (k < nargs) then
+        call get_command_argument(k + 1, arg)
+        if (arg(1:1) /= '-') then
+          profprefix = arg
+          k = k + 1
+        end if
+      end if
+    case ('-q')
+      if (k < nargs) then
+        call get_command_argument(k + 1, arg)
+        if (arg(1:1) /= '-') k = k + 1
+      end if
+    case ('-h')
+      write(*, '(a)') 'usage: doug_aggr [-f DOUG.ctl] [-p [prefix]] [-q [file]] [-h]'
+      stop
+    case default
+      call fatal('Unrecognised command line argument: '//trim(arg))
+    end select
+    k = k + 1
+  end do
+
+  write(*, '(a)') ' Parsing DOUG control file: '//trim(ctlfile)
+  call ctl_read(ctlfile, c, ok)
+  if (.not. ok) call fatal('Unable to open DOUG control file: '//trim(ctlfile))
+  if (c%solver /= 2) call fatal('[DOUG main] : Wrong solution method specified')
+  if (c%input_type /= 2 .and. c%input_type /= -1) &
+    call fatal('only assembled input (input_type 2) is supported')
+  if (c%coarse_method > 1) call fatal('Wrong coarse method')
+
+  ! matrix and right-hand side (Distribution_assm.F90:65-76)
+  write(*, '(a,a)') ' ##### Assembled input file: ##### ', trim(c%assembled_mtx_file)
+  call read_assembled(c%assembled_mtx_file, max(c%assembled_mtx_format, 0), n, nnz, &
+                      indi, indj, val, ok)
+  if (.not. ok) call fatal('Unable to read assembled file: '//trim(c%assembled_mtx_file))
+  write(*, *) 'n,nnz:', n, nnz
+  allocate(b(n), x(n))
+  if (len_trim(c%assembled_rhs_file) > 0) then
+    write(*, '(a,a)') ' ##### Assembled RHS file: ##### ', trim(c%assembled_rhs_file)
+    call read_vector(c%assembled_rhs_file, c%assembled_rhs_format, b, ok)
+    if (.not. ok) call fatal('Unable to read vector file: '//trim(c%assembled_rhs_file))
+  else
+    b = 1.0_c_double
+  end if
+
+  rc = doug_ctx_create(0_c_int, ctx)
+  if (rc /= DOUG_OK) call fatal('no GPU available for libdoug_hip')
+  call doug_params_default(p)
+  p%radius1 = c%radius1
+  p%radius2 = c%radius2
+  p%minasize1 = c%minasize1
+  p%minasize2 = c%minasize2
+  p%maxasize1 = c%maxasize1
+  p%maxasize2 = c%maxasize2
+  p%strong1 = c%strong1
+  p%strong2 = c%strong2
+  p%smoothers = c%smoothers
+  p%overlap = c%overlap
+  if (c%levels >= 1) p%levels = c%levels
+  if (c%method >= 0) p%method = c%method
+  if (c%fine_method >= 0) p%fine_method = c%fine_method
+  p%symmstruct = merge(1, 0, c%symmstruct)
+  p%symmnumeric = merge(1, 0, c%symmnumeric)
+  call check(doug_set_params(ctx, p))
+  call check(doug_set_matrix_coo(ctx, int(n, c_int32_t), int(nnz, c_int64_t), indi, indj, val, &
+                                 1_c_int32_t))
+  if (len_trim(c%partition_file) > 0) then
+    call read_partition(c%partition_file, n, pstarts, pnodes, ok)
+    if (.not. ok) call fatal('Unable to read partition file: '//trim(c%partition_file))
+    call check(doug_set_partition(ctx, int(size(pstarts) - 1, c_int32_t), pstarts, pnodes, &
+                                  1_c_int32_t))
+  end if
+
+  call cpu_time(t0)
+  call check(doug_setup(ctx, DOUG_PREC_FP64))
+  call check(doug_get_info(ctx, info))
+  write(*, *) '# fine aggregates:', info%nagr_fine
+  write(*, *) '# coarse aggregates:', info%nagr_coarse
+  write(*, *) '# subdomains:', info%nsub, ' overlap:', info%overlap
+
+  ! pcg_weigs (pcg.F90:289-462)
+  tol = c%solve_tolerance
+  maxit = c%solve_maxiters
+  if (maxit == -1) maxit = 100000
+  allocate(hist(maxit))
+  x = 0.0_c_double
+  write(*, '(/a)') 'Preconditioned conjugate gradient:'
+  write(*, '(a,i6,a,e8.2)') 'maxit = ', maxit, ', tol = ', tol
+  call check(doug_pcg(ctx, b, x, tol, int(maxit, c_int32_t), it, cond, hist))
+  call cpu_time(t1)
+  do k = 1, it
+    write(*, '(i5,a,e22.15)') k, ': res_norm=', hist(k)
+  end do
+  write(*, '(a,i3,a,e10.4)') '#it:', it, ' Cond#: ', cond
+  write(*, *) 'time spent in pcg():', t1 - t0
+  write(*, *) '    ...of which setup:', info%t_total
+  write(*, *) '       ...of which factorisation:', info%t_factor + info%t_coarse_factor
+
+  if (prof) then
+    open(newunit=pu, file=trim(profprefix)//'.00', status='replace', form='formatted')
+    write(pu, "(I0,':fine aggregates:',I0)") 0, info%nagr_fine
+    write(pu, "(I0,':coarse aggregates:',I0)") 0, info%nagr_coarse
+    write(pu, "(I0,':pcg iterations:',I0)") 0, it
+    write(pu, "(I0,':setup time:',F0.3)") 0, info%t_total
+    close(pu)
+  end if
+
+  if (len_trim(c%solution_file) > 0 .and. c%solution_format >= 0) then
+    call write_solution(c%solution_file, c%solution_format, x, ok)
+    if (.not. ok) write(*, *) 'Solution not written.'
+  end if
+  rc = doug_ctx_destroy(ctx)
+
+contains
+
+  subroutine fatal(msg)
+    character(*), intent(in) :: msg
+    write(*, '(a)') ' DOUG_abort: '//trim(msg)
+    error stop 1
+  end subroutine fatal
+
+  subroutine check(code)
+    integer(c_int), intent(in) :: code
+    if (code /= DOUG_OK) call fatal(trim(doug_error_string(ctx)))
+  end subroutine check
+
+end program doug_aggr
