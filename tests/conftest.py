@@ -14,6 +14,14 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs libdoug_hip.so kernels)")
+    # torch ships its own HIP runtime; let it initialise before libdoug_hip.so
+    # loads /opt/rocm's so that torch tensors can be handed to the C-ABI
+    try:
+        import torch
+
+        torch.cuda.is_available()
+    except Exception:
+        pass
 
 
 def _ensure_built():

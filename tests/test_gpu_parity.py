@@ -99,7 +99,7 @@ def test_coarse_aggregate_subdomains_3d():
 def test_device_pointer_entry_points():
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
-        pytest.skip("no torch GPU")
+        pytest.skip(f"torch sees no GPU (device_count={torch.cuda.device_count()})")
     n, i, j, v = P.laplace2d(64)
     print(f"device pointers n={n}", flush=True)
     o, s = _pair(n, i, j, v, P.block_partition((64, 64), 32), **CFG0_CTL)
