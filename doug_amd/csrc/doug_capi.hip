@@ -65,7 +65,9 @@ struct doug_ctx {
   int nsub = 0;
   BlockDesc* d_sub = nullptr;
   std::vector<BlockDesc> h_sub;
-  double* d_tiles = nullptr;
+  double* d_tiles = nullptr;   // FP64 factors
+  float* d_tiles32 = nullptr;  // FP32 factors (DOUG_PREC_FP32_FACTORS)
+  int precision = DOUG_PREC_FP64;
   int32_t *d_inds = nullptr, *d_occ_ptr = nullptr, *d_occ_pos = nullptr;
   double* d_subsol = nullptr;
   // coarse
@@ -290,12 +292,15 @@ int doug_set_partition(doug_ctx* ctx, int32_t nparts, const int32_t* starts, con
   return DOUG_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------------------
 // Dense tile storage for a set of index blocks: descriptors, scatter map and
 // factorisation on the device.
+template <typename T>
 static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& starts,
                         const std::vector<int>& inds, bool one_block_solve,
-                        std::vector<BlockDesc>& desc, BlockDesc** d_desc, double** d_tiles,
+                        std::vector<BlockDesc>& desc, BlockDesc** d_desc, T** d_tiles,
                         int64_t* total_elems) {
   const int nb = (int)starts.size() - 1;
   desc.resize((size_t)nb);
@@ -346,7 +351,7 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& sta
   if ((rc = upload(ctx, &d_src, src))) return rc;
   double* d_vals;
   if ((rc = upload(ctx, &d_vals, M.val))) return rc;
-  HIPCHK(hipMemsetAsync(*d_tiles, 0, (size_t)base * sizeof(double), ctx->stream));
+  HIPCHK(hipMemsetAsync(*d_tiles, 0, (size_t)base * sizeof(T), ctx->stream));
   int maxpad = 0;
   for (auto& d : desc) maxpad = std::max(maxpad, d.nt * kTile - d.s);
   launch_pad_identity(*d_tiles, *d_desc, nb, maxpad, ctx->stream);
@@ -355,7 +360,8 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& sta
   return DOUG_OK;
 }
 
-static int factor_blocks(doug_ctx* ctx, double* tiles, const BlockDesc* d_desc,
+template <typename T>
+static int factor_blocks(doug_ctx* ctx, T* tiles, const BlockDesc* d_desc,
                          const std::vector<BlockDesc>& desc) {
   int ntmax = 0;
   for (auto& d : desc) ntmax = std::max(ntmax, d.nt);
@@ -373,6 +379,8 @@ static int factor_blocks(doug_ctx* ctx, double* tiles, const BlockDesc* d_desc,
                 "matrix block " + std::to_string(flag - 1) + " is not positive definite");
   return DOUG_OK;
 }
+
+extern "C" {
 
 static int setup_host(doug_ctx* ctx) {
   if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "setup before doug_set_matrix_coo");
@@ -534,12 +542,21 @@ static int setup_device(doug_ctx* ctx) {
   // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
   double t4 = now();
   int64_t elems = 0;
+  size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
   if (p.fine_method == 1) {
-    if ((rc = build_blocks(ctx, A, ctx->sstarts, ctx->sinds, true, ctx->h_sub, &ctx->d_sub, &ctx->d_tiles, &elems)))
-      return rc;
-    if ((rc = factor_blocks(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub))) return rc;
+    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
+      if ((rc = build_blocks(ctx, A, ctx->sstarts, ctx->sinds, true, ctx->h_sub, &ctx->d_sub,
+                             &ctx->d_tiles32, &elems)))
+        return rc;
+      if ((rc = factor_blocks(ctx, ctx->d_tiles32, ctx->d_sub, ctx->h_sub))) return rc;
+    } else {
+      if ((rc = build_blocks(ctx, A, ctx->sstarts, ctx->sinds, true, ctx->h_sub, &ctx->d_sub,
+                             &ctx->d_tiles, &elems)))
+        return rc;
+      if ((rc = factor_blocks(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub))) return rc;
+    }
   }
-  info.factor_bytes = elems * (int64_t)sizeof(double);
+  info.factor_bytes = elems * (int64_t)esize;
   info.t_factor = now() - t4;
 
   // ---- coarse factorisation and explicit inverse factor ----
@@ -580,7 +597,9 @@ int doug_setup_host(doug_ctx* ctx) {
 int doug_setup(doug_ctx* ctx, int32_t precision) {
   if (!ctx) return DOUG_E_ARG;
   if (ctx->device < 0) return fail(ctx, DOUG_E_STATE, "host-only context: no GPU work");
-  if (precision != DOUG_PREC_FP64) return fail(ctx, DOUG_E_UNSUPPORTED, "only DOUG_PREC_FP64 is available");
+  if (precision != DOUG_PREC_FP64 && precision != DOUG_PREC_FP32_FACTORS)
+    return fail(ctx, DOUG_E_ARG, "precision must be DOUG_PREC_FP64 or DOUG_PREC_FP32_FACTORS");
+  ctx->precision = precision;
   int rc = setup_host(ctx);
   if (rc) return rc;
   return setup_device(ctx);
@@ -648,6 +667,13 @@ static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
   launch_spmv(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, x, y, ctx->stream);
 }
 
+static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
+  if (ctx->precision == DOUG_PREC_FP32_FACTORS)
+    launch_trsv(ctx->d_tiles32, ctx->d_sub, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+  else
+    launch_trsv(ctx->d_tiles, ctx->d_sub, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+}
+
 // preconditioner (pcg.F90:216-270): fine solves, then coarse correction added
 static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
   const Params& p = ctx->p;
@@ -663,7 +689,7 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
   }
   const double* fine_src = r;
   if (p.fine_method == 1) {
-    launch_trsv(ctx->d_tiles, ctx->d_sub, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    enqueue_subdomain_solves(ctx, r);
     fine_src = ctx->d_subsol;
   }
   launch_overlap_sum(ctx->n, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
@@ -883,8 +909,7 @@ int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
     switch (which) {
       case 0: enqueue_spmv(ctx, ctx->d_p, ctx->d_q); break;
       case 1:
-        if (ctx->p.fine_method == 1)
-          launch_trsv(ctx->d_tiles, ctx->d_sub, ctx->nsub, ctx->d_inds, ctx->d_r, ctx->d_subsol, st);
+        if (ctx->p.fine_method == 1) enqueue_subdomain_solves(ctx, ctx->d_r);
         break;
       case 2: enqueue_prec(ctx, ctx->d_r, ctx->d_z); break;
       default: break;

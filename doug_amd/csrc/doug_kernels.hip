@@ -15,14 +15,36 @@ namespace doug {
 
 namespace {
 
-constexpr int LD = kTile + 1;  // padded LDS row stride (doubles)
-typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int LD = kTile + 1;  // padded LDS row stride (elements)
+
+// Matrix-core tile product for each element type.  Both forms take A[l%16][l/16]
+// and B[l/16][l%16] from lane l.  Result register r of lane l holds
+//   f64 16x16x4: D[l/16 + 4r][l%16]   (single-row groups strided by 4)
+//   f32 16x16x4: D[4(l/16) + r][l%16] (groups of four consecutive rows)
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
 
 __device__ __forceinline__ int64_t toff(int i, int k) {
   return ((int64_t)i * (i + 1) / 2 + k) * kTileElems;
 }
 
-// 64x64 row-major global tile -> LDS (stride LD); 256 threads.
+// 64x64 row-major global tile -> LDS (stride LD); 256 threads, 16 B loads.
 __device__ __forceinline__ void tile_to_lds(double* __restrict__ S, const double* __restrict__ g) {
   const int t = threadIdx.x;
 #pragma unroll
@@ -34,57 +56,72 @@ __device__ __forceinline__ void tile_to_lds(double* __restrict__ S, const double
     S[r * LD + c + 1] = v.y;
   }
 }
+__device__ __forceinline__ void tile_to_lds(float* __restrict__ S, const float* __restrict__ g) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    int e = (t + 256 * q) * 4;
+    int r = e >> 6, c = e & 63;
+    float4 v = *reinterpret_cast<const float4*>(g + e);
+    S[r * LD + c] = v.x;
+    S[r * LD + c + 1] = v.y;
+    S[r * LD + c + 2] = v.z;
+    S[r * LD + c + 3] = v.w;
+  }
+}
 
 // Wave w of 4 owns the 32x32 quadrant (wr, wc) = (w/2, w%2) of the 64x64
-// output as 2x2 blocks of 16x16.  MFMA f64 16x16x4 operand layout: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16]; result register r of lane l is
-// D[l/16 + 4r][l%16] (single-row groups strided by 4, unlike the f32 form).
+// output as 2x2 blocks of 16x16.
 // acc += A * B^T  (both operands row-major tiles in LDS)
-__device__ __forceinline__ void mma_abt(d4 (&acc)[2][2], const double* As, const double* Bs,
+template <typename T>
+__device__ __forceinline__ void mma_abt(typename Mma<T>::v4 (&acc)[2][2], const T* As, const T* Bs,
                                         int lane, int wr, int wc) {
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll 4
   for (int kk = 0; kk < 16; kk++) {
     int k = 4 * kk + lk;
-    double a0 = As[(wr * 32 + li) * LD + k];
-    double a1 = As[(wr * 32 + 16 + li) * LD + k];
-    double b0 = Bs[(wc * 32 + li) * LD + k];
-    double b1 = Bs[(wc * 32 + 16 + li) * LD + k];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    T a0 = As[(wr * 32 + li) * LD + k];
+    T a1 = As[(wr * 32 + 16 + li) * LD + k];
+    T b0 = Bs[(wc * 32 + li) * LD + k];
+    T b1 = Bs[(wc * 32 + 16 + li) * LD + k];
+    acc[0][0] = Mma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mma<T>::op(a1, b1, acc[1][1]);
   }
 }
 
 // acc += A * B  (B row-major [k][c])
-__device__ __forceinline__ void mma_ab(d4 (&acc)[2][2], const double* As, const double* Bs,
+template <typename T>
+__device__ __forceinline__ void mma_ab(typename Mma<T>::v4 (&acc)[2][2], const T* As, const T* Bs,
                                        int lane, int wr, int wc) {
   const int li = lane & 15, lk = lane >> 4;
 #pragma unroll 4
   for (int kk = 0; kk < 16; kk++) {
     int k = 4 * kk + lk;
-    double a0 = As[(wr * 32 + li) * LD + k];
-    double a1 = As[(wr * 32 + 16 + li) * LD + k];
-    double b0 = Bs[k * LD + wc * 32 + li];
-    double b1 = Bs[k * LD + wc * 32 + 16 + li];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    T a0 = As[(wr * 32 + li) * LD + k];
+    T a1 = As[(wr * 32 + 16 + li) * LD + k];
+    T b0 = Bs[k * LD + wc * 32 + li];
+    T b1 = Bs[k * LD + wc * 32 + 16 + li];
+    acc[0][0] = Mma<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mma<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mma<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mma<T>::op(a1, b1, acc[1][1]);
   }
 }
 
-__device__ __forceinline__ void zero_acc(d4 (&acc)[2][2]) {
+template <typename V>
+__device__ __forceinline__ void zero_acc(V (&acc)[2][2]) {
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
 }
 
 // element (row, col) held by accumulator (bi, bj, r) of this lane
+template <typename T>
 __device__ __forceinline__ int acc_row(int wr, int bi, int lane, int r) {
-  return wr * 32 + bi * 16 + (lane >> 4) + 4 * r;
+  return wr * 32 + bi * 16 + Mma<T>::row(lane, r);
 }
 __device__ __forceinline__ int acc_col(int wc, int bj, int lane) {
   return wc * 32 + bj * 16 + (lane & 15);
@@ -93,38 +130,42 @@ __device__ __forceinline__ int acc_col(int wc, int bj, int lane) {
 // ------------------------------------------------------------------------
 // Setup kernels
 // ------------------------------------------------------------------------
-__global__ void k_scatter(double* __restrict__ tiles, const int64_t* __restrict__ dst,
+template <typename T>
+__global__ void k_scatter(T* __restrict__ tiles, const int64_t* __restrict__ dst,
                           const int32_t* __restrict__ src, const double* __restrict__ val,
                           int64_t ne) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < ne) tiles[dst[e]] = val[src[e]];
+  if (e < ne) tiles[dst[e]] = (T)val[src[e]];
 }
 
-__global__ void k_pad_identity(double* __restrict__ tiles, const BlockDesc* __restrict__ d,
+template <typename T>
+__global__ void k_pad_identity(T* __restrict__ tiles, const BlockDesc* __restrict__ d,
                                int maxpad) {
   const BlockDesc b = d[blockIdx.y];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   int npad = b.nt * kTile - b.s;
   if (p >= npad || p >= maxpad) return;
   int lr = kTile - npad + p;  // local row inside the last diagonal tile
-  tiles[b.base + toff(b.nt - 1, b.nt - 1) + lr * kTile + lr] = 1.0;
+  tiles[b.base + toff(b.nt - 1, b.nt - 1) + lr * kTile + lr] = (T)1;
 }
 
-__global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ tiles,
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
                                                    const BlockDesc* __restrict__ d, int j,
                                                    int* __restrict__ flag) {
-  __shared__ double As[kTile * LD];
-  __shared__ double Bs[kTile * LD];
+  __shared__ T As[kTile * LD];
+  __shared__ T Bs[kTile * LD];
+  typedef typename Mma<T>::v4 V;
   const BlockDesc b = d[blockIdx.x];
   if (j >= b.nt) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  double* Cg = tiles + b.base + toff(j, j);
-  d4 acc[2][2];
+  T* Cg = tiles + b.base + toff(j, j);
+  V acc[2][2];
   zero_acc(acc);
   for (int k = 0; k < j; k++) {
     tile_to_lds(As, tiles + b.base + toff(j, k));
     __syncthreads();
-    mma_abt(acc, As, As, lane, wr, wc);
+    mma_abt<T>(acc, As, As, lane, wr, wc);
     __syncthreads();
   }
 #pragma unroll
@@ -133,22 +174,22 @@ __global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ tiles,
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = acc_row(wr, bi, lane, r), col = acc_col(wc, bj, lane);
+        int row = acc_row<T>(wr, bi, lane, r), col = acc_col(wc, bj, lane);
         As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
       }
   __syncthreads();
   // unblocked Cholesky of the 64x64 tile (lower triangle)
   for (int c = 0; c < kTile; c++) {
     if (t == 0) {
-      double dv = As[c * LD + c];
-      if (!(dv > 0.0)) {
+      T dv = As[c * LD + c];
+      if (!(dv > (T)0)) {
         atomicCAS(flag, 0, (int)blockIdx.x + 1);
-        dv = 1.0;
+        dv = (T)1;
       }
       As[c * LD + c] = sqrt(dv);
     }
     __syncthreads();
-    const double piv = As[c * LD + c];
+    const T piv = As[c * LD + c];
     for (int r = c + 1 + t; r < kTile; r += 256) As[r * LD + c] = As[r * LD + c] / piv;
     __syncthreads();
     const int m = kTile - 1 - c;
@@ -162,12 +203,12 @@ __global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ tiles,
     __syncthreads();
   }
   // inverse of the lower factor, one column per thread of the first wave
-  for (int idx = t; idx < kTileElems; idx += 256) Bs[(idx >> 6) * LD + (idx & 63)] = 0.0;
+  for (int idx = t; idx < kTileElems; idx += 256) Bs[(idx >> 6) * LD + (idx & 63)] = (T)0;
   __syncthreads();
   if (t < kTile) {
     const int col = t;
     for (int r = col; r < kTile; r++) {
-      double s = (r == col) ? 1.0 : 0.0;
+      T s = (r == col) ? (T)1 : (T)0;
       for (int k = col; k < r; k++) s -= As[r * LD + k] * Bs[k * LD + col];
       Bs[r * LD + col] = s / As[r * LD + r];
     }
@@ -176,22 +217,24 @@ __global__ __launch_bounds__(256) void k_chol_diag(double* __restrict__ tiles,
   for (int idx = t; idx < kTileElems; idx += 256) Cg[idx] = Bs[(idx >> 6) * LD + (idx & 63)];
 }
 
-__global__ __launch_bounds__(256) void k_chol_offdiag(double* __restrict__ tiles,
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_offdiag(T* __restrict__ tiles,
                                                       const BlockDesc* __restrict__ d, int j) {
-  __shared__ double As[kTile * LD];
-  __shared__ double Bs[kTile * LD];
+  __shared__ T As[kTile * LD];
+  __shared__ T Bs[kTile * LD];
+  typedef typename Mma<T>::v4 V;
   const BlockDesc b = d[blockIdx.y];
   const int i = j + 1 + blockIdx.x;
   if (i >= b.nt) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  double* Cg = tiles + b.base + toff(i, j);
-  d4 acc[2][2];
+  T* Cg = tiles + b.base + toff(i, j);
+  V acc[2][2];
   zero_acc(acc);
   for (int k = 0; k < j; k++) {
     tile_to_lds(As, tiles + b.base + toff(i, k));
     tile_to_lds(Bs, tiles + b.base + toff(j, k));
     __syncthreads();
-    mma_abt(acc, As, Bs, lane, wr, wc);
+    mma_abt<T>(acc, As, Bs, lane, wr, wc);
     __syncthreads();
   }
 #pragma unroll
@@ -200,20 +243,20 @@ __global__ __launch_bounds__(256) void k_chol_offdiag(double* __restrict__ tiles
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = acc_row(wr, bi, lane, r), col = acc_col(wc, bj, lane);
+        int row = acc_row<T>(wr, bi, lane, r), col = acc_col(wc, bj, lane);
         As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
       }
   tile_to_lds(Bs, tiles + b.base + toff(j, j));  // inv(L_jj)
   __syncthreads();
   zero_acc(acc);
-  mma_abt(acc, As, Bs, lane, wr, wc);  // L_ij = C inv(L_jj)^T
+  mma_abt<T>(acc, As, Bs, lane, wr, wc);  // L_ij = C inv(L_jj)^T
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        Cg[acc_row(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = acc[bi][bj][r];
+        Cg[acc_row<T>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = acc[bi][bj][r];
 }
 
 // W_{i,j} = -inv(L_ii) * sum_{k=j}^{i-1} L_ik W_kj, i = j + dd
@@ -226,13 +269,13 @@ __global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
   const int j = blockIdx.x, i = j + dd;
   if (i >= b.nt) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  d4 acc[2][2];
+  Mma<double>::v4 acc[2][2];
   zero_acc(acc);
   for (int k = j; k < i; k++) {
     tile_to_lds(As, L + b.base + toff(i, k));
     tile_to_lds(Bs, W + b.base + toff(k, j));
     __syncthreads();
-    mma_ab(acc, As, Bs, lane, wr, wc);
+    mma_ab<double>(acc, As, Bs, lane, wr, wc);
     __syncthreads();
   }
 #pragma unroll
@@ -241,11 +284,11 @@ __global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        Bs[acc_row(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
+        Bs[acc_row<double>(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
   tile_to_lds(As, L + b.base + toff(i, i));  // inv(L_ii)
   __syncthreads();
   zero_acc(acc);
-  mma_ab(acc, As, Bs, lane, wr, wc);
+  mma_ab<double>(acc, As, Bs, lane, wr, wc);
   double* Wg = W + b.base + toff(i, j);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
@@ -253,7 +296,7 @@ __global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        Wg[acc_row(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = -acc[bi][bj][r];
+        Wg[acc_row<double>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = -acc[bi][bj][r];
 }
 
 // ------------------------------------------------------------------------
@@ -262,28 +305,29 @@ __global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
 // 16 row partial sums spread over the 64 lanes (lane = column) -> lane l
 // returns the full sum of row (l >> 2); four reduce-scatter halvings then
 // two butterflies.
-__device__ __forceinline__ double reduce_scatter16(double (&v)[16], int lane) {
+template <typename T>
+__device__ __forceinline__ T reduce_scatter16(T (&v)[16], int lane) {
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     bool hi = lane & 32;
-    double keep = hi ? v[q + 8] : v[q], send = hi ? v[q] : v[q + 8];
+    T keep = hi ? v[q + 8] : v[q], send = hi ? v[q] : v[q + 8];
     v[q] = keep + __shfl_xor(send, 32);
   }
 #pragma unroll
   for (int q = 0; q < 4; q++) {
     bool hi = lane & 16;
-    double keep = hi ? v[q + 4] : v[q], send = hi ? v[q] : v[q + 4];
+    T keep = hi ? v[q + 4] : v[q], send = hi ? v[q] : v[q + 4];
     v[q] = keep + __shfl_xor(send, 16);
   }
 #pragma unroll
   for (int q = 0; q < 2; q++) {
     bool hi = lane & 8;
-    double keep = hi ? v[q + 2] : v[q], send = hi ? v[q] : v[q + 2];
+    T keep = hi ? v[q + 2] : v[q], send = hi ? v[q] : v[q + 2];
     v[q] = keep + __shfl_xor(send, 8);
   }
   {
     bool hi = lane & 4;
-    double keep = hi ? v[1] : v[0], send = hi ? v[0] : v[1];
+    T keep = hi ? v[1] : v[0], send = hi ? v[0] : v[1];
     v[0] = keep + __shfl_xor(send, 4);
   }
   v[0] += __shfl_xor(v[0], 2);
@@ -294,40 +338,43 @@ __device__ __forceinline__ double reduce_scatter16(double (&v)[16], int lane) {
 // ------------------------------------------------------------------------
 // Per-iteration kernels
 // ------------------------------------------------------------------------
-// Forward then backward substitution for one subdomain per block.
-__global__ __launch_bounds__(256) void k_trsv(const double* __restrict__ tiles,
+// Forward then backward substitution for one subdomain per block, in the
+// factor's precision T; the right-hand side is gathered from (and the
+// solution written to) FP64 vectors.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
                                               const BlockDesc* __restrict__ d,
                                               const int32_t* __restrict__ inds,
                                               const double* __restrict__ rhs,
                                               double* __restrict__ sol) {
-  __shared__ double y[kMaxTrsvTiles * kTile];
-  __shared__ double tv[kTile];
-  __shared__ double red[4 * kTile];
+  __shared__ T y[kMaxTrsvTiles * kTile];
+  __shared__ T tv[kTile];
+  __shared__ T red[4 * kTile];
   const BlockDesc b = d[blockIdx.x];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nt = b.nt, npad = nt * kTile;
-  const double* T0 = tiles + b.base;
-  for (int p = t; p < npad; p += 256) y[p] = p < b.s ? rhs[inds[b.vec_off + p]] : 0.0;
+  const T* T0 = tiles + b.base;
+  for (int p = t; p < npad; p += 256) y[p] = p < b.s ? (T)rhs[inds[b.vec_off + p]] : (T)0;
   __syncthreads();
   // forward: y_i = inv(L_ii) (r_i - sum_{k<i} L_ik y_k)
   for (int i = 0; i < nt; i++) {
-    double acc[16];
+    T acc[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] = 0.0;
+    for (int q = 0; q < 16; q++) acc[q] = (T)0;
     for (int k = 0; k < i; k++) {
-      const double* Tk = T0 + toff(i, k) + (16 * w) * kTile + lane;
-      const double yk = y[k * kTile + lane];
+      const T* Tk = T0 + toff(i, k) + (16 * w) * kTile + lane;
+      const T yk = y[k * kTile + lane];
 #pragma unroll
       for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * yk;
     }
-    double s = reduce_scatter16(acc, lane);
+    T s = reduce_scatter16(acc, lane);
     if ((lane & 3) == 0) {
       int row = 16 * w + (lane >> 2);
       tv[row] = y[i * kTile + row] - s;
     }
     __syncthreads();
-    const double* Ti = T0 + toff(i, i) + (16 * w) * kTile + lane;
-    const double tl = tv[lane];
+    const T* Ti = T0 + toff(i, i) + (16 * w) * kTile + lane;
+    const T tl = tv[lane];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = Ti[q * kTile] * tl;
     s = reduce_scatter16(acc, lane);
@@ -336,10 +383,10 @@ __global__ __launch_bounds__(256) void k_trsv(const double* __restrict__ tiles,
   }
   // backward: x_j = inv(L_jj)^T (y_j - sum_{i>j} L_ij^T x_i)
   for (int j = nt - 1; j >= 0; j--) {
-    double acc = 0.0;
+    T acc = (T)0;
     for (int i = j + 1; i < nt; i++) {
-      const double* Ti = T0 + toff(i, j) + (16 * w) * kTile + lane;
-      const double* xi = y + i * kTile + 16 * w;
+      const T* Ti = T0 + toff(i, j) + (16 * w) * kTile + lane;
+      const T* xi = y + i * kTile + 16 * w;
 #pragma unroll
       for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * xi[q];
     }
@@ -349,8 +396,8 @@ __global__ __launch_bounds__(256) void k_trsv(const double* __restrict__ tiles,
       tv[lane] = y[j * kTile + lane] -
                  ((red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]));
     __syncthreads();
-    const double* Tj = T0 + toff(j, j) + (16 * w) * kTile + lane;
-    acc = 0.0;
+    const T* Tj = T0 + toff(j, j) + (16 * w) * kTile + lane;
+    acc = (T)0;
 #pragma unroll
     for (int q = 0; q < 16; q++) acc += Tj[q * kTile] * tv[16 * w + q];
     red[w * kTile + lane] = acc;
@@ -360,7 +407,7 @@ __global__ __launch_bounds__(256) void k_trsv(const double* __restrict__ tiles,
           (red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]);
     __syncthreads();
   }
-  for (int p = t; p < b.s; p += 256) sol[b.vec_off + p] = y[p];
+  for (int p = t; p < b.s; p += 256) sol[b.vec_off + p] = (double)y[p];
 }
 
 // u_i = sum_{k<=i} W_ik c_k, one block per tile row
@@ -573,29 +620,51 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 
 }  // namespace
 
-void launch_scatter_entries(double* tiles, const int64_t* dst, const int32_t* src,
-                            const double* val, int64_t ne, hipStream_t st) {
+template <typename T>
+void launch_scatter_entries(T* tiles, const int64_t* dst, const int32_t* src, const double* val,
+                            int64_t ne, hipStream_t st) {
   if (ne <= 0) return;
-  hipLaunchKernelGGL(k_scatter, dim3(cdiv(ne, 256)), dim3(256), 0, st, tiles, dst, src, val, ne);
+  hipLaunchKernelGGL(k_scatter<T>, dim3(cdiv(ne, 256)), dim3(256), 0, st, tiles, dst, src, val, ne);
 }
 
-void launch_pad_identity(double* tiles, const BlockDesc* d, int nblocks, int maxpad,
-                         hipStream_t st) {
+template <typename T>
+void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, hipStream_t st) {
   if (maxpad <= 0 || nblocks <= 0) return;
-  hipLaunchKernelGGL(k_pad_identity, dim3(cdiv(maxpad, 64), nblocks), dim3(64), 0, st, tiles, d,
-                     maxpad);
+  hipLaunchKernelGGL(k_pad_identity<T>, dim3(cdiv(maxpad, 64), nblocks), dim3(64), 0, st, tiles,
+                     d, maxpad);
 }
 
-void launch_chol_diag(double* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
+template <typename T>
+void launch_chol_diag(T* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
                       hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_diag, dim3(nblocks), dim3(256), 0, st, tiles, d, j, flag);
+  hipLaunchKernelGGL(k_chol_diag<T>, dim3(nblocks), dim3(256), 0, st, tiles, d, j, flag);
 }
 
-void launch_chol_offdiag(double* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
+template <typename T>
+void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
                          hipStream_t st) {
   if (maxcount <= 0) return;
-  hipLaunchKernelGGL(k_chol_offdiag, dim3(maxcount, nblocks), dim3(256), 0, st, tiles, d, j);
+  hipLaunchKernelGGL(k_chol_offdiag<T>, dim3(maxcount, nblocks), dim3(256), 0, st, tiles, d, j);
 }
+
+template <typename T>
+void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+                 const double* r, double* sol, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_trsv<T>, dim3(nblocks), dim3(256), 0, st, tiles, d, inds, r, sol);
+}
+
+#define DOUG_INSTANTIATE(T)                                                                    \
+  template void launch_scatter_entries<T>(T*, const int64_t*, const int32_t*, const double*,   \
+                                          int64_t, hipStream_t);                               \
+  template void launch_pad_identity<T>(T*, const BlockDesc*, int, int, hipStream_t);           \
+  template void launch_chol_diag<T>(T*, const BlockDesc*, int, int, int*, hipStream_t);        \
+  template void launch_chol_offdiag<T>(T*, const BlockDesc*, int, int, int, hipStream_t);      \
+  template void launch_trsv<T>(const T*, const BlockDesc*, int, const int32_t*, const double*, \
+                               double*, hipStream_t);
+DOUG_INSTANTIATE(double)
+DOUG_INSTANTIATE(float)
+#undef DOUG_INSTANTIATE
 
 void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, int dd,
                        hipStream_t st) {
@@ -607,12 +676,6 @@ void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32
                  const double* val, const double* x, double* y, hipStream_t st) {
   if (nrb <= 0) return;
   hipLaunchKernelGGL(k_spmv, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x, y);
-}
-
-void launch_trsv(const double* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
-                 const double* r, double* sol, hipStream_t st) {
-  if (nblocks <= 0) return;
-  hipLaunchKernelGGL(k_trsv, dim3(nblocks), dim3(256), 0, st, tiles, d, inds, r, sol);
 }
 
 void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,

@@ -23,18 +23,22 @@ struct BlockDesc {
   int32_t s;        // true size (rows), <= nt*64
 };
 
-// ---- dense batched factorisation ----
-// Zero the tile storage, put ones on the padded diagonal, scatter A entries.
-void launch_scatter_entries(double* tiles, const int64_t* dst, const int32_t* src,
-                            const double* val, int64_t nentries, hipStream_t st);
-void launch_pad_identity(double* tiles, const BlockDesc* d, int nblocks, int maxpad,
-                         hipStream_t st);
+// ---- dense batched factorisation (T = double, or float for FP32 factors) ----
+// Scatter A entries (converted to T) into zeroed tile storage; ones on the
+// padded diagonal.
+template <typename T>
+void launch_scatter_entries(T* tiles, const int64_t* dst, const int32_t* src, const double* val,
+                            int64_t nentries, hipStream_t st);
+template <typename T>
+void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, hipStream_t st);
 // Left-looking tile Cholesky, step j: diagonal tiles (SYRK + POTRF + inverse)
-// then the tiles below them (GEMM + TRSM).  `flag` receives the first failing
-// block+1 on a non-positive pivot.
-void launch_chol_diag(double* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
+// then the tiles below them (GEMM + TRSM), on the matrix cores in T.  `flag`
+// receives the first failing block+1 on a non-positive pivot.
+template <typename T>
+void launch_chol_diag(T* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
                       hipStream_t st);
-void launch_chol_offdiag(double* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
+template <typename T>
+void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
                          hipStream_t st);
 // Explicit inverse of a tile-packed lower factor (single block d[0]):
 // W = L^{-1}, tiles at diagonal offset `dd` (W_{j+dd,j} for all j); W must
@@ -48,8 +52,10 @@ void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, i
 void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                  const double* val, const double* x, double* y, hipStream_t st);
 // Batched forward/backward substitution, one block per subdomain: gathers
-// r(inds), solves L L^T u = r, writes u to sol (concatenated layout).
-void launch_trsv(const double* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+// r(inds), solves L L^T u = r in the factor's precision, writes u to sol
+// (FP64, concatenated layout).
+template <typename T>
+void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
                  const double* r, double* sol, hipStream_t st);
 // u = W c and c' = W^T u for the coarse inverse factor.
 void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,

@@ -37,7 +37,13 @@ enum {
   DOUG_E_UNSUPPORTED = 6
 };
 
-enum { DOUG_PREC_FP64 = 0 };
+/* Precision of the subdomain factors.  The outer PCG (residuals, dot
+ * products, SpMV), the restriction/prolongation and the coarse factor are
+ * FP64 in both forms; with DOUG_PREC_FP32_FACTORS the subdomain Cholesky
+ * factors are computed and applied in FP32 (matrix cores, FP32
+ * accumulation) and the FP64 Krylov iteration refines the result to the
+ * requested FP64 tolerance. */
+enum { DOUG_PREC_FP64 = 0, DOUG_PREC_FP32_FACTORS = 1 };
 
 typedef struct doug_ctx doug_ctx;
 
@@ -94,7 +100,8 @@ int doug_set_partition(doug_ctx *ctx, int32_t nparts, const int32_t *starts,
  * CoarsePreconditioner_smooth_Init, Partitionings_aggr_InitCoarse,
  * FinePreconditioner_Init and the factorisations of Factorise_subdomains and
  * of the coarse matrix (src/main/aggr.F90:131-219,
- * src/solvers/subsolvers.F90:78-163).  precision: DOUG_PREC_FP64. */
+ * src/solvers/subsolvers.F90:78-163).  precision: DOUG_PREC_FP64 or
+ * DOUG_PREC_FP32_FACTORS. */
 int doug_setup(doug_ctx *ctx, int32_t precision);
 /* The host half of doug_setup alone: aggregates, restriction, coarse matrix
  * and subdomain index sets, without any device allocation or factorisation. */

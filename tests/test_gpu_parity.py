@@ -12,14 +12,14 @@ pytestmark = pytest.mark.gpu
 CFG0_CTL = dict(radius1=2, strong1=0.08, smoothers=1, overlap=1, symmstruct=1, symmnumeric=1)
 
 
-def _pair(n, i, j, v, part=None, **ctl):
+def _pair(n, i, j, v, part=None, precision=0, **ctl):
     o = O.Oracle(n, i, j, v, O.params(**ctl), partition=part)
     s = D.Solver(0)
     s.set_params(**ctl)
     s.set_matrix(n, i, j, v, 1)
     if part is not None:
         s.set_partition(*part)
-    s.setup()
+    s.setup(precision)
     return o, s
 
 
@@ -138,3 +138,36 @@ def test_time_operator_runs():
         s.setup()
         for which in (0, 1, 2):
             assert s.time_operator(which, 3) > 0
+
+
+@pytest.mark.parametrize("sigma", [0.0, 1.0])
+def test_fp32_factors_reach_fp64_tolerance(sigma):
+    # DOUG_PREC_FP32_FACTORS: FP32 subdomain factors inside the FP64 Krylov
+    # iteration; the result must meet the same FP64 tolerance as the oracle's
+    n, i, j, v = P.diffusion3d(24, sigma, 4)
+    b = P.uniform_rhs(n, 8)
+    print(f"fp32 factors sigma={sigma} n={n}", flush=True)
+    o, s = _pair(n, i, j, v, P.block_partition((24, 24, 24), 12), precision=1, **CFG0_CTL)
+    with s:
+        info = s.info()
+        x, it, cond, hist = s.pcg(b, 1e-8, 500)
+        xo, ito, _, _ = o.pcg(b, 1e-8, 500)
+        r = b - o.spmv(x)
+        assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.01
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+        assert it <= ito + 2, (it, ito)
+        # the factor storage is half of the FP64 form
+        with D.Solver(0) as s64:
+            s64.set_params(**CFG0_CTL)
+            s64.set_matrix(n, i, j, v, 1)
+            s64.set_partition(*P.block_partition((24, 24, 24), 12))
+            assert s64.setup(0)["factor_bytes"] == 2 * info["factor_bytes"]
+
+
+def test_fp32_preconditioner_close_to_fp64():
+    n, i, j, v = P.diffusion3d(20, 1.0, 3)
+    o, s = _pair(n, i, j, v, P.block_partition((20, 20, 20), 10), precision=1, **CFG0_CTL)
+    with s:
+        r = np.random.default_rng(0).uniform(-1, 1, n)
+        z, zo = s.prec(r), o.prec(r)
+        assert np.linalg.norm(z - zo) <= 1e-4 * np.linalg.norm(zo)
