@@ -32,6 +32,14 @@ HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE (each
+# scaled by the same counters on a copy of known size) for the 128^3
+# workload; the run and its summary are in profiles/hbm_traffic_pmc_128cube.md
+MEASURED_TRAFFIC_128 = {
+    ("k_trsv", 1): 61_373_604_128 + 22_282_240,
+    ("k_spmv", None): 222_930_140 + 16_777_216,
+}
+
 CTL = dict(radius1=4, minasize1=100, maxasize1=343, strong1=0.08, smoothers=1, overlap=1,
            symmstruct=1, symmnumeric=1)
 
@@ -76,6 +84,10 @@ def main():
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=["solve", "kernels"], default="solve",
+                    help="kernels: one setup, then a few launches of the copy, SpMV and "
+                         "subdomain-solve kernels only (for hardware-counter runs)")
+    ap.add_argument("--precision", type=int, default=1, help="factor precision for --mode kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the whole solve "
@@ -101,6 +113,19 @@ def main():
     b = torch.tensor(b_h, dtype=torch.float64, device=dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
     nnz = len(v)
+
+    if args.mode == "kernels":
+        s.setup(args.precision)
+        s.pcg_device(b.data_ptr(), x.data_ptr(), 1e-8, 2)
+        for which, reps in ((3, 5), (0, 5), (1, 3)):
+            ms = s.time_operator(which, reps)
+            log(f"[bench] operator {which}: {ms:.4f} ms")
+        inf = s.info()
+        print(json.dumps({"mode": "kernels", "precision": args.precision, "n": n, "nnz": nnz,
+                          "copy_bytes_per_launch": 16 * n, "spmv_bytes": spmv_bytes(n, nnz),
+                          "trsv_bytes": 2 * inf["factor_bytes"] + 20 * inf["sub_total"]}))
+        s.close()
+        return
 
     def step(precision):
         s.setup(precision)
@@ -174,9 +199,11 @@ def main():
     info = s.info()
     dominant_trsv = best["iterations"] * best["trsv_ms"] * 1e-3 >= best["cholesky_s"]
     if dominant_trsv:
+        traffic = MEASURED_TRAFFIC_128.get(("k_trsv", 1 if best_name == "fp32_factors" else 0)) \
+            if N == 128 and blk == 16 else None
         roof = {"kernel": "k_trsv (batched subdomain forward+backward solves)", "bound": "hbm",
                 "achieved": best["trsv_GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(best["trsv_GB/s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(best["trsv_GB/s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": best["trsv_bytes"], "ms_per_launch": best["trsv_ms"]}
     else:
         peak = FP64_MFMA_PEAK_TFLOPS if best_name == "fp64" else FP32_MFMA_PEAK_TFLOPS
@@ -205,7 +232,8 @@ def main():
         "form": best_name,
         "speedup_fp32_factors_vs_fp64": round(fp64["time_to_solve_s"] / fp32["time_to_solve_s"], 3),
         "spmv": {"GB/s": fp64["spmv_GB/s"], "frac_of_hbm_peak": round(fp64["spmv_GB/s"] / HBM_PEAK_GBS, 4),
-                 "ms": round(fp64["spmv_ms"], 5), "bytes": spmv_bytes(n, nnz)},
+                 "ms": round(fp64["spmv_ms"], 5), "bytes": spmv_bytes(n, nnz),
+                 "traffic": MEASURED_TRAFFIC_128[("k_spmv", None)] if N == 128 else None},
         "fp64": fp64,
         "fp32_factors": fp32,
         "roofline": roof,

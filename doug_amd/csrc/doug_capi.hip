@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/doug.h"
@@ -296,12 +297,13 @@ int doug_set_partition(doug_ctx* ctx, int32_t nparts, const int32_t* starts, con
 
 // ---------------------------------------------------------------------------
 // Dense tile storage for a set of index blocks: descriptors, scatter map and
-// factorisation on the device.
+// factorisation on the device.  Everything is enqueued on the context stream
+// without waiting, so that host setup work can overlap the factorisation.
 template <typename T>
-static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& starts,
-                        const std::vector<int>& inds, bool one_block_solve,
-                        std::vector<BlockDesc>& desc, BlockDesc** d_desc, T** d_tiles,
-                        int64_t* total_elems) {
+static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
+                        const std::vector<int>& starts, const std::vector<int>& inds,
+                        bool one_block_solve, std::vector<BlockDesc>& desc, BlockDesc** d_desc,
+                        T** d_tiles, int64_t* total_elems) {
   const int nb = (int)starts.size() - 1;
   desc.resize((size_t)nb);
   int64_t base = 0;
@@ -316,31 +318,48 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& sta
     base += tile_count(nt) * kTileElems;
   }
   *total_elems = base;
-  // scatter map: lower-triangle entries of M(inds, inds)
+  // scatter map: lower-triangle entries of M(inds, inds), blocks in parallel
+  const int nth = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
+  std::vector<std::vector<int64_t>> dsts((size_t)nth);
+  std::vector<std::vector<int32_t>> srcs((size_t)nth);
+  std::vector<int> dup((size_t)nth, 0);
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+      th.emplace_back([&, t]() {
+        std::vector<int> floc((size_t)M.nrows, -1);
+        std::vector<int64_t> chk;
+        for (int b = t; b < nb; b += nth) {
+          const int s0 = starts[b], s = starts[b + 1] - s0;
+          for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
+          for (int li = 0; li < s; li++) {
+            int g = inds[s0 + li];
+            size_t row_begin = dsts[t].size();
+            for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
+              int lj = floc[M.col[p]];
+              if (lj < 0 || lj > li) continue;
+              dsts[t].push_back(desc[b].base + tile_pos(li, lj));
+              srcs[t].push_back(p);
+            }
+            // duplicate (i,j) pairs would race in the scatter
+            if (dsts[t].size() - row_begin > 1) {
+              chk.assign(dsts[t].begin() + row_begin, dsts[t].end());
+              std::sort(chk.begin(), chk.end());
+              if (std::adjacent_find(chk.begin(), chk.end()) != chk.end()) dup[t] = 1;
+            }
+          }
+          for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < nth; t++)
+    if (dup[t]) return fail(ctx, DOUG_E_UNSUPPORTED, "duplicate matrix entries inside a subdomain are not supported");
   std::vector<int64_t> dst;
   std::vector<int32_t> src;
-  std::vector<int> floc((size_t)M.nrows, -1);
-  for (int b = 0; b < nb; b++) {
-    const int s0 = starts[b], s = starts[b + 1] - s0;
-    for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
-    for (int li = 0; li < s; li++) {
-      int g = inds[s0 + li];
-      size_t row_begin = dst.size();
-      for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
-        int lj = floc[M.col[p]];
-        if (lj < 0 || lj > li) continue;
-        dst.push_back(desc[b].base + tile_pos(li, lj));
-        src.push_back(p);
-      }
-      // duplicate (i,j) pairs would race in the scatter
-      if (dst.size() - row_begin > 1) {
-        std::vector<int64_t> chk(dst.begin() + row_begin, dst.end());
-        std::sort(chk.begin(), chk.end());
-        if (std::adjacent_find(chk.begin(), chk.end()) != chk.end())
-          return fail(ctx, DOUG_E_UNSUPPORTED, "duplicate matrix entries inside a subdomain are not supported");
-      }
-    }
-    for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
+  for (int t = 0; t < nth; t++) {
+    dst.insert(dst.end(), dsts[t].begin(), dsts[t].end());
+    src.insert(src.end(), srcs[t].begin(), srcs[t].end());
   }
   int rc;
   if ((rc = dalloc(ctx, d_tiles, (size_t)base))) return rc;
@@ -349,85 +368,79 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const std::vector<int>& sta
   int32_t* d_src;
   if ((rc = upload(ctx, &d_dst, dst))) return rc;
   if ((rc = upload(ctx, &d_src, src))) return rc;
-  double* d_vals;
-  if ((rc = upload(ctx, &d_vals, M.val))) return rc;
   HIPCHK(hipMemsetAsync(*d_tiles, 0, (size_t)base * sizeof(T), ctx->stream));
   int maxpad = 0;
   for (auto& d : desc) maxpad = std::max(maxpad, d.nt * kTile - d.s);
   launch_pad_identity(*d_tiles, *d_desc, nb, maxpad, ctx->stream);
   launch_scatter_entries(*d_tiles, d_dst, d_src, d_vals, (int64_t)dst.size(), ctx->stream);
+  // the host copies above must stay alive until the stream consumed them
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipGetLastError());
   return DOUG_OK;
 }
 
+// Enqueue the left-looking tile Cholesky of every block; a non-positive
+// pivot sets *flag (checked once the stream has drained).
 template <typename T>
-static int factor_blocks(doug_ctx* ctx, T* tiles, const BlockDesc* d_desc,
-                         const std::vector<BlockDesc>& desc) {
+static int enqueue_factor(doug_ctx* ctx, T* tiles, const BlockDesc* d_desc,
+                          const std::vector<BlockDesc>& desc, int* flag) {
   int ntmax = 0;
   for (auto& d : desc) ntmax = std::max(ntmax, d.nt);
-  HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int), ctx->stream));
   for (int j = 0; j < ntmax; j++) {
-    launch_chol_diag(tiles, d_desc, (int)desc.size(), j, ctx->d_flag, ctx->stream);
+    launch_chol_diag(tiles, d_desc, (int)desc.size(), j, flag, ctx->stream);
     launch_chol_offdiag(tiles, d_desc, (int)desc.size(), j, ntmax - j - 1, ctx->stream);
   }
   HIPCHK(hipGetLastError());
-  int flag = 0;
-  HIPCHK(hipMemcpyAsync(&flag, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  if (flag)
-    return fail(ctx, DOUG_E_NUMERIC,
-                "matrix block " + std::to_string(flag - 1) + " is not positive definite");
   return DOUG_OK;
 }
 
 extern "C" {
 
-static int setup_host(doug_ctx* ctx) {
-  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "setup before doug_set_matrix_coo");
+// ---- setup phases --------------------------------------------------------
+// (1) values and strength: SpMtx_scale / SpMtx_find_strong / SpMtx_unscale
+//     (SpMtx_arrangement.F90:298-388, SpMtx_aggregation.F90:829-899)
+static void phase_values(doug_ctx* ctx, std::vector<double>& scaled, std::vector<uint8_t>& strong) {
   const Params& p = ctx->p;
-  if (p.fine_method != 0 && p.fine_method != 1)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "fine_method must be 0 or 1 on this path");
-  if (p.method != 0 && p.method != 1)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none) or 1 (additive Schwarz)");
-  ctx->ready = false;
-  ctx->host_ready = false;
-  double t0 = now();
-  doug_info& info = ctx->info;
-  info = doug_info{};
   Csr& A = ctx->A;
   A.val = ctx->A_input_val;
-  const int n = A.nrows;
-  ctx->ps = ProcessState{};
-
-  // ---- Partitionings_aggr_InitFine (Partitioning_aggr.F90:33-117) ----
   const bool symm = p.symmstruct && p.symmnumeric;
+  const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
+  std::vector<double> diag;
+  scale_values(A, symm, scaled, diag);
+  strong = find_strong(A, scaled, diag, strong_conn1);
+  A.val = scaled;
+  unscale_values(A, symm, diag, A.val);  // the reference continues with these values
+}
+
+// (2) fine aggregates, smoothed restriction and Galerkin coarse matrix
+//     (Partitioning_aggr.F90:33-117, CoarsePreconditioner_smooth.F90:47-89)
+static int phase_coarse(doug_ctx* ctx, const std::vector<double>& scaled,
+                        const std::vector<uint8_t>& strong) {
+  const Params& p = ctx->p;
+  doug_info& info = ctx->info;
+  Csr& A = ctx->A;
+  double t0 = now();
   const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
   const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
   const int min_asize1 = p.minasize1 > 0 ? p.minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
-  std::vector<double> scaled, diag;
-  scale_values(A, symm, scaled, diag);
-  std::vector<uint8_t> strong = find_strong(A, scaled, diag, strong_conn1);
   aggregate(A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
             ctx->fin, ctx->ffull);
-  A.val = scaled;
-  unscale_values(A, symm, diag, A.val);  // the reference continues with these values
   info.t_aggregate = now() - t0;
-
-  // ---- CoarsePreconditioner_smooth_Init (CoarsePreconditioner_smooth.F90:47-89) ----
   double t1 = now();
-  int smoothers = p.smoothers;
-  if (smoothers > p.radius1) smoothers = p.radius1;  // CoarseMtx.F90:75-78
-  if (smoothers != 0 && smoothers != 1)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "smoothers must resolve to 0 or 1 (got " + std::to_string(smoothers) + ")");
-  ctx->smoothers = smoothers;
-  ctx->R = restriction(A, strong, ctx->fin, smoothers, p.omega);
+  ctx->R = restriction(A, strong, ctx->fin, ctx->smoothers, p.omega);
   ctx->Rt = transpose(ctx->R);
-  ctx->AC = galerkin(A, ctx->R);
-  ctx->has_coarse = p.levels > 1;
+  ctx->AC = galerkin(A, ctx->R, ctx->Rt);
   info.t_coarse_build = now() - t1;
+  return DOUG_OK;
+}
 
-  // ---- subdomains ----
+// (3) subdomain index sets: injected partition, or coarse aggregates
+//     (Partitioning_aggr.F90:120-193), plus `ol` layers (Preconditioner_base.F90:96-123)
+static void phase_subdomains(doug_ctx* ctx) {
+  const Params& p = ctx->p;
+  Csr& A = ctx->A;
+  const int n = A.nrows;
   double t2 = now();
   std::vector<int> pstarts, pnodes;
   ctx->cin = Aggregates{};
@@ -436,7 +449,8 @@ static int setup_host(doug_ctx* ctx) {
     pstarts = ctx->part_starts;
     pnodes = ctx->part_nodes;
   } else {
-    // Partitionings_aggr_InitCoarse (Partitioning_aggr.F90:120-193)
+    const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
+    const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
     const double strong_conn2 = p.strong2 > 0 ? p.strong2 : strong_conn1 / 2.0;
     Csr& AC = ctx->AC;
     std::vector<double> cscaled, cdiag;
@@ -457,12 +471,36 @@ static int setup_host(doug_ctx* ctx) {
     unscale_values(AC, false, cdiag, AC.val);
     coarse_partition(ctx->cfull, ctx->ffull, n, pstarts, pnodes);
   }
-  ctx->ol = p.overlap < 0 ? std::max(smoothers, 0) : p.overlap;  // aggr.F90:207-212
   add_layers_all(A, pstarts, pnodes, ctx->ol, ctx->sstarts, ctx->sinds);
   ctx->nsub = (int)ctx->sstarts.size() - 1;
-  info.t_subdomains = now() - t2;
-  info.n = n;
-  info.nnz = A.nnz();
+  ctx->info.t_subdomains = now() - t2;
+}
+
+static int check_params(doug_ctx* ctx) {
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "setup before doug_set_matrix_coo");
+  const Params& p = ctx->p;
+  if (p.fine_method != 0 && p.fine_method != 1)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "fine_method must be 0 or 1 on this path");
+  if (p.method != 0 && p.method != 1)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none) or 1 (additive Schwarz)");
+  int smoothers = p.smoothers;
+  if (smoothers > p.radius1) smoothers = p.radius1;  // CoarseMtx.F90:75-78
+  if (smoothers != 0 && smoothers != 1)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "smoothers must resolve to 0 or 1 (got " + std::to_string(smoothers) + ")");
+  ctx->smoothers = smoothers;
+  ctx->ol = p.overlap < 0 ? std::max(smoothers, 0) : p.overlap;  // aggr.F90:207-212
+  ctx->has_coarse = p.levels > 1;
+  ctx->ps = ProcessState{};
+  ctx->ready = false;
+  ctx->host_ready = false;
+  ctx->info = doug_info{};
+  return DOUG_OK;
+}
+
+static void fill_info(doug_ctx* ctx) {
+  doug_info& info = ctx->info;
+  info.n = ctx->A.nrows;
+  info.nnz = ctx->A.nnz();
   info.nagr_fine = ctx->fin.nagr;
   info.nagr_coarse = ctx->cfull.nagr;
   info.nsub = ctx->nsub;
@@ -470,20 +508,29 @@ static int setup_host(doug_ctx* ctx) {
   info.overlap = ctx->ol;
   info.smoothers = ctx->smoothers;
   info.sub_total = (int64_t)ctx->sinds.size();
-  info.t_total = now() - t0;
+}
+
+static int setup_host(doug_ctx* ctx) {
+  int rc = check_params(ctx);
+  if (rc) return rc;
+  double t0 = now();
+  std::vector<double> scaled;
+  std::vector<uint8_t> strong;
+  phase_values(ctx, scaled, strong);
+  if ((rc = phase_coarse(ctx, scaled, strong))) return rc;
+  phase_subdomains(ctx);
+  fill_info(ctx);
+  ctx->info.t_total = now() - t0;
   ctx->host_ready = true;
   return DOUG_OK;
 }
 
-static int setup_device(doug_ctx* ctx) {
-  HIPCHK(hipSetDevice(ctx->device));
-  free_device(ctx);
-  double t0d = now();
+// Matrix, vectors, subdomain data and the subdomain factorisation (enqueued).
+static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const Params& p = ctx->p;
   doug_info& info = ctx->info;
   Csr& A = ctx->A;
   const int n = A.nrows;
-  // ---- device data ----
   double t3 = now();
   int rc;
   std::vector<int32_t> ptr(A.ptr.begin(), A.ptr.end()), col(A.col.begin(), A.col.end());
@@ -493,8 +540,67 @@ static int setup_device(doug_ctx* ctx) {
   std::vector<int32_t> rb = row_blocks(A);
   ctx->nrb = (int)rb.size() - 1;
   if ((rc = upload(ctx, &ctx->d_rowblk, rb))) return rc;
-  if ((rc = dalloc(ctx, &ctx->d_flag, 1))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
+  // occurrence lists for the deterministic overlap sum
+  std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(ctx->sinds.size());
+  if (p.fine_method == 1) {
+    for (int v : ctx->sinds) occ_ptr[v + 1]++;
+    for (int i = 0; i < n; i++) occ_ptr[i + 1] += occ_ptr[i];
+    std::vector<int32_t> fill(occ_ptr.begin(), occ_ptr.end() - 1);
+    for (size_t q = 0; q < ctx->sinds.size(); q++) occ_pos[fill[ctx->sinds[q]]++] = (int32_t)q;
+  } else {  // no fine solves: z starts as r (FinePreconditioner_Apply, type NONE)
+    occ_pos.resize((size_t)n);
+    for (int i = 0; i < n; i++) {
+      occ_ptr[i + 1] = i + 1;
+      occ_pos[i] = i;
+    }
+  }
+  if ((rc = upload(ctx, &ctx->d_occ_ptr, occ_ptr)) || (rc = upload(ctx, &ctx->d_occ_pos, occ_pos)))
+    return rc;
+  std::vector<int32_t> inds(ctx->sinds.begin(), ctx->sinds.end());
+  if ((rc = upload(ctx, &ctx->d_inds, inds))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
+  for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
+    if ((rc = dalloc(ctx, v, (size_t)n))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) || (rc = dalloc(ctx, &ctx->d_partial, kRedBlocks)) ||
+      (rc = dalloc(ctx, &ctx->d_counter, 1)))
+    return rc;
+  HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_scal, 0, S_N * sizeof(double), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // host staging vectors go out of scope
+  info.t_upload = now() - t3;
+
+  // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
+  int64_t elems = 0;
+  size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
+  if (p.fine_method == 1) {
+    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
+      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, ctx->sinds, true, ctx->h_sub,
+                             &ctx->d_sub, &ctx->d_tiles32, &elems)))
+        return rc;
+      HIPCHK(hipEventRecord(ev0, ctx->stream));
+      if ((rc = enqueue_factor(ctx, ctx->d_tiles32, ctx->d_sub, ctx->h_sub, ctx->d_flag))) return rc;
+    } else {
+      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, ctx->sinds, true, ctx->h_sub,
+                             &ctx->d_sub, &ctx->d_tiles, &elems)))
+        return rc;
+      HIPCHK(hipEventRecord(ev0, ctx->stream));
+      if ((rc = enqueue_factor(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub, ctx->d_flag))) return rc;
+    }
+  } else {
+    HIPCHK(hipEventRecord(ev0, ctx->stream));
+  }
+  HIPCHK(hipEventRecord(ev1, ctx->stream));
+  info.factor_bytes = elems * (int64_t)esize;
+  return DOUG_OK;
+}
+
+// Restriction/prolongation, coarse factor and its explicit inverse (enqueued).
+static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
+  int rc;
   ctx->nc = ctx->R.nrows;
+  HIPCHK(hipEventRecord(ev2, ctx->stream));
   if (ctx->has_coarse) {
     const Csr& R = ctx->R;
     if ((rc = upload(ctx, &ctx->d_Rptr, R.ptr)) || (rc = upload(ctx, &ctx->d_Rcol, R.col)) ||
@@ -507,68 +613,16 @@ static int setup_device(doug_ctx* ctx) {
     if ((rc = upload(ctx, &ctx->d_Rtptr, Rt.ptr)) || (rc = upload(ctx, &ctx->d_Rtcol, Rt.col)) ||
         (rc = upload(ctx, &ctx->d_Rtval, Rt.val)))
       return rc;
-  }
-  // occurrence lists for the deterministic overlap sum
-  {
-    std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(ctx->sinds.size());
-    if (p.fine_method == 1) {
-      for (int v : ctx->sinds) occ_ptr[v + 1]++;
-      for (int i = 0; i < n; i++) occ_ptr[i + 1] += occ_ptr[i];
-      std::vector<int32_t> fill(occ_ptr.begin(), occ_ptr.end() - 1);
-      for (size_t q = 0; q < ctx->sinds.size(); q++) occ_pos[fill[ctx->sinds[q]]++] = (int32_t)q;
-    } else {  // no fine solves: z starts as r (FinePreconditioner_Apply, type NONE)
-      occ_pos.resize((size_t)n);
-      for (int i = 0; i < n; i++) {
-        occ_ptr[i + 1] = i + 1;
-        occ_pos[i] = i;
-      }
-    }
-    if ((rc = upload(ctx, &ctx->d_occ_ptr, occ_ptr)) || (rc = upload(ctx, &ctx->d_occ_pos, occ_pos)))
-      return rc;
-    std::vector<int32_t> inds(ctx->sinds.begin(), ctx->sinds.end());
-    if ((rc = upload(ctx, &ctx->d_inds, inds))) return rc;
-    if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
-  }
-  for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
-    if ((rc = dalloc(ctx, v, (size_t)n))) return rc;
-  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) || (rc = dalloc(ctx, &ctx->d_partial, kRedBlocks)) ||
-      (rc = dalloc(ctx, &ctx->d_counter, 1)))
-    return rc;
-  HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned), ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->d_scal, 0, S_N * sizeof(double), ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  info.t_upload = now() - t3;
-
-  // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
-  double t4 = now();
-  int64_t elems = 0;
-  size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
-  if (p.fine_method == 1) {
-    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
-      if ((rc = build_blocks(ctx, A, ctx->sstarts, ctx->sinds, true, ctx->h_sub, &ctx->d_sub,
-                             &ctx->d_tiles32, &elems)))
-        return rc;
-      if ((rc = factor_blocks(ctx, ctx->d_tiles32, ctx->d_sub, ctx->h_sub))) return rc;
-    } else {
-      if ((rc = build_blocks(ctx, A, ctx->sstarts, ctx->sinds, true, ctx->h_sub, &ctx->d_sub,
-                             &ctx->d_tiles, &elems)))
-        return rc;
-      if ((rc = factor_blocks(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub))) return rc;
-    }
-  }
-  info.factor_bytes = elems * (int64_t)esize;
-  info.t_factor = now() - t4;
-
-  // ---- coarse factorisation and explicit inverse factor ----
-  double t5 = now();
-  if (ctx->has_coarse) {
+    double* d_acval;
+    if ((rc = upload(ctx, &d_acval, ctx->AC.val))) return rc;
     std::vector<int> cst{0, ctx->nc}, cinds((size_t)ctx->nc);
     for (int i = 0; i < ctx->nc; i++) cinds[i] = i;
     std::vector<BlockDesc> cdesc;
     int64_t celems = 0;
-    if ((rc = build_blocks(ctx, ctx->AC, cst, cinds, false, cdesc, &ctx->d_cdesc, &ctx->d_ctiles, &celems)))
+    if ((rc = build_blocks(ctx, ctx->AC, d_acval, cst, cinds, false, cdesc, &ctx->d_cdesc,
+                           &ctx->d_ctiles, &celems)))
       return rc;
-    if ((rc = factor_blocks(ctx, ctx->d_ctiles, ctx->d_cdesc, cdesc))) return rc;
+    if ((rc = enqueue_factor(ctx, ctx->d_ctiles, ctx->d_cdesc, cdesc, ctx->d_flag + 1))) return rc;
     ctx->cnt = cdesc[0].nt;
     if ((rc = dalloc(ctx, &ctx->d_cW, (size_t)celems))) return rc;
     HIPCHK(hipMemcpyAsync(ctx->d_cW, ctx->d_ctiles, (size_t)celems * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -580,12 +634,9 @@ static int setup_device(doug_ctx* ctx) {
       if ((rc = dalloc(ctx, v, cpad))) return rc;
       HIPCHK(hipMemsetAsync(*v, 0, cpad * sizeof(double), ctx->stream));
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    info.coarse_factor_bytes = 2 * celems * (int64_t)sizeof(double);
+    ctx->info.coarse_factor_bytes = 2 * celems * (int64_t)sizeof(double);
   }
-  info.t_coarse_factor = now() - t5;
-  info.t_total += now() - t0d;
-  ctx->ready = true;
+  HIPCHK(hipEventRecord(ev3, ctx->stream));
   return DOUG_OK;
 }
 
@@ -599,10 +650,46 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   if (ctx->device < 0) return fail(ctx, DOUG_E_STATE, "host-only context: no GPU work");
   if (precision != DOUG_PREC_FP64 && precision != DOUG_PREC_FP32_FACTORS)
     return fail(ctx, DOUG_E_ARG, "precision must be DOUG_PREC_FP64 or DOUG_PREC_FP32_FACTORS");
-  ctx->precision = precision;
-  int rc = setup_host(ctx);
+  int rc = check_params(ctx);
   if (rc) return rc;
-  return setup_device(ctx);
+  ctx->precision = precision;
+  HIPCHK(hipSetDevice(ctx->device));
+  free_device(ctx);
+  double t0 = now();
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  std::vector<double> scaled;
+  std::vector<uint8_t> strong;
+  phase_values(ctx, scaled, strong);
+  if (ctx->have_partition) {
+    // the subdomains do not depend on the aggregation: factorise them on the
+    // GPU while the host builds the coarse space
+    phase_subdomains(ctx);
+    if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;
+    if ((rc = phase_coarse(ctx, scaled, strong))) return rc;
+  } else {
+    if ((rc = phase_coarse(ctx, scaled, strong))) return rc;
+    phase_subdomains(ctx);
+    if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;
+  }
+  fill_info(ctx);
+  ctx->host_ready = true;
+  if ((rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+  int flag[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(flag, ctx->d_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms_f = 0.f, ms_c = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms_f, ev[0], ev[1]));
+  HIPCHK(hipEventElapsedTime(&ms_c, ev[2], ev[3]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  ctx->info.t_factor = ms_f * 1e-3;
+  ctx->info.t_coarse_factor = ms_c * 1e-3;
+  ctx->info.t_total = now() - t0;
+  if (flag[0])
+    return fail(ctx, DOUG_E_NUMERIC, "subdomain " + std::to_string(flag[0] - 1) + " is not positive definite");
+  if (flag[1]) return fail(ctx, DOUG_E_NUMERIC, "the coarse matrix is not positive definite");
+  ctx->ready = true;
+  return DOUG_OK;
 }
 
 int doug_get_info(const doug_ctx* ctx, doug_info* info) {
@@ -912,10 +999,11 @@ int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
         if (ctx->p.fine_method == 1) enqueue_subdomain_solves(ctx, ctx->d_r);
         break;
       case 2: enqueue_prec(ctx, ctx->d_r, ctx->d_z); break;
+      case 3: launch_copy(ctx->d_q, ctx->d_p, ctx->n, st); break;
       default: break;
     }
   };
-  if (which < 0 || which > 2) return fail(ctx, DOUG_E_ARG, "unknown operator");
+  if (which < 0 || which > 3) return fail(ctx, DOUG_E_ARG, "unknown operator");
   run();  // warm-up
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

@@ -469,8 +469,7 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(const int32_t* __restrict
   const int e0 = ptr[r0], e1 = ptr[r1];
   const int t = threadIdx.x;
   if (e1 - e0 <= kSpmvChunk) {
-    for (int e = e0 + t; e < e1; e += kSpmvThreads)
-      prod[e - e0] = __builtin_nontemporal_load(val + e) * x[__builtin_nontemporal_load(col + e)];
+    for (int e = e0 + t; e < e1; e += kSpmvThreads) prod[e - e0] = val[e] * x[col[e]];
     __syncthreads();
     const int row = r0 + t;
     if (row < r1) {

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <queue>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
@@ -13,6 +15,59 @@ namespace doug {
 static constexpr int kMaxInt = 2147483647;            // parameters.F90:99
 static constexpr int kAggregated = kMaxInt;           // parameters.F90:101
 static constexpr int kPending = kMaxInt / 2;          // parameters.F90:102
+
+// Row-parallel loops of the setup: static contiguous chunks, one per thread,
+// so that per-chunk outputs concatenate in row order.  Results do not
+// depend on the thread count.
+static int setup_threads() {
+  unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+}
+
+template <class F>
+static void parallel_chunks(int n, int nthreads, F&& f) {
+  if (nthreads <= 1 || n < 4096) {
+    f(0, 0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; t++) {
+    int b = (int)((int64_t)n * t / nthreads), e = (int)((int64_t)n * (t + 1) / nthreads);
+    th.emplace_back([&f, t, b, e]() { f(t, b, e); });
+  }
+  for (auto& x : th) x.join();
+}
+
+template <class F>
+static void parallel_rows(int n, F&& f) {
+  parallel_chunks(n, setup_threads(), [&f](int, int b, int e) {
+    for (int i = b; i < e; i++) f(i);
+  });
+}
+
+// Concatenate per-chunk CSR pieces (rows in chunk order).
+struct CsrPiece {
+  std::vector<int> rowlen, col;
+  std::vector<double> val;
+};
+static void join_pieces(std::vector<CsrPiece>& pieces, int nrows, Csr& M) {
+  M.ptr.assign((size_t)nrows + 1, 0);
+  size_t total = 0;
+  for (auto& p : pieces) total += p.col.size();
+  M.col.resize(total);
+  M.val.resize(total);
+  int row = 0;
+  size_t off = 0;
+  for (auto& p : pieces) {
+    for (int len : p.rowlen) {
+      M.ptr[row + 1] = M.ptr[row] + len;
+      row++;
+    }
+    std::copy(p.col.begin(), p.col.end(), M.col.begin() + off);
+    std::copy(p.val.begin(), p.val.end(), M.val.begin() + off);
+    off += p.col.size();
+  }
+}
 
 Csr coo_to_csr(int n, int64_t nnz, const int32_t* indi, const int32_t* indj, const double* val,
                int base, std::string& err) {
@@ -43,47 +98,38 @@ Csr coo_to_csr(int n, int64_t nnz, const int32_t* indi, const int32_t* indj, con
 void scale_values(const Csr& A, bool symm, std::vector<double>& scaled, std::vector<double>& diag) {
   int n = std::max(A.nrows, A.ncols);
   diag.assign((size_t)n, 0.0);
-  for (int i = 0; i < A.nrows; i++)
+  parallel_rows(A.nrows, [&](int i) {
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
       if (A.col[p] == i) diag[i] = A.val[p];
-  scaled = A.val;
+  });
+  scaled.resize(A.val.size());
   std::vector<double> sv((size_t)n);
-  if (symm) {
-    for (int i = 0; i < n; i++) sv[i] = std::sqrt(std::fabs(diag[i]));
-    for (int i = 0; i < A.nrows; i++)
-      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
-        double v = scaled[p] / sv[i];
-        scaled[p] = v / sv[A.col[p]];
-      }
-  } else {
-    for (int i = 0; i < n; i++) sv[i] = std::fabs(diag[i]);
-    for (int i = 0; i < A.nrows; i++)
-      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) scaled[p] = scaled[p] / sv[i];
-  }
+  for (int i = 0; i < n; i++) sv[i] = symm ? std::sqrt(std::fabs(diag[i])) : std::fabs(diag[i]);
+  parallel_rows(A.nrows, [&](int i) {
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      double v = A.val[p] / sv[i];
+      scaled[p] = symm ? v / sv[A.col[p]] : v;
+    }
+  });
 }
 
 void unscale_values(const Csr& A, bool symm, const std::vector<double>& diag,
                     std::vector<double>& vals) {
   int n = (int)diag.size();
   std::vector<double> sv((size_t)n);
-  if (symm) {
-    for (int i = 0; i < n; i++) sv[i] = std::sqrt(std::fabs(diag[i]));
-    for (int i = 0; i < A.nrows; i++)
-      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
-        double v = vals[p] * sv[i];
-        vals[p] = v * sv[A.col[p]];
-      }
-  } else {
-    for (int i = 0; i < n; i++) sv[i] = std::fabs(diag[i]);
-    for (int i = 0; i < A.nrows; i++)
-      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) vals[p] = vals[p] * sv[i];
-  }
+  for (int i = 0; i < n; i++) sv[i] = symm ? std::sqrt(std::fabs(diag[i])) : std::fabs(diag[i]);
+  parallel_rows(A.nrows, [&](int i) {
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      double v = vals[p] * sv[i];
+      vals[p] = symm ? v * sv[A.col[p]] : v;
+    }
+  });
 }
 
 std::vector<uint8_t> find_strong(const Csr& A, const std::vector<double>& scaled,
                                  const std::vector<double>& diag, double alpha) {
   std::vector<uint8_t> strong((size_t)A.nnz(), 0);
-  for (int i = 0; i < A.nrows; i++) {
+  parallel_rows(A.nrows, [&](int i) {
     // single-precision literal in the reference (SpMtx_aggregation.F90:867)
     double maxndiag = (double)(-1.0e15f);
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
@@ -95,7 +141,7 @@ std::vector<uint8_t> find_strong(const Csr& A, const std::vector<double>& scaled
       else
         strong[p] = diag[i] > maxndiag;
     }
-  }
+  });
   return strong;
 }
 
@@ -107,21 +153,26 @@ static void strong_refs(const Csr& A, const std::vector<double>& scaled,
                         std::vector<int>& scol) {
   int n = A.nrows;
   srow.assign((size_t)n + 1, 0);
-  for (int i = 0; i < n; i++)
+  parallel_rows(n, [&](int i) {
+    int c = 0;
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
-      if (strong[p] && A.col[p] != i && A.col[p] < n) srow[i + 1]++;
+      if (strong[p] && A.col[p] != i && A.col[p] < n) c++;
+    srow[i + 1] = c;
+  });
   for (int i = 0; i < n; i++) srow[i + 1] += srow[i];
   scol.resize((size_t)srow[n]);
-  std::vector<int> tmp;
-  for (int i = 0; i < n; i++) {
-    tmp.clear();
-    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
-      if (strong[p] && A.col[p] != i && A.col[p] < n) tmp.push_back(p);
-    std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int b) {
-      return std::fabs(scaled[a]) > std::fabs(scaled[b]);
-    });
-    for (size_t k = 0; k < tmp.size(); k++) scol[srow[i] + k] = A.col[tmp[k]];
-  }
+  parallel_chunks(n, setup_threads(), [&](int, int b, int e) {
+    std::vector<int> tmp;
+    for (int i = b; i < e; i++) {
+      tmp.clear();
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+        if (strong[p] && A.col[p] != i && A.col[p] < n) tmp.push_back(p);
+      std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int c) {
+        return std::fabs(scaled[a]) > std::fabs(scaled[c]);
+      });
+      for (size_t k = 0; k < tmp.size(); k++) scol[srow[i] + k] = A.col[tmp[k]];
+    }
+  });
 }
 
 namespace {
@@ -243,6 +294,41 @@ struct Grower {
   }
 };
 
+// The reference's `nextgoodstart` list (SpMtx_aggregation.F90:185-223,
+// 291-314): after each aggregate its best outer-layer nodes are appended;
+// before each seed choice the nodes aggregated meanwhile are dropped (order
+// kept); the seed is the first entry that repeats an earlier entry, else the
+// first entry.  An aggregated node never returns to the list, so the same
+// choice follows from append positions alone: the first repeat is the node
+// whose second occurrence came earliest (a heap), the first entry is the
+// oldest live one (a FIFO).  O(log n) per step instead of a scan of the list.
+struct SeedList {
+  std::vector<int> cnt;
+  std::vector<std::pair<int64_t, int>> fifo;
+  size_t head = 0;
+  std::priority_queue<std::pair<int64_t, int>, std::vector<std::pair<int64_t, int>>,
+                      std::greater<std::pair<int64_t, int>>>
+      rep;
+  int64_t pos = 0, live = 0;
+  explicit SeedList(int n) : cnt((size_t)n, 0) {}
+  void append(int v) {
+    ++pos;
+    ++live;
+    fifo.emplace_back(pos, v);
+    if (++cnt[v] == 2) rep.emplace(pos, v);
+  }
+  void aggregated(int v) {
+    live -= cnt[v];
+    cnt[v] = 0;
+  }
+  int choose(const std::vector<int>& stat) {
+    while (!rep.empty() && stat[rep.top().second] >= kPending) rep.pop();
+    if (!rep.empty()) return rep.top().second;
+    while (stat[fifo[head].second] >= kPending) head++;
+    return fifo[head].second;
+  }
+};
+
 void form_aggr(Aggregates& ag, int nagrs, int n, const std::vector<int>& num) {
   ag.nagr = nagrs;
   ag.num = num;
@@ -273,21 +359,13 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
     std::vector<int> stat((size_t)n, 0), nodes((size_t)n + 1, 0), distance((size_t)n, 0);
     maxasizelargest = coarse_level ? 3 * maxasize : maxasize + 4 * (2 * neighood);
     if (!coarse_level) ps.beta = alpha / 4.0;
-    int ngoodstarts = 0;
-    std::vector<int> nextgood((size_t)n, 0);
+    SeedList seeds(n);
+    std::vector<int> outer;  // outer-layer candidates of the current aggregate
     int firstwocol = 0;
-    std::vector<int64_t> stamp((size_t)n, 0);
-    int64_t stamp_id = 0;
     Grower g(srow, scol, stat, distance, nodes, ps.allmind, neighood, maxasize);
     while (firstwocol < n) {
-      if (ngoodstarts > 0) {
-        int j = 0;
-        for (int i = 0; i < ngoodstarts; i++)
-          if (stat[nextgood[i]] < kPending) nextgood[j++] = nextgood[i];
-        ngoodstarts = j;
-      }
       int startnode = -1;
-      if (ngoodstarts == 0) {
+      if (seeds.live == 0) {
         bool done = false;
         while (stat[firstwocol] >= kPending) {
           firstwocol++;
@@ -299,18 +377,7 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
         if (done) break;
         startnode = firstwocol;
       } else {
-        // the first entry that repeats an earlier one (SpMtx_aggregation.F90:
-        // 209-216 scans all pairs); a stamp per node finds it in one pass
-        stamp_id++;
-        for (int i = 0; i < ngoodstarts; i++) {
-          int v = nextgood[i];
-          if (stamp[v] == stamp_id) {
-            startnode = v;
-            break;
-          }
-          stamp[v] = stamp_id;
-        }
-        if (startnode < 0) startnode = nextgood[0];
+        startnode = seeds.choose(stat);
       }
       int ok = g.colour(startnode);
       int nneigs = g.nneigs;
@@ -341,21 +408,25 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
         } else {
           next_start_layer = 0;
         }
+        outer.clear();
         for (int j = 0; j < nneigs; j++) {
           int node = nodes[j];
           if (stat[node] <= neighood + 1 && j + 1 <= maxasize) {
             stat[node] = kAggregated;
             aggrnum[node] = nagrs;
+            seeds.aggregated(node);
           } else if (next_start_layer > neighood + 1) {
             if (stat[node] == next_start_layer) {
               mindistance = std::min(mindistance, distance[node]);
               nouters++;
-              nextgood[n - nouters] = node;
+              outer.push_back(node);
             }
           }
         }
-        for (int j = 1; j <= nouters; j++)
-          if (distance[nextgood[n - j]] == mindistance) nextgood[ngoodstarts++] = nextgood[n - j];
+        // the reference parks them at the tail of nextgoodstart (first one
+        // at the very end) and reads them back from the end: discovery order
+        for (int j = 0; j < nouters; j++)
+          if (distance[outer[j]] == mindistance) seeds.append(outer[j]);
         for (int j = 0; j < nneigs; j++) {
           int node = nodes[j];
           if (stat[node] != kAggregated) {
@@ -493,18 +564,36 @@ Csr transpose(const Csr& A) {
   Csr T;
   T.nrows = A.ncols;
   T.ncols = A.nrows;
+  const int nth = A.nrows >= 4096 ? setup_threads() : 1;
+  // per-chunk column counts give every chunk its write offsets, so each
+  // output row lists its entries in ascending source-row order
+  std::vector<std::vector<int>> cnt((size_t)nth);
+  parallel_chunks(A.nrows, nth, [&](int t, int b, int e) {
+    cnt[t].assign((size_t)T.nrows, 0);
+    for (int p = A.ptr[b]; p < A.ptr[e]; p++) cnt[t][A.col[p]]++;
+  });
   T.ptr.assign((size_t)T.nrows + 1, 0);
-  for (int c : A.col) T.ptr[c + 1]++;
-  for (int i = 0; i < T.nrows; i++) T.ptr[i + 1] += T.ptr[i];
+  for (int c = 0; c < T.nrows; c++) {
+    int tot = 0;
+    for (int t = 0; t < nth; t++) {
+      int v = cnt[t][c];
+      cnt[t][c] = tot;
+      tot += v;
+    }
+    T.ptr[c + 1] = T.ptr[c] + tot;
+  }
   T.col.resize(A.col.size());
   T.val.resize(A.val.size());
-  std::vector<int> fill(T.ptr.begin(), T.ptr.end() - 1);
-  for (int i = 0; i < A.nrows; i++)
-    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
-      int q = fill[A.col[p]]++;
-      T.col[q] = i;
-      T.val[q] = A.val[p];
-    }
+  parallel_chunks(A.nrows, nth, [&](int t, int b, int e) {
+    std::vector<int>& off = cnt[t];
+    for (int i = b; i < e; i++)
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+        int c = A.col[p];
+        int q = T.ptr[c] + off[c]++;
+        T.col[q] = i;
+        T.val[q] = A.val[p];
+      }
+  });
   return T;
 }
 
@@ -522,16 +611,17 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
   }
   // filtered diagonal and the Jacobi smoother S (CoarseMtx.F90:152-233)
   std::vector<double> diag((size_t)n, 0.0);
-  for (int i = 0; i < n; i++)
+  parallel_rows(n, [&](int i) {
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
       if (A.col[p] == i || !strong[p]) diag[i] = diag[i] + A.val[p];
+  });
   Csr S;
   S.nrows = S.ncols = n;
   S.ptr = A.ptr;
   S.col = A.col;
   S.val.resize(A.val.size());
   const double one_minus = 1.0 - omega;
-  for (int i = 0; i < n; i++)
+  parallel_rows(n, [&](int i) {
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
       if (A.col[p] == i)
         S.val[p] = one_minus;
@@ -540,93 +630,114 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
       else
         S.val[p] = 0.0;  // weak entries stay as explicit zeros (CoarseMtx.F90:187-191)
     }
-  // R = T S^T (SpMtx_op_AB.f90:125-176): row a of R visits the aggregate's
-  // nodes kk ascending and, for each, column kk of S in ascending row order;
-  // an entry is first assigned, then accumulated in that order.
-  Csr St = transpose(S);
-  std::vector<int> mark((size_t)n, -1), list;
-  std::vector<double> acc((size_t)n, 0.0);
-  R.ptr.assign((size_t)nagr + 1, 0);
-  for (int a = 0; a < nagr; a++) {
-    list.clear();
-    for (int q = ag.starts[a]; q < ag.starts[a + 1]; q++) {
-      int kk = ag.nodes[q];
-      for (int p = St.ptr[kk]; p < St.ptr[kk + 1]; p++) {
-        int j = St.col[p];
-        double prod = 1.0 * St.val[p];
-        if (mark[j] != a) {
-          mark[j] = a;
-          list.push_back(j);
-          acc[j] = prod;
-        } else {
-          acc[j] = acc[j] + prod;
-        }
+  });
+  // R = T S^T (SpMtx_op_AB.f90:125-176): the reference visits, for row a of
+  // R, the aggregate's nodes kk ascending and for each the entries S(j,kk);
+  // R(a,j) is the first such S(j,kk) followed by the others in ascending kk.
+  // Built here column by column: for node j the entries S(j,kk) of row j are
+  // ordered by (aggregate of kk, kk) and summed per aggregate in that order,
+  // which gives the same values; R^T comes out row-sorted, R by transposing.
+  Csr Rt;
+  Rt.nrows = n;
+  Rt.ncols = nagr;
+  const int nth = setup_threads();
+  std::vector<CsrPiece> pieces((size_t)nth);
+  parallel_chunks(n, nth, [&](int t, int b, int e) {
+    CsrPiece& out = pieces[t];
+    std::vector<std::pair<std::pair<int, int>, int>> ent;  // ((aggregate, kk), position)
+    for (int j = b; j < e; j++) {
+      ent.clear();
+      for (int p = S.ptr[j]; p < S.ptr[j + 1]; p++) {
+        int a = ag.num[S.col[p]] - 1;
+        if (a >= 0) ent.push_back({{a, S.col[p]}, p});
       }
+      std::stable_sort(ent.begin(), ent.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      int len = 0;
+      for (size_t q = 0; q < ent.size();) {
+        int a = ent[q].first.first;
+        double acc = 1.0 * S.val[ent[q].second];
+        size_t r = q + 1;
+        for (; r < ent.size() && ent[r].first.first == a; r++) acc = acc + 1.0 * S.val[ent[r].second];
+        out.col.push_back(a);
+        out.val.push_back(acc);
+        len++;
+        q = r;
+      }
+      out.rowlen.push_back(len);
     }
-    std::sort(list.begin(), list.end());
-    for (int j : list) {
-      R.col.push_back(j);
-      R.val.push_back(acc[j]);
-    }
-    R.ptr[a + 1] = (int)R.col.size();
-  }
-  return R;
+  });
+  join_pieces(pieces, n, Rt);
+  return transpose(Rt);
 }
 
-Csr galerkin(const Csr& A, const Csr& R) {
+Csr galerkin(const Csr& A, const Csr& R, const Csr& Rt) {
   const int n = A.nrows, nc = R.nrows;
+  const int nth = setup_threads();
   // T' = A R^T: row i visits A's row in stored order, then column k of R in
-  // ascending aggregate order (SpMtx_op_AB.f90 with BT).
-  Csr Rt = transpose(R);  // rows = fine nodes, aggregates ascending
+  // ascending aggregate order (SpMtx_op_AB.f90 with BT); columns of a row in
+  // first-touch order, each entry first assigned then accumulated.
   Csr Tp;
   Tp.nrows = n;
   Tp.ncols = nc;
-  Tp.ptr.assign((size_t)n + 1, 0);
-  std::vector<int> mark((size_t)std::max(n, nc), -1), pos((size_t)std::max(n, nc), 0);
-  for (int i = 0; i < n; i++) {
-    int start = (int)Tp.col.size();
-    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
-      int k = A.col[p];
-      for (int q = Rt.ptr[k]; q < Rt.ptr[k + 1]; q++) {
-        int jj = Rt.col[q];
-        double prod = A.val[p] * Rt.val[q];
-        if (mark[jj] != i) {
-          mark[jj] = i;
-          pos[jj] = (int)Tp.col.size();
-          Tp.col.push_back(jj);
-          Tp.val.push_back(prod);
-        } else {
-          double& v = Tp.val[pos[jj]];
-          v = v + prod;
+  {
+    std::vector<CsrPiece> pieces((size_t)nth);
+    parallel_chunks(n, nth, [&](int t, int b, int e) {
+      CsrPiece& out = pieces[t];
+      std::vector<int> mark((size_t)nc, -1), pos((size_t)nc, 0);
+      for (int i = b; i < e; i++) {
+        size_t start = out.col.size();
+        for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+          int k = A.col[p];
+          for (int q = Rt.ptr[k]; q < Rt.ptr[k + 1]; q++) {
+            int jj = Rt.col[q];
+            double prod = A.val[p] * Rt.val[q];
+            if (mark[jj] != i) {
+              mark[jj] = i;
+              pos[jj] = (int)out.col.size();
+              out.col.push_back(jj);
+              out.val.push_back(prod);
+            } else {
+              double& v = out.val[pos[jj]];
+              v = v + prod;
+            }
+          }
         }
+        out.rowlen.push_back((int)(out.col.size() - start));
       }
-    }
-    (void)start;
-    Tp.ptr[i + 1] = (int)Tp.col.size();
+    });
+    join_pieces(pieces, n, Tp);
   }
   // A_c = R T' with R's rows in ascending column order.
   Csr Ac;
   Ac.nrows = Ac.ncols = nc;
-  Ac.ptr.assign((size_t)nc + 1, 0);
-  std::fill(mark.begin(), mark.end(), -1);
-  for (int i = 0; i < nc; i++) {
-    for (int p = R.ptr[i]; p < R.ptr[i + 1]; p++) {
-      int kk = R.col[p];
-      for (int q = Tp.ptr[kk]; q < Tp.ptr[kk + 1]; q++) {
-        int jj = Tp.col[q];
-        double prod = R.val[p] * Tp.val[q];
-        if (mark[jj] != i) {
-          mark[jj] = i;
-          pos[jj] = (int)Ac.col.size();
-          Ac.col.push_back(jj);
-          Ac.val.push_back(prod);
-        } else {
-          double& v = Ac.val[pos[jj]];
-          v = v + prod;
+  {
+    std::vector<CsrPiece> pieces((size_t)nth);
+    parallel_chunks(nc, nth, [&](int t, int b, int e) {
+      CsrPiece& out = pieces[t];
+      std::vector<int> mark((size_t)nc, -1), pos((size_t)nc, 0);
+      for (int i = b; i < e; i++) {
+        size_t start = out.col.size();
+        for (int p = R.ptr[i]; p < R.ptr[i + 1]; p++) {
+          int kk = R.col[p];
+          for (int q = Tp.ptr[kk]; q < Tp.ptr[kk + 1]; q++) {
+            int jj = Tp.col[q];
+            double prod = R.val[p] * Tp.val[q];
+            if (mark[jj] != i) {
+              mark[jj] = i;
+              pos[jj] = (int)out.col.size();
+              out.col.push_back(jj);
+              out.val.push_back(prod);
+            } else {
+              double& v = out.val[pos[jj]];
+              v = v + prod;
+            }
+          }
         }
+        out.rowlen.push_back((int)(out.col.size() - start));
       }
-    }
-    Ac.ptr[i + 1] = (int)Ac.col.size();
+    });
+    join_pieces(pieces, nc, Ac);
   }
   return Ac;
 }

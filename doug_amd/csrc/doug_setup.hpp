@@ -71,8 +71,8 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
                 int smoothers, double omega);
 
 // CoarseMtxBuild (CoarseMtx.F90:476-516): A_c = R (A R^T), rows in the
-// reference's first-touch column order.
-Csr galerkin(const Csr& A, const Csr& R);
+// reference's first-touch column order; Rt = transpose(R).
+Csr galerkin(const Csr& A, const Csr& R, const Csr& Rt);
 
 // Transpose keeping ascending source-row order inside each output row.
 Csr transpose(const Csr& A);

@@ -144,7 +144,8 @@ int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int
 /* Timing of one operator over `reps` back-to-back launches on the context's
  * stream (HIP events); ms receives the average milliseconds per launch.
  * which: 0 = SpMV, 1 = subdomain forward/backward solves, 2 = whole
- * preconditioner apply. */
+ * preconditioner apply, 3 = copy of one n-vector (a known-size streaming
+ * kernel for calibrating hardware byte counters). */
 int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
 
 #ifdef __cplusplus
