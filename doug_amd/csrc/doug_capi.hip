@@ -83,6 +83,20 @@ struct doug_ctx {
   unsigned* d_counter = nullptr;
   int* d_flag = nullptr;
   double* h_scal = nullptr;  // pinned
+  // several right-hand sides (doug_prepare_batch): W = L^{-1} per subdomain,
+  // padded workspaces, node-major multi-vectors of kMaxRhs columns
+  bool batch_ready = false;
+  int maxnt = 0;
+  int64_t work_rows = 0;
+  double* d_W64 = nullptr;
+  float* d_W32 = nullptr;
+  void* d_G = nullptr;
+  void* d_U = nullptr;
+  double *d_cG = nullptr, *d_cU = nullptr, *d_cS = nullptr, *d_subsolm = nullptr;
+  double *d_Xm = nullptr, *d_Rm = nullptr, *d_Zm = nullptr, *d_Pm = nullptr, *d_Qm = nullptr;
+  double *d_Bm = nullptr, *d_Tm = nullptr, *d_scalm = nullptr, *d_partialm = nullptr;
+  unsigned* d_counterm = nullptr;
+  double* h_scalm = nullptr;  // pinned, kMaxRhs * 8
 };
 
 #define HIPCHK(expr)                                                              \
@@ -126,6 +140,7 @@ static void free_device(doug_ctx* ctx) {
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->ready = false;
+  ctx->batch_ready = false;
 }
 
 // CSR-stream row blocks: consecutive rows with at most kSpmvThreads rows and
@@ -197,7 +212,8 @@ int doug_ctx_create(int device, doug_ctx** out) {
   doug_ctx* ctx = new doug_ctx();
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_scalm, kMaxRhs * 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     delete ctx;
     return DOUG_E_HIP;
   }
@@ -215,6 +231,7 @@ int doug_ctx_destroy(doug_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_device(ctx);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  if (ctx->h_scalm) (void)hipHostFree(ctx->h_scalm);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DOUG_OK;
@@ -306,7 +323,7 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
                         T** d_tiles, int64_t* total_elems) {
   const int nb = (int)starts.size() - 1;
   desc.resize((size_t)nb);
-  int64_t base = 0;
+  int64_t base = 0, woff = 0;
   for (int b = 0; b < nb; b++) {
     int s = starts[b + 1] - starts[b];
     int nt = (s + kTile - 1) / kTile;
@@ -314,8 +331,9 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
       return fail(ctx, DOUG_E_UNSUPPORTED,
                   "subdomain of " + std::to_string(s) + " unknowns exceeds the one-block solve limit " +
                       std::to_string(kMaxTrsvTiles * kTile));
-    desc[b] = BlockDesc{base, (int64_t)starts[b], nt, s};
+    desc[b] = BlockDesc{base, (int64_t)starts[b], woff, nt, s};
     base += tile_count(nt) * kTileElems;
+    woff += (int64_t)nt * kTile;
   }
   *total_elems = base;
   // scatter map: lower-triangle entries of M(inds, inds), blocks in parallel
@@ -593,6 +611,12 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   }
   HIPCHK(hipEventRecord(ev1, ctx->stream));
   info.factor_bytes = elems * (int64_t)esize;
+  ctx->maxnt = 0;
+  ctx->work_rows = 0;
+  for (auto& d : ctx->h_sub) {
+    ctx->maxnt = std::max(ctx->maxnt, d.nt);
+    ctx->work_rows += (int64_t)d.nt * kTile;
+  }
   return DOUG_OK;
 }
 
@@ -988,6 +1012,211 @@ int doug_pcg(doug_ctx* ctx, const double* b, double* x, double tol, int32_t maxi
   return DOUG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Several right-hand sides
+static int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+int doug_prepare_batch(doug_ctx* ctx) {
+  if (!ctx) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_prepare_batch before doug_setup");
+  if (ctx->batch_ready) return DOUG_OK;
+  const int n = ctx->n, NR = kMaxRhs;
+  hipStream_t st = ctx->stream;
+  int rc;
+  if (ctx->p.fine_method == 1) {
+    size_t elems = (size_t)(ctx->info.factor_bytes /
+                            (ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double)));
+    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
+      if ((rc = dalloc(ctx, &ctx->d_W32, elems))) return rc;
+      HIPCHK(hipMemcpyAsync(ctx->d_W32, ctx->d_tiles32, elems * sizeof(float), hipMemcpyDeviceToDevice, st));
+      for (int dd = 1; dd < ctx->maxnt; dd++)
+        launch_trinv_batched(ctx->d_tiles32, ctx->d_W32, ctx->d_sub, ctx->nsub, ctx->maxnt, dd, st);
+      float *g, *u;
+      if ((rc = dalloc(ctx, &g, (size_t)ctx->work_rows * NR)) || (rc = dalloc(ctx, &u, (size_t)ctx->work_rows * NR)))
+        return rc;
+      ctx->d_G = g;
+      ctx->d_U = u;
+    } else {
+      if ((rc = dalloc(ctx, &ctx->d_W64, elems))) return rc;
+      HIPCHK(hipMemcpyAsync(ctx->d_W64, ctx->d_tiles, elems * sizeof(double), hipMemcpyDeviceToDevice, st));
+      for (int dd = 1; dd < ctx->maxnt; dd++)
+        launch_trinv_batched(ctx->d_tiles, ctx->d_W64, ctx->d_sub, ctx->nsub, ctx->maxnt, dd, st);
+      double *g, *u;
+      if ((rc = dalloc(ctx, &g, (size_t)ctx->work_rows * NR)) || (rc = dalloc(ctx, &u, (size_t)ctx->work_rows * NR)))
+        return rc;
+      ctx->d_G = g;
+      ctx->d_U = u;
+    }
+    HIPCHK(hipGetLastError());
+  }
+  if (ctx->has_coarse) {
+    size_t cpad = (size_t)ctx->cnt * kTile * NR;
+    for (double** v : {&ctx->d_cG, &ctx->d_cU, &ctx->d_cS}) {
+      if ((rc = dalloc(ctx, v, cpad))) return rc;
+      HIPCHK(hipMemsetAsync(*v, 0, cpad * sizeof(double), st));
+    }
+  }
+  if ((rc = dalloc(ctx, &ctx->d_subsolm, ctx->sinds.size() * (size_t)NR))) return rc;
+  for (double** v : {&ctx->d_Xm, &ctx->d_Rm, &ctx->d_Zm, &ctx->d_Pm, &ctx->d_Qm, &ctx->d_Bm, &ctx->d_Tm})
+    if ((rc = dalloc(ctx, v, (size_t)n * NR))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_scalm, (size_t)kMaxRhs * 8)) ||
+      (rc = dalloc(ctx, &ctx->d_partialm, (size_t)kRedBlocks * kMaxRhs)) ||
+      (rc = dalloc(ctx, &ctx->d_counterm, 1)))
+    return rc;
+  HIPCHK(hipMemsetAsync(ctx->d_counterm, 0, sizeof(unsigned), st));
+  HIPCHK(hipStreamSynchronize(st));
+  ctx->batch_ready = true;
+  return DOUG_OK;
+}
+
+// z = M^{-1} r for nrp columns (node-major, leading dimension nrp)
+static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* z) {
+  const Params& p = ctx->p;
+  hipStream_t st = ctx->stream;
+  const int n = ctx->n, NR = nrp <= 16 ? 16 : 32;
+  if (p.method == 0) {
+    launch_copy(z, r, n * nrp, st);
+    return;
+  }
+  const bool coarse = ctx->has_coarse;
+  if (coarse) {
+    launch_spmv_multi(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, nrp,
+                      ctx->d_cG, NR, nrp, st);
+    launch_wgemm_fwd(ctx->d_cW, ctx->d_cdesc, 1, ctx->cnt, ctx->d_cG, ctx->d_cU, NR, st);
+    launch_wgemm_bwd(ctx->d_cW, ctx->d_cdesc, 1, ctx->cnt, ctx->d_cU, ctx->d_cS, NR, nrp, NR, st);
+  }
+  const double* fine_src = r;
+  if (p.fine_method == 1) {
+    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
+      float *G = (float*)ctx->d_G, *U = (float*)ctx->d_U;
+      launch_gather_multi(ctx->d_sub, ctx->nsub, ctx->maxnt, ctx->d_inds, r, nrp, NR, G, st);
+      launch_wgemm_fwd(ctx->d_W32, ctx->d_sub, ctx->nsub, ctx->maxnt, G, U, NR, st);
+      launch_wgemm_bwd(ctx->d_W32, ctx->d_sub, ctx->nsub, ctx->maxnt, U, ctx->d_subsolm, nrp, nrp, NR, st);
+    } else {
+      double *G = (double*)ctx->d_G, *U = (double*)ctx->d_U;
+      launch_gather_multi(ctx->d_sub, ctx->nsub, ctx->maxnt, ctx->d_inds, r, nrp, NR, G, st);
+      launch_wgemm_fwd(ctx->d_W64, ctx->d_sub, ctx->nsub, ctx->maxnt, G, U, NR, st);
+      launch_wgemm_bwd(ctx->d_W64, ctx->d_sub, ctx->nsub, ctx->maxnt, U, ctx->d_subsolm, nrp, nrp, NR, st);
+    }
+    fine_src = ctx->d_subsolm;
+  }
+  launch_overlap_sum_multi(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
+                           ctx->d_Rtcol, ctx->d_Rtval, ctx->d_cS, NR, z, st);
+}
+
+int doug_prec_apply_batch_device(doug_ctx* ctx, int32_t nrhs, const double* dr, double* dz) {
+  if (!ctx || !dr || !dz || nrhs < 1 || nrhs > kMaxRhs) return DOUG_E_ARG;
+  if (!ctx->batch_ready) return fail(ctx, DOUG_E_STATE, "doug_prec_apply_batch before doug_prepare_batch");
+  const int nrp = next_pow2(nrhs), n = ctx->n;
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemsetAsync(ctx->d_Rm, 0, (size_t)n * nrp * sizeof(double), st));
+  HIPCHK(hipMemcpy2DAsync(ctx->d_Rm, nrp * sizeof(double), dr, nrhs * sizeof(double), nrhs * sizeof(double), n,
+                          hipMemcpyDeviceToDevice, st));
+  enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm);
+  HIPCHK(hipMemcpy2DAsync(dz, nrhs * sizeof(double), ctx->d_Zm, nrp * sizeof(double), nrhs * sizeof(double), n,
+                          hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return DOUG_OK;
+}
+
+// pcg_weigs run on every column at once; each column stops on its own test
+static int pcg_batch_core(doug_ctx* ctx, int nrhs, double tol, int maxit, int32_t* its,
+                          double* res_hist) {
+  if (maxit == -1) maxit = 100000;
+  const int n = ctx->n, nrp = next_pow2(nrhs);
+  hipStream_t st = ctx->stream;
+  double* hs = ctx->h_scalm;
+  enum { SM_ACTIVE = 6 };
+  // r = b - A x
+  launch_spmv_multi(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Xm, nrp,
+                    ctx->d_Tm, nrp, nrp, st);
+  launch_residual(n * nrp, ctx->d_Bm, ctx->d_Tm, ctx->d_Rm, st);
+  HIPCHK(hipMemsetAsync(ctx->d_Pm, 0, (size_t)n * nrp * sizeof(double), st));
+  for (int k = 0; k < kMaxRhs * 8; k++) hs[k] = 0.0;
+  for (int r = 0; r < nrhs; r++) hs[r * 8 + SM_ACTIVE] = 1.0;
+  HIPCHK(hipMemcpyAsync(ctx->d_scalm, hs, kMaxRhs * 8 * sizeof(double), hipMemcpyHostToDevice, st));
+  launch_dot_multi(ctx->d_Rm, ctx->d_Rm, n, nrp, ctx->d_partialm, ctx->d_counterm, 3, ctx->d_scalm, st);
+  HIPCHK(hipMemcpyAsync(hs, ctx->d_scalm, kMaxRhs * 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::vector<double> init((size_t)nrhs);
+  for (int r = 0; r < nrhs; r++) {
+    init[r] = hs[r * 8 + S_RES] == 0.0 ? 1.0 : hs[r * 8 + S_RES];
+    its[r] = 0;
+  }
+  int it = 0, nactive = nrhs;
+  while (nactive > 0 && it < maxit) {
+    it++;
+    enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm);
+    launch_dot_multi(ctx->d_Rm, ctx->d_Zm, n, nrp, ctx->d_partialm, ctx->d_counterm, 1, ctx->d_scalm, st);
+    launch_p_update_multi(n, nrp, ctx->d_Zm, ctx->d_Pm, ctx->d_scalm, it == 1, st);
+    launch_spmv_multi(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Pm, nrp,
+                      ctx->d_Qm, nrp, nrp, st);
+    launch_dot_multi(ctx->d_Pm, ctx->d_Qm, n, nrp, ctx->d_partialm, ctx->d_counterm, 2, ctx->d_scalm, st);
+    launch_xr_update_multi(n, nrp, ctx->d_Xm, ctx->d_Rm, ctx->d_Pm, ctx->d_Qm, ctx->d_partialm,
+                           ctx->d_counterm, ctx->d_scalm, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hs, ctx->d_scalm, kMaxRhs * 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool changed = false;
+    for (int r = 0; r < nrhs; r++) {
+      if (hs[r * 8 + SM_ACTIVE] == 0.0) continue;
+      double res = hs[r * 8 + S_RES];
+      if (res_hist) res_hist[(int64_t)(it - 1) * nrhs + r] = std::sqrt(res);
+      its[r] = it;
+      if (!(res / init[r] > tol * tol) || it >= maxit) {
+        hs[r * 8 + SM_ACTIVE] = 0.0;
+        nactive--;
+        changed = true;
+      }
+    }
+    if (changed)
+      HIPCHK(hipMemcpyAsync(ctx->d_scalm, hs, kMaxRhs * 8 * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  return DOUG_OK;
+}
+
+int doug_pcg_batch_device(doug_ctx* ctx, int32_t nrhs, const double* dB, double* dX, double tol,
+                          int32_t maxit, int32_t* its, double* res_hist) {
+  if (!ctx || !dB || !dX || !its || nrhs < 1 || nrhs > kMaxRhs) return DOUG_E_ARG;
+  if (!ctx->batch_ready) return fail(ctx, DOUG_E_STATE, "doug_pcg_batch before doug_prepare_batch");
+  const int n = ctx->n, nrp = next_pow2(nrhs);
+  hipStream_t st = ctx->stream;
+  const size_t w = nrhs * sizeof(double), wp = nrp * sizeof(double);
+  HIPCHK(hipMemsetAsync(ctx->d_Bm, 0, (size_t)n * wp, st));
+  HIPCHK(hipMemsetAsync(ctx->d_Xm, 0, (size_t)n * wp, st));
+  HIPCHK(hipMemcpy2DAsync(ctx->d_Bm, wp, dB, w, w, n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpy2DAsync(ctx->d_Xm, wp, dX, w, w, n, hipMemcpyDeviceToDevice, st));
+  int rc = pcg_batch_core(ctx, nrhs, tol, maxit, its, res_hist);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy2DAsync(dX, w, ctx->d_Xm, wp, w, n, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return DOUG_OK;
+}
+
+int doug_pcg_batch(doug_ctx* ctx, int32_t nrhs, const double* B, double* X, double tol,
+                   int32_t maxit, int32_t* its, double* res_hist) {
+  if (!ctx || !B || !X || !its || nrhs < 1 || nrhs > kMaxRhs) return DOUG_E_ARG;
+  if (!ctx->batch_ready) return fail(ctx, DOUG_E_STATE, "doug_pcg_batch before doug_prepare_batch");
+  const int n = ctx->n, nrp = next_pow2(nrhs);
+  hipStream_t st = ctx->stream;
+  const size_t w = nrhs * sizeof(double), wp = nrp * sizeof(double);
+  HIPCHK(hipMemsetAsync(ctx->d_Bm, 0, (size_t)n * wp, st));
+  HIPCHK(hipMemsetAsync(ctx->d_Xm, 0, (size_t)n * wp, st));
+  HIPCHK(hipMemcpy2DAsync(ctx->d_Bm, wp, B, w, w, n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpy2DAsync(ctx->d_Xm, wp, X, w, w, n, hipMemcpyHostToDevice, st));
+  int rc = pcg_batch_core(ctx, nrhs, tol, maxit, its, res_hist);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy2DAsync(X, w, ctx->d_Xm, wp, w, n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return DOUG_OK;
+}
+
 int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
   if (!ctx || !ms || reps <= 0) return DOUG_E_ARG;
   if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_time_operator before doug_setup");
@@ -1000,10 +1229,14 @@ int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
         break;
       case 2: enqueue_prec(ctx, ctx->d_r, ctx->d_z); break;
       case 3: launch_copy(ctx->d_q, ctx->d_p, ctx->n, st); break;
+      case 4:
+        if (ctx->batch_ready) enqueue_prec_multi(ctx, kMaxRhs, ctx->d_Rm, ctx->d_Zm);
+        break;
       default: break;
     }
   };
-  if (which < 0 || which > 3) return fail(ctx, DOUG_E_ARG, "unknown operator");
+  if (which < 0 || which > 4) return fail(ctx, DOUG_E_ARG, "unknown operator");
+  if (which == 4 && !ctx->batch_ready) return fail(ctx, DOUG_E_STATE, "operator 4 needs doug_prepare_batch");
   run();  // warm-up
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

@@ -615,6 +615,304 @@ __global__ void k_residual(int n, const double* __restrict__ b, const double* __
   if (i < n) r[i] = b[i] - ax[i];
 }
 
+
+// ------------------------------------------------------------------------
+// Several right-hand sides: explicit inverse factors and batched GEMMs
+// ------------------------------------------------------------------------
+// W_{i,j} = -inv(L_ii) * sum_{k=j}^{i-1} L_ik W_kj, i = j + dd, every block
+template <typename T>
+__global__ __launch_bounds__(256) void k_trinv_batched(const T* __restrict__ L, T* __restrict__ W,
+                                                       const BlockDesc* __restrict__ d, int dd) {
+  __shared__ T As[kTile * LD];
+  __shared__ T Bs[kTile * LD];
+  typedef typename Mma<T>::v4 V;
+  const BlockDesc b = d[blockIdx.y];
+  const int j = blockIdx.x, i = j + dd;
+  if (i >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  V acc[2][2];
+  zero_acc(acc);
+  for (int k = j; k < i; k++) {
+    tile_to_lds(As, L + b.base + toff(i, k));
+    tile_to_lds(Bs, W + b.base + toff(k, j));
+    __syncthreads();
+    mma_ab<T>(acc, As, Bs, lane, wr, wc);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        Bs[acc_row<T>(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
+  tile_to_lds(As, L + b.base + toff(i, i));  // inv(L_ii)
+  __syncthreads();
+  zero_acc(acc);
+  mma_ab<T>(acc, As, Bs, lane, wr, wc);
+  T* Wg = W + b.base + toff(i, j);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        Wg[acc_row<T>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = -acc[bi][bj][r];
+}
+
+template <typename T>
+__global__ void k_gather_multi(const BlockDesc* __restrict__ d, const int32_t* __restrict__ inds,
+                               const double* __restrict__ rhs, int nrhs, int NR,
+                               T* __restrict__ G) {
+  const BlockDesc b = d[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)b.nt * kTile * NR;
+  if (e >= total) return;
+  const int p = (int)(e / NR), r = (int)(e - (int64_t)p * NR);
+  T v = (T)0;
+  if (p < b.s && r < nrhs) v = (T)rhs[(int64_t)inds[b.vec_off + p] * nrhs + r];
+  G[(b.woff + p) * NR + r] = v;
+}
+
+// 64 x NR block of a workspace -> LDS (stride NR+1)
+template <typename T, int NR>
+__device__ __forceinline__ void panel_to_lds(T* __restrict__ S, const T* __restrict__ g) {
+  for (int e = threadIdx.x; e < kTile * NR; e += 256) S[(e / NR) * (NR + 1) + (e % NR)] = g[e];
+}
+
+// U_i = sum_{k<=i} W_ik G_k: wave w owns rows 16w..16w+15 and all NR columns
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_wgemm_fwd(const T* __restrict__ W,
+                                                   const BlockDesc* __restrict__ d,
+                                                   const T* __restrict__ G, T* __restrict__ U) {
+  __shared__ T As[kTile * LD];
+  __shared__ T Bs[kTile * (NR + 1)];
+  typedef typename Mma<T>::v4 V;
+  const BlockDesc b = d[blockIdx.y];
+  const int i = blockIdx.x;
+  if (i >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  V acc[NR / 16];
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+  for (int k = 0; k <= i; k++) {
+    tile_to_lds(As, W + b.base + toff(i, k));
+    panel_to_lds<T, NR>(Bs, G + (b.woff + (int64_t)k * kTile) * NR);
+    __syncthreads();
+#pragma unroll 4
+    for (int ks = 0; ks < 16; ks++) {
+      const int kk = 4 * ks + lk;
+      const T a = As[(16 * w + li) * LD + kk];
+#pragma unroll
+      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+    }
+    __syncthreads();
+  }
+  T* Ug = U + (b.woff + (int64_t)i * kTile) * NR;
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) Ug[(16 * w + Mma<T>::row(lane, r)) * NR + c * 16 + li] = acc[c][r];
+}
+
+// Z_j = sum_{i>=j} W_ij^T U_i
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_wgemm_bwd(const T* __restrict__ W,
+                                                   const BlockDesc* __restrict__ d,
+                                                   const T* __restrict__ U,
+                                                   double* __restrict__ out, int ldo, int nrhs) {
+  __shared__ T As[kTile * LD];
+  __shared__ T Bs[kTile * (NR + 1)];
+  typedef typename Mma<T>::v4 V;
+  const BlockDesc b = d[blockIdx.y];
+  const int j = blockIdx.x;
+  if (j >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  V acc[NR / 16];
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+  for (int i = j; i < b.nt; i++) {
+    tile_to_lds(As, W + b.base + toff(i, j));
+    panel_to_lds<T, NR>(Bs, U + (b.woff + (int64_t)i * kTile) * NR);
+    __syncthreads();
+#pragma unroll 4
+    for (int ks = 0; ks < 16; ks++) {
+      const int kk = 4 * ks + lk;
+      const T a = As[kk * LD + 16 * w + li];  // (W_ij)^T
+#pragma unroll
+      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int p = j * kTile + 16 * w + Mma<T>::row(lane, r);
+      const int col = c * 16 + li;
+      if (p < b.s && col < nrhs) out[(b.vec_off + p) * ldo + col] = (double)acc[c][r];
+    }
+}
+
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv_multi(
+    const int32_t* __restrict__ rowblk, const int32_t* __restrict__ ptr,
+    const int32_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ x,
+    int ldx, double* __restrict__ y, int ldy, int nrhs) {
+  __shared__ double sval[kSpmvChunk];
+  __shared__ int32_t scol[kSpmvChunk];
+  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int e0 = ptr[r0], e1 = ptr[r1];
+  const int t = threadIdx.x;
+  const bool staged = e1 - e0 <= kSpmvChunk;
+  if (staged) {
+    for (int e = e0 + t; e < e1; e += kSpmvThreads) {
+      sval[e - e0] = val[e];
+      scol[e - e0] = col[e];
+    }
+    __syncthreads();
+  }
+  const int work = (r1 - r0) * nrhs;
+  for (int idx = t; idx < work; idx += kSpmvThreads) {
+    const int row = r0 + idx / nrhs, r = idx % nrhs;
+    double acc = 0.0;
+    if (staged) {
+      for (int e = ptr[row]; e < ptr[row + 1]; e++)
+        acc = acc + sval[e - e0] * x[(int64_t)scol[e - e0] * ldx + r];
+    } else {
+      for (int e = ptr[row]; e < ptr[row + 1]; e++) acc = acc + val[e] * x[(int64_t)col[e] * ldx + r];
+    }
+    y[(int64_t)row * ldy + r] = acc;
+  }
+}
+
+__global__ void k_overlap_sum_multi(int n, int nrhs, const int32_t* __restrict__ occ_ptr,
+                                    const int32_t* __restrict__ occ_pos,
+                                    const double* __restrict__ sol,
+                                    const int32_t* __restrict__ rt_ptr,
+                                    const int32_t* __restrict__ rt_col,
+                                    const double* __restrict__ rt_val,
+                                    const double* __restrict__ csol, int ldc,
+                                    double* __restrict__ z) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * nrhs) return;
+  const int i = (int)(e / nrhs), r = (int)(e - (int64_t)i * nrhs);
+  double acc = 0.0;
+  for (int p = occ_ptr[i]; p < occ_ptr[i + 1]; p++) acc = acc + sol[(int64_t)occ_pos[p] * nrhs + r];
+  if (rt_ptr) {
+    double tmp = 0.0;
+    for (int p = rt_ptr[i]; p < rt_ptr[i + 1]; p++)
+      tmp = tmp + rt_val[p] * csol[(int64_t)rt_col[p] * ldc + r];
+    acc = acc + tmp;
+  }
+  z[e] = acc;
+}
+
+// Per-column reductions over an n x nrhs node-major block: thread t always
+// handles column t % nrhs (the stride is a multiple of nrhs), block sums per
+// column in a fixed order, and the last block finishes every column.
+constexpr int kMultiMax = kMaxRhs;
+enum { SM_ACTIVE = 6 };
+
+__device__ void finish_multi(double* partial, unsigned* counter, int nrhs, int mode, double* scal,
+                             double* colsum) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int r = threadIdx.x;
+  if (r < nrhs) {
+    double v = 0.0;
+    for (int b = 0; b < (int)gridDim.x; b++)
+      v += __hip_atomic_load(&partial[b * kMultiMax + r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double* sc = scal + r * 8;
+    const bool active = sc[SM_ACTIVE] != 0.0;
+    if (mode == 1) {
+      sc[S_BETA] = v / sc[S_RHO];
+      sc[S_RHO_PREV] = sc[S_RHO];
+      sc[S_RHO] = v;
+    } else if (mode == 2) {
+      sc[S_PQ] = v;
+      sc[S_ALPHA] = active ? sc[S_RHO] / v : 0.0;
+    } else {
+      sc[S_RES] = v;
+    }
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+  (void)colsum;
+}
+
+__device__ __forceinline__ void block_colsums(double v, int nrhs, double* sh, double* partial) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  if (t < nrhs) {
+    double s = 0.0;
+    for (int q = t; q < 256; q += nrhs) s += sh[q];
+    partial[blockIdx.x * kMultiMax + t] = s;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_dot_multi(const double* __restrict__ a,
+                                                   const double* __restrict__ b, int64_t total,
+                                                   int nrhs, double* partial, unsigned* counter,
+                                                   int mode, double* scal) {
+  __shared__ double sh[256];
+  const int64_t stride = ((int64_t)gridDim.x * 256 / nrhs) * nrhs;
+  const int64_t start = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (start < stride)
+    for (int64_t e = start; e < total; e += stride) v = v + a[e] * b[e];
+  block_colsums(v, nrhs, sh, partial);
+  finish_multi(partial, counter, nrhs, mode, scal, sh);
+}
+
+__global__ void k_p_update_multi(int64_t total, int nrhs, const double* __restrict__ z,
+                                 double* __restrict__ p, const double* __restrict__ scal,
+                                 int first) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int r = (int)(e % nrhs);
+  if (scal[r * 8 + SM_ACTIVE] == 0.0) return;  // converged columns stay frozen
+  if (first)
+    p[e] = z[e];
+  else
+    p[e] = z[e] + scal[r * 8 + S_BETA] * p[e];
+}
+
+__global__ __launch_bounds__(256) void k_xr_update_multi(int64_t total, int nrhs,
+                                                         double* __restrict__ x,
+                                                         double* __restrict__ r,
+                                                         const double* __restrict__ p,
+                                                         const double* __restrict__ q,
+                                                         double* partial, unsigned* counter,
+                                                         double* scal) {
+  __shared__ double sh[256];
+  const int64_t stride = ((int64_t)gridDim.x * 256 / nrhs) * nrhs;
+  const int64_t start = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (start < stride) {
+    const double* sc = scal + (int)(start % nrhs) * 8;
+    const double alpha = sc[S_ALPHA];
+    if (sc[SM_ACTIVE] != 0.0) {
+      for (int64_t e = start; e < total; e += stride) {
+        x[e] = x[e] + alpha * p[e];
+        const double ri = r[e] - alpha * q[e];
+        r[e] = ri;
+        v = v + ri * ri;
+      }
+    } else {  // converged columns stay frozen
+      for (int64_t e = start; e < total; e += stride) v = v + r[e] * r[e];
+    }
+  }
+  block_colsums(v, nrhs, sh, partial);
+  finish_multi(partial, counter, nrhs, 3, scal, sh);
+}
+
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
@@ -717,6 +1015,89 @@ void launch_copy(double* dst, const double* src, int n, hipStream_t st) {
 
 void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st) {
   hipLaunchKernelGGL(k_residual, dim3(cdiv(n, 256)), dim3(256), 0, st, n, b, ax, r);
+}
+
+template <typename T>
+void launch_trinv_batched(const T* L, T* W, const BlockDesc* d, int nblocks, int maxnt, int dd,
+                          hipStream_t st) {
+  if (maxnt - dd <= 0 || nblocks <= 0) return;
+  hipLaunchKernelGGL(k_trinv_batched<T>, dim3(maxnt - dd, nblocks), dim3(256), 0, st, L, W, d, dd);
+}
+
+template <typename T>
+void launch_gather_multi(const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
+                         const double* rhs, int nrhs, int NR, T* G, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_multi<T>, dim3(cdiv((int64_t)maxnt * kTile * NR, 256), nblocks),
+                     dim3(256), 0, st, d, inds, rhs, nrhs, NR, G);
+}
+
+template <typename T>
+void launch_wgemm_fwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* G, T* U,
+                      int NR, hipStream_t st) {
+  if (NR == 16)
+    hipLaunchKernelGGL((k_wgemm_fwd<T, 16>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, G, U);
+  else
+    hipLaunchKernelGGL((k_wgemm_fwd<T, 32>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, G, U);
+}
+
+template <typename T>
+void launch_wgemm_bwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* U,
+                      double* out, int ldo, int nrhs, int NR, hipStream_t st) {
+  if (NR == 16)
+    hipLaunchKernelGGL((k_wgemm_bwd<T, 16>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, U, out,
+                       ldo, nrhs);
+  else
+    hipLaunchKernelGGL((k_wgemm_bwd<T, 32>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, U, out,
+                       ldo, nrhs);
+}
+
+#define DOUG_INSTANTIATE_MULTI(T)                                                                \
+  template void launch_trinv_batched<T>(const T*, T*, const BlockDesc*, int, int, int,           \
+                                        hipStream_t);                                            \
+  template void launch_gather_multi<T>(const BlockDesc*, int, int, const int32_t*, const double*, \
+                                       int, int, T*, hipStream_t);                               \
+  template void launch_wgemm_fwd<T>(const T*, const BlockDesc*, int, int, const T*, T*, int,     \
+                                    hipStream_t);                                                \
+  template void launch_wgemm_bwd<T>(const T*, const BlockDesc*, int, int, const T*, double*, int, \
+                                    int, int, hipStream_t);
+DOUG_INSTANTIATE_MULTI(double)
+DOUG_INSTANTIATE_MULTI(float)
+#undef DOUG_INSTANTIATE_MULTI
+
+void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
+                       const double* val, const double* x, int ldx, double* y, int ldy, int nrhs,
+                       hipStream_t st) {
+  if (nrb <= 0) return;
+  hipLaunchKernelGGL(k_spmv_multi, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x,
+                     ldx, y, ldy, nrhs);
+}
+
+void launch_overlap_sum_multi(int n, int nrhs, const int32_t* occ_ptr, const int32_t* occ_pos,
+                              const double* sol, const int32_t* rt_ptr, const int32_t* rt_col,
+                              const double* rt_val, const double* csol, int ldc, double* z,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_overlap_sum_multi, dim3(cdiv((int64_t)n * nrhs, 256)), dim3(256), 0, st, n,
+                     nrhs, occ_ptr, occ_pos, sol, rt_ptr, rt_col, rt_val, csol, ldc, z);
+}
+
+void launch_dot_multi(const double* a, const double* b, int n, int nrhs, double* partial,
+                      unsigned* counter, int mode, double* scal, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_multi, dim3(kRedBlocks), dim3(256), 0, st, a, b, (int64_t)n * nrhs,
+                     nrhs, partial, counter, mode, scal);
+}
+
+void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const double* scal,
+                           int first, hipStream_t st) {
+  int64_t total = (int64_t)n * nrhs;
+  hipLaunchKernelGGL(k_p_update_multi, dim3(cdiv(total, 256)), dim3(256), 0, st, total, nrhs, z, p,
+                     scal, first);
+}
+
+void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double* p,
+                            const double* q, double* partial, unsigned* counter, double* scal,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_xr_update_multi, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs, nrhs,
+                     x, r, p, q, partial, counter, scal);
 }
 
 }  // namespace doug

@@ -11,6 +11,7 @@ constexpr int kSpmvThreads = 256;      // CSR-stream block
 constexpr int kSpmvChunk = 2048;       // products staged in LDS per block
 constexpr int kMaxTrsvTiles = 128;     // subdomain size limit of the one-block solve (8192)
 constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reductions
+constexpr int kMaxRhs = 32;            // right-hand sides per batched solve
 
 // Dense subdomain block in tile-packed lower storage: tile (i,k), i>=k, of
 // block b starts at base + (i*(i+1)/2 + k) * kTileElems; each tile is 64x64
@@ -19,6 +20,7 @@ constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reduct
 struct BlockDesc {
   int64_t base;     // element offset of the block's tile storage
   int64_t vec_off;  // offset into the concatenated subdomain vectors
+  int64_t woff;     // first row of the block in the padded multi-vector workspaces
   int32_t nt;       // tiles per dimension
   int32_t s;        // true size (rows), <= nt*64
 };
@@ -78,5 +80,44 @@ void launch_xr_update(int n, double* x, double* r, const double* p, const double
                       double* partial, unsigned* counter, double* scal, hipStream_t st);
 void launch_copy(double* dst, const double* src, int n, hipStream_t st);
 void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st);
+
+// ---- several right-hand sides at once ----
+// Multi-vectors are node-major: entry (i, r) of an n x nrhs block sits at
+// i*ld + r.  Subdomain workspaces hold nt*64 rows of NR (16 or 32) columns
+// per block, starting at row woff.
+// W = L^{-1} for every block (grid.y = block), tiles at diagonal offset dd;
+// W must start as a copy of L.
+template <typename T>
+void launch_trinv_batched(const T* L, T* W, const BlockDesc* d, int nblocks, int maxnt, int dd,
+                          hipStream_t st);
+// G(block rows, :) = rhs(inds, :) converted to T, zero padded
+template <typename T>
+void launch_gather_multi(const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
+                         const double* rhs, int nrhs, int NR, T* G, hipStream_t st);
+// U_i = sum_{k<=i} W_ik G_k for every block and tile row i
+template <typename T>
+void launch_wgemm_fwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* G, T* U,
+                      int NR, hipStream_t st);
+// Z_j = sum_{i>=j} W_ij^T U_i, written (FP64) to out[(vec_off + p) * ldo + r]
+// for rows p < s and columns r < nrhs
+template <typename T>
+void launch_wgemm_bwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* U,
+                      double* out, int ldo, int nrhs, int NR, hipStream_t st);
+// y(:, r) = A x(:, r) for r < nrhs, each row summed in stored order
+void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
+                       const double* val, const double* x, int ldx, double* y, int ldy, int nrhs,
+                       hipStream_t st);
+void launch_overlap_sum_multi(int n, int nrhs, const int32_t* occ_ptr, const int32_t* occ_pos,
+                              const double* sol, const int32_t* rt_ptr, const int32_t* rt_col,
+                              const double* rt_val, const double* csol, int ldc, double* z,
+                              hipStream_t st);
+// per-column dot products; mode as launch_dot, scalars at scal[r*8 + k]
+void launch_dot_multi(const double* a, const double* b, int n, int nrhs, double* partial,
+                      unsigned* counter, int mode, double* scal, hipStream_t st);
+void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const double* scal,
+                           int first, hipStream_t st);
+void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double* p,
+                            const double* q, double* partial, unsigned* counter, double* scal,
+                            hipStream_t st);
 
 }  // namespace doug

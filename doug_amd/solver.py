@@ -193,6 +193,35 @@ class Solver:
                                             int(maxit), C.byref(it), C.byref(cond), _lib.ptr(hist)))
         return it.value, cond.value, hist[: it.value].copy()
 
+    # -- several right-hand sides --
+    def prepare_batch(self):
+        """Explicit inverse subdomain factors for batched applies (doug_prepare_batch)."""
+        self._check(self._L.doug_prepare_batch(self.h))
+
+    def pcg_batch(self, B, tol=1e-8, maxit=-1, X0=None):
+        """Solve for every column of B (n x nrhs, nrhs <= 32) in one pass.
+        Returns (X, iterations per column, residual norms (iterations x nrhs))."""
+        B = np.ascontiguousarray(B, np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        nrhs = B.shape[1]
+        X = np.zeros_like(B) if X0 is None else np.array(X0, np.float64, order="C")
+        cap = maxit if maxit > 0 else 100000
+        hist = np.zeros((cap, nrhs))
+        its = np.zeros(nrhs, np.int32)
+        self._check(self._L.doug_pcg_batch(self.h, nrhs, _lib.ptr(B), _lib.ptr(X), float(tol),
+                                           int(maxit), _lib.ptr(its), _lib.ptr(hist)))
+        return X, its, hist[: int(its.max(initial=0))]
+
+    def pcg_batch_device(self, nrhs: int, dB: int, dX: int, tol=1e-8, maxit=-1):
+        its = np.zeros(nrhs, np.int32)
+        self._check(self._L.doug_pcg_batch_device(self.h, int(nrhs), C.c_void_p(dB), C.c_void_p(dX),
+                                                  float(tol), int(maxit), _lib.ptr(its), None))
+        return its
+
+    def prec_batch_device(self, nrhs: int, dR: int, dZ: int):
+        self._check(self._L.doug_prec_apply_batch_device(self.h, int(nrhs), C.c_void_p(dR), C.c_void_p(dZ)))
+
     def time_operator(self, which: int, reps: int = 10) -> float:
         ms = C.c_double(0.0)
         self._check(self._L.doug_time_operator(self.h, int(which), int(reps), C.byref(ms)))

@@ -145,8 +145,25 @@ int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int
  * stream (HIP events); ms receives the average milliseconds per launch.
  * which: 0 = SpMV, 1 = subdomain forward/backward solves, 2 = whole
  * preconditioner apply, 3 = copy of one n-vector (a known-size streaming
- * kernel for calibrating hardware byte counters). */
+ * kernel for calibrating hardware byte counters), 4 = preconditioner apply
+ * to 32 right-hand sides at once (needs doug_prepare_batch). */
 int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
+
+/* Several right-hand sides in one pass.  After doug_setup, doug_prepare_batch
+ * forms the explicit inverse factors W_i = L_i^{-1} of every subdomain on the
+ * matrix cores; each preconditioner apply is then one gather, two batched
+ * tile GEMMs (W_i, W_i^T) and one overlap sum for all columns, reading each
+ * factor tile once per apply whatever the number of columns.  Multi-vectors
+ * are node-major: entry (i, r) at i*nrhs + r; 1 <= nrhs <= 32.  Every column
+ * runs pcg_weigs's recurrence (pcg.F90:387-435) and stops on its own test;
+ * its[r] receives the iteration count of column r, res_hist (optional,
+ * maxit*nrhs) the residual norms. */
+int doug_prepare_batch(doug_ctx *ctx);
+int doug_prec_apply_batch_device(doug_ctx *ctx, int32_t nrhs, const double *dr, double *dz);
+int doug_pcg_batch(doug_ctx *ctx, int32_t nrhs, const double *B, double *X, double tol,
+                   int32_t maxit, int32_t *its, double *res_hist);
+int doug_pcg_batch_device(doug_ctx *ctx, int32_t nrhs, const double *dB, double *dX, double tol,
+                          int32_t maxit, int32_t *its, double *res_hist);
 
 #ifdef __cplusplus
 }

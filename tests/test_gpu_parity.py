@@ -171,3 +171,42 @@ def test_fp32_preconditioner_close_to_fp64():
         r = np.random.default_rng(0).uniform(-1, 1, n)
         z, zo = s.prec(r), o.prec(r)
         assert np.linalg.norm(z - zo) <= 1e-4 * np.linalg.norm(zo)
+
+
+@pytest.mark.parametrize("precision,nrhs", [(0, 5), (1, 32)])
+def test_batched_right_hand_sides(precision, nrhs):
+    # doug_pcg_batch: every column follows its own pcg_weigs recurrence
+    n, i, j, v = P.diffusion3d(20, 1.0, 6)
+    part = P.block_partition((20, 20, 20), 10)
+    B = np.stack([P.uniform_rhs(n, 100 + k) for k in range(nrhs)], axis=1)
+    print(f"batch precision={precision} nrhs={nrhs} n={n}", flush=True)
+    o, s = _pair(n, i, j, v, part, precision=precision, **CFG0_CTL)
+    with s:
+        s.prepare_batch()
+        X, its, hist = s.pcg_batch(B, 1e-8, 500)
+        for k in (0, nrhs // 2, nrhs - 1):
+            xo, ito, _, _ = o.pcg(B[:, k], 1e-8, 500)
+            assert abs(int(its[k]) - ito) <= 1, (k, its[k], ito)
+            r = B[:, k] - o.spmv(X[:, k])
+            assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, k]) * 1.01
+            tol_x = 1e-8 if precision == 0 else 1e-6
+            assert np.linalg.norm(X[:, k] - xo) <= tol_x * np.linalg.norm(xo)
+
+
+def test_batched_preconditioner_matches_single():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("torch sees no GPU")
+    n, i, j, v = P.diffusion3d(20, 1.0, 3)
+    o, s = _pair(n, i, j, v, P.block_partition((20, 20, 20), 10), **CFG0_CTL)
+    with s:
+        s.prepare_batch()
+        R = np.random.default_rng(4).uniform(-1, 1, (n, 3))
+        dR = torch.tensor(R, device="cuda:0")
+        dZ = torch.empty_like(dR)
+        torch.cuda.synchronize()
+        s.prec_batch_device(3, dR.data_ptr(), dZ.data_ptr())
+        Z = dZ.cpu().numpy()
+        for k in range(3):
+            zo = o.prec(R[:, k])
+            assert np.linalg.norm(Z[:, k] - zo) <= 1e-11 * np.linalg.norm(zo)
