@@ -53,7 +53,7 @@ struct doug_ctx {
 
   // device: matrix
   std::vector<void*> allocs;
-  int32_t *d_ptr = nullptr, *d_col = nullptr, *d_rowblk = nullptr;
+  int32_t *d_ptr = nullptr, *d_col = nullptr, *d_rowblk = nullptr, *d_src = nullptr;
   double* d_val = nullptr;
   int nrb = 0;
   // restriction (CSR rows = aggregates), prolongation (CSR rows = nodes)
@@ -558,6 +558,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   std::vector<int32_t> rb = row_blocks(A);
   ctx->nrb = (int)rb.size() - 1;
   if ((rc = upload(ctx, &ctx->d_rowblk, rb))) return rc;
+  if ((rc = upload(ctx, &ctx->d_src, A.src))) return rc;
   if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
   HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
   // occurrence lists for the deterministic overlap sum
@@ -1214,6 +1215,15 @@ int doug_pcg_batch(doug_ctx* ctx, int32_t nrhs, const double* B, double* X, doub
   if (rc) return rc;
   HIPCHK(hipMemcpy2DAsync(X, w, ctx->d_Xm, wp, w, n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return DOUG_OK;
+}
+
+int doug_value_grad_device(doug_ctx* ctx, const double* dlam, const double* dx, double* dgrad) {
+  if (!ctx || !dlam || !dx || !dgrad) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_value_grad before doug_setup");
+  launch_value_grad(ctx->n, ctx->d_ptr, ctx->d_col, ctx->d_src, dlam, dx, dgrad, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return DOUG_OK;
 }
 

@@ -913,6 +913,17 @@ __global__ __launch_bounds__(256) void k_xr_update_multi(int64_t total, int nrhs
   finish_multi(partial, counter, nrhs, 3, scal, sh);
 }
 
+// d loss / d a_ij = -lambda_i x_j for every stored entry, written to the
+// entry's position in the caller's COO input
+__global__ void k_value_grad(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                             const int32_t* __restrict__ src, const double* __restrict__ lam,
+                             const double* __restrict__ x, double* __restrict__ grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double li = lam[i];
+  for (int p = ptr[i]; p < ptr[i + 1]; p++) grad[src[p]] = -li * x[col[p]];
+}
+
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
@@ -1098,6 +1109,11 @@ void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double*
                             hipStream_t st) {
   hipLaunchKernelGGL(k_xr_update_multi, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs, nrhs,
                      x, r, p, q, partial, counter, scal);
+}
+
+void launch_value_grad(int n, const int32_t* ptr, const int32_t* col, const int32_t* src,
+                       const double* lam, const double* x, double* grad, hipStream_t st) {
+  hipLaunchKernelGGL(k_value_grad, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ptr, col, src, lam, x, grad);
 }
 
 }  // namespace doug

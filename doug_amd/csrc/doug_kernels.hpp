@@ -120,4 +120,8 @@ void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double*
                             const double* q, double* partial, unsigned* counter, double* scal,
                             hipStream_t st);
 
+// grad[src[p]] = -lam[row(p)] * x[col[p]] over the CSR entries of A
+void launch_value_grad(int n, const int32_t* ptr, const int32_t* col, const int32_t* src,
+                       const double* lam, const double* x, double* grad, hipStream_t st);
+
 }  // namespace doug

@@ -85,12 +85,14 @@ Csr coo_to_csr(int n, int64_t nnz, const int32_t* indi, const int32_t* indj, con
   for (int i = 0; i < n; i++) A.ptr[i + 1] += A.ptr[i];
   A.col.resize((size_t)nnz);
   A.val.resize((size_t)nnz);
+  A.src.resize((size_t)nnz);
   std::vector<int> fill(A.ptr.begin(), A.ptr.end() - 1);
   for (int64_t t = 0; t < nnz; t++) {
     int i = indi[t] - base;
     int p = fill[i]++;
     A.col[p] = indj[t] - base;
     A.val[p] = val[t];
+    A.src[p] = (int)t;
   }
   return A;
 }

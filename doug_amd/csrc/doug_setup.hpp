@@ -18,6 +18,7 @@ struct Csr {
   int nrows = 0, ncols = 0;
   std::vector<int> ptr, col;
   std::vector<double> val;
+  std::vector<int> src;  // input position of each entry (coo_to_csr only)
   int64_t nnz() const { return (int64_t)col.size(); }
 };
 

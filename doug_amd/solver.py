@@ -193,6 +193,11 @@ class Solver:
                                             int(maxit), C.byref(it), C.byref(cond), _lib.ptr(hist)))
         return it.value, cond.value, hist[: it.value].copy()
 
+    def value_grad_device(self, dlam: int, dx: int, dgrad: int):
+        """dgrad[e] = -lam[i_e] * x[j_e] for every input entry e (device pointers)."""
+        self._check(self._L.doug_value_grad_device(self.h, C.c_void_p(dlam), C.c_void_p(dx),
+                                                   C.c_void_p(dgrad)))
+
     # -- several right-hand sides --
     def prepare_batch(self):
         """Explicit inverse subdomain factors for batched applies (doug_prepare_batch)."""

@@ -149,6 +149,14 @@ int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int
  * to 32 right-hand sides at once (needs doug_prepare_batch). */
 int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
 
+/* Adjoint of the solve with respect to the matrix values.  For a loss L with
+ * x = A^{-1} b and lambda = A^{-T} dL/dx (a second doug_pcg with A^T = A),
+ * dL/da_ij = -lambda_i x_j.  dgrad (device, nnz entries) receives that value
+ * for every entry of the doug_set_matrix_coo input, in the input's order
+ * (duplicates each receive the full value).  No counterpart in the reference:
+ * it serves differentiable callers (torch.autograd). */
+int doug_value_grad_device(doug_ctx *ctx, const double *dlam, const double *dx, double *dgrad);
+
 /* Several right-hand sides in one pass.  After doug_setup, doug_prepare_batch
  * forms the explicit inverse factors W_i = L_i^{-1} of every subdomain on the
  * matrix cores; each preconditioner apply is then one gather, two batched
