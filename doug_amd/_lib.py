@@ -77,6 +77,7 @@ SIGNATURES = {
     "doug_time_operator": (_i32, [_vp, _i32, _i32, C.POINTER(_dbl)]),
     "doug_prepare_batch": (_i32, [_vp]),
     "doug_value_grad_device": (_i32, [_vp, _vp, _vp, _vp]),
+    "doug_set_option": (_i32, [_vp, _i32, _i64]),
     "doug_prec_apply_batch_device": (_i32, [_vp, _i32, _vp, _vp]),
     "doug_pcg_batch": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, _vp, _vp]),
     "doug_pcg_batch_device": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, _vp, _vp]),

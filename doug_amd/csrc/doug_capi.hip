@@ -97,6 +97,9 @@ struct doug_ctx {
   double *d_Bm = nullptr, *d_Tm = nullptr, *d_scalm = nullptr, *d_partialm = nullptr;
   unsigned* d_counterm = nullptr;
   double* h_scalm = nullptr;  // pinned, kMaxRhs * 8
+  // one PCG iteration captured as a HIP graph (iterations >= 2)
+  bool use_graph = true;
+  hipGraphExec_t iter_graph = nullptr;
 };
 
 #define HIPCHK(expr)                                                              \
@@ -137,6 +140,10 @@ static int upload(doug_ctx* ctx, T** out, const std::vector<T>& h) {
 }
 
 static void free_device(doug_ctx* ctx) {
+  if (ctx->iter_graph) {
+    (void)hipGraphExecDestroy(ctx->iter_graph);
+    ctx->iter_graph = nullptr;
+  }
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->ready = false;
@@ -961,16 +968,34 @@ static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, doubl
   std::vector<double> alpha((size_t)std::min(maxit, 1 << 20) + 2, 0.0), beta(alpha.size(), 0.0);
   int it = 0;
   double ratio = 1.0;
-  while (ratio > tol * tol && it < maxit) {
-    it++;
+  // iteration body (pcg.F90:387-435); from the second iteration on it is
+  // replayed from a captured HIP graph
+  auto enqueue_iteration = [&](bool first) {
     enqueue_prec(ctx, ctx->d_r, ctx->d_z);
     launch_dot(ctx->d_r, ctx->d_z, n, ctx->d_partial, ctx->d_counter, nullptr, 1, ctx->d_scal, st);
-    launch_p_update(n, ctx->d_z, ctx->d_p, ctx->d_scal, it == 1, st);
+    launch_p_update(n, ctx->d_z, ctx->d_p, ctx->d_scal, first ? 1 : 0, st);
     enqueue_spmv(ctx, ctx->d_p, ctx->d_q);
     launch_dot(ctx->d_p, ctx->d_q, n, ctx->d_partial, ctx->d_counter, nullptr, 2, ctx->d_scal, st);
     launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_counter, ctx->d_scal, st);
+    (void)hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st);
+  };
+  while (ratio > tol * tol && it < maxit) {
+    it++;
+    if (it >= 2 && ctx->use_graph) {
+      if (!ctx->iter_graph) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration(false);
+        HIPCHK(hipStreamEndCapture(st, &g));
+        hipError_t ie = hipGraphInstantiate(&ctx->iter_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ie);
+      }
+      HIPCHK(hipGraphLaunch(ctx->iter_graph, st));
+    } else {
+      enqueue_iteration(it == 1);
+    }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if ((size_t)it < alpha.size()) {
       alpha[it] = scal[S_ALPHA];
@@ -1216,6 +1241,17 @@ int doug_pcg_batch(doug_ctx* ctx, int32_t nrhs, const double* B, double* X, doub
   HIPCHK(hipMemcpy2DAsync(X, w, ctx->d_Xm, wp, w, n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return DOUG_OK;
+}
+
+int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
+  if (!ctx) return DOUG_E_ARG;
+  switch (option) {
+    case DOUG_OPT_ITERATION_GRAPH:
+      ctx->use_graph = value != 0;
+      return DOUG_OK;
+    default:
+      return fail(ctx, DOUG_E_ARG, "unknown option " + std::to_string(option));
+  }
 }
 
 int doug_value_grad_device(doug_ctx* ctx, const double* dlam, const double* dx, double* dgrad) {

@@ -193,6 +193,10 @@ class Solver:
                                             int(maxit), C.byref(it), C.byref(cond), _lib.ptr(hist)))
         return it.value, cond.value, hist[: it.value].copy()
 
+    def set_option(self, option: int, value: int):
+        """doug_set_option; option 1 = replay PCG iterations from a HIP graph."""
+        self._check(self._L.doug_set_option(self.h, int(option), int(value)))
+
     def value_grad_device(self, dlam: int, dx: int, dgrad: int):
         """dgrad[e] = -lam[i_e] * x[j_e] for every input entry e (device pointers)."""
         self._check(self._L.doug_value_grad_device(self.h, C.c_void_p(dlam), C.c_void_p(dx),

@@ -149,6 +149,12 @@ int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int
  * to 32 right-hand sides at once (needs doug_prepare_batch). */
 int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
 
+/* Execution options.  DOUG_OPT_ITERATION_GRAPH (default 1): from the second
+ * PCG iteration on, replay the iteration's kernels (preconditioner, dots,
+ * updates, SpMV, scalar read-back) from one captured HIP graph. */
+enum { DOUG_OPT_ITERATION_GRAPH = 1 };
+int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
+
 /* Adjoint of the solve with respect to the matrix values.  For a loss L with
  * x = A^{-1} b and lambda = A^{-T} dL/dx (a second doug_pcg with A^T = A),
  * dL/da_ij = -lambda_i x_j.  dgrad (device, nnz entries) receives that value

