@@ -43,10 +43,24 @@ class DougInfo(C.Structure):
         ("nnz", _i64), ("sub_total", _i64), ("factor_bytes", _i64), ("coarse_factor_bytes", _i64),
         ("t_aggregate", _dbl), ("t_coarse_build", _dbl), ("t_subdomains", _dbl),
         ("t_upload", _dbl), ("t_factor", _dbl), ("t_coarse_factor", _dbl), ("t_total", _dbl),
+        ("factor_tiles", _i64), ("factor_flops", _dbl),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class DougTuning(C.Structure):
+    _fields_ = [
+        ("spmv_shape", _i32), ("spmv_rows", _i32), ("spmv_xcd_order", _i32), ("trsv_order", _i32),
+        ("spmv_ms", _dbl * 6), ("trsv_ms", _dbl * 2),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["spmv_ms"] = list(self.spmv_ms)
+        d["trsv_ms"] = list(self.trsv_ms)
+        return d
 
 
 # every symbol include/doug.h declares, with its ctypes signature
@@ -78,6 +92,8 @@ SIGNATURES = {
     "doug_prepare_batch": (_i32, [_vp]),
     "doug_value_grad_device": (_i32, [_vp, _vp, _vp, _vp]),
     "doug_set_option": (_i32, [_vp, _i32, _i64]),
+    "doug_autotune": (_i32, [_vp, _i32, C.POINTER(DougTuning)]),
+    "doug_get_tuning": (_i32, [_vp, C.POINTER(DougTuning)]),
     "doug_prec_apply_batch_device": (_i32, [_vp, _i32, _vp, _vp]),
     "doug_pcg_batch": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, _vp, _vp]),
     "doug_pcg_batch_device": (_i32, [_vp, _i32, _vp, _vp, _dbl, _i32, _vp, _vp]),

@@ -55,7 +55,11 @@ struct doug_ctx {
   std::vector<void*> allocs;
   int32_t *d_ptr = nullptr, *d_col = nullptr, *d_rowblk = nullptr, *d_src = nullptr;
   double* d_val = nullptr;
-  int nrb = 0;
+  int nrb = 0;  // row blocks of kSpmvThreads rows (multi-vector SpMV)
+  // the single-vector SpMV runs with the tuned shape over its own row blocks
+  int spmv_shape = kSpmvDefaultShape;
+  int32_t* d_rowblk_a = nullptr;
+  int nrb_a = 0;
   // restriction (CSR rows = aggregates), prolongation (CSR rows = nodes)
   int32_t *d_Rptr = nullptr, *d_Rcol = nullptr, *d_Rrowblk = nullptr;
   double* d_Rval = nullptr;
@@ -66,6 +70,10 @@ struct doug_ctx {
   int nsub = 0;
   BlockDesc* d_sub = nullptr;
   std::vector<BlockDesc> h_sub;
+  int trsv_order = 0;               // 1: largest subdomain dispatched first
+  int local_order = 1;              // 1: reverse Cuthill-McKee inside each subdomain
+  BlockDesc* d_sub_trsv = nullptr;  // descriptors in dispatch order
+  doug_tuning tuning{};
   double* d_tiles = nullptr;   // FP64 factors
   float* d_tiles32 = nullptr;  // FP32 factors (DOUG_PREC_FP32_FACTORS)
   int precision = DOUG_PREC_FP64;
@@ -88,9 +96,7 @@ struct doug_ctx {
   bool batch_ready = false;
   int maxnt = 0;
   int64_t work_rows = 0;
-  double* d_W64 = nullptr;
-  float* d_W32 = nullptr;
-  void* d_G = nullptr;
+
   void* d_U = nullptr;
   double *d_cG = nullptr, *d_cU = nullptr, *d_cS = nullptr, *d_subsolm = nullptr;
   double *d_Xm = nullptr, *d_Rm = nullptr, *d_Zm = nullptr, *d_Pm = nullptr, *d_Qm = nullptr;
@@ -139,11 +145,15 @@ static int upload(doug_ctx* ctx, T** out, const std::vector<T>& h) {
   return DOUG_OK;
 }
 
-static void free_device(doug_ctx* ctx) {
+static void drop_iteration_graph(doug_ctx* ctx) {
   if (ctx->iter_graph) {
     (void)hipGraphExecDestroy(ctx->iter_graph);
     ctx->iter_graph = nullptr;
   }
+}
+
+static void free_device(doug_ctx* ctx) {
+  drop_iteration_graph(ctx);
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->ready = false;
@@ -152,18 +162,19 @@ static void free_device(doug_ctx* ctx) {
 
 // CSR-stream row blocks: consecutive rows with at most kSpmvThreads rows and
 // kSpmvChunk nonzeros; a longer row gets a block of its own.
-static std::vector<int32_t> row_blocks(const Csr& M) {
+static std::vector<int32_t> row_blocks(const Csr& M, int rows = kSpmvThreads) {
+  const int64_t chunk = 8 * (int64_t)rows;
   std::vector<int32_t> rb{0};
   int r = 0;
   while (r < M.nrows) {
     int r1 = r;
     int64_t nz = 0;
-    while (r1 < M.nrows && r1 - r < kSpmvThreads) {
+    while (r1 < M.nrows && r1 - r < rows) {
       int64_t rn = M.ptr[r1 + 1] - M.ptr[r1];
-      if (nz + rn > kSpmvChunk && r1 > r) break;
+      if (nz + rn > chunk && r1 > r) break;
       nz += rn;
       r1++;
-      if (nz > kSpmvChunk) break;
+      if (nz > chunk) break;
     }
     rb.push_back(r1);
     r = r1;
@@ -171,10 +182,11 @@ static std::vector<int32_t> row_blocks(const Csr& M) {
   return rb;
 }
 
-static int64_t tile_count(int nt) { return (int64_t)nt * (nt + 1) / 2; }
-static int64_t tile_pos(int li, int lj) {  // element offset of (li, lj), li >= lj, in a block
-  int ti = li / kTile, tj = lj / kTile;
-  return (tile_count(ti) + tj) * kTileElems + (int64_t)(li % kTile) * kTile + (lj % kTile);
+// element offset of (li, lj), li >= lj inside the envelope, from a block's base
+static int64_t tile_pos(const TileRow* tr, int li, int lj) {
+  const int ti = li / kTile, tj = lj / kTile;
+  return (int64_t)(tr[ti].off + tj - tr[ti].fc) * kTileElems + (int64_t)(li % kTile) * kTile +
+         (lj % kTile);
 }
 
 // ---------------------------------------------------------------------------
@@ -320,17 +332,22 @@ int doug_set_partition(doug_ctx* ctx, int32_t nparts, const int32_t* starts, con
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Dense tile storage for a set of index blocks: descriptors, scatter map and
-// factorisation on the device.  Everything is enqueued on the context stream
-// without waiting, so that host setup work can overlap the factorisation.
+// Tile storage for a set of index blocks: descriptors, tile-row tables,
+// scatter map and factorisation on the device.  With `envelope` each block
+// keeps only the tiles inside the lower envelope of M(inds, inds) (its row
+// order is the order of `inds`); otherwise every lower tile.  `offcount[j]`
+// receives max over blocks of lr_j - j.  Everything is enqueued on the
+// context stream, so that host setup work can overlap the factorisation.
 template <typename T>
 static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
                         const std::vector<int>& starts, const std::vector<int>& inds,
-                        bool one_block_solve, std::vector<BlockDesc>& desc, BlockDesc** d_desc,
-                        T** d_tiles, int64_t* total_elems) {
+                        bool one_block_solve, bool envelope, std::vector<BlockDesc>& desc,
+                        BlockDesc** d_desc, T** d_tiles, int64_t* total_elems,
+                        std::vector<int>& offcount, double* chol_flops = nullptr) {
   const int nb = (int)starts.size() - 1;
   desc.resize((size_t)nb);
-  int64_t base = 0, woff = 0;
+  std::vector<int64_t> row0((size_t)nb + 1, 0);
+  int ntmax = 0;
   for (int b = 0; b < nb; b++) {
     int s = starts[b + 1] - starts[b];
     int nt = (s + kTile - 1) / kTile;
@@ -338,46 +355,103 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
       return fail(ctx, DOUG_E_UNSUPPORTED,
                   "subdomain of " + std::to_string(s) + " unknowns exceeds the one-block solve limit " +
                       std::to_string(kMaxTrsvTiles * kTile));
-    desc[b] = BlockDesc{base, (int64_t)starts[b], woff, nt, s};
-    base += tile_count(nt) * kTileElems;
-    woff += (int64_t)nt * kTile;
+    row0[b + 1] = row0[b] + nt;
+    ntmax = std::max(ntmax, nt);
   }
-  *total_elems = base;
-  // scatter map: lower-triangle entries of M(inds, inds), blocks in parallel
+  std::vector<TileRow> rows((size_t)row0[nb]);
   const int nth = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
-  std::vector<std::vector<int64_t>> dsts((size_t)nth);
-  std::vector<std::vector<int32_t>> srcs((size_t)nth);
-  std::vector<int> dup((size_t)nth, 0);
-  {
+  auto for_blocks = [&](auto&& body) {
     std::vector<std::thread> th;
     for (int t = 0; t < nth; t++)
       th.emplace_back([&, t]() {
         std::vector<int> floc((size_t)M.nrows, -1);
-        std::vector<int64_t> chk;
-        for (int b = t; b < nb; b += nth) {
-          const int s0 = starts[b], s = starts[b + 1] - s0;
-          for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
-          for (int li = 0; li < s; li++) {
-            int g = inds[s0 + li];
-            size_t row_begin = dsts[t].size();
-            for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
-              int lj = floc[M.col[p]];
-              if (lj < 0 || lj > li) continue;
-              dsts[t].push_back(desc[b].base + tile_pos(li, lj));
-              srcs[t].push_back(p);
-            }
-            // duplicate (i,j) pairs would race in the scatter
-            if (dsts[t].size() - row_begin > 1) {
-              chk.assign(dsts[t].begin() + row_begin, dsts[t].end());
-              std::sort(chk.begin(), chk.end());
-              if (std::adjacent_find(chk.begin(), chk.end()) != chk.end()) dup[t] = 1;
-            }
-          }
-          for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
-        }
+        for (int b = t; b < nb; b += nth) body(t, b, floc);
       });
     for (auto& x : th) x.join();
+  };
+  // tile-row envelopes: fc from the first lower entry of each row, made
+  // non-decreasing so that every column's tiles are contiguous
+  for_blocks([&](int, int b, std::vector<int>& floc) {
+    const int s0 = starts[b], s = starts[b + 1] - s0, nt = (s + kTile - 1) / kTile;
+    TileRow* tr = rows.data() + row0[b];
+    for (int i = 0; i < nt; i++) tr[i] = TileRow{0, envelope ? i : 0, i, 0};
+    if (envelope) {
+      for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
+      for (int li = 0; li < s; li++) {
+        int g = inds[s0 + li], f = li;
+        for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
+          int lj = floc[M.col[p]];
+          if (lj >= 0 && lj < f) f = lj;
+        }
+        TileRow& r = tr[li / kTile];
+        r.fc = std::min(r.fc, f / kTile);
+      }
+      for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
+      for (int i = nt - 2; i >= 0; i--) tr[i].fc = std::min(tr[i].fc, tr[i + 1].fc);
+    }
+    int off = 0;
+    for (int i = 0; i < nt; i++) {
+      tr[i].off = off;
+      off += i - tr[i].fc + 1;
+      for (int j = tr[i].fc; j <= i; j++) tr[j].lr = std::max(tr[j].lr, i);
+    }
+  });
+  int rc;
+  TileRow* d_rows;
+  if ((rc = upload(ctx, &d_rows, rows))) return rc;
+  int64_t base = 0, woff = 0;
+  offcount.assign((size_t)ntmax, 0);
+  for (int b = 0; b < nb; b++) {
+    const int s = starts[b + 1] - starts[b], nt = (int)(row0[b + 1] - row0[b]);
+    const TileRow* tr = rows.data() + row0[b];
+    desc[b] = BlockDesc{base, (int64_t)starts[b], woff, d_rows + row0[b], nt, s};
+    base += (int64_t)(tr[nt - 1].off + nt - tr[nt - 1].fc) * kTileElems;
+    woff += (int64_t)nt * kTile;
+    for (int j = 0; j < nt; j++) offcount[j] = std::max(offcount[j], tr[j].lr - j);
   }
+  *total_elems = base;
+  if (chol_flops) {
+    // matrix-core work of the tile factorisation: per step j, (j - fc_j)
+    // SYRK tiles, one 64x64 POTRF and inverse, and for each tile (i, j) below
+    // the diagonal (j - fc_i) GEMM tiles plus one triangular product
+    const double g = 2.0 * kTile * kTile * kTile;
+    double f = 0.0;
+    for (int b = 0; b < nb; b++) {
+      const TileRow* tr = rows.data() + row0[b];
+      for (int j = 0; j < desc[b].nt; j++) {
+        f += g * (j - tr[j].fc) + 2.0 * g / 3.0;
+        for (int i = j + 1; i <= tr[j].lr; i++) f += g * (j - tr[i].fc + 1);
+      }
+    }
+    *chol_flops = f;
+  }
+  // scatter map: lower-triangle entries of M(inds, inds), blocks in parallel
+  std::vector<std::vector<int64_t>> dsts((size_t)nth);
+  std::vector<std::vector<int32_t>> srcs((size_t)nth);
+  std::vector<int> dup((size_t)nth, 0);
+  for_blocks([&](int t, int b, std::vector<int>& floc) {
+    std::vector<int64_t> chk;
+    const TileRow* tr = rows.data() + row0[b];
+    const int s0 = starts[b], s = starts[b + 1] - s0;
+    for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
+    for (int li = 0; li < s; li++) {
+      int g = inds[s0 + li];
+      size_t row_begin = dsts[t].size();
+      for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
+        int lj = floc[M.col[p]];
+        if (lj < 0 || lj > li) continue;
+        dsts[t].push_back(desc[b].base + tile_pos(tr, li, lj));
+        srcs[t].push_back(p);
+      }
+      // duplicate (i,j) pairs would race in the scatter
+      if (dsts[t].size() - row_begin > 1) {
+        chk.assign(dsts[t].begin() + row_begin, dsts[t].end());
+        std::sort(chk.begin(), chk.end());
+        if (std::adjacent_find(chk.begin(), chk.end()) != chk.end()) dup[t] = 1;
+      }
+    }
+    for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
+  });
   for (int t = 0; t < nth; t++)
     if (dup[t]) return fail(ctx, DOUG_E_UNSUPPORTED, "duplicate matrix entries inside a subdomain are not supported");
   std::vector<int64_t> dst;
@@ -386,7 +460,6 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
     dst.insert(dst.end(), dsts[t].begin(), dsts[t].end());
     src.insert(src.end(), srcs[t].begin(), srcs[t].end());
   }
-  int rc;
   if ((rc = dalloc(ctx, d_tiles, (size_t)base))) return rc;
   if ((rc = upload(ctx, d_desc, desc))) return rc;
   int64_t* d_dst;
@@ -408,12 +481,11 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
 // pivot sets *flag (checked once the stream has drained).
 template <typename T>
 static int enqueue_factor(doug_ctx* ctx, T* tiles, const BlockDesc* d_desc,
-                          const std::vector<BlockDesc>& desc, int* flag) {
-  int ntmax = 0;
-  for (auto& d : desc) ntmax = std::max(ntmax, d.nt);
-  for (int j = 0; j < ntmax; j++) {
+                          const std::vector<BlockDesc>& desc, const std::vector<int>& offcount,
+                          int* flag) {
+  for (int j = 0; j < (int)offcount.size(); j++) {
     launch_chol_diag(tiles, d_desc, (int)desc.size(), j, flag, ctx->stream);
-    launch_chol_offdiag(tiles, d_desc, (int)desc.size(), j, ntmax - j - 1, ctx->stream);
+    launch_chol_offdiag(tiles, d_desc, (int)desc.size(), j, offcount[j], ctx->stream);
   }
   HIPCHK(hipGetLastError());
   return DOUG_OK;
@@ -550,6 +622,18 @@ static int setup_host(doug_ctx* ctx) {
   return DOUG_OK;
 }
 
+// Dispatch order of the subdomain solves: 0 as partitioned, 1 largest first
+// (longest-processing-time order, which shortens the tail when sizes vary).
+static int set_trsv_order(doug_ctx* ctx, int order) {
+  ctx->trsv_order = order;
+  ctx->d_sub_trsv = ctx->d_sub;
+  if (order == 0 || ctx->h_sub.empty()) return DOUG_OK;
+  std::vector<BlockDesc> sorted(ctx->h_sub);
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [](const BlockDesc& a, const BlockDesc& b) { return a.nt > b.nt; });
+  return upload(ctx, &ctx->d_sub_trsv, sorted);
+}
+
 // Matrix, vectors, subdomain data and the subdomain factorisation (enqueued).
 static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const Params& p = ctx->p;
@@ -565,16 +649,32 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   std::vector<int32_t> rb = row_blocks(A);
   ctx->nrb = (int)rb.size() - 1;
   if ((rc = upload(ctx, &ctx->d_rowblk, rb))) return rc;
+  ctx->d_rowblk_a = ctx->d_rowblk;
+  ctx->nrb_a = ctx->nrb;
+  if (spmv_shape_rows(ctx->spmv_shape) != kSpmvThreads) {
+    std::vector<int32_t> rba = row_blocks(A, spmv_shape_rows(ctx->spmv_shape));
+    ctx->nrb_a = (int)rba.size() - 1;
+    if ((rc = upload(ctx, &ctx->d_rowblk_a, rba))) return rc;
+  }
   if ((rc = upload(ctx, &ctx->d_src, A.src))) return rc;
   if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
   HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
-  // occurrence lists for the deterministic overlap sum
-  std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(ctx->sinds.size());
+  // device order of each subdomain's unknowns (the reported index sets keep
+  // the reference's order): reverse Cuthill-McKee narrows the factor's
+  // tile envelope
+  std::vector<int> dinds;
+  if (p.fine_method == 1 && ctx->local_order == 1)
+    block_rcm(A, ctx->sstarts, ctx->sinds, dinds);
+  else
+    dinds = ctx->sinds;
+  // occurrence lists for the deterministic overlap sum (a node's
+  // occurrences in ascending subdomain order)
+  std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(dinds.size());
   if (p.fine_method == 1) {
-    for (int v : ctx->sinds) occ_ptr[v + 1]++;
+    for (int v : dinds) occ_ptr[v + 1]++;
     for (int i = 0; i < n; i++) occ_ptr[i + 1] += occ_ptr[i];
     std::vector<int32_t> fill(occ_ptr.begin(), occ_ptr.end() - 1);
-    for (size_t q = 0; q < ctx->sinds.size(); q++) occ_pos[fill[ctx->sinds[q]]++] = (int32_t)q;
+    for (size_t q = 0; q < dinds.size(); q++) occ_pos[fill[dinds[q]]++] = (int32_t)q;
   } else {  // no fine solves: z starts as r (FinePreconditioner_Apply, type NONE)
     occ_pos.resize((size_t)n);
     for (int i = 0; i < n; i++) {
@@ -584,7 +684,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   }
   if ((rc = upload(ctx, &ctx->d_occ_ptr, occ_ptr)) || (rc = upload(ctx, &ctx->d_occ_pos, occ_pos)))
     return rc;
-  std::vector<int32_t> inds(ctx->sinds.begin(), ctx->sinds.end());
+  std::vector<int32_t> inds(dinds.begin(), dinds.end());
   if ((rc = upload(ctx, &ctx->d_inds, inds))) return rc;
   if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
   for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
@@ -600,25 +700,30 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
   int64_t elems = 0;
   size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
+  std::vector<int> offcount;
   if (p.fine_method == 1) {
     if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
-      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, ctx->sinds, true, ctx->h_sub,
-                             &ctx->d_sub, &ctx->d_tiles32, &elems)))
+      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, dinds, true, true, ctx->h_sub,
+                             &ctx->d_sub, &ctx->d_tiles32, &elems, offcount, &info.factor_flops)))
         return rc;
       HIPCHK(hipEventRecord(ev0, ctx->stream));
-      if ((rc = enqueue_factor(ctx, ctx->d_tiles32, ctx->d_sub, ctx->h_sub, ctx->d_flag))) return rc;
+      if ((rc = enqueue_factor(ctx, ctx->d_tiles32, ctx->d_sub, ctx->h_sub, offcount, ctx->d_flag)))
+        return rc;
     } else {
-      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, ctx->sinds, true, ctx->h_sub,
-                             &ctx->d_sub, &ctx->d_tiles, &elems)))
+      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, dinds, true, true, ctx->h_sub,
+                             &ctx->d_sub, &ctx->d_tiles, &elems, offcount, &info.factor_flops)))
         return rc;
       HIPCHK(hipEventRecord(ev0, ctx->stream));
-      if ((rc = enqueue_factor(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub, ctx->d_flag))) return rc;
+      if ((rc = enqueue_factor(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub, offcount, ctx->d_flag)))
+        return rc;
     }
   } else {
     HIPCHK(hipEventRecord(ev0, ctx->stream));
   }
   HIPCHK(hipEventRecord(ev1, ctx->stream));
   info.factor_bytes = elems * (int64_t)esize;
+  info.factor_tiles = elems / kTileElems;
+  if ((rc = set_trsv_order(ctx, ctx->trsv_order))) return rc;
   ctx->maxnt = 0;
   ctx->work_rows = 0;
   for (auto& d : ctx->h_sub) {
@@ -650,11 +755,14 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
     std::vector<int> cst{0, ctx->nc}, cinds((size_t)ctx->nc);
     for (int i = 0; i < ctx->nc; i++) cinds[i] = i;
     std::vector<BlockDesc> cdesc;
+    std::vector<int> coffcount;
     int64_t celems = 0;
-    if ((rc = build_blocks(ctx, ctx->AC, d_acval, cst, cinds, false, cdesc, &ctx->d_cdesc,
-                           &ctx->d_ctiles, &celems)))
+    // dense: the explicit inverse shares the factor's tile layout
+    if ((rc = build_blocks(ctx, ctx->AC, d_acval, cst, cinds, false, false, cdesc, &ctx->d_cdesc,
+                           &ctx->d_ctiles, &celems, coffcount)))
       return rc;
-    if ((rc = enqueue_factor(ctx, ctx->d_ctiles, ctx->d_cdesc, cdesc, ctx->d_flag + 1))) return rc;
+    if ((rc = enqueue_factor(ctx, ctx->d_ctiles, ctx->d_cdesc, cdesc, coffcount, ctx->d_flag + 1)))
+      return rc;
     ctx->cnt = cdesc[0].nt;
     if ((rc = dalloc(ctx, &ctx->d_cW, (size_t)celems))) return rc;
     HIPCHK(hipMemcpyAsync(ctx->d_cW, ctx->d_ctiles, (size_t)celems * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -783,14 +891,15 @@ int doug_get_coarse_matrix(const doug_ctx* ctx, int32_t* rowptr, int32_t* cols, 
 
 // ---------------------------------------------------------------------------
 static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
-  launch_spmv(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, x, y, ctx->stream);
+  launch_spmv(ctx->nrb_a, ctx->d_rowblk_a, ctx->d_ptr, ctx->d_col, ctx->d_val, x, y, ctx->stream,
+              ctx->spmv_shape);
 }
 
 static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
   if (ctx->precision == DOUG_PREC_FP32_FACTORS)
-    launch_trsv(ctx->d_tiles32, ctx->d_sub, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
   else
-    launch_trsv(ctx->d_tiles, ctx->d_sub, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    launch_trsv(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
 }
 
 // preconditioner (pcg.F90:216-270): fine solves, then coarse correction added
@@ -1055,30 +1164,17 @@ int doug_prepare_batch(doug_ctx* ctx) {
   hipStream_t st = ctx->stream;
   int rc;
   if (ctx->p.fine_method == 1) {
-    size_t elems = (size_t)(ctx->info.factor_bytes /
-                            (ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double)));
+    // panels of the envelope solves; the factors themselves are shared with
+    // the single right-hand-side path
     if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
-      if ((rc = dalloc(ctx, &ctx->d_W32, elems))) return rc;
-      HIPCHK(hipMemcpyAsync(ctx->d_W32, ctx->d_tiles32, elems * sizeof(float), hipMemcpyDeviceToDevice, st));
-      for (int dd = 1; dd < ctx->maxnt; dd++)
-        launch_trinv_batched(ctx->d_tiles32, ctx->d_W32, ctx->d_sub, ctx->nsub, ctx->maxnt, dd, st);
-      float *g, *u;
-      if ((rc = dalloc(ctx, &g, (size_t)ctx->work_rows * NR)) || (rc = dalloc(ctx, &u, (size_t)ctx->work_rows * NR)))
-        return rc;
-      ctx->d_G = g;
+      float* u;
+      if ((rc = dalloc(ctx, &u, (size_t)ctx->work_rows * NR))) return rc;
       ctx->d_U = u;
     } else {
-      if ((rc = dalloc(ctx, &ctx->d_W64, elems))) return rc;
-      HIPCHK(hipMemcpyAsync(ctx->d_W64, ctx->d_tiles, elems * sizeof(double), hipMemcpyDeviceToDevice, st));
-      for (int dd = 1; dd < ctx->maxnt; dd++)
-        launch_trinv_batched(ctx->d_tiles, ctx->d_W64, ctx->d_sub, ctx->nsub, ctx->maxnt, dd, st);
-      double *g, *u;
-      if ((rc = dalloc(ctx, &g, (size_t)ctx->work_rows * NR)) || (rc = dalloc(ctx, &u, (size_t)ctx->work_rows * NR)))
-        return rc;
-      ctx->d_G = g;
+      double* u;
+      if ((rc = dalloc(ctx, &u, (size_t)ctx->work_rows * NR))) return rc;
       ctx->d_U = u;
     }
-    HIPCHK(hipGetLastError());
   }
   if (ctx->has_coarse) {
     size_t cpad = (size_t)ctx->cnt * kTile * NR;
@@ -1118,17 +1214,12 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
   }
   const double* fine_src = r;
   if (p.fine_method == 1) {
-    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
-      float *G = (float*)ctx->d_G, *U = (float*)ctx->d_U;
-      launch_gather_multi(ctx->d_sub, ctx->nsub, ctx->maxnt, ctx->d_inds, r, nrp, NR, G, st);
-      launch_wgemm_fwd(ctx->d_W32, ctx->d_sub, ctx->nsub, ctx->maxnt, G, U, NR, st);
-      launch_wgemm_bwd(ctx->d_W32, ctx->d_sub, ctx->nsub, ctx->maxnt, U, ctx->d_subsolm, nrp, nrp, NR, st);
-    } else {
-      double *G = (double*)ctx->d_G, *U = (double*)ctx->d_U;
-      launch_gather_multi(ctx->d_sub, ctx->nsub, ctx->maxnt, ctx->d_inds, r, nrp, NR, G, st);
-      launch_wgemm_fwd(ctx->d_W64, ctx->d_sub, ctx->nsub, ctx->maxnt, G, U, NR, st);
-      launch_wgemm_bwd(ctx->d_W64, ctx->d_sub, ctx->nsub, ctx->maxnt, U, ctx->d_subsolm, nrp, nrp, NR, st);
-    }
+    if (ctx->precision == DOUG_PREC_FP32_FACTORS)
+      launch_trsm_multi(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, nrp,
+                        (float*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, st);
+    else
+      launch_trsm_multi(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, nrp,
+                        (double*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, st);
     fine_src = ctx->d_subsolm;
   }
   launch_overlap_sum_multi(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
@@ -1249,6 +1340,28 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
     case DOUG_OPT_ITERATION_GRAPH:
       ctx->use_graph = value != 0;
       return DOUG_OK;
+    case DOUG_OPT_SPMV_SHAPE: {
+      if (value < 0 || value >= kSpmvShapes) return fail(ctx, DOUG_E_ARG, "SpMV shape out of range");
+      ctx->spmv_shape = (int)value;
+      if (!ctx->ready) return DOUG_OK;  // applied at the next doug_setup
+      drop_iteration_graph(ctx);
+      std::vector<int32_t> rb = row_blocks(ctx->A, spmv_shape_rows(ctx->spmv_shape));
+      ctx->nrb_a = (int)rb.size() - 1;
+      return upload(ctx, &ctx->d_rowblk_a, rb);
+    }
+    case DOUG_OPT_LOCAL_ORDER:
+      if (value != 0 && value != 1) return fail(ctx, DOUG_E_ARG, "local order must be 0 or 1");
+      ctx->local_order = (int)value;
+      ctx->ready = false;  // takes effect at the next doug_setup
+      return DOUG_OK;
+    case DOUG_OPT_TRSV_ORDER:
+      if (value != 0 && value != 1) return fail(ctx, DOUG_E_ARG, "subdomain order must be 0 or 1");
+      if (!ctx->ready) {
+        ctx->trsv_order = (int)value;
+        return DOUG_OK;
+      }
+      drop_iteration_graph(ctx);
+      return set_trsv_order(ctx, (int)value);
     default:
       return fail(ctx, DOUG_E_ARG, "unknown option " + std::to_string(option));
   }
@@ -1297,6 +1410,95 @@ int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
   (void)hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   *ms = (double)t / reps;
+  return DOUG_OK;
+}
+
+}  // extern "C"
+
+// Average milliseconds of `reps` launches of f after one warm-up launch.
+template <typename F>
+static int time_launches(doug_ctx* ctx, int reps, F&& f, double* ms) {
+  hipStream_t st = ctx->stream;
+  f();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; r++) f();
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  *ms = (double)t / reps;
+  return DOUG_OK;
+}
+
+extern "C" {
+
+int doug_autotune(doug_ctx* ctx, int32_t reps, doug_tuning* out) {
+  if (!ctx) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_autotune before doug_setup");
+  if (reps <= 0) reps = 20;
+  int rc;
+  doug_tuning tu{};
+  // SpMV: every shape over row blocks built for its row count
+  int32_t* rbs[3] = {nullptr, nullptr, nullptr};
+  int nrbs[3] = {0, 0, 0};
+  for (int k = 0; k < 3; k++) {
+    std::vector<int32_t> rb = row_blocks(ctx->A, spmv_shape_rows(2 * k));
+    nrbs[k] = (int)rb.size() - 1;
+    if ((rc = upload(ctx, &rbs[k], rb))) return rc;
+  }
+  int best = kSpmvDefaultShape;
+  for (int s = 0; s < kSpmvShapes; s++) {
+    const int k = s >> 1;
+    rc = time_launches(ctx, reps, [&]() {
+      launch_spmv(nrbs[k], rbs[k], ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_p, ctx->d_q, ctx->stream, s);
+    }, &tu.spmv_ms[s]);
+    if (rc) return rc;
+    if (tu.spmv_ms[s] < tu.spmv_ms[best]) best = s;
+  }
+  ctx->spmv_shape = best;
+  ctx->d_rowblk_a = rbs[best >> 1];
+  ctx->nrb_a = nrbs[best >> 1];
+  // subdomain solves: as partitioned, then largest first when that differs
+  int order = 0;
+  if (ctx->p.fine_method == 1 && ctx->nsub > 0) {
+    const int rt = std::max(3, reps / 4);
+    if ((rc = set_trsv_order(ctx, 0)) ||
+        (rc = time_launches(ctx, rt, [&]() { enqueue_subdomain_solves(ctx, ctx->d_r); }, &tu.trsv_ms[0])))
+      return rc;
+    bool sorted = true;
+    for (int i = 1; i < ctx->nsub; i++) sorted = sorted && ctx->h_sub[i - 1].nt >= ctx->h_sub[i].nt;
+    if (!sorted) {
+      if ((rc = set_trsv_order(ctx, 1)) ||
+          (rc = time_launches(ctx, rt, [&]() { enqueue_subdomain_solves(ctx, ctx->d_r); }, &tu.trsv_ms[1])))
+        return rc;
+      order = tu.trsv_ms[1] < tu.trsv_ms[0] ? 1 : 0;
+    }
+    if ((rc = set_trsv_order(ctx, order))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  tu.spmv_shape = best;
+  tu.spmv_rows = spmv_shape_rows(best);
+  tu.spmv_xcd_order = best & 1;
+  tu.trsv_order = order;
+  ctx->tuning = tu;
+  drop_iteration_graph(ctx);  // its launches were recorded with the old choices
+  if (out) *out = tu;
+  return DOUG_OK;
+}
+
+int doug_get_tuning(const doug_ctx* ctx, doug_tuning* out) {
+  if (!ctx || !out) return DOUG_E_ARG;
+  *out = ctx->tuning;
+  out->spmv_shape = ctx->spmv_shape;
+  out->spmv_rows = spmv_shape_rows(ctx->spmv_shape);
+  out->spmv_xcd_order = ctx->spmv_shape & 1;
+  out->trsv_order = ctx->trsv_order;
   return DOUG_OK;
 }
 

@@ -40,8 +40,10 @@ struct Mma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-__device__ __forceinline__ int64_t toff(int i, int k) {
-  return ((int64_t)i * (i + 1) / 2 + k) * kTileElems;
+// element offset of tile (i, k) from the block's base; k inside row i's envelope
+__device__ __forceinline__ int64_t toff(const BlockDesc& b, int i, int k) {
+  const TileRow r = b.tr[i];
+  return (int64_t)(r.off + k - r.fc) * kTileElems;
 }
 
 // 64x64 row-major global tile -> LDS (stride LD); 256 threads, 16 B loads.
@@ -146,7 +148,7 @@ __global__ void k_pad_identity(T* __restrict__ tiles, const BlockDesc* __restric
   int npad = b.nt * kTile - b.s;
   if (p >= npad || p >= maxpad) return;
   int lr = kTile - npad + p;  // local row inside the last diagonal tile
-  tiles[b.base + toff(b.nt - 1, b.nt - 1) + lr * kTile + lr] = (T)1;
+  tiles[b.base + toff(b, b.nt - 1, b.nt - 1) + lr * kTile + lr] = (T)1;
 }
 
 template <typename T>
@@ -159,11 +161,11 @@ __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
   const BlockDesc b = d[blockIdx.x];
   if (j >= b.nt) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  T* Cg = tiles + b.base + toff(j, j);
+  T* Cg = tiles + b.base + toff(b, j, j);
   V acc[2][2];
   zero_acc(acc);
-  for (int k = 0; k < j; k++) {
-    tile_to_lds(As, tiles + b.base + toff(j, k));
+  for (int k = b.tr[j].fc; k < j; k++) {
+    tile_to_lds(As, tiles + b.base + toff(b, j, k));
     __syncthreads();
     mma_abt<T>(acc, As, As, lane, wr, wc);
     __syncthreads();
@@ -224,15 +226,16 @@ __global__ __launch_bounds__(256) void k_chol_offdiag(T* __restrict__ tiles,
   __shared__ T Bs[kTile * LD];
   typedef typename Mma<T>::v4 V;
   const BlockDesc b = d[blockIdx.y];
+  if (j >= b.nt) return;
   const int i = j + 1 + blockIdx.x;
-  if (i >= b.nt) return;
+  if (i > b.tr[j].lr) return;  // tile (i, j) outside the envelope
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  T* Cg = tiles + b.base + toff(i, j);
+  T* Cg = tiles + b.base + toff(b, i, j);
   V acc[2][2];
   zero_acc(acc);
-  for (int k = 0; k < j; k++) {
-    tile_to_lds(As, tiles + b.base + toff(i, k));
-    tile_to_lds(Bs, tiles + b.base + toff(j, k));
+  for (int k = b.tr[i].fc; k < j; k++) {
+    tile_to_lds(As, tiles + b.base + toff(b, i, k));
+    tile_to_lds(Bs, tiles + b.base + toff(b, j, k));
     __syncthreads();
     mma_abt<T>(acc, As, Bs, lane, wr, wc);
     __syncthreads();
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256) void k_chol_offdiag(T* __restrict__ tiles,
         int row = acc_row<T>(wr, bi, lane, r), col = acc_col(wc, bj, lane);
         As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
       }
-  tile_to_lds(Bs, tiles + b.base + toff(j, j));  // inv(L_jj)
+  tile_to_lds(Bs, tiles + b.base + toff(b, j, j));  // inv(L_jj)
   __syncthreads();
   zero_acc(acc);
   mma_abt<T>(acc, As, Bs, lane, wr, wc);  // L_ij = C inv(L_jj)^T
@@ -272,8 +275,8 @@ __global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
   Mma<double>::v4 acc[2][2];
   zero_acc(acc);
   for (int k = j; k < i; k++) {
-    tile_to_lds(As, L + b.base + toff(i, k));
-    tile_to_lds(Bs, W + b.base + toff(k, j));
+    tile_to_lds(As, L + b.base + toff(b, i, k));
+    tile_to_lds(Bs, W + b.base + toff(b, k, j));
     __syncthreads();
     mma_ab<double>(acc, As, Bs, lane, wr, wc);
     __syncthreads();
@@ -285,11 +288,11 @@ __global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
 #pragma unroll
       for (int r = 0; r < 4; r++)
         Bs[acc_row<double>(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
-  tile_to_lds(As, L + b.base + toff(i, i));  // inv(L_ii)
+  tile_to_lds(As, L + b.base + toff(b, i, i));  // inv(L_ii)
   __syncthreads();
   zero_acc(acc);
   mma_ab<double>(acc, As, Bs, lane, wr, wc);
-  double* Wg = W + b.base + toff(i, j);
+  double* Wg = W + b.base + toff(b, i, j);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
@@ -361,8 +364,8 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
     T acc[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = (T)0;
-    for (int k = 0; k < i; k++) {
-      const T* Tk = T0 + toff(i, k) + (16 * w) * kTile + lane;
+    for (int k = b.tr[i].fc; k < i; k++) {
+      const T* Tk = T0 + toff(b, i, k) + (16 * w) * kTile + lane;
       const T yk = y[k * kTile + lane];
 #pragma unroll
       for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * yk;
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       tv[row] = y[i * kTile + row] - s;
     }
     __syncthreads();
-    const T* Ti = T0 + toff(i, i) + (16 * w) * kTile + lane;
+    const T* Ti = T0 + toff(b, i, i) + (16 * w) * kTile + lane;
     const T tl = tv[lane];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] = Ti[q * kTile] * tl;
@@ -384,8 +387,9 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
   // backward: x_j = inv(L_jj)^T (y_j - sum_{i>j} L_ij^T x_i)
   for (int j = nt - 1; j >= 0; j--) {
     T acc = (T)0;
-    for (int i = j + 1; i < nt; i++) {
-      const T* Ti = T0 + toff(i, j) + (16 * w) * kTile + lane;
+    const int lr = b.tr[j].lr;
+    for (int i = j + 1; i <= lr; i++) {
+      const T* Ti = T0 + toff(b, i, j) + (16 * w) * kTile + lane;
       const T* xi = y + i * kTile + 16 * w;
 #pragma unroll
       for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * xi[q];
@@ -396,7 +400,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       tv[lane] = y[j * kTile + lane] -
                  ((red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]));
     __syncthreads();
-    const T* Tj = T0 + toff(j, j) + (16 * w) * kTile + lane;
+    const T* Tj = T0 + toff(b, j, j) + (16 * w) * kTile + lane;
     acc = (T)0;
 #pragma unroll
     for (int q = 0; q < 16; q++) acc += Tj[q * kTile] * tv[16 * w + q];
@@ -423,7 +427,7 @@ __global__ __launch_bounds__(256) void k_gemv_w(const double* __restrict__ W,
 #pragma unroll
   for (int q = 0; q < 16; q++) acc[q] = 0.0;
   for (int k = 0; k <= i; k++) {
-    const double* Tk = W + b.base + toff(i, k) + (16 * w) * kTile + lane;
+    const double* Tk = W + b.base + toff(b, i, k) + (16 * w) * kTile + lane;
     const double ck = c[k * kTile + lane];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * ck;
@@ -444,7 +448,7 @@ __global__ __launch_bounds__(256) void k_gemv_wt(const double* __restrict__ W,
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double acc = 0.0;
   for (int i = j; i < b.nt; i++) {
-    const double* Ti = W + b.base + toff(i, j) + (16 * w) * kTile + lane;
+    const double* Ti = W + b.base + toff(b, i, j) + (16 * w) * kTile + lane;
     const double* ui = u + i * kTile + 16 * w;
 #pragma unroll
     for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * ui[q];
@@ -458,18 +462,33 @@ __global__ __launch_bounds__(256) void k_gemv_wt(const double* __restrict__ W,
 
 // CSR-stream SpMV: the block's nonzeros are multiplied with coalesced loads
 // into LDS, then each row is summed in stored order by one thread.
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv(const int32_t* __restrict__ rowblk,
-                                                       const int32_t* __restrict__ ptr,
-                                                       const int32_t* __restrict__ col,
-                                                       const double* __restrict__ val,
-                                                       const double* __restrict__ x,
-                                                       double* __restrict__ y) {
-  __shared__ double prod[kSpmvChunk];
-  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+// Row blocks are dispatched round-robin over the XCDs (block b runs on XCD
+// b % 8).  With SWZ the first 8*(nb/8) block ids are renumbered so that XCD k
+// takes the k-th contiguous eighth of the rows: a row's stencil neighbours
+// then mostly sit in the same XCD's L2.  Bijective on [0, nb).
+__device__ __forceinline__ int xcd_swizzle(int b, int nb) {
+  const int q = nb >> 3;
+  return b < (q << 3) ? (b & 7) * q + (b >> 3) : b;
+}
+
+// CSR-stream SpMV: the block's products val*x[col] are formed with coalesced
+// loads into LDS (8 per thread), then each thread sums one row in entry
+// order, so every shape gives the same bits.  TH = rows per block.
+template <int TH, bool SWZ>
+__global__ __launch_bounds__(TH) void k_spmv(const int32_t* __restrict__ rowblk,
+                                             const int32_t* __restrict__ ptr,
+                                             const int32_t* __restrict__ col,
+                                             const double* __restrict__ val,
+                                             const double* __restrict__ x,
+                                             double* __restrict__ y, int nb) {
+  constexpr int CH = 8 * TH;
+  __shared__ double prod[CH];
+  const int b = SWZ ? xcd_swizzle(blockIdx.x, nb) : (int)blockIdx.x;
+  const int r0 = rowblk[b], r1 = rowblk[b + 1];
   const int e0 = ptr[r0], e1 = ptr[r1];
   const int t = threadIdx.x;
-  if (e1 - e0 <= kSpmvChunk) {
-    for (int e = e0 + t; e < e1; e += kSpmvThreads) prod[e - e0] = val[e] * x[col[e]];
+  if (e1 - e0 <= CH) {
+    for (int e = e0 + t; e < e1; e += TH) prod[e - e0] = val[e] * x[col[e]];
     __syncthreads();
     const int row = r0 + t;
     if (row < r1) {
@@ -480,9 +499,9 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(const int32_t* __restrict
   } else {
     // a single long row, streamed through LDS chunk by chunk
     double acc = 0.0;
-    for (int c0 = e0; c0 < e1; c0 += kSpmvChunk) {
-      int c1 = min(e1, c0 + kSpmvChunk);
-      for (int e = c0 + t; e < c1; e += kSpmvThreads) prod[e - c0] = val[e] * x[col[e]];
+    for (int c0 = e0; c0 < e1; c0 += CH) {
+      int c1 = min(e1, c0 + CH);
+      for (int e = c0 + t; e < c1; e += TH) prod[e - c0] = val[e] * x[col[e]];
       __syncthreads();
       if (t == 0)
         for (int e = c0; e < c1; e++) acc = acc + prod[e - c0];
@@ -619,61 +638,6 @@ __global__ void k_residual(int n, const double* __restrict__ b, const double* __
 // ------------------------------------------------------------------------
 // Several right-hand sides: explicit inverse factors and batched GEMMs
 // ------------------------------------------------------------------------
-// W_{i,j} = -inv(L_ii) * sum_{k=j}^{i-1} L_ik W_kj, i = j + dd, every block
-template <typename T>
-__global__ __launch_bounds__(256) void k_trinv_batched(const T* __restrict__ L, T* __restrict__ W,
-                                                       const BlockDesc* __restrict__ d, int dd) {
-  __shared__ T As[kTile * LD];
-  __shared__ T Bs[kTile * LD];
-  typedef typename Mma<T>::v4 V;
-  const BlockDesc b = d[blockIdx.y];
-  const int j = blockIdx.x, i = j + dd;
-  if (i >= b.nt) return;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-  V acc[2][2];
-  zero_acc(acc);
-  for (int k = j; k < i; k++) {
-    tile_to_lds(As, L + b.base + toff(i, k));
-    tile_to_lds(Bs, W + b.base + toff(k, j));
-    __syncthreads();
-    mma_ab<T>(acc, As, Bs, lane, wr, wc);
-    __syncthreads();
-  }
-#pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        Bs[acc_row<T>(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
-  tile_to_lds(As, L + b.base + toff(i, i));  // inv(L_ii)
-  __syncthreads();
-  zero_acc(acc);
-  mma_ab<T>(acc, As, Bs, lane, wr, wc);
-  T* Wg = W + b.base + toff(i, j);
-#pragma unroll
-  for (int bi = 0; bi < 2; bi++)
-#pragma unroll
-    for (int bj = 0; bj < 2; bj++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        Wg[acc_row<T>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = -acc[bi][bj][r];
-}
-
-template <typename T>
-__global__ void k_gather_multi(const BlockDesc* __restrict__ d, const int32_t* __restrict__ inds,
-                               const double* __restrict__ rhs, int nrhs, int NR,
-                               T* __restrict__ G) {
-  const BlockDesc b = d[blockIdx.y];
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)b.nt * kTile * NR;
-  if (e >= total) return;
-  const int p = (int)(e / NR), r = (int)(e - (int64_t)p * NR);
-  T v = (T)0;
-  if (p < b.s && r < nrhs) v = (T)rhs[(int64_t)inds[b.vec_off + p] * nrhs + r];
-  G[(b.woff + p) * NR + r] = v;
-}
-
 // 64 x NR block of a workspace -> LDS (stride NR+1)
 template <typename T, int NR>
 __device__ __forceinline__ void panel_to_lds(T* __restrict__ S, const T* __restrict__ g) {
@@ -696,7 +660,7 @@ __global__ __launch_bounds__(256) void k_wgemm_fwd(const T* __restrict__ W,
 #pragma unroll
   for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
   for (int k = 0; k <= i; k++) {
-    tile_to_lds(As, W + b.base + toff(i, k));
+    tile_to_lds(As, W + b.base + toff(b, i, k));
     panel_to_lds<T, NR>(Bs, G + (b.woff + (int64_t)k * kTile) * NR);
     __syncthreads();
 #pragma unroll 4
@@ -732,7 +696,7 @@ __global__ __launch_bounds__(256) void k_wgemm_bwd(const T* __restrict__ W,
 #pragma unroll
   for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
   for (int i = j; i < b.nt; i++) {
-    tile_to_lds(As, W + b.base + toff(i, j));
+    tile_to_lds(As, W + b.base + toff(b, i, j));
     panel_to_lds<T, NR>(Bs, U + (b.woff + (int64_t)i * kTile) * NR);
     __syncthreads();
 #pragma unroll 4
@@ -752,6 +716,120 @@ __global__ __launch_bounds__(256) void k_wgemm_bwd(const T* __restrict__ W,
       const int col = c * 16 + li;
       if (p < b.s && col < nrhs) out[(b.vec_off + p) * ldo + col] = (double)acc[c][r];
     }
+}
+
+// Envelope forward/backward substitution for NR columns, one workgroup per
+// block.  Wave w owns rows 16w..16w+15 of each 64 x NR panel; panels of
+// finished tile rows live in U (global, re-read through L2); the workgroup's
+// barriers order its own global writes before its later reads.
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
+                                                    const BlockDesc* __restrict__ d,
+                                                    const int32_t* __restrict__ inds,
+                                                    const double* __restrict__ rhs, int ldr,
+                                                    T* __restrict__ U, double* __restrict__ out,
+                                                    int ldo, int nrhs) {
+  __shared__ T As[kTile * LD];
+  __shared__ T Bs[kTile * (NR + 1)];
+  typedef typename Mma<T>::v4 V;
+  const BlockDesc b = d[blockIdx.x];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  const T* Lb = L + b.base;
+  T* Ub = U + b.woff * NR;
+  V acc[NR / 16];
+  // forward: Y_i = inv(L_ii) (R_i - sum_{k=fc_i}^{i-1} L_ik Y_k)
+  for (int i = 0; i < b.nt; i++) {
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+    for (int k = b.tr[i].fc; k < i; k++) {
+      tile_to_lds(As, Lb + toff(b, i, k));
+      panel_to_lds<T, NR>(Bs, Ub + (int64_t)k * kTile * NR);
+      __syncthreads();
+#pragma unroll 4
+      for (int ks = 0; ks < 16; ks++) {
+        const int kk = 4 * ks + lk;
+        const T a = As[(16 * w + li) * LD + kk];
+#pragma unroll
+        for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
+        const int p = i * kTile + row;
+        T v = (T)0;
+        if (p < b.s && col < nrhs) v = (T)rhs[(int64_t)inds[b.vec_off + p] * ldr + col];
+        Bs[row * (NR + 1) + col] = v - acc[c][r];
+      }
+    tile_to_lds(As, Lb + toff(b, i, i));  // inv(L_ii)
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+#pragma unroll 4
+    for (int ks = 0; ks < 16; ks++) {
+      const int kk = 4 * ks + lk;
+      const T a = As[(16 * w + li) * LD + kk];
+#pragma unroll
+      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+    }
+    T* Ui = Ub + (int64_t)i * kTile * NR;
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) Ui[(16 * w + Mma<T>::row(lane, r)) * NR + c * 16 + li] = acc[c][r];
+    __syncthreads();
+  }
+  // backward: X_j = inv(L_jj)^T (Y_j - sum_{i=j+1}^{lr_j} L_ij^T X_i), X_j over Y_j
+  for (int j = b.nt - 1; j >= 0; j--) {
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+    const int lr = b.tr[j].lr;
+    for (int i = j + 1; i <= lr; i++) {
+      tile_to_lds(As, Lb + toff(b, i, j));
+      panel_to_lds<T, NR>(Bs, Ub + (int64_t)i * kTile * NR);
+      __syncthreads();
+#pragma unroll 4
+      for (int ks = 0; ks < 16; ks++) {
+        const int kk = 4 * ks + lk;
+        const T a = As[kk * LD + 16 * w + li];  // (L_ij)^T
+#pragma unroll
+        for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+      }
+      __syncthreads();
+    }
+    T* Uj = Ub + (int64_t)j * kTile * NR;
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
+        Bs[row * (NR + 1) + col] = Uj[row * NR + col] - acc[c][r];
+      }
+    tile_to_lds(As, Lb + toff(b, j, j));  // inv(L_jj)
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+#pragma unroll 4
+    for (int ks = 0; ks < 16; ks++) {
+      const int kk = 4 * ks + lk;
+      const T a = As[kk * LD + 16 * w + li];  // inv(L_jj)^T
+#pragma unroll
+      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
+        Uj[row * NR + col] = acc[c][r];
+        const int p = j * kTile + row;
+        if (p < b.s && col < nrhs) out[(b.vec_off + p) * ldo + col] = (double)acc[c][r];
+      }
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv_multi(
@@ -981,9 +1059,17 @@ void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, i
 }
 
 void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
-                 const double* val, const double* x, double* y, hipStream_t st) {
+                 const double* val, const double* x, double* y, hipStream_t st, int shape) {
   if (nrb <= 0) return;
-  hipLaunchKernelGGL(k_spmv, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x, y);
+  const dim3 g(nrb);
+  switch (shape) {
+    case 0: hipLaunchKernelGGL((k_spmv<128, false>), g, dim3(128), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
+    case 1: hipLaunchKernelGGL((k_spmv<128, true>), g, dim3(128), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
+    case 3: hipLaunchKernelGGL((k_spmv<256, true>), g, dim3(256), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
+    case 4: hipLaunchKernelGGL((k_spmv<512, false>), g, dim3(512), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
+    case 5: hipLaunchKernelGGL((k_spmv<512, true>), g, dim3(512), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
+    default: hipLaunchKernelGGL((k_spmv<256, false>), g, dim3(256), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
+  }
 }
 
 void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
@@ -1029,17 +1115,16 @@ void launch_residual(int n, const double* b, const double* ax, double* r, hipStr
 }
 
 template <typename T>
-void launch_trinv_batched(const T* L, T* W, const BlockDesc* d, int nblocks, int maxnt, int dd,
-                          hipStream_t st) {
-  if (maxnt - dd <= 0 || nblocks <= 0) return;
-  hipLaunchKernelGGL(k_trinv_batched<T>, dim3(maxnt - dd, nblocks), dim3(256), 0, st, L, W, d, dd);
-}
-
-template <typename T>
-void launch_gather_multi(const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
-                         const double* rhs, int nrhs, int NR, T* G, hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_multi<T>, dim3(cdiv((int64_t)maxnt * kTile * NR, 256), nblocks),
-                     dim3(256), 0, st, d, inds, rhs, nrhs, NR, G);
+void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
+                       const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
+                       hipStream_t st) {
+  if (nblocks <= 0) return;
+  if (NR == 16)
+    hipLaunchKernelGGL((k_trsm_multi<T, 16>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs, ldr,
+                       U, out, ldo, nrhs);
+  else
+    hipLaunchKernelGGL((k_trsm_multi<T, 32>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs, ldr,
+                       U, out, ldo, nrhs);
 }
 
 template <typename T>
@@ -1062,17 +1147,15 @@ void launch_wgemm_bwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, co
                        ldo, nrhs);
 }
 
-#define DOUG_INSTANTIATE_MULTI(T)                                                                \
-  template void launch_trinv_batched<T>(const T*, T*, const BlockDesc*, int, int, int,           \
-                                        hipStream_t);                                            \
-  template void launch_gather_multi<T>(const BlockDesc*, int, int, const int32_t*, const double*, \
-                                       int, int, T*, hipStream_t);                               \
-  template void launch_wgemm_fwd<T>(const T*, const BlockDesc*, int, int, const T*, T*, int,     \
-                                    hipStream_t);                                                \
-  template void launch_wgemm_bwd<T>(const T*, const BlockDesc*, int, int, const T*, double*, int, \
-                                    int, int, hipStream_t);
+#define DOUG_INSTANTIATE_MULTI(T)                                                              \
+  template void launch_trsm_multi<T>(const T*, const BlockDesc*, int, const int32_t*,          \
+                                     const double*, int, T*, double*, int, int, int, hipStream_t);
 DOUG_INSTANTIATE_MULTI(double)
 DOUG_INSTANTIATE_MULTI(float)
+template void launch_wgemm_fwd<double>(const double*, const BlockDesc*, int, int, const double*,
+                                       double*, int, hipStream_t);
+template void launch_wgemm_bwd<double>(const double*, const BlockDesc*, int, int, const double*,
+                                       double*, int, int, int, hipStream_t);
 #undef DOUG_INSTANTIATE_MULTI
 
 void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,

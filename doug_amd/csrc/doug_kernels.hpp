@@ -13,16 +13,28 @@ constexpr int kMaxTrsvTiles = 128;     // subdomain size limit of the one-block 
 constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reductions
 constexpr int kMaxRhs = 32;            // right-hand sides per batched solve
 
-// Dense subdomain block in tile-packed lower storage: tile (i,k), i>=k, of
-// block b starts at base + (i*(i+1)/2 + k) * kTileElems; each tile is 64x64
-// row-major.  After factorisation the off-diagonal tiles hold L and the
-// diagonal tiles hold inv(L_ii).
+// Tile row i of a block's lower envelope: tiles (i, fc..i) are stored, the
+// ones left of fc are zero.  fc never decreases with i, so the tiles of
+// column j are rows j..lr.  Cholesky fill stays inside a row envelope, so
+// the factor needs no other tiles.
+struct TileRow {
+  int32_t off;  // tile index of (i, fc) within the block's storage
+  int32_t fc;   // first stored tile column
+  int32_t lr;   // last tile row whose envelope reaches column i
+  int32_t pad;
+};
+
+// Symmetric block in tile-envelope lower storage: tile (i,k), fc_i <= k <= i,
+// starts at base + (tr[i].off + k - tr[i].fc) * kTileElems; each tile is
+// 64x64 row-major.  After factorisation the off-diagonal tiles hold L and the
+// diagonal tiles hold inv(L_ii).  A dense block has fc = 0 on every row.
 struct BlockDesc {
-  int64_t base;     // element offset of the block's tile storage
-  int64_t vec_off;  // offset into the concatenated subdomain vectors
-  int64_t woff;     // first row of the block in the padded multi-vector workspaces
-  int32_t nt;       // tiles per dimension
-  int32_t s;        // true size (rows), <= nt*64
+  int64_t base;        // element offset of the block's tile storage
+  int64_t vec_off;     // offset into the concatenated subdomain vectors
+  int64_t woff;        // first row of the block in the padded multi-vector workspaces
+  const TileRow* tr;   // device pointer to the block's nt tile rows
+  int32_t nt;          // tiles per dimension
+  int32_t s;           // true size (rows), <= nt*64
 };
 
 // ---- dense batched factorisation (T = double, or float for FP32 factors) ----
@@ -35,14 +47,15 @@ template <typename T>
 void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, hipStream_t st);
 // Left-looking tile Cholesky, step j: diagonal tiles (SYRK + POTRF + inverse)
 // then the tiles below them (GEMM + TRSM), on the matrix cores in T.  `flag`
-// receives the first failing block+1 on a non-positive pivot.
+// receives the first failing block+1 on a non-positive pivot.  `maxcount`
+// bounds lr_j - j over the blocks.
 template <typename T>
 void launch_chol_diag(T* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
                       hipStream_t st);
 template <typename T>
 void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
                          hipStream_t st);
-// Explicit inverse of a tile-packed lower factor (single block d[0]):
+// Explicit inverse of a dense tile-packed lower factor (single block d[0]):
 // W = L^{-1}, tiles at diagonal offset `dd` (W_{j+dd,j} for all j); W must
 // start as a copy of L so that its diagonal tiles hold inv(L_jj).
 void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, int dd,
@@ -51,8 +64,15 @@ void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, i
 // ---- per-iteration operators ----
 // y = A x, CSR-stream: each row summed in stored order, products rounded
 // before the add (bit-identical to a sequential row loop without FMA).
+// `shape` (0..kSpmvShapes-1) picks rows per block spmv_shape_rows(shape)
+// and, for odd shapes, the XCD-contiguous block order; the row blocks must
+// have been built for that row count (at most 8 entries per row on average).
+constexpr int kSpmvShapes = 6;
+constexpr int kSpmvDefaultShape = 2;
+constexpr int spmv_shape_rows(int shape) { return 128 << (shape >> 1); }
 void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
-                 const double* val, const double* x, double* y, hipStream_t st);
+                 const double* val, const double* x, double* y, hipStream_t st,
+                 int shape = kSpmvDefaultShape);
 // Batched forward/backward substitution, one block per subdomain: gathers
 // r(inds), solves L L^T u = r in the factor's precision, writes u to sol
 // (FP64, concatenated layout).
@@ -85,16 +105,16 @@ void launch_residual(int n, const double* b, const double* ax, double* r, hipStr
 // Multi-vectors are node-major: entry (i, r) of an n x nrhs block sits at
 // i*ld + r.  Subdomain workspaces hold nt*64 rows of NR (16 or 32) columns
 // per block, starting at row woff.
-// W = L^{-1} for every block (grid.y = block), tiles at diagonal offset dd;
-// W must start as a copy of L.
+// Forward then backward substitution with the envelope factor of every
+// block for NR (16 or 32) columns at once, one workgroup per block, tile
+// products on the matrix cores: U(block rows) <- L^{-T} L^{-1} rhs(inds, :),
+// then out[(vec_off + p) * ldo + r] = U for p < s, r < nrhs.  rhs has
+// leading dimension ldr; U is T workspace of nt*64 rows of NR from woff.
 template <typename T>
-void launch_trinv_batched(const T* L, T* W, const BlockDesc* d, int nblocks, int maxnt, int dd,
-                          hipStream_t st);
-// G(block rows, :) = rhs(inds, :) converted to T, zero padded
-template <typename T>
-void launch_gather_multi(const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
-                         const double* rhs, int nrhs, int NR, T* G, hipStream_t st);
-// U_i = sum_{k<=i} W_ik G_k for every block and tile row i
+void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
+                       const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
+                       hipStream_t st);
+// Dense explicit inverse (coarse block): U_i = sum_{k<=i} W_ik G_k
 template <typename T>
 void launch_wgemm_fwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* G, T* U,
                       int NR, hipStream_t st);

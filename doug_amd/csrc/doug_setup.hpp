@@ -88,4 +88,10 @@ void coarse_partition(const Aggregates& cfull, const Aggregates& ffull, int n,
 void add_layers_all(const Csr& A, const std::vector<int>& pstarts, const std::vector<int>& pnodes,
                     int ol, std::vector<int>& sstarts, std::vector<int>& sinds);
 
+// Reverse Cuthill-McKee order of each block of `inds` over the graph of A
+// restricted to the block: out holds the same blocks, each a permutation of
+// its input.  Independent of the thread count.
+void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
+               std::vector<int>& out);
+
 }  // namespace doug

@@ -15,6 +15,7 @@ program doug_aggr
   type(ctl_values) :: c
   type(doug_params) :: p
   type(doug_info) :: info
+  type(doug_tuning) :: tuning
   type(c_ptr) :: ctx
   integer(c_int32_t), allocatable :: indi(:), indj(:), pstarts(:), pnodes(:)
   real(c_double), allocatable :: val(:), b(:), x(:), hist(:)
@@ -111,8 +112,15 @@ program doug_aggr
   end if
 
   call cpu_time(t0)
-  call check(doug_setup(ctx, DOUG_PREC_FP64))
+  if (c%factor_precision /= 0 .and. c%factor_precision /= 1) &
+    call fatal('factor_precision must be 0 or 1')
+  call check(doug_setup(ctx, int(c%factor_precision, c_int32_t)))
   call check(doug_get_info(ctx, info))
+  if (c%autotune) then
+    call check(doug_autotune(ctx, 20_c_int32_t, tuning))
+    write(*, '(a,i0,a,i0,a,i0)') ' autotune: SpMV rows/workgroup ', tuning%spmv_rows, &
+      ', XCD order ', tuning%spmv_xcd_order, ', subdomain order ', tuning%trsv_order
+  end if
   write(*, *) '# fine aggregates:', info%nagr_fine
   write(*, *) '# coarse aggregates:', info%nagr_coarse
   write(*, *) '# subdomains:', info%nsub, ' overlap:', info%overlap

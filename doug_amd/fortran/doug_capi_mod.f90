@@ -3,7 +3,7 @@ module doug_capi
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: DOUG_OK = 0, DOUG_PREC_FP64 = 0
+  integer(c_int), parameter :: DOUG_OK = 0, DOUG_PREC_FP64 = 0, DOUG_PREC_FP32_FACTORS = 1
 
   !> doug_params (field order and meaning as in include/doug.h)
   type, bind(C) :: doug_params
@@ -19,7 +19,14 @@ module doug_capi
     integer(c_int64_t) :: nnz, sub_total, factor_bytes, coarse_factor_bytes
     real(c_double) :: t_aggregate, t_coarse_build, t_subdomains, t_upload, t_factor
     real(c_double) :: t_coarse_factor, t_total
+    integer(c_int64_t) :: factor_tiles
+    real(c_double) :: factor_flops
   end type doug_info
+
+  type, bind(C) :: doug_tuning
+    integer(c_int32_t) :: spmv_shape, spmv_rows, spmv_xcd_order, trsv_order
+    real(c_double) :: spmv_ms(6), trsv_ms(2)
+  end type doug_tuning
 
   interface
     function doug_ctx_create(device, ctx) bind(C, name="doug_ctx_create") result(rc)
@@ -84,6 +91,14 @@ module doug_capi
       import :: c_int, c_ptr, doug_info
       type(c_ptr), value :: ctx
       type(doug_info) :: info
+      integer(c_int) :: rc
+    end function
+
+    function doug_autotune(ctx, reps, tu) bind(C, name="doug_autotune") result(rc)
+      import :: c_int, c_int32_t, c_ptr, doug_tuning
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: reps
+      type(doug_tuning) :: tu
       integer(c_int) :: rc
     end function
 

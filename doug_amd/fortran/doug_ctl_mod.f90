@@ -22,10 +22,14 @@ module doug_ctl
     ! extension of this build: a non-overlapping partition of the unknowns
     ! (text: nparts, then nparts+1 starts, then the node list, 1-based)
     character(L) :: partition_file = ''
+    ! extensions of this build: factor precision of the subdomain solves
+    ! (0 FP64, 1 FP32 factors inside FP64 PCG) and the launch-shape autotuner
+    integer :: factor_precision = 0
+    logical :: autotune = .false.
   end type ctl_values
 
   ! every word the reference's control file accepts (src/utils/controls.F90)
-  character(len=24), parameter :: known_words(53) = [character(len=24) :: &
+  character(len=24), parameter :: known_words(55) = [character(len=24) :: &
     'solver', 'method', 'levels', 'overlap', 'smoothers', 'input_type', 'matrix_type', &
     'info_file', 'freedom_lists_file', 'elemmat_rhs_file', 'coords_file', 'freemap_file', &
     'freedom_mask_file', 'number_of_blocks', 'strong1', 'strong2', 'solve_tolerance', &
@@ -35,7 +39,8 @@ module doug_ctl
     'maxcie', 'maxnd', 'cutbal', 'center_type', 'hanging_nodes', 'interpolation_type', &
     'assembled_mtx_file', 'assembled_mtx_format', 'assembled_rhs_file', &
     'assembled_rhs_format', 'dump_matrix_only', 'dump_matrix_file', 'num_subdomains', &
-    'fine_method', 'num_iters', 'grid_size', 'coarse_method', 'partition_file', 'dummy']
+    'fine_method', 'num_iters', 'grid_size', 'coarse_method', 'partition_file', &
+    'factor_precision', 'autotune', 'dummy']
 
 contains
 
@@ -178,6 +183,8 @@ contains
       case ('assembled_rhs_file');   call set_string(c%assembled_rhs_file, w1, w2)
       case ('solution_file');        call set_string(c%solution_file, w1, w2)
       case ('partition_file');       call set_string(c%partition_file, w1, w2)
+      case ('factor_precision');     call set_int(c%factor_precision, w1, w2)
+      case ('autotune');             call set_logical(c%autotune, w1, w2)
       case ('assembled_mtx_format'); call set_int(c%assembled_mtx_format, w1, w2)
       case ('assembled_rhs_format'); call set_int(c%assembled_rhs_format, w1, w2)
       case ('solution_format');      call set_int(c%solution_format, w1, w2)

@@ -231,6 +231,18 @@ class Solver:
     def prec_batch_device(self, nrhs: int, dR: int, dZ: int):
         self._check(self._L.doug_prec_apply_batch_device(self.h, int(nrhs), C.c_void_p(dR), C.c_void_p(dZ)))
 
+    def autotune(self, reps: int = 20) -> dict:
+        """doug_autotune: time every SpMV launch shape and subdomain dispatch
+        order, keep the fastest; returns the measurements and the choice."""
+        tu = _lib.DougTuning()
+        self._check(self._L.doug_autotune(self.h, int(reps), C.byref(tu)))
+        return tu.as_dict()
+
+    def tuning(self) -> dict:
+        tu = _lib.DougTuning()
+        self._check(self._L.doug_get_tuning(self.h, C.byref(tu)))
+        return tu.as_dict()
+
     def time_operator(self, which: int, reps: int = 10) -> float:
         ms = C.c_double(0.0)
         self._check(self._L.doug_time_operator(self.h, int(which), int(reps), C.byref(ms)))

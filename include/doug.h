@@ -59,12 +59,17 @@ typedef struct {
   double omega;
 } doug_params;
 
-/* Sizes and timings of the last doug_setup (seconds). */
+/* Sizes and timings of the last doug_setup (seconds).  factor_bytes and
+ * factor_tiles count the stored subdomain factor tiles (64x64, inside each
+ * factor's envelope); factor_flops is the matrix-core work of their
+ * factorisation. */
 typedef struct {
   int32_t n, nagr_fine, nagr_coarse, nsub, ncoarse, overlap, smoothers;
   int64_t nnz, sub_total, factor_bytes, coarse_factor_bytes;
   double t_aggregate, t_coarse_build, t_subdomains, t_upload, t_factor, t_coarse_factor;
   double t_total;
+  int64_t factor_tiles;
+  double factor_flops;
 } doug_info;
 
 /* Context.  Replaces the process-wide state DOUG_Init sets up
@@ -149,10 +154,39 @@ int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int
  * to 32 right-hand sides at once (needs doug_prepare_batch). */
 int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose solve path
+ * has no launch shapes).  Times, with HIP events on the context's stream,
+ * every SpMV shape -- 128, 256 or 512 rows per workgroup, each with and
+ * without XCD-contiguous block order -- and the dispatch order of the
+ * subdomain solves (as partitioned, or largest first when sizes differ),
+ * `reps` launches each (<= 0: 20), and keeps the fastest.  All choices give
+ * bit-identical results.  The choice stays with the context across later
+ * doug_setup calls.  `out` may be NULL. */
+typedef struct doug_tuning {
+  int32_t spmv_shape;     /* 0..5: rows = 128 << (shape / 2); odd = XCD order */
+  int32_t spmv_rows;      /* rows per workgroup of the chosen shape */
+  int32_t spmv_xcd_order; /* 1: consecutive row blocks kept on one XCD */
+  int32_t trsv_order;     /* 0 as partitioned, 1 largest subdomain first */
+  double spmv_ms[6];      /* measured average per shape */
+  double trsv_ms[2];      /* measured per order; 0 where not timed */
+} doug_tuning;
+int doug_autotune(doug_ctx *ctx, int32_t reps, doug_tuning *out);
+int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
+
 /* Execution options.  DOUG_OPT_ITERATION_GRAPH (default 1): from the second
  * PCG iteration on, replay the iteration's kernels (preconditioner, dots,
- * updates, SpMV, scalar read-back) from one captured HIP graph. */
-enum { DOUG_OPT_ITERATION_GRAPH = 1 };
+ * updates, SpMV, scalar read-back) from one captured HIP graph.
+ * DOUG_OPT_SPMV_SHAPE (0..5) and DOUG_OPT_TRSV_ORDER (0/1) set what
+ * doug_autotune would choose, see doug_tuning.  DOUG_OPT_LOCAL_ORDER
+ * (default 1) places each subdomain's unknowns in its factor in reverse
+ * Cuthill-McKee order, which narrows the stored tile envelope; 0 keeps the
+ * index-set order.  It needs a new doug_setup and changes rounding only. */
+enum {
+  DOUG_OPT_ITERATION_GRAPH = 1,
+  DOUG_OPT_SPMV_SHAPE = 2,
+  DOUG_OPT_TRSV_ORDER = 3,
+  DOUG_OPT_LOCAL_ORDER = 4
+};
 int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 
 /* Adjoint of the solve with respect to the matrix values.  For a loss L with

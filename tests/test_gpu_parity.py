@@ -230,3 +230,29 @@ def test_iteration_graph_replays_match_eager():
         assert ite == itg
         assert np.array_equal(xe.view(np.int64), xg.view(np.int64))
         assert np.array_equal(he, hg)
+
+
+def test_local_order_narrows_envelope_rounding_only():
+    # reverse Cuthill-McKee inside each subdomain stores fewer factor tiles;
+    # both device orders solve the same system as the oracle
+    N = 20
+    n, i, j, v = P.diffusion3d(N, 1.0, 9)
+    part = P.block_partition((N,) * 3, 10)
+    b = P.uniform_rhs(n, 10)
+    print(f"local order n={n}", flush=True)
+    o = O.Oracle(n, i, j, v, O.params(**CFG0_CTL), partition=part)
+    xo, ito, _, _ = o.pcg(b, 1e-10, 500)
+    tiles = {}
+    for order in (0, 1):
+        with D.Solver(0) as s:
+            s.set_option(4, order)
+            s.set_params(**CFG0_CTL)
+            s.set_matrix(n, i, j, v, 1)
+            s.set_partition(*part)
+            s.setup()
+            tiles[order] = s.info()["factor_tiles"]
+            x, it, _, _ = s.pcg(b, 1e-10, 500)
+            assert it == ito, order
+            assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo), order
+    print(f"factor tiles: index order {tiles[0]}, reverse Cuthill-McKee {tiles[1]}")
+    assert tiles[1] < tiles[0]
