@@ -42,6 +42,8 @@ struct doug_ctx {
   bool have_matrix = false, ready = false, host_ready = false;
   Csr A;                // values after the scale/unscale round trip once set up
   std::vector<double> A_input_val;
+  std::vector<double> scaled;  // strength-test values (kept to reuse the allocation)
+  std::vector<uint8_t> strong;
   std::vector<int> part_starts, part_nodes;
   bool have_partition = false;
   ProcessState ps;
@@ -50,6 +52,8 @@ struct doug_ctx {
   int ol = 0, smoothers = 0;
   std::vector<int> sstarts, sinds;
   bool has_coarse = false;
+  bool coarse_on_device = false;   // R, R^T built on the device (not uploaded)
+  bool use_device_coarse = true;   // DOUG_OPT_DEVICE_COARSE
 
   // device: matrix
   std::vector<void*> allocs;
@@ -499,38 +503,212 @@ extern "C" {
 static void phase_values(doug_ctx* ctx, std::vector<double>& scaled, std::vector<uint8_t>& strong) {
   const Params& p = ctx->p;
   Csr& A = ctx->A;
-  A.val = ctx->A_input_val;
   const bool symm = p.symmstruct && p.symmnumeric;
   const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
   std::vector<double> diag;
-  scale_values(A, symm, scaled, diag);
+  // the reference continues with the values after the scale/unscale round trip
+  scale_round_trip(A, ctx->A_input_val, symm, scaled, diag, A.val);
   strong = find_strong(A, scaled, diag, strong_conn1);
-  A.val = scaled;
-  unscale_values(A, symm, diag, A.val);  // the reference continues with these values
 }
 
 // (2) fine aggregates, smoothed restriction and Galerkin coarse matrix
 //     (Partitioning_aggr.F90:33-117, CoarsePreconditioner_smooth.F90:47-89)
-static int phase_coarse(doug_ctx* ctx, const std::vector<double>& scaled,
-                        const std::vector<uint8_t>& strong) {
+static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
+                            const std::vector<uint8_t>& strong) {
   const Params& p = ctx->p;
-  doug_info& info = ctx->info;
-  Csr& A = ctx->A;
   double t0 = now();
   const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
   const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
   const int min_asize1 = p.minasize1 > 0 ? p.minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
-  aggregate(A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
+  aggregate(ctx->A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
             ctx->fin, ctx->ffull);
-  info.t_aggregate = now() - t0;
+  ctx->info.t_aggregate = now() - t0;
+}
+
+// host build of R, R^T and A_c (the host-only setup and smoothers = 0)
+static void phase_coarse_host(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
   double t1 = now();
-  ctx->R = restriction(A, strong, ctx->fin, ctx->smoothers, p.omega);
+  ctx->R = restriction(ctx->A, strong, ctx->fin, ctx->smoothers, ctx->p.omega);
   ctx->Rt = transpose(ctx->R);
-  ctx->AC = galerkin(A, ctx->R, ctx->Rt);
-  info.t_coarse_build = now() - t1;
+  ctx->AC = galerkin(ctx->A, ctx->R, ctx->Rt);
+  ctx->coarse_on_device = false;
+  ctx->info.t_coarse_build = now() - t1;
+}
+
+}  // extern "C"
+
+// Device build of the smoothed restriction and the Galerkin matrix (the
+// same bits as phase_coarse_host): R^T row by row, R by a stable sort of
+// R^T's entries on their aggregate, T' = A R^T and A_c = R T' by
+// first-touch products.  R and R^T stay on the device for the
+// preconditioner; R and A_c are copied to the host (getters, coarse factor).
+struct DevTemp {
+  std::vector<void*> ptrs;
+  ~DevTemp() {
+    for (void* q : ptrs) (void)hipFree(q);
+  }
+  template <typename T>
+  hipError_t alloc(T** q, size_t count) {
+    hipError_t e = hipMalloc((void**)q, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) ptrs.push_back((void*)*q);
+    return e;
+  }
+};
+
+static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
+  double t1 = now();
+  const Csr& A = ctx->A;
+  const int n = A.nrows, nagr = ctx->fin.nagr;
+  const int64_t nnz = A.nnz();
+  hipStream_t st = ctx->stream;
+  DevTemp tmp;
+  int rc;
+  void* scan_tmp = nullptr;
+  size_t scan_cap = 0;
+  // hipcub temporary storage, grown on demand
+  auto tmp_bytes = [&](size_t bytes) -> int {
+    if (bytes <= scan_cap) return DOUG_OK;
+    HIPCHK(tmp.alloc((char**)&scan_tmp, bytes));
+    scan_cap = bytes;
+    return DOUG_OK;
+  };
+  auto scan32 = [&](const int32_t* in, int32_t* out, int cnt) -> int {
+    size_t bytes = 0;
+    HIPCHK(exclusive_scan(nullptr, bytes, in, out, cnt, st));
+    if (int r = tmp_bytes(bytes)) return r;
+    HIPCHK(exclusive_scan(scan_tmp, bytes, in, out, cnt, st));
+    return DOUG_OK;
+  };
+  auto scan64 = [&](const int64_t* in, int64_t* out, int cnt) -> int {
+    size_t bytes = 0;
+    HIPCHK(exclusive_scan(nullptr, bytes, in, out, cnt, st));
+    if (int r = tmp_bytes(bytes)) return r;
+    HIPCHK(exclusive_scan(scan_tmp, bytes, in, out, cnt, st));
+    return DOUG_OK;
+  };
+  auto read_back = [&](const auto* dptr, auto& value) -> int {
+    HIPCHK(hipMemcpyAsync(&value, dptr, sizeof(value), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return DOUG_OK;
+  };
+  uint8_t* d_strong;
+  int32_t* d_num;
+  HIPCHK(tmp.alloc(&d_strong, (size_t)nnz));
+  HIPCHK(tmp.alloc(&d_num, (size_t)n));
+  HIPCHK(hipMemcpyAsync(d_strong, strong.data(), (size_t)nnz, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_num, ctx->fin.num.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  // R^T
+  int32_t *s_col, *cnt;
+  double* s_val;
+  HIPCHK(tmp.alloc(&s_col, (size_t)nnz));
+  HIPCHK(tmp.alloc(&s_val, (size_t)nnz));
+  HIPCHK(tmp.alloc(&cnt, (size_t)n + 1));
+  HIPCHK(hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), st));
+  launch_rt_rows(n, ctx->d_ptr, ctx->d_col, ctx->d_val, d_strong, d_num, ctx->p.omega, s_col, s_val,
+                 cnt, st);
+  if ((rc = dalloc(ctx, &ctx->d_Rtptr, (size_t)n + 1)) || (rc = scan32(cnt, ctx->d_Rtptr, n + 1)))
+    return rc;
+  int32_t nnz_rt = 0;
+  if ((rc = read_back(ctx->d_Rtptr + n, nnz_rt))) return rc;
+  if ((rc = dalloc(ctx, &ctx->d_Rtcol, (size_t)nnz_rt)) || (rc = dalloc(ctx, &ctx->d_Rtval, (size_t)nnz_rt)))
+    return rc;
+  launch_compact(n, ctx->d_ptr, ctx->d_Rtptr, s_col, s_val, ctx->d_Rtcol, ctx->d_Rtval, st);
+  // R = transpose(R^T), rows listing nodes in ascending order
+  int32_t *rowid, *iota, *keys, *order, *rcount;
+  HIPCHK(tmp.alloc(&rowid, (size_t)nnz_rt));
+  HIPCHK(tmp.alloc(&iota, (size_t)nnz_rt));
+  HIPCHK(tmp.alloc(&keys, (size_t)nnz_rt));
+  HIPCHK(tmp.alloc(&order, (size_t)nnz_rt));
+  HIPCHK(tmp.alloc(&rcount, (size_t)nagr + 1));
+  launch_row_ids(n, ctx->d_Rtptr, rowid, st);
+  launch_iota(nnz_rt, iota, st);
+  int end_bit = 1;
+  while (end_bit < 31 && (1 << end_bit) <= nagr) end_bit++;
+  {
+    size_t bytes = 0;
+    HIPCHK(stable_sort_pairs(nullptr, bytes, ctx->d_Rtcol, keys, iota, order, nnz_rt, end_bit, st));
+    if ((rc = tmp_bytes(bytes))) return rc;
+    HIPCHK(stable_sort_pairs(scan_tmp, bytes, ctx->d_Rtcol, keys, iota, order, nnz_rt, end_bit, st));
+  }
+  HIPCHK(hipMemsetAsync(rcount, 0, ((size_t)nagr + 1) * sizeof(int32_t), st));
+  launch_key_histogram(nnz_rt, ctx->d_Rtcol, rcount, st);
+  if ((rc = dalloc(ctx, &ctx->d_Rptr, (size_t)nagr + 1)) || (rc = scan32(rcount, ctx->d_Rptr, nagr + 1)))
+    return rc;
+  if ((rc = dalloc(ctx, &ctx->d_Rcol, (size_t)nnz_rt)) || (rc = dalloc(ctx, &ctx->d_Rval, (size_t)nnz_rt)))
+    return rc;
+  launch_gather_entries(nnz_rt, order, rowid, ctx->d_Rtval, ctx->d_Rcol, ctx->d_Rval, st);
+  // X*Y with first-touch rows: bound, slots, products, compaction
+  auto product = [&](int nrows, const int32_t* xp, const int32_t* xc, const double* xv,
+                     const int32_t* yp, const int32_t* yc, const double* yv, int32_t** op,
+                     int32_t** oc, double** ov, int32_t* onnz, bool keep) -> int {
+    int64_t *ub, *off;
+    int32_t *pc, *ccount;
+    double* pv;
+    HIPCHK(tmp.alloc(&ub, (size_t)nrows + 1));
+    HIPCHK(tmp.alloc(&off, (size_t)nrows + 1));
+    HIPCHK(hipMemsetAsync(ub, 0, ((size_t)nrows + 1) * sizeof(int64_t), st));
+    launch_product_bound(nrows, xp, xc, yp, ub, st);
+    if (int r = scan64(ub, off, nrows + 1)) return r;
+    int64_t total = 0;
+    if (int r = read_back(off + nrows, total)) return r;
+    HIPCHK(tmp.alloc(&pc, (size_t)total));
+    HIPCHK(tmp.alloc(&pv, (size_t)total));
+    HIPCHK(tmp.alloc(&ccount, (size_t)nrows + 1));
+    HIPCHK(hipMemsetAsync(ccount, 0, ((size_t)nrows + 1) * sizeof(int32_t), st));
+    launch_first_touch(nrows, xp, xc, xv, yp, yc, yv, off, pc, pv, ccount, st);
+    if (keep) {
+      if (int r = dalloc(ctx, op, (size_t)nrows + 1)) return r;
+    } else {
+      HIPCHK(tmp.alloc(op, (size_t)nrows + 1));
+    }
+    if (int r = scan32(ccount, *op, nrows + 1)) return r;
+    if (int r = read_back(*op + nrows, *onnz)) return r;
+    if (*onnz < 0) return fail(ctx, DOUG_E_UNSUPPORTED, "coarse product above 2^31-1 entries");
+    HIPCHK(tmp.alloc(oc, (size_t)*onnz));
+    HIPCHK(tmp.alloc(ov, (size_t)*onnz));
+    launch_compact(nrows, off, *op, pc, pv, *oc, *ov, st);
+    HIPCHK(hipGetLastError());
+    return DOUG_OK;
+  };
+  int32_t *tp_ptr, *tp_col, *ac_ptr, *ac_col;
+  double *tp_val, *ac_val;
+  int32_t nnz_tp = 0, nnz_ac = 0;
+  if ((rc = product(n, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Rtptr, ctx->d_Rtcol, ctx->d_Rtval,
+                    &tp_ptr, &tp_col, &tp_val, &nnz_tp, false)))
+    return rc;
+  if ((rc = product(nagr, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, tp_ptr, tp_col, tp_val, &ac_ptr,
+                    &ac_col, &ac_val, &nnz_ac, false)))
+    return rc;
+  // host copies: R (getters, row blocks) and A_c (coarse factor)
+  Csr& R = ctx->R;
+  R.nrows = nagr;
+  R.ncols = n;
+  R.ptr.resize((size_t)nagr + 1);
+  R.col.resize((size_t)nnz_rt);
+  R.val.resize((size_t)nnz_rt);
+  Csr& AC = ctx->AC;
+  AC.nrows = AC.ncols = nagr;
+  AC.ptr.resize((size_t)nagr + 1);
+  AC.col.resize((size_t)nnz_ac);
+  AC.val.resize((size_t)nnz_ac);
+  HIPCHK(hipMemcpyAsync(R.ptr.data(), ctx->d_Rptr, R.ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(R.col.data(), ctx->d_Rcol, R.col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(R.val.data(), ctx->d_Rval, R.val.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(AC.ptr.data(), ac_ptr, AC.ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(AC.col.data(), ac_col, AC.col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(AC.val.data(), ac_val, AC.val.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  ctx->Rt = Csr{};
+  ctx->Rt.nrows = n;
+  ctx->Rt.ncols = nagr;
+  ctx->coarse_on_device = true;
+  ctx->info.t_coarse_build = now() - t1;
   return DOUG_OK;
 }
+
+extern "C" {
 
 // (3) subdomain index sets: injected partition, or coarse aggregates
 //     (Partitioning_aggr.F90:120-193), plus `ol` layers (Preconditioner_base.F90:96-123)
@@ -614,7 +792,8 @@ static int setup_host(doug_ctx* ctx) {
   std::vector<double> scaled;
   std::vector<uint8_t> strong;
   phase_values(ctx, scaled, strong);
-  if ((rc = phase_coarse(ctx, scaled, strong))) return rc;
+  phase_aggregate(ctx, scaled, strong);
+  phase_coarse_host(ctx, strong);
   phase_subdomains(ctx);
   fill_info(ctx);
   ctx->info.t_total = now() - t0;
@@ -635,6 +814,31 @@ static int set_trsv_order(doug_ctx* ctx, int order) {
 }
 
 // Matrix, vectors, subdomain data and the subdomain factorisation (enqueued).
+// The matrix (values after the scale/unscale round trip) and its SpMV row
+// blocks on the device.
+static int upload_matrix(doug_ctx* ctx) {
+  const Csr& A = ctx->A;
+  double t = now();
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_ptr, A.ptr)) || (rc = upload(ctx, &ctx->d_col, A.col)) ||
+      (rc = upload(ctx, &ctx->d_val, A.val)) || (rc = upload(ctx, &ctx->d_src, A.src)))
+    return rc;
+  std::vector<int32_t> rb = row_blocks(A);
+  ctx->nrb = (int)rb.size() - 1;
+  if ((rc = upload(ctx, &ctx->d_rowblk, rb))) return rc;
+  ctx->d_rowblk_a = ctx->d_rowblk;
+  ctx->nrb_a = ctx->nrb;
+  std::vector<int32_t> rba;
+  if (spmv_shape_rows(ctx->spmv_shape) != kSpmvThreads) {
+    rba = row_blocks(A, spmv_shape_rows(ctx->spmv_shape));
+    ctx->nrb_a = (int)rba.size() - 1;
+    if ((rc = upload(ctx, &ctx->d_rowblk_a, rba))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // staging vectors go out of scope
+  ctx->info.t_upload = now() - t;
+  return DOUG_OK;
+}
+
 static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const Params& p = ctx->p;
   doug_info& info = ctx->info;
@@ -642,21 +846,6 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const int n = A.nrows;
   double t3 = now();
   int rc;
-  std::vector<int32_t> ptr(A.ptr.begin(), A.ptr.end()), col(A.col.begin(), A.col.end());
-  if ((rc = upload(ctx, &ctx->d_ptr, ptr)) || (rc = upload(ctx, &ctx->d_col, col)) ||
-      (rc = upload(ctx, &ctx->d_val, A.val)))
-    return rc;
-  std::vector<int32_t> rb = row_blocks(A);
-  ctx->nrb = (int)rb.size() - 1;
-  if ((rc = upload(ctx, &ctx->d_rowblk, rb))) return rc;
-  ctx->d_rowblk_a = ctx->d_rowblk;
-  ctx->nrb_a = ctx->nrb;
-  if (spmv_shape_rows(ctx->spmv_shape) != kSpmvThreads) {
-    std::vector<int32_t> rba = row_blocks(A, spmv_shape_rows(ctx->spmv_shape));
-    ctx->nrb_a = (int)rba.size() - 1;
-    if ((rc = upload(ctx, &ctx->d_rowblk_a, rba))) return rc;
-  }
-  if ((rc = upload(ctx, &ctx->d_src, A.src))) return rc;
   if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
   HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
   // device order of each subdomain's unknowns (the reported index sets keep
@@ -695,7 +884,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned), ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->d_scal, 0, S_N * sizeof(double), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));  // host staging vectors go out of scope
-  info.t_upload = now() - t3;
+  info.t_upload += now() - t3;
 
   // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
   int64_t elems = 0;
@@ -740,16 +929,16 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
   HIPCHK(hipEventRecord(ev2, ctx->stream));
   if (ctx->has_coarse) {
     const Csr& R = ctx->R;
-    if ((rc = upload(ctx, &ctx->d_Rptr, R.ptr)) || (rc = upload(ctx, &ctx->d_Rcol, R.col)) ||
-        (rc = upload(ctx, &ctx->d_Rval, R.val)))
-      return rc;
+    if (!ctx->coarse_on_device) {
+      const Csr& Rt = ctx->Rt;
+      if ((rc = upload(ctx, &ctx->d_Rptr, R.ptr)) || (rc = upload(ctx, &ctx->d_Rcol, R.col)) ||
+          (rc = upload(ctx, &ctx->d_Rval, R.val)) || (rc = upload(ctx, &ctx->d_Rtptr, Rt.ptr)) ||
+          (rc = upload(ctx, &ctx->d_Rtcol, Rt.col)) || (rc = upload(ctx, &ctx->d_Rtval, Rt.val)))
+        return rc;
+    }
     std::vector<int32_t> rbR = row_blocks(R);
     ctx->nrbR = (int)rbR.size() - 1;
     if ((rc = upload(ctx, &ctx->d_Rrowblk, rbR))) return rc;
-    const Csr& Rt = ctx->Rt;
-    if ((rc = upload(ctx, &ctx->d_Rtptr, Rt.ptr)) || (rc = upload(ctx, &ctx->d_Rtcol, Rt.col)) ||
-        (rc = upload(ctx, &ctx->d_Rtval, Rt.val)))
-      return rc;
     double* d_acval;
     if ((rc = upload(ctx, &d_acval, ctx->AC.val))) return rc;
     std::vector<int> cst{0, ctx->nc}, cinds((size_t)ctx->nc);
@@ -798,17 +987,30 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   double t0 = now();
   hipEvent_t ev[4];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  std::vector<double> scaled;
-  std::vector<uint8_t> strong;
+  std::vector<double>& scaled = ctx->scaled;
+  std::vector<uint8_t>& strong = ctx->strong;
   phase_values(ctx, scaled, strong);
+  if ((rc = upload_matrix(ctx))) return rc;
+  // R = T S^T and A_c on the device for the smoothed prolongator
+  auto coarse_space = [&]() -> int {
+    if (ctx->smoothers == 1 && ctx->use_device_coarse) return coarse_build_device(ctx, strong);
+    phase_coarse_host(ctx, strong);
+    return DOUG_OK;
+  };
   if (ctx->have_partition) {
-    // the subdomains do not depend on the aggregation: factorise them on the
-    // GPU while the host builds the coarse space
+    // the subdomains do not depend on the aggregation: a second host thread
+    // aggregates while this one prepares the subdomains and the GPU
+    // factorises them.  The two touch disjoint context members and read A,
+    // scaled and strong only.
+    std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
     phase_subdomains(ctx);
-    if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;
-    if ((rc = phase_coarse(ctx, scaled, strong))) return rc;
+    rc = device_fine(ctx, ev[0], ev[1]);
+    agg.join();
+    if (rc) return rc;
+    if ((rc = coarse_space())) return rc;
   } else {
-    if ((rc = phase_coarse(ctx, scaled, strong))) return rc;
+    phase_aggregate(ctx, scaled, strong);
+    if ((rc = coarse_space())) return rc;
     phase_subdomains(ctx);
     if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;
   }
@@ -1352,6 +1554,10 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
     case DOUG_OPT_LOCAL_ORDER:
       if (value != 0 && value != 1) return fail(ctx, DOUG_E_ARG, "local order must be 0 or 1");
       ctx->local_order = (int)value;
+      ctx->ready = false;  // takes effect at the next doug_setup
+      return DOUG_OK;
+    case DOUG_OPT_DEVICE_COARSE:
+      ctx->use_device_coarse = value != 0;
       ctx->ready = false;  // takes effect at the next doug_setup
       return DOUG_OK;
     case DOUG_OPT_TRSV_ORDER:

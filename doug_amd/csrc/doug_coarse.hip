@@ -1,0 +1,279 @@
+// gfx950 kernels that build the coarse space on the device during setup:
+// the smoothed restriction R = T S^T (CoarseMtx.F90:152-233, 422-425) and the
+// Galerkin product A_c = R (A R^T) (CoarseMtx.F90:476-516 through
+// SpMtx_op_AB.f90:39-200).  Every value is summed in the order the
+// reference's sequential loops sum it and every output row lists its
+// columns in the reference's order, so the results are bit-identical to the
+// host build.  Integer bookkeeping (prefix sums, the stable sort that
+// transposes R) uses hipcub.
+#include <hipcub/hipcub.hpp>
+
+#include "doug_kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace doug {
+
+namespace {
+
+int cdiv64(int64_t a, int b) { return (int)((a + b - 1) / b); }
+
+// Row j of R^T: the entries S(j, kk) of row j of A's pattern, grouped by the
+// aggregate a of kk and visited in ascending (a, kk, position) order; each
+// group is summed in that order.  S_jj = 1 - omega, S_jk = -omega/d_j * a_jk
+// for strong off-diagonal entries, 0 for weak ones, with d_j = a_jj + the
+// weak entries of row j.  Written to the slot starting at ptr[j].
+__global__ void k_rt_rows(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                          const double* __restrict__ val, const uint8_t* __restrict__ strong,
+                          const int32_t* __restrict__ num, double omega,
+                          int32_t* __restrict__ slot_col, double* __restrict__ slot_val,
+                          int32_t* __restrict__ cnt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int p0 = ptr[j], p1 = ptr[j + 1];
+  double diag = 0.0;
+  for (int p = p0; p < p1; p++)
+    if (col[p] == j || !strong[p]) diag = diag + val[p];
+  const double one_minus = 1.0 - omega, negw = -omega;
+  int m = 0;
+  int64_t prev_key = -1;
+  int prev_p = -1;
+  double acc = 0.0;
+  int acc_a = -1;
+  while (true) {
+    int64_t best_key = INT64_MAX;
+    int bp = -1;
+    for (int p = p0; p < p1; p++) {
+      const int kk = col[p], a = num[kk] - 1;
+      if (a < 0) continue;
+      const int64_t key = ((int64_t)a << 32) | (uint32_t)kk;
+      const bool after = key > prev_key || (key == prev_key && p > prev_p);
+      if (after && (key < best_key || (key == best_key && p < bp))) {
+        best_key = key;
+        bp = p;
+      }
+    }
+    if (bp < 0) break;
+    const double s = col[bp] == j ? one_minus : (strong[bp] ? negw / diag * val[bp] : 0.0);
+    const int a = (int)(best_key >> 32);
+    if (a == acc_a) {
+      acc = acc + 1.0 * s;
+    } else {
+      if (acc_a >= 0) {
+        slot_col[p0 + m] = acc_a;
+        slot_val[p0 + m] = acc;
+        m++;
+      }
+      acc_a = a;
+      acc = 1.0 * s;
+    }
+    prev_key = best_key;
+    prev_p = bp;
+  }
+  if (acc_a >= 0) {
+    slot_col[p0 + m] = acc_a;
+    slot_val[p0 + m] = acc;
+    m++;
+  }
+  cnt[j] = m;
+}
+
+// Bound on the entries of row i of X*Y: sum over X's row of Y's row lengths.
+__global__ void k_product_bound(int nrows, const int32_t* __restrict__ xptr,
+                                const int32_t* __restrict__ xcol, const int32_t* __restrict__ yptr,
+                                int64_t* __restrict__ ub) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  int64_t s = 0;
+  for (int p = xptr[i]; p < xptr[i + 1]; p++) s += yptr[xcol[p] + 1] - yptr[xcol[p]];
+  ub[i] = s;
+}
+
+// Row i of C = X*Y in Gustavson order (SpMtx_op_AB.f90:125-176): X's row in
+// stored order, for each entry Y's row in stored order; a column is appended
+// when first met (value = the product) and accumulated afterwards.  One
+// thread per row; the first kFirstTouchLds columns of the row are kept in
+// LDS, later ones in the row's global slot, whose capacity is the bound.
+constexpr int kFirstTouchLds = 64;
+__global__ __launch_bounds__(64) void k_first_touch(
+    int nrows, const int32_t* __restrict__ xptr, const int32_t* __restrict__ xcol,
+    const double* __restrict__ xval, const int32_t* __restrict__ yptr,
+    const int32_t* __restrict__ ycol, const double* __restrict__ yval,
+    const int64_t* __restrict__ slot_off, int32_t* __restrict__ slot_col,
+    double* __restrict__ slot_val, int32_t* __restrict__ cnt) {
+  constexpr int C = kFirstTouchLds;
+  __shared__ int32_t kc[C][64];
+  __shared__ double kv[C][64];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * 64 + t;
+  if (i >= nrows) return;
+  const int64_t base = slot_off[i];
+  int m = 0;
+  for (int p = xptr[i]; p < xptr[i + 1]; p++) {
+    const int k = xcol[p];
+    const double xv = xval[p];
+    for (int q = yptr[k]; q < yptr[k + 1]; q++) {
+      const int jj = ycol[q];
+      const double prod = xv * yval[q];
+      const int mc = m < C ? m : C;
+      int f = -1;
+      for (int e = 0; e < mc; e++)
+        if (kc[e][t] == jj) {
+          f = e;
+          break;
+        }
+      if (f >= 0) {
+        kv[f][t] = kv[f][t] + prod;
+        continue;
+      }
+      for (int e = C; e < m; e++)
+        if (slot_col[base + e] == jj) {
+          f = e;
+          break;
+        }
+      if (f >= 0) {
+        slot_val[base + f] = slot_val[base + f] + prod;
+        continue;
+      }
+      if (m < C) {
+        kc[m][t] = jj;
+        kv[m][t] = prod;
+      } else {
+        slot_col[base + m] = jj;
+        slot_val[base + m] = prod;
+      }
+      m++;
+    }
+  }
+  const int mc = m < C ? m : C;
+  for (int e = 0; e < mc; e++) {
+    slot_col[base + e] = kc[e][t];
+    slot_val[base + e] = kv[e][t];
+  }
+  cnt[i] = m;
+}
+
+// Copy each row's cnt[i] entries from its slot to the compacted CSR.
+template <typename O>
+__global__ void k_compact(int nrows, const O* __restrict__ slot_off, const int32_t* __restrict__ out_ptr,
+                          const int32_t* __restrict__ slot_col, const double* __restrict__ slot_val,
+                          int32_t* __restrict__ out_col, double* __restrict__ out_val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const int64_t s = slot_off[i];
+  const int o = out_ptr[i], m = out_ptr[i + 1] - o;
+  for (int e = 0; e < m; e++) {
+    out_col[o + e] = slot_col[s + e];
+    out_val[o + e] = slot_val[s + e];
+  }
+}
+
+__global__ void k_row_ids(int nrows, const int32_t* __restrict__ ptr, int32_t* __restrict__ rowid) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  for (int p = ptr[i]; p < ptr[i + 1]; p++) rowid[p] = i;
+}
+
+__global__ void k_iota(int64_t n, int32_t* __restrict__ v) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) v[e] = (int32_t)e;
+}
+
+// integer histogram (order-independent result)
+__global__ void k_key_histogram(int64_t n, const int32_t* __restrict__ keys, int32_t* __restrict__ counts) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) atomicAdd(&counts[keys[e]], 1);
+}
+
+__global__ void k_gather_entries(int64_t n, const int32_t* __restrict__ order,
+                                 const int32_t* __restrict__ rowid, const double* __restrict__ val,
+                                 int32_t* __restrict__ out_col, double* __restrict__ out_val) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int32_t q = order[e];
+  out_col[e] = rowid[q];
+  out_val[e] = val[q];
+}
+
+}  // namespace
+
+void launch_rt_rows(int n, const int32_t* ptr, const int32_t* col, const double* val,
+                    const uint8_t* strong, const int32_t* num, double omega, int32_t* slot_col,
+                    double* slot_val, int32_t* cnt, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_rt_rows, dim3(cdiv64(n, 256)), dim3(256), 0, st, n, ptr, col, val, strong,
+                     num, omega, slot_col, slot_val, cnt);
+}
+
+void launch_product_bound(int nrows, const int32_t* xptr, const int32_t* xcol, const int32_t* yptr,
+                          int64_t* ub, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_product_bound, dim3(cdiv64(nrows, 256)), dim3(256), 0, st, nrows, xptr, xcol,
+                     yptr, ub);
+}
+
+void launch_first_touch(int nrows, const int32_t* xptr, const int32_t* xcol, const double* xval,
+                        const int32_t* yptr, const int32_t* ycol, const double* yval,
+                        const int64_t* slot_off, int32_t* slot_col, double* slot_val, int32_t* cnt,
+                        hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_first_touch, dim3(cdiv64(nrows, 64)), dim3(64), 0, st, nrows, xptr, xcol,
+                     xval, yptr, ycol, yval, slot_off, slot_col, slot_val, cnt);
+}
+
+void launch_compact(int nrows, const int32_t* slot_off, const int32_t* out_ptr,
+                    const int32_t* slot_col, const double* slot_val, int32_t* out_col,
+                    double* out_val, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_compact<int32_t>, dim3(cdiv64(nrows, 256)), dim3(256), 0, st, nrows,
+                     slot_off, out_ptr, slot_col, slot_val, out_col, out_val);
+}
+
+void launch_compact(int nrows, const int64_t* slot_off, const int32_t* out_ptr,
+                    const int32_t* slot_col, const double* slot_val, int32_t* out_col,
+                    double* out_val, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_compact<int64_t>, dim3(cdiv64(nrows, 256)), dim3(256), 0, st, nrows,
+                     slot_off, out_ptr, slot_col, slot_val, out_col, out_val);
+}
+
+void launch_row_ids(int nrows, const int32_t* ptr, int32_t* rowid, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_row_ids, dim3(cdiv64(nrows, 256)), dim3(256), 0, st, nrows, ptr, rowid);
+}
+
+void launch_iota(int64_t n, int32_t* v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_iota, dim3(cdiv64(n, 256)), dim3(256), 0, st, n, v);
+}
+
+void launch_key_histogram(int64_t n, const int32_t* keys, int32_t* counts, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_key_histogram, dim3(cdiv64(n, 256)), dim3(256), 0, st, n, keys, counts);
+}
+
+void launch_gather_entries(int64_t n, const int32_t* order, const int32_t* rowid, const double* val,
+                           int32_t* out_col, double* out_val, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_entries, dim3(cdiv64(n, 256)), dim3(256), 0, st, n, order, rowid, val,
+                     out_col, out_val);
+}
+
+hipError_t exclusive_scan(void* tmp, size_t& bytes, const int32_t* in, int32_t* out, int n,
+                          hipStream_t st) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st);
+}
+
+hipError_t exclusive_scan(void* tmp, size_t& bytes, const int64_t* in, int64_t* out, int n,
+                          hipStream_t st) {
+  return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st);
+}
+
+hipError_t stable_sort_pairs(void* tmp, size_t& bytes, const int32_t* keys_in, int32_t* keys_out,
+                             const int32_t* vals_in, int32_t* vals_out, int n, int end_bit,
+                             hipStream_t st) {
+  return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys_in, keys_out, vals_in, vals_out, n, 0,
+                                            end_bit, st);
+}
+
+}  // namespace doug

@@ -140,6 +140,43 @@ void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double*
                             const double* q, double* partial, unsigned* counter, double* scal,
                             hipStream_t st);
 
+// ---- coarse space on the device (setup; bit-identical to the host build) ----
+// Rows of R^T = S T^T into slots at ptr[j] (capacity = row length of A),
+// cnt[j] = entries written.  strong: per entry of A; num: 1-based aggregate
+// of each node (0 = none).
+void launch_rt_rows(int n, const int32_t* ptr, const int32_t* col, const double* val,
+                    const uint8_t* strong, const int32_t* num, double omega, int32_t* slot_col,
+                    double* slot_val, int32_t* cnt, hipStream_t st);
+// ub[i] = number of products of row i of X*Y
+void launch_product_bound(int nrows, const int32_t* xptr, const int32_t* xcol, const int32_t* yptr,
+                          int64_t* ub, hipStream_t st);
+// rows of X*Y in first-touch column order into slots at slot_off[i]
+void launch_first_touch(int nrows, const int32_t* xptr, const int32_t* xcol, const double* xval,
+                        const int32_t* yptr, const int32_t* ycol, const double* yval,
+                        const int64_t* slot_off, int32_t* slot_col, double* slot_val, int32_t* cnt,
+                        hipStream_t st);
+// slots -> CSR with row pointers out_ptr
+void launch_compact(int nrows, const int32_t* slot_off, const int32_t* out_ptr,
+                    const int32_t* slot_col, const double* slot_val, int32_t* out_col,
+                    double* out_val, hipStream_t st);
+void launch_compact(int nrows, const int64_t* slot_off, const int32_t* out_ptr,
+                    const int32_t* slot_col, const double* slot_val, int32_t* out_col,
+                    double* out_val, hipStream_t st);
+void launch_row_ids(int nrows, const int32_t* ptr, int32_t* rowid, hipStream_t st);
+void launch_iota(int64_t n, int32_t* v, hipStream_t st);
+void launch_key_histogram(int64_t n, const int32_t* keys, int32_t* counts, hipStream_t st);
+// out[e] = (rowid[order[e]], val[order[e]])
+void launch_gather_entries(int64_t n, const int32_t* order, const int32_t* rowid, const double* val,
+                           int32_t* out_col, double* out_val, hipStream_t st);
+// hipcub prefix sums and stable radix sort; tmp == nullptr queries `bytes`
+hipError_t exclusive_scan(void* tmp, size_t& bytes, const int32_t* in, int32_t* out, int n,
+                          hipStream_t st);
+hipError_t exclusive_scan(void* tmp, size_t& bytes, const int64_t* in, int64_t* out, int n,
+                          hipStream_t st);
+hipError_t stable_sort_pairs(void* tmp, size_t& bytes, const int32_t* keys_in, int32_t* keys_out,
+                             const int32_t* vals_in, int32_t* vals_out, int n, int end_bit,
+                             hipStream_t st);
+
 // grad[src[p]] = -lam[row(p)] * x[col[p]] over the CSR entries of A
 void launch_value_grad(int n, const int32_t* ptr, const int32_t* col, const int32_t* src,
                        const double* lam, const double* x, double* grad, hipStream_t st);

@@ -115,6 +115,31 @@ void scale_values(const Csr& A, bool symm, std::vector<double>& scaled, std::vec
   });
 }
 
+void scale_round_trip(const Csr& A, const std::vector<double>& input, bool symm,
+                      std::vector<double>& scaled, std::vector<double>& diag,
+                      std::vector<double>& round_trip) {
+  const int n = std::max(A.nrows, A.ncols);
+  diag.assign((size_t)n, 0.0);
+  std::vector<double> sv((size_t)n, 0.0);
+  parallel_rows(A.nrows, [&](int i) {
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
+      if (A.col[p] == i) diag[i] = input[p];
+    sv[i] = symm ? std::sqrt(std::fabs(diag[i])) : std::fabs(diag[i]);
+  });
+  scaled.resize(input.size());
+  round_trip.resize(input.size());
+  parallel_rows(A.nrows, [&](int i) {
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      const double si = sv[i], sj = sv[A.col[p]];
+      double v = input[p] / si;
+      const double s = symm ? v / sj : v;
+      scaled[p] = s;
+      v = s * si;
+      round_trip[p] = symm ? v * sj : v;
+    }
+  });
+}
+
 void unscale_values(const Csr& A, bool symm, const std::vector<double>& diag,
                     std::vector<double>& vals) {
   int n = (int)diag.size();

@@ -53,6 +53,12 @@ Csr coo_to_csr(int n, int64_t nnz, const int32_t* indi, const int32_t* indj, con
 // SpMtx_scale (SpMtx_arrangement.F90:298-351): returns the scaled values and
 // the stored diagonal; symmetric scaling if `symm`, row scaling otherwise.
 void scale_values(const Csr& A, bool symm, std::vector<double>& scaled, std::vector<double>& diag);
+// scale_values on `input` (A's values), and the values after the reference's
+// scale/unscale round trip, in one pass: scaled and round_trip are what
+// scale_values followed by unscale_values give.
+void scale_round_trip(const Csr& A, const std::vector<double>& input, bool symm,
+                      std::vector<double>& scaled, std::vector<double>& diag,
+                      std::vector<double>& round_trip);
 // SpMtx_unscale (SpMtx_arrangement.F90:355-388) applied to scaled values.
 void unscale_values(const Csr& A, bool symm, const std::vector<double>& diag,
                     std::vector<double>& vals);

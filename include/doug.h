@@ -180,12 +180,16 @@ int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
  * doug_autotune would choose, see doug_tuning.  DOUG_OPT_LOCAL_ORDER
  * (default 1) places each subdomain's unknowns in its factor in reverse
  * Cuthill-McKee order, which narrows the stored tile envelope; 0 keeps the
- * index-set order.  It needs a new doug_setup and changes rounding only. */
+ * index-set order.  It needs a new doug_setup and changes rounding only.
+ * DOUG_OPT_DEVICE_COARSE (default 1): build the smoothed restriction and the
+ * Galerkin coarse matrix on the GPU (same bits as the host build, which 0
+ * selects); applies to smoothers = 1 at the next doug_setup. */
 enum {
   DOUG_OPT_ITERATION_GRAPH = 1,
   DOUG_OPT_SPMV_SHAPE = 2,
   DOUG_OPT_TRSV_ORDER = 3,
-  DOUG_OPT_LOCAL_ORDER = 4
+  DOUG_OPT_LOCAL_ORDER = 4,
+  DOUG_OPT_DEVICE_COARSE = 5
 };
 int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 

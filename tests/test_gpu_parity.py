@@ -256,3 +256,43 @@ def test_local_order_narrows_envelope_rounding_only():
             assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo), order
     print(f"factor tiles: index order {tiles[0]}, reverse Cuthill-McKee {tiles[1]}")
     assert tiles[1] < tiles[0]
+
+
+@pytest.mark.parametrize("case", ["3d-blocks", "3d-coarse-aggregates", "2d-blocks"])
+def test_device_coarse_build_bits(case):
+    # R = T S^T and A_c = R A R^T built on the GPU: the oracle's entries,
+    # order and bits, and the same preconditioner as the host build
+    if case == "2d-blocks":
+        n, i, j, v = P.laplace2d(96)
+        part = P.block_partition((96, 96), 48)
+        ctl = dict(CFG0_CTL)
+    else:
+        n, i, j, v = P.diffusion3d(20, 1.0, 12)
+        part = P.block_partition((20,) * 3, 10) if case == "3d-blocks" else None
+        ctl = dict(CFG0_CTL, radius2=2) if part is None else dict(CFG0_CTL)
+    print(f"device coarse build {case} n={n}", flush=True)
+    o, s = _pair(n, i, j, v, part, **ctl)
+    with s:
+        ri, rj, rv = o.R()
+        rp, rc, rvals = s.restriction()
+        order = np.lexsort((rj, ri))
+        np.testing.assert_array_equal(ri[order] - 1, np.repeat(np.arange(len(rp) - 1), np.diff(rp)))
+        np.testing.assert_array_equal(rj[order] - 1, rc)
+        np.testing.assert_array_equal(rv[order].view(np.int64), rvals.view(np.int64))
+        ci, cj, cv = o.AC()
+        cp, cc, cvals = s.coarse_matrix()
+        np.testing.assert_array_equal(ci - 1, np.repeat(np.arange(len(cp) - 1), np.diff(cp)))
+        np.testing.assert_array_equal(cj - 1, cc)
+        np.testing.assert_array_equal(cv.view(np.int64), cvals.view(np.int64))
+        for a, b in zip(o.subdomains(), s.subdomains()):
+            np.testing.assert_array_equal(a, b)
+        r = np.random.default_rng(3).uniform(-1, 1, n)
+        z = s.prec(r)
+        with D.Solver(0) as h:
+            h.set_option(5, 0)  # host build of R and A_c
+            h.set_params(**ctl)
+            h.set_matrix(n, i, j, v, 1)
+            if part is not None:
+                h.set_partition(*part)
+            h.setup()
+            assert _bits_equal(h.prec(r), z)
