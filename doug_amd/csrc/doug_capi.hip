@@ -87,6 +87,7 @@ struct doug_ctx {
   int nc = 0, cnt = 0;
   BlockDesc* d_cdesc = nullptr;
   double *d_ctiles = nullptr, *d_cW = nullptr;
+  double* d_cpart = nullptr;  // partial sums of the chunked coarse GEMVs
   double *d_crhs = nullptr, *d_cu = nullptr, *d_csol = nullptr;
   // PCG
   int n = 0;
@@ -950,15 +951,21 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
     if ((rc = build_blocks(ctx, ctx->AC, d_acval, cst, cinds, false, false, cdesc, &ctx->d_cdesc,
                            &ctx->d_ctiles, &celems, coffcount)))
       return rc;
-    if ((rc = enqueue_factor(ctx, ctx->d_ctiles, ctx->d_cdesc, cdesc, coffcount, ctx->d_flag + 1)))
-      return rc;
+    // right-looking: serial depth of one tile product per step, the
+    // trailing updates of a step run side by side
     ctx->cnt = cdesc[0].nt;
+    const int cnt = ctx->cnt;
+    for (int j = 0; j < cnt; j++) {
+      launch_chol_diag(ctx->d_ctiles, ctx->d_cdesc, 1, j, ctx->d_flag + 1, ctx->stream, false);
+      launch_chol_offdiag(ctx->d_ctiles, ctx->d_cdesc, 1, j, cnt - 1 - j, ctx->stream, false);
+      launch_rl_update(ctx->d_ctiles, ctx->d_cdesc, cnt, j, ctx->stream);
+    }
     if ((rc = dalloc(ctx, &ctx->d_cW, (size_t)celems))) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->d_cW, ctx->d_ctiles, (size_t)celems * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    for (int dd = 1; dd < ctx->cnt; dd++)
-      launch_trinv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, ctx->cnt, dd, ctx->stream);
+    launch_inv_init(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, ctx->stream);
+    for (int m = 0; m < cnt; m++) launch_inv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, m, ctx->stream);
     HIPCHK(hipGetLastError());
     size_t cpad = (size_t)ctx->cnt * kTile;
+    if ((rc = dalloc(ctx, &ctx->d_cpart, cpad * gemv_chunks(ctx->cnt)))) return rc;
     for (double** v : {&ctx->d_crhs, &ctx->d_cu, &ctx->d_csol}) {
       if ((rc = dalloc(ctx, v, cpad))) return rc;
       HIPCHK(hipMemsetAsync(*v, 0, cpad * sizeof(double), ctx->stream));
@@ -1114,8 +1121,8 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
   const bool coarse = ctx->has_coarse;
   if (coarse) {
     launch_spmv(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, ctx->d_crhs, ctx->stream);
-    launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_crhs, ctx->d_cu, ctx->stream);
-    launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cu, ctx->d_csol, ctx->stream);
+    launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_crhs, ctx->d_cu, ctx->d_cpart, ctx->stream);
+    launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cu, ctx->d_csol, ctx->d_cpart, ctx->stream);
   }
   const double* fine_src = r;
   if (p.fine_method == 1) {

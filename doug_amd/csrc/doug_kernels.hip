@@ -154,7 +154,7 @@ __global__ void k_pad_identity(T* __restrict__ tiles, const BlockDesc* __restric
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
                                                    const BlockDesc* __restrict__ d, int j,
-                                                   int* __restrict__ flag) {
+                                                   int* __restrict__ flag, int left) {
   __shared__ T As[kTile * LD];
   __shared__ T Bs[kTile * LD];
   typedef typename Mma<T>::v4 V;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
   T* Cg = tiles + b.base + toff(b, j, j);
   V acc[2][2];
   zero_acc(acc);
-  for (int k = b.tr[j].fc; k < j; k++) {
+  for (int k = left ? b.tr[j].fc : j; k < j; k++) {
     tile_to_lds(As, tiles + b.base + toff(b, j, k));
     __syncthreads();
     mma_abt<T>(acc, As, As, lane, wr, wc);
@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_offdiag(T* __restrict__ tiles,
-                                                      const BlockDesc* __restrict__ d, int j) {
+                                                      const BlockDesc* __restrict__ d, int j,
+                                                      int left) {
   __shared__ T As[kTile * LD];
   __shared__ T Bs[kTile * LD];
   typedef typename Mma<T>::v4 V;
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(256) void k_chol_offdiag(T* __restrict__ tiles,
   T* Cg = tiles + b.base + toff(b, i, j);
   V acc[2][2];
   zero_acc(acc);
-  for (int k = b.tr[i].fc; k < j; k++) {
+  for (int k = left ? b.tr[i].fc : j; k < j; k++) {
     tile_to_lds(As, tiles + b.base + toff(b, i, k));
     tile_to_lds(Bs, tiles + b.base + toff(b, j, k));
     __syncthreads();
@@ -262,44 +263,107 @@ __global__ __launch_bounds__(256) void k_chol_offdiag(T* __restrict__ tiles,
         Cg[acc_row<T>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = acc[bi][bj][r];
 }
 
-// W_{i,j} = -inv(L_ii) * sum_{k=j}^{i-1} L_ik W_kj, i = j + dd
-__global__ __launch_bounds__(256) void k_trinv(const double* __restrict__ L,
-                                               double* __restrict__ W,
-                                               const BlockDesc* __restrict__ d, int dd) {
+// Right-looking trailing update of a dense block (d[0]) after step j:
+// A_ik -= L_ij L_kj^T for every j < k <= i < nt, one tile per workgroup.
+__device__ __forceinline__ void tri_pair(int idx, int& ii, int& kk) {
+  int r = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= idx) r++;
+  while (r * (r + 1) / 2 > idx) r--;
+  ii = r;
+  kk = idx - r * (r + 1) / 2;
+}
+
+__global__ __launch_bounds__(256) void k_rl_update(double* __restrict__ tiles,
+                                                   const BlockDesc* __restrict__ d, int j) {
   __shared__ double As[kTile * LD];
   __shared__ double Bs[kTile * LD];
   const BlockDesc b = d[0];
-  const int j = blockIdx.x, i = j + dd;
+  int ii, kk;
+  tri_pair(blockIdx.x, ii, kk);
+  const int i = j + 1 + ii, k = j + 1 + kk;
   if (i >= b.nt) return;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  tile_to_lds(As, tiles + b.base + toff(b, i, j));
+  tile_to_lds(Bs, tiles + b.base + toff(b, k, j));
+  __syncthreads();
   Mma<double>::v4 acc[2][2];
   zero_acc(acc);
-  for (int k = j; k < i; k++) {
-    tile_to_lds(As, L + b.base + toff(b, i, k));
-    tile_to_lds(Bs, W + b.base + toff(b, k, j));
-    __syncthreads();
-    mma_ab<double>(acc, As, Bs, lane, wr, wc);
-    __syncthreads();
-  }
+  mma_abt<double>(acc, As, Bs, lane, wr, wc);
+  double* Cg = tiles + b.base + toff(b, i, k);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        Bs[acc_row<double>(wr, bi, lane, r) * LD + acc_col(wc, bj, lane)] = acc[bi][bj][r];
-  tile_to_lds(As, L + b.base + toff(b, i, i));  // inv(L_ii)
+      for (int r = 0; r < 4; r++) {
+        const int e = acc_row<double>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane);
+        Cg[e] = Cg[e] - acc[bi][bj][r];
+      }
+}
+
+// Inverse W = L^{-1} of a dense factor by right-looking substitution on
+// L W = I (W starts as I with inv(L_mm) on the diagonal tiles).  Step m:
+// finish block row m, W_mc = inv(L_mm) W_mc for c < m ...
+__global__ __launch_bounds__(256) void k_inv_finish(const double* __restrict__ L,
+                                                    double* __restrict__ W,
+                                                    const BlockDesc* __restrict__ d, int m) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * LD];
+  const BlockDesc b = d[0];
+  const int c = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  tile_to_lds(As, L + b.base + toff(b, m, m));  // inv(L_mm)
+  tile_to_lds(Bs, W + b.base + toff(b, m, c));
   __syncthreads();
+  Mma<double>::v4 acc[2][2];
   zero_acc(acc);
   mma_ab<double>(acc, As, Bs, lane, wr, wc);
-  double* Wg = W + b.base + toff(b, i, j);
+  double* Wg = W + b.base + toff(b, m, c);
 #pragma unroll
   for (int bi = 0; bi < 2; bi++)
 #pragma unroll
     for (int bj = 0; bj < 2; bj++)
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        Wg[acc_row<double>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = -acc[bi][bj][r];
+        Wg[acc_row<double>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane)] = acc[bi][bj][r];
+}
+
+// ... then W_ic -= L_im W_mc for every i > m, c <= m (grid: c, i - m - 1).
+__global__ __launch_bounds__(256) void k_inv_update(const double* __restrict__ L,
+                                                    double* __restrict__ W,
+                                                    const BlockDesc* __restrict__ d, int m) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * LD];
+  const BlockDesc b = d[0];
+  const int c = blockIdx.x, i = m + 1 + blockIdx.y;
+  if (i >= b.nt) return;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  tile_to_lds(As, L + b.base + toff(b, i, m));
+  tile_to_lds(Bs, W + b.base + toff(b, m, c));
+  __syncthreads();
+  Mma<double>::v4 acc[2][2];
+  zero_acc(acc);
+  mma_ab<double>(acc, As, Bs, lane, wr, wc);
+  double* Wg = W + b.base + toff(b, i, c);
+#pragma unroll
+  for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+    for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int e = acc_row<double>(wr, bi, lane, r) * kTile + acc_col(wc, bj, lane);
+        Wg[e] = Wg[e] - acc[bi][bj][r];
+      }
+}
+
+// W's diagonal tiles = the factor's (inv(L_ii)), its other tiles zero
+__global__ void k_inv_init(const double* __restrict__ L, double* __restrict__ W,
+                           const BlockDesc* __restrict__ d) {
+  const BlockDesc b = d[0];
+  const int i = blockIdx.y, k = blockIdx.x;
+  if (k > i) return;
+  const int64_t o = b.base + toff(b, i, k);
+  for (int e = threadIdx.x; e < kTileElems; e += blockDim.x) W[o + e] = i == k ? L[o + e] : 0.0;
 }
 
 // ------------------------------------------------------------------------
@@ -414,40 +478,47 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
   for (int p = t; p < b.s; p += 256) sol[b.vec_off + p] = (double)y[p];
 }
 
-// u_i = sum_{k<=i} W_ik c_k, one block per tile row
-__global__ __launch_bounds__(256) void k_gemv_w(const double* __restrict__ W,
-                                                const BlockDesc* __restrict__ d,
-                                                const double* __restrict__ c,
-                                                double* __restrict__ u) {
+// u = W c and c' = W^T u for the dense coarse inverse, split into chunks of
+// kGemvChunk tiles so that the whole chip streams W; each chunk writes a
+// partial 64-vector, summed afterwards chunk by chunk in a fixed order.
+// part_i(ch) = sum_{k in chunk ch, k <= i} W_ik c_k   (grid: chunk, tile row)
+__global__ __launch_bounds__(256) void k_gemv_w_part(const double* __restrict__ W,
+                                                     const BlockDesc* __restrict__ d,
+                                                     const double* __restrict__ c,
+                                                     double* __restrict__ part, int nch) {
   const BlockDesc b = d[0];
-  const int i = blockIdx.x;
-  if (i >= b.nt) return;
+  const int ch = blockIdx.x, i = blockIdx.y;
+  const int k0 = ch * kGemvChunk;
+  if (i >= b.nt || k0 > i) return;
+  const int k1 = min(i + 1, k0 + kGemvChunk);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double acc[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) acc[q] = 0.0;
-  for (int k = 0; k <= i; k++) {
+  for (int k = k0; k < k1; k++) {
     const double* Tk = W + b.base + toff(b, i, k) + (16 * w) * kTile + lane;
     const double ck = c[k * kTile + lane];
 #pragma unroll
     for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * ck;
   }
   double s = reduce_scatter16(acc, lane);
-  if ((lane & 3) == 0) u[i * kTile + 16 * w + (lane >> 2)] = s;
+  if ((lane & 3) == 0) part[((int64_t)i * nch + ch) * kTile + 16 * w + (lane >> 2)] = s;
 }
 
-// c_j = sum_{i>=j} W_ij^T u_i, one block per tile column
-__global__ __launch_bounds__(256) void k_gemv_wt(const double* __restrict__ W,
-                                                 const BlockDesc* __restrict__ d,
-                                                 const double* __restrict__ u,
-                                                 double* __restrict__ c) {
+// part_j(ch) = sum_{i in chunk ch of [j, nt)} W_ij^T u_i   (grid: chunk, tile column)
+__global__ __launch_bounds__(256) void k_gemv_wt_part(const double* __restrict__ W,
+                                                      const BlockDesc* __restrict__ d,
+                                                      const double* __restrict__ u,
+                                                      double* __restrict__ part, int nch) {
   __shared__ double red[4 * kTile];
   const BlockDesc b = d[0];
-  const int j = blockIdx.x;
-  if (j >= b.nt) return;
+  const int ch = blockIdx.x, j = blockIdx.y;
+  const int i0 = j + ch * kGemvChunk;
+  if (j >= b.nt || i0 >= b.nt) return;
+  const int i1 = min(b.nt, i0 + kGemvChunk);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double acc = 0.0;
-  for (int i = j; i < b.nt; i++) {
+  for (int i = i0; i < i1; i++) {
     const double* Ti = W + b.base + toff(b, i, j) + (16 * w) * kTile + lane;
     const double* ui = u + i * kTile + 16 * w;
 #pragma unroll
@@ -456,8 +527,22 @@ __global__ __launch_bounds__(256) void k_gemv_wt(const double* __restrict__ W,
   red[w * kTile + lane] = acc;
   __syncthreads();
   if (w == 0)
-    c[j * kTile + lane] =
+    part[((int64_t)j * nch + ch) * kTile + lane] =
         (red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]);
+}
+
+// out_r(tile i) = sum over the tile's chunks in order; rows 0 .. nt*64-1.
+// transposed: chunks of tile j cover [j, nt), otherwise [0, j].
+__global__ void k_gemv_sum(const double* __restrict__ part, int nt, int nch, int transposed,
+                           double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt * kTile) return;
+  const int i = e / kTile, r = e - i * kTile;
+  const int len = transposed ? nt - i : i + 1;
+  const int nc = (len + kGemvChunk - 1) / kGemvChunk;
+  double s = 0.0;
+  for (int ch = 0; ch < nc; ch++) s = s + part[((int64_t)i * nch + ch) * kTile + r];
+  out[e] = s;
 }
 
 // CSR-stream SpMV: the block's nonzeros are multiplied with coalesced loads
@@ -1022,15 +1107,34 @@ void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, 
 
 template <typename T>
 void launch_chol_diag(T* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_diag<T>, dim3(nblocks), dim3(256), 0, st, tiles, d, j, flag);
+                      hipStream_t st, bool left) {
+  hipLaunchKernelGGL(k_chol_diag<T>, dim3(nblocks), dim3(256), 0, st, tiles, d, j, flag, left ? 1 : 0);
 }
 
 template <typename T>
 void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
-                         hipStream_t st) {
+                         hipStream_t st, bool left) {
   if (maxcount <= 0) return;
-  hipLaunchKernelGGL(k_chol_offdiag<T>, dim3(maxcount, nblocks), dim3(256), 0, st, tiles, d, j);
+  hipLaunchKernelGGL(k_chol_offdiag<T>, dim3(maxcount, nblocks), dim3(256), 0, st, tiles, d, j,
+                     left ? 1 : 0);
+}
+
+void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, hipStream_t st) {
+  const int m = nt - 1 - j;
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_rl_update, dim3(m * (m + 1) / 2), dim3(256), 0, st, tiles, d, j);
+}
+
+void launch_inv_init(const double* L, double* W, const BlockDesc* d, int nt, hipStream_t st) {
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_inv_init, dim3(nt, nt), dim3(256), 0, st, L, W, d);
+}
+
+void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m,
+                     hipStream_t st) {
+  if (m > 0) hipLaunchKernelGGL(k_inv_finish, dim3(m), dim3(256), 0, st, L, W, d, m);
+  if (nt - m - 1 > 0)
+    hipLaunchKernelGGL(k_inv_update, dim3(m + 1, nt - m - 1), dim3(256), 0, st, L, W, d, m);
 }
 
 template <typename T>
@@ -1044,19 +1148,14 @@ void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t*
   template void launch_scatter_entries<T>(T*, const int64_t*, const int32_t*, const double*,   \
                                           int64_t, hipStream_t);                               \
   template void launch_pad_identity<T>(T*, const BlockDesc*, int, int, hipStream_t);           \
-  template void launch_chol_diag<T>(T*, const BlockDesc*, int, int, int*, hipStream_t);        \
-  template void launch_chol_offdiag<T>(T*, const BlockDesc*, int, int, int, hipStream_t);      \
+  template void launch_chol_diag<T>(T*, const BlockDesc*, int, int, int*, hipStream_t, bool);  \
+  template void launch_chol_offdiag<T>(T*, const BlockDesc*, int, int, int, hipStream_t, bool); \
   template void launch_trsv<T>(const T*, const BlockDesc*, int, const int32_t*, const double*, \
                                double*, hipStream_t);
 DOUG_INSTANTIATE(double)
 DOUG_INSTANTIATE(float)
 #undef DOUG_INSTANTIATE
 
-void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, int dd,
-                       hipStream_t st) {
-  if (nt - dd <= 0) return;
-  hipLaunchKernelGGL(k_trinv, dim3(nt - dd), dim3(256), 0, st, L, W, d, dd);
-}
 
 void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                  const double* val, const double* x, double* y, hipStream_t st, int shape) {
@@ -1073,13 +1172,17 @@ void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32
 }
 
 void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_w, dim3(nt), dim3(256), 0, st, W, d, c, u);
+                   double* part, hipStream_t st) {
+  const int nch = gemv_chunks(nt);
+  hipLaunchKernelGGL(k_gemv_w_part, dim3(nch, nt), dim3(256), 0, st, W, d, c, part, nch);
+  hipLaunchKernelGGL(k_gemv_sum, dim3(cdiv(nt * kTile, 256)), dim3(256), 0, st, part, nt, nch, 0, u);
 }
 
 void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u, double* c,
-                    hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_wt, dim3(nt), dim3(256), 0, st, W, d, u, c);
+                    double* part, hipStream_t st) {
+  const int nch = gemv_chunks(nt);
+  hipLaunchKernelGGL(k_gemv_wt_part, dim3(nch, nt), dim3(256), 0, st, W, d, u, part, nch);
+  hipLaunchKernelGGL(k_gemv_sum, dim3(cdiv(nt * kTile, 256)), dim3(256), 0, st, part, nt, nch, 1, c);
 }
 
 void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,

@@ -49,17 +49,22 @@ void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, 
 // then the tiles below them (GEMM + TRSM), on the matrix cores in T.  `flag`
 // receives the first failing block+1 on a non-positive pivot.  `maxcount`
 // bounds lr_j - j over the blocks.
+// With left = false the kernels skip the left-looking sums (the trailing
+// updates were applied by launch_rl_update).
 template <typename T>
 void launch_chol_diag(T* tiles, const BlockDesc* d, int nblocks, int j, int* flag,
-                      hipStream_t st);
+                      hipStream_t st, bool left = true);
 template <typename T>
 void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
-                         hipStream_t st);
-// Explicit inverse of a dense tile-packed lower factor (single block d[0]):
-// W = L^{-1}, tiles at diagonal offset `dd` (W_{j+dd,j} for all j); W must
-// start as a copy of L so that its diagonal tiles hold inv(L_jj).
-void launch_trinv_step(const double* L, double* W, const BlockDesc* d, int nt, int dd,
-                       hipStream_t st);
+                         hipStream_t st, bool left = true);
+// Dense single block d[0]: right-looking trailing update after step j,
+// A_ik -= L_ij L_kj^T for j < k <= i.
+void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, hipStream_t st);
+// W = L^{-1} of a dense factor (diagonal tiles inv(L_ii)): launch_inv_init,
+// then launch_inv_step for m = 0 .. nt-1 (right-looking substitution).
+void launch_inv_init(const double* L, double* W, const BlockDesc* d, int nt, hipStream_t st);
+void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m,
+                     hipStream_t st);
 
 // ---- per-iteration operators ----
 // y = A x, CSR-stream: each row summed in stored order, products rounded
@@ -79,11 +84,14 @@ void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, co
 template <typename T>
 void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
                  const double* r, double* sol, hipStream_t st);
-// u = W c and c' = W^T u for the coarse inverse factor.
+// u = W c and c' = W^T u for the coarse inverse factor; `part` holds
+// nt * gemv_chunks(nt) * 64 doubles of partial sums.
+constexpr int kGemvChunk = 8;  // tiles per workgroup
+constexpr int gemv_chunks(int nt) { return (nt + kGemvChunk - 1) / kGemvChunk; }
 void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
-                   hipStream_t st);
+                   double* part, hipStream_t st);
 void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u, double* c,
-                    hipStream_t st);
+                    double* part, hipStream_t st);
 // z[i] = sum over occurrences (ascending subdomain) of sol, then
 //        + sum_a Rt(i,a) csol[a]  (coarse prolongation), fine part first.
 void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
