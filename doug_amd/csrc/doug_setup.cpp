@@ -194,9 +194,14 @@ static void strong_refs(const Csr& A, const std::vector<double>& scaled,
       tmp.clear();
       for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
         if (strong[p] && A.col[p] != i && A.col[p] < n) tmp.push_back(p);
-      std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int c) {
-        return std::fabs(scaled[a]) > std::fabs(scaled[c]);
-      });
+      // stable insertion sort, |scaled| descending (rows are short)
+      for (size_t q = 1; q < tmp.size(); q++) {
+        const int e = tmp[q];
+        const double ve = std::fabs(scaled[e]);
+        size_t r = q;
+        for (; r > 0 && ve > std::fabs(scaled[tmp[r - 1]]); r--) tmp[r] = tmp[r - 1];
+        tmp[r] = e;
+      }
       for (size_t k = 0; k < tmp.size(); k++) scol[srow[i] + k] = A.col[tmp[k]];
     }
   });
@@ -480,19 +485,30 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
     for (int v : fullnum) full_nagrs = std::max(full_nagrs, v);
     form_aggr(inner, nagrs, n, aggrnum);
   } else {
-    // merge too-small aggregates (SpMtx_aggregation.F90:365-591)
-    std::vector<std::vector<int>> members((size_t)nagrs + 1);
-    for (int i = 0; i < n; i++) members[aggrnum[i]].push_back(i);
-    std::vector<int> colsaround;
+    // merge too-small aggregates (SpMtx_aggregation.F90:365-591).  Members
+    // are kept as lists in insertion order (aggregate 0 = unaggregated), so
+    // that a whole aggregate moves to its eater in O(1).
+    std::vector<int> head((size_t)nagrs + 1, -1), tail((size_t)nagrs + 1, -1), next((size_t)n, -1),
+        msize((size_t)nagrs + 1, 0);
+    auto append = [&](int a, int node) {
+      next[node] = -1;
+      if (tail[a] < 0)
+        head[a] = node;
+      else
+        next[tail[a]] = node;
+      tail[a] = node;
+      msize[a]++;
+    };
+    for (int i = 0; i < n; i++) append(aggrnum[i], i);
+    std::vector<int> colsaround, mem;
     std::vector<double> connw;
     int nagrs_new = nagrs;
     const double beta = ps.beta;
     for (int a = 1; a <= nagrs; a++) {
-      std::vector<int>& mem = members[a];
-      if ((int)mem.size() >= minasize) continue;
+      if (msize[a] >= minasize) continue;
       colsaround.clear();
       connw.clear();
-      for (int node : mem) {
+      for (int node = head[a]; node >= 0; node = next[node]) {
         for (int p = A.ptr[node]; p < A.ptr[node + 1]; p++) {
           int c = A.col[p];
           if (c >= n) continue;
@@ -514,20 +530,27 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
       double maxcw = 0.0;
       int eater = 0;
       for (size_t j = 0; j < colsaround.size(); j++)
-        if (maxcw < connw[j] &&
-            (int)members[colsaround[j]].size() + (int)mem.size() <= maxasize) {
+        if (maxcw < connw[j] && msize[colsaround[j]] + msize[a] <= maxasize) {
           maxcw = connw[j];
           eater = colsaround[j];
         }
       if (maxcw >= alpha || (coarse_level && maxcw >= beta)) {
-        for (int node : mem) {
-          members[eater].push_back(node);
-          aggrnum[node] = eater;
+        for (int node = head[a]; node >= 0; node = next[node]) aggrnum[node] = eater;
+        if (head[a] >= 0) {
+          if (tail[eater] < 0)
+            head[eater] = head[a];
+          else
+            next[tail[eater]] = head[a];
+          tail[eater] = tail[a];
+          msize[eater] += msize[a];
         }
-        mem.clear();
+        head[a] = tail[a] = -1;
+        msize[a] = 0;
         nagrs_new--;
       } else {
         // give the aggregate away node by node, from its last node back
+        mem.clear();
+        for (int node = head[a]; node >= 0; node = next[node]) mem.push_back(node);
         int nleft = (int)mem.size();
         int sz = nleft;
         bool reduced = true;
@@ -540,9 +563,8 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
               int c = A.col[p];
               if (c >= n) continue;
               int colr = aggrnum[c];
-              if (colr != a && (int)members[colr].size() < maxasizelargest &&
-                  std::fabs(scaled[p]) >= beta) {
-                members[colr].push_back(node);
+              if (colr != a && msize[colr] < maxasizelargest && std::fabs(scaled[p]) >= beta) {
+                append(colr, node);
                 aggrnum[node] = colr;
                 mem[k] = -1;
                 nleft--;
@@ -552,19 +574,17 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
             }
           }
         }
-        if (nleft > 0 && nleft < sz) {
-          int k = 0;
+        if (nleft < sz) {  // the nodes left stay, in their order
+          head[a] = tail[a] = -1;
+          msize[a] = 0;
           for (int j = 0; j < sz; j++)
-            if (mem[j] >= 0) mem[k++] = mem[j];
-          mem.resize((size_t)nleft);
-        } else if (nleft == 0) {
-          mem.clear();
-          nagrs_new--;
+            if (mem[j] >= 0) append(a, mem[j]);
+          if (nleft == 0) nagrs_new--;
         }
       }
     }
     int64_t naggregated = 0;
-    for (int a = 1; a <= nagrs; a++) naggregated += (int64_t)members[a].size();
+    for (int a = 1; a <= nagrs; a++) naggregated += (int64_t)msize[a];
     full_nagrs = nagrs_new;
     int cnt = *std::max_element(aggrnum.begin(), aggrnum.end());
     if (cnt > nagrs_new) {
