@@ -417,22 +417,49 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
   __shared__ T y[kMaxTrsvTiles * kTile];
   __shared__ T tv[kTile];
   __shared__ T red[4 * kTile];
+  __shared__ int rbase[kMaxTrsvTiles], rfc[kMaxTrsvTiles], rlr[kMaxTrsvTiles];
   const BlockDesc b = d[blockIdx.x];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nt = b.nt, npad = nt * kTile;
-  const T* T0 = tiles + b.base;
+  const T* T0 = tiles + b.base + (16 * w) * kTile + lane;
+  // the block's envelope table in LDS: tile (i, k) at T0 + (rbase[i] + k) * 4096
+  for (int i = t; i < nt; i += 256) {
+    const TileRow r = b.tr[i];
+    rbase[i] = r.off - r.fc;
+    rfc[i] = r.fc;
+    rlr[i] = r.lr;
+  }
   for (int p = t; p < npad; p += 256) y[p] = p < b.s ? (T)rhs[inds[b.vec_off + p]] : (T)0;
   __syncthreads();
-  // forward: y_i = inv(L_ii) (r_i - sum_{k<i} L_ik y_k)
+  // forward: y_i = inv(L_ii) (r_i - sum_{k<i} L_ik y_k).  The diagonal tile
+  // is fetched with the first tiles of the row and off-diagonal tiles are
+  // taken two at a time, so that more loads are in flight per wait.
   for (int i = 0; i < nt; i++) {
-    T acc[16];
+    const T* Ri = T0 + (int64_t)rbase[i] * kTileElems;
+    T dg[16], acc[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] = (T)0;
-    for (int k = b.tr[i].fc; k < i; k++) {
-      const T* Tk = T0 + toff(b, i, k) + (16 * w) * kTile + lane;
-      const T yk = y[k * kTile + lane];
+    for (int q = 0; q < 16; q++) {
+      dg[q] = Ri[(int64_t)i * kTileElems + q * kTile];
+      acc[q] = (T)0;
+    }
+    int k = rfc[i];
+    for (; k + 2 <= i; k += 2) {
+      const T* A0 = Ri + (int64_t)k * kTileElems;
+      T a0[16], a1[16];
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * yk;
+      for (int q = 0; q < 16; q++) {
+        a0[q] = A0[q * kTile];
+        a1[q] = A0[kTileElems + q * kTile];
+      }
+      const T y0 = y[k * kTile + lane], y1 = y[(k + 1) * kTile + lane];
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[q] += a0[q] * y0 + a1[q] * y1;
+    }
+    if (k < i) {
+      const T* A0 = Ri + (int64_t)k * kTileElems;
+      const T y0 = y[k * kTile + lane];
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc[q] += A0[q * kTile] * y0;
     }
     T s = reduce_scatter16(acc, lane);
     if ((lane & 3) == 0) {
@@ -440,23 +467,39 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       tv[row] = y[i * kTile + row] - s;
     }
     __syncthreads();
-    const T* Ti = T0 + toff(b, i, i) + (16 * w) * kTile + lane;
     const T tl = tv[lane];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] = Ti[q * kTile] * tl;
+    for (int q = 0; q < 16; q++) acc[q] = dg[q] * tl;
     s = reduce_scatter16(acc, lane);
     if ((lane & 3) == 0) y[i * kTile + 16 * w + (lane >> 2)] = s;
     __syncthreads();
   }
   // backward: x_j = inv(L_jj)^T (y_j - sum_{i>j} L_ij^T x_i)
   for (int j = nt - 1; j >= 0; j--) {
-    T acc = (T)0;
-    const int lr = b.tr[j].lr;
-    for (int i = j + 1; i <= lr; i++) {
-      const T* Ti = T0 + toff(b, i, j) + (16 * w) * kTile + lane;
-      const T* xi = y + i * kTile + 16 * w;
+    T dg[16];
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * xi[q];
+    for (int q = 0; q < 16; q++) dg[q] = T0[((int64_t)rbase[j] + j) * kTileElems + q * kTile];
+    T acc = (T)0;
+    const int lr = rlr[j];
+    int i = j + 1;
+    for (; i + 2 <= lr + 1; i += 2) {
+      const T* A0 = T0 + ((int64_t)rbase[i] + j) * kTileElems;
+      const T* A1 = T0 + ((int64_t)rbase[i + 1] + j) * kTileElems;
+      T a0[16], a1[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        a0[q] = A0[q * kTile];
+        a1[q] = A1[q * kTile];
+      }
+      const T* x0 = y + i * kTile + 16 * w;
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc += a0[q] * x0[q] + a1[q] * x0[kTile + q];
+    }
+    if (i <= lr) {
+      const T* A0 = T0 + ((int64_t)rbase[i] + j) * kTileElems;
+      const T* x0 = y + i * kTile + 16 * w;
+#pragma unroll
+      for (int q = 0; q < 16; q++) acc += A0[q * kTile] * x0[q];
     }
     red[w * kTile + lane] = acc;
     __syncthreads();
@@ -464,10 +507,9 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       tv[lane] = y[j * kTile + lane] -
                  ((red[lane] + red[kTile + lane]) + (red[2 * kTile + lane] + red[3 * kTile + lane]));
     __syncthreads();
-    const T* Tj = T0 + toff(b, j, j) + (16 * w) * kTile + lane;
     acc = (T)0;
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc += Tj[q * kTile] * tv[16 * w + q];
+    for (int q = 0; q < 16; q++) acc += dg[q] * tv[16 * w + q];
     red[w * kTile + lane] = acc;
     __syncthreads();
     if (w == 0)
