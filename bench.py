@@ -36,8 +36,15 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 # scaled by the same counters on a copy of known size) for the 128^3
 # workload; the run and its summary are in profiles/hbm_traffic_pmc_128cube.md
 MEASURED_TRAFFIC_128 = {
-    ("k_trsv", 1): 61_373_604_128 + 22_282_240,
-    ("k_spmv", None): 222_930_140 + 16_777_216,
+    # HBM bytes per launch (read + written) from the FETCH_SIZE / WRITE_SIZE runs of
+    # profiles/hbm_traffic_pmc_128cube.md (FETCH_SIZE scaled by the copy-kernel factor)
+    ("k_trsv", 1): 6_053_640_466 + 22_282_240,
+    ("k_spmv", 0): 244_765_692 + 16_777_216,
+    ("k_spmv", 1): 213_560_110 + 16_777_216,
+    ("k_spmv", 2): 222_965_693 + 16_777_216,
+    ("k_spmv", 3): 208_320_176 + 16_777_216,
+    ("k_spmv", 4): 210_944_933 + 16_777_216,
+    ("k_spmv", 5): 207_943_516 + 16_777_216,
 }
 
 CTL = dict(radius1=4, minasize1=100, maxasize1=343, strong1=0.08, smoothers=1, overlap=1,
@@ -221,13 +228,16 @@ def main():
         tb2 = time.perf_counter()
         if k >= args.warmup:
             bt.append((tb1 - tb0, tb2 - tb1))
-        log(f"[bench] batched {nb} rhs: setup+inverse {tb1 - tb0:.3f}s, solve {tb2 - tb1:.3f}s, "
+        log(f"[bench] batched {nb} rhs: setup {tb1 - tb0:.3f}s, solve {tb2 - tb1:.3f}s, "
             f"iterations {int(its_b.min())}-{int(its_b.max())}")
     prec32_ms = s.time_operator(4, 3)
+    # SpMV on its own: the tuned shape, many back-to-back launches
+    spmv_ms = s.time_operator(0, 200)
+    spmv_gbs = spmv_bytes(n, nnz) / (spmv_ms * 1e-3) / 1e9
     xr0 = Xm[:, 0].contiguous()
     rr = torch.empty_like(xr0)
     s.spmv_device(xr0.data_ptr(), rr.data_ptr())
-    batched = {"nrhs": nb, "setup_and_inverse_s": round(float(np.mean([a for a, _ in bt])), 4),
+    batched = {"nrhs": nb, "setup_s": round(float(np.mean([a for a, _ in bt])), 4),
                "solve_s": round(float(np.mean([c for _, c in bt])), 4),
                "time_to_solve_all_s": round(float(np.mean([a + c for a, c in bt])), 4),
                "time_per_rhs_s": round(float(np.mean([a + c for a, c in bt])) / nb, 5),
@@ -273,9 +283,10 @@ def main():
                    "ncoarse": info["ncoarse"], "ctl": CTL},
         "form": best_name,
         "speedup_fp32_factors_vs_fp64": round(fp64["time_to_solve_s"] / fp32["time_to_solve_s"], 3),
-        "spmv": {"GB/s": fp64["spmv_GB/s"], "frac_of_hbm_peak": round(fp64["spmv_GB/s"] / HBM_PEAK_GBS, 4),
-                 "ms": round(fp64["spmv_ms"], 5), "bytes": spmv_bytes(n, nnz),
-                 "traffic": MEASURED_TRAFFIC_128[("k_spmv", None)] if N == 128 else None},
+        "spmv": {"GB/s": round(spmv_gbs, 1), "frac_of_hbm_peak": round(spmv_gbs / HBM_PEAK_GBS, 4),
+                 "ms": round(spmv_ms, 5), "bytes": spmv_bytes(n, nnz),
+                 "shape": {k: tune[k] for k in ("spmv_rows", "spmv_xcd_order")},
+                 "traffic": MEASURED_TRAFFIC_128.get(("k_spmv", tune["spmv_shape"])) if N == 128 else None},
         "fp64": fp64,
         "fp32_factors": fp32,
         "batched_32rhs": batched,
