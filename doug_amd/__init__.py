@@ -6,6 +6,7 @@ DOUG.ctl / file-format readers and synthetic problem generators.
 """
 from . import problems  # noqa: F401
 from ._lib import LIB_PATH, load  # noqa: F401
-from .solver import DougError, Solver, default_params, make_params  # noqa: F401
+from .solver import DougError, Solver, default_params, make_params, partition_graph  # noqa: F401
 
-__all__ = ["Solver", "DougError", "make_params", "default_params", "problems", "load", "LIB_PATH"]
+__all__ = ["Solver", "DougError", "make_params", "default_params", "partition_graph", "problems", "load",
+           "LIB_PATH"]

@@ -72,6 +72,8 @@ SIGNATURES = {
     "doug_set_params": (_i32, [_vp, C.POINTER(DougParams)]),
     "doug_set_matrix_coo": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _i32]),
     "doug_set_partition": (_i32, [_vp, _i32, _vp, _vp, _i32]),
+    "doug_set_num_subdomains": (_i32, [_vp, _i32]),
+    "doug_partition_graph": (_i32, [_i32, _vp, _vp, _vp, _i32, _i32, _vp]),
     "doug_setup": (_i32, [_vp, _i32]),
     "doug_setup_host": (_i32, [_vp]),
     "doug_get_info": (_i32, [_vp, C.POINTER(DougInfo)]),

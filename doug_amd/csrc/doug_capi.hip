@@ -46,6 +46,7 @@ struct doug_ctx {
   std::vector<uint8_t> strong;
   std::vector<int> part_starts, part_nodes;
   bool have_partition = false;
+  int num_subdomains = 0;  // > 1: parts of the fine-aggregate graph (doug_set_num_subdomains)
   ProcessState ps;
   Aggregates fin, ffull, cin, cfull;
   Csr R, Rt, AC;
@@ -300,6 +301,30 @@ int doug_set_matrix_coo(doug_ctx* ctx, int32_t n, int64_t nnz, const int32_t* in
   ctx->n = n;
   ctx->have_matrix = true;
   ctx->ready = false;
+  return DOUG_OK;
+}
+
+int doug_set_num_subdomains(doug_ctx* ctx, int32_t nsub) {
+  if (!ctx) return DOUG_E_ARG;
+  if (nsub < 0) return fail(ctx, DOUG_E_ARG, "number of subdomains must be >= 0");
+  ctx->num_subdomains = nsub;
+  ctx->ready = false;
+  return DOUG_OK;
+}
+
+int doug_partition_graph(int32_t n, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+                         int32_t nparts, int32_t base, int32_t* part) {
+  if (n < 0 || nparts < 1 || !xadj || !part || (base != 0 && base != 1)) return DOUG_E_ARG;
+  if (n == 0) return DOUG_OK;
+  std::vector<int> x((size_t)n + 1), a((size_t)(xadj[n] - xadj[0]));
+  for (int i = 0; i <= n; i++) x[i] = xadj[i] - xadj[0];
+  for (size_t p = 0; p < a.size(); p++) {
+    a[p] = adjncy[p] - base;
+    if (a[p] < 0 || a[p] >= n) return DOUG_E_ARG;
+  }
+  std::vector<int> out;
+  partition_graph(n, x.data(), a.data(), vwgt, nparts, out);
+  for (int i = 0; i < n; i++) part[i] = out[i] + base;
   return DOUG_OK;
 }
 
@@ -724,6 +749,25 @@ static void phase_subdomains(doug_ctx* ctx) {
   if (ctx->have_partition) {
     pstarts = ctx->part_starts;
     pnodes = ctx->part_nodes;
+  } else if (ctx->num_subdomains > 1) {
+    // Partitionings_metis_InitCoarse (Partitioning_metis.F90:34-86) with the
+    // project partitioner: parts of the fine-aggregate graph, each listing
+    // its nodes in node order
+    std::vector<int> xadj, adjncy, part;
+    aggregate_graph(A, ctx->fin, ctx->ffull, xadj, adjncy);
+    partition_graph(ctx->ffull.nagr, xadj.data(), adjncy.data(), nullptr, ctx->num_subdomains, part);
+    const int k = ctx->num_subdomains;
+    pstarts.assign((size_t)k + 1, 0);
+    for (int i = 0; i < n; i++) pstarts[part[ctx->ffull.num[i] - 1] + 1]++;
+    for (int q = 0; q < k; q++) pstarts[q + 1] += pstarts[q];
+    pnodes.assign((size_t)n, 0);
+    std::vector<int> fill(pstarts.begin(), pstarts.end() - 1);
+    for (int i = 0; i < n; i++) pnodes[fill[part[ctx->ffull.num[i] - 1]]++] = i;
+    // parts the partitioner left empty are dropped
+    std::vector<int> ps{0};
+    for (int q = 0; q < k; q++)
+      if (pstarts[q + 1] > pstarts[q]) ps.push_back(pstarts[q + 1]);
+    pstarts.swap(ps);
   } else {
     const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
     const int radius1 = p.radius1 > 0 ? p.radius1 : 2;

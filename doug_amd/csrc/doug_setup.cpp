@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <queue>
+#include <tuple>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -979,6 +980,200 @@ void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<i
       }
     });
   for (auto& x : th) x.join();
+}
+
+// ---------------------------------------------------------------------------
+// Graph partitioner standing in for METIS 4.0 (absent here): recursive
+// bisection.  Each bisection grows one side from a pseudo-peripheral vertex,
+// greedily by edge gain (greedy graph growing, G. Karypis and V. Kumar, SIAM
+// J. Sci. Comput. 20 (1998) 359-392), until it holds its share of the vertex
+// weight (disconnected pieces are taken in vertex order), then moves boundary
+// vertices whose move cuts fewer edges while both sides stay within the
+// balance tolerance (greedy refinement in the manner of B. Kernighan and
+// S. Lin, Bell Syst. Tech. J. 49 (1970) 291-307).  Deterministic.
+namespace {
+
+struct Bisector {
+  const int* xadj;
+  const int* adj;
+  const int* vw;
+  std::vector<int>& part;
+  std::vector<int> loc, mark, side, queue, gain;
+  std::vector<int64_t> disc;
+  int stamp = 0;
+
+  Bisector(int n, const int* x, const int* a, const int* w, std::vector<int>& p)
+      : xadj(x), adj(a), vw(w), part(p), loc((size_t)n, -1), mark((size_t)n, 0),
+        side((size_t)n, 0), gain((size_t)n, 0), disc((size_t)n, 0) {}
+
+  int64_t weight(int v) const { return vw ? vw[v] : 1; }
+
+  // BFS over `verts` (loc >= 0) from root; returns the last vertex reached
+  // with the smallest degree in the last level
+  int far_vertex(int root) {
+    ++stamp;
+    queue.clear();
+    queue.push_back(root);
+    mark[root] = stamp;
+    size_t lb = 0, le = 1;
+    while (true) {
+      for (size_t q = lb; q < le; q++) {
+        int v = queue[q];
+        for (int p = xadj[v]; p < xadj[v + 1]; p++) {
+          int u = adj[p];
+          if (loc[u] >= 0 && mark[u] != stamp) {
+            mark[u] = stamp;
+            queue.push_back(u);
+          }
+        }
+      }
+      if (queue.size() == le) break;
+      lb = le;
+      le = queue.size();
+    }
+    int best = queue[lb];
+    for (size_t q = lb; q < le; q++) {
+      int v = queue[q];
+      int dv = xadj[v + 1] - xadj[v], db = xadj[best + 1] - xadj[best];
+      if (dv < db || (dv == db && v < best)) best = v;
+    }
+    return best;
+  }
+
+  void run(std::vector<int>& verts, int k, int base) {
+    if (k <= 1 || verts.size() <= 1) {
+      for (int v : verts) part[v] = base;
+      return;
+    }
+    const int k1 = k / 2;
+    int64_t total = 0;
+    for (int v : verts) total += weight(v);
+    const int64_t target = total * k1 / k;
+    const int64_t tol = std::max<int64_t>(1, total / (50 * (int64_t)k));
+    for (size_t q = 0; q < verts.size(); q++) loc[verts[q]] = (int)q;
+    // side 0 grows from a pseudo-peripheral vertex of the first component,
+    // always taking the frontier vertex with the largest gain (edges into
+    // side 0 minus edges out of it; ties: first discovered)
+    int start = verts[0];
+    for (int r = 0; r < 3; r++) start = far_vertex(start);
+    for (int v : verts) {
+      side[v] = 1;
+      gain[v] = 0;
+    }
+    int64_t w0 = 0;
+    ++stamp;  // mark: discovered (in the heap or on side 0)
+    size_t next = 0;
+    int64_t order = 0;
+    std::priority_queue<std::tuple<int, int64_t, int>> heap;  // (gain, -order, v)
+    auto discover = [&](int v) {
+      mark[v] = stamp;
+      int g = 0;
+      for (int p = xadj[v]; p < xadj[v + 1]; p++)
+        if (loc[adj[p]] >= 0) g += side[adj[p]] == 0 ? 1 : -1;
+      gain[v] = g;
+      disc[v] = order++;
+      heap.emplace(g, -disc[v], v);
+    };
+    discover(start);
+    while (w0 < target) {
+      if (heap.empty()) {  // component exhausted: next undiscovered vertex
+        while (next < verts.size() && mark[verts[next]] == stamp) next++;
+        if (next == verts.size()) break;
+        discover(verts[next]);
+      }
+      auto [g, no, v] = heap.top();
+      heap.pop();
+      if (side[v] == 0 || g != gain[v]) continue;  // stale entry
+      side[v] = 0;
+      w0 += weight(v);
+      for (int p = xadj[v]; p < xadj[v + 1]; p++) {
+        int u = adj[p];
+        if (loc[u] < 0 || side[u] == 0) continue;
+        if (mark[u] != stamp) {
+          discover(u);
+        } else {
+          gain[u] += 2;
+          heap.emplace(gain[u], -disc[u], u);
+        }
+      }
+    }
+    // boundary refinement: move a vertex when it has more neighbours on the
+    // other side and both sides stay within target +- tol
+    for (int pass = 0; pass < 8; pass++) {
+      bool moved = false;
+      for (int v : verts) {
+        int same = 0, other = 0;
+        for (int p = xadj[v]; p < xadj[v + 1]; p++) {
+          int u = adj[p];
+          if (loc[u] < 0) continue;
+          (side[u] == side[v] ? same : other)++;
+        }
+        if (other <= same) continue;
+        const int64_t nw0 = side[v] == 0 ? w0 - weight(v) : w0 + weight(v);
+        if (nw0 < target - tol || nw0 > target + tol) continue;
+        side[v] ^= 1;
+        w0 = nw0;
+        moved = true;
+      }
+      if (!moved) break;
+    }
+    std::vector<int> a, b;
+    for (int v : verts) (side[v] == 0 ? a : b).push_back(v);
+    for (int v : verts) loc[v] = -1;
+    if (a.empty() || b.empty()) {  // cannot split further: share what there is
+      std::vector<int>& all = a.empty() ? b : a;
+      for (size_t q = 0; q < all.size(); q++) part[all[q]] = base + (int)((int64_t)q * k / all.size());
+      return;
+    }
+    verts.clear();
+    verts.shrink_to_fit();
+    run(a, k1, base);
+    run(b, k - k1, base + k1);
+  }
+};
+
+}  // namespace
+
+void partition_graph(int n, const int* xadj, const int* adjncy, const int* vwgt, int nparts,
+                     std::vector<int>& part) {
+  part.assign((size_t)n, 0);
+  if (n <= 0 || nparts <= 1) return;
+  Bisector bis(n, xadj, adjncy, vwgt, part);
+  std::vector<int> verts((size_t)n);
+  for (int v = 0; v < n; v++) verts[v] = v;
+  bis.run(verts, nparts, 0);
+}
+
+void aggregate_graph(const Csr& A, const Aggregates& inner, const Aggregates& full,
+                     std::vector<int>& xadj, std::vector<int>& adjncy) {
+  // SpMtx_buildAggrAdjncy (SpMtx_arrangement.F90:2558-2639): aggregates c1 !=
+  // c2 joined by an entry (i, j), i < j, of the inner aggregation; each pair
+  // once per direction, neighbours in order of discovery
+  const int na = full.nagr, n = A.nrows;
+  std::vector<std::vector<int>> nb((size_t)na);
+  for (int i = 0; i < n; i++)
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      const int j = A.col[p];
+      if (j >= n || i >= j) continue;
+      const int c1 = inner.num[i] - 1, c2 = inner.num[j] - 1;
+      if (c1 < 0 || c2 < 0 || c1 == c2) continue;
+      std::vector<int>& l = nb[c1];
+      if (std::find(l.begin(), l.end(), c2) == l.end()) l.push_back(c2);
+    }
+  xadj.assign((size_t)na + 1, 0);
+  for (int c1 = 0; c1 < na; c1++)
+    for (int c2 : nb[c1]) {
+      xadj[c1 + 1]++;
+      xadj[c2 + 1]++;
+    }
+  for (int c = 0; c < na; c++) xadj[c + 1] += xadj[c];
+  adjncy.assign((size_t)xadj[na], 0);
+  std::vector<int> fill(xadj.begin(), xadj.end() - 1);
+  for (int c1 = 0; c1 < na; c1++)
+    for (int c2 : nb[c1]) {
+      adjncy[fill[c1]++] = c2;
+      adjncy[fill[c2]++] = c1;
+    }
 }
 
 }  // namespace doug

@@ -100,4 +100,16 @@ void add_layers_all(const Csr& A, const std::vector<int>& pstarts, const std::ve
 void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
                std::vector<int>& out);
 
+// Project graph partitioner (stands in for METIS): recursive bisection of a
+// symmetric adjacency (0-based CSR, no self loops) into nparts parts of
+// balanced vertex weight (vwgt may be null: unit weights).  part[v] in
+// [0, nparts).  Deterministic.
+void partition_graph(int n, const int* xadj, const int* adjncy, const int* vwgt, int nparts,
+                     std::vector<int>& part);
+
+// Adjacency of the fine aggregates (SpMtx_buildAggrAdjncy,
+// SpMtx_arrangement.F90:2558-2639) over `full.nagr` vertices.
+void aggregate_graph(const Csr& A, const Aggregates& inner, const Aggregates& full,
+                     std::vector<int>& xadj, std::vector<int>& adjncy);
+
 }  // namespace doug

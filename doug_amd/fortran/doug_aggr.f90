@@ -104,6 +104,10 @@ program doug_aggr
   call check(doug_set_params(ctx, p))
   call check(doug_set_matrix_coo(ctx, int(n, c_int32_t), int(nnz, c_int64_t), indi, indj, val, &
                                  1_c_int32_t))
+  ! DOUG reads num_subdomains only with several processes; a value > 1 selects
+  ! that path's subdomain source (parts of the fine-aggregate graph)
+  if (len_trim(c%partition_file) == 0 .and. c%num_subdomains > 1) &
+    call check(doug_set_num_subdomains(ctx, int(c%num_subdomains, c_int32_t)))
   if (len_trim(c%partition_file) > 0) then
     call read_partition(c%partition_file, n, pstarts, pnodes, ok)
     if (.not. ok) call fatal('Unable to read partition file: '//trim(c%partition_file))

@@ -80,6 +80,13 @@ module doug_capi
       integer(c_int) :: rc
     end function
 
+    function doug_set_num_subdomains(ctx, nsub) bind(C, name="doug_set_num_subdomains") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsub
+      integer(c_int) :: rc
+    end function
+
     function doug_setup(ctx, precision) bind(C, name="doug_setup") result(rc)
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

@@ -95,6 +95,10 @@ class Solver:
         self._check(self._L.doug_set_partition(self.h, len(starts) - 1, _lib.ptr(starts),
                                                _lib.ptr(nodes), int(base)))
 
+    def set_num_subdomains(self, nsub: int):
+        """doug_set_num_subdomains: subdomains from parts of the fine-aggregate graph."""
+        self._check(self._L.doug_set_num_subdomains(self.h, int(nsub)))
+
     def setup(self, precision: int = 0) -> dict:
         self._check(self._L.doug_setup(self.h, int(precision)))
         return self.info()
@@ -247,3 +251,18 @@ class Solver:
         ms = C.c_double(0.0)
         self._check(self._L.doug_time_operator(self.h, int(which), int(reps), C.byref(ms)))
         return ms.value
+
+
+def partition_graph(xadj, adjncy, nparts: int, vwgt=None) -> np.ndarray:
+    """doug_partition_graph on a 0-based symmetric adjacency: part number per vertex."""
+    L = _lib.load()
+    xadj = np.ascontiguousarray(xadj, np.int32)
+    adjncy = np.ascontiguousarray(adjncy, np.int32)
+    n = len(xadj) - 1
+    w = None if vwgt is None else np.ascontiguousarray(vwgt, np.int32)
+    part = np.empty(n, np.int32)
+    rc = L.doug_partition_graph(n, _lib.ptr(xadj), _lib.ptr(adjncy),
+                                None if w is None else _lib.ptr(w), int(nparts), 0, _lib.ptr(part))
+    if rc:
+        raise DougError(rc, "doug_partition_graph failed")
+    return part

@@ -101,6 +101,26 @@ int doug_set_matrix_coo(doug_ctx *ctx, int32_t n, int64_t nnz, const int32_t *in
 int doug_set_partition(doug_ctx *ctx, int32_t nparts, const int32_t *starts,
                        const int32_t *nodes, int32_t index_base);
 
+/* Subdomains from `nsub` parts of the fine-aggregate adjacency graph, the
+ * subdomain source of DOUG's multi-process runs: Partitionings_metis_InitCoarse
+ * (src/components/Partitioning_metis.F90:34-86) over SpMtx_buildAggrAdjncy
+ * (src/datatypes/SpMtx/SpMtx_arrangement.F90:2558-2639), with the project
+ * partitioner (doug_partition_graph) in place of METIS; each subdomain lists
+ * its nodes in node order.  Used when no partition is set and nsub > 1;
+ * 0 or 1 restores the one-process rule. */
+int doug_set_num_subdomains(doug_ctx *ctx, int32_t nsub);
+
+/* The project graph partitioner, standing in for METIS_PartGraphKway
+ * (called from src/datatypes/Graph.f90:295-315): recursive bisection by
+ * breadth-first growth from a pseudo-peripheral vertex plus boundary
+ * refinement, balancing the vertex weights (vwgt may be NULL) to within 2 %
+ * of a part's share.  xadj/adjncy: symmetric adjacency without self loops,
+ * indices in `index_base`; part[v] receives a part number in index_base ..
+ * index_base + nparts - 1.  Host only, deterministic. */
+int doug_partition_graph(int32_t n, const int32_t *xadj, const int32_t *adjncy,
+                         const int32_t *vwgt, int32_t nparts, int32_t index_base,
+                         int32_t *part);
+
 /* Builds the preconditioner: Partitionings_aggr_InitFine,
  * CoarsePreconditioner_smooth_Init, Partitionings_aggr_InitCoarse,
  * FinePreconditioner_Init and the factorisations of Factorise_subdomains and

@@ -296,3 +296,30 @@ def test_device_coarse_build_bits(case):
                 h.set_partition(*part)
             h.setup()
             assert _bits_equal(h.prec(r), z)
+
+
+def test_num_subdomains_matches_oracle_on_same_parts():
+    # subdomains from parts of the fine-aggregate graph (the multi-process
+    # subdomain source): the oracle, given the same non-overlapping parts,
+    # builds the same index sets and takes the same iterations
+    n, i, j, v = P.diffusion3d(20, 1.0, 14)
+    b = P.uniform_rhs(n, 15)
+    print(f"num_subdomains n={n}", flush=True)
+    with D.Solver(-1) as h:
+        h.set_params(**dict(CFG0_CTL, overlap=0))
+        h.set_matrix(n, i, j, v, 1)
+        h.set_num_subdomains(6)
+        h.setup_host()
+        cores = h.subdomains()
+    o = O.Oracle(n, i, j, v, O.params(**CFG0_CTL), partition=cores)
+    with D.Solver(0) as s:
+        s.set_params(**CFG0_CTL)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_num_subdomains(6)
+        s.setup()
+        for a, c in zip(o.subdomains(), s.subdomains()):
+            np.testing.assert_array_equal(a, c)
+        x, it, _, _ = s.pcg(b, 1e-8, 500)
+        xo, ito, _, _ = o.pcg(b, 1e-8, 500)
+        assert it == ito
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
