@@ -81,6 +81,139 @@ def cpu_baseline(size, block, sigma, seed):
                       f"scaled x{scale:.0f} by unknown count"}
 
 
+FEM_CTL = dict(radius1=5, minasize1=60, maxasize1=121, strong1=0.08, smoothers=1, overlap=2,
+               symmstruct=1, symmnumeric=1)
+
+
+def fem_batched_leg(npoints, nsub, nrhs, steps, warmup, seed):
+    """Batching form on the unstructured workload: P1 FEM Poisson on the
+    Delaunay mesh of `npoints` seeded points (2 % clustered near a circle),
+    `nsub` subdomains from parts of the fine-aggregate graph, overlap 2, FP32
+    subdomain factors, `nrhs` right-hand sides in one batched PCG to 1e-8 with
+    the graph-captured iteration; a single-RHS FP64 solve alongside."""
+    import torch
+    import doug_amd as D
+    from doug_amd import problems as P
+
+    t0 = time.perf_counter()
+    n, i, j, v, _ = P.fem_p1_square(npoints, seed)
+    log(f"[bench] fem mesh: n={n} nnz={len(v)} in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    s = D.Solver(0)
+    s.set_params(**FEM_CTL)
+    s.set_matrix(n, i, j, v, 1)
+    s.set_num_subdomains(nsub)
+    B = torch.tensor(np.stack([P.uniform_rhs(n, seed + 100 + k) for k in range(nrhs)], axis=1),
+                     dtype=torch.float64, device=dev).contiguous()
+    X = torch.zeros_like(B)
+    s.setup(1)
+    tune = s.autotune(20)
+    times = []
+    for k in range(warmup + steps):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        s.setup(1)
+        s.prepare_batch()
+        tb = time.perf_counter()
+        X.zero_()
+        its = s.pcg_batch_device(nrhs, B.data_ptr(), X.data_ptr(), 1e-8, 2000)
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        if k >= warmup:
+            times.append((tb - ta, tc - tb))
+        log(f"[bench] fem batched {nrhs} rhs: setup {tb - ta:.3f}s solve {tc - tb:.3f}s "
+            f"iterations {int(its.min())}-{int(its.max())}")
+    info = s.info()
+    sizes = np.diff(s.subdomains()[0])
+    # relative residual of every column
+    res = []
+    for k in range(nrhs):
+        xk = X[:, k].contiguous()
+        r = torch.empty_like(xk)
+        s.spmv_device(xk.data_ptr(), r.data_ptr())
+        res.append(float(torch.linalg.norm(B[:, k] - r) / torch.linalg.norm(B[:, k])))
+    prec_ms = s.time_operator(4, 3)
+    # single right-hand side, plain FP64
+    b0 = B[:, 0].contiguous()
+    x0 = torch.zeros_like(b0)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    s.setup(0)
+    tb = time.perf_counter()
+    it64, _, _ = s.pcg_device(b0.data_ptr(), x0.data_ptr(), 1e-8, 2000)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    s.close()
+    setup_s = float(np.mean([a for a, _ in times]))
+    solve_s = float(np.mean([c for _, c in times]))
+    per_rhs = (setup_s + solve_s) / nrhs
+    return {"workload": f"P1 FEM Poisson, Delaunay mesh of {npoints} seeded points in the unit square "
+                        f"(2% near a circle), {info['nsub']} subdomains (parts of the fine-aggregate "
+                        f"graph), overlap 2, FP32 factors, {nrhs} RHS, PCG rtol 1e-8",
+            "n": n, "nnz": int(len(v)), "ncoarse": info["ncoarse"],
+            "subdomain_sizes_min_median_max": [int(sizes.min()), int(np.median(sizes)), int(sizes.max())],
+            "setup_s": round(setup_s, 4), "solve_s": round(solve_s, 4),
+            "time_to_solve_all_s": round(setup_s + solve_s, 4), "time_per_rhs_s": round(per_rhs, 5),
+            "iterations_min_max": [int(its.min()), int(its.max())],
+            "rel_residual_max": max(res), "prec_apply_ms": round(prec_ms, 3),
+            "fp64_single": {"setup_s": round(tb - ta, 4), "solve_s": round(tc - tb, 4),
+                            "time_to_solve_s": round(tc - ta, 4), "iterations": int(it64)},
+            "speedup_per_rhs_vs_fp64_single": round((tc - ta) / per_rhs, 2),
+            "autotune": {k: tune[k] for k in ("spmv_rows", "spmv_xcd_order", "trsv_order")}}
+
+
+def differentiation_leg(N, block, ndir, seed):
+    """Differentiation form: loss 0.5 |x(theta) - x_target|^2 for 3-D diffusion
+    on N^3 with per-cell log-coefficients theta, forward and adjoint PCG to
+    1e-10; the adjoint gradient against central differences (h = 1e-6) along
+    `ndir` random directions."""
+    import torch
+    from doug_amd import problems as P
+    from doug_amd.autograd import DougSolve, PatternSolver, diffusion3d_pattern, diffusion3d_values
+
+    dev = torch.device("cuda:0")
+    n, indi, indj = diffusion3d_pattern(N)
+    ctl = dict(radius1=2, strong1=0.08, smoothers=1, overlap=1, symmstruct=1, symmnumeric=1)
+    ps = PatternSolver(n, indi, indj, ctl, partition=P.block_partition((N,) * 3, block), tol=1e-10)
+    theta = torch.tensor(0.5 * P.smoothed_gaussian(N, seed).ravel(), dtype=torch.float64, device=dev)
+    theta_t = torch.tensor(0.5 * P.smoothed_gaussian(N, seed + 1).ravel(), dtype=torch.float64, device=dev)
+    b = torch.tensor(P.uniform_rhs(n, seed + 2), dtype=torch.float64, device=dev)
+
+    def loss_of(th):
+        x = DougSolve.apply(diffusion3d_values(th, N), b, ps)
+        return 0.5 * torch.sum((x - x_target) ** 2)
+
+    with torch.no_grad():
+        x_target = DougSolve.apply(diffusion3d_values(theta_t, N), b, ps)
+    th = theta.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = loss_of(th)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = th.grad.detach()
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    h = 1e-6
+    errs = []
+    with torch.no_grad():
+        for _ in range(ndir):
+            d = torch.randn(n, generator=gen, dtype=torch.float64).to(dev)
+            d /= torch.linalg.norm(d)
+            fd = (loss_of(theta + h * d) - loss_of(theta - h * d)).item() / (2 * h)
+            an = float(torch.dot(g, d))
+            errs.append(abs(fd - an) / max(abs(an), 1e-300))
+    ps.close()
+    return {"workload": f"3d diffusion {N}^3, theta = 0.5 g (seeded smoothed Gaussian), "
+                        f"{(N // block) ** 3} subdomains, forward + adjoint PCG rtol 1e-10",
+            "n": n, "forward_s": round(t1 - t0, 4), "backward_s": round(t2 - t1, 4),
+            "iterations_forward_adjoint": ps.iterations[1:3],
+            "fd_directions": ndir, "fd_h": h, "fd_rel_err_max": max(errs) if errs else None,
+            "fd_rel_err_median": float(np.median(errs)) if errs else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +228,10 @@ def main():
                     help="kernels: one setup, then a few launches of the copy, SpMV and "
                          "subdomain-solve kernels only (for hardware-counter runs)")
     ap.add_argument("--precision", type=int, default=1, help="factor precision for --mode kernels")
+    ap.add_argument("--fem-points", type=int, default=1_000_000,
+                    help="points of the FEM batching workload (0: skip it)")
+    ap.add_argument("--diff-size", type=int, default=64,
+                    help="grid edge of the differentiation workload (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the whole solve "
@@ -293,10 +430,16 @@ def main():
         "autotune": tune,
         "roofline": roof,
     }
+    s.close()
+    if args.fem_points > 0:
+        out["fem_batched_32rhs"] = fem_batched_leg(args.fem_points, 256, 32, min(args.steps, 2),
+                                                   min(args.warmup, 1), args.seed + 7)
+    if args.diff_size > 0:
+        log("[bench] differentiation ...")
+        out["differentiation"] = differentiation_leg(args.diff_size, 16, 20, args.seed + 11)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(N, blk, args.sigma, args.seed)
-    s.close()
     print(json.dumps(out), flush=True)
 
 

@@ -117,3 +117,58 @@ def block_partition(shape, block):
 
 def uniform_rhs(n: int, seed: int) -> np.ndarray:
     return rng(seed).uniform(-1.0, 1.0, n)
+
+
+def fem_p1_square(npoints: int, seed: int, circle_fraction: float = 0.02):
+    """P1 finite-element Poisson (-Laplace u = f, u = 0 on the boundary) on the
+    Delaunay triangulation of seeded points in the unit square: uniform points,
+    plus `circle_fraction` of them clustered within 0.002 of a random circle
+    (local refinement), plus evenly spaced points on the four sides.  Element
+    matrices K_e = area * grad(phi_i) . grad(phi_j) are summed into the assembled
+    matrix (the elemental-to-assembled step of DOUG's input_type 1), boundary
+    unknowns eliminated.  Returns (n, indi, indj, val) 1-based, rows ascending,
+    columns ascending within a row, no duplicates, and the interior points."""
+    from scipy.sparse import coo_matrix
+    from scipy.spatial import Delaunay
+
+    g = rng(seed)
+    m = max(2, int(round(np.sqrt(npoints))))  # points per side
+    n_circle = int(npoints * circle_fraction)
+    n_uniform = max(0, npoints - n_circle - 4 * (m - 1))
+    inner = g.uniform(0.0, 1.0, (n_uniform, 2))
+    center = g.uniform(0.3, 0.7, 2)
+    radius = g.uniform(0.1, 0.25)
+    theta = g.uniform(0.0, 2.0 * np.pi, n_circle)
+    rr = radius + g.normal(0.0, 0.002, n_circle)
+    ring = center + rr[:, None] * np.stack([np.cos(theta), np.sin(theta)], axis=1)
+    ring = ring[(ring > 1e-6).all(axis=1) & (ring < 1.0 - 1e-6).all(axis=1)]
+    t = np.linspace(0.0, 1.0, m, endpoint=False)
+    side = np.concatenate([np.stack([t, np.zeros(m)], 1), np.stack([np.ones(m), t], 1),
+                           np.stack([1.0 - t, np.ones(m)], 1), np.stack([np.zeros(m), 1.0 - t], 1)])
+    pts = np.concatenate([side, inner, ring])
+    tri = Delaunay(pts).simplices
+    p0, p1, p2 = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    det = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])
+    area = 0.5 * np.abs(det)
+    keep = area > 1e-14
+    tri, area, det = tri[keep], area[keep], det[keep]
+    P = pts[tri]  # (ne, 3, 2)
+    # gradients of the barycentric functions: grad phi_i = perp(p_k - p_j) / det
+    grads = np.empty((len(tri), 3, 2))
+    for i in range(3):
+        j, k = (i + 1) % 3, (i + 2) % 3
+        grads[:, i, 0] = (P[:, j, 1] - P[:, k, 1]) / det
+        grads[:, i, 1] = (P[:, k, 0] - P[:, j, 0]) / det
+    Ke = area[:, None, None] * np.einsum("eid,ejd->eij", grads, grads)
+    rows = np.repeat(tri, 3, axis=1).ravel()
+    cols = np.tile(tri, (1, 3)).ravel()
+    npts = len(pts)
+    K = coo_matrix((Ke.ravel(), (rows, cols)), shape=(npts, npts)).tocsr()
+    boundary = (pts <= 1e-12).any(axis=1) | (pts >= 1.0 - 1e-12).any(axis=1)
+    interior = np.flatnonzero(~boundary)
+    K = K[interior][:, interior].tocsr()
+    K.sort_indices()
+    K.sum_duplicates()
+    coo = K.tocoo()
+    return (len(interior), (coo.row + 1).astype(np.int32), (coo.col + 1).astype(np.int32),
+            coo.data.astype(np.float64), pts[interior])

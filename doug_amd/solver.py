@@ -8,6 +8,7 @@ libdoug_hip.so; errors raise DougError, the Python counterpart of DOUG_abort.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 
 import numpy as np
 
@@ -37,6 +38,15 @@ def make_params(**kw) -> _lib.DougParams:
             raise KeyError(f"unknown control {k!r}")
         setattr(p, k, int(v) if k in CTL_INT_KEYS or k in ("symmstruct", "symmnumeric") else float(v))
     return p
+
+
+def _fence():
+    """Finish torch's queued device work before the library reads or writes
+    device pointers: its kernels run on their own stream, which does not wait
+    for torch's stream (a torch copy or fill may otherwise still be running)."""
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        torch.cuda.synchronize()
 
 
 class Solver:
@@ -183,12 +193,15 @@ class Solver:
 
     # -- operators (device pointers, e.g. torch.Tensor.data_ptr()) --
     def spmv_device(self, dx: int, dy: int):
+        _fence()
         self._check(self._L.doug_spmv_device(self.h, C.c_void_p(dx), C.c_void_p(dy)))
 
     def prec_device(self, dr: int, dz: int):
+        _fence()
         self._check(self._L.doug_prec_apply_device(self.h, C.c_void_p(dr), C.c_void_p(dz)))
 
     def pcg_device(self, db: int, dx: int, tol=1e-8, maxit=-1):
+        _fence()
         cap = maxit if maxit > 0 else 100000
         hist = np.zeros(cap)
         it = C.c_int32(0)
@@ -203,6 +216,7 @@ class Solver:
 
     def value_grad_device(self, dlam: int, dx: int, dgrad: int):
         """dgrad[e] = -lam[i_e] * x[j_e] for every input entry e (device pointers)."""
+        _fence()
         self._check(self._L.doug_value_grad_device(self.h, C.c_void_p(dlam), C.c_void_p(dx),
                                                    C.c_void_p(dgrad)))
 
@@ -227,12 +241,14 @@ class Solver:
         return X, its, hist[: int(its.max(initial=0))]
 
     def pcg_batch_device(self, nrhs: int, dB: int, dX: int, tol=1e-8, maxit=-1):
+        _fence()
         its = np.zeros(nrhs, np.int32)
         self._check(self._L.doug_pcg_batch_device(self.h, int(nrhs), C.c_void_p(dB), C.c_void_p(dX),
                                                   float(tol), int(maxit), _lib.ptr(its), None))
         return its
 
     def prec_batch_device(self, nrhs: int, dR: int, dZ: int):
+        _fence()
         self._check(self._L.doug_prec_apply_batch_device(self.h, int(nrhs), C.c_void_p(dR), C.c_void_p(dZ)))
 
     def autotune(self, reps: int = 20) -> dict:

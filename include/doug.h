@@ -147,6 +147,10 @@ int doug_get_coarse_matrix(const doug_ctx *ctx, int32_t *rowptr, int32_t *cols, 
 int64_t doug_restriction_nnz(const doug_ctx *ctx);
 int64_t doug_coarse_matrix_nnz(const doug_ctx *ctx);
 
+/* The *_device entry points take device pointers and run on the context's
+ * own stream, returning when it has drained; that stream does not wait for
+ * work on other streams, so the inputs must be complete when they are called. */
+
 /* y = A x (SpMtx_Ax / Distribution_pmvm, SpMtx_op_Ax.f90:44-101) */
 int doug_spmv(doug_ctx *ctx, const double *x, double *y);
 int doug_spmv_device(doug_ctx *ctx, const double *dx, double *dy);

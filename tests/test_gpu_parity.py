@@ -323,3 +323,23 @@ def test_num_subdomains_matches_oracle_on_same_parts():
         xo, ito, _, _ = o.pcg(b, 1e-8, 500)
         assert it == ito
         assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_batched_graph_replays_match_eager():
+    n, i, j, v = P.diffusion3d(16, 1.0, 16)
+    part = P.block_partition((16,) * 3, 8)
+    B = np.stack([P.uniform_rhs(n, 200 + k) for k in range(12)], axis=1)
+    print(f"batched graph n={n}", flush=True)
+    runs = {}
+    for graph in (0, 1):
+        with D.Solver(0) as s:
+            s.set_option(1, graph)
+            s.set_params(**CFG0_CTL)
+            s.set_matrix(n, i, j, v, 1)
+            s.set_partition(*part)
+            s.setup(1)
+            s.prepare_batch()
+            runs[graph] = [s.pcg_batch(B[:, :k], 1e-8, 500) for k in (12, 5, 12)]
+    for (Xe, ie, he), (Xg, ig, hg) in zip(runs[0], runs[1]):
+        np.testing.assert_array_equal(ie, ig)
+        assert _bits_equal(Xe, Xg)
