@@ -119,6 +119,30 @@ def uniform_rhs(n: int, seed: int) -> np.ndarray:
     return rng(seed).uniform(-1.0, 1.0, n)
 
 
+def p1_stiffness(pts: np.ndarray, tri: np.ndarray):
+    """Assembled P1 stiffness matrix (scipy CSR, duplicates summed) of a
+    triangulation: K_e = area * grad(phi_i) . grad(phi_j), with grad(phi_i) =
+    perp(p_k - p_j) / det for the cyclic vertex order (i, j, k).  Triangles of
+    area below 1e-14 are skipped."""
+    from scipy.sparse import coo_matrix
+
+    p0, p1, p2 = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    det = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])
+    area = 0.5 * np.abs(det)
+    keep = area > 1e-14
+    tri, area, det = tri[keep], area[keep], det[keep]
+    P = pts[tri]  # (ne, 3, 2)
+    grads = np.empty((len(tri), 3, 2))
+    for i in range(3):
+        j, k = (i + 1) % 3, (i + 2) % 3
+        grads[:, i, 0] = (P[:, j, 1] - P[:, k, 1]) / det
+        grads[:, i, 1] = (P[:, k, 0] - P[:, j, 0]) / det
+    Ke = area[:, None, None] * np.einsum("eid,ejd->eij", grads, grads)
+    rows = np.repeat(tri, 3, axis=1).ravel()
+    cols = np.tile(tri, (1, 3)).ravel()
+    return coo_matrix((Ke.ravel(), (rows, cols)), shape=(len(pts), len(pts))).tocsr()
+
+
 def fem_p1_square(npoints: int, seed: int, circle_fraction: float = 0.02):
     """P1 finite-element Poisson (-Laplace u = f, u = 0 on the boundary) on the
     Delaunay triangulation of seeded points in the unit square: uniform points,
@@ -128,7 +152,6 @@ def fem_p1_square(npoints: int, seed: int, circle_fraction: float = 0.02):
     matrix (the elemental-to-assembled step of DOUG's input_type 1), boundary
     unknowns eliminated.  Returns (n, indi, indj, val) 1-based, rows ascending,
     columns ascending within a row, no duplicates, and the interior points."""
-    from scipy.sparse import coo_matrix
     from scipy.spatial import Delaunay
 
     g = rng(seed)
@@ -147,23 +170,7 @@ def fem_p1_square(npoints: int, seed: int, circle_fraction: float = 0.02):
                            np.stack([1.0 - t, np.ones(m)], 1), np.stack([np.zeros(m), 1.0 - t], 1)])
     pts = np.concatenate([side, inner, ring])
     tri = Delaunay(pts).simplices
-    p0, p1, p2 = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
-    det = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1])
-    area = 0.5 * np.abs(det)
-    keep = area > 1e-14
-    tri, area, det = tri[keep], area[keep], det[keep]
-    P = pts[tri]  # (ne, 3, 2)
-    # gradients of the barycentric functions: grad phi_i = perp(p_k - p_j) / det
-    grads = np.empty((len(tri), 3, 2))
-    for i in range(3):
-        j, k = (i + 1) % 3, (i + 2) % 3
-        grads[:, i, 0] = (P[:, j, 1] - P[:, k, 1]) / det
-        grads[:, i, 1] = (P[:, k, 0] - P[:, j, 0]) / det
-    Ke = area[:, None, None] * np.einsum("eid,ejd->eij", grads, grads)
-    rows = np.repeat(tri, 3, axis=1).ravel()
-    cols = np.tile(tri, (1, 3)).ravel()
-    npts = len(pts)
-    K = coo_matrix((Ke.ravel(), (rows, cols)), shape=(npts, npts)).tocsr()
+    K = p1_stiffness(pts, tri)
     boundary = (pts <= 1e-12).any(axis=1) | (pts >= 1.0 - 1e-12).any(axis=1)
     interior = np.flatnonzero(~boundary)
     K = K[interior][:, interior].tocsr()

@@ -343,3 +343,37 @@ def test_batched_graph_replays_match_eager():
     for (Xe, ie, he), (Xg, ig, hg) in zip(runs[0], runs[1]):
         np.testing.assert_array_equal(ie, ig)
         assert _bits_equal(Xe, Xg)
+
+
+def test_fem_workload_against_oracle_and_batched():
+    # the P1 FEM workload at small size: single FP64 solve equals the oracle
+    # on the same parts of the fine-aggregate graph; the FP32 batch meets
+    # the FP64 tolerance on every column
+    n, i, j, v, _ = P.fem_p1_square(20000, 9)
+    ctl = dict(radius1=4, minasize1=40, maxasize1=81, strong1=0.08, smoothers=1, overlap=2,
+               symmstruct=1, symmnumeric=1)
+    print(f"fem n={n}", flush=True)
+    with D.Solver(-1) as h:
+        h.set_params(**dict(ctl, overlap=0))
+        h.set_matrix(n, i, j, v, 1)
+        h.set_num_subdomains(8)
+        h.setup_host()
+        cores = h.subdomains()
+    o = O.Oracle(n, i, j, v, O.params(**ctl), partition=cores)
+    b = P.uniform_rhs(n, 3)
+    with D.Solver(0) as s:
+        s.set_params(**ctl)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_num_subdomains(8)
+        s.setup(0)
+        x, it, _, _ = s.pcg(b, 1e-8, 1000)
+        xo, ito, _, _ = o.pcg(b, 1e-8, 1000)
+        assert it == ito
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        s.setup(1)
+        s.prepare_batch()
+        B = np.stack([P.uniform_rhs(n, 40 + k) for k in range(32)], axis=1)
+        X, its, _ = s.pcg_batch(B, 1e-8, 1000)
+        for k in range(32):
+            r = B[:, k] - o.spmv(X[:, k])
+            assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, k]) * 1.01, k
