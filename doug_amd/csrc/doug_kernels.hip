@@ -845,19 +845,92 @@ __global__ __launch_bounds__(256) void k_wgemm_bwd(const T* __restrict__ W,
     }
 }
 
+// Register staging of 64x64 tiles and 64xNR panels: every thread loads its
+// share with 16-byte loads, so that all loads of a step are in flight before
+// the first LDS store.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float4 type;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double2 type;
+  static constexpr int n = 2;
+};
+
+template <typename T>
+struct TileRegs {
+  static constexpr int V = Vec16<T>::n, N = kTileElems / (256 * V);
+  typename Vec16<T>::type r[N];
+  __device__ __forceinline__ void load(const T* __restrict__ g) {
+#pragma unroll
+    for (int q = 0; q < N; q++)
+      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + 256 * q];
+  }
+  __device__ __forceinline__ void store(T* __restrict__ S) const {
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      const int e = (threadIdx.x + 256 * q) * V, row = e >> 6, col = e & 63;
+      const T* v = reinterpret_cast<const T*>(&r[q]);
+#pragma unroll
+      for (int c = 0; c < V; c++) S[row * LD + col + c] = v[c];
+    }
+  }
+};
+
+template <typename T, int NR>
+struct PanelRegs {
+  static constexpr int V = Vec16<T>::n, N = kTile * NR / (256 * V);
+  typename Vec16<T>::type r[N];
+  __device__ __forceinline__ void load(const T* __restrict__ g) {
+#pragma unroll
+    for (int q = 0; q < N; q++)
+      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + 256 * q];
+  }
+  __device__ __forceinline__ void store(T* __restrict__ S) const {
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+      const int e = (threadIdx.x + 256 * q) * V, row = e / NR, col = e % NR;
+      const T* v = reinterpret_cast<const T*>(&r[q]);
+#pragma unroll
+      for (int c = 0; c < V; c++) S[row * (NR + 1) + col + c] = v[c];
+    }
+  }
+};
+
+// acc[c] += A(16w.., :) * B(:, 16c..) from LDS; TR: use A^T
+template <typename T, int NR, bool TR>
+__device__ __forceinline__ void panel_mma(typename Mma<T>::v4 (&acc)[NR / 16], const T* As,
+                                          const T* Bs, int w, int li, int lk) {
+#pragma unroll 4
+  for (int ks = 0; ks < 16; ks++) {
+    const int kk = 4 * ks + lk;
+    const T a = TR ? As[kk * LD + 16 * w + li] : As[(16 * w + li) * LD + kk];
+#pragma unroll
+    for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+  }
+}
+
 // Envelope forward/backward substitution for NR columns, one workgroup per
 // block.  Wave w owns rows 16w..16w+15 of each 64 x NR panel; panels of
 // finished tile rows live in U (global, re-read through L2); the workgroup's
-// barriers order its own global writes before its later reads.
-template <typename T, int NR>
+// barriers order its own global writes before its later reads.  Each step
+// stages KB off-diagonal tiles with their panels, and the first step of a
+// tile row also the diagonal tile, with all loads issued before any LDS
+// store.
+template <typename T, int NR, int KB>
 __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
                                                     const BlockDesc* __restrict__ d,
                                                     const int32_t* __restrict__ inds,
                                                     const double* __restrict__ rhs, int ldr,
                                                     T* __restrict__ U, double* __restrict__ out,
                                                     int ldo, int nrhs) {
-  __shared__ T As[kTile * LD];
-  __shared__ T Bs[kTile * (NR + 1)];
+  __shared__ T As[KB][kTile * LD];
+  __shared__ T Ds[kTile * LD];
+  __shared__ T Bs[KB][kTile * (NR + 1)];
   typedef typename Mma<T>::v4 V;
   const BlockDesc b = d[blockIdx.x];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
@@ -868,17 +941,30 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
   for (int i = 0; i < b.nt; i++) {
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-    for (int k = b.tr[i].fc; k < i; k++) {
-      tile_to_lds(As, Lb + toff(b, i, k));
-      panel_to_lds<T, NR>(Bs, Ub + (int64_t)k * kTile * NR);
-      __syncthreads();
-#pragma unroll 4
-      for (int ks = 0; ks < 16; ks++) {
-        const int kk = 4 * ks + lk;
-        const T a = As[(16 * w + li) * LD + kk];
+    const int fc = b.tr[i].fc;
+    bool diag_staged = false;
+    for (int k = fc; k < i || !diag_staged; k += KB) {
+      TileRegs<T> ra[KB], rd;
+      PanelRegs<T, NR> rb[KB];
+      if (!diag_staged) rd.load(Lb + toff(b, i, i));
 #pragma unroll
-        for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
-      }
+      for (int u = 0; u < KB; u++)
+        if (k + u < i) {
+          ra[u].load(Lb + toff(b, i, k + u));
+          rb[u].load(Ub + (int64_t)(k + u) * kTile * NR);
+        }
+      if (!diag_staged) rd.store(Ds);
+#pragma unroll
+      for (int u = 0; u < KB; u++)
+        if (k + u < i) {
+          ra[u].store(As[u]);
+          rb[u].store(Bs[u]);
+        }
+      diag_staged = true;
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < KB; u++)
+        if (k + u < i) panel_mma<T, NR, false>(acc, As[u], Bs[u], w, li, lk);
       __syncthreads();
     }
 #pragma unroll
@@ -889,19 +975,12 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
         const int p = i * kTile + row;
         T v = (T)0;
         if (p < b.s && col < nrhs) v = (T)rhs[(int64_t)inds[b.vec_off + p] * ldr + col];
-        Bs[row * (NR + 1) + col] = v - acc[c][r];
+        Bs[0][row * (NR + 1) + col] = v - acc[c][r];
       }
-    tile_to_lds(As, Lb + toff(b, i, i));  // inv(L_ii)
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-#pragma unroll 4
-    for (int ks = 0; ks < 16; ks++) {
-      const int kk = 4 * ks + lk;
-      const T a = As[(16 * w + li) * LD + kk];
-#pragma unroll
-      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
-    }
+    panel_mma<T, NR, false>(acc, Ds, Bs[0], w, li, lk);
     T* Ui = Ub + (int64_t)i * kTile * NR;
 #pragma unroll
     for (int c = 0; c < NR / 16; c++)
@@ -914,17 +993,29 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
     const int lr = b.tr[j].lr;
-    for (int i = j + 1; i <= lr; i++) {
-      tile_to_lds(As, Lb + toff(b, i, j));
-      panel_to_lds<T, NR>(Bs, Ub + (int64_t)i * kTile * NR);
-      __syncthreads();
-#pragma unroll 4
-      for (int ks = 0; ks < 16; ks++) {
-        const int kk = 4 * ks + lk;
-        const T a = As[kk * LD + 16 * w + li];  // (L_ij)^T
+    bool diag_staged = false;
+    for (int i = j + 1; i <= lr || !diag_staged; i += KB) {
+      TileRegs<T> ra[KB], rd;
+      PanelRegs<T, NR> rb[KB];
+      if (!diag_staged) rd.load(Lb + toff(b, j, j));
 #pragma unroll
-        for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
-      }
+      for (int u = 0; u < KB; u++)
+        if (i + u <= lr) {
+          ra[u].load(Lb + toff(b, i + u, j));
+          rb[u].load(Ub + (int64_t)(i + u) * kTile * NR);
+        }
+      if (!diag_staged) rd.store(Ds);
+#pragma unroll
+      for (int u = 0; u < KB; u++)
+        if (i + u <= lr) {
+          ra[u].store(As[u]);
+          rb[u].store(Bs[u]);
+        }
+      diag_staged = true;
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < KB; u++)
+        if (i + u <= lr) panel_mma<T, NR, true>(acc, As[u], Bs[u], w, li, lk);
       __syncthreads();
     }
     T* Uj = Ub + (int64_t)j * kTile * NR;
@@ -933,19 +1024,12 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
-        Bs[row * (NR + 1) + col] = Uj[row * NR + col] - acc[c][r];
+        Bs[0][row * (NR + 1) + col] = Uj[row * NR + col] - acc[c][r];
       }
-    tile_to_lds(As, Lb + toff(b, j, j));  // inv(L_jj)
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-#pragma unroll 4
-    for (int ks = 0; ks < 16; ks++) {
-      const int kk = 4 * ks + lk;
-      const T a = As[kk * LD + 16 * w + li];  // inv(L_jj)^T
-#pragma unroll
-      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
-    }
+    panel_mma<T, NR, true>(acc, Ds, Bs[0], w, li, lk);  // inv(L_jj)^T
 #pragma unroll
     for (int c = 0; c < NR / 16; c++)
 #pragma unroll
@@ -1076,6 +1160,41 @@ __global__ __launch_bounds__(256) void k_dot_multi(const double* __restrict__ a,
   finish_multi(partial, counter, nrhs, mode, scal, sh);
 }
 
+// Column pairs (nrhs even): 16-byte loads, thread t owns columns 2(t % (nrhs/2)) and +1.
+__device__ __forceinline__ void block_colsums2(double v0, double v1, int nrhs, double* sh,
+                                               double* partial) {
+  const int t = threadIdx.x, nr2 = nrhs >> 1;
+  sh[t] = v0;
+  sh[256 + t] = v1;
+  __syncthreads();
+  if (t < nrhs) {
+    const double* h = sh + (t & 1) * 256;
+    double s = 0.0;
+    for (int q = t >> 1; q < 256; q += nr2) s += h[q];
+    partial[blockIdx.x * kMultiMax + t] = s;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_dot_multi2(const double2* __restrict__ a,
+                                                    const double2* __restrict__ b, int64_t total2,
+                                                    int nrhs, double* partial, unsigned* counter,
+                                                    int mode, double* scal) {
+  __shared__ double sh[512];
+  const int nr2 = nrhs >> 1;
+  const int64_t stride = ((int64_t)gridDim.x * 256 / nr2) * nr2;
+  const int64_t start = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v0 = 0.0, v1 = 0.0;
+  if (start < stride)
+    for (int64_t e = start; e < total2; e += stride) {
+      const double2 x = a[e], y = b[e];
+      v0 = v0 + x.x * y.x;
+      v1 = v1 + x.y * y.y;
+    }
+  block_colsums2(v0, v1, nrhs, sh, partial);
+  finish_multi(partial, counter, nrhs, mode, scal, sh);
+}
+
 __global__ void k_p_update_multi(int64_t total, int nrhs, const double* __restrict__ z,
                                  double* __restrict__ p, const double* __restrict__ scal,
                                  int first) {
@@ -1115,6 +1234,50 @@ __global__ __launch_bounds__(256) void k_xr_update_multi(int64_t total, int nrhs
     }
   }
   block_colsums(v, nrhs, sh, partial);
+  finish_multi(partial, counter, nrhs, 3, scal, sh);
+}
+
+__global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nrhs,
+                                                          double2* __restrict__ x,
+                                                          double2* __restrict__ r,
+                                                          const double2* __restrict__ p,
+                                                          const double2* __restrict__ q,
+                                                          double* partial, unsigned* counter,
+                                                          double* scal) {
+  __shared__ double sh[512];
+  const int nr2 = nrhs >> 1;
+  const int64_t stride = ((int64_t)gridDim.x * 256 / nr2) * nr2;
+  const int64_t start = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v0 = 0.0, v1 = 0.0;
+  if (start < stride) {
+    const int c0 = 2 * (int)(start % nr2);
+    const double* s0 = scal + c0 * 8;
+    const double* s1 = s0 + 8;
+    // a frozen column keeps x and r (alpha 0 leaves them unchanged)
+    const double a0 = s0[SM_ACTIVE] != 0.0 ? s0[S_ALPHA] : 0.0;
+    const double a1 = s1[SM_ACTIVE] != 0.0 ? s1[S_ALPHA] : 0.0;
+    const bool any = s0[SM_ACTIVE] != 0.0 || s1[SM_ACTIVE] != 0.0;
+    for (int64_t e = start; e < total2; e += stride) {
+      double2 ri = r[e];
+      if (any) {
+        const double2 pe = p[e], qe = q[e];
+        double2 xe = x[e];
+        if (s0[SM_ACTIVE] != 0.0) {
+          xe.x = xe.x + a0 * pe.x;
+          ri.x = ri.x - a0 * qe.x;
+        }
+        if (s1[SM_ACTIVE] != 0.0) {
+          xe.y = xe.y + a1 * pe.y;
+          ri.y = ri.y - a1 * qe.y;
+        }
+        x[e] = xe;
+        r[e] = ri;
+      }
+      v0 = v0 + ri.x * ri.x;
+      v1 = v1 + ri.y * ri.y;
+    }
+  }
+  block_colsums2(v0, v1, nrhs, sh, partial);
   finish_multi(partial, counter, nrhs, 3, scal, sh);
 }
 
@@ -1264,12 +1427,14 @@ void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_
                        const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
                        hipStream_t st) {
   if (nblocks <= 0) return;
+  // two off-diagonal tiles per step in FP32; one in FP64 (LDS of two workgroups per CU)
+  constexpr int KB = sizeof(T) == 4 ? 2 : 1;
   if (NR == 16)
-    hipLaunchKernelGGL((k_trsm_multi<T, 16>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs, ldr,
-                       U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 16, KB>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs,
+                       ldr, U, out, ldo, nrhs);
   else
-    hipLaunchKernelGGL((k_trsm_multi<T, 32>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs, ldr,
-                       U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 32, KB>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs,
+                       ldr, U, out, ldo, nrhs);
 }
 
 template <typename T>
@@ -1321,8 +1486,13 @@ void launch_overlap_sum_multi(int n, int nrhs, const int32_t* occ_ptr, const int
 
 void launch_dot_multi(const double* a, const double* b, int n, int nrhs, double* partial,
                       unsigned* counter, int mode, double* scal, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot_multi, dim3(kRedBlocks), dim3(256), 0, st, a, b, (int64_t)n * nrhs,
-                     nrhs, partial, counter, mode, scal);
+  if (nrhs % 2 == 0)
+    hipLaunchKernelGGL(k_dot_multi2, dim3(kRedBlocks), dim3(256), 0, st,
+                       reinterpret_cast<const double2*>(a), reinterpret_cast<const double2*>(b),
+                       (int64_t)n * nrhs / 2, nrhs, partial, counter, mode, scal);
+  else
+    hipLaunchKernelGGL(k_dot_multi, dim3(kRedBlocks), dim3(256), 0, st, a, b, (int64_t)n * nrhs,
+                       nrhs, partial, counter, mode, scal);
 }
 
 void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const double* scal,
@@ -1335,8 +1505,14 @@ void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const do
 void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double* p,
                             const double* q, double* partial, unsigned* counter, double* scal,
                             hipStream_t st) {
-  hipLaunchKernelGGL(k_xr_update_multi, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs, nrhs,
-                     x, r, p, q, partial, counter, scal);
+  if (nrhs % 2 == 0)
+    hipLaunchKernelGGL(k_xr_update_multi2, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs / 2,
+                       nrhs, reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
+                       reinterpret_cast<const double2*>(p), reinterpret_cast<const double2*>(q), partial,
+                       counter, scal);
+  else
+    hipLaunchKernelGGL(k_xr_update_multi, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs, nrhs,
+                       x, r, p, q, partial, counter, scal);
 }
 
 void launch_value_grad(int n, const int32_t* ptr, const int32_t* col, const int32_t* src,
