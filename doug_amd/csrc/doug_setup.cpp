@@ -319,6 +319,15 @@ struct Grower {
     nneigs = 0;
     stat[seed] = 1;
     distance[seed] = 0;
+    // A seed whose strong neighbours are all taken grows no layer: every
+    // round_layer level is then empty, allmind is untouched (its candidate
+    // minimum stays kMaxInt) and the result is the seed alone with status 3.
+    bool free_neighbour = false;
+    for (int i = rs[seed]; i < rs[seed + 1] && !free_neighbour; i++) free_neighbour = stat[cn[i]] <= 0;
+    if (!free_neighbour) {
+      nodes[nneigs++] = seed;
+      return 3;
+    }
     int ninnodes = 1;
     if (inlayer.empty()) inlayer.resize(64);
     if (newlayer.empty()) newlayer.resize(64);
@@ -394,6 +403,7 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
     if (!coarse_level) ps.beta = alpha / 4.0;
     SeedList seeds(n);
     std::vector<int> outer;  // outer-layer candidates of the current aggregate
+    std::vector<int> layerlen((size_t)(2 * neighood + 3), 0);
     int firstwocol = 0;
     Grower g(srow, scol, stat, distance, nodes, ps.allmind, neighood, maxasize);
     while (firstwocol < n) {
@@ -419,7 +429,7 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
         nagrs++;
         int nouters = 0, next_start_layer;
         if (ok == 3) {
-          std::vector<int> layerlen((size_t)(2 * neighood + 3), 0);
+          std::fill(layerlen.begin(), layerlen.end(), 0);
           int maxlayer = 0;
           if (!toosmall && nneigs < minasize) toosmall = true;
           for (int j = nneigs - 1; j >= 0; j--) {
