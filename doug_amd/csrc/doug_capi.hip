@@ -89,6 +89,7 @@ struct doug_ctx {
   BlockDesc* d_cdesc = nullptr;
   double *d_ctiles = nullptr, *d_cW = nullptr;
   double* d_cpart = nullptr;  // partial sums of the chunked coarse GEMVs
+  double* d_cPm = nullptr;    // partial panels of the chunked coarse products (batch)
   double *d_crhs = nullptr, *d_cu = nullptr, *d_csol = nullptr;
   // PCG
   int n = 0;
@@ -971,6 +972,10 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
     ctx->maxnt = std::max(ctx->maxnt, d.nt);
     ctx->work_rows += (int64_t)d.nt * kTile;
   }
+  if (ctx->precision == DOUG_PREC_FP32_FACTORS)
+    prepare_trsv<float>(ctx->maxnt);
+  else
+    prepare_trsv<double>(ctx->maxnt);
   return DOUG_OK;
 }
 
@@ -1157,9 +1162,11 @@ static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
 
 static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
   if (ctx->precision == DOUG_PREC_FP32_FACTORS)
-    launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, ctx->d_subsol,
+                ctx->stream);
   else
-    launch_trsv(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    launch_trsv(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, ctx->d_subsol,
+                ctx->stream);
 }
 
 // preconditioner (pcg.F90:216-270): fine solves, then coarse correction added
@@ -1438,6 +1445,7 @@ int doug_prepare_batch(doug_ctx* ctx) {
   }
   if (ctx->has_coarse) {
     size_t cpad = (size_t)ctx->cnt * kTile * NR;
+    if ((rc = dalloc(ctx, &ctx->d_cPm, cpad * gemv_chunks(ctx->cnt)))) return rc;
     for (double** v : {&ctx->d_cG, &ctx->d_cU, &ctx->d_cS}) {
       if ((rc = dalloc(ctx, v, cpad))) return rc;
       HIPCHK(hipMemsetAsync(*v, 0, cpad * sizeof(double), st));
@@ -1469,8 +1477,9 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
   if (coarse) {
     launch_spmv_multi(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, nrp,
                       ctx->d_cG, NR, nrp, st);
-    launch_wgemm_fwd(ctx->d_cW, ctx->d_cdesc, 1, ctx->cnt, ctx->d_cG, ctx->d_cU, NR, st);
-    launch_wgemm_bwd(ctx->d_cW, ctx->d_cdesc, 1, ctx->cnt, ctx->d_cU, ctx->d_cS, NR, nrp, NR, st);
+    launch_wgemm_fwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cG, ctx->d_cU, ctx->d_cPm, NR, st);
+    launch_wgemm_bwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->cnt * kTile, ctx->d_cU, ctx->d_cS, NR, nrp,
+                     ctx->d_cPm, NR, st);
   }
   const double* fine_src = r;
   if (p.fine_method == 1) {

@@ -414,7 +414,9 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
                                               const int32_t* __restrict__ inds,
                                               const double* __restrict__ rhs,
                                               double* __restrict__ sol) {
-  __shared__ T y[kMaxTrsvTiles * kTile];
+  // the subdomain vector: nt*64 entries in dynamic LDS sized for the largest block
+  extern __shared__ __align__(16) unsigned char trsv_dyn[];
+  T* y = reinterpret_cast<T*>(trsv_dyn);
   __shared__ T tv[kTile];
   __shared__ T red[4 * kTile];
   __shared__ int rbase[kMaxTrsvTiles], rfc[kMaxTrsvTiles], rlr[kMaxTrsvTiles];
@@ -771,80 +773,6 @@ __device__ __forceinline__ void panel_to_lds(T* __restrict__ S, const T* __restr
   for (int e = threadIdx.x; e < kTile * NR; e += 256) S[(e / NR) * (NR + 1) + (e % NR)] = g[e];
 }
 
-// U_i = sum_{k<=i} W_ik G_k: wave w owns rows 16w..16w+15 and all NR columns
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_wgemm_fwd(const T* __restrict__ W,
-                                                   const BlockDesc* __restrict__ d,
-                                                   const T* __restrict__ G, T* __restrict__ U) {
-  __shared__ T As[kTile * LD];
-  __shared__ T Bs[kTile * (NR + 1)];
-  typedef typename Mma<T>::v4 V;
-  const BlockDesc b = d[blockIdx.y];
-  const int i = blockIdx.x;
-  if (i >= b.nt) return;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
-  V acc[NR / 16];
-#pragma unroll
-  for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-  for (int k = 0; k <= i; k++) {
-    tile_to_lds(As, W + b.base + toff(b, i, k));
-    panel_to_lds<T, NR>(Bs, G + (b.woff + (int64_t)k * kTile) * NR);
-    __syncthreads();
-#pragma unroll 4
-    for (int ks = 0; ks < 16; ks++) {
-      const int kk = 4 * ks + lk;
-      const T a = As[(16 * w + li) * LD + kk];
-#pragma unroll
-      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
-    }
-    __syncthreads();
-  }
-  T* Ug = U + (b.woff + (int64_t)i * kTile) * NR;
-#pragma unroll
-  for (int c = 0; c < NR / 16; c++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) Ug[(16 * w + Mma<T>::row(lane, r)) * NR + c * 16 + li] = acc[c][r];
-}
-
-// Z_j = sum_{i>=j} W_ij^T U_i
-template <typename T, int NR>
-__global__ __launch_bounds__(256) void k_wgemm_bwd(const T* __restrict__ W,
-                                                   const BlockDesc* __restrict__ d,
-                                                   const T* __restrict__ U,
-                                                   double* __restrict__ out, int ldo, int nrhs) {
-  __shared__ T As[kTile * LD];
-  __shared__ T Bs[kTile * (NR + 1)];
-  typedef typename Mma<T>::v4 V;
-  const BlockDesc b = d[blockIdx.y];
-  const int j = blockIdx.x;
-  if (j >= b.nt) return;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
-  V acc[NR / 16];
-#pragma unroll
-  for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-  for (int i = j; i < b.nt; i++) {
-    tile_to_lds(As, W + b.base + toff(b, i, j));
-    panel_to_lds<T, NR>(Bs, U + (b.woff + (int64_t)i * kTile) * NR);
-    __syncthreads();
-#pragma unroll 4
-    for (int ks = 0; ks < 16; ks++) {
-      const int kk = 4 * ks + lk;
-      const T a = As[kk * LD + 16 * w + li];  // (W_ij)^T
-#pragma unroll
-      for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int c = 0; c < NR / 16; c++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int p = j * kTile + 16 * w + Mma<T>::row(lane, r);
-      const int col = c * 16 + li;
-      if (p < b.s && col < nrhs) out[(b.vec_off + p) * ldo + col] = (double)acc[c][r];
-    }
-}
-
 // Register staging of 64x64 tiles and 64xNR panels: every thread loads its
 // share with 16-byte loads, so that all loads of a step are in flight before
 // the first LDS store.
@@ -912,6 +840,91 @@ __device__ __forceinline__ void panel_mma(typename Mma<T>::v4 (&acc)[NR / 16], c
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
   }
+}
+
+// Dense coarse inverse (block d[0]) times NR columns, in chunks of
+// kGemvChunk tiles per workgroup so that the whole chip streams W; each chunk
+// writes a partial 64 x NR panel, summed afterwards in a fixed order.
+// Wave w owns rows 16w..16w+15 and all NR columns.
+// forward, part (ch, i) = sum_{k in chunk ch, k <= i} W_ik G_k
+template <int NR>
+__global__ __launch_bounds__(256) void k_wgemm_fwd_part(const double* __restrict__ W,
+                                                        const BlockDesc* __restrict__ d,
+                                                        const double* __restrict__ G,
+                                                        double* __restrict__ part, int nch) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * (NR + 1)];
+  typedef Mma<double>::v4 V;
+  const BlockDesc b = d[0];
+  const int ch = blockIdx.x, i = blockIdx.y;
+  const int k0 = ch * kGemvChunk;
+  if (i >= b.nt || k0 > i) return;
+  const int k1 = min(i + 1, k0 + kGemvChunk);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  V acc[NR / 16];
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+  for (int k = k0; k < k1; k++) {
+    tile_to_lds(As, W + b.base + toff(b, i, k));
+    panel_to_lds<double, NR>(Bs, G + (int64_t)k * kTile * NR);
+    __syncthreads();
+    panel_mma<double, NR, false>(acc, As, Bs, w, li, lk);
+    __syncthreads();
+  }
+  double* P = part + ((int64_t)i * nch + ch) * kTile * NR;
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) P[(16 * w + Mma<double>::row(lane, r)) * NR + c * 16 + li] = acc[c][r];
+}
+
+// backward, part (ch, j) = sum_{i in chunk ch of [j, nt)} W_ij^T U_i
+template <int NR>
+__global__ __launch_bounds__(256) void k_wgemm_bwd_part(const double* __restrict__ W,
+                                                        const BlockDesc* __restrict__ d,
+                                                        const double* __restrict__ U,
+                                                        double* __restrict__ part, int nch) {
+  __shared__ double As[kTile * LD];
+  __shared__ double Bs[kTile * (NR + 1)];
+  typedef Mma<double>::v4 V;
+  const BlockDesc b = d[0];
+  const int ch = blockIdx.x, j = blockIdx.y;
+  const int i0 = j + ch * kGemvChunk;
+  if (j >= b.nt || i0 >= b.nt) return;
+  const int i1 = min(b.nt, i0 + kGemvChunk);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  V acc[NR / 16];
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+  for (int i = i0; i < i1; i++) {
+    tile_to_lds(As, W + b.base + toff(b, i, j));
+    panel_to_lds<double, NR>(Bs, U + (int64_t)i * kTile * NR);
+    __syncthreads();
+    panel_mma<double, NR, true>(acc, As, Bs, w, li, lk);
+    __syncthreads();
+  }
+  double* P = part + ((int64_t)j * nch + ch) * kTile * NR;
+#pragma unroll
+  for (int c = 0; c < NR / 16; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) P[(16 * w + Mma<double>::row(lane, r)) * NR + c * 16 + li] = acc[c][r];
+}
+
+// out(p, c) = sum over the chunks of p's tile in order, for p < rows, c < ncols
+// (leading dimension ldo); transposed: chunks of tile j cover [j, nt)
+template <int NR>
+__global__ void k_panel_sum(const double* __restrict__ part, int nt, int nch, int transposed,
+                            double* __restrict__ out, int ldo, int rows, int ncols) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nt * kTile * NR) return;
+  const int p = (int)(e / NR), c = (int)(e - (int64_t)p * NR);
+  if (p >= rows || c >= ncols) return;
+  const int i = p / kTile, rr = p - i * kTile;
+  const int len = transposed ? nt - i : i + 1;
+  const int nc = (len + kGemvChunk - 1) / kGemvChunk;
+  double s = 0.0;
+  for (int ch = 0; ch < nc; ch++) s = s + part[(((int64_t)i * nch + ch) * kTile + rr) * NR + c];
+  out[(int64_t)p * ldo + c] = s;
 }
 
 // Envelope forward/backward substitution for NR columns, one workgroup per
@@ -1103,7 +1116,7 @@ constexpr int kMultiMax = kMaxRhs;
 enum { SM_ACTIVE = 6 };
 
 __device__ void finish_multi(double* partial, unsigned* counter, int nrhs, int mode, double* scal,
-                             double* colsum) {
+                             double* sh) {
   __shared__ bool last;
   if (threadIdx.x == 0) {
     __threadfence();
@@ -1112,11 +1125,20 @@ __device__ void finish_multi(double* partial, unsigned* counter, int nrhs, int m
   __syncthreads();
   if (!last) return;
   __threadfence();
-  const int r = threadIdx.x;
-  if (r < nrhs) {
-    double v = 0.0;
-    for (int b = 0; b < (int)gridDim.x; b++)
-      v += __hip_atomic_load(&partial[b * kMultiMax + r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // 256 / nrhs threads per column, each over a fixed stride of the block
+  // partials, then a fixed-order tree (nrhs is a power of two <= 32)
+  const int t = threadIdx.x, per = 256 / nrhs, r = t / per, sub = t - r * per;
+  double v = 0.0;
+  for (int b = sub; b < (int)gridDim.x; b += per)
+    v += __hip_atomic_load(&partial[b * kMultiMax + r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  sh[t] = v;
+  __syncthreads();
+  for (int o = per >> 1; o > 0; o >>= 1) {
+    if (sub < o) sh[t] = sh[t] + sh[t + o];
+    __syncthreads();
+  }
+  if (sub == 0) {
+    v = sh[t];
     double* sc = scal + r * 8;
     const bool active = sc[SM_ACTIVE] != 0.0;
     if (mode == 1) {
@@ -1131,7 +1153,6 @@ __device__ void finish_multi(double* partial, unsigned* counter, int nrhs, int m
     }
   }
   if (threadIdx.x == 0) *counter = 0u;
-  (void)colsum;
 }
 
 __device__ __forceinline__ void block_colsums(double v, int nrhs, double* sh, double* partial) {
@@ -1343,10 +1364,19 @@ void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int
 }
 
 template <typename T>
-void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+void prepare_trsv(int maxnt) {
+  const int bytes = maxnt * kTile * (int)sizeof(T);
+  if (bytes > 64 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_trsv<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+template <typename T>
+void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
                  const double* r, double* sol, hipStream_t st) {
   if (nblocks <= 0) return;
-  hipLaunchKernelGGL(k_trsv<T>, dim3(nblocks), dim3(256), 0, st, tiles, d, inds, r, sol);
+  const size_t bytes = (size_t)maxnt * kTile * sizeof(T);
+  hipLaunchKernelGGL(k_trsv<T>, dim3(nblocks), dim3(256), bytes, st, tiles, d, inds, r, sol);
 }
 
 #define DOUG_INSTANTIATE(T)                                                                    \
@@ -1355,8 +1385,9 @@ void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t*
   template void launch_pad_identity<T>(T*, const BlockDesc*, int, int, hipStream_t);           \
   template void launch_chol_diag<T>(T*, const BlockDesc*, int, int, int*, hipStream_t, bool);  \
   template void launch_chol_offdiag<T>(T*, const BlockDesc*, int, int, int, hipStream_t, bool); \
-  template void launch_trsv<T>(const T*, const BlockDesc*, int, const int32_t*, const double*, \
-                               double*, hipStream_t);
+  template void launch_trsv<T>(const T*, const BlockDesc*, int, int, const int32_t*,          \
+                               const double*, double*, hipStream_t);                          \
+  template void prepare_trsv<T>(int);
 DOUG_INSTANTIATE(double)
 DOUG_INSTANTIATE(float)
 #undef DOUG_INSTANTIATE
@@ -1437,24 +1468,30 @@ void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_
                        ldr, U, out, ldo, nrhs);
 }
 
-template <typename T>
-void launch_wgemm_fwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* G, T* U,
-                      int NR, hipStream_t st) {
-  if (NR == 16)
-    hipLaunchKernelGGL((k_wgemm_fwd<T, 16>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, G, U);
-  else
-    hipLaunchKernelGGL((k_wgemm_fwd<T, 32>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, G, U);
+void launch_wgemm_fwd(const double* W, const BlockDesc* d, int nt, const double* G, double* U,
+                      double* part, int NR, hipStream_t st) {
+  const int nch = gemv_chunks(nt);
+  const dim3 g(nch, nt), gs(cdiv((int64_t)nt * kTile * NR, 256));
+  if (NR == 16) {
+    hipLaunchKernelGGL(k_wgemm_fwd_part<16>, g, dim3(256), 0, st, W, d, G, part, nch);
+    hipLaunchKernelGGL(k_panel_sum<16>, gs, dim3(256), 0, st, part, nt, nch, 0, U, NR, nt * kTile, NR);
+  } else {
+    hipLaunchKernelGGL(k_wgemm_fwd_part<32>, g, dim3(256), 0, st, W, d, G, part, nch);
+    hipLaunchKernelGGL(k_panel_sum<32>, gs, dim3(256), 0, st, part, nt, nch, 0, U, NR, nt * kTile, NR);
+  }
 }
 
-template <typename T>
-void launch_wgemm_bwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* U,
-                      double* out, int ldo, int nrhs, int NR, hipStream_t st) {
-  if (NR == 16)
-    hipLaunchKernelGGL((k_wgemm_bwd<T, 16>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, U, out,
-                       ldo, nrhs);
-  else
-    hipLaunchKernelGGL((k_wgemm_bwd<T, 32>), dim3(maxnt, nblocks), dim3(256), 0, st, W, d, U, out,
-                       ldo, nrhs);
+void launch_wgemm_bwd(const double* W, const BlockDesc* d, int nt, int rows, const double* U,
+                      double* out, int ldo, int nrhs, double* part, int NR, hipStream_t st) {
+  const int nch = gemv_chunks(nt);
+  const dim3 g(nch, nt), gs(cdiv((int64_t)nt * kTile * NR, 256));
+  if (NR == 16) {
+    hipLaunchKernelGGL(k_wgemm_bwd_part<16>, g, dim3(256), 0, st, W, d, U, part, nch);
+    hipLaunchKernelGGL(k_panel_sum<16>, gs, dim3(256), 0, st, part, nt, nch, 1, out, ldo, rows, nrhs);
+  } else {
+    hipLaunchKernelGGL(k_wgemm_bwd_part<32>, g, dim3(256), 0, st, W, d, U, part, nch);
+    hipLaunchKernelGGL(k_panel_sum<32>, gs, dim3(256), 0, st, part, nt, nch, 1, out, ldo, rows, nrhs);
+  }
 }
 
 #define DOUG_INSTANTIATE_MULTI(T)                                                              \
@@ -1462,10 +1499,6 @@ void launch_wgemm_bwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, co
                                      const double*, int, T*, double*, int, int, int, hipStream_t);
 DOUG_INSTANTIATE_MULTI(double)
 DOUG_INSTANTIATE_MULTI(float)
-template void launch_wgemm_fwd<double>(const double*, const BlockDesc*, int, int, const double*,
-                                       double*, int, hipStream_t);
-template void launch_wgemm_bwd<double>(const double*, const BlockDesc*, int, int, const double*,
-                                       double*, int, int, int, hipStream_t);
 #undef DOUG_INSTANTIATE_MULTI
 
 void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,

@@ -9,7 +9,7 @@ constexpr int kTile = 64;              // dense tile edge (one wavefront wide)
 constexpr int kTileElems = kTile * kTile;
 constexpr int kSpmvThreads = 256;      // CSR-stream block
 constexpr int kSpmvChunk = 2048;       // products staged in LDS per block
-constexpr int kMaxTrsvTiles = 128;     // subdomain size limit of the one-block solve (8192)
+constexpr int kMaxTrsvTiles = 256;     // subdomain size limit of the one-block solve (16384)
 constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reductions
 constexpr int kMaxRhs = 32;            // right-hand sides per batched solve
 
@@ -81,8 +81,12 @@ void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, co
 // Batched forward/backward substitution, one block per subdomain: gathers
 // r(inds), solves L L^T u = r in the factor's precision, writes u to sol
 // (FP64, concatenated layout).
+// maxnt: the largest block's tile count (sets the dynamic LDS); call
+// prepare_trsv<T>(maxnt) once before the first launch for that size.
 template <typename T>
-void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+void prepare_trsv(int maxnt);
+template <typename T>
+void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
                  const double* r, double* sol, hipStream_t st);
 // u = W c and c' = W^T u for the coarse inverse factor; `part` holds
 // nt * gemv_chunks(nt) * 64 doubles of partial sums.
@@ -122,15 +126,14 @@ template <typename T>
 void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
                        const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
                        hipStream_t st);
-// Dense explicit inverse (coarse block): U_i = sum_{k<=i} W_ik G_k
-template <typename T>
-void launch_wgemm_fwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* G, T* U,
-                      int NR, hipStream_t st);
-// Z_j = sum_{i>=j} W_ij^T U_i, written (FP64) to out[(vec_off + p) * ldo + r]
-// for rows p < s and columns r < nrhs
-template <typename T>
-void launch_wgemm_bwd(const T* W, const BlockDesc* d, int nblocks, int maxnt, const T* U,
-                      double* out, int ldo, int nrhs, int NR, hipStream_t st);
+// Dense coarse inverse (block d[0], nt tiles) times NR columns, chunked with
+// partial panels in `part` (nt * gemv_chunks(nt) * 64 * NR doubles):
+// U_i = sum_{k<=i} W_ik G_k (all nt*64 rows), and
+// out(p, c) = (sum_{i>=j} W_ij^T U_i)(p, c) for p < rows, c < nrhs (leading dimension ldo).
+void launch_wgemm_fwd(const double* W, const BlockDesc* d, int nt, const double* G, double* U,
+                      double* part, int NR, hipStream_t st);
+void launch_wgemm_bwd(const double* W, const BlockDesc* d, int nt, int rows, const double* U,
+                      double* out, int ldo, int nrhs, double* part, int NR, hipStream_t st);
 // y(:, r) = A x(:, r) for r < nrhs, each row summed in stored order
 void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                        const double* val, const double* x, int ldx, double* y, int ldy, int nrhs,

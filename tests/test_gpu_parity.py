@@ -377,3 +377,28 @@ def test_fem_workload_against_oracle_and_batched():
         for k in range(32):
             r = B[:, k] - o.spmv(X[:, k])
             assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(B[:, k]) * 1.01, k
+
+
+def test_subdomain_beyond_8192_unknowns():
+    # one subdomain of 10,648 unknowns (167 tiles): the one-workgroup solve
+    # sizes its LDS vector for the largest block
+    N = 22
+    n, i, j, v = P.diffusion3d(N, 1.0, 17)
+    part = (np.array([1, n + 1], np.int32), np.arange(1, n + 1, dtype=np.int32))
+    print(f"large subdomain n={n}", flush=True)
+    for precision in (0, 1):
+        o, s = _pair(n, i, j, v, part, precision=precision, **CFG0_CTL)
+        with s:
+            r = np.random.default_rng(precision).uniform(-1, 1, n)
+            z, zo = s.prec(r), o.prec(r)
+            assert np.linalg.norm(z - zo) <= (1e-10 if precision == 0 else 1e-4) * np.linalg.norm(zo)
+            b = P.uniform_rhs(n, 21)
+            x, it, _, _ = s.pcg(b, 1e-8, 100)
+            xo, ito, _, _ = o.pcg(b, 1e-8, 100)
+            # with the whole domain in one block the FP64 preconditioner is
+            # nearly exact (2 iterations); FP32 factors leave an error of
+            # order cond * 6e-8 that costs a few more
+            assert it == ito if precision == 0 else it <= ito + 4
+            r = b - o.spmv(x)
+            assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.01
+            assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
