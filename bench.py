@@ -107,7 +107,8 @@ def fem_batched_leg(npoints, nsub, nrhs, steps, warmup, seed):
                      dtype=torch.float64, device=dev).contiguous()
     X = torch.zeros_like(B)
     s.setup(1)
-    tune = s.autotune(20)
+    s.prepare_batch()
+    tune = s.autotune(20)  # includes the batched solve's tiles per step
     times = []
     for k in range(warmup + steps):
         torch.cuda.synchronize()
@@ -159,7 +160,8 @@ def fem_batched_leg(npoints, nsub, nrhs, steps, warmup, seed):
             "fp64_single": {"setup_s": round(tb - ta, 4), "solve_s": round(tc - tb, 4),
                             "time_to_solve_s": round(tc - ta, 4), "iterations": int(it64)},
             "speedup_per_rhs_vs_fp64_single": round((tc - ta) / per_rhs, 2),
-            "autotune": {k: tune[k] for k in ("spmv_rows", "spmv_xcd_order", "trsv_order")}}
+            "autotune": {k: tune[k] for k in ("spmv_rows", "spmv_xcd_order", "trsv_order",
+                                              "trsm_tiles_per_step", "trsm_ms")}}
 
 
 def differentiation_leg(N, block, ndir, seed):
@@ -326,6 +328,8 @@ def main():
                 "trsv_GB/s": round(trsv_gbs, 1), "factor_bytes": info["factor_bytes"],
                 "factor_tiles": info["factor_tiles"],
                 "cholesky_s": round(info["t_factor"], 4), "cholesky_tflops": round(chol_tflops, 3),
+                "cholesky_mfma_frac": round(chol_tflops / (FP32_MFMA_PEAK_TFLOPS if precision == 1
+                                                           else FP64_MFMA_PEAK_TFLOPS), 4),
                 "spmv_ms": spmv_ms, "spmv_GB/s": round(spmv_gbs, 1),
                 "setup_breakdown_s": {k: round(info[k], 4) for k in
                                       ("t_aggregate", "t_coarse_build", "t_subdomains",
@@ -359,6 +363,8 @@ def main():
         s.setup(1)
         s.prepare_batch()
         tb1 = time.perf_counter()
+        if k == 0:
+            tune_batch = s.autotune(20)  # the batched solve's tiles per step
         Xm.zero_()
         its_b = s.pcg_batch_device(nb, Bm.data_ptr(), Xm.data_ptr(), 1e-8, 1000)
         torch.cuda.synchronize()
@@ -380,6 +386,8 @@ def main():
                "time_per_rhs_s": round(float(np.mean([a + c for a, c in bt])) / nb, 5),
                "iterations_min_max": [int(its_b.min()), int(its_b.max())],
                "prec_apply_32rhs_ms": round(prec32_ms, 3),
+               "tiles_per_step": tune_batch["trsm_tiles_per_step"],
+               "prec_apply_ms_per_tiles_per_step": [round(t, 3) for t in tune_batch["trsm_ms"]],
                "rel_residual_col0": float(torch.linalg.norm(Bm[:, 0] - rr) / torch.linalg.norm(Bm[:, 0])),
                "speedup_per_rhs_vs_fp64_single": round(fp64["time_to_solve_s"] /
                                                        (float(np.mean([a + c for a, c in bt])) / nb), 2)}

@@ -76,6 +76,7 @@ struct doug_ctx {
   BlockDesc* d_sub = nullptr;
   std::vector<BlockDesc> h_sub;
   int trsv_order = 0;               // 1: largest subdomain dispatched first
+  int trsm_kb = 2;                  // tiles per step of the batched envelope solve
   int local_order = 1;              // 1: reverse Cuthill-McKee inside each subdomain
   BlockDesc* d_sub_trsv = nullptr;  // descriptors in dispatch order
   doug_tuning tuning{};
@@ -1485,10 +1486,10 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
   if (p.fine_method == 1) {
     if (ctx->precision == DOUG_PREC_FP32_FACTORS)
       launch_trsm_multi(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, nrp,
-                        (float*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, st);
+                        (float*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, st);
     else
       launch_trsm_multi(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, nrp,
-                        (double*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, st);
+                        (double*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, st);
     fine_src = ctx->d_subsolm;
   }
   launch_overlap_sum_multi(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
@@ -1646,6 +1647,11 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
       ctx->local_order = (int)value;
       ctx->ready = false;  // takes effect at the next doug_setup
       return DOUG_OK;
+    case DOUG_OPT_BATCH_TILES:
+      if (value != 1 && value != 2) return fail(ctx, DOUG_E_ARG, "tiles per step must be 1 or 2");
+      drop_iteration_graph(ctx);
+      ctx->trsm_kb = (int)value;
+      return DOUG_OK;
     case DOUG_OPT_DEVICE_COARSE:
       ctx->use_device_coarse = value != 0;
       ctx->ready = false;  // takes effect at the next doug_setup
@@ -1778,7 +1784,21 @@ int doug_autotune(doug_ctx* ctx, int32_t reps, doug_tuning* out) {
     }
     if ((rc = set_trsv_order(ctx, order))) return rc;
   }
+  // batched envelope solve: off-diagonal tiles staged per step
+  int kb = ctx->trsm_kb;
+  if (ctx->batch_ready && ctx->p.fine_method == 1 && ctx->nsub > 0) {
+    const int rt = std::max(2, reps / 8);
+    for (int q = 1; q <= 2; q++) {
+      ctx->trsm_kb = q;
+      if ((rc = time_launches(ctx, rt, [&]() { enqueue_prec_multi(ctx, kMaxRhs, ctx->d_Rm, ctx->d_Zm); },
+                              &tu.trsm_ms[q - 1])))
+        return rc;
+    }
+    kb = tu.trsm_ms[0] < tu.trsm_ms[1] ? 1 : 2;
+  }
+  ctx->trsm_kb = kb;
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  tu.trsm_tiles_per_step = kb;
   tu.spmv_shape = best;
   tu.spmv_rows = spmv_shape_rows(best);
   tu.spmv_xcd_order = best & 1;
@@ -1796,6 +1816,7 @@ int doug_get_tuning(const doug_ctx* ctx, doug_tuning* out) {
   out->spmv_rows = spmv_shape_rows(ctx->spmv_shape);
   out->spmv_xcd_order = ctx->spmv_shape & 1;
   out->trsv_order = ctx->trsv_order;
+  out->trsm_tiles_per_step = ctx->trsm_kb;
   return DOUG_OK;
 }
 

@@ -1456,16 +1456,17 @@ void launch_residual(int n, const double* b, const double* ax, double* r, hipStr
 template <typename T>
 void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
                        const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
-                       hipStream_t st) {
+                       int kb, hipStream_t st) {
   if (nblocks <= 0) return;
-  // two off-diagonal tiles per step in FP32; one in FP64 (LDS of two workgroups per CU)
-  constexpr int KB = sizeof(T) == 4 ? 2 : 1;
-  if (NR == 16)
-    hipLaunchKernelGGL((k_trsm_multi<T, 16, KB>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs,
-                       ldr, U, out, ldo, nrhs);
+  const dim3 g(nblocks), b(256);
+  if (NR == 16 && kb == 1)
+    hipLaunchKernelGGL((k_trsm_multi<T, 16, 1>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
+  else if (NR == 16)
+    hipLaunchKernelGGL((k_trsm_multi<T, 16, 2>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
+  else if (kb == 1)
+    hipLaunchKernelGGL((k_trsm_multi<T, 32, 1>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
   else
-    hipLaunchKernelGGL((k_trsm_multi<T, 32, KB>), dim3(nblocks), dim3(256), 0, st, L, d, inds, rhs,
-                       ldr, U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 32, 2>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
 }
 
 void launch_wgemm_fwd(const double* W, const BlockDesc* d, int nt, const double* G, double* U,
@@ -1496,7 +1497,8 @@ void launch_wgemm_bwd(const double* W, const BlockDesc* d, int nt, int rows, con
 
 #define DOUG_INSTANTIATE_MULTI(T)                                                              \
   template void launch_trsm_multi<T>(const T*, const BlockDesc*, int, const int32_t*,          \
-                                     const double*, int, T*, double*, int, int, int, hipStream_t);
+                                     const double*, int, T*, double*, int, int, int, int,       \
+                                     hipStream_t);
 DOUG_INSTANTIATE_MULTI(double)
 DOUG_INSTANTIATE_MULTI(float)
 #undef DOUG_INSTANTIATE_MULTI

@@ -122,10 +122,11 @@ void launch_residual(int n, const double* b, const double* ax, double* r, hipStr
 // products on the matrix cores: U(block rows) <- L^{-T} L^{-1} rhs(inds, :),
 // then out[(vec_off + p) * ldo + r] = U for p < s, r < nrhs.  rhs has
 // leading dimension ldr; U is T workspace of nt*64 rows of NR from woff.
+// kb: off-diagonal tiles staged per step (1 or 2).
 template <typename T>
 void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
                        const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
-                       hipStream_t st);
+                       int kb, hipStream_t st);
 // Dense coarse inverse (block d[0], nt tiles) times NR columns, chunked with
 // partial panels in `part` (nt * gemv_chunks(nt) * 64 * NR doubles):
 // U_i = sum_{k<=i} W_ik G_k (all nt*64 rows), and

@@ -181,8 +181,9 @@ int doug_time_operator(doug_ctx *ctx, int32_t which, int32_t reps, double *ms);
 /* Launch-shape autotuner (no counterpart in the reference, whose solve path
  * has no launch shapes).  Times, with HIP events on the context's stream,
  * every SpMV shape -- 128, 256 or 512 rows per workgroup, each with and
- * without XCD-contiguous block order -- and the dispatch order of the
- * subdomain solves (as partitioned, or largest first when sizes differ),
+ * without XCD-contiguous block order --, the dispatch order of the
+ * subdomain solves (as partitioned, or largest first when sizes differ) and,
+ * after doug_prepare_batch, the tiles per step of the batched solve,
  * `reps` launches each (<= 0: 20), and keeps the fastest.  All choices give
  * bit-identical results.  The choice stays with the context across later
  * doug_setup calls.  `out` may be NULL. */
@@ -193,6 +194,10 @@ typedef struct doug_tuning {
   int32_t trsv_order;     /* 0 as partitioned, 1 largest subdomain first */
   double spmv_ms[6];      /* measured average per shape */
   double trsv_ms[2];      /* measured per order; 0 where not timed */
+  int32_t trsm_tiles_per_step; /* batched envelope solve: 1 or 2 tiles staged per step */
+  int32_t pad;
+  double trsm_ms[2];      /* 32-column preconditioner apply per choice; 0 where not timed
+                             (timed only after doug_prepare_batch) */
 } doug_tuning;
 int doug_autotune(doug_ctx *ctx, int32_t reps, doug_tuning *out);
 int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
@@ -207,13 +212,16 @@ int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
  * index-set order.  It needs a new doug_setup and changes rounding only.
  * DOUG_OPT_DEVICE_COARSE (default 1): build the smoothed restriction and the
  * Galerkin coarse matrix on the GPU (same bits as the host build, which 0
- * selects); applies to smoothers = 1 at the next doug_setup. */
+ * selects); applies to smoothers = 1 at the next doug_setup.
+ * DOUG_OPT_BATCH_TILES (1 or 2, default 2): off-diagonal tiles the batched
+ * envelope solve stages per step (what doug_autotune sets). */
 enum {
   DOUG_OPT_ITERATION_GRAPH = 1,
   DOUG_OPT_SPMV_SHAPE = 2,
   DOUG_OPT_TRSV_ORDER = 3,
   DOUG_OPT_LOCAL_ORDER = 4,
-  DOUG_OPT_DEVICE_COARSE = 5
+  DOUG_OPT_DEVICE_COARSE = 5,
+  DOUG_OPT_BATCH_TILES = 6
 };
 int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 

@@ -77,3 +77,26 @@ def test_dispatch_order_and_autotune_keep_the_bits():
         assert s.tuning()["spmv_shape"] == tu["spmv_shape"]
         x3, it3, _, _ = s.pcg(b, 1e-8, 500)
         assert it3 == it0 and _bits_equal(x3, x0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_tiles_per_step_keep_the_bits(precision):
+    # one or two off-diagonal tiles staged per step: the same products in the
+    # same order, so the batched preconditioner gives the same bits
+    N = 16
+    n, i, j, v = P.diffusion3d(N, 1.0, 8)
+    part = P.block_partition((N,) * 3, 8)
+    B = np.stack([P.uniform_rhs(n, 300 + k) for k in range(6)], axis=1)
+    print(f"batch tiles per step precision={precision}", flush=True)
+    with _solver(n, i, j, v, part) as s:
+        s.setup(precision)
+        s.prepare_batch()
+        out = {}
+        for kb in (1, 2):
+            s.set_option(6, kb)
+            assert s.tuning()["trsm_tiles_per_step"] == kb
+            out[kb] = s.pcg_batch(B, 1e-8, 500)
+        np.testing.assert_array_equal(out[1][1], out[2][1])
+        assert _bits_equal(out[1][0], out[2][0])
+        tu = s.autotune(4)
+        assert tu["trsm_tiles_per_step"] in (1, 2) and min(tu["trsm_ms"]) > 0
