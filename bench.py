@@ -164,6 +164,37 @@ def fem_batched_leg(npoints, nsub, nrhs, steps, warmup, seed):
                                               "trsm_tiles_per_step", "trsm_ms")}}
 
 
+def direct_accuracy_leg(N, block, sigma, seed):
+    """Relative error of both precision forms at rtol 1e-8 against an FP64
+    sparse-direct solve (scipy's SuperLU) on an N^3 instance of the same
+    field and control parameters."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+    import doug_amd as D
+    from doug_amd import problems as P
+
+    n, i, j, v = P.diffusion3d(N, sigma, seed)
+    b = P.uniform_rhs(n, seed + 1)
+    A = sps.coo_matrix((v, (i - 1, j - 1)), shape=(n, n)).tocsc()
+    t0 = time.perf_counter()
+    xd = spla.spsolve(A, b, permc_spec="MMD_AT_PLUS_A")
+    td = time.perf_counter() - t0
+    out = {"instance": f"3d diffusion {N}^3 (n={n}), {(N // block) ** 3} subdomains of {block}^3",
+           "direct_solver": "scipy.sparse.linalg.spsolve (SuperLU, MMD_AT_PLUS_A), FP64",
+           "direct_s": round(td, 2),
+           "direct_rel_residual": float(np.linalg.norm(A @ xd - b) / np.linalg.norm(b))}
+    with D.Solver(0) as s:
+        s.set_params(**CTL)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_partition(*P.block_partition((N,) * 3, block))
+        for precision, name in ((0, "fp64"), (1, "fp32_factors")):
+            s.setup(precision)
+            x, it, _, _ = s.pcg(b, 1e-8, 1000)
+            out[f"rel_err_{name}"] = float(np.linalg.norm(x - xd) / np.linalg.norm(xd))
+            out[f"iterations_{name}"] = it
+    return out
+
+
 def differentiation_leg(N, block, ndir, seed):
     """Differentiation form: loss 0.5 |x(theta) - x_target|^2 for 3-D diffusion
     on N^3 with per-cell log-coefficients theta, forward and adjoint PCG to
@@ -234,6 +265,8 @@ def main():
                     help="points of the FEM batching workload (0: skip it)")
     ap.add_argument("--diff-size", type=int, default=64,
                     help="grid edge of the differentiation workload (0: skip it)")
+    ap.add_argument("--direct-size", type=int, default=40,
+                    help="grid edge of the sparse-direct accuracy check (0: skip it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the whole solve "
@@ -442,6 +475,9 @@ def main():
     if args.fem_points > 0:
         out["fem_batched_32rhs"] = fem_batched_leg(args.fem_points, 256, 32, min(args.steps, 2),
                                                    min(args.warmup, 1), args.seed + 7)
+    if args.direct_size > 0:
+        log("[bench] accuracy against a sparse-direct solve ...")
+        out["accuracy_vs_sparse_direct"] = direct_accuracy_leg(args.direct_size, 8, args.sigma, args.seed)
     if args.diff_size > 0:
         log("[bench] differentiation ...")
         out["differentiation"] = differentiation_leg(args.diff_size, 16, 20, args.seed + 11)
