@@ -1476,8 +1476,11 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
   }
   const bool coarse = ctx->has_coarse;
   if (coarse) {
-    launch_spmv_multi(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, nrp,
-                      ctx->d_cG, NR, nrp, st);
+    if (nrp % 2 == 0)  // restriction rows hold hundreds of entries
+      launch_spmv_multi_rows(ctx->nc, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, nrp, ctx->d_cG, NR, nrp, st);
+    else
+      launch_spmv_multi(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, nrp,
+                        ctx->d_cG, NR, nrp, st);
     launch_wgemm_fwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cG, ctx->d_cU, ctx->d_cPm, NR, st);
     launch_wgemm_bwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->cnt * kTile, ctx->d_cU, ctx->d_cS, NR, nrp,
                      ctx->d_cPm, NR, st);

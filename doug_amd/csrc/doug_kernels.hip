@@ -1109,6 +1109,106 @@ __global__ void k_overlap_sum_multi(int n, int nrhs, const int32_t* __restrict__
   z[e] = acc;
 }
 
+// Column-pair forms (nrhs, ldx, ldy, ldc even): one thread per (row, pair of
+// columns) with 16-byte loads; every column is summed in the same order as
+// in the one-column forms above, so the results are the same bits.
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv_multi2(
+    const int32_t* __restrict__ rowblk, const int32_t* __restrict__ ptr,
+    const int32_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ x,
+    int ldx, double* __restrict__ y, int ldy, int nrhs) {
+  __shared__ double sval[kSpmvChunk];
+  __shared__ int32_t scol[kSpmvChunk];
+  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int e0 = ptr[r0], e1 = ptr[r1];
+  const int t = threadIdx.x, np = nrhs >> 1;
+  const bool staged = e1 - e0 <= kSpmvChunk;
+  if (staged) {
+    for (int e = e0 + t; e < e1; e += kSpmvThreads) {
+      sval[e - e0] = val[e];
+      scol[e - e0] = col[e];
+    }
+    __syncthreads();
+  }
+  const int work = (r1 - r0) * np;
+  for (int idx = t; idx < work; idx += kSpmvThreads) {
+    const int row = r0 + idx / np, c = 2 * (idx % np);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int e = ptr[row]; e < ptr[row + 1]; e++) {
+      const double a = staged ? sval[e - e0] : val[e];
+      const int k = staged ? scol[e - e0] : col[e];
+      const double2 xv = *reinterpret_cast<const double2*>(x + (int64_t)k * ldx + c);
+      acc.x = acc.x + a * xv.x;
+      acc.y = acc.y + a * xv.y;
+    }
+    *reinterpret_cast<double2*>(y + (int64_t)row * ldy + c) = acc;
+  }
+}
+
+// Long rows (the restriction R, hundreds of entries per row) for column
+// pairs: one workgroup per row; thread t takes column pair t % np and every
+// (256/np)-th entry from t / np, then the partials of a pair are added in a
+// fixed tree order.
+__global__ __launch_bounds__(256) void k_spmv_multi_rows(const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ col,
+                                                         const double* __restrict__ val,
+                                                         const double* __restrict__ x, int ldx,
+                                                         double* __restrict__ y, int ldy,
+                                                         int nrhs) {
+  __shared__ double2 part[256];
+  const int row = blockIdx.x, t = threadIdx.x;
+  const int np = nrhs >> 1, lanes = 256 / np, pr = t % np, ln = t / np, c = 2 * pr;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int e = ptr[row] + ln; e < ptr[row + 1]; e += lanes) {
+    const double a = val[e];
+    const double2 xv = *reinterpret_cast<const double2*>(x + (int64_t)col[e] * ldx + c);
+    acc.x = acc.x + a * xv.x;
+    acc.y = acc.y + a * xv.y;
+  }
+  part[t] = acc;
+  __syncthreads();
+  for (int o = lanes >> 1; o > 0; o >>= 1) {
+    if (ln < o) {
+      const double2 q = part[t + o * np];
+      part[t].x = part[t].x + q.x;
+      part[t].y = part[t].y + q.y;
+    }
+    __syncthreads();
+  }
+  if (ln == 0) *reinterpret_cast<double2*>(y + (int64_t)row * ldy + c) = part[t];
+}
+
+__global__ void k_overlap_sum_multi2(int n, int nrhs, const int32_t* __restrict__ occ_ptr,
+                                     const int32_t* __restrict__ occ_pos,
+                                     const double* __restrict__ sol,
+                                     const int32_t* __restrict__ rt_ptr,
+                                     const int32_t* __restrict__ rt_col,
+                                     const double* __restrict__ rt_val,
+                                     const double* __restrict__ csol, int ldc,
+                                     double* __restrict__ z) {
+  const int np = nrhs >> 1;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * np) return;
+  const int i = (int)(e / np), c = 2 * (int)(e - (int64_t)i * np);
+  double2 acc = make_double2(0.0, 0.0);
+  for (int p = occ_ptr[i]; p < occ_ptr[i + 1]; p++) {
+    const double2 s = *reinterpret_cast<const double2*>(sol + (int64_t)occ_pos[p] * nrhs + c);
+    acc.x = acc.x + s.x;
+    acc.y = acc.y + s.y;
+  }
+  if (rt_ptr) {
+    double2 tmp = make_double2(0.0, 0.0);
+    for (int p = rt_ptr[i]; p < rt_ptr[i + 1]; p++) {
+      const double w = rt_val[p];
+      const double2 cs = *reinterpret_cast<const double2*>(csol + (int64_t)rt_col[p] * ldc + c);
+      tmp.x = tmp.x + w * cs.x;
+      tmp.y = tmp.y + w * cs.y;
+    }
+    acc.x = acc.x + tmp.x;
+    acc.y = acc.y + tmp.y;
+  }
+  *reinterpret_cast<double2*>(z + (int64_t)i * nrhs + c) = acc;
+}
+
 // Per-column reductions over an n x nrhs node-major block: thread t always
 // handles column t % nrhs (the stride is a multiple of nrhs), block sums per
 // column in a fixed order, and the last block finishes every column.
@@ -1507,16 +1607,31 @@ void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const
                        const double* val, const double* x, int ldx, double* y, int ldy, int nrhs,
                        hipStream_t st) {
   if (nrb <= 0) return;
-  hipLaunchKernelGGL(k_spmv_multi, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x,
-                     ldx, y, ldy, nrhs);
+  if (nrhs % 2 == 0 && ldx % 2 == 0 && ldy % 2 == 0)
+    hipLaunchKernelGGL(k_spmv_multi2, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x,
+                       ldx, y, ldy, nrhs);
+  else
+    hipLaunchKernelGGL(k_spmv_multi, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, x,
+                       ldx, y, ldy, nrhs);
+}
+
+void launch_spmv_multi_rows(int nrows, const int32_t* ptr, const int32_t* col, const double* val,
+                            const double* x, int ldx, double* y, int ldy, int nrhs, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_spmv_multi_rows, dim3(nrows), dim3(256), 0, st, ptr, col, val, x, ldx, y, ldy,
+                     nrhs);
 }
 
 void launch_overlap_sum_multi(int n, int nrhs, const int32_t* occ_ptr, const int32_t* occ_pos,
                               const double* sol, const int32_t* rt_ptr, const int32_t* rt_col,
                               const double* rt_val, const double* csol, int ldc, double* z,
                               hipStream_t st) {
-  hipLaunchKernelGGL(k_overlap_sum_multi, dim3(cdiv((int64_t)n * nrhs, 256)), dim3(256), 0, st, n,
-                     nrhs, occ_ptr, occ_pos, sol, rt_ptr, rt_col, rt_val, csol, ldc, z);
+  if (nrhs % 2 == 0 && ldc % 2 == 0)
+    hipLaunchKernelGGL(k_overlap_sum_multi2, dim3(cdiv((int64_t)n * (nrhs / 2), 256)), dim3(256), 0, st,
+                       n, nrhs, occ_ptr, occ_pos, sol, rt_ptr, rt_col, rt_val, csol, ldc, z);
+  else
+    hipLaunchKernelGGL(k_overlap_sum_multi, dim3(cdiv((int64_t)n * nrhs, 256)), dim3(256), 0, st, n,
+                       nrhs, occ_ptr, occ_pos, sol, rt_ptr, rt_col, rt_val, csol, ldc, z);
 }
 
 void launch_dot_multi(const double* a, const double* b, int n, int nrhs, double* partial,

@@ -139,6 +139,11 @@ void launch_wgemm_bwd(const double* W, const BlockDesc* d, int nt, int rows, con
 void launch_spmv_multi(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                        const double* val, const double* x, int ldx, double* y, int ldy, int nrhs,
                        hipStream_t st);
+// The same for long rows (the restriction): one workgroup per row, each
+// row's entries split over 256/(nrhs/2) threads per column pair and added in
+// a fixed tree order.  nrhs, ldx, ldy even, nrhs <= 32.
+void launch_spmv_multi_rows(int nrows, const int32_t* ptr, const int32_t* col, const double* val,
+                            const double* x, int ldx, double* y, int ldy, int nrhs, hipStream_t st);
 void launch_overlap_sum_multi(int n, int nrhs, const int32_t* occ_ptr, const int32_t* occ_pos,
                               const double* sol, const int32_t* rt_ptr, const int32_t* rt_col,
                               const double* rt_val, const double* csol, int ldc, double* z,
