@@ -402,3 +402,31 @@ def test_subdomain_beyond_8192_unknowns():
             r = b - o.spmv(x)
             assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(b) * 1.01
             assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("variant", [
+    dict(levels=1),                # one level: Schwarz without the coarse grid
+    dict(method=0),                # no preconditioner: plain CG
+    dict(fine_method=0),           # coarse correction only
+    dict(overlap=0),               # non-overlapping subdomains
+    dict(smoothers=0),             # unsmoothed aggregates (host build of R and A_c)
+])
+def test_option_variants_against_oracle(variant):
+    n, i, j, v = P.diffusion3d(16, 1.0, 23)
+    part = P.block_partition((16,) * 3, 8)
+    b = P.uniform_rhs(n, 24)
+    ctl = dict(CFG0_CTL, **variant)
+    print(f"variant {variant}", flush=True)
+    o, s = _pair(n, i, j, v, part, **ctl)
+    with s:
+        r = np.random.default_rng(25).uniform(-1, 1, n)
+        z, zo = s.prec(r), o.prec(r)
+        assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+        x, it, cond, _ = s.pcg(b, 1e-8, 2000)
+        xo, ito, condo, _ = o.pcg(b, 1e-8, 2000)
+        # dot products are fixed-order trees, not DOUG's sequential sums: over
+        # hundreds of weakly preconditioned iterations the stop may drift
+        long_run = ito > 100
+        assert abs(it - ito) <= (max(1, ito // 100) if long_run else 0), (it, ito)
+        assert np.linalg.norm(x - xo) <= (1e-7 if long_run else 1e-8) * np.linalg.norm(xo)
+        assert abs(cond - condo) <= (1e-3 if long_run else 1e-6) * condo
