@@ -398,7 +398,7 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
     ntmax = std::max(ntmax, nt);
   }
   std::vector<TileRow> rows((size_t)row0[nb]);
-  const int nth = std::max(1, std::min<int>((int)std::thread::hardware_concurrency(), 16));
+  const int nth = setup_threads();
   auto for_blocks = [&](auto&& body) {
     std::vector<std::thread> th;
     for (int t = 0; t < nth; t++)
@@ -1054,7 +1054,6 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   std::vector<double>& scaled = ctx->scaled;
   std::vector<uint8_t>& strong = ctx->strong;
   phase_values(ctx, scaled, strong);
-  if ((rc = upload_matrix(ctx))) return rc;
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
     if (ctx->smoothers == 1 && ctx->use_device_coarse) return coarse_build_device(ctx, strong);
@@ -1062,18 +1061,28 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     return DOUG_OK;
   };
   if (ctx->have_partition) {
-    // the subdomains do not depend on the aggregation: a second host thread
-    // aggregates while this one prepares the subdomains and the GPU
-    // factorises them.  The two touch disjoint context members and read A,
-    // scaled and strong only.
+    // neither the matrix upload nor the subdomains depend on the
+    // aggregation: a second host thread aggregates while this one uploads
+    // the matrix, prepares the subdomains and the GPU factorises them.  The
+    // two touch disjoint context members and read A, scaled and strong only.
+    // The aggregation is sequential and the longest setup step, so this
+    // thread's parallel loops leave it half of the hardware threads.
     std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
-    phase_subdomains(ctx);
-    rc = device_fine(ctx, ev[0], ev[1]);
+    set_setup_thread_cap(std::max(1, setup_threads() / 2 - 1));
+    rc = upload_matrix(ctx);
+    if (!rc) {
+      phase_subdomains(ctx);
+      rc = device_fine(ctx, ev[0], ev[1]);
+    }
+    set_setup_thread_cap(0);
     agg.join();
     if (rc) return rc;
     if ((rc = coarse_space())) return rc;
   } else {
-    phase_aggregate(ctx, scaled, strong);
+    std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
+    rc = upload_matrix(ctx);
+    agg.join();
+    if (rc) return rc;
     if ((rc = coarse_space())) return rc;
     phase_subdomains(ctx);
     if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;

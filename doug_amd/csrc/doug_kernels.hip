@@ -180,43 +180,67 @@ __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
         As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
       }
   __syncthreads();
-  // unblocked Cholesky of the 64x64 tile (lower triangle)
-  for (int c = 0; c < kTile; c++) {
-    if (t == 0) {
-      T dv = As[c * LD + c];
-      if (!(dv > (T)0)) {
-        atomicCAS(flag, 0, (int)blockIdx.x + 1);
-        dv = (T)1;
+  // Unblocked Cholesky of the 64x64 tile in registers: lane r of wave g
+  // holds row r, columns 16g..16g+15.  Step c: the wave owning column c
+  // takes the pivot from lane c, scales the column and publishes it
+  // (column-major in Bs, which ends up holding L); then every lane updates
+  // its entries right of c.  One barrier per step: each step writes its own
+  // column of Bs.  The same operations, in the same order, as the textbook
+  // right-looking loop A_rq -= L_rc L_qc, L_rc = A_rc / sqrt(A_cc).
+  T a[16];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) a[jj] = As[lane * LD + 16 * w + jj];
+  for (int cb = 0; cb < 4; cb++) {
+#pragma unroll
+    for (int cc = 0; cc < 16; cc++) {
+      const int c = 16 * cb + cc;
+      if (w == cb) {
+        T dv = __shfl(a[cc], c);
+        if (!(dv > (T)0)) {
+          if (lane == 0) atomicCAS(flag, 0, (int)blockIdx.x + 1);
+          dv = (T)1;
+        }
+        const T piv = sqrt(dv);
+        Bs[c * LD + lane] = lane > c ? a[cc] / piv : (lane == c ? piv : (T)0);
       }
-      As[c * LD + c] = sqrt(dv);
-    }
-    __syncthreads();
-    const T piv = As[c * LD + c];
-    for (int r = c + 1 + t; r < kTile; r += 256) As[r * LD + c] = As[r * LD + c] / piv;
-    __syncthreads();
-    const int m = kTile - 1 - c;
-    for (int idx = t; idx < m * m; idx += 256) {
-      int rr = idx / m, qq = idx - rr * m;
-      if (qq <= rr) {
-        int r = c + 1 + rr, q = c + 1 + qq;
-        As[r * LD + q] -= As[r * LD + c] * As[q * LD + c];
+      __syncthreads();
+      const T lrc = Bs[c * LD + lane];
+      if (w > cb) {
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) a[jj] = a[jj] - lrc * Bs[c * LD + 16 * w + jj];
+      } else if (w == cb) {
+#pragma unroll
+        for (int jj = cc + 1; jj < 16; jj++) a[jj] = a[jj] - lrc * Bs[c * LD + 16 * w + jj];
       }
     }
-    __syncthreads();
   }
-  // inverse of the lower factor, one column per thread of the first wave
-  for (int idx = t; idx < kTileElems; idx += 256) Bs[(idx >> 6) * LD + (idx & 63)] = (T)0;
-  __syncthreads();
-  if (t < kTile) {
-    const int col = t;
-    for (int r = col; r < kTile; r++) {
-      T s = (r == col) ? (T)1 : (T)0;
-      for (int k = col; k < r; k++) s -= As[r * LD + k] * Bs[k * LD + col];
-      Bs[r * LD + col] = s / As[r * LD + r];
+  // W = inv(L) by forward substitution, now with lane q of wave g holding
+  // column q of rows 16g..16g+15: W_kq = (delta_kq - sum_{m<k} L_km W_mq) / L_kk,
+  // the sum taken in ascending m.  Row k is divided by the wave owning it and
+  // published in As (row-major); every later row then subtracts L_rk W_kq.
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) a[jj] = (16 * w + jj == lane) ? (T)1 : (T)0;
+  for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) {
+      const int k = 16 * kb + kk;
+      if (w == kb) {
+        a[kk] = a[kk] / Bs[k * LD + k];
+        As[k * LD + lane] = a[kk];
+      }
+      __syncthreads();
+      const T wkq = As[k * LD + lane];
+      if (w > kb) {
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) a[jj] = a[jj] - Bs[k * LD + 16 * w + jj] * wkq;
+      } else if (w == kb) {
+#pragma unroll
+        for (int jj = kk + 1; jj < 16; jj++) a[jj] = a[jj] - Bs[k * LD + 16 * w + jj] * wkq;
+      }
     }
   }
-  __syncthreads();
-  for (int idx = t; idx < kTileElems; idx += 256) Cg[idx] = Bs[(idx >> 6) * LD + (idx & 63)];
+#pragma unroll
+  for (int jj = 0; jj < 16; jj++) Cg[(16 * w + jj) * kTile + lane] = a[jj];
 }
 
 template <typename T>

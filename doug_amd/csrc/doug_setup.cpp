@@ -20,9 +20,14 @@ static constexpr int kPending = kMaxInt / 2;          // parameters.F90:102
 // Row-parallel loops of the setup: static contiguous chunks, one per thread,
 // so that per-chunk outputs concatenate in row order.  Results do not
 // depend on the thread count.
-static int setup_threads() {
+static thread_local int t_thread_cap = 0;
+
+void set_setup_thread_cap(int cap) { t_thread_cap = cap; }
+
+int setup_threads() {
   unsigned hw = std::thread::hardware_concurrency();
-  return (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+  const int n = (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+  return t_thread_cap > 0 ? std::min(n, t_thread_cap) : n;
 }
 
 template <class F>
@@ -239,17 +244,28 @@ struct Grower {
     bool layerfits = true;
     int ntoadd = 0, nrounders = 0;
     if (layer <= 2 * neighood) {
+      // each node not yet placed is counted once per hit; a fresh one (stat
+      // 0) joins the new layer, with its distance, at its first hit.  The
+      // new layer holds distinct nodes here, so n + 1 slots always suffice;
+      // written without branches (the hit pattern is not predictable).
+      int* nl = newlayer.data();
+      int* st = stat.data();
+      int* di = distance.data();
+      const int* rsp = rs.data();
+      const int* cnp = cn.data();
       for (int k = 0; k < ninnodes; k++) {
-        int ln = inlayer[k];
-        for (int i = rs[ln]; i < rs[ln + 1]; i++) {
-          int c = cn[i];
-          if (stat[c] <= 0) {
-            if (stat[c] == 0) {
-              push(newlayer, ntoadd, c);
-              distance[c] = distance[ln] + 1;
-            }
-            stat[c] = stat[c] - 1;
-          }
+        const int ln = inlayer[k];
+        const int dn = di[ln] + 1;
+        const int e = rsp[ln + 1];
+        for (int i = rsp[ln]; i < e; i++) {
+          const int c = cnp[i];
+          const int s = st[c];
+          const int fresh = s == 0;
+          nl[ntoadd] = c;
+          ntoadd += fresh;
+          const int dc = di[c];
+          di[c] = fresh ? dn : dc;
+          st[c] = s - (s <= 0);
         }
       }
     }
@@ -330,7 +346,7 @@ struct Grower {
     }
     int ninnodes = 1;
     if (inlayer.empty()) inlayer.resize(64);
-    if (newlayer.empty()) newlayer.resize(64);
+    if (newlayer.size() < stat.size() + 1) newlayer.resize(stat.size() + 1);
     inlayer[0] = seed;
     return round_layer(0, ninnodes);
   }

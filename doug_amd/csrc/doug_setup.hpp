@@ -94,6 +94,13 @@ void coarse_partition(const Aggregates& cfull, const Aggregates& ffull, int n,
 void add_layers_all(const Csr& A, const std::vector<int>& pstarts, const std::vector<int>& pnodes,
                     int ol, std::vector<int>& sstarts, std::vector<int>& sinds);
 
+// Worker threads of the row-parallel setup loops started from the calling
+// thread: min(hardware threads, 16), or fewer after set_setup_thread_cap(c)
+// with c > 0 (c = 0 lifts the cap).  The cap is per calling thread; results
+// never depend on the thread count.
+int setup_threads();
+void set_setup_thread_cap(int cap);
+
 // Reverse Cuthill-McKee order of each block of `inds` over the graph of A
 // restricted to the block: out holds the same blocks, each a permutation of
 // its input.  Independent of the thread count.
