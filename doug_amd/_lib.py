@@ -33,6 +33,7 @@ class DougParams(C.Structure):
         ("smoothers", _i32), ("overlap", _i32), ("levels", _i32), ("method", _i32),
         ("fine_method", _i32), ("symmstruct", _i32), ("symmnumeric", _i32),
         ("omega", _dbl),
+        ("num_iters", _i32), ("reserved", _i32),
     ]
 
 

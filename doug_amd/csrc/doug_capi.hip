@@ -85,6 +85,11 @@ struct doug_ctx {
   int precision = DOUG_PREC_FP64;
   int32_t *d_inds = nullptr, *d_occ_ptr = nullptr, *d_occ_pos = nullptr;
   double* d_subsol = nullptr;
+  // fine_method 2: symmetric Gauss-Seidel on each subdomain's matrix
+  SgsDesc* d_sgs = nullptr;
+  int32_t *d_sgs_blk = nullptr, *d_sgs_col = nullptr, *d_sgs_info = nullptr;
+  double *d_sgs_val = nullptr, *d_sgs_diag = nullptr;
+  int sgs_mmax = 0, sgs_iters = 3;
   // coarse
   int nc = 0, cnt = 0;
   BlockDesc* d_cdesc = nullptr;
@@ -225,6 +230,8 @@ void doug_params_default(doug_params* p) {
   p->symmstruct = d.symmstruct;
   p->symmnumeric = d.symmnumeric;
   p->omega = d.omega;
+  p->num_iters = d.num_iters;
+  p->reserved = 0;
 }
 
 int doug_ctx_create(int device, doug_ctx** out) {
@@ -292,6 +299,7 @@ int doug_set_params(doug_ctx* ctx, const doug_params* q) {
   p.symmstruct = q->symmstruct;
   p.symmnumeric = q->symmnumeric;
   p.omega = q->omega;
+  p.num_iters = q->num_iters;
   ctx->ready = false;
   return DOUG_OK;
 }
@@ -808,14 +816,19 @@ static void phase_subdomains(doug_ctx* ctx) {
 static int check_params(doug_ctx* ctx) {
   if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "setup before doug_set_matrix_coo");
   const Params& p = ctx->p;
-  if (p.fine_method != 0 && p.fine_method != 1)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "fine_method must be 0 or 1 on this path");
-  if (p.method != 0 && p.method != 1)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none) or 1 (additive Schwarz)");
+  if (p.fine_method < 0 || p.fine_method > 2)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "fine_method must be 0 (none), 1 (exact solves) or 2 (SGS)");
+  // method 5 ("multiplicative on the fine level, additive with the coarse")
+  // is method 1 with one process: the coarse correction is restricted from
+  // the same residual and added to the same sum (pcg.F90:145-150,205-208).
+  // Methods 2-4 restrict a residual vector that the one-process preconditioner
+  // never computes (pcg.F90:146-147,253-255), so they have no defined result.
+  if (p.method != 0 && p.method != 1 && p.method != 5)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none), 1 or 5 (additive Schwarz)");
   int smoothers = p.smoothers;
   if (smoothers > p.radius1) smoothers = p.radius1;  // CoarseMtx.F90:75-78
-  if (smoothers != 0 && smoothers != 1)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "smoothers must resolve to 0 or 1 (got " + std::to_string(smoothers) + ")");
+  if (smoothers < 0)  // -1 selects an exact smoother the reference has commented out
+    return fail(ctx, DOUG_E_UNSUPPORTED, "smoothers must resolve to 0 or more (got " + std::to_string(smoothers) + ")");
   ctx->smoothers = smoothers;
   ctx->ol = p.overlap < 0 ? std::max(smoothers, 0) : p.overlap;  // aggr.F90:207-212
   ctx->has_coarse = p.levels > 1;
@@ -893,6 +906,123 @@ static int upload_matrix(doug_ctx* ctx) {
   return DOUG_OK;
 }
 
+// FinePreconditioner_sgs_Init (FinePreconditioner_sgs.F90:32-64): each
+// subdomain keeps the entries of the consolidated matrix (SpMtx_addAll,
+// SpMtx_operation.F90:728-759: rows and columns sorted, duplicates summed in
+// stored order) with both ends in its index set; the sweeps visit its rows
+// ascending and each row's entries by ascending column.  Rows go into blocks
+// of at most kSgsBlock rows and kSgsBlock off-diagonal entries for k_sgs.
+static int build_sgs(doug_ctx* ctx, const std::vector<int>& dinds) {
+  const Csr& A = ctx->A;
+  const int n = A.nrows;
+  const int nb = (int)ctx->sstarts.size() - 1;
+  struct Piece {
+    std::vector<int32_t> blk, col, info;
+    std::vector<double> val, diag;
+    int err_rows = 0, err_row = -1;
+  };
+  std::vector<Piece> pieces((size_t)nb);
+  int mmax = 0;
+  for (int b = 0; b < nb; b++) mmax = std::max(mmax, ctx->sstarts[b + 1] - ctx->sstarts[b]);
+  if (mmax > kSgsMaxRows)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "subdomain of " + std::to_string(mmax) +
+                                             " unknowns exceeds the Gauss-Seidel limit " +
+                                             std::to_string(kSgsMaxRows));
+  const int nth = std::max(1, std::min(setup_threads(), nb));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; t++)
+    th.emplace_back([&, t]() {
+      std::vector<int> floc((size_t)n, -1);
+      std::vector<std::pair<int, double>> ent;
+      for (int b = t; b < nb; b += nth) {
+        Piece& pc = pieces[b];
+        const int s0 = ctx->sstarts[b], m = ctx->sstarts[b + 1] - s0;
+        for (int r = 0; r < m; r++) floc[dinds[s0 + r]] = r;
+        pc.info.resize((size_t)m);
+        pc.diag.assign((size_t)m, 0.0);
+        int bent = 0, brows = 0;
+        auto open_block = [&](int r) {
+          pc.blk.push_back(r);
+          pc.col.resize(pc.col.size() + kSgsBlock, 0);
+          pc.val.resize(pc.val.size() + kSgsBlock, 0.0);
+          bent = brows = 0;
+        };
+        for (int r = 0; r < m; r++) {
+          const int g = dinds[s0 + r];
+          ent.clear();
+          for (int q = A.ptr[g]; q < A.ptr[g + 1]; q++)
+            if (floc[A.col[q]] >= 0) ent.push_back({A.col[q], A.val[q]});
+          std::stable_sort(ent.begin(), ent.end(),
+                           [](const auto& x, const auto& y) { return x.first < y.first; });
+          size_t w = 0;  // consolidate: duplicates summed in stored order
+          for (size_t q = 0; q < ent.size(); q++) {
+            if (w > 0 && ent[w - 1].first == ent[q].first)
+              ent[w - 1].second = ent[w - 1].second + ent[q].second;
+            else
+              ent[w++] = ent[q];
+          }
+          ent.resize(w);
+          int len = 0;
+          for (auto& e : ent) {
+            if (e.first == g)
+              pc.diag[r] = e.second;
+            else
+              len++;
+          }
+          if (len > kSgsBlock) {
+            pc.err_rows++;
+            pc.err_row = g;
+            len = 0;
+            ent.clear();
+          }
+          if (pc.blk.empty() || brows == kSgsBlock || bent + len > kSgsBlock) open_block(r);
+          const size_t base = (pc.blk.size() - 1) * kSgsBlock;
+          const int s = bent;
+          for (auto& e : ent)
+            if (e.first != g) {
+              pc.col[base + bent] = floc[e.first];
+              pc.val[base + bent] = e.second;
+              bent++;
+            }
+          pc.info[r] = s | (bent << 8);
+          brows++;
+          pc.blk.back() += 1 << 16;
+        }
+        for (int r = 0; r < m; r++) floc[dinds[s0 + r]] = -1;
+      }
+    });
+  for (auto& x : th) x.join();
+  for (int b = 0; b < nb; b++)
+    if (pieces[b].err_rows)
+      return fail(ctx, DOUG_E_UNSUPPORTED,
+                  "row " + std::to_string(pieces[b].err_row + 1) + " has more than " +
+                      std::to_string(kSgsBlock) + " off-diagonal entries inside a subdomain");
+  std::vector<SgsDesc> desc((size_t)nb);
+  std::vector<int32_t> blk, col, info;
+  std::vector<double> val, diag;
+  info.reserve(dinds.size());
+  diag.reserve(dinds.size());
+  for (int b = 0; b < nb; b++) {
+    const Piece& pc = pieces[b];
+    desc[b] = SgsDesc{(int64_t)ctx->sstarts[b], (int32_t)blk.size(), (int32_t)pc.blk.size(),
+                      ctx->sstarts[b + 1] - ctx->sstarts[b], 0};
+    blk.insert(blk.end(), pc.blk.begin(), pc.blk.end());
+    col.insert(col.end(), pc.col.begin(), pc.col.end());
+    val.insert(val.end(), pc.val.begin(), pc.val.end());
+    info.insert(info.end(), pc.info.begin(), pc.info.end());
+    diag.insert(diag.end(), pc.diag.begin(), pc.diag.end());
+  }
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_sgs, desc)) || (rc = upload(ctx, &ctx->d_sgs_blk, blk)) ||
+      (rc = upload(ctx, &ctx->d_sgs_col, col)) || (rc = upload(ctx, &ctx->d_sgs_val, val)) ||
+      (rc = upload(ctx, &ctx->d_sgs_info, info)) || (rc = upload(ctx, &ctx->d_sgs_diag, diag)))
+    return rc;
+  ctx->sgs_mmax = mmax;
+  ctx->sgs_iters = ctx->p.num_iters > 0 ? ctx->p.num_iters : 3;  // aggr.F90:220-222
+  prepare_sgs(mmax);
+  return DOUG_OK;
+}
+
 static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const Params& p = ctx->p;
   doug_info& info = ctx->info;
@@ -906,14 +1036,18 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   // the reference's order): reverse Cuthill-McKee narrows the factor's
   // tile envelope
   std::vector<int> dinds;
-  if (p.fine_method == 1 && ctx->local_order == 1)
+  if (p.fine_method == 1 && ctx->local_order == 1) {
     block_rcm(A, ctx->sstarts, ctx->sinds, dinds);
-  else
+  } else {
     dinds = ctx->sinds;
+    if (p.fine_method == 2)  // Gauss-Seidel sweeps the rows in ascending order
+      for (size_t b = 0; b + 1 < ctx->sstarts.size(); b++)
+        std::sort(dinds.begin() + ctx->sstarts[b], dinds.begin() + ctx->sstarts[b + 1]);
+  }
   // occurrence lists for the deterministic overlap sum (a node's
   // occurrences in ascending subdomain order)
   std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(dinds.size());
-  if (p.fine_method == 1) {
+  if (p.fine_method >= 1) {
     for (int v : dinds) occ_ptr[v + 1]++;
     for (int i = 0; i < n; i++) occ_ptr[i + 1] += occ_ptr[i];
     std::vector<int32_t> fill(occ_ptr.begin(), occ_ptr.end() - 1);
@@ -940,6 +1074,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   HIPCHK(hipStreamSynchronize(ctx->stream));  // host staging vectors go out of scope
   info.t_upload += now() - t3;
 
+  if (p.fine_method == 2 && (rc = build_sgs(ctx, dinds))) return rc;
   // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
   int64_t elems = 0;
   size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
@@ -1171,6 +1306,12 @@ static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
 }
 
 static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
+  if (ctx->p.fine_method == 2) {  // FinePreconditioner_sgs_Apply
+    launch_gather_positions((int64_t)ctx->sinds.size(), ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    launch_sgs(ctx->d_sgs, ctx->nsub, ctx->sgs_mmax, ctx->d_sgs_blk, ctx->d_sgs_col, ctx->d_sgs_val,
+               ctx->d_sgs_info, ctx->d_sgs_diag, ctx->d_subsol, ctx->sgs_iters, ctx->stream);
+    return;
+  }
   if (ctx->precision == DOUG_PREC_FP32_FACTORS)
     launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, ctx->d_subsol,
                 ctx->stream);
@@ -1193,7 +1334,7 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
     launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cu, ctx->d_csol, ctx->d_cpart, ctx->stream);
   }
   const double* fine_src = r;
-  if (p.fine_method == 1) {
+  if (p.fine_method >= 1) {
     enqueue_subdomain_solves(ctx, r);
     fine_src = ctx->d_subsol;
   }
@@ -1437,6 +1578,8 @@ int doug_prepare_batch(doug_ctx* ctx) {
   if (!ctx) return DOUG_E_ARG;
   if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_prepare_batch before doug_setup");
   if (ctx->batch_ready) return DOUG_OK;
+  if (ctx->p.fine_method == 2)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "the batched solve has no Gauss-Seidel fine level (fine_method 2)");
   const int n = ctx->n, NR = kMaxRhs;
   hipStream_t st = ctx->stream;
   int rc;
@@ -1698,7 +1841,7 @@ int doug_time_operator(doug_ctx* ctx, int32_t which, int32_t reps, double* ms) {
     switch (which) {
       case 0: enqueue_spmv(ctx, ctx->d_p, ctx->d_q); break;
       case 1:
-        if (ctx->p.fine_method == 1) enqueue_subdomain_solves(ctx, ctx->d_r);
+        if (ctx->p.fine_method >= 1) enqueue_subdomain_solves(ctx, ctx->d_r);
         break;
       case 2: enqueue_prec(ctx, ctx->d_r, ctx->d_z); break;
       case 3: launch_copy(ctx->d_q, ctx->d_p, ctx->n, st); break;

@@ -1517,6 +1517,130 @@ DOUG_INSTANTIATE(float)
 #undef DOUG_INSTANTIATE
 
 
+// ------------------------------------------------------------------------
+// Symmetric Gauss-Seidel sweeps (SymGaussSeidel, stationary.F90:33-88):
+// x_i = (b_i - sum_{j != i} a_ij x_j) / a_ii in ascending column order, rows
+// ascending then descending, rows without a diagonal entry left at zero.
+// The rows are a sequential chain, so one wavefront owns a subdomain: the
+// lanes form a row's products a_ij x_j from the iterate in LDS and the
+// subtractions run in entry order on values read across lanes.  The next
+// block's entries and row data are loaded while the current block runs.
+// ------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+struct SgsRows {
+  int row0, nrows;
+  int col, inf;
+  double val, dg, rhs;
+};
+
+__device__ __forceinline__ void sgs_load(SgsRows& B, const SgsDesc& b, int k,
+                                         const int32_t* __restrict__ blk,
+                                         const int32_t* __restrict__ ecol,
+                                         const double* __restrict__ eval,
+                                         const int32_t* __restrict__ info,
+                                         const double* __restrict__ diag, const double* sub,
+                                         int lane) {
+  const int64_t id = (int64_t)b.blk0 + k;
+  const int pk = blk[id];
+  B.row0 = pk & 0xffff;
+  B.nrows = pk >> 16;
+  B.col = ecol[id * kSgsBlock + lane];
+  B.val = eval[id * kSgsBlock + lane];
+  const bool in = lane < B.nrows;
+  const int64_t pos = b.off + B.row0 + lane;
+  B.inf = in ? info[pos] : 0;
+  B.dg = in ? diag[pos] : 0.0;
+  B.rhs = in ? sub[pos] : 0.0;
+}
+
+__device__ __forceinline__ void sgs_row(const SgsRows& B, int rr, double* xs, int lane) {
+  const double d = lane_value(B.dg, rr);
+  if (d == 0.0) return;  // no diagonal entry: the row is skipped
+  const int se = __builtin_amdgcn_readlane(B.inf, rr);
+  const int s = se & 0xff, e = se >> 8;
+  double p = 0.0;
+  if (lane >= s && lane < e) p = B.val * xs[B.col];
+  double xi = lane_value(B.rhs, rr);
+  for (int k = s; k < e; k++) xi = xi - lane_value(p, k);
+  xi = xi / d;
+  if (lane == 0) xs[B.row0 + rr] = xi;
+  // one wavefront: its LDS accesses complete in order; keep the compiler
+  // from moving the next row's reads above this write
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(64) void k_sgs(const SgsDesc* __restrict__ d,
+                                            const int32_t* __restrict__ blk,
+                                            const int32_t* __restrict__ ecol,
+                                            const double* __restrict__ eval,
+                                            const int32_t* __restrict__ info,
+                                            const double* __restrict__ diag, double* sub,
+                                            int iters) {
+  extern __shared__ double xs[];
+  const SgsDesc b = d[blockIdx.x];
+  const int lane = threadIdx.x;
+  for (int r = lane; r < b.m; r += 64) xs[r] = 0.0;
+  __syncthreads();
+  for (int it = 0; it < iters && b.nblk > 0; it++) {
+    SgsRows cur, nxt;
+    sgs_load(cur, b, 0, blk, ecol, eval, info, diag, sub, lane);
+    nxt = cur;
+    for (int k = 0; k < b.nblk; k++) {
+      if (k + 1 < b.nblk) sgs_load(nxt, b, k + 1, blk, ecol, eval, info, diag, sub, lane);
+      for (int rr = 0; rr < cur.nrows; rr++) sgs_row(cur, rr, xs, lane);
+      cur = nxt;
+    }
+    sgs_load(cur, b, b.nblk - 1, blk, ecol, eval, info, diag, sub, lane);
+    nxt = cur;
+    for (int k = b.nblk - 1; k >= 0; k--) {
+      if (k > 0) sgs_load(nxt, b, k - 1, blk, ecol, eval, info, diag, sub, lane);
+      for (int rr = cur.nrows - 1; rr >= 0; rr--) sgs_row(cur, rr, xs, lane);
+      cur = nxt;
+    }
+  }
+  __syncthreads();
+  for (int r = lane; r < b.m; r += 64) sub[b.off + r] = xs[r];
+}
+
+__global__ void k_gather_positions(int64_t n, const int32_t* __restrict__ idx,
+                                   const double* __restrict__ x, double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) out[p] = x[idx[p]];
+}
+
+}  // namespace
+
+void prepare_sgs(int mmax) {
+  const int bytes = mmax * (int)sizeof(double);
+  if (bytes > 64 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sgs),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+void launch_gather_positions(int64_t n, const int32_t* idx, const double* x, double* out,
+                             hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_positions, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     idx, x, out);
+}
+
+void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* blk, const int32_t* ecol,
+                const double* eval, const int32_t* info, const double* diag, double* sub, int iters,
+                hipStream_t st) {
+  if (nsub <= 0) return;
+  const size_t bytes = (size_t)std::max(mmax, 1) * sizeof(double);
+  hipLaunchKernelGGL(k_sgs, dim3(nsub), dim3(64), bytes, st, d, blk, ecol, eval, info, diag, sub,
+                     iters);
+}
+
 void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                  const double* val, const double* x, double* y, hipStream_t st, int shape) {
   if (nrb <= 0) return;

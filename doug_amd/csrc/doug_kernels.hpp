@@ -98,6 +98,30 @@ void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u
                     double* part, hipStream_t st);
 // z[i] = sum over occurrences (ascending subdomain) of sol, then
 //        + sum_a Rt(i,a) csol[a]  (coarse prolongation), fine part first.
+// Symmetric Gauss-Seidel sweeps over each subdomain's own matrix (the
+// fine_method 2 preconditioner): one wavefront per subdomain, its iterate in
+// LDS, rows in blocks of at most kSgsBlock rows and kSgsBlock off-diagonal
+// entries (entries of block k at [k * kSgsBlock, (k + 1) * kSgsBlock)).
+constexpr int kSgsBlock = 64;
+constexpr int kSgsMaxRows = 16384;  // LDS holds the iterate
+struct SgsDesc {
+  int64_t off;   // first position in the subdomain-vector layout
+  int32_t blk0;  // first block
+  int32_t nblk;
+  int32_t m;     // rows (the subdomain's unknowns, ascending)
+  int32_t pad;
+};
+void prepare_sgs(int mmax);
+// out[p] = x[idx[p]] for p < n
+void launch_gather_positions(int64_t n, const int32_t* idx, const double* x, double* out,
+                             hipStream_t st);
+// blk[k] = first local row | rows << 16; info[p] = first entry | end << 8
+// inside the row's block; sub holds the right-hand side on entry and the
+// iterate after `iters` forward and backward sweeps on return
+void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* blk, const int32_t* ecol,
+                const double* eval, const int32_t* info, const double* diag, double* sub, int iters,
+                hipStream_t st);
+
 void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
                         const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,
                         const double* csol, double* z, hipStream_t st);

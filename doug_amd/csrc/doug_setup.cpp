@@ -309,6 +309,11 @@ struct Grower {
       }
     }
     if (layer <= neighood && nneigs + ninnodes > maxasize) layerfits = false;
+    // a rounder's fresh neighbours are appended without being marked, so a
+    // node can enter a layer twice and the explored list can outgrow n when
+    // the layers cover the whole graph (the reference's nodes(n) would
+    // overflow there): grow the list instead
+    if (nodes.size() < (size_t)nneigs + (size_t)ninnodes) nodes.resize(2 * ((size_t)nneigs + ninnodes));
     if (layerfits) {
       for (int k = 0; k < ninnodes; k++) {
         stat[inlayer[k]] = layer + 1;
@@ -671,6 +676,76 @@ Csr transpose(const Csr& A) {
   return T;
 }
 
+// IntRestBuild with smoothers >= 2 (CoarseMtx.F90:236-417).  The second
+// smoother S2 has the same entries as S but its own diagonal: the loop that
+// forms it assigns a_ii when it meets the diagonal entry and adds the weak
+// entries met afterwards (CoarseMtx.F90:238-245), so weak entries stored
+// before the diagonal drop out (the reference's array is not cleared first;
+// a row without a diagonal entry starts from zero here).  SS = S2 S, and the
+// restriction is T (SS)^T for 2 (and above 8), T (SS S2) for 3, T (SS SS) for
+// 4, T (SS SS S2) for 5, T (SS SS SS) for 6, T (SS SS SS S2) for 7 and
+// T ((SS SS)(SS SS)) for 8, every product in first-touch order.  Rows of the
+// result are returned sorted by column, as the later products use them.
+static Csr restriction_powers(const Csr& A, const std::vector<uint8_t>& strong,
+                              const Aggregates& ag, int smoothers, double omega) {
+  const int n = A.nrows;
+  std::vector<double> d1((size_t)n, 0.0), d2((size_t)n, 0.0);
+  parallel_rows(n, [&](int i) {
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      if (A.col[p] == i) {
+        d1[i] = d1[i] + A.val[p];
+        d2[i] = A.val[p];
+      } else if (!strong[p]) {
+        d1[i] = d1[i] + A.val[p];
+        d2[i] = d2[i] + A.val[p];
+      }
+    }
+  });
+  auto smoother = [&](const std::vector<double>& diag) {
+    Csr S;
+    S.nrows = S.ncols = n;
+    S.ptr = A.ptr;
+    S.col = A.col;
+    S.val.resize(A.val.size());
+    const double one_minus = 1.0 - omega;
+    parallel_rows(n, [&](int i) {
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+        if (A.col[p] == i)
+          S.val[p] = one_minus;
+        else if (strong[p])
+          S.val[p] = -omega / diag[i] * A.val[p];
+        else
+          S.val[p] = 0.0;
+      }
+    });
+    return S;
+  };
+  const Csr S = smoother(d1), S2 = smoother(d2);
+  const Csr SS = first_touch_product(S2, S);
+  Csr X;
+  switch (smoothers) {
+    case 3: X = first_touch_product(SS, S2); break;
+    case 4: X = first_touch_product(SS, SS); break;
+    case 5: X = first_touch_product(first_touch_product(SS, SS), S2); break;
+    case 6: X = first_touch_product(first_touch_product(SS, SS), SS); break;
+    case 7: X = first_touch_product(first_touch_product(first_touch_product(SS, SS), SS), S2); break;
+    case 8: {
+      const Csr S4 = first_touch_product(SS, SS);
+      X = first_touch_product(S4, S4);
+      break;
+    }
+    default: X = transpose(SS); break;  // 2, and the reference's fallback above 8
+  }
+  Csr T;
+  T.nrows = ag.nagr;
+  T.ncols = n;
+  T.ptr = ag.starts;
+  T.col = ag.nodes;
+  T.val.assign(T.col.size(), 1.0);
+  // the values do not depend on where a column first appears; sort each row
+  return transpose(transpose(first_touch_product(T, X)));
+}
+
 Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregates& ag,
                 int smoothers, double omega) {
   const int n = A.nrows, nagr = ag.nagr;
@@ -683,6 +758,7 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
     R.val.assign(R.col.size(), 1.0);
     return R;
   }
+  if (smoothers >= 2) return restriction_powers(A, strong, ag, smoothers, omega);
   // filtered diagonal and the Jacobi smoother S (CoarseMtx.F90:152-233)
   std::vector<double> diag((size_t)n, 0.0);
   parallel_rows(n, [&](int i) {
@@ -745,75 +821,49 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
   return transpose(Rt);
 }
 
-Csr galerkin(const Csr& A, const Csr& R, const Csr& Rt) {
-  const int n = A.nrows, nc = R.nrows;
+// SpMtx_AB (SpMtx_op_AB.f90:39-200) for row-arranged operands: row i
+// visits X's row in stored order and, for each entry (i, k), Y's row k in
+// stored order; the columns of a row come in first-touch order, each entry
+// first assigned its product, then accumulated.
+Csr first_touch_product(const Csr& X, const Csr& Y) {
+  const int nr = X.nrows, nc = Y.ncols;
   const int nth = setup_threads();
-  // T' = A R^T: row i visits A's row in stored order, then column k of R in
-  // ascending aggregate order (SpMtx_op_AB.f90 with BT); columns of a row in
-  // first-touch order, each entry first assigned then accumulated.
-  Csr Tp;
-  Tp.nrows = n;
-  Tp.ncols = nc;
-  {
-    std::vector<CsrPiece> pieces((size_t)nth);
-    parallel_chunks(n, nth, [&](int t, int b, int e) {
-      CsrPiece& out = pieces[t];
-      std::vector<int> mark((size_t)nc, -1), pos((size_t)nc, 0);
-      for (int i = b; i < e; i++) {
-        size_t start = out.col.size();
-        for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
-          int k = A.col[p];
-          for (int q = Rt.ptr[k]; q < Rt.ptr[k + 1]; q++) {
-            int jj = Rt.col[q];
-            double prod = A.val[p] * Rt.val[q];
-            if (mark[jj] != i) {
-              mark[jj] = i;
-              pos[jj] = (int)out.col.size();
-              out.col.push_back(jj);
-              out.val.push_back(prod);
-            } else {
-              double& v = out.val[pos[jj]];
-              v = v + prod;
-            }
+  Csr C;
+  C.nrows = nr;
+  C.ncols = nc;
+  std::vector<CsrPiece> pieces((size_t)nth);
+  parallel_chunks(nr, nth, [&](int t, int b, int e) {
+    CsrPiece& out = pieces[t];
+    std::vector<int> mark((size_t)nc, -1), pos((size_t)nc, 0);
+    for (int i = b; i < e; i++) {
+      size_t start = out.col.size();
+      for (int p = X.ptr[i]; p < X.ptr[i + 1]; p++) {
+        int k = X.col[p];
+        for (int q = Y.ptr[k]; q < Y.ptr[k + 1]; q++) {
+          int jj = Y.col[q];
+          double prod = X.val[p] * Y.val[q];
+          if (mark[jj] != i) {
+            mark[jj] = i;
+            pos[jj] = (int)out.col.size();
+            out.col.push_back(jj);
+            out.val.push_back(prod);
+          } else {
+            double& v = out.val[pos[jj]];
+            v = v + prod;
           }
         }
-        out.rowlen.push_back((int)(out.col.size() - start));
       }
-    });
-    join_pieces(pieces, n, Tp);
-  }
+      out.rowlen.push_back((int)(out.col.size() - start));
+    }
+  });
+  join_pieces(pieces, nr, C);
+  return C;
+}
+
+Csr galerkin(const Csr& A, const Csr& R, const Csr& Rt) {
+  // T' = A R^T: R^T's rows list aggregates ascending (R arranged by columns);
   // A_c = R T' with R's rows in ascending column order.
-  Csr Ac;
-  Ac.nrows = Ac.ncols = nc;
-  {
-    std::vector<CsrPiece> pieces((size_t)nth);
-    parallel_chunks(nc, nth, [&](int t, int b, int e) {
-      CsrPiece& out = pieces[t];
-      std::vector<int> mark((size_t)nc, -1), pos((size_t)nc, 0);
-      for (int i = b; i < e; i++) {
-        size_t start = out.col.size();
-        for (int p = R.ptr[i]; p < R.ptr[i + 1]; p++) {
-          int kk = R.col[p];
-          for (int q = Tp.ptr[kk]; q < Tp.ptr[kk + 1]; q++) {
-            int jj = Tp.col[q];
-            double prod = R.val[p] * Tp.val[q];
-            if (mark[jj] != i) {
-              mark[jj] = i;
-              pos[jj] = (int)out.col.size();
-              out.col.push_back(jj);
-              out.val.push_back(prod);
-            } else {
-              double& v = out.val[pos[jj]];
-              v = v + prod;
-            }
-          }
-        }
-        out.rowlen.push_back((int)(out.col.size() - start));
-      }
-    });
-    join_pieces(pieces, nc, Ac);
-  }
-  return Ac;
+  return first_touch_product(R, first_touch_product(A, Rt));
 }
 
 void coarse_partition(const Aggregates& cfull, const Aggregates& ffull, int n,

@@ -29,6 +29,7 @@ struct Params {
   int smoothers = -1, overlap = -1, levels = 2, method = 1, fine_method = 1;
   int symmstruct = 0, symmnumeric = 0;
   double omega = 0.33333333333333;  // CoarseMtx.F90:64
+  int num_iters = -1;               // fine_method 2 sweeps (aggr.F90:220-222)
 };
 
 // Aggregates (Aggregate_mod.F90:39-51): num is 1-based aggregate number per
@@ -72,14 +73,20 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
                int neighood, int minasize, int maxasize, double alpha, bool coarse_level,
                ProcessState& ps, Aggregates& inner, Aggregates& full);
 
-// IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 and 1.  Returned as
-// CSR with each row sorted by column (the order prec2Level restricts in).
+// IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 to 8 (above 8 the
+// reference falls back to 2).  Returned as CSR with each row sorted by
+// column: CoarseMtxBuild arranges R by columns in place, and every later
+// use of R (the Galerkin product, prec2Level) sees that order.
 Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregates& inner,
                 int smoothers, double omega);
 
 // CoarseMtxBuild (CoarseMtx.F90:476-516): A_c = R (A R^T), rows in the
 // reference's first-touch column order; Rt = transpose(R).
 Csr galerkin(const Csr& A, const Csr& R, const Csr& Rt);
+
+// SpMtx_AB (SpMtx_op_AB.f90:39-200) on row-arranged operands: output rows
+// in first-touch column order, products summed in traversal order.
+Csr first_touch_product(const Csr& X, const Csr& Y);
 
 // Transpose keeping ascending source-row order inside each output row.
 Csr transpose(const Csr& A);

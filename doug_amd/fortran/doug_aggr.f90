@@ -99,6 +99,7 @@ program doug_aggr
   if (c%levels >= 1) p%levels = c%levels
   if (c%method >= 0) p%method = c%method
   if (c%fine_method >= 0) p%fine_method = c%fine_method
+  p%num_iters = c%num_iters
   p%symmstruct = merge(1, 0, c%symmstruct)
   p%symmnumeric = merge(1, 0, c%symmnumeric)
   call check(doug_set_params(ctx, p))

@@ -12,6 +12,7 @@ module doug_capi
     integer(c_int32_t) :: smoothers, overlap, levels, method, fine_method
     integer(c_int32_t) :: symmstruct, symmnumeric
     real(c_double) :: omega
+    integer(c_int32_t) :: num_iters, reserved
   end type doug_params
 
   type, bind(C) :: doug_info

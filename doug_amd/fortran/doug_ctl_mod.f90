@@ -14,7 +14,7 @@ module doug_ctl
     integer :: num_subdomains = -1, levels = -1, overlap = -1, smoothers = -1
     integer :: input_type = -1, radius1 = -1, radius2 = -1
     integer :: minasize1 = -1, minasize2 = -1, maxasize1 = -1, maxasize2 = -1
-    integer :: debug = -1, verbose = -1, plotting = -1, solve_maxiters = -1
+    integer :: debug = -1, verbose = -1, plotting = -1, solve_maxiters = -1, num_iters = -1
     real(rk) :: strong1 = -1.0_rk, strong2 = -1.0_rk, solve_tolerance = -1.0_rk
     logical :: symmstruct = .false., symmnumeric = .false.
     character(L) :: assembled_mtx_file = '', assembled_rhs_file = '', solution_file = ''
@@ -158,6 +158,7 @@ contains
       case ('solver');               call set_int(c%solver, w1, w2)
       case ('method');               call set_int(c%method, w1, w2)
       case ('fine_method');          call set_int(c%fine_method, w1, w2)
+      case ('num_iters');            call set_int(c%num_iters, w1, w2)
       case ('coarse_method');        call set_int(c%coarse_method, w1, w2)
       case ('num_subdomains');       call set_int(c%num_subdomains, w1, w2)
       case ('levels');               call set_int(c%levels, w1, w2)

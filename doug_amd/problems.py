@@ -67,14 +67,24 @@ def diffusion3d(N: int, sigma: float = 1.0, seed: int = 0, kappa: np.ndarray | N
     """
     if kappa is None:
         kappa = np.exp(sigma * smoothed_gaussian(N, seed))
-    kappa = np.asarray(kappa, np.float64).reshape(N, N, N)
-    n = N ** 3
-    idx = np.arange(n).reshape(N, N, N)
-    diag = np.zeros((N, N, N))
+    return _diffusion(N, 3, kappa)
+
+
+def diffusion2d(N: int, sigma: float = 1.0, seed: int = 0):
+    """The 5-point analogue of diffusion3d on an N^2 grid (kappa from a
+    smoothed Gaussian field on N^2 cells)."""
+    return _diffusion(N, 2, np.exp(sigma * smoothed_gaussian(N, seed, dims=2)))
+
+
+def _diffusion(N: int, dims: int, kappa: np.ndarray):
+    kappa = np.asarray(kappa, np.float64).reshape((N,) * dims)
+    n = N ** dims
+    idx = np.arange(n).reshape((N,) * dims)
+    diag = np.zeros((N,) * dims)
     rows, cols, vals = [], [], []
-    for ax in range(3):
-        sl_a = [slice(None)] * 3
-        sl_b = [slice(None)] * 3
+    for ax in range(dims):
+        sl_a = [slice(None)] * dims
+        sl_b = [slice(None)] * dims
         sl_a[ax] = slice(0, N - 1)
         sl_b[ax] = slice(1, N)
         ka, kb = kappa[tuple(sl_a)], kappa[tuple(sl_b)]
@@ -85,8 +95,8 @@ def diffusion3d(N: int, sigma: float = 1.0, seed: int = 0, kappa: np.ndarray | N
         rows += [ia, ib]
         cols += [ib, ia]
         vals += [-kf.ravel(), -kf.ravel()]
-        lo = [slice(None)] * 3
-        hi = [slice(None)] * 3
+        lo = [slice(None)] * dims
+        hi = [slice(None)] * dims
         lo[ax] = 0
         hi[ax] = N - 1
         diag[tuple(lo)] += kappa[tuple(lo)]

@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 
 CTL_INT_KEYS = ("radius1", "radius2", "minasize1", "minasize2", "maxasize1", "maxasize2",
-                "smoothers", "overlap", "levels", "method", "fine_method")
+                "smoothers", "overlap", "levels", "method", "fine_method", "num_iters")
 
 
 class DougError(RuntimeError):

@@ -57,6 +57,9 @@ typedef struct {
   /* prolongator damping: DOUG hard-codes 0.33333333333333
    * (src/coarse/CoarseMtx.F90:64); other values are a documented extension */
   double omega;
+  /* symmetric Gauss-Seidel sweeps of fine_method 2 (ctl num_iters; -1 or
+   * less than 1 means 3, src/main/aggr.F90:220-222) */
+  int32_t num_iters, reserved;
 } doug_params;
 
 /* Sizes and timings of the last doug_setup (seconds).  factor_bytes and

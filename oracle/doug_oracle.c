@@ -86,6 +86,7 @@ void or_params_default(or_params *p) {
   p->symmstruct = 0;
   p->symmnumeric = 0;
   p->omega = 0.33333333333333;
+  p->num_iters = -1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -381,7 +382,7 @@ static void form_aggr(Aggrs *ag, int nagrs, int n, const int *aggrnum) {
 /* work buffers of lets_colour (Aggregate_mod.F90:287-321) */
 typedef struct {
   int *inlayer, *newlayer, *roundernodes;
-  int cap;
+  int cap, nodes_cap;
 } Lbuf;
 
 static void lcheck(const Lbuf *b, int idx) {
@@ -469,6 +470,12 @@ static int can_add_roundlayer(int layer, int *ninnodes, Lbuf *b, int neighood,
     }
   }
   if (layer <= neighood && *nneigs + *ninnodes > maxasize) layerfits = 0;
+  if (*nneigs + *ninnodes > b->nodes_cap) {
+    /* duplicate layer entries (a rounder's fresh neighbours are not marked)
+     * made the explored list outgrow the reference's nodes(n) */
+    fprintf(stderr, "doug_oracle: explored list exceeds n (%d > %d)\n", *nneigs + *ninnodes, b->nodes_cap);
+    abort();
+  }
   if (layerfits) {
     for (int k = 1; k <= *ninnodes; k++) {
       stat[inlayer[k]] = layer + 1;
@@ -537,6 +544,7 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
     long capl = virt > n ? virt : n;
     if (capl > 4L * n + 16) capl = 4L * n + 16;
     lb.cap = (int)capl;
+    lb.nodes_cap = n;
     lb.inlayer = IALLOC(lb.cap);
     lb.newlayer = IALLOC(lb.cap);
     lb.roundernodes = IALLOC(maxasize);
@@ -937,7 +945,8 @@ static void om_keep_columns_upto(OM *A, int ninner) {
   }
 }
 
-/* IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 and 1. */
+/* IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 to 8 (above 8 the
+ * reference warns and builds 2). */
 static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
   int nagr = ag->nagr;
   long nz = ag->starts[nagr + 1] - 1;
@@ -977,7 +986,76 @@ static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
   }
   S->arr = ARR_NO;
   free(diag);
-  OM *R = om_AB(T, S, 0, 1);
+  if (smoothers == 1) {
+    OM *R = om_AB(T, S, 0, 1);
+    om_free(S);
+    om_free(T);
+    return R;
+  }
+  /* second smoother (CoarseMtx.F90:236-320): its diagonal array is assigned
+   * a_ii at the diagonal entry and accumulates the weak entries met after
+   * it; the reference does not clear the array first (zero here). */
+  double *diag2 = DALLOC(nd);
+  for (long i = 1; i <= A->nnz; i++) {
+    if (A->indi[i] == A->indj[i])
+      diag2[A->indi[i]] = A->val[i];
+    else if (!A->strong[i])
+      diag2[A->indi[i]] = diag2[A->indi[i]] + A->val[i];
+  }
+  OM *S2 = om_new(A->nnz, Tnrows, Tnrows);
+  for (long i = 1; i <= A->nnz; i++) {
+    S2->indi[i] = A->indi[i];
+    S2->indj[i] = A->indj[i];
+    if (A->indi[i] == A->indj[i])
+      S2->val[i] = one_minus;
+    else if (A->strong[i])
+      S2->val[i] = -omega / diag2[A->indi[i]] * A->val[i];
+    else
+      S2->val[i] = 0.0;
+  }
+  S2->arr = ARR_NO;
+  free(diag2);
+  OM *SS = om_AB(S2, S, 0, 0);
+  OM *R;
+  if (smoothers == 3) {
+    OM *SSS = om_AB(SS, S2, 0, 0);
+    R = om_AB(T, SSS, 0, 0);
+    om_free(SSS);
+  } else if (smoothers == 4) {
+    OM *SSSS = om_AB(SS, SS, 0, 0);
+    R = om_AB(T, SSSS, 0, 0);
+    om_free(SSSS);
+  } else if (smoothers == 5) {
+    OM *SSSS = om_AB(SS, SS, 0, 0);
+    OM *SSS = om_AB(SSSS, S2, 0, 0);
+    R = om_AB(T, SSS, 0, 0);
+    om_free(SSSS);
+    om_free(SSS);
+  } else if (smoothers == 6) {
+    OM *SSSS = om_AB(SS, SS, 0, 0);
+    OM *SSS = om_AB(SSSS, SS, 0, 0);
+    R = om_AB(T, SSS, 0, 0);
+    om_free(SSSS);
+    om_free(SSS);
+  } else if (smoothers == 7) {
+    OM *SSSS = om_AB(SS, SS, 0, 0);
+    OM *SSS = om_AB(SSSS, SS, 0, 0);
+    om_free(SSSS);
+    SSSS = om_AB(SSS, S2, 0, 0);
+    om_free(SSS);
+    R = om_AB(T, SSSS, 0, 0);
+    om_free(SSSS);
+  } else if (smoothers == 8) {
+    OM *SSSS = om_AB(SS, SS, 0, 0);
+    OM *SSS = om_AB(SSSS, SSSS, 0, 0);
+    R = om_AB(T, SSS, 0, 0);
+    om_free(SSSS);
+    om_free(SSS);
+  } else {
+    R = om_AB(T, SS, 0, 1);
+  }
+  om_free(SS);
+  om_free(S2);
   om_free(S);
   om_free(T);
   return R;
@@ -1208,6 +1286,12 @@ struct or_state {
   int *sstarts;     /* 1..nsub+1 */
   int *sinds;       /* 1..total */
   Env **fact;       /* per subdomain */
+  /* fine_method 2: per subdomain, the consolidated matrix's entries with
+   * both ends in the subdomain (rows ascending, columns ascending) */
+  long *sgs_start;  /* 1..nsub+1 into sgs_i/j/v (1-based) */
+  int *sgs_i, *sgs_j;
+  double *sgs_v;
+  int sgs_iters;
   Env *cfact;
   int has_coarse;
   double *w1, *w2, *w3, *w4;
@@ -1221,6 +1305,7 @@ static void state_free(or_state *s) {
   free(s->fact);
   env_free(s->cfact);
   free(s->sstarts); free(s->sinds);
+  free(s->sgs_start); free(s->sgs_i); free(s->sgs_j); free(s->sgs_v);
   free(s->w1); free(s->w2); free(s->w3); free(s->w4);
   free(s);
 }
@@ -1298,8 +1383,8 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
     int smoothers = p->smoothers;
     if (smoothers > p->radius1) smoothers = p->radius1;  /* CoarseMtx.F90:75-78 */
     s->smoothers = smoothers;
-    if (smoothers != 0 && smoothers != 1) {
-      seterr("smoothers=%d not supported on this path (0 or 1)", smoothers);
+    if (smoothers < 0) {
+      seterr("smoothers=%d not supported (the exact smoother -1 is commented out in the reference)", smoothers);
       state_free(s);
       return NULL;
     }
@@ -1406,6 +1491,62 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
     }
     free(floc);
   }
+  if (p->fine_method == 2) {
+    /* FinePreconditioner_sgs_Init (FinePreconditioner_sgs.F90:32-64):
+     * A + A_ghost (no ghost with one process) is consolidated
+     * (SpMtx_operation.F90:728-759: sorted by row and column, duplicates
+     * summed in stored order), and each subdomain keeps the entries with
+     * both ends in its index set (SpMtx_findElems, SpMtx_util.f90:105-135). */
+    OM *C = om_copy(A);  /* a new matrix: not yet arranged */
+    free(C->bound);
+    C->bound = NULL;
+    C->nbound = 0;
+    C->arr = ARR_NO;
+    om_arrange(C, ARR_ROWS, 1, 0, 0);
+    long k = 1;
+    for (long i = 2; i <= C->nnz; i++) {
+      if (C->indi[i] == C->indi[k] && C->indj[i] == C->indj[k]) {
+        C->val[k] = C->val[k] + C->val[i];
+      } else {
+        k++;
+        C->indi[k] = C->indi[i];
+        C->indj[k] = C->indj[i];
+        C->val[k] = C->val[i];
+      }
+    }
+    if (C->nnz == 0) k = 0;
+    long total = 0;
+    char *isin = (char *)xcalloc((size_t)n + 1, 1);
+    for (int pass = 0; pass < 2; pass++) {
+      if (pass == 1) {
+        s->sgs_start = (long *)xcalloc((size_t)np + 2, sizeof(long));
+        s->sgs_i = IALLOC(total);
+        s->sgs_j = IALLOC(total);
+        s->sgs_v = DALLOC(total);
+        s->sgs_start[1] = 1;
+      }
+      long t = 0;
+      for (int ip = 1; ip <= np; ip++) {
+        int st = s->sstarts[ip], nn = s->sstarts[ip + 1] - st;
+        for (int q = 0; q < nn; q++) isin[s->sinds[st + q]] = 1;
+        for (long e = 1; e <= k; e++)
+          if (isin[C->indi[e]] && isin[C->indj[e]]) {
+            t++;
+            if (pass == 1) {
+              s->sgs_i[t] = C->indi[e];
+              s->sgs_j[t] = C->indj[e];
+              s->sgs_v[t] = C->val[e];
+            }
+          }
+        for (int q = 0; q < nn; q++) isin[s->sinds[st + q]] = 0;
+        if (pass == 1) s->sgs_start[ip + 1] = t + 1;
+      }
+      total = t;
+    }
+    free(isin);
+    om_free(C);
+    s->sgs_iters = p->num_iters > 0 ? p->num_iters : 3;
+  }
   if (s->has_coarse) {
     OM *AC = s->AC;
     int nc = AC->nrows;
@@ -1503,6 +1644,38 @@ void or_prec_apply(or_state *s, const double *r, double *z) {
   /* fine level */
   if (p->fine_method == 0) {
     memcpy(z, r, sizeof(double) * n);
+  } else if (p->fine_method == 2) {
+    /* FinePreconditioner_sgs_Apply (FinePreconditioner_sgs.F90:67-84) and
+     * SymGaussSeidel (stationary.F90:33-88): per subdomain, x = 0, then
+     * `iters` forward and backward sweeps over rows 1..n (rows without a
+     * diagonal entry skipped), and sol = sol + x over all n. */
+    double *x = DALLOC(n), *diag = DALLOC(n);
+    long *rb = (long *)xcalloc((size_t)n + 2, sizeof(long));
+    for (int ip = 1; ip <= s->nsub; ip++) {
+      long e0 = s->sgs_start[ip], e1 = s->sgs_start[ip + 1];
+      for (int i = 1; i <= n; i++) { x[i] = 0.0; diag[i] = 0.0; rb[i] = 0; }
+      rb[n + 1] = 0;
+      /* arranged by rows (stable): bounds of each row's entries */
+      for (long e = e0; e < e1; e++) {
+        rb[s->sgs_i[e] + 1]++;
+        if (s->sgs_i[e] == s->sgs_j[e]) diag[s->sgs_i[e]] = s->sgs_v[e];
+      }
+      rb[1] = e0;
+      for (int i = 1; i <= n; i++) rb[i + 1] += rb[i];
+      for (int it = 0; it < s->sgs_iters; it++) {
+        for (int pass = 0; pass < 2; pass++)
+          for (int q = 1; q <= n; q++) {
+            int i = pass == 0 ? q : n + 1 - q;
+            if (diag[i] == 0.0) continue;
+            double xi = r[i - 1];
+            for (long e = rb[i]; e < rb[i + 1]; e++)
+              if (s->sgs_j[e] != i) xi = xi - s->sgs_v[e] * x[s->sgs_j[e]];
+            x[i] = xi / diag[i];
+          }
+      }
+      for (int i = 1; i <= n; i++) z[i - 1] = z[i - 1] + x[i];
+    }
+    free(x); free(diag); free(rb);
   } else {
     int maxs = 0;
     for (int ip = 1; ip <= s->nsub; ip++) {

@@ -29,6 +29,9 @@ typedef struct {
   /* prolongator damping; the reference hard-codes 0.33333333333333
    * (src/coarse/CoarseMtx.F90:64) */
   double omega;
+  /* symmetric Gauss-Seidel sweeps of fine_method 2 (ctl num_iters; <= 0
+   * means 3, src/main/aggr.F90:220-222) */
+  int num_iters;
 } or_params;
 
 typedef struct or_state or_state;

@@ -32,6 +32,7 @@ class Params(C.Structure):
         ("method", C.c_int), ("fine_method", C.c_int),
         ("symmstruct", C.c_int), ("symmnumeric", C.c_int),
         ("omega", C.c_double),
+        ("num_iters", C.c_int),
     ]
 
 

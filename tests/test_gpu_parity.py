@@ -410,6 +410,10 @@ def test_subdomain_beyond_8192_unknowns():
     dict(fine_method=0),           # coarse correction only
     dict(overlap=0),               # non-overlapping subdomains
     dict(smoothers=0),             # unsmoothed aggregates (host build of R and A_c)
+    dict(smoothers=2, radius1=4),  # R = T (S2 S)^T, host build
+    dict(smoothers=3, radius1=4),  # R = T (S2 S S2)
+    dict(smoothers=4, radius1=4),  # R = T (S2 S)(S2 S)
+    dict(method=5),                # one process: the same as method 1
 ])
 def test_option_variants_against_oracle(variant):
     n, i, j, v = P.diffusion3d(16, 1.0, 23)
@@ -430,3 +434,47 @@ def test_option_variants_against_oracle(variant):
         assert abs(it - ito) <= (max(1, ito // 100) if long_run else 0), (it, ito)
         assert np.linalg.norm(x - xo) <= (1e-7 if long_run else 1e-8) * np.linalg.norm(xo)
         assert abs(cond - condo) <= (1e-3 if long_run else 1e-6) * condo
+
+
+@pytest.mark.parametrize("levels,num_iters", [(1, 1), (2, -1), (2, 2)])
+def test_sgs_fine_level_bit_exact(levels, num_iters):
+    # fine_method 2: symmetric Gauss-Seidel on each subdomain's matrix
+    # (FinePreconditioner_sgs.F90, stationary.F90:33-88); the sweeps keep the
+    # reference's operation order, so the preconditioner is bit-exact
+    n, i, j, v = P.diffusion3d(16, 1.0, 31)
+    part = P.block_partition((16,) * 3, 8)
+    b = P.uniform_rhs(n, 32)
+    ctl = dict(CFG0_CTL, fine_method=2, num_iters=num_iters, levels=levels)
+    print(f"sgs levels={levels} num_iters={num_iters}", flush=True)
+    o, s = _pair(n, i, j, v, part, **ctl)
+    with s:
+        r = np.random.default_rng(33).uniform(-1, 1, n)
+        z, zo = s.prec(r), o.prec(r)
+        if levels == 1:
+            assert _bits_equal(z, zo)
+        else:  # the coarse solve is a different (explicit-inverse) factorisation
+            assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo)
+        x, it, cond, _ = s.pcg(b, 1e-8, 2000)
+        xo, ito, condo, _ = o.pcg(b, 1e-8, 2000)
+        assert abs(it - ito) <= max(1, ito // 100), (it, ito)
+        assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+        if levels == 2:
+            with pytest.raises(D.DougError, match="UNSUPPORTED"):
+                s.prepare_batch()
+
+
+def test_sgs_uneven_subdomains_and_fem_rows():
+    # unstructured rows (P1 triangles) and uneven, overlapping subdomains
+    n, i, j, v, _ = P.fem_p1_square(4000, 5)
+    ctl = dict(CFG0_CTL, fine_method=2, levels=1, overlap=2)
+    with D.Solver(-1) as h:
+        h.set_params(**dict(ctl, overlap=0))
+        h.set_matrix(n, i, j, v, 1)
+        h.set_num_subdomains(6)
+        h.setup_host()
+        cores = h.subdomains()
+    print("sgs fem", n, flush=True)
+    o, s = _pair(n, i, j, v, cores, **ctl)
+    with s:
+        r = np.random.default_rng(34).uniform(-1, 1, n)
+        assert _bits_equal(s.prec(r), o.prec(r))

@@ -98,3 +98,21 @@ def test_errors_are_reported():
             s.set_partition([1, 3], [1, 1], 1)
         with pytest.raises(D.DougError, match="STATE"):
             s.setup()  # host-only context does no GPU work
+
+
+@pytest.mark.parametrize("smoothers", [2, 3, 4, 5, 6, 7, 8, 9])
+def test_smoothed_restriction_powers(smoothers):
+    # smoothers >= 2 (CoarseMtx.F90:236-417); 9 takes the reference's
+    # fallback to the smoothers=2 product
+    n, i, j, v = P.diffusion2d(40, 2.0, seed=3)
+    info = _compare(n, i, j, v, part=P.block_partition((40, 40), 20), radius1=9, strong1=0.6,
+                    smoothers=smoothers, overlap=1, symmstruct=1, symmnumeric=1)
+    assert info["smoothers"] == smoothers
+
+
+def test_smoothers_clamped_to_radius():
+    # CoarseMtx.F90:75-78: smoothers > radius1 is reduced to radius1
+    n, i, j, v = P.diffusion2d(24, 1.0, seed=4)
+    info = _compare(n, i, j, v, part=P.block_partition((24, 24), 12), radius1=2, strong1=0.3,
+                    smoothers=5, overlap=1, symmstruct=1, symmnumeric=1)
+    assert info["smoothers"] == 2

@@ -184,3 +184,100 @@ def test_restriction_matches_formula():
     got_pat = np.zeros_like(pattern)
     got_pat[ri - 1, rj - 1] = True
     assert (got_pat == pattern).all()
+
+
+def _smoothers_reference(o, n, alpha, smoothers, omega=0.33333333333333):
+    """R for smoothers >= 2 from the formulas of CoarseMtx.F90:152-417 with
+    scipy products: S = I(1-w) - w D1^-1 A_strong (D1 = a_ii + weak row
+    sum), S2 the same with D2 = a_ii + the weak entries stored after the
+    diagonal, SS = S2 S, and the chain each value of smoothers selects."""
+    Ai, Aj, Av = o.A()  # storage order: rows arranged, entries as stored
+    r, c = Ai - 1, Aj - 1
+    d = np.zeros(n)
+    d[r[r == c]] = Av[r == c]
+    sc = Av / np.sqrt(np.abs(d[r])) / np.sqrt(np.abs(d[c]))
+    off = r != c
+    mx = np.zeros(n)
+    np.maximum.at(mx, r[off], np.abs(sc[off]))
+    strong = off & (np.abs(sc) > alpha * mx[r])
+    d1 = d + np.bincount(r[off & ~strong], Av[off & ~strong], minlength=n)
+    d2 = np.zeros(n)
+    weak_before_diag = 0
+    seen_diag = np.zeros(n, bool)
+    for k in range(len(Av)):
+        if r[k] == c[k]:
+            d2[r[k]] = Av[k]
+            seen_diag[r[k]] = True
+        elif not strong[k]:
+            d2[r[k]] += Av[k]
+            weak_before_diag += not seen_diag[r[k]]
+
+    def smoother(dd):
+        val = np.where(~off, 1 - omega, np.where(strong, -omega * Av / dd[r], 0.0))
+        return sp.csr_matrix((val, (r, c)), shape=(n, n))
+
+    S, S2 = smoother(d1), smoother(d2)
+    SS = S2 @ S
+    X = {3: lambda: SS @ S2, 4: lambda: SS @ SS, 5: lambda: SS @ SS @ S2,
+         6: lambda: SS @ SS @ SS, 7: lambda: SS @ SS @ SS @ S2,
+         8: lambda: (SS @ SS) @ (SS @ SS)}.get(smoothers, lambda: SS.T)()
+    num, st, _ = o.fine_aggregates()
+    T = sp.csr_matrix((np.ones(n), (num - 1, np.arange(n))), shape=(st.size - 1, n))
+    return (T @ X).toarray(), weak_before_diag
+
+
+@pytest.mark.parametrize("smoothers", [2, 3, 4, 5, 6, 7, 8, 9])
+def test_restriction_powers_match_formula(smoothers):
+    # above 8 the reference warns and builds the smoothers=2 product
+    n, i, j, v = P.diffusion2d(40, 2.0, seed=3)
+    alpha = 0.6
+    o = O.Oracle(n, i, j, v, O.params(radius1=9, strong1=alpha, smoothers=smoothers, overlap=1,
+                                      symmstruct=1, symmnumeric=1),
+                 partition=P.block_partition((40, 40), 20))
+    assert o.smoothers == smoothers
+    Rexp, weak_before = _smoothers_reference(o, n, alpha, smoothers)
+    assert weak_before > 0  # the second smoother's diagonal differs from the first's
+    ri, rj, rv = o.R()
+    Rgot = sp.coo_matrix((rv, (ri - 1, rj - 1)), shape=Rexp.shape).toarray()
+    np.testing.assert_allclose(Rgot, Rexp, rtol=1e-13, atol=1e-15)
+    # the coarse matrix is still the Galerkin product R A R^T
+    Ai, Aj, Av = o.A()
+    A = sp.csr_matrix((Av, (Ai - 1, Aj - 1)), shape=(n, n)).toarray()
+    ci, cj, cv = o.AC()
+    Ac = sp.coo_matrix((cv, (ci - 1, cj - 1)), shape=(Rexp.shape[0],) * 2).toarray()
+    np.testing.assert_allclose(Ac, Rexp @ A @ Rexp.T, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("iters", [1, 3])
+def test_sgs_fine_preconditioner_matches_formula(iters):
+    # fine_method 2 (FinePreconditioner_sgs.F90, stationary.F90:33-88): per
+    # subdomain I, `iters` symmetric Gauss-Seidel sweeps on A_II from x = 0,
+    # forward x <- (D+L)^-1 (b - U x), backward x <- (D+U)^-1 (b - L x),
+    # summed over the overlapping subdomains (one level: no coarse term)
+    import scipy.linalg as sla
+    n, i, j, v = P.diffusion3d(8, 1.0, seed=6)
+    o = O.Oracle(n, i, j, v, O.params(radius1=2, strong1=0.08, smoothers=1, overlap=1, levels=1,
+                                      fine_method=2, num_iters=iters, symmstruct=1,
+                                      symmnumeric=1),
+                 partition=P.block_partition((8, 8, 8), 4))
+    Ai, Aj, Av = o.A()
+    A = sp.csr_matrix((Av, (Ai - 1, Aj - 1)), shape=(n, n)).toarray()
+    r = np.random.default_rng(7).uniform(-1, 1, n)
+    starts, inds = o.subdomains()
+    z = np.zeros(n)
+    for a in range(len(starts) - 1):
+        I = np.sort(inds[starts[a] - 1:starts[a + 1] - 1] - 1)
+        M = A[np.ix_(I, I)]
+        Lo, Up, Dg = np.tril(M, -1), np.triu(M, 1), np.diag(np.diag(M))
+        x = np.zeros(len(I))
+        for _ in range(iters):
+            x = sla.solve_triangular(Dg + Lo, r[I] - Up @ x, lower=True)
+            x = sla.solve_triangular(Dg + Up, r[I] - Lo @ x, lower=False)
+        z[I] += x
+    np.testing.assert_allclose(o.prec(r), z, rtol=1e-12, atol=1e-14)
+    # num_iters unset means three sweeps (aggr.F90:220-222)
+    o3 = O.Oracle(n, i, j, v, O.params(radius1=2, strong1=0.08, smoothers=1, overlap=1, levels=1,
+                                       fine_method=2, symmstruct=1, symmnumeric=1),
+                  partition=P.block_partition((8, 8, 8), 4))
+    if iters == 3:
+        np.testing.assert_array_equal(o3.prec(r), o.prec(r))
