@@ -11,14 +11,14 @@ program doug_aggr
   use doug_io
   implicit none
 
-  character(len=300) :: ctlfile, arg, profprefix
+  character(len=300) :: ctlfile, arg, profprefix, msg
   type(ctl_values) :: c
   type(doug_params) :: p
   type(doug_info) :: info
   type(doug_tuning) :: tuning
   type(c_ptr) :: ctx
   integer(c_int32_t), allocatable :: indi(:), indj(:), pstarts(:), pnodes(:)
-  real(c_double), allocatable :: val(:), b(:), x(:), hist(:)
+  real(c_double), allocatable :: val(:), b(:), x(:), hist(:), xg(:)
   integer :: n, nnz, k, nargs, maxit, pu
   integer(c_int32_t) :: it
   real(c_double) :: cond, tol, t0, t1
@@ -64,23 +64,53 @@ program doug_aggr
   call ctl_read(ctlfile, c, ok)
   if (.not. ok) call fatal('Unable to open DOUG control file: '//trim(ctlfile))
   if (c%solver /= 2) call fatal('[DOUG main] : Wrong solution method specified')
-  if (c%input_type /= 2 .and. c%input_type /= -1) &
-    call fatal('only assembled input (input_type 2) is supported')
+  if (c%input_type /= 1 .and. c%input_type /= 2 .and. c%input_type /= -1) &
+    call fatal('[DOUG main] : Unrecognised input type.')
   if (c%coarse_method > 1) call fatal('Wrong coarse method')
 
-  ! matrix and right-hand side (Distribution_assm.F90:65-76)
-  write(*, '(a,a)') ' ##### Assembled input file: ##### ', trim(c%assembled_mtx_file)
-  call read_assembled(c%assembled_mtx_file, max(c%assembled_mtx_format, 0), n, nnz, &
-                      indi, indj, val, ok)
-  if (.not. ok) call fatal('Unable to read assembled file: '//trim(c%assembled_mtx_file))
-  write(*, *) 'n,nnz:', n, nnz
-  allocate(b(n), x(n))
-  if (len_trim(c%assembled_rhs_file) > 0) then
-    write(*, '(a,a)') ' ##### Assembled RHS file: ##### ', trim(c%assembled_rhs_file)
-    call read_vector(c%assembled_rhs_file, c%assembled_rhs_format, b, ok)
-    if (.not. ok) call fatal('Unable to read vector file: '//trim(c%assembled_rhs_file))
+  if (c%input_type == 1) then
+    ! elemental input (Distribution_elem.F90:27-163), one partition
+    if (c%number_of_blocks > 1) call fatal('multi-variable elemental input (number_of_blocks > 1) is not supported')
+    write(*, '(a,a)') ' ##### Elemental input: ##### ', trim(c%elemmat_rhs_file)
+    call read_elemental(c%info_file, c%freedom_lists_file, c%freemap_file, c%elemmat_rhs_file, &
+                        n, nnz, indi, indj, val, b, msg, ok)
+    if (.not. ok) call fatal(trim(msg))
+    write(*, *) 'n,nnz:', n, nnz
+    allocate(x(n))
+    ! a separate global RHS replaces the elements' ones when its file exists
+    ! (DOUG_utils.F90:685-691, Vect.F90:603-632)
+    inquire(file=trim(c%assembled_rhs_file), exist=ok)
+    if (len_trim(c%assembled_rhs_file) > 0 .and. ok) then
+      write(*, '(a,a)') ' ##### Assembled RHS file: ##### ', trim(c%assembled_rhs_file)
+      call info_ngf(c%info_file, k)
+      allocate(xg(k))
+      call read_vector(c%assembled_rhs_file, max(c%assembled_rhs_format, 0), xg, ok)
+      if (.not. ok) call fatal('Unable to read vector file: '//trim(c%assembled_rhs_file))
+      call remap_global_rhs(c%freedom_lists_file, c%info_file, xg, b, ok)
+      if (.not. ok) call fatal('RHS file does not match the freedoms')
+    end if
+    ! conversion of elemental input to an assembled matrix file
+    ! (main/geom.F90:105-110)
+    if (c%dump_matrix_only) then
+      call dump_matrix(c%dump_matrix_file, n, nnz, indi, indj, val, ok)
+      if (.not. ok) call fatal('[SpMtx_dumpMatrix] : Could not open file.')
+      stop
+    end if
   else
-    b = 1.0_c_double
+    ! matrix and right-hand side (Distribution_assm.F90:65-76)
+    write(*, '(a,a)') ' ##### Assembled input file: ##### ', trim(c%assembled_mtx_file)
+    call read_assembled(c%assembled_mtx_file, max(c%assembled_mtx_format, 0), n, nnz, &
+                        indi, indj, val, ok)
+    if (.not. ok) call fatal('Unable to read assembled file: '//trim(c%assembled_mtx_file))
+    write(*, *) 'n,nnz:', n, nnz
+    allocate(b(n), x(n))
+    if (len_trim(c%assembled_rhs_file) > 0) then
+      write(*, '(a,a)') ' ##### Assembled RHS file: ##### ', trim(c%assembled_rhs_file)
+      call read_vector(c%assembled_rhs_file, c%assembled_rhs_format, b, ok)
+      if (.not. ok) call fatal('Unable to read vector file: '//trim(c%assembled_rhs_file))
+    else
+      b = 1.0_c_double
+    end if
   end if
 
   rc = doug_ctx_create(0_c_int, ctx)

@@ -18,6 +18,11 @@ module doug_ctl
     real(rk) :: strong1 = -1.0_rk, strong2 = -1.0_rk, solve_tolerance = -1.0_rk
     logical :: symmstruct = .false., symmnumeric = .false.
     character(L) :: assembled_mtx_file = '', assembled_rhs_file = '', solution_file = ''
+    ! elemental input (input_type 1)
+    character(L) :: info_file = '', freedom_lists_file = '', elemmat_rhs_file = ''
+    character(L) :: freemap_file = '', dump_matrix_file = ''
+    integer :: number_of_blocks = -1
+    logical :: dump_matrix_only = .false.
     integer :: assembled_mtx_format = -1, assembled_rhs_format = -1, solution_format = -1
     ! extension of this build: a non-overlapping partition of the unknowns
     ! (text: nparts, then nparts+1 starts, then the node list, 1-based)
@@ -188,6 +193,13 @@ contains
       case ('autotune');             call set_logical(c%autotune, w1, w2)
       case ('assembled_mtx_format'); call set_int(c%assembled_mtx_format, w1, w2)
       case ('assembled_rhs_format'); call set_int(c%assembled_rhs_format, w1, w2)
+      case ('info_file');            call set_string(c%info_file, w1, w2)
+      case ('freedom_lists_file');   call set_string(c%freedom_lists_file, w1, w2)
+      case ('elemmat_rhs_file');     call set_string(c%elemmat_rhs_file, w1, w2)
+      case ('freemap_file');         call set_string(c%freemap_file, w1, w2)
+      case ('number_of_blocks');     call set_int(c%number_of_blocks, w1, w2)
+      case ('dump_matrix_only');     call set_logical(c%dump_matrix_only, w1, w2)
+      case ('dump_matrix_file');     call set_string(c%dump_matrix_file, w1, w2)
       case ('solution_format');      call set_int(c%solution_format, w1, w2)
       case default
         ! known to DOUG but not used on this path

@@ -207,4 +207,288 @@ contains
     ok = .true.
   end subroutine read_partition
 
+  !> Elemental input (input_type 1) for one process: the files
+  !> Mesh_initFromFile / Mesh_readFileFreelists / Mesh_readFileFreemap
+  !> (Mesh.F90:293-322,427-517) read, assembled as
+  !> ElemMtxs_readAndDistribute + ElemMtxsAssembleContext (ElemMtxs_distribute.F90:509-631,
+  !> ElemMtxs_assemble.F90:82-289) assemble them with one partition:
+  !>  - every freedom that some element lists is local, numbered in ascending
+  !>    global order (Mesh_MapsAndNghbrs, Mesh.F90:1098-1195);
+  !>  - element e's matrix arrives as nfrelt(e) records, record j holding
+  !>    column j; entry (i, j) with a nonzero value is added, in element
+  !>    order, to the slot (lf(i), lf(j)), and the element right-hand side to
+  !>    b(lf(i));
+  !>  - slots come out by ascending first index, then ascending second index,
+  !>    stored with row and column swapped (ElemMtxs_assemble.F90:256-262).
+  !> Returns the COO (1-based) in that order and b.
+  subroutine read_elemental(finfo, ffree, fmap, felem, n, nnz, indi, indj, val, b, msg, ok)
+    character(*), intent(in) :: finfo, ffree, fmap, felem
+    integer, intent(out) :: n, nnz
+    integer(c_int32_t), allocatable, intent(out) :: indi(:), indj(:)
+    real(c_double), allocatable, intent(out) :: val(:), b(:)
+    character(*), intent(out) :: msg
+    logical, intent(out) :: ok
+    integer(c_int32_t) :: tmp(5), nf4
+    integer :: nell, ngf, mfrelt, u, ios, e, i, j, k, nf, nt, t, lf
+    integer(c_int32_t), allocatable :: nfrelt(:), mhead(:, :), freemap(:)
+    integer, allocatable :: gl(:), ti(:), tj(:), ord(:), ord2(:), cnt(:)
+    real(c_double), allocatable :: elem(:, :), erhs(:), tv(:)
+    real(c_double) :: acc
+    ok = .false.
+    msg = ''
+    open(newunit=u, file=trim(finfo), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) then
+      msg = 'Unable to open info file: '//trim(finfo)
+      return
+    end if
+    read(u, iostat=ios) tmp
+    close(u)
+    if (ios /= 0) then
+      msg = 'Unable to read info file: '//trim(finfo)
+      return
+    end if
+    nell = tmp(1)
+    ngf = tmp(2)
+    mfrelt = tmp(4)
+    if (nell <= 0 .or. ngf <= 0 .or. mfrelt <= 0) then
+      msg = 'info file holds no elements or freedoms'
+      return
+    end if
+    ! element freedom lists
+    allocate(nfrelt(nell), mhead(mfrelt, nell))
+    mhead = 0
+    open(newunit=u, file=trim(ffree), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) then
+      msg = 'Unable to open freedom lists file: '//trim(ffree)
+      return
+    end if
+    do e = 1, nell
+      read(u, iostat=ios) nf4, (mhead(k, e), k = 1, min(int(nf4), mfrelt))
+      if (ios /= 0 .or. nf4 < 0 .or. nf4 > mfrelt) then
+        msg = 'Unable to read freedom lists file: '//trim(ffree)
+        close(u)
+        return
+      end if
+      nfrelt(e) = nf4
+      if (maxval(mhead(:, e)) > ngf .or. any(mhead(1:nf4, e) < 1)) then
+        msg = '[Mesh_readFileFreelists] - freedom index out of range'
+        close(u)
+        return
+      end if
+    end do
+    close(u)
+    ! freedom map (read as the reference does; not used by the solve)
+    if (len_trim(fmap) > 0) then
+      allocate(freemap(ngf))
+      open(newunit=u, file=trim(fmap), status='old', form='unformatted', iostat=ios)
+      if (ios == 0) read(u, iostat=ios) freemap
+      if (ios /= 0) then
+        msg = 'Unable to read freemap file: '//trim(fmap)
+        return
+      end if
+      close(u)
+    end if
+    ! local numbering: freedoms that appear, ascending
+    allocate(gl(ngf))
+    gl = 0
+    do e = 1, nell
+      do k = 1, nfrelt(e)
+        gl(mhead(k, e)) = 1
+      end do
+    end do
+    n = 0
+    do k = 1, ngf
+      if (gl(k) /= 0) then
+        n = n + 1
+        gl(k) = n
+      end if
+    end do
+    ! element matrices and right-hand sides
+    nt = 0
+    do e = 1, nell
+      nt = nt + nfrelt(e) * nfrelt(e)
+    end do
+    allocate(ti(nt), tj(nt), tv(nt), b(n), elem(mfrelt, mfrelt), erhs(mfrelt))
+    b = 0.0_c_double
+    open(newunit=u, file=trim(felem), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) then
+      msg = 'Unable to open element matrices file: '//trim(felem)
+      return
+    end if
+    t = 0
+    do e = 1, nell
+      nf = nfrelt(e)
+      do j = 1, nf
+        read(u, iostat=ios) (elem(k, j), k = 1, nf)
+        if (ios /= 0) exit
+      end do
+      if (ios == 0) read(u, iostat=ios) (erhs(k), k = 1, nf)
+      if (ios /= 0) then
+        msg = 'Unable to read element matrices file: '//trim(felem)
+        close(u)
+        return
+      end if
+      do i = 1, nf
+        lf = gl(mhead(i, e))
+        do j = 1, nf
+          if (elem(i, j) == 0.0_c_double) cycle
+          t = t + 1
+          ti(t) = lf
+          tj(t) = gl(mhead(j, e))
+          tv(t) = elem(i, j)
+        end do
+        b(lf) = b(lf) + erhs(i)
+      end do
+    end do
+    close(u)
+    ! stable sort of the contributions by (first, second) index: by the
+    ! second, then by the first; equal slots keep element order
+    allocate(ord(t), ord2(t), cnt(n + 1))
+    cnt = 0
+    do k = 1, t
+      cnt(tj(k) + 1) = cnt(tj(k) + 1) + 1
+    end do
+    cnt(1) = 1
+    do k = 2, n + 1
+      cnt(k) = cnt(k) + cnt(k - 1)
+    end do
+    do k = 1, t
+      ord2(cnt(tj(k))) = k
+      cnt(tj(k)) = cnt(tj(k)) + 1
+    end do
+    cnt = 0
+    do k = 1, t
+      cnt(ti(k) + 1) = cnt(ti(k) + 1) + 1
+    end do
+    cnt(1) = 1
+    do k = 2, n + 1
+      cnt(k) = cnt(k) + cnt(k - 1)
+    end do
+    do k = 1, t
+      i = ord2(k)
+      ord(cnt(ti(i))) = i
+      cnt(ti(i)) = cnt(ti(i)) + 1
+    end do
+    ! one slot per (first, second), summed from zero in element order
+    nnz = 0
+    do k = 1, t
+      if (k == 1) then
+        nnz = 1
+      else if (ti(ord(k)) /= ti(ord(k - 1)) .or. tj(ord(k)) /= tj(ord(k - 1))) then
+        nnz = nnz + 1
+      end if
+    end do
+    allocate(indi(nnz), indj(nnz), val(nnz))
+    nnz = 0
+    acc = 0.0_c_double
+    do k = 1, t
+      if (k > 1) then
+        if (ti(ord(k)) /= ti(ord(k - 1)) .or. tj(ord(k)) /= tj(ord(k - 1))) then
+          nnz = nnz + 1
+          indi(nnz) = tj(ord(k - 1))
+          indj(nnz) = ti(ord(k - 1))
+          val(nnz) = acc
+          acc = 0.0_c_double
+        end if
+      end if
+      acc = acc + tv(ord(k))
+    end do
+    if (t > 0) then
+      nnz = nnz + 1
+      indi(nnz) = tj(ord(t))
+      indj(nnz) = ti(ord(t))
+      val(nnz) = acc
+    end if
+    ok = .true.
+  end subroutine read_elemental
+
+  !> Global right-hand side of ngf values remapped to the local freedoms
+  !> (Vect_readAndBroadcastRHS, Vect.F90:603-632): b(lf) = x(lg(lf)).
+  subroutine remap_global_rhs(ffree, finfo, xg, b, ok)
+    character(*), intent(in) :: ffree, finfo
+    real(c_double), intent(in) :: xg(:)
+    real(c_double), intent(out) :: b(:)
+    logical, intent(out) :: ok
+    integer(c_int32_t) :: tmp(5), nf4
+    integer(c_int32_t), allocatable :: mh(:)
+    logical, allocatable :: seen(:)
+    integer :: u, ios, e, k, lf
+    ok = .false.
+    open(newunit=u, file=trim(finfo), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) return
+    read(u, iostat=ios) tmp
+    close(u)
+    if (ios /= 0 .or. size(xg) /= tmp(2)) return
+    allocate(seen(tmp(2)), mh(tmp(4)))
+    seen = .false.
+    open(newunit=u, file=trim(ffree), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) return
+    do e = 1, tmp(1)
+      read(u, iostat=ios) nf4, (mh(k), k = 1, min(int(nf4), int(tmp(4))))
+      if (ios /= 0) return
+      do k = 1, nf4
+        seen(mh(k)) = .true.
+      end do
+    end do
+    close(u)
+    lf = 0
+    do k = 1, tmp(2)
+      if (seen(k)) then
+        lf = lf + 1
+        if (lf > size(b)) return
+        b(lf) = xg(k)
+      end if
+    end do
+    ok = lf == size(b)
+  end subroutine remap_global_rhs
+
+  !> SpMtx_writeMatrix (SpMtx_util.f90:486-507): `ncols nnz`, then one
+  !> `(I10,I10,E24.16)` line per entry, rows ascending in stored order (the
+  !> matrix as arranged by rows when the reference dumps it).
+  subroutine dump_matrix(fname, n, nnz, indi, indj, val, ok)
+    character(*), intent(in) :: fname
+    integer, intent(in) :: n, nnz
+    integer(c_int32_t), intent(in) :: indi(:), indj(:)
+    real(c_double), intent(in) :: val(:)
+    logical, intent(out) :: ok
+    integer, allocatable :: cnt(:), ord(:)
+    integer :: u, ios, k
+    ok = .false.
+    allocate(cnt(n + 1), ord(nnz))
+    cnt = 0
+    do k = 1, nnz
+      cnt(indi(k) + 1) = cnt(indi(k) + 1) + 1
+    end do
+    cnt(1) = 1
+    do k = 2, n + 1
+      cnt(k) = cnt(k) + cnt(k - 1)
+    end do
+    do k = 1, nnz
+      ord(cnt(indi(k))) = k
+      cnt(indi(k)) = cnt(indi(k)) + 1
+    end do
+    open(newunit=u, file=trim(fname), status='replace', form='formatted', iostat=ios)
+    if (ios /= 0) return
+    write(u, *) n, nnz
+    do k = 1, nnz
+      write(u, '(I10,I10,E24.16)') indi(ord(k)), indj(ord(k)), val(ord(k))
+    end do
+    close(u)
+    ok = .true.
+  end subroutine dump_matrix
+
+  !> ngf, the number of global freedoms, from the info file
+  subroutine info_ngf(finfo, ngf)
+    character(*), intent(in) :: finfo
+    integer, intent(out) :: ngf
+    integer(c_int32_t) :: tmp(5)
+    integer :: u, ios
+    ngf = 0
+    open(newunit=u, file=trim(finfo), status='old', form='unformatted', iostat=ios)
+    if (ios /= 0) return
+    read(u, iostat=ios) tmp
+    close(u)
+    if (ios == 0) ngf = tmp(2)
+  end subroutine info_ngf
+
 end module doug_io
