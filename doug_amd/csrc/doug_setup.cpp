@@ -226,7 +226,7 @@ struct Grower {
   std::vector<int>& nodes;
   int& allmind;
   int neighood, maxasize;
-  std::vector<int> inlayer, newlayer, rounder;
+  std::vector<int> inlayer, newlayer, newdist, rounder;
   int nneigs = 0;
 
   Grower(const std::vector<int>& rs_, const std::vector<int>& cn_, std::vector<int>& st,
@@ -245,12 +245,15 @@ struct Grower {
     int ntoadd = 0, nrounders = 0;
     if (layer <= 2 * neighood) {
       // each node not yet placed is counted once per hit; a fresh one (stat
-      // 0) joins the new layer, with its distance, at its first hit.  The
-      // new layer holds distinct nodes here, so n + 1 slots always suffice;
-      // written without branches (the hit pattern is not predictable).
+      // 0) joins the new layer at its first hit, with the distance of its
+      // discoverer + 1.  The new layer holds distinct nodes here, so n + 1
+      // slots always suffice.  Written without branches (the hit pattern is
+      // not predictable); the distances go to a side list first and into
+      // `distance` after the scan (nothing in the scan reads them).
       int* nl = newlayer.data();
+      int* nd = newdist.data();
       int* st = stat.data();
-      int* di = distance.data();
+      const int* di = distance.data();
       const int* rsp = rs.data();
       const int* cnp = cn.data();
       for (int k = 0; k < ninnodes; k++) {
@@ -260,14 +263,13 @@ struct Grower {
         for (int i = rsp[ln]; i < e; i++) {
           const int c = cnp[i];
           const int s = st[c];
-          const int fresh = s == 0;
           nl[ntoadd] = c;
-          ntoadd += fresh;
-          const int dc = di[c];
-          di[c] = fresh ? dn : dc;
+          nd[ntoadd] = dn;
+          ntoadd += s == 0;
           st[c] = s - (s <= 0);
         }
       }
+      for (int k = 0; k < ntoadd; k++) distance[nl[k]] = nd[k];
     }
     if (layer <= neighood) {
       int mind = kMaxInt, maxd = 0;
@@ -352,6 +354,7 @@ struct Grower {
     int ninnodes = 1;
     if (inlayer.empty()) inlayer.resize(64);
     if (newlayer.size() < stat.size() + 1) newlayer.resize(stat.size() + 1);
+    if (newdist.size() < stat.size() + 1) newdist.resize(stat.size() + 1);
     inlayer[0] = seed;
     return round_layer(0, ninnodes);
   }
