@@ -164,6 +164,42 @@ def fem_batched_leg(npoints, nsub, nrhs, steps, warmup, seed):
                                               "trsm_tiles_per_step", "trsm_ms")}}
 
 
+def sgs_leg(n, i, j, v, part, b_h, num_iters=3):
+    """The symmetric Gauss-Seidel fine level (fine_method 2) on the same
+    workload: time per preconditioner apply (k_sgs and the overlap sum) and
+    an FP64 solve to 1e-8.  Not the benchmarked form: a row of subdomain
+    Gauss-Seidel is a chain of dependent rows, so each subdomain runs on one
+    wavefront and the apply is bound by that chain's latency."""
+    import torch
+    import doug_amd as D
+    dev = torch.device("cuda:0")
+    b = torch.tensor(b_h, dtype=torch.float64, device=dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    with D.Solver(0) as s:
+        s.set_params(**dict(CTL, fine_method=2, num_iters=num_iters))
+        s.set_matrix(n, i, j, v, 1)
+        s.set_partition(*part)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = s.setup(0)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        it, cond, hist = s.pcg_device(b.data_ptr(), x.data_ptr(), 1e-8, 100000)
+        torch.cuda.synchronize()
+        t_iter = time.perf_counter() - t0
+        apply_ms = s.time_operator(1, 10)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((v, (i - 1, j - 1)), shape=(n, n))
+    rel = float(np.linalg.norm(b_h - A @ x.cpu().numpy()) / np.linalg.norm(b_h))
+    out = {"num_iters": num_iters, "setup_s": round(t_setup, 4), "iterations": int(it),
+           "iterations_s": round(t_iter, 4), "time_to_solve_s": round(t_setup + t_iter, 4),
+           "sgs_apply_ms": round(apply_ms, 4), "rel_residual": rel,
+           "sub_total": int(info["sub_total"]), "nsub": int(info["nsub"])}
+    log(f"[bench] sgs fine level: {out}")
+    return out
+
+
 def direct_accuracy_leg(N, block, sigma, seed):
     """Relative error of both precision forms at rtol 1e-8 against an FP64
     sparse-direct solve (scipy's SuperLU) on an N^3 instance of the same
@@ -265,6 +301,8 @@ def main():
                     help="points of the FEM batching workload (0: skip it)")
     ap.add_argument("--diff-size", type=int, default=64,
                     help="grid edge of the differentiation workload (0: skip it)")
+    ap.add_argument("--sgs", type=int, default=1,
+                    help="1: also time the Gauss-Seidel fine level (fine_method 2) on the workload")
     ap.add_argument("--direct-size", type=int, default=40,
                     help="grid edge of the sparse-direct accuracy check (0: skip it)")
     args = ap.parse_args()
@@ -472,6 +510,8 @@ def main():
         "roofline": roof,
     }
     s.close()
+    if args.sgs:
+        out["sgs_fine_level"] = sgs_leg(n, i, j, v, part, b_h)
     if args.fem_points > 0:
         out["fem_batched_32rhs"] = fem_batched_leg(args.fem_points, 256, 32, min(args.steps, 2),
                                                    min(args.warmup, 1), args.seed + 7)
