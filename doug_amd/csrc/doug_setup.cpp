@@ -536,11 +536,35 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
     };
     for (int i = 0; i < n; i++) append(aggrnum[i], i);
     std::vector<int> colsaround, mem;
+    // position of each neighbouring aggregate in colsaround for the current
+    // aggregate (stamped with its number): the same first-seen order and the
+    // same sums as a search of colsaround
+    std::vector<int> cstamp((size_t)nagrs + 1, 0), cslot((size_t)nagrs + 1, 0);
     std::vector<double> connw;
     int nagrs_new = nagrs;
     const double beta = ps.beta;
     for (int a = 1; a <= nagrs; a++) {
       if (msize[a] >= minasize) continue;
+      // The small aggregates lie scattered over the graph and the loop is
+      // bound by the latency of their rows: prefetch, a few aggregates
+      // ahead, the row bounds (32 ahead), then the rows (16 ahead), then
+      // the aggregate numbers of their columns (6 ahead).  Lists of later
+      // aggregates can still grow; the prefetches are only hints.
+      for (int s = 0; s < 3; s++) {
+        const int pa = a + (s == 0 ? 32 : s == 1 ? 16 : 6);
+        if (pa > nagrs || msize[pa] >= minasize) continue;
+        for (int node = head[pa]; node >= 0; node = next[node]) {
+          if (s == 0) {
+            __builtin_prefetch(&A.ptr[node]);
+          } else if (s == 1) {
+            __builtin_prefetch(&A.col[A.ptr[node]]);
+            __builtin_prefetch(&scaled[A.ptr[node]]);
+          } else {
+            for (int p = A.ptr[node]; p < A.ptr[node + 1]; p++)
+              if (A.col[p] < n) __builtin_prefetch(&aggrnum[A.col[p]]);
+          }
+        }
+      }
       colsaround.clear();
       connw.clear();
       for (int node = head[a]; node >= 0; node = next[node]) {
@@ -549,13 +573,11 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
           if (c >= n) continue;
           int colr = aggrnum[c];
           if (colr > 0 && colr != a) {
-            size_t kk = 0;
-            for (; kk < colsaround.size(); kk++)
-              if (colsaround[kk] == colr) {
-                connw[kk] = connw[kk] + std::fabs(scaled[p]);
-                break;
-              }
-            if (kk == colsaround.size()) {
+            if (cstamp[colr] == a) {
+              connw[cslot[colr]] = connw[cslot[colr]] + std::fabs(scaled[p]);
+            } else {
+              cstamp[colr] = a;
+              cslot[colr] = (int)colsaround.size();
               colsaround.push_back(colr);
               connw.push_back(std::fabs(scaled[p]));
             }
