@@ -87,7 +87,8 @@ struct doug_ctx {
   double* d_subsol = nullptr;
   // fine_method 2: symmetric Gauss-Seidel on each subdomain's matrix
   SgsDesc* d_sgs = nullptr;
-  int32_t *d_sgs_blk = nullptr, *d_sgs_col = nullptr, *d_sgs_info = nullptr;
+  int32_t *d_sgs_levptr = nullptr, *d_sgs_lrows = nullptr, *d_sgs_rbeg = nullptr;
+  int32_t *d_sgs_rlen = nullptr, *d_sgs_col = nullptr;
   double *d_sgs_val = nullptr, *d_sgs_diag = nullptr;
   int sgs_mmax = 0, sgs_iters = 3;
   // coarse
@@ -910,16 +911,18 @@ static int upload_matrix(doug_ctx* ctx) {
 // subdomain keeps the entries of the consolidated matrix (SpMtx_addAll,
 // SpMtx_operation.F90:728-759: rows and columns sorted, duplicates summed in
 // stored order) with both ends in its index set; the sweeps visit its rows
-// ascending and each row's entries by ascending column.  Rows go into blocks
-// of at most kSgsBlock rows and kSgsBlock off-diagonal entries for k_sgs.
+// ascending and each row's entries by ascending column.  For k_sgs the rows
+// are levelled: in the forward sweep row i comes after every row j < i that
+// shares an entry with it (i reads the new x_j, or j reads the old x_i) and
+// gets level 1 + the largest such level; the backward sweep mirrors this.
 static int build_sgs(doug_ctx* ctx, const std::vector<int>& dinds) {
   const Csr& A = ctx->A;
   const int n = A.nrows;
   const int nb = (int)ctx->sstarts.size() - 1;
   struct Piece {
-    std::vector<int32_t> blk, col, info;
+    std::vector<int32_t> ptr{0}, col, levf, levb, rowsf, rowsb;
     std::vector<double> val, diag;
-    int err_rows = 0, err_row = -1;
+    int nlf = 0, nlb = 0;
   };
   std::vector<Piece> pieces((size_t)nb);
   int mmax = 0;
@@ -932,21 +935,14 @@ static int build_sgs(doug_ctx* ctx, const std::vector<int>& dinds) {
   std::vector<std::thread> th;
   for (int t = 0; t < nth; t++)
     th.emplace_back([&, t]() {
-      std::vector<int> floc((size_t)n, -1);
+      std::vector<int> floc((size_t)n, -1), lev, cnt;
       std::vector<std::pair<int, double>> ent;
+      std::vector<int32_t> sptr, sadj;
       for (int b = t; b < nb; b += nth) {
         Piece& pc = pieces[b];
         const int s0 = ctx->sstarts[b], m = ctx->sstarts[b + 1] - s0;
         for (int r = 0; r < m; r++) floc[dinds[s0 + r]] = r;
-        pc.info.resize((size_t)m);
         pc.diag.assign((size_t)m, 0.0);
-        int bent = 0, brows = 0;
-        auto open_block = [&](int r) {
-          pc.blk.push_back(r);
-          pc.col.resize(pc.col.size() + kSgsBlock, 0);
-          pc.val.resize(pc.val.size() + kSgsBlock, 0.0);
-          bent = brows = 0;
-        };
         for (int r = 0; r < m; r++) {
           const int g = dinds[s0 + r];
           ent.clear();
@@ -962,60 +958,94 @@ static int build_sgs(doug_ctx* ctx, const std::vector<int>& dinds) {
               ent[w++] = ent[q];
           }
           ent.resize(w);
-          int len = 0;
           for (auto& e : ent) {
-            if (e.first == g)
+            if (e.first == g) {
               pc.diag[r] = e.second;
-            else
-              len++;
-          }
-          if (len > kSgsBlock) {
-            pc.err_rows++;
-            pc.err_row = g;
-            len = 0;
-            ent.clear();
-          }
-          if (pc.blk.empty() || brows == kSgsBlock || bent + len > kSgsBlock) open_block(r);
-          const size_t base = (pc.blk.size() - 1) * kSgsBlock;
-          const int s = bent;
-          for (auto& e : ent)
-            if (e.first != g) {
-              pc.col[base + bent] = floc[e.first];
-              pc.val[base + bent] = e.second;
-              bent++;
+            } else {
+              pc.col.push_back(floc[e.first]);
+              pc.val.push_back(e.second);
             }
-          pc.info[r] = s | (bent << 8);
-          brows++;
-          pc.blk.back() += 1 << 16;
+          }
+          pc.ptr.push_back((int32_t)pc.col.size());
         }
         for (int r = 0; r < m; r++) floc[dinds[s0 + r]] = -1;
+        // symmetrised coupling graph
+        sptr.assign((size_t)m + 1, 0);
+        for (int r = 0; r < m; r++)
+          for (int q = pc.ptr[r]; q < pc.ptr[r + 1]; q++) {
+            sptr[r + 1]++;
+            sptr[pc.col[q] + 1]++;
+          }
+        for (int r = 0; r < m; r++) sptr[r + 1] += sptr[r];
+        sadj.resize((size_t)sptr[m]);
+        cnt.assign(sptr.begin(), sptr.end() - 1);
+        for (int r = 0; r < m; r++)
+          for (int q = pc.ptr[r]; q < pc.ptr[r + 1]; q++) {
+            sadj[cnt[r]++] = pc.col[q];
+            sadj[cnt[pc.col[q]]++] = r;
+          }
+        // levels, then rows grouped by level (ascending rows inside a level)
+        auto levels = [&](bool fwd, std::vector<int32_t>& rows, int& nl) {
+          lev.assign((size_t)m, 0);
+          nl = 0;
+          for (int k = 0; k < m; k++) {
+            const int r = fwd ? k : m - 1 - k;
+            int l = 0;
+            for (int q = sptr[r]; q < sptr[r + 1]; q++) {
+              const int j = sadj[q];
+              if (fwd ? j < r : j > r) l = std::max(l, lev[j] + 1);
+            }
+            lev[r] = l;
+            nl = std::max(nl, l + 1);
+          }
+          std::vector<int32_t> start((size_t)nl + 1, 0);
+          for (int r = 0; r < m; r++) start[lev[r] + 1]++;
+          for (int l = 0; l < nl; l++) start[l + 1] += start[l];
+          rows.resize((size_t)m);
+          std::vector<int32_t> fill(start.begin(), start.end() - 1);
+          for (int r = 0; r < m; r++) rows[fill[lev[r]]++] = r;
+          return start;
+        };
+        pc.levf = levels(true, pc.rowsf, pc.nlf);
+        pc.levb = levels(false, pc.rowsb, pc.nlb);
       }
     });
   for (auto& x : th) x.join();
-  for (int b = 0; b < nb; b++)
-    if (pieces[b].err_rows)
-      return fail(ctx, DOUG_E_UNSUPPORTED,
-                  "row " + std::to_string(pieces[b].err_row + 1) + " has more than " +
-                      std::to_string(kSgsBlock) + " off-diagonal entries inside a subdomain");
   std::vector<SgsDesc> desc((size_t)nb);
-  std::vector<int32_t> blk, col, info;
+  std::vector<int32_t> levptr, lrows, rbeg, rlen, col;
   std::vector<double> val, diag;
-  info.reserve(dinds.size());
+  rbeg.reserve(dinds.size());
+  rlen.reserve(dinds.size());
   diag.reserve(dinds.size());
+  lrows.reserve(2 * dinds.size());
   for (int b = 0; b < nb; b++) {
     const Piece& pc = pieces[b];
-    desc[b] = SgsDesc{(int64_t)ctx->sstarts[b], (int32_t)blk.size(), (int32_t)pc.blk.size(),
-                      ctx->sstarts[b + 1] - ctx->sstarts[b], 0};
-    blk.insert(blk.end(), pc.blk.begin(), pc.blk.end());
+    const int m = (int)pc.diag.size();
+    desc[b].off = ctx->sstarts[b];
+    desc[b].e0 = (int64_t)col.size();
+    desc[b].m = m;
+    desc[b].nlf = pc.nlf;
+    desc[b].pad = 0;
+    desc[b].lf0 = (int32_t)levptr.size();
+    levptr.insert(levptr.end(), pc.levf.begin(), pc.levf.end());
+    desc[b].nlb = pc.nlb;
+    desc[b].lb0 = (int32_t)levptr.size();
+    levptr.insert(levptr.end(), pc.levb.begin(), pc.levb.end());
+    lrows.insert(lrows.end(), pc.rowsf.begin(), pc.rowsf.end());
+    lrows.insert(lrows.end(), pc.rowsb.begin(), pc.rowsb.end());
+    for (int r = 0; r < m; r++) {
+      rbeg.push_back(pc.ptr[r]);
+      rlen.push_back(pc.ptr[r + 1] - pc.ptr[r]);
+    }
     col.insert(col.end(), pc.col.begin(), pc.col.end());
     val.insert(val.end(), pc.val.begin(), pc.val.end());
-    info.insert(info.end(), pc.info.begin(), pc.info.end());
     diag.insert(diag.end(), pc.diag.begin(), pc.diag.end());
   }
   int rc;
-  if ((rc = upload(ctx, &ctx->d_sgs, desc)) || (rc = upload(ctx, &ctx->d_sgs_blk, blk)) ||
-      (rc = upload(ctx, &ctx->d_sgs_col, col)) || (rc = upload(ctx, &ctx->d_sgs_val, val)) ||
-      (rc = upload(ctx, &ctx->d_sgs_info, info)) || (rc = upload(ctx, &ctx->d_sgs_diag, diag)))
+  if ((rc = upload(ctx, &ctx->d_sgs, desc)) || (rc = upload(ctx, &ctx->d_sgs_levptr, levptr)) ||
+      (rc = upload(ctx, &ctx->d_sgs_lrows, lrows)) || (rc = upload(ctx, &ctx->d_sgs_rbeg, rbeg)) ||
+      (rc = upload(ctx, &ctx->d_sgs_rlen, rlen)) || (rc = upload(ctx, &ctx->d_sgs_col, col)) ||
+      (rc = upload(ctx, &ctx->d_sgs_val, val)) || (rc = upload(ctx, &ctx->d_sgs_diag, diag)))
     return rc;
   ctx->sgs_mmax = mmax;
   ctx->sgs_iters = ctx->p.num_iters > 0 ? ctx->p.num_iters : 3;  // aggr.F90:220-222
@@ -1308,8 +1338,9 @@ static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
 static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
   if (ctx->p.fine_method == 2) {  // FinePreconditioner_sgs_Apply
     launch_gather_positions((int64_t)ctx->sinds.size(), ctx->d_inds, r, ctx->d_subsol, ctx->stream);
-    launch_sgs(ctx->d_sgs, ctx->nsub, ctx->sgs_mmax, ctx->d_sgs_blk, ctx->d_sgs_col, ctx->d_sgs_val,
-               ctx->d_sgs_info, ctx->d_sgs_diag, ctx->d_subsol, ctx->sgs_iters, ctx->stream);
+    launch_sgs(ctx->d_sgs, ctx->nsub, ctx->sgs_mmax, ctx->d_sgs_levptr, ctx->d_sgs_lrows,
+               ctx->d_sgs_rbeg, ctx->d_sgs_rlen, ctx->d_sgs_col, ctx->d_sgs_val, ctx->d_sgs_diag,
+               ctx->d_subsol, ctx->sgs_iters, ctx->stream);
     return;
   }
   if (ctx->precision == DOUG_PREC_FP32_FACTORS)

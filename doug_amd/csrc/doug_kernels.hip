@@ -1521,93 +1521,65 @@ DOUG_INSTANTIATE(float)
 // Symmetric Gauss-Seidel sweeps (SymGaussSeidel, stationary.F90:33-88):
 // x_i = (b_i - sum_{j != i} a_ij x_j) / a_ii in ascending column order, rows
 // ascending then descending, rows without a diagonal entry left at zero.
-// The rows are a sequential chain, so one wavefront owns a subdomain: the
-// lanes form a row's products a_ij x_j from the iterate in LDS and the
-// subtractions run in entry order on values read across lanes.  The next
-// block's entries and row data are loaded while the current block runs.
+// Level scheduling: row i of a forward sweep must follow every j < i it or
+// its neighbours couple to and precede every such j > i, so rows at the same
+// depth of that order are independent and one workgroup computes each
+// level's rows side by side, a barrier between levels.  Each row still sums
+// its own entries in column order, so the result is the sequential sweep's.
 // ------------------------------------------------------------------------
 namespace {
 
-__device__ __forceinline__ double lane_value(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-struct SgsRows {
-  int row0, nrows;
-  int col, inf;
-  double val, dg, rhs;
-};
-
-__device__ __forceinline__ void sgs_load(SgsRows& B, const SgsDesc& b, int k,
-                                         const int32_t* __restrict__ blk,
-                                         const int32_t* __restrict__ ecol,
-                                         const double* __restrict__ eval,
-                                         const int32_t* __restrict__ info,
-                                         const double* __restrict__ diag, const double* sub,
-                                         int lane) {
-  const int64_t id = (int64_t)b.blk0 + k;
-  const int pk = blk[id];
-  B.row0 = pk & 0xffff;
-  B.nrows = pk >> 16;
-  B.col = ecol[id * kSgsBlock + lane];
-  B.val = eval[id * kSgsBlock + lane];
-  const bool in = lane < B.nrows;
-  const int64_t pos = b.off + B.row0 + lane;
-  B.inf = in ? info[pos] : 0;
-  B.dg = in ? diag[pos] : 0.0;
-  B.rhs = in ? sub[pos] : 0.0;
-}
-
-__device__ __forceinline__ void sgs_row(const SgsRows& B, int rr, double* xs, int lane) {
-  const double d = lane_value(B.dg, rr);
-  if (d == 0.0) return;  // no diagonal entry: the row is skipped
-  const int se = __builtin_amdgcn_readlane(B.inf, rr);
-  const int s = se & 0xff, e = se >> 8;
-  double p = 0.0;
-  if (lane >= s && lane < e) p = B.val * xs[B.col];
-  double xi = lane_value(B.rhs, rr);
-  for (int k = s; k < e; k++) xi = xi - lane_value(p, k);
-  xi = xi / d;
-  if (lane == 0) xs[B.row0 + rr] = xi;
-  // one wavefront: its LDS accesses complete in order; keep the compiler
-  // from moving the next row's reads above this write
-  __builtin_amdgcn_wave_barrier();
-}
-
-__global__ __launch_bounds__(64) void k_sgs(const SgsDesc* __restrict__ d,
-                                            const int32_t* __restrict__ blk,
-                                            const int32_t* __restrict__ ecol,
-                                            const double* __restrict__ eval,
-                                            const int32_t* __restrict__ info,
-                                            const double* __restrict__ diag, double* sub,
-                                            int iters) {
+__global__ __launch_bounds__(256) void k_sgs(const SgsDesc* __restrict__ d,
+                                             const int32_t* __restrict__ levptr,
+                                             const int32_t* __restrict__ lrows,
+                                             const int32_t* __restrict__ rbeg,
+                                             const int32_t* __restrict__ rlen,
+                                             const int32_t* __restrict__ ecol,
+                                             const double* __restrict__ eval,
+                                             const double* __restrict__ diag, double* sub,
+                                             int iters) {
   extern __shared__ double xs[];
   const SgsDesc b = d[blockIdx.x];
-  const int lane = threadIdx.x;
-  for (int r = lane; r < b.m; r += 64) xs[r] = 0.0;
+  const int t = threadIdx.x;
+  for (int r = t; r < b.m; r += 256) xs[r] = 0.0;
   __syncthreads();
-  for (int it = 0; it < iters && b.nblk > 0; it++) {
-    SgsRows cur, nxt;
-    sgs_load(cur, b, 0, blk, ecol, eval, info, diag, sub, lane);
-    nxt = cur;
-    for (int k = 0; k < b.nblk; k++) {
-      if (k + 1 < b.nblk) sgs_load(nxt, b, k + 1, blk, ecol, eval, info, diag, sub, lane);
-      for (int rr = 0; rr < cur.nrows; rr++) sgs_row(cur, rr, xs, lane);
-      cur = nxt;
-    }
-    sgs_load(cur, b, b.nblk - 1, blk, ecol, eval, info, diag, sub, lane);
-    nxt = cur;
-    for (int k = b.nblk - 1; k >= 0; k--) {
-      if (k > 0) sgs_load(nxt, b, k - 1, blk, ecol, eval, info, diag, sub, lane);
-      for (int rr = cur.nrows - 1; rr >= 0; rr--) sgs_row(cur, rr, xs, lane);
-      cur = nxt;
+  for (int it = 0; it < iters; it++) {
+    for (int dir = 0; dir < 2; dir++) {
+      const int nl = dir ? b.nlb : b.nlf;
+      const int32_t* lp = levptr + (dir ? b.lb0 : b.lf0);
+      const int32_t* rows = lrows + 2 * b.off + (dir ? b.m : 0);
+      for (int lev = 0; lev < nl; lev++) {
+        const int e = lp[lev + 1];
+        for (int q = lp[lev] + t; q < e; q += 256) {
+          const int r = rows[q];
+          const int64_t pos = b.off + r;
+          const double dg = diag[pos];
+          if (dg != 0.0) {
+            double xi = sub[pos];
+            const int64_t e0 = b.e0 + rbeg[pos];
+            const int len = rlen[pos];
+            for (int k0 = 0; k0 < len; k0 += 8) {
+              int c[8];
+              double v[8];
+#pragma unroll
+              for (int k = 0; k < 8; k++) {
+                c[k] = k0 + k < len ? ecol[e0 + k0 + k] : 0;
+                v[k] = k0 + k < len ? eval[e0 + k0 + k] : 0.0;
+              }
+#pragma unroll
+              for (int k = 0; k < 8; k++) {
+                const double tk = xi - v[k] * xs[c[k]];
+                xi = k0 + k < len ? tk : xi;
+              }
+            }
+            xs[r] = xi / dg;
+          }
+        }
+        __syncthreads();
+      }
     }
   }
-  __syncthreads();
-  for (int r = lane; r < b.m; r += 64) sub[b.off + r] = xs[r];
+  for (int r = t; r < b.m; r += 256) sub[b.off + r] = xs[r];
 }
 
 __global__ void k_gather_positions(int64_t n, const int32_t* __restrict__ idx,
@@ -1632,13 +1604,13 @@ void launch_gather_positions(int64_t n, const int32_t* idx, const double* x, dou
                      idx, x, out);
 }
 
-void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* blk, const int32_t* ecol,
-                const double* eval, const int32_t* info, const double* diag, double* sub, int iters,
-                hipStream_t st) {
+void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* levptr, const int32_t* lrows,
+                const int32_t* rbeg, const int32_t* rlen, const int32_t* ecol, const double* eval,
+                const double* diag, double* sub, int iters, hipStream_t st) {
   if (nsub <= 0) return;
   const size_t bytes = (size_t)std::max(mmax, 1) * sizeof(double);
-  hipLaunchKernelGGL(k_sgs, dim3(nsub), dim3(64), bytes, st, d, blk, ecol, eval, info, diag, sub,
-                     iters);
+  hipLaunchKernelGGL(k_sgs, dim3(nsub), dim3(256), bytes, st, d, levptr, lrows, rbeg, rlen, ecol,
+                     eval, diag, sub, iters);
 }
 
 void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,

@@ -99,28 +99,30 @@ void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u
 // z[i] = sum over occurrences (ascending subdomain) of sol, then
 //        + sum_a Rt(i,a) csol[a]  (coarse prolongation), fine part first.
 // Symmetric Gauss-Seidel sweeps over each subdomain's own matrix (the
-// fine_method 2 preconditioner): one wavefront per subdomain, its iterate in
-// LDS, rows in blocks of at most kSgsBlock rows and kSgsBlock off-diagonal
-// entries (entries of block k at [k * kSgsBlock, (k + 1) * kSgsBlock)).
-constexpr int kSgsBlock = 64;
+// fine_method 2 preconditioner), one workgroup per subdomain with its
+// iterate in LDS.  Rows are level-scheduled: levptr[lf0 .. lf0 + nlf] (and
+// [lb0 .. lb0 + nlb] for the backward sweep) delimit each level's rows in
+// lrows[2 off ..] (forward, m rows) and lrows[2 off + m ..] (backward).  Row
+// r's off-diagonal entries are ecol/eval[e0 + rbeg[off + r] ..] (rlen[off + r]
+// of them, ascending column, local numbering).
 constexpr int kSgsMaxRows = 16384;  // LDS holds the iterate
 struct SgsDesc {
-  int64_t off;   // first position in the subdomain-vector layout
-  int32_t blk0;  // first block
-  int32_t nblk;
-  int32_t m;     // rows (the subdomain's unknowns, ascending)
+  int64_t off;  // first position in the subdomain-vector layout
+  int64_t e0;   // first entry
+  int32_t m;    // rows (the subdomain's unknowns, ascending)
+  int32_t nlf, nlb;
+  int32_t lf0, lb0;
   int32_t pad;
 };
 void prepare_sgs(int mmax);
 // out[p] = x[idx[p]] for p < n
 void launch_gather_positions(int64_t n, const int32_t* idx, const double* x, double* out,
                              hipStream_t st);
-// blk[k] = first local row | rows << 16; info[p] = first entry | end << 8
-// inside the row's block; sub holds the right-hand side on entry and the
-// iterate after `iters` forward and backward sweeps on return
-void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* blk, const int32_t* ecol,
-                const double* eval, const int32_t* info, const double* diag, double* sub, int iters,
-                hipStream_t st);
+// sub holds the right-hand side on entry and the iterate after `iters`
+// forward and backward sweeps on return
+void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* levptr, const int32_t* lrows,
+                const int32_t* rbeg, const int32_t* rlen, const int32_t* ecol, const double* eval,
+                const double* diag, double* sub, int iters, hipStream_t st);
 
 void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
                         const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,

@@ -240,6 +240,10 @@ struct Grower {
   }
 
   int round_layer(int layer, int& ninnodes) {
+    // An empty layer changes nothing (no hits, allmind untouched, the size
+    // check holds since every earlier inner layer passed it) and leaves the
+    // next layer empty: the recursion would end in 3.
+    if (ninnodes == 0) return 3;
     const int nxtlayer = layer + 1;
     bool layerfits = true;
     int ntoadd = 0, nrounders = 0;

@@ -478,3 +478,23 @@ def test_sgs_uneven_subdomains_and_fem_rows():
     with s:
         r = np.random.default_rng(34).uniform(-1, 1, n)
         assert _bits_equal(s.prec(r), o.prec(r))
+
+
+def test_sgs_long_rows():
+    # rows of more than 8 off-diagonal entries inside a subdomain (A = L^2 of
+    # a 3-D Laplacian: 24 neighbours) take several entry chunks per row
+    import scipy.sparse as sp
+    n, i, j, v = P.diffusion3d(12, 1.0, 35)
+    L = sp.csr_matrix((v, (i - 1, j - 1)), shape=(n, n))
+    A2 = (L @ L).tocoo()
+    order = np.lexsort((A2.col, A2.row))
+    i2 = (A2.row[order] + 1).astype(np.int32)
+    j2 = (A2.col[order] + 1).astype(np.int32)
+    v2 = A2.data[order]
+    part = P.block_partition((12,) * 3, 6)
+    ctl = dict(CFG0_CTL, fine_method=2, levels=1)
+    print("sgs long rows", n, flush=True)
+    o, s = _pair(n, i2, j2, v2, part, **ctl)
+    with s:
+        r = np.random.default_rng(36).uniform(-1, 1, n)
+        assert _bits_equal(s.prec(r), o.prec(r))
