@@ -54,6 +54,9 @@ struct doug_ctx {
   std::vector<int> sstarts, sinds;
   bool has_coarse = false;
   bool coarse_on_device = false;   // R, R^T built on the device (not uploaded)
+  int64_t nnz_R = 0;               // entries of R; with coarse_on_device R's columns and
+                                   // values stay on the device until a getter asks
+  uint8_t* d_strong = nullptr;     // strength mask, uploaded while the aggregation runs
   bool use_device_coarse = true;   // DOUG_OPT_DEVICE_COARSE
 
   // device: matrix
@@ -638,9 +641,13 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   };
   uint8_t* d_strong;
   int32_t* d_num;
-  HIPCHK(tmp.alloc(&d_strong, (size_t)nnz));
+  if (ctx->d_strong) {
+    d_strong = ctx->d_strong;
+  } else {
+    HIPCHK(tmp.alloc(&d_strong, (size_t)nnz));
+    HIPCHK(hipMemcpyAsync(d_strong, strong.data(), (size_t)nnz, hipMemcpyHostToDevice, st));
+  }
   HIPCHK(tmp.alloc(&d_num, (size_t)n));
-  HIPCHK(hipMemcpyAsync(d_strong, strong.data(), (size_t)nnz, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_num, ctx->fin.num.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   // R^T
   int32_t *s_col, *cnt;
@@ -685,7 +692,7 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   // X*Y with first-touch rows: bound, slots, products, compaction
   auto product = [&](int nrows, const int32_t* xp, const int32_t* xc, const double* xv,
                      const int32_t* yp, const int32_t* yc, const double* yv, int32_t** op,
-                     int32_t** oc, double** ov, int32_t* onnz, bool keep) -> int {
+                     int32_t** oc, double** ov, int32_t* onnz, bool keep, bool wave) -> int {
     int64_t *ub, *off;
     int32_t *pc, *ccount;
     double* pv;
@@ -700,7 +707,18 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
     HIPCHK(tmp.alloc(&pv, (size_t)total));
     HIPCHK(tmp.alloc(&ccount, (size_t)nrows + 1));
     HIPCHK(hipMemsetAsync(ccount, 0, ((size_t)nrows + 1) * sizeof(int32_t), st));
-    launch_first_touch(nrows, xp, xc, xv, yp, yc, yv, off, pc, pv, ccount, st);
+    bool done = false;
+    if (wave) {  // long rows: a wavefront per row, unless some row has > 128 columns
+      int* ovf;
+      HIPCHK(tmp.alloc(&ovf, 1));
+      HIPCHK(hipMemsetAsync(ovf, 0, sizeof(int), st));
+      launch_first_touch_wave(nrows, xp, xc, xv, yp, yc, yv, off, pc, pv, ccount, ovf, st);
+      int o = 0;
+      if (int r = read_back(ovf, o)) return r;
+      done = o == 0;
+      if (!done) HIPCHK(hipMemsetAsync(ccount, 0, ((size_t)nrows + 1) * sizeof(int32_t), st));
+    }
+    if (!done) launch_first_touch(nrows, xp, xc, xv, yp, yc, yv, off, pc, pv, ccount, st);
     if (keep) {
       if (int r = dalloc(ctx, op, (size_t)nrows + 1)) return r;
     } else {
@@ -719,26 +737,26 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   double *tp_val, *ac_val;
   int32_t nnz_tp = 0, nnz_ac = 0;
   if ((rc = product(n, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Rtptr, ctx->d_Rtcol, ctx->d_Rtval,
-                    &tp_ptr, &tp_col, &tp_val, &nnz_tp, false)))
+                    &tp_ptr, &tp_col, &tp_val, &nnz_tp, false, false)))
     return rc;
   if ((rc = product(nagr, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, tp_ptr, tp_col, tp_val, &ac_ptr,
-                    &ac_col, &ac_val, &nnz_ac, false)))
+                    &ac_col, &ac_val, &nnz_ac, false, true)))
     return rc;
-  // host copies: R (getters, row blocks) and A_c (coarse factor)
+  // host copies: R's row pointers (row blocks; its entries are fetched by
+  // doug_get_restriction) and A_c (coarse factor)
   Csr& R = ctx->R;
   R.nrows = nagr;
   R.ncols = n;
   R.ptr.resize((size_t)nagr + 1);
-  R.col.resize((size_t)nnz_rt);
-  R.val.resize((size_t)nnz_rt);
+  R.col.clear();
+  R.val.clear();
+  ctx->nnz_R = nnz_rt;
   Csr& AC = ctx->AC;
   AC.nrows = AC.ncols = nagr;
   AC.ptr.resize((size_t)nagr + 1);
   AC.col.resize((size_t)nnz_ac);
   AC.val.resize((size_t)nnz_ac);
   HIPCHK(hipMemcpyAsync(R.ptr.data(), ctx->d_Rptr, R.ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(R.col.data(), ctx->d_Rcol, R.col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(R.val.data(), ctx->d_Rval, R.val.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(AC.ptr.data(), ac_ptr, AC.ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(AC.col.data(), ac_col, AC.col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(AC.val.data(), ac_val, AC.val.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1053,6 +1071,14 @@ static int build_sgs(doug_ctx* ctx, const std::vector<int>& dinds) {
   return DOUG_OK;
 }
 
+// the strength mask for the device build of R (read by the aggregation too,
+// never written once set up)
+static int upload_strong(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
+  ctx->d_strong = nullptr;
+  if (!(ctx->smoothers == 1 && ctx->use_device_coarse && ctx->has_coarse)) return DOUG_OK;
+  return upload(ctx, &ctx->d_strong, strong);
+}
+
 static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const Params& p = ctx->p;
   doug_info& info = ctx->info;
@@ -1235,6 +1261,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
     set_setup_thread_cap(std::max(1, setup_threads() / 2 - 1));
     rc = upload_matrix(ctx);
+    if (!rc) rc = upload_strong(ctx, strong);
     if (!rc) {
       phase_subdomains(ctx);
       rc = device_fine(ctx, ev[0], ev[1]);
@@ -1246,6 +1273,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   } else {
     std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
     rc = upload_matrix(ctx);
+    if (!rc) rc = upload_strong(ctx, strong);
     agg.join();
     if (rc) return rc;
     if ((rc = coarse_space())) return rc;
@@ -1314,12 +1342,24 @@ static void put_csr(const Csr& M, int32_t* rowptr, int32_t* cols, double* vals) 
   }
 }
 
-int64_t doug_restriction_nnz(const doug_ctx* ctx) { return ctx ? ctx->R.nnz() : -1; }
+int64_t doug_restriction_nnz(const doug_ctx* ctx) {
+  if (!ctx) return -1;
+  return ctx->coarse_on_device ? ctx->nnz_R : ctx->R.nnz();
+}
 int64_t doug_coarse_matrix_nnz(const doug_ctx* ctx) { return ctx ? ctx->AC.nnz() : -1; }
 
 int doug_get_restriction(const doug_ctx* ctx, int32_t* rowptr, int32_t* cols, double* vals) {
   if (!ctx || !ctx->host_ready) return DOUG_E_STATE;
-  put_csr(ctx->R, rowptr, cols, vals);
+  if (!ctx->coarse_on_device) {
+    put_csr(ctx->R, rowptr, cols, vals);
+    return DOUG_OK;
+  }
+  for (size_t i = 0; i < ctx->R.ptr.size(); i++) rowptr[i] = ctx->R.ptr[i];
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess ||
+      hipMemcpy(cols, ctx->d_Rcol, (size_t)ctx->nnz_R * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(vals, ctx->d_Rval, (size_t)ctx->nnz_R * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return DOUG_E_HIP;
   return DOUG_OK;
 }
 

@@ -153,6 +153,86 @@ __global__ __launch_bounds__(64) void k_first_touch(
   cnt[i] = m;
 }
 
+// The same product with one wavefront per row, for rows that gather many
+// terms (A_c = R T'): the row's distinct columns sit one per lane (up to 128,
+// two per lane) and each product finds its column with one comparison
+// across the lanes.  Products are taken in Gustavson order, one at a time,
+// and added by the lane that holds their column, so every sum is the
+// sequential one and the columns come out in first-touch order.  A row with
+// more than 128 columns raises *overflow; the caller then uses
+// k_first_touch.
+__device__ __forceinline__ double lane_double(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__global__ __launch_bounds__(256) void k_first_touch_wave(
+    int nrows, const int32_t* __restrict__ xptr, const int32_t* __restrict__ xcol,
+    const double* __restrict__ xval, const int32_t* __restrict__ yptr,
+    const int32_t* __restrict__ ycol, const double* __restrict__ yval,
+    const int64_t* __restrict__ slot_off, int32_t* __restrict__ slot_col,
+    double* __restrict__ slot_val, int32_t* __restrict__ cnt, int* __restrict__ overflow) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= nrows) return;  // the whole wavefront
+  int c0 = -1, c1 = -1;
+  double a0 = 0.0, a1 = 0.0;
+  int m = 0;
+  for (int p = xptr[i]; p < xptr[i + 1]; p++) {
+    const int k = xcol[p];
+    const double xv = xval[p];
+    const int q1 = yptr[k + 1];
+    for (int qb = yptr[k]; qb < q1; qb += 64) {
+      const int q = qb + lane;
+      const int jl = q < q1 ? ycol[q] : -1;
+      const double pl = q < q1 ? xv * yval[q] : 0.0;
+      const int L = q1 - qb < 64 ? q1 - qb : 64;
+      for (int u = 0; u < L; u++) {
+        const int jj = __builtin_amdgcn_readlane(jl, u);
+        const double pr = lane_double(pl, u);
+        const unsigned long long h0 = __ballot(c0 == jj);
+        if (h0) {
+          if (lane == __ffsll((long long)h0) - 1) a0 = a0 + pr;
+          continue;
+        }
+        const unsigned long long h1 = __ballot(c1 == jj);
+        if (h1) {
+          if (lane == __ffsll((long long)h1) - 1) a1 = a1 + pr;
+          continue;
+        }
+        if (m < 64) {
+          if (lane == m) {
+            c0 = jj;
+            a0 = pr;
+          }
+        } else if (m < 128) {
+          if (lane == m - 64) {
+            c1 = jj;
+            a1 = pr;
+          }
+        }
+        m++;
+      }
+    }
+  }
+  if (m > 128) {
+    if (lane == 0) atomicExch(overflow, 1);
+    return;
+  }
+  const int64_t base = slot_off[i];
+  if (lane < m) {
+    slot_col[base + lane] = c0;
+    slot_val[base + lane] = a0;
+  }
+  if (lane + 64 < m) {
+    slot_col[base + 64 + lane] = c1;
+    slot_val[base + 64 + lane] = a1;
+  }
+  if (lane == 0) cnt[i] = m;
+}
+
 // Copy each row's cnt[i] entries from its slot to the compacted CSR.
 template <typename O>
 __global__ void k_compact(int nrows, const O* __restrict__ slot_off, const int32_t* __restrict__ out_ptr,
@@ -219,6 +299,16 @@ void launch_first_touch(int nrows, const int32_t* xptr, const int32_t* xcol, con
   if (nrows <= 0) return;
   hipLaunchKernelGGL(k_first_touch, dim3(cdiv64(nrows, 64)), dim3(64), 0, st, nrows, xptr, xcol,
                      xval, yptr, ycol, yval, slot_off, slot_col, slot_val, cnt);
+}
+
+void launch_first_touch_wave(int nrows, const int32_t* xptr, const int32_t* xcol,
+                             const double* xval, const int32_t* yptr, const int32_t* ycol,
+                             const double* yval, const int64_t* slot_off, int32_t* slot_col,
+                             double* slot_val, int32_t* cnt, int* overflow, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_first_touch_wave, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, st, nrows,
+                     xptr, xcol, xval, yptr, ycol, yval, slot_off, slot_col, slot_val, cnt,
+                     overflow);
 }
 
 void launch_compact(int nrows, const int32_t* slot_off, const int32_t* out_ptr,

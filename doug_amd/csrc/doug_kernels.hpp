@@ -194,6 +194,12 @@ void launch_rt_rows(int n, const int32_t* ptr, const int32_t* col, const double*
 void launch_product_bound(int nrows, const int32_t* xptr, const int32_t* xcol, const int32_t* yptr,
                           int64_t* ub, hipStream_t st);
 // rows of X*Y in first-touch column order into slots at slot_off[i]
+// one wavefront per row; *overflow set (and the rows incomplete) when a row
+// has more than 128 distinct columns
+void launch_first_touch_wave(int nrows, const int32_t* xptr, const int32_t* xcol,
+                             const double* xval, const int32_t* yptr, const int32_t* ycol,
+                             const double* yval, const int64_t* slot_off, int32_t* slot_col,
+                             double* slot_val, int32_t* cnt, int* overflow, hipStream_t st);
 void launch_first_touch(int nrows, const int32_t* xptr, const int32_t* xcol, const double* xval,
                         const int32_t* yptr, const int32_t* ycol, const double* yval,
                         const int64_t* slot_off, int32_t* slot_col, double* slot_val, int32_t* cnt,
