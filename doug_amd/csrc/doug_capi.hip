@@ -47,6 +47,12 @@ struct doug_ctx {
   std::vector<int> part_starts, part_nodes;
   bool have_partition = false;
   int num_subdomains = 0;  // > 1: parts of the fine-aggregate graph (doug_set_num_subdomains)
+  // virtual ranks (doug_set_ranks): rank of each unknown, and the per-rank
+  // decomposition of the last setup
+  int nranks = 0;
+  std::vector<int> eptn;
+  int ol_dist = 0;
+  RankSetup rsetup;
   ProcessState ps;
   Aggregates fin, ffull, cin, cfull;
   Csr R, Rt, AC;
@@ -333,6 +339,67 @@ int doug_set_num_subdomains(doug_ctx* ctx, int32_t nsub) {
   return DOUG_OK;
 }
 
+int doug_set_ranks(doug_ctx* ctx, int32_t nranks, const int32_t* rank_of_node, int32_t base) {
+  if (!ctx) return DOUG_E_ARG;
+  if (nranks <= 0) {
+    ctx->nranks = 0;
+    ctx->eptn.clear();
+    ctx->rsetup = RankSetup{};
+    ctx->ready = ctx->host_ready = false;
+    return DOUG_OK;
+  }
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "doug_set_ranks before doug_set_matrix_coo");
+  if (base != 0 && base != 1) return fail(ctx, DOUG_E_ARG, "base must be 0 or 1");
+  const int n = ctx->n;
+  std::vector<int> eptn((size_t)n);
+  if (rank_of_node) {
+    for (int i = 0; i < n; i++) {
+      eptn[i] = rank_of_node[i] - base;
+      if (eptn[i] < 0 || eptn[i] >= nranks) return fail(ctx, DOUG_E_ARG, "rank of unknown " + std::to_string(i + base) + " out of range");
+    }
+  } else {
+    // SpMtx_buildAdjncy + METIS on the matrix graph (Distribution_assm.F90:
+    // 191-197): here the project partitioner on the symmetrised pattern
+    const Csr& A = ctx->A;
+    std::vector<std::vector<int>> adj((size_t)n);
+    for (int i = 0; i < n; i++)
+      for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+        const int j = A.col[p];
+        if (j == i) continue;
+        adj[i].push_back(j);
+        adj[j].push_back(i);
+      }
+    std::vector<int> xadj((size_t)n + 1, 0), adjncy;
+    for (int i = 0; i < n; i++) {
+      std::sort(adj[i].begin(), adj[i].end());
+      adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+      adjncy.insert(adjncy.end(), adj[i].begin(), adj[i].end());
+      xadj[i + 1] = (int)adjncy.size();
+    }
+    partition_graph(n, xadj.data(), adjncy.data(), nullptr, nranks, eptn);
+  }
+  std::vector<int> cnt((size_t)nranks, 0);
+  for (int r : eptn) cnt[r]++;
+  for (int r = 0; r < nranks; r++)
+    if (cnt[r] == 0) return fail(ctx, DOUG_E_ARG, "rank " + std::to_string(r + base) + " owns no unknown");
+  ctx->eptn.swap(eptn);
+  ctx->nranks = nranks;
+  ctx->ready = ctx->host_ready = false;
+  return DOUG_OK;
+}
+
+int doug_get_rank_map(const doug_ctx* ctx, int32_t rank, int32_t* ninner, int32_t* nlocal, int32_t* lg) {
+  if (!ctx) return DOUG_E_ARG;
+  if (!ctx->host_ready || ctx->nranks <= 0) return DOUG_E_STATE;
+  if (rank < 0 || rank >= (int)ctx->rsetup.ranks.size()) return DOUG_E_ARG;
+  const RankLocal& R = ctx->rsetup.ranks[rank];
+  if (ninner) *ninner = R.ninner;
+  if (nlocal) *nlocal = (int32_t)R.lg.size();
+  if (lg)
+    for (size_t k = 0; k < R.lg.size(); k++) lg[k] = R.lg[k] + 1;
+  return DOUG_OK;
+}
+
 int doug_partition_graph(int32_t n, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
                          int32_t nparts, int32_t base, int32_t* part) {
   if (n < 0 || nparts < 1 || !xadj || !part || (base != 0 && base != 1)) return DOUG_E_ARG;
@@ -568,8 +635,17 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
   const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
   const int min_asize1 = p.minasize1 > 0 ? p.minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
-  aggregate(ctx->A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
-            ctx->fin, ctx->ffull);
+  if (ctx->nranks > 0) {
+    // each rank aggregates its own rows of the values as read
+    Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
+    virtual_ranks(As, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains, radius1,
+                  min_asize1, max_asize1, strong_conn1, ctx->rsetup);
+    ctx->fin = ctx->rsetup.inner;
+    ctx->ffull = ctx->rsetup.full;
+  } else {
+    aggregate(ctx->A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
+              ctx->fin, ctx->ffull);
+  }
   ctx->info.t_aggregate = now() - t0;
 }
 
@@ -782,6 +858,13 @@ static void phase_subdomains(doug_ctx* ctx) {
   std::vector<int> pstarts, pnodes;
   ctx->cin = Aggregates{};
   ctx->cfull = Aggregates{};
+  if (ctx->nranks > 0) {  // the ranks' subdomains, layers included
+    ctx->sstarts = ctx->rsetup.sstarts;
+    ctx->sinds = ctx->rsetup.sinds;
+    ctx->nsub = (int)ctx->sstarts.size() - 1;
+    ctx->info.t_subdomains = now() - t2;
+    return;
+  }
   if (ctx->have_partition) {
     pstarts = ctx->part_starts;
     pnodes = ctx->part_nodes;
@@ -844,6 +927,18 @@ static int check_params(doug_ctx* ctx) {
   // never computes (pcg.F90:146-147,253-255), so they have no defined result.
   if (p.method != 0 && p.method != 1 && p.method != 5)
     return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none), 1 or 5 (additive Schwarz)");
+  if (ctx->nranks > 0) {
+    // several processes: a negative radius1 selects the graph partition of
+    // the matrix and is negated (Distribution_assm.F90:127-134); the
+    // distribution ships max(overlap, smoothers) layers (:138), before the
+    // smoothers are reduced to radius1
+    if (ctx->have_partition)
+      return fail(ctx, DOUG_E_ARG, "virtual ranks choose their own subdomains: clear the partition");
+    if (ctx->p.radius1 < 0) ctx->p.radius1 = -ctx->p.radius1;
+    ctx->ol_dist = std::max(p.overlap, p.smoothers);
+    if (ctx->ol_dist < 1)
+      return fail(ctx, DOUG_E_UNSUPPORTED, "virtual ranks need max(overlap, smoothers) >= 1");
+  }
   int smoothers = p.smoothers;
   if (smoothers > p.radius1) smoothers = p.radius1;  // CoarseMtx.F90:75-78
   if (smoothers < 0)  // -1 selects an exact smoother the reference has commented out

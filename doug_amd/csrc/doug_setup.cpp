@@ -1281,4 +1281,234 @@ void aggregate_graph(const Csr& A, const Aggregates& inner, const Aggregates& fu
     }
 }
 
+// ---------------------------------------------------------------------------
+// Virtual ranks.  The reference's master sends each rank the rows within
+// max(1, 2 ol) graph layers of its nodes (SpMtx_distributeWithOverlap,
+// SpMtx_arrangement.F90:895-1021, rows arranged with sorted columns); the
+// rank then labels its nodes by graph distance (SpMtx_build_ghost,
+// :1028-1710) and numbers them by label (SpMtx_Build_lggl, :744-866).  Those
+// loops are restated here as distances: the layer of an outside node is its
+// distance from the rank's node set; an own node is in a neighbour's inner
+// overlap when it lies within `ol` steps of that neighbour's nodes and is sent
+// to it when exactly `ol` steps away.
+
+Csr sort_rows_by_column(const Csr& A, const std::vector<double>& val) {
+  Csr S;
+  S.nrows = A.nrows;
+  S.ncols = A.ncols;
+  S.ptr = A.ptr;
+  S.col.resize(A.col.size());
+  S.val.resize(A.col.size());
+  parallel_rows(A.nrows, [&](int i) {
+    const int b = A.ptr[i], e = A.ptr[i + 1];
+    std::vector<int> idx;
+    idx.reserve((size_t)(e - b));
+    for (int p = b; p < e; p++) idx.push_back(p);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A.col[x] < A.col[y]; });
+    for (int k = 0; k < e - b; k++) {
+      S.col[b + k] = A.col[idx[k]];
+      S.val[b + k] = val[idx[k]];
+    }
+  });
+  return S;
+}
+
+namespace {
+
+struct RankScratch {
+  std::vector<int> layer, stamp, label, gl;
+  int tick = 0;
+  explicit RankScratch(int n) : layer((size_t)n, -1), stamp((size_t)n, 0), label((size_t)n, 0), gl((size_t)n, -1) {}
+};
+
+// Labels: bit 0 inner overlap of some neighbour, bit 1 sent to some neighbour.
+void rank_local(const Csr& As, const std::vector<int>& eptn, int r, const std::vector<int>& own,
+                int ol, RankScratch& S, RankLocal& out) {
+  const int L = std::max(ol, 1), D = 2 * L;
+  std::vector<int> touched(own.begin(), own.end());
+  for (int v : own) S.layer[v] = 0;
+  // layers 1..D of outside nodes (build_ghost's first front)
+  std::vector<int> front, frontier(own.begin(), own.end()), next;
+  for (int l = 1; l <= D && !frontier.empty(); l++) {
+    next.clear();
+    for (int v : frontier)
+      for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
+        const int w = As.col[p];
+        if (S.layer[w] < 0) {
+          S.layer[w] = l;
+          next.push_back(w);
+          touched.push_back(w);
+        }
+      }
+    front.insert(front.end(), next.begin(), next.end());
+    frontier.swap(next);
+  }
+  // per neighbour rank: own nodes within L steps of its nodes on the front
+  std::vector<int> nbr;
+  for (int w : front) nbr.push_back(eptn[w]);
+  std::sort(nbr.begin(), nbr.end());
+  nbr.erase(std::unique(nbr.begin(), nbr.end()), nbr.end());
+  std::vector<std::vector<int>> by_rank(nbr.size());
+  for (int w : front) by_rank[std::lower_bound(nbr.begin(), nbr.end(), eptn[w]) - nbr.begin()].push_back(w);
+  for (size_t k = 0; k < nbr.size(); k++) {
+    const int t = ++S.tick;
+    frontier = by_rank[k];
+    for (int w : frontier) S.stamp[w] = t;
+    for (int d = 1; d <= L && !frontier.empty(); d++) {
+      next.clear();
+      for (int v : frontier)
+        for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
+          const int w = As.col[p];
+          if (S.layer[w] < 0 || S.stamp[w] == t) continue;  // off the rank's local matrix, or seen
+          S.stamp[w] = t;
+          next.push_back(w);
+          if (S.layer[w] == 0) S.label[w] |= (d == L) ? 3 : 1;
+        }
+      frontier.swap(next);
+    }
+  }
+  (void)r;
+  // numbering: [sent | inner overlap | inner | outer overlap | received]
+  std::vector<int> grp[5];
+  for (int v : own) grp[(S.label[v] & 2) ? 0 : (S.label[v] & 1) ? 1 : 2].push_back(v);
+  for (int w : front)
+    if (S.layer[w] <= L) grp[S.layer[w] == L ? 4 : 3].push_back(w);
+  std::sort(grp[3].begin(), grp[3].end());
+  std::sort(grp[4].begin(), grp[4].end());
+  out.lg.clear();
+  for (auto& g : grp) out.lg.insert(out.lg.end(), g.begin(), g.end());
+  out.ninner = (int)own.size();
+  const int nl = (int)out.lg.size();
+  for (int k = 0; k < nl; k++) S.gl[out.lg[k]] = k;
+  // local matrix: own rows whole; rows of outside nodes nearer than L keep
+  // the columns of own nodes and of outside nodes up to layer L; the outer
+  // layer's rows are the ghost matrix, not part of it
+  Csr& A = out.A;
+  A.nrows = A.ncols = nl;
+  A.ptr.assign((size_t)nl + 1, 0);
+  A.col.clear();
+  A.val.clear();
+  for (int k = 0; k < nl; k++) {
+    const int v = out.lg[k];
+    if (S.layer[v] < L) {
+      const bool ownrow = S.layer[v] == 0;
+      for (int pass = 0; pass < 2; pass++)
+        for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
+          const int w = As.col[p];
+          const int lw = S.layer[w];
+          if (lw < 0 || (!ownrow && lw > L)) continue;
+          const bool sent = lw == 0 && (S.label[w] & 2);
+          if (sent != (pass == 0)) continue;
+          A.col.push_back(S.gl[w]);
+          A.val.push_back(As.val[p]);
+        }
+    }
+    A.ptr[k + 1] = (int)A.col.size();
+  }
+  for (int v : touched) {
+    S.layer[v] = -1;
+    S.label[v] = 0;
+    S.gl[v] = -1;
+  }
+}
+
+// Aggregates listed per aggregate in ascending node order from `num`.
+void list_aggregates(int nagr, const std::vector<int>& num, Aggregates& a) {
+  a.nagr = nagr;
+  a.num = num;
+  a.starts.assign((size_t)nagr + 1, 0);
+  for (int x : num)
+    if (x > 0) a.starts[x]++;
+  for (int c = 0; c < nagr; c++) a.starts[c + 1] += a.starts[c];
+  a.nodes.assign((size_t)a.starts[nagr], 0);
+  std::vector<int> fill(a.starts.begin(), a.starts.end() - 1);
+  for (int i = 0; i < (int)num.size(); i++)
+    if (num[i] > 0) a.nodes[fill[num[i] - 1]++] = i;
+}
+
+}  // namespace
+
+void virtual_ranks(const Csr& As, const std::vector<int>& eptn, int nranks, int ol_dist, int ol_sub,
+                   int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
+                   RankSetup& out) {
+  const int n = As.nrows;
+  std::vector<std::vector<int>> own((size_t)nranks);
+  for (int i = 0; i < n; i++) own[eptn[i]].push_back(i);
+  out.ranks.assign((size_t)nranks, RankLocal{});
+  std::vector<Aggregates> inner((size_t)nranks), full((size_t)nranks);
+  std::vector<std::vector<int>> sst((size_t)nranks), sin((size_t)nranks);
+  const int nparts = std::max(nsub_per_rank, 1);
+  // the ranks are independent processes in the reference: one worker each
+  const int nw = std::max(1, std::min(nranks, setup_threads()));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nw; t++)
+    th.emplace_back([&, t]() {
+      set_setup_thread_cap(1);
+      RankScratch S(n);
+      for (int r = t; r < nranks; r += nw) {
+        RankLocal& R = out.ranks[r];
+        rank_local(As, eptn, r, own[r], ol_dist, S, R);
+        // getLocal (Aggregate_utils.f95:33-51): the own rows; a new matrix
+        // carries no symmetry flags, so the strength test scales by rows
+        Csr LA;
+        LA.nrows = R.ninner;
+        LA.ncols = R.A.ncols;
+        LA.ptr.assign(R.A.ptr.begin(), R.A.ptr.begin() + R.ninner + 1);
+        LA.col.assign(R.A.col.begin(), R.A.col.begin() + R.A.ptr[R.ninner]);
+        LA.val.assign(R.A.val.begin(), R.A.val.begin() + R.A.ptr[R.ninner]);
+        std::vector<double> scaled, diag;
+        scale_values(LA, false, scaled, diag);
+        std::vector<uint8_t> strong = find_strong(LA, scaled, diag, strong1);
+        ProcessState ps;
+        aggregate(LA, scaled, strong, radius1, minasize1, maxasize1, strong1, false, ps, inner[r], full[r]);
+        // Partitionings_metis_InitCoarse: inner nodes in local order, grouped
+        // by the part of their aggregate (one part: all of them)
+        std::vector<int> part;
+        if (nparts > 1) {
+          std::vector<int> xadj, adjncy;
+          aggregate_graph(LA, inner[r], inner[r], xadj, adjncy);
+          partition_graph(inner[r].nagr, xadj.data(), adjncy.data(), nullptr, nparts, part);
+        }
+        std::vector<int> pst((size_t)nparts + 1, 0), pnd;
+        for (int i = 0; i < R.ninner; i++) {
+          const int a = inner[r].num[i];
+          if (a > 0) pst[(nparts > 1 ? part[a - 1] : 0) + 1]++;
+        }
+        for (int q = 0; q < nparts; q++) pst[q + 1] += pst[q];
+        pnd.assign((size_t)pst[nparts], 0);
+        std::vector<int> fill(pst.begin(), pst.end() - 1);
+        for (int i = 0; i < R.ninner; i++) {
+          const int a = inner[r].num[i];
+          if (a > 0) pnd[fill[nparts > 1 ? part[a - 1] : 0]++] = i;
+        }
+        std::vector<int> ps2{0};
+        for (int q = 0; q < nparts; q++)
+          if (pst[q + 1] > pst[q]) ps2.push_back(pst[q + 1]);
+        add_layers_all(R.A, ps2, pnd, ol_sub, sst[r], sin[r]);
+        for (int& v : sin[r]) v = R.lg[v];
+      }
+      set_setup_thread_cap(0);
+    });
+  for (auto& x : th) x.join();
+  // rank-major global numbering of aggregates and subdomains
+  std::vector<int> num_in((size_t)n, 0), num_full((size_t)n, 0);
+  int off_in = 0, off_full = 0;
+  out.sstarts.assign(1, 0);
+  out.sinds.clear();
+  for (int r = 0; r < nranks; r++) {
+    const RankLocal& R = out.ranks[r];
+    for (int i = 0; i < R.ninner; i++) {
+      const int v = R.lg[i];
+      if (inner[r].num[i] > 0) num_in[v] = inner[r].num[i] + off_in;
+      num_full[v] = full[r].num[i] + off_full;
+    }
+    off_in += inner[r].nagr;
+    off_full += full[r].nagr;
+    for (size_t q = 1; q < sst[r].size(); q++) out.sstarts.push_back((int)out.sinds.size() + sst[r][q]);
+    out.sinds.insert(out.sinds.end(), sin[r].begin(), sin[r].end());
+  }
+  list_aggregates(off_in, num_in, out.inner);
+  list_aggregates(off_full, num_full, out.full);
+}
+
 }  // namespace doug

@@ -126,4 +126,43 @@ void partition_graph(int n, const int* xadj, const int* adjncy, const int* vwgt,
 void aggregate_graph(const Csr& A, const Aggregates& inner, const Aggregates& full,
                      std::vector<int>& xadj, std::vector<int>& adjncy);
 
+// ---- virtual ranks: DOUG's decomposition over several MPI processes ----
+//
+// One rank's view after SpMtx_distributeWithOverlap / SpMtx_build_ghost /
+// SpMtx_Build_lggl (SpMtx_arrangement.F90:744-1710, SpMtx_distribution.F90:
+// 139-290): local freedoms ordered [sent to a neighbour | rest of the inner
+// overlap | other inner | outer overlap | received], each ascending in global
+// number; `A` is the rank's local matrix (own rows and the rows of outside
+// nodes nearer than the outer layer) in local numbering, each row's entries
+// ordered as the reference's blocks leave them: columns of sent nodes first,
+// then the others, each group in ascending global column order.
+struct RankLocal {
+  int ninner = 0;
+  std::vector<int> lg;  // local -> global (0-based)
+  Csr A;                // nrows = ncols = lg.size(); rows of the outer layer are empty
+};
+
+// Per-rank results of the decomposition: each rank aggregates its own rows
+// (Partitioning_aggr.F90:81-92) with a fresh process state and splits its
+// inner nodes into `nsub_per_rank` parts of its aggregate graph
+// (Partitioning_metis.F90:34-86, project partitioner for more than one
+// part), then adds `ol_sub` layers over its local matrix
+// (Preconditioner_base.F90:96-123).  Aggregates are numbered rank by rank
+// (CoarseAllgathers.F90:531-540); subdomains are listed rank by rank.
+struct RankSetup {
+  std::vector<RankLocal> ranks;
+  Aggregates inner, full;             // global numbering
+  std::vector<int> sstarts, sinds;    // subdomains with their layers, global ids
+};
+
+// `As`: the assembled matrix with each row sorted by column (stable), values
+// as read.  `eptn[i]` in [0, nranks).  `ol_dist` = max(overlap, smoothers)
+// >= 1; `ol_sub` <= ol_dist.
+void virtual_ranks(const Csr& As, const std::vector<int>& eptn, int nranks, int ol_dist, int ol_sub,
+                   int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
+                   RankSetup& out);
+
+// Stable per-row sort by column (SpMtx_arrange with sort, SpMtx_arrangement.F90:159-227).
+Csr sort_rows_by_column(const Csr& A, const std::vector<double>& val);
+
 }  // namespace doug

@@ -18,6 +18,7 @@ program doug_aggr
   type(doug_tuning) :: tuning
   type(c_ptr) :: ctx
   integer(c_int32_t), allocatable :: indi(:), indj(:), pstarts(:), pnodes(:)
+  integer(c_int32_t), allocatable, target :: ranks(:)
   real(c_double), allocatable :: val(:), b(:), x(:), hist(:), xg(:)
   integer :: n, nnz, k, nargs, maxit, pu
   integer(c_int32_t) :: it
@@ -137,13 +138,27 @@ program doug_aggr
                                  1_c_int32_t))
   ! DOUG reads num_subdomains only with several processes; a value > 1 selects
   ! that path's subdomain source (parts of the fine-aggregate graph)
-  if (len_trim(c%partition_file) == 0 .and. c%num_subdomains > 1) &
+  if (len_trim(c%partition_file) == 0 .and. c%num_subdomains > 1 .and. c%virtual_ranks <= 0) &
     call check(doug_set_num_subdomains(ctx, int(c%num_subdomains, c_int32_t)))
   if (len_trim(c%partition_file) > 0) then
     call read_partition(c%partition_file, n, pstarts, pnodes, ok)
     if (.not. ok) call fatal('Unable to read partition file: '//trim(c%partition_file))
     call check(doug_set_partition(ctx, int(size(pstarts) - 1, c_int32_t), pstarts, pnodes, &
                                   1_c_int32_t))
+  end if
+
+  ! the decomposition of a run on several processes (virtual ranks); the
+  ! rank map file takes the place of METIS's partition
+  if (c%virtual_ranks > 0) then
+    if (len_trim(c%rank_map_file) > 0) then
+      call read_rank_map(c%rank_map_file, n, ranks, ok)
+      if (.not. ok) call fatal('Unable to read rank map file: '//trim(c%rank_map_file))
+      call check(doug_set_ranks(ctx, int(c%virtual_ranks, c_int32_t), c_loc(ranks), 1_c_int32_t))
+    else
+      call check(doug_set_ranks(ctx, int(c%virtual_ranks, c_int32_t), c_null_ptr, 1_c_int32_t))
+    end if
+    write(*, '(a,i0)') ' virtual ranks: ', c%virtual_ranks
+    if (c%num_subdomains > 1) call check(doug_set_num_subdomains(ctx, int(c%num_subdomains, c_int32_t)))
   end if
 
   call cpu_time(t0)

@@ -90,6 +90,14 @@ module doug_capi
       integer(c_int) :: rc
     end function
 
+    function doug_set_ranks(ctx, nranks, rank_of_node, base) bind(C, name="doug_set_ranks") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nranks, base
+      type(c_ptr), value :: rank_of_node
+      integer(c_int) :: rc
+    end function
+
     function doug_setup(ctx, precision) bind(C, name="doug_setup") result(rc)
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

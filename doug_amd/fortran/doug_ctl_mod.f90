@@ -31,10 +31,14 @@ module doug_ctl
     ! (0 FP64, 1 FP32 factors inside FP64 PCG) and the launch-shape autotuner
     integer :: factor_precision = 0
     logical :: autotune = .false.
+    ! extension of this build: the decomposition of a run on this many MPI
+    ! processes (virtual ranks), with an optional rank map file
+    integer :: virtual_ranks = 0
+    character(L) :: rank_map_file = ''
   end type ctl_values
 
   ! every word the reference's control file accepts (src/utils/controls.F90)
-  character(len=24), parameter :: known_words(55) = [character(len=24) :: &
+  character(len=24), parameter :: known_words(57) = [character(len=24) :: &
     'solver', 'method', 'levels', 'overlap', 'smoothers', 'input_type', 'matrix_type', &
     'info_file', 'freedom_lists_file', 'elemmat_rhs_file', 'coords_file', 'freemap_file', &
     'freedom_mask_file', 'number_of_blocks', 'strong1', 'strong2', 'solve_tolerance', &
@@ -45,7 +49,7 @@ module doug_ctl
     'assembled_mtx_file', 'assembled_mtx_format', 'assembled_rhs_file', &
     'assembled_rhs_format', 'dump_matrix_only', 'dump_matrix_file', 'num_subdomains', &
     'fine_method', 'num_iters', 'grid_size', 'coarse_method', 'partition_file', &
-    'factor_precision', 'autotune', 'dummy']
+    'factor_precision', 'autotune', 'virtual_ranks', 'rank_map_file', 'dummy']
 
 contains
 
@@ -101,6 +105,17 @@ contains
       read(w2, '(i10)') v
     end if
   end subroutine set_int
+
+  !> like set_int, for a key whose unset value is 0
+  subroutine set_ranks_key(v, w1, w2)
+    integer, intent(inout) :: v
+    character(*), intent(in) :: w1, w2
+    if (v /= 0) then
+      call repeated(w1, w2)
+    else
+      read(w2, '(i10)') v
+    end if
+  end subroutine set_ranks_key
 
   subroutine set_real(v, w1, w2)
     real(rk), intent(inout) :: v
@@ -191,6 +206,8 @@ contains
       case ('partition_file');       call set_string(c%partition_file, w1, w2)
       case ('factor_precision');     call set_int(c%factor_precision, w1, w2)
       case ('autotune');             call set_logical(c%autotune, w1, w2)
+      case ('virtual_ranks');        call set_ranks_key(c%virtual_ranks, w1, w2)
+      case ('rank_map_file');        call set_string(c%rank_map_file, w1, w2)
       case ('assembled_mtx_format'); call set_int(c%assembled_mtx_format, w1, w2)
       case ('assembled_rhs_format'); call set_int(c%assembled_rhs_format, w1, w2)
       case ('info_file');            call set_string(c%info_file, w1, w2)

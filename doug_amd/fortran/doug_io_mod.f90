@@ -207,6 +207,29 @@ contains
     ok = .true.
   end subroutine read_partition
 
+  !> Rank of each unknown, one integer per line (1-based), the layout of the
+  !> reference's domnums.txt (Distribution_assm.F90:202-216).
+  subroutine read_rank_map(fname, n, ranks, ok)
+    character(*), intent(in) :: fname
+    integer, intent(in) :: n
+    integer(c_int32_t), allocatable, intent(out) :: ranks(:)
+    logical, intent(out) :: ok
+    integer :: u, ios, i
+    ok = .false.
+    open(newunit=u, file=trim(fname), status='old', form='formatted', iostat=ios)
+    if (ios /= 0) return
+    allocate(ranks(n))
+    do i = 1, n
+      read(u, *, iostat=ios) ranks(i)
+      if (ios /= 0) then
+        close(u)
+        return
+      end if
+    end do
+    close(u)
+    ok = .true.
+  end subroutine read_rank_map
+
   !> Elemental input (input_type 1) for one process: the files
   !> Mesh_initFromFile / Mesh_readFileFreelists / Mesh_readFileFreemap
   !> (Mesh.F90:293-322,427-517) read, assembled as

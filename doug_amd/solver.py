@@ -109,6 +109,27 @@ class Solver:
         """doug_set_num_subdomains: subdomains from parts of the fine-aggregate graph."""
         self._check(self._L.doug_set_num_subdomains(self.h, int(nsub)))
 
+    def set_ranks(self, nranks: int, rank_of_node=None, base=1):
+        """doug_set_ranks: the decomposition of a DOUG run on `nranks`
+        processes (rank_of_node: owning rank of each unknown, or None for the
+        project partitioner on the matrix graph); 0 restores one process."""
+        if rank_of_node is None:
+            self._check(self._L.doug_set_ranks(self.h, int(nranks), None, int(base)))
+        else:
+            r = np.ascontiguousarray(rank_of_node, np.int32)
+            if len(r) != self.n:
+                raise ValueError("rank_of_node needs one rank per unknown")
+            self._check(self._L.doug_set_ranks(self.h, int(nranks), _lib.ptr(r), int(base)))
+
+    def rank_map(self, rank: int):
+        """(ninner, lg): a virtual rank's local numbering, 1-based global ids."""
+        ni, nl = C.c_int32(0), C.c_int32(0)
+        self._check(self._L.doug_get_rank_map(self.h, int(rank), C.byref(ni), C.byref(nl), None))
+        lg = np.zeros(max(nl.value, 1), np.int32)
+        self._check(self._L.doug_get_rank_map(self.h, int(rank), C.byref(ni), C.byref(nl),
+                                              _lib.ptr(lg)))
+        return ni.value, lg[:nl.value].copy()
+
     def setup(self, precision: int = 0) -> dict:
         self._check(self._L.doug_setup(self.h, int(precision)))
         return self.info()

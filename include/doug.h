@@ -113,6 +113,29 @@ int doug_set_partition(doug_ctx *ctx, int32_t nparts, const int32_t *starts,
  * 0 or 1 restores the one-process rule. */
 int doug_set_num_subdomains(doug_ctx *ctx, int32_t nsub);
 
+/* Virtual ranks: the decomposition of a DOUG run on `nranks` MPI processes,
+ * restated on one host and solved on one GPU.  rank_of_node[i] (index_base
+ * 0 or 1) is the rank that owns unknown i, M%eptnmap of
+ * SpMtx_DistributeAssembled (src/components/Distribution_assm.F90:105-270);
+ * NULL partitions the matrix graph with the project partitioner, in place of
+ * METIS on SpMtx_buildAdjncy (Distribution_assm.F90:191-197).  At setup each
+ * rank builds DOUG's local numbering (SpMtx_localize/SpMtx_build_ghost/
+ * SpMtx_Build_lggl, src/datatypes/SpMtx/SpMtx_distribution.F90:139-290,
+ * SpMtx_arrangement.F90:744-1710), aggregates its own rows
+ * (Partitioning_aggr.F90:81-92), splits them into doug_set_num_subdomains
+ * parts (one when unset) and adds the overlap layers over its local matrix;
+ * aggregates and subdomains are then numbered rank by rank.  A negative
+ * radius1 is negated as the reference does with several processes.
+ * Requires max(overlap, smoothers) >= 1 and no doug_set_partition.
+ * nranks <= 0 restores the one-process setup. */
+int doug_set_ranks(doug_ctx *ctx, int32_t nranks, const int32_t *rank_of_node,
+                   int32_t index_base);
+/* After setup with virtual ranks: rank `rank`'s (0-based) local numbering,
+ * M%lg_fmap (1-based global numbers, *nlocal of them, the first *ninner its
+ * own).  lg may be NULL to query the sizes. */
+int doug_get_rank_map(const doug_ctx *ctx, int32_t rank, int32_t *ninner, int32_t *nlocal,
+                      int32_t *lg);
+
 /* The project graph partitioner, standing in for METIS_PartGraphKway
  * (called from src/datatypes/Graph.f90:295-315): recursive bisection by
  * breadth-first growth from a pseudo-peripheral vertex plus boundary

@@ -1310,6 +1310,7 @@ static void state_free(or_state *s) {
   free(s);
 }
 void or_free(or_state *s) { state_free(s); }
+void or_free_ptr(void *q) { free(q); }
 
 /* Factorise_subdomain (subsolvers.F90:104-163): entries of A, in A's order,
  * with both ends inside the subdomain, renumbered through the local map. */
@@ -1340,6 +1341,8 @@ static Env *factor_subdomain(const OM *A, const int *inds, int nn, int *floc) {
   for (int i = 1; i <= nn; i++) floc[inds[i]] = 0;
   return e;
 }
+
+static or_state *setup_finish(or_state *s);
 
 or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
                    const double *val, const or_params *pp, int nparts,
@@ -1475,7 +1478,15 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
     memcpy(s->sinds + s->sstarts[ip], nodes + 1, sizeof(int) * on);
   }
   free(sizes); free(onfront); free(nodes); free(partstarts); free(partnodes);
+  return setup_finish(s);
+}
 
+/* Factorisations, the SGS matrices and the coarse factor, given the
+ * subdomains and the coarse space. */
+static or_state *setup_finish(or_state *s) {
+  or_params *p = &s->p;
+  OM *A = s->A;
+  int n = s->n, np = s->nsub;
   /* factorisations (Factorise_subdomains, subsolvers.F90:78-102) */
   if (p->fine_method == 1) {
     s->fact = (Env **)xcalloc((size_t)np, sizeof(Env *));
@@ -1559,6 +1570,414 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
   }
   s->w1 = DALLOC(n); s->w2 = DALLOC(n); s->w3 = DALLOC(n); s->w4 = DALLOC(n);
   return s;
+}
+
+/* ------------------------------------------------------------------ */
+/* Several processes ("virtual ranks"): the decomposition of a DOUG run   */
+/* on nranks MPI processes, each rank restated in turn on one host.       */
+/* ------------------------------------------------------------------ */
+
+/* SpMtx_addFront (SpMtx_arrangement.F90:870-890) for one processor p: marks
+ * the nodes within `ol` steps of i, recursing through nodes of other ranks. */
+static void add_front(unsigned char *mask, const OM *A, const int *eptn, int i, int p, int ol) {
+  if (mask[i] < ol + 1) mask[i] = (unsigned char)(ol + 1);
+  if (ol > 0)
+    for (int k = A->bound[i]; k <= A->bound[i + 1] - 1; k++) {
+      int j = A->indj[k];
+      if (p != eptn[j] && mask[j] < ol) add_front(mask, A, eptn, j, p, ol - 1);
+    }
+}
+
+typedef struct {
+  int ninner, nlf;
+  int *lg;       /* 1..nlf, global numbers */
+  OM *A;         /* local matrix in local numbering, entries in block order */
+} RankView;
+
+/* One rank `clr` (1-based): SpMtx_distributeWithOverlap (SpMtx_arrangement.
+ * F90:895-1021), SpMtx_build_ghost with ol > 0 (:1028-1710), SpMtx_Build_lggl
+ * (:744-866) and the localisation of SpMtx_localize (SpMtx_distribution.F90:
+ * 139-290).  Ag: the assembled matrix arranged by rows with sorted columns;
+ * eptn[1..n] the rank of each node (1-based). */
+static void rank_view(const OM *Ag, const int *eptn, int nprocs, int clr, int ol, RankView *rv) {
+  int n = Ag->nrows;
+  /* distribution mask and the rank's share of the matrix */
+  unsigned char *mask = (unsigned char *)xcalloc((size_t)n + 1, 1);
+  int depth = ol * 2 > 1 ? ol * 2 : 1;
+  for (long k = 1; k <= Ag->nnz; k++) {
+    int i = Ag->indi[k];
+    if (eptn[i] == clr) add_front(mask, Ag, eptn, i, clr, depth);
+  }
+  long nnz = 0;
+  for (int i = 1; i <= n; i++)
+    if (mask[i])
+      for (int k = Ag->bound[i]; k <= Ag->bound[i + 1] - 1; k++)
+        if (mask[Ag->indj[k]]) nnz++;
+  OM *A = om_new(nnz, n, n);
+  long t = 0;
+  for (int i = 1; i <= n; i++)
+    if (mask[i])
+      for (int k = Ag->bound[i]; k <= Ag->bound[i + 1] - 1; k++)
+        if (mask[Ag->indj[k]]) {
+          t++;
+          A->indi[t] = i;
+          A->indj[t] = Ag->indj[k];
+          A->val[t] = Ag->val[k];
+        }
+  free(mask);
+  om_arrange(A, ARR_ROWS, 0, 0, 0);
+
+  /* build_ghost: the front of 2*lastlayer layers around the rank */
+  int lastlayer = ol > 1 ? ol : 1, hl = 2 * lastlayer + 1;
+  int *front = IALLOC(n), *onfront = IALLOC(n);
+  int *fs = IALLOC(2 * lastlayer + 1), *fe = IALLOC(2 * lastlayer + 1); /* [layer+1] */
+  int *nnodesonclrol = IALLOC(nprocs), *neighmap = IALLOC(nprocs);
+  int nfront = 0;
+  fs[1] = 1;
+  for (int i = 1; i <= n; i++)
+    if (eptn[i] == clr && onfront[i] == 0) {
+      front[++nfront] = i;
+      onfront[i] = -1;
+    }
+  int nown = nfront;
+  fe[1] = nfront;
+  fs[2] = nfront + 1;
+  for (int q = 1; q <= nown; q++) {
+    int node = front[q];
+    for (int j = A->bound[node]; j <= A->bound[node + 1] - 1; j++) {
+      int neigh = A->indj[j], cn = eptn[neigh];
+      if (cn != clr && onfront[neigh] == 0) {
+        onfront[neigh] = 1;
+        nnodesonclrol[cn]++;
+        front[++nfront] = neigh;
+      }
+    }
+  }
+  fe[2] = nfront;
+  for (int layer = 2; layer <= 2 * lastlayer; layer++) {
+    fs[layer + 1] = nfront + 1;
+    for (int q = fs[layer]; q <= fe[layer]; q++) {
+      int node = front[q];
+      for (int j = A->bound[node]; j <= A->bound[node + 1] - 1; j++) {
+        int neigh = A->indj[j], cn = eptn[neigh];
+        if (cn != clr && onfront[neigh] == 0) {
+          onfront[neigh] = layer;
+          nnodesonclrol[cn]++;
+          front[++nfront] = neigh;
+        }
+      }
+    }
+    fe[layer + 1] = nfront;
+  }
+  /* the matrix reduced to entries between front nodes */
+  {
+    long m = 0;
+    for (long k = 1; k <= A->nnz; k++)
+      if (onfront[A->indi[k]] != 0 && onfront[A->indj[k]] != 0) {
+        m++;
+        A->indi[m] = A->indi[k];
+        A->indj[m] = A->indj[k];
+        A->val[m] = A->val[k];
+      }
+    A->nnz = m;
+    A->arr = ARR_NO;
+    om_arrange(A, ARR_ROWS, 0, 0, 0);
+  }
+  int nnghbrs = 0;
+  int *nghbrs = IALLOC(nprocs + 1);
+  for (int q = 1; q <= nprocs; q++)
+    if (nnodesonclrol[q] > 0) {
+      nghbrs[++nnghbrs] = q;
+      neighmap[q] = nnghbrs;
+    }
+  neighmap[clr] = nnghbrs + 1;
+  /* front regrouped by neighbour, own nodes last, each in front order */
+  int *ccount = IALLOC(nnghbrs + 1), *cst = IALLOC(nnghbrs + 2), *corder = IALLOC(nfront);
+  for (int q = 1; q <= nfront; q++) ccount[neighmap[eptn[front[q]]]]++;
+  cst[1] = 1;
+  for (int q = 1; q <= nnghbrs + 1; q++) cst[q + 1] = cst[q] + ccount[q];
+  for (int q = 1; q <= nnghbrs + 1; q++) ccount[q] = cst[q] - 1;
+  for (int q = 1; q <= nfront; q++) {
+    int g = neighmap[eptn[front[q]]];
+    corder[++ccount[g]] = front[q];
+  }
+  for (int q = 1; q <= nown; q++) onfront[front[q]] = 0;
+  /* per neighbour: inner overlap, sent and outer overlap nodes */
+  unsigned char *sendnodes = (unsigned char *)xcalloc((size_t)n + 1, 1);
+  int *mark = IALLOC(n); /* Build_lggl's organisation marks */
+  int *olin_flag = IALLOC(n), *olout_flag = IALLOC(n), *recv_flag = IALLOC(n);
+  int *bfront = IALLOC(n), *bfs = IALLOC(lastlayer + 1), *bfe = IALLOC(lastlayer + 1);
+  for (int q = fs[lastlayer + 1]; q <= fe[lastlayer + 1]; q++) recv_flag[front[q]] = 1;
+  for (int k = 1; k <= nnghbrs; k++) {
+    int offset = k * hl, nb = 0;
+    bfs[1] = 1;
+    for (int q = cst[k]; q <= cst[k + 1] - 1; q++) {
+      int node = corder[q];
+      int layer = onfront[node] % hl;
+      onfront[node] = offset + layer;
+      bfront[++nb] = node;
+    }
+    bfe[1] = nb;
+    for (int kl = 1; kl <= lastlayer; kl++) {
+      bfs[kl + 1] = nb + 1;
+      for (int q = bfs[kl]; q <= bfe[kl]; q++) {
+        int node = bfront[q];
+        for (int j = A->bound[node]; j <= A->bound[node + 1] - 1; j++) {
+          int neigh = A->indj[j];
+          if (onfront[neigh] < offset) {
+            int layer = onfront[neigh] % hl;
+            onfront[neigh] = offset + layer;
+            bfront[++nb] = neigh;
+          }
+        }
+      }
+      bfe[kl + 1] = nb;
+    }
+    for (int q = 1; q <= bfe[lastlayer]; q++) {
+      int node = bfront[q], layer = onfront[node] % hl;
+      if (layer == 0)
+        olin_flag[node] = 1;
+      else if (layer <= lastlayer && eptn[node] == nghbrs[k])
+        olout_flag[node] = 1;
+    }
+    for (int q = bfs[lastlayer + 1]; q <= bfe[lastlayer + 1]; q++) {
+      int node = bfront[q], layer = onfront[node] % hl;
+      if (layer == 0) {
+        sendnodes[node] = 1;
+        olin_flag[node] = 1;
+      } else if (layer <= lastlayer && eptn[node] == nghbrs[k]) {
+        olout_flag[node] = 1;
+      }
+    }
+  }
+  /* the matrix kept by the rank: blocks 11, 12, 21, 22 (ol > 0) */
+  long m = 0;
+  int *ti = IALLOC(A->nnz), *tj = IALLOC(A->nnz);
+  double *tv = DALLOC(A->nnz);
+  for (int blk = 0; blk < 4; blk++) {
+    for (int q = cst[nnghbrs + 1]; q <= cst[nnghbrs + 2] - 1; q++) {
+      int node = corder[q];
+      int is_send = sendnodes[node];
+      if ((blk < 2) != (is_send == 1)) continue;
+      for (int j = A->bound[node]; j <= A->bound[node + 1] - 1; j++) {
+        int neigh = A->indj[j];
+        int to_send = sendnodes[neigh] == 1;
+        if ((blk % 2 == 0) != to_send) continue;
+        m++;
+        ti[m] = node; tj[m] = neigh; tv[m] = A->val[j];
+      }
+    }
+    if (blk >= 2)
+      for (int k = 1; k <= nnghbrs; k++)
+        for (int q = cst[k]; q <= cst[k + 1] - 1; q++) {
+          int node = corder[q], layer = onfront[node] % hl;
+          if (layer >= lastlayer) continue;
+          for (int j = A->bound[node]; j <= A->bound[node + 1] - 1; j++) {
+            int neigh = A->indj[j];
+            if (sendnodes[neigh] == 1) {
+              if (blk != 2) continue;
+            } else if (onfront[neigh] % hl <= lastlayer) {
+              if (blk != 3) continue;
+            } else {
+              continue;
+            }
+            m++;
+            ti[m] = node; tj[m] = neigh; tv[m] = A->val[j];
+          }
+        }
+  }
+  /* Build_lggl: organisation marks, then numbering by mark */
+  for (long k = 1; k <= m; k++) {
+    mark[ti[k]] = -3;
+    mark[tj[k]] = -3;
+  }
+  for (int i = 1; i <= n; i++) {
+    if (olin_flag[i]) mark[i] = -2;
+    if (olout_flag[i]) mark[i] = 2;
+  }
+  for (int i = 1; i <= n; i++) {
+    if (sendnodes[i]) mark[i] = -1;
+    if (recv_flag[i]) mark[i] = 1;
+  }
+  static const int order[5] = {-1, -2, -3, 2, 1};
+  int nlf = 0, ninner = 0;
+  int *gl = IALLOC(n);
+  rv->lg = IALLOC(n);
+  for (int c = 0; c < 5; c++)
+    for (int i = 1; i <= n; i++)
+      if (mark[i] == order[c] && mark[i] != 0) {
+        gl[i] = ++nlf;
+        rv->lg[nlf] = i;
+        if (c < 3) ninner++;
+      }
+  rv->nlf = nlf;
+  rv->ninner = ninner;
+  OM *L = om_new(m, nlf, nlf);
+  for (long k = 1; k <= m; k++) {
+    L->indi[k] = gl[ti[k]];
+    L->indj[k] = gl[tj[k]];
+    L->val[k] = tv[k];
+  }
+  rv->A = L;
+  free(ti); free(tj); free(tv); free(gl);
+  free(sendnodes); free(mark); free(olin_flag); free(olout_flag); free(recv_flag);
+  free(bfront); free(bfs); free(bfe);
+  free(ccount); free(cst); free(corder); free(nghbrs);
+  free(front); free(onfront); free(fs); free(fe); free(nnodesonclrol); free(neighmap);
+  om_free(A);
+}
+
+or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, const double *val,
+                         const or_params *pp, int nranks, const int *rank_of_node,
+                         int nsub_per_rank, int **lg_out, int *nlf_out, int *ninner_out) {
+  or_state *s = (or_state *)xcalloc(1, sizeof(or_state));
+  s->n = n;
+  s->p = *pp;
+  or_params *p = &s->p;
+  for (long i = 0; i < nnz; i++)
+    if (indi[i] < 1 || indi[i] > n || indj[i] < 1 || indj[i] > n) {
+      seterr("matrix entry %ld out of range", i + 1);
+      state_free(s);
+      return NULL;
+    }
+  if (nsub_per_rank > 1) {
+    seterr("the oracle splits each rank into one subdomain only");
+    state_free(s);
+    return NULL;
+  }
+  int *eptn = IALLOC(n);
+  for (int i = 1; i <= n; i++) eptn[i] = rank_of_node[i - 1];
+  /* Distribution_assm.F90:127-138 */
+  if (p->radius1 < 0) p->radius1 = -p->radius1;
+  int ol_dist = p->overlap > p->smoothers ? p->overlap : p->smoothers;
+  if (ol_dist < 1) {
+    seterr("virtual ranks need max(overlap, smoothers) >= 1");
+    free(eptn);
+    state_free(s);
+    return NULL;
+  }
+  OM *Ag = om_new(nnz, n, n);
+  for (long i = 0; i < nnz; i++) {
+    Ag->indi[i + 1] = indi[i];
+    Ag->indj[i + 1] = indj[i];
+    Ag->val[i + 1] = val[i];
+  }
+  om_arrange(Ag, ARR_ROWS, 1, 0, 0);
+  double strong_conn1 = p->strong1 != 0.0 ? p->strong1 : 0.67;
+  int radius1 = p->radius1 > 0 ? p->radius1 : 2;
+  int min_asize1 = p->minasize1 > 0 ? p->minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
+  int max_asize1 = p->maxasize1 > 0 ? p->maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
+  int smoothers = p->smoothers > p->radius1 ? p->radius1 : p->smoothers; /* CoarseMtx.F90:75-78 */
+  int ol_sub = p->overlap < 0 ? (smoothers > 0 ? smoothers : 0) : p->overlap;
+
+  int *num_in = IALLOC(n), *num_full = IALLOC(n);
+  int off_in = 0, off_full = 0;
+  long cap = 0;
+  int **sub = (int **)xcalloc((size_t)nranks + 1, sizeof(int *));
+  int *subn = IALLOC(nranks);
+  for (int r = 1; r <= nranks; r++) {
+    RankView rv;
+    rank_view(Ag, eptn, nranks, r, ol_dist, &rv);
+    if (lg_out) {
+      lg_out[r - 1] = (int *)xcalloc((size_t)rv.nlf + 1, sizeof(int));
+      memcpy(lg_out[r - 1], rv.lg + 1, sizeof(int) * (size_t)rv.nlf);
+      nlf_out[r - 1] = rv.nlf;
+      ninner_out[r - 1] = rv.ninner;
+    }
+    /* getLocal (Aggregate_utils.f95:33-51): the own rows in stored order */
+    long lnz = 0;
+    for (long k = 1; k <= rv.A->nnz; k++)
+      if (rv.A->indi[k] <= rv.ninner) lnz++;
+    int mi = 0, mj = 0;
+    for (long k = 1; k <= rv.A->nnz; k++)
+      if (rv.A->indi[k] <= rv.ninner) {
+        if (rv.A->indi[k] > mi) mi = rv.A->indi[k];
+        if (rv.A->indj[k] > mj) mj = rv.A->indj[k];
+      }
+    OM *LA = om_new(lnz, mi, mj);
+    long t = 0;
+    for (long k = 1; k <= rv.A->nnz; k++)
+      if (rv.A->indi[k] <= rv.ninner) {
+        t++;
+        LA->indi[t] = rv.A->indi[k];
+        LA->indj[t] = rv.A->indj[k];
+        LA->val[t] = rv.A->val[k];
+      }
+    /* a fresh process: its own saved aggregation state */
+    or_reset_process();
+    Aggrs in = {0}, full = {0};
+    om_find_strong(LA, strong_conn1);
+    om_aggregate(LA, &in, &full, radius1, min_asize1, max_asize1, strong_conn1, 0);
+    om_free(LA);
+    /* one part: the aggregated inner nodes in local order
+     * (Partitioning_metis.F90:63-82), then FinePreconditioner_Init's layers
+     * over the rank's matrix arranged by rows */
+    int *nodes = IALLOC(rv.nlf), nn = 0;
+    for (int i = 1; i <= rv.ninner; i++)
+      if (i <= in.n && in.num[i] > 0) nodes[++nn] = i;
+    om_arrange(rv.A, ARR_ROWS, 0, rv.nlf, 0);
+    int *onf = IALLOC(rv.nlf);
+    int on = add_layers(rv.A->bound, rv.A->indj, rv.nlf, nodes, nn, ol_sub, onf);
+    free(onf);
+    sub[r] = IALLOC(on);
+    for (int q = 1; q <= on; q++) sub[r][q] = rv.lg[nodes[q]];
+    subn[r] = on;
+    cap += on;
+    free(nodes);
+    for (int i = 1; i <= rv.ninner; i++) {
+      int g = rv.lg[i];
+      if (i <= in.n && in.num[i] > 0) num_in[g] = in.num[i] + off_in;
+      if (i <= full.n) num_full[g] = full.num[i] + off_full;
+    }
+    off_in += in.nagr;
+    off_full += full.nagr;
+    aggrs_free(&in);
+    aggrs_free(&full);
+    om_free(rv.A);
+    free(rv.lg);
+  }
+  om_free(Ag);
+  form_aggr(&s->fin, off_in, n, num_in);
+  form_aggr(&s->ffull, off_full, n, num_full);
+  free(num_in); free(num_full); free(eptn);
+  s->nsub = nranks;
+  s->sstarts = IALLOC(nranks + 1);
+  s->sinds = IALLOC(cap);
+  s->sstarts[1] = 1;
+  for (int r = 1; r <= nranks; r++) {
+    memcpy(s->sinds + s->sstarts[r], sub[r] + 1, sizeof(int) * (size_t)subn[r]);
+    s->sstarts[r + 1] = s->sstarts[r] + subn[r];
+    free(sub[r]);
+  }
+  free(sub); free(subn);
+
+  /* the assembled matrix for the solve: strength on the whole matrix and the
+   * scale/unscale round trip, as each rank does on its rows
+   * (CoarsePreconditioner_smooth.F90:58-66); the coarse space from the
+   * rank-numbered aggregates */
+  OM *A = om_new(nnz, n, n);
+  for (long i = 0; i < nnz; i++) {
+    A->indi[i + 1] = indi[i];
+    A->indj[i + 1] = indj[i];
+    A->val[i + 1] = val[i];
+  }
+  A->symmstruct = p->symmstruct;
+  A->symmnumeric = p->symmstruct && p->symmnumeric;
+  s->A = A;
+  om_find_strong(A, strong_conn1);
+  om_unscale(A);
+  s->smoothers = smoothers;
+  if (smoothers < 0) {
+    seterr("smoothers=%d not supported", smoothers);
+    state_free(s);
+    return NULL;
+  }
+  s->R = int_rest_build(A, &s->fin, smoothers, p->omega);
+  s->AC = coarse_mtx_build(A, s->R, n);
+  s->has_coarse = 1;
+  s->ol = ol_sub;
+  om_arrange(A, ARR_ROWS, 0, 0, 0);
+  return setup_finish(s);
 }
 
 /* ------------------------------------------------------------------ */

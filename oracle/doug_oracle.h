@@ -51,7 +51,18 @@ void or_reset_process(void);
 or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
                    const double *val, const or_params *p, int nparts,
                    const int *pstarts, const int *pnodes);
+/* Several processes: the per-rank decomposition of a DOUG run on nranks
+ * MPI processes (rank_of_node[i] in 1..nranks, the METIS part of unknown i),
+ * each rank aggregating its own rows with fresh process state and keeping
+ * one subdomain (nsub_per_rank <= 1); the solve then uses the assembled
+ * matrix with the rank-numbered aggregates and subdomains.  When lg_out is
+ * not NULL, lg_out[r] receives a malloc'd copy of rank r's local-to-global
+ * map (nlf_out[r] entries, ninner_out[r] of them its own). */
+or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, const double *val,
+                         const or_params *p, int nranks, const int *rank_of_node,
+                         int nsub_per_rank, int **lg_out, int *nlf_out, int *ninner_out);
 void or_free(or_state *s);
+void or_free_ptr(void *q);
 const char *or_last_error(void);
 
 /* sizes */

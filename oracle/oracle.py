@@ -54,6 +54,10 @@ def lib():
         L.or_setup.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
                                C.c_int, C.c_void_p, C.c_void_p]
         L.or_free.argtypes = [C.c_void_p]
+        L.or_free_ptr.argtypes = [C.c_void_p]
+        L.or_setup_ranks.restype = C.c_void_p
+        L.or_setup_ranks.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
+                                     C.c_int, _i32p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_last_error.restype = C.c_char_p
         for name in ("or_n", "or_nagr_fine", "or_nagr_coarse", "or_nsub", "or_overlap",
                      "or_smoothers"):
@@ -117,6 +121,33 @@ class Oracle:
             raise RuntimeError("oracle setup failed: " + L.or_last_error().decode())
         self.h = h
         self.n = n
+
+    @classmethod
+    def ranks(cls, n, indi, indj, val, p: Params, rank_of_node):
+        """One DOUG run on several processes restated on the CPU:
+        rank_of_node[i] in 1..nranks.  `self.rank_maps` holds each rank's
+        (ninner, local-to-global map), 1-based."""
+        L = lib()
+        self = cls.__new__(cls)
+        self.p = p
+        self.n = n
+        r = np.ascontiguousarray(rank_of_node, dtype=np.int32)
+        nr = int(r.max())
+        lgp = (C.POINTER(C.c_int) * nr)()
+        nlf = np.zeros(nr, np.int32)
+        nin = np.zeros(nr, np.int32)
+        h = L.or_setup_ranks(n, len(val), np.ascontiguousarray(indi, dtype=np.int32),
+                             np.ascontiguousarray(indj, dtype=np.int32),
+                             np.ascontiguousarray(val, dtype=np.float64), C.byref(self.p), nr, r, 1,
+                             C.cast(lgp, C.c_void_p), nlf.ctypes.data, nin.ctypes.data)
+        if not h:
+            raise RuntimeError("oracle setup failed: " + L.or_last_error().decode())
+        self.h = h
+        self.rank_maps = []
+        for k in range(nr):
+            self.rank_maps.append((int(nin[k]), np.ctypeslib.as_array(lgp[k], (int(nlf[k]),)).copy()))
+            L.or_free_ptr(C.cast(lgp[k], C.c_void_p))
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -498,3 +498,44 @@ def test_sgs_long_rows():
     with s:
         r = np.random.default_rng(36).uniform(-1, 1, n)
         assert _bits_equal(s.prec(r), o.prec(r))
+
+
+@pytest.mark.parametrize("case", ["cfg0-4-ranks", "3d-8-ranks-overlap2"])
+def test_virtual_ranks_against_oracle(case):
+    # DOUG's decomposition over several processes (rank-local numbering,
+    # aggregation and subdomains) on one GPU against the oracle's restatement
+    if case == "cfg0-4-ranks":
+        n, i, j, v = P.laplace2d(128)
+        nr, ctl = 4, dict(CFG0_CTL, radius1=-2)
+    else:
+        n, i, j, v = P.diffusion3d(24, 1.0, 5)
+        nr, ctl = 8, dict(CFG0_CTL, radius1=-2, overlap=2, minasize1=20, maxasize1=60)
+    b = P.uniform_rhs(n, 2024)
+    print(f"virtual ranks {case} n={n}", flush=True)
+    with D.Solver(-1) as h:  # the rank map of the project partitioner
+        h.set_params(**ctl)
+        h.set_matrix(n, i, j, v, 1)
+        h.set_ranks(nr)
+        h.setup_host()
+        rank = np.zeros(n, np.int32)
+        for r in range(nr):
+            ni, lg = h.rank_map(r)
+            rank[lg[:ni] - 1] = r + 1
+    o = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank)
+    with D.Solver(0) as s:
+        s.set_params(**ctl)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_ranks(nr, rank)
+        s.setup(0)
+        assert s.info()["nsub"] == nr
+        for a, c in zip(s.subdomains(), o.subdomains()):
+            np.testing.assert_array_equal(a, c)
+        x, it, cond, _ = s.pcg(b, 1e-8, 1000)
+        xo, ito, condo, _ = o.pcg(b, 1e-8, 1000)
+        assert it == ito
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        assert abs(cond - condo) <= 1e-8 * condo
+        s.setup(1)
+        x32, it32, _, _ = s.pcg(b, 1e-8, 1000)
+        assert abs(it32 - ito) <= 2
+        assert np.linalg.norm(x32 - xo) <= 1e-6 * np.linalg.norm(xo)
