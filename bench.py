@@ -200,6 +200,55 @@ def sgs_leg(n, i, j, v, part, b_h, num_iters=3):
     return out
 
 
+def ranks_leg(n, i, j, v, N, b_h, x_ref_h, steps, warmup, nranks=8, nsub=64):
+    """The same workload as a DOUG run on `nranks` processes with `nsub`
+    subdomains each (doug_set_ranks): ranks own the octants of the cube,
+    each rank aggregates its own rows (the ranks run side by side on the
+    host) and splits them into `nsub` subdomains of its aggregate graph.
+    Time to solve for both precision forms, as for the headline line."""
+    import torch
+    import doug_amd as D
+    from doug_amd import problems as P
+    dev = torch.device("cuda:0")
+    st, nd = P.block_partition((N, N, N), N // 2)
+    rank = np.zeros(n, np.int32)
+    for r in range(len(st) - 1):
+        rank[nd[st[r] - 1:st[r + 1] - 1] - 1] = r + 1
+    b = torch.tensor(b_h, dtype=torch.float64, device=dev)
+    x_ref = torch.tensor(x_ref_h, dtype=torch.float64, device=dev)
+    out = {"ranks": nranks, "subdomains_per_rank": nsub}
+    with D.Solver(0) as s:
+        s.set_params(**CTL)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_ranks(nranks, rank)
+        s.set_num_subdomains(nsub)
+        for name, prec in (("fp64", 0), ("fp32_factors", 1)):
+            times, setups = [], []
+            for rep in range(warmup + steps):
+                x = torch.zeros(n, dtype=torch.float64, device=dev)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                info = s.setup(prec)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                it, cond, hist = s.pcg_device(b.data_ptr(), x.data_ptr(), 1e-8, 100000)
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                if rep >= warmup:
+                    times.append(t2 - t0)
+                    setups.append((t1 - t0, t2 - t1))
+            rel_err = float(torch.linalg.norm(x - x_ref) / torch.linalg.norm(x_ref))
+            out[name] = {"time_to_solve_s": round(float(np.mean(times)), 4),
+                         "setup_s": round(float(np.mean([a for a, _ in setups])), 4),
+                         "iterations_s": round(float(np.mean([c for _, c in setups])), 4),
+                         "iterations": int(it), "rel_err_vs_fp64_1e-12_solve": rel_err,
+                         "t_aggregate": round(info["t_aggregate"], 4),
+                         "nsub": int(info["nsub"]), "ncoarse": int(info["ncoarse"]),
+                         "sub_total": int(info["sub_total"])}
+    log(f"[bench] virtual ranks: {out}")
+    return out
+
+
 def direct_accuracy_leg(N, block, sigma, seed):
     """Relative error of both precision forms at rtol 1e-8 against an FP64
     sparse-direct solve (scipy's SuperLU) on an N^3 instance of the same
@@ -303,6 +352,8 @@ def main():
                     help="grid edge of the differentiation workload (0: skip it)")
     ap.add_argument("--sgs", type=int, default=1,
                     help="1: also time the Gauss-Seidel fine level (fine_method 2) on the workload")
+    ap.add_argument("--ranks", type=int, default=8,
+                    help="virtual ranks of the multi-process leg (0: skip it)")
     ap.add_argument("--direct-size", type=int, default=40,
                     help="grid edge of the sparse-direct accuracy check (0: skip it)")
     args = ap.parse_args()
@@ -510,6 +561,10 @@ def main():
         "roofline": roof,
     }
     s.close()
+    if args.ranks > 0:
+        out["virtual_ranks"] = ranks_leg(n, i, j, v, N, b_h, x_ref.cpu().numpy(),
+                                         min(args.steps, 2), min(args.warmup, 1), args.ranks,
+                                         512 // args.ranks if N == 128 else 8)
     if args.sgs:
         out["sgs_fine_level"] = sgs_leg(n, i, j, v, part, b_h)
     if args.fem_points > 0:

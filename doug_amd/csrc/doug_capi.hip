@@ -637,9 +637,14 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
   if (ctx->nranks > 0) {
     // each rank aggregates its own rows of the values as read
-    Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
-    virtual_ranks(As, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains, radius1,
-                  min_asize1, max_asize1, strong_conn1, ctx->rsetup);
+    if (rows_sorted_by_column(ctx->A)) {
+      virtual_ranks(ctx->A, ctx->A_input_val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol,
+                    ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup);
+    } else {
+      Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
+      virtual_ranks(As, As.val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains,
+                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup);
+    }
     ctx->fin = ctx->rsetup.inner;
     ctx->ffull = ctx->rsetup.full;
   } else {

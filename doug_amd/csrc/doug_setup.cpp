@@ -1297,20 +1297,42 @@ Csr sort_rows_by_column(const Csr& A, const std::vector<double>& val) {
   S.nrows = A.nrows;
   S.ncols = A.ncols;
   S.ptr = A.ptr;
-  S.col.resize(A.col.size());
-  S.val.resize(A.col.size());
+  S.col = A.col;
+  S.val = val;
+  // rows are short: a stable insertion sort in place (strict <), nothing to
+  // do for rows already in column order
   parallel_rows(A.nrows, [&](int i) {
-    const int b = A.ptr[i], e = A.ptr[i + 1];
-    std::vector<int> idx;
-    idx.reserve((size_t)(e - b));
-    for (int p = b; p < e; p++) idx.push_back(p);
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return A.col[x] < A.col[y]; });
-    for (int k = 0; k < e - b; k++) {
-      S.col[b + k] = A.col[idx[k]];
-      S.val[b + k] = val[idx[k]];
+    int* c = S.col.data();
+    double* x = S.val.data();
+    for (int p = A.ptr[i] + 1; p < A.ptr[i + 1]; p++) {
+      if (c[p - 1] <= c[p]) continue;
+      const int ck = c[p];
+      const double xk = x[p];
+      int q = p;
+      for (; q > A.ptr[i] && c[q - 1] > ck; q--) {
+        c[q] = c[q - 1];
+        x[q] = x[q - 1];
+      }
+      c[q] = ck;
+      x[q] = xk;
     }
   });
   return S;
+}
+
+bool rows_sorted_by_column(const Csr& A) {
+  std::vector<char> ok((size_t)std::max(1, setup_threads()), 1);
+  parallel_chunks(A.nrows, setup_threads(), [&](int t, int b, int e) {
+    for (int i = b; i < e && ok[t]; i++)
+      for (int p = A.ptr[i] + 1; p < A.ptr[i + 1]; p++)
+        if (A.col[p - 1] > A.col[p]) {
+          ok[t] = 0;
+          break;
+        }
+  });
+  for (char c : ok)
+    if (!c) return false;
+  return true;
 }
 
 namespace {
@@ -1322,8 +1344,8 @@ struct RankScratch {
 };
 
 // Labels: bit 0 inner overlap of some neighbour, bit 1 sent to some neighbour.
-void rank_local(const Csr& As, const std::vector<int>& eptn, int r, const std::vector<int>& own,
-                int ol, RankScratch& S, RankLocal& out) {
+void rank_local(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn, int r,
+                const std::vector<int>& own, int ol, RankScratch& S, RankLocal& out) {
   const int L = std::max(ol, 1), D = 2 * L;
   std::vector<int> touched(own.begin(), own.end());
   for (int v : own) S.layer[v] = 0;
@@ -1400,7 +1422,7 @@ void rank_local(const Csr& As, const std::vector<int>& eptn, int r, const std::v
           const bool sent = lw == 0 && (S.label[w] & 2);
           if (sent != (pass == 0)) continue;
           A.col.push_back(S.gl[w]);
-          A.val.push_back(As.val[p]);
+          A.val.push_back(val[p]);
         }
     }
     A.ptr[k + 1] = (int)A.col.size();
@@ -1428,7 +1450,8 @@ void list_aggregates(int nagr, const std::vector<int>& num, Aggregates& a) {
 
 }  // namespace
 
-void virtual_ranks(const Csr& As, const std::vector<int>& eptn, int nranks, int ol_dist, int ol_sub,
+void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
+                   int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
                    RankSetup& out) {
   const int n = As.nrows;
@@ -1447,7 +1470,7 @@ void virtual_ranks(const Csr& As, const std::vector<int>& eptn, int nranks, int 
       RankScratch S(n);
       for (int r = t; r < nranks; r += nw) {
         RankLocal& R = out.ranks[r];
-        rank_local(As, eptn, r, own[r], ol_dist, S, R);
+        rank_local(As, val, eptn, r, own[r], ol_dist, S, R);
         // getLocal (Aggregate_utils.f95:33-51): the own rows; a new matrix
         // carries no symmetry flags, so the strength test scales by rows
         Csr LA;

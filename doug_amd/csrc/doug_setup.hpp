@@ -155,14 +155,18 @@ struct RankSetup {
   std::vector<int> sstarts, sinds;    // subdomains with their layers, global ids
 };
 
-// `As`: the assembled matrix with each row sorted by column (stable), values
-// as read.  `eptn[i]` in [0, nranks).  `ol_dist` = max(overlap, smoothers)
-// >= 1; `ol_sub` <= ol_dist.
-void virtual_ranks(const Csr& As, const std::vector<int>& eptn, int nranks, int ol_dist, int ol_sub,
+// `As`: the assembled matrix pattern with each row sorted by column
+// (stable); `val`: its values as read, in As's entry order.  `eptn[i]` in
+// [0, nranks).  `ol_dist` = max(overlap, smoothers) >= 1; `ol_sub` <= ol_dist.
+void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
+                   int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
                    RankSetup& out);
 
-// Stable per-row sort by column (SpMtx_arrange with sort, SpMtx_arrangement.F90:159-227).
+// Stable per-row sort by column (SpMtx_arrange with sort, SpMtx_arrangement.F90:159-227),
+// carrying `val`.
 Csr sort_rows_by_column(const Csr& A, const std::vector<double>& val);
+// Whether every row of A already lists its columns in non-decreasing order.
+bool rows_sorted_by_column(const Csr& A);
 
 }  // namespace doug

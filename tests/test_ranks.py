@@ -67,7 +67,17 @@ CASES = {
                                       dict(BASE, radius1=-1, strong1=0.1, smoothers=2, overlap=-1,
                                            symmstruct=0, symmnumeric=0)),
     "3d-scattered": lambda: (P.diffusion3d(10, 1.0, 2), "random", 6, dict(BASE, radius1=2)),
+    # entries in random order: the rows reach the ranks sorted by column
+    # (SpMtx_distributeWithOverlap arranges with sort)
+    "2d-shuffled-input": lambda: (_shuffled(P.diffusion2d(24, 1.0, 9)), "stripes", 3,
+                                  dict(BASE, radius1=-2)),
 }
+
+
+def _shuffled(m):
+    n, i, j, v = m
+    q = np.random.default_rng(3).permutation(len(v))
+    return n, i[q], j[q], v[q]
 
 
 def _case(name):
