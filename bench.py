@@ -37,14 +37,16 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 # workload; the run and its summary are in profiles/hbm_traffic_pmc_128cube.md
 MEASURED_TRAFFIC_128 = {
     # HBM bytes per launch (read + written) from the FETCH_SIZE / WRITE_SIZE runs of
-    # profiles/hbm_traffic_pmc_128cube.md (FETCH_SIZE scaled by the copy-kernel factor)
-    ("k_trsv", 1): 6_053_640_466 + 22_282_240,
-    ("k_spmv", 0): 244_765_692 + 16_777_216,
-    ("k_spmv", 1): 213_560_110 + 16_777_216,
-    ("k_spmv", 2): 222_965_693 + 16_777_216,
-    ("k_spmv", 3): 208_320_176 + 16_777_216,
-    ("k_spmv", 4): 210_944_933 + 16_777_216,
-    ("k_spmv", 5): 207_943_516 + 16_777_216,
+    # profiles/hbm_traffic_pmc_128cube.md (FETCH_SIZE scaled by the copy-kernel factor);
+    # the subdomain solve per decomposition of the workload
+    ("k_trsv", 1, "ranks"): 6_053_464_144 + 22_282_240,
+    ("k_trsv", 1, "boxes"): 6_053_640_466 + 22_282_240,
+    ("k_spmv", 0): 244_837_813 + 16_777_216,
+    ("k_spmv", 1): 213_531_288 + 16_777_216,
+    ("k_spmv", 2): 222_975_915 + 16_777_216,
+    ("k_spmv", 3): 208_325_852 + 16_777_216,
+    ("k_spmv", 4): 210_949_730 + 16_777_216,
+    ("k_spmv", 5): 207_899_218 + 16_777_216,
 }
 
 CTL = dict(radius1=4, minasize1=100, maxasize1=343, strong1=0.08, smoothers=1, overlap=1,
@@ -59,7 +61,7 @@ def spmv_bytes(n, nnz):
     return 12 * nnz + 4 * (n + 1) + 8 * n + 8 * n
 
 
-def cpu_baseline(size, block, sigma, seed):
+def cpu_baseline(size, block, sigma, seed, decomp):
     """The oracle (a sequential C restatement of the reference) on a bounded
     sample: the same operator family at size^3 with the same subdomain edge,
     setup + PCG to 1e-8, scaled to the full problem by the unknown count."""
@@ -71,14 +73,19 @@ def cpu_baseline(size, block, sigma, seed):
     n, i, j, v = P.diffusion3d(m, sigma, seed)
     b = P.uniform_rhs(n, seed + 1)
     t0 = time.perf_counter()
-    o = O.Oracle(n, i, j, v, O.params(**CTL), partition=P.block_partition((m,) * 3, block))
+    if decomp == "ranks":
+        rank, nr = box_ranks(m, block)
+        o = O.Oracle.ranks(n, i, j, v, O.params(**CTL), rank)
+        how = f"{nr} processes owning one {block}^3 box each"
+    else:
+        o = O.Oracle(n, i, j, v, O.params(**CTL), partition=P.block_partition((m,) * 3, block))
+        how = f"one process, {(m // block) ** 3} subdomains of {block}^3"
     x, it, cond, _ = o.pcg(b, 1e-8, 1000)
     dt = time.perf_counter() - t0
     scale = (size ** 3) / n
     return {"value": round(dt * scale, 3), "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"oracle setup+PCG(1e-8) on the {m}^3 instance ({(m // block) ** 3} "
-                      f"subdomains of {block}^3, {it} iterations) took {dt:.2f} s, "
-                      f"scaled x{scale:.0f} by unknown count"}
+            "sample": f"oracle setup+PCG(1e-8) on the {m}^3 instance ({how}, {it} iterations) "
+                      f"took {dt:.2f} s, scaled x{scale:.0f} by unknown count"}
 
 
 FEM_CTL = dict(radius1=5, minasize1=60, maxasize1=121, strong1=0.08, smoothers=1, overlap=2,
@@ -200,28 +207,44 @@ def sgs_leg(n, i, j, v, part, b_h, num_iters=3):
     return out
 
 
-def ranks_leg(n, i, j, v, N, b_h, x_ref_h, steps, warmup, nranks=8, nsub=64):
-    """The same workload as a DOUG run on `nranks` processes with `nsub`
-    subdomains each (doug_set_ranks): ranks own the octants of the cube,
-    each rank aggregates its own rows (the ranks run side by side on the
-    host) and splits them into `nsub` subdomains of its aggregate graph.
-    Time to solve for both precision forms, as for the headline line."""
+def box_ranks(N, edge):
+    """Rank (1-based) of each unknown of the N^3 grid when processes own
+    boxes of edge `edge`, numbered box-lexicographically."""
+    from doug_amd import problems as P
+    st, nd = P.block_partition((N, N, N), edge)
+    rank = np.zeros(N ** 3, np.int32)
+    for r in range(len(st) - 1):
+        rank[nd[st[r] - 1:st[r + 1] - 1] - 1] = r + 1
+    return rank, len(st) - 1
+
+
+def form_leg(n, i, j, v, N, b_h, x_ref_h, steps, warmup, nranks, nsub, edge=16):
+    """The same workload decomposed another way, timed as the headline is
+    (setup + PCG to 1e-8, both precision forms).  nranks > 0: a DOUG run on
+    `nranks` processes with `nsub` subdomains each (doug_set_ranks), ranks
+    owning boxes of edge N / nranks^(1/3); nranks = 0: one process with the
+    `edge`^3 boxes as the subdomain partition."""
     import torch
     import doug_amd as D
     from doug_amd import problems as P
     dev = torch.device("cuda:0")
-    st, nd = P.block_partition((N, N, N), N // 2)
-    rank = np.zeros(n, np.int32)
-    for r in range(len(st) - 1):
-        rank[nd[st[r] - 1:st[r + 1] - 1] - 1] = r + 1
+    if nranks > 0:
+        rank, nr = box_ranks(N, int(round(N / round(nranks ** (1.0 / 3.0)))))
+        assert nr == nranks
+        out = {"ranks": nranks, "rank_box": int(round(N / round(nranks ** (1.0 / 3.0)))),
+               "subdomains_per_rank": nsub}
+    else:
+        out = {"ranks": 1, "partition": f"{edge}^3 boxes"}
     b = torch.tensor(b_h, dtype=torch.float64, device=dev)
     x_ref = torch.tensor(x_ref_h, dtype=torch.float64, device=dev)
-    out = {"ranks": nranks, "subdomains_per_rank": nsub}
     with D.Solver(0) as s:
         s.set_params(**CTL)
         s.set_matrix(n, i, j, v, 1)
-        s.set_ranks(nranks, rank)
-        s.set_num_subdomains(nsub)
+        if nranks > 0:
+            s.set_ranks(nranks, rank)
+            s.set_num_subdomains(nsub)
+        else:
+            s.set_partition(*P.block_partition((N, N, N), edge))
         for name, prec in (("fp64", 0), ("fp32_factors", 1)):
             times, setups = [], []
             for rep in range(warmup + steps):
@@ -245,7 +268,7 @@ def ranks_leg(n, i, j, v, N, b_h, x_ref_h, steps, warmup, nranks=8, nsub=64):
                          "t_aggregate": round(info["t_aggregate"], 4),
                          "nsub": int(info["nsub"]), "ncoarse": int(info["ncoarse"]),
                          "sub_total": int(info["sub_total"])}
-    log(f"[bench] virtual ranks: {out}")
+    log(f"[bench] decomposition {out}")
     return out
 
 
@@ -352,8 +375,12 @@ def main():
                     help="grid edge of the differentiation workload (0: skip it)")
     ap.add_argument("--sgs", type=int, default=1,
                     help="1: also time the Gauss-Seidel fine level (fine_method 2) on the workload")
-    ap.add_argument("--ranks", type=int, default=8,
-                    help="virtual ranks of the multi-process leg (0: skip it)")
+    ap.add_argument("--decomp", choices=["ranks", "boxes"], default="ranks",
+                    help="headline decomposition: DOUG on (size/block)^3 processes with one "
+                         "block^3 box each (ranks) or one process with the boxes as partition")
+    ap.add_argument("--ranks", default="8x64",
+                    help="further multi-process legs as RANKSxSUBDOMAINS, comma separated "
+                         "('' skips them)")
     ap.add_argument("--direct-size", type=int, default=40,
                     help="grid edge of the sparse-direct accuracy check (0: skip it)")
     args = ap.parse_args()
@@ -376,7 +403,13 @@ def main():
     s = D.Solver(0)
     s.set_params(**CTL)
     s.set_matrix(n, i, j, v, 1)
-    s.set_partition(*part)
+    if args.decomp == "ranks":
+        # "METIS into K subdomains" as DOUG runs it: K processes with one
+        # subdomain each; process r owns the r-th blk^3 box
+        rank, nranks = box_ranks(N, blk)
+        s.set_ranks(nranks, rank)
+    else:
+        s.set_partition(*part)
     dev = torch.device("cuda:0")
     b = torch.tensor(b_h, dtype=torch.float64, device=dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
@@ -518,8 +551,8 @@ def main():
     info = s.info()
     dominant_trsv = best["iterations"] * best["trsv_ms"] * 1e-3 >= best["cholesky_s"]
     if dominant_trsv:
-        traffic = MEASURED_TRAFFIC_128.get(("k_trsv", 1 if best_name == "fp32_factors" else 0)) \
-            if N == 128 and blk == 16 else None
+        traffic = MEASURED_TRAFFIC_128.get(("k_trsv", 1 if best_name == "fp32_factors" else 0,
+                                            args.decomp)) if N == 128 and blk == 16 else None
         roof = {"kernel": "k_trsv (batched subdomain forward+backward solves)", "bound": "hbm",
                 "achieved": best["trsv_GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(best["trsv_GB/s"] / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -544,8 +577,11 @@ def main():
         "dtype": "f64 (fp32 subdomain factors)" if best_name == "fp32_factors" else "f64",
         "data": "synthetic",
         "config": {"workload": f"3d 7-point exp(sigma g) diffusion {N}^3, {info['nsub']} subdomains "
-                               f"({blk}^3 boxes) overlap {info['overlap']}, smoothed aggregation, "
-                               "PCG rtol 1e-8",
+                               + (f"(DOUG decomposition over {info['nsub']} processes, one {blk}^3 "
+                                  "box and one subdomain each)" if args.decomp == "ranks" else
+                                  f"({blk}^3 boxes, one process)")
+                               + f" overlap {info['overlap']}, smoothed aggregation, PCG rtol 1e-8",
+                   "decomposition": args.decomp,
                    "n": n, "nnz": nnz, "sigma": args.sigma, "seed": args.seed,
                    "ncoarse": info["ncoarse"], "ctl": CTL},
         "form": best_name,
@@ -561,10 +597,15 @@ def main():
         "roofline": roof,
     }
     s.close()
-    if args.ranks > 0:
-        out["virtual_ranks"] = ranks_leg(n, i, j, v, N, b_h, x_ref.cpu().numpy(),
-                                         min(args.steps, 2), min(args.warmup, 1), args.ranks,
-                                         512 // args.ranks if N == 128 else 8)
+    # the other decompositions of the same workload
+    out["decompositions"] = {}
+    if args.decomp == "ranks":
+        out["decompositions"]["one_process"] = form_leg(
+            n, i, j, v, N, b_h, x_ref.cpu().numpy(), min(args.steps, 2), min(args.warmup, 1), 0, 1, blk)
+    for cfg in [c for c in args.ranks.split(",") if c]:
+        nr, ns = (int(t) for t in cfg.split("x"))
+        out["decompositions"][f"{nr}_ranks_x_{ns}"] = form_leg(
+            n, i, j, v, N, b_h, x_ref.cpu().numpy(), min(args.steps, 2), min(args.warmup, 1), nr, ns)
     if args.sgs:
         out["sgs_fine_level"] = sgs_leg(n, i, j, v, part, b_h)
     if args.fem_points > 0:
@@ -578,7 +619,7 @@ def main():
         out["differentiation"] = differentiation_leg(args.diff_size, 16, 20, args.seed + 11)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(N, blk, args.sigma, args.seed)
+        out["cpu_baseline"] = cpu_baseline(N, blk, args.sigma, args.seed, args.decomp)
     print(json.dumps(out), flush=True)
 
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -40,6 +41,7 @@ struct doug_ctx {
 
   // host problem
   bool have_matrix = false, ready = false, host_ready = false;
+  int dup_state = -1;   // A holds a duplicate (i, j) pair: 1, none: 0, not yet known: -1
   Csr A;                // values after the scale/unscale round trip once set up
   std::vector<double> A_input_val;
   std::vector<double> scaled;  // strength-test values (kept to reuse the allocation)
@@ -64,6 +66,7 @@ struct doug_ctx {
                                    // values stay on the device until a getter asks
   uint8_t* d_strong = nullptr;     // strength mask, uploaded while the aggregation runs
   bool use_device_coarse = true;   // DOUG_OPT_DEVICE_COARSE
+  bool use_device_blocks = true;   // DOUG_OPT_DEVICE_BLOCKS
 
   // device: matrix
   std::vector<void*> allocs;
@@ -327,6 +330,7 @@ int doug_set_matrix_coo(doug_ctx* ctx, int32_t n, int64_t nnz, const int32_t* in
   ctx->A_input_val = ctx->A.val;
   ctx->n = n;
   ctx->have_matrix = true;
+  ctx->dup_state = -1;
   ctx->ready = false;
   return DOUG_OK;
 }
@@ -590,6 +594,112 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
   launch_pad_identity(*d_tiles, *d_desc, nb, maxpad, ctx->stream);
   launch_scatter_entries(*d_tiles, d_dst, d_src, d_vals, (int64_t)dst.size(), ctx->stream);
   // the host copies above must stay alive until the stream consumed them
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipGetLastError());
+  return DOUG_OK;
+}
+
+// Whether some row of A holds the same column twice.
+static bool has_duplicate_entries(const Csr& A) {
+  std::atomic<bool> dup{false};
+  const int nth = setup_threads();
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; t++)
+    th.emplace_back([&, t]() {
+      std::vector<int> row;
+      const int b = (int)((int64_t)A.nrows * t / nth), e = (int)((int64_t)A.nrows * (t + 1) / nth);
+      for (int i = b; i < e && !dup.load(std::memory_order_relaxed); i++) {
+        row.assign(A.col.begin() + A.ptr[i], A.col.begin() + A.ptr[i + 1]);
+        std::sort(row.begin(), row.end());
+        if (std::adjacent_find(row.begin(), row.end()) != row.end()) dup = true;
+      }
+    });
+  for (auto& x : th) x.join();
+  return dup;
+}
+
+// build_blocks for the subdomain factors (envelope storage, one-block solve
+// limit) with the envelope and the entries taken on the device from the
+// uploaded matrix, unknown lists (ctx->d_inds) and occurrence lists, so the
+// host builds no scatter map.  A must have no duplicate (i, j) pair.
+template <typename T>
+static int build_blocks_device(doug_ctx* ctx, const std::vector<int>& starts, std::vector<BlockDesc>& desc,
+                               BlockDesc** d_desc, T** d_tiles, int64_t* total_elems,
+                               std::vector<int>& offcount, double* chol_flops) {
+  const int nb = (int)starts.size() - 1;
+  desc.resize((size_t)nb);
+  std::vector<int32_t> row0((size_t)nb + 1, 0), st32(starts.begin(), starts.end());
+  int ntmax = 0;
+  for (int b = 0; b < nb; b++) {
+    int s = starts[b + 1] - starts[b];
+    int nt = (s + kTile - 1) / kTile;
+    if (nt > kMaxTrsvTiles)
+      return fail(ctx, DOUG_E_UNSUPPORTED,
+                  "subdomain of " + std::to_string(s) + " unknowns exceeds the one-block solve limit " +
+                      std::to_string(kMaxTrsvTiles * kTile));
+    row0[b + 1] = row0[b] + nt;
+    ntmax = std::max(ntmax, nt);
+  }
+  std::vector<int32_t> fc((size_t)row0[nb]);
+  for (int b = 0; b < nb; b++)
+    for (int i = 0; i < row0[b + 1] - row0[b]; i++) fc[row0[b] + i] = i;
+  int rc;
+  int32_t *d_st, *d_r0, *d_fc;
+  if ((rc = upload(ctx, &d_st, st32)) || (rc = upload(ctx, &d_r0, row0)) || (rc = upload(ctx, &d_fc, fc)))
+    return rc;
+  launch_block_envelope(d_st, d_r0, nb, ctx->d_inds, ctx->d_occ_ptr, ctx->d_occ_pos, ctx->d_ptr,
+                        ctx->d_col, d_fc, ctx->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(fc.data(), d_fc, fc.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // tile-row tables as build_blocks makes them from the first columns
+  std::vector<TileRow> rows((size_t)row0[nb]);
+  for (int b = 0; b < nb; b++) {
+    const int nt = row0[b + 1] - row0[b];
+    TileRow* tr = rows.data() + row0[b];
+    for (int i = 0; i < nt; i++) tr[i] = TileRow{0, fc[row0[b] + i], i, 0};
+    for (int i = nt - 2; i >= 0; i--) tr[i].fc = std::min(tr[i].fc, tr[i + 1].fc);
+    int off = 0;
+    for (int i = 0; i < nt; i++) {
+      tr[i].off = off;
+      off += i - tr[i].fc + 1;
+      for (int j = tr[i].fc; j <= i; j++) tr[j].lr = std::max(tr[j].lr, i);
+    }
+  }
+  TileRow* d_rows;
+  if ((rc = upload(ctx, &d_rows, rows))) return rc;
+  int64_t base = 0, woff = 0;
+  offcount.assign((size_t)ntmax, 0);
+  for (int b = 0; b < nb; b++) {
+    const int s = starts[b + 1] - starts[b], nt = row0[b + 1] - row0[b];
+    const TileRow* tr = rows.data() + row0[b];
+    desc[b] = BlockDesc{base, (int64_t)starts[b], woff, d_rows + row0[b], nt, s};
+    base += (int64_t)(tr[nt - 1].off + nt - tr[nt - 1].fc) * kTileElems;
+    woff += (int64_t)nt * kTile;
+    for (int j = 0; j < nt; j++) offcount[j] = std::max(offcount[j], tr[j].lr - j);
+  }
+  *total_elems = base;
+  if (chol_flops) {
+    const double g = 2.0 * kTile * kTile * kTile;
+    double f = 0.0;
+    for (int b = 0; b < nb; b++) {
+      const TileRow* tr = rows.data() + row0[b];
+      for (int j = 0; j < desc[b].nt; j++) {
+        f += g * (j - tr[j].fc) + 2.0 * g / 3.0;
+        for (int i = j + 1; i <= tr[j].lr; i++) f += g * (j - tr[i].fc + 1);
+      }
+    }
+    *chol_flops = f;
+  }
+  if ((rc = dalloc(ctx, d_tiles, (size_t)base))) return rc;
+  if ((rc = upload(ctx, d_desc, desc))) return rc;
+  HIPCHK(hipMemsetAsync(*d_tiles, 0, (size_t)base * sizeof(T), ctx->stream));
+  int maxpad = 0;
+  for (auto& d : desc) maxpad = std::max(maxpad, d.nt * kTile - d.s);
+  launch_pad_identity(*d_tiles, *d_desc, nb, maxpad, ctx->stream);
+  launch_scatter_blocks(*d_tiles, *d_desc, nb, ctx->d_inds, ctx->d_occ_ptr, ctx->d_occ_pos, ctx->d_ptr,
+                        ctx->d_col, ctx->d_val, ctx->stream);
+  // the host tables above must stay alive until the stream consumed them
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipGetLastError());
   return DOUG_OK;
@@ -1236,16 +1346,25 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
   std::vector<int> offcount;
   if (p.fine_method == 1) {
+    // duplicate (i, j) pairs would race in the device scatter: the host
+    // scatter map finds whether one falls inside a subdomain (and is the
+    // one DOUG_OPT_DEVICE_BLOCKS 0 asks for)
+    if (ctx->dup_state < 0) ctx->dup_state = has_duplicate_entries(A) ? 1 : 0;
+    const bool dup = ctx->dup_state == 1 || !ctx->use_device_blocks;
     if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
-      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, dinds, true, true, ctx->h_sub,
-                             &ctx->d_sub, &ctx->d_tiles32, &elems, offcount, &info.factor_flops)))
+      if ((rc = dup ? build_blocks(ctx, A, ctx->d_val, ctx->sstarts, dinds, true, true, ctx->h_sub,
+                                   &ctx->d_sub, &ctx->d_tiles32, &elems, offcount, &info.factor_flops)
+                    : build_blocks_device(ctx, ctx->sstarts, ctx->h_sub, &ctx->d_sub, &ctx->d_tiles32,
+                                          &elems, offcount, &info.factor_flops)))
         return rc;
       HIPCHK(hipEventRecord(ev0, ctx->stream));
       if ((rc = enqueue_factor(ctx, ctx->d_tiles32, ctx->d_sub, ctx->h_sub, offcount, ctx->d_flag)))
         return rc;
     } else {
-      if ((rc = build_blocks(ctx, A, ctx->d_val, ctx->sstarts, dinds, true, true, ctx->h_sub,
-                             &ctx->d_sub, &ctx->d_tiles, &elems, offcount, &info.factor_flops)))
+      if ((rc = dup ? build_blocks(ctx, A, ctx->d_val, ctx->sstarts, dinds, true, true, ctx->h_sub,
+                                   &ctx->d_sub, &ctx->d_tiles, &elems, offcount, &info.factor_flops)
+                    : build_blocks_device(ctx, ctx->sstarts, ctx->h_sub, &ctx->d_sub, &ctx->d_tiles,
+                                          &elems, offcount, &info.factor_flops)))
         return rc;
       HIPCHK(hipEventRecord(ev0, ctx->stream));
       if ((rc = enqueue_factor(ctx, ctx->d_tiles, ctx->d_sub, ctx->h_sub, offcount, ctx->d_flag)))
@@ -1977,6 +2096,10 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
       if (value != 1 && value != 2) return fail(ctx, DOUG_E_ARG, "tiles per step must be 1 or 2");
       drop_iteration_graph(ctx);
       ctx->trsm_kb = (int)value;
+      return DOUG_OK;
+    case DOUG_OPT_DEVICE_BLOCKS:
+      ctx->use_device_blocks = value != 0;
+      ctx->ready = false;  // takes effect at the next doug_setup
       return DOUG_OK;
     case DOUG_OPT_DEVICE_COARSE:
       ctx->use_device_coarse = value != 0;

@@ -1517,6 +1517,89 @@ DOUG_INSTANTIATE(float)
 #undef DOUG_INSTANTIATE
 
 
+
+// ------------------------------------------------------------------------
+// Subdomain blocks from the device's copies of A, the blocks' unknown lists
+// and each unknown's occurrence list (its positions q in the concatenated
+// lists, ascending).  Row li of block b is unknown inds[s0 + li]; column c of
+// A is local column q - s0 for the occurrence q of c inside [s0, s0 + s),
+// found by scanning c's short list.  One workgroup per block.
+
+__device__ __forceinline__ int local_col(const int32_t* __restrict__ occ_ptr,
+                                         const int32_t* __restrict__ occ_pos, int c, int s0, int s) {
+  for (int k = occ_ptr[c]; k < occ_ptr[c + 1]; k++) {
+    const int q = occ_pos[k];
+    if (q >= s0 + s) break;
+    if (q >= s0) return q - s0;
+  }
+  return -1;
+}
+
+// First tile column of each tile row's envelope: min over the row's lower
+// entries of their tile column; `fc` holds the tile row index on entry.
+__global__ void k_block_envelope(const int32_t* __restrict__ starts, const int32_t* __restrict__ row0,
+                                 const int32_t* __restrict__ inds, const int32_t* __restrict__ occ_ptr,
+                                 const int32_t* __restrict__ occ_pos, const int32_t* __restrict__ a_ptr,
+                                 const int32_t* __restrict__ a_col, int32_t* __restrict__ fc) {
+  const int b = blockIdx.x;
+  const int s0 = starts[b], s = starts[b + 1] - s0;
+  int32_t* f = fc + row0[b];
+  for (int li = threadIdx.x; li < s; li += blockDim.x) {
+    const int g = inds[s0 + li];
+    int m = li;
+    for (int p = a_ptr[g]; p < a_ptr[g + 1]; p++) {
+      const int lj = local_col(occ_ptr, occ_pos, a_col[p], s0, s);
+      if (lj >= 0 && lj < m) m = lj;
+    }
+    atomicMin(&f[li / kTile], m / kTile);
+  }
+}
+
+// The lower entries of each block into its tile storage (converted to T).
+// Requires no duplicate (i, j) pair in A, which would race.
+template <typename T>
+__global__ void k_scatter_blocks(T* __restrict__ tiles, const BlockDesc* __restrict__ d,
+                                 const int32_t* __restrict__ inds, const int32_t* __restrict__ occ_ptr,
+                                 const int32_t* __restrict__ occ_pos, const int32_t* __restrict__ a_ptr,
+                                 const int32_t* __restrict__ a_col, const double* __restrict__ a_val) {
+  const BlockDesc b = d[blockIdx.x];
+  const int s0 = (int)b.vec_off, s = b.s;
+  for (int li = threadIdx.x; li < s; li += blockDim.x) {
+    const int g = inds[s0 + li];
+    const TileRow ri = b.tr[li / kTile];
+    for (int p = a_ptr[g]; p < a_ptr[g + 1]; p++) {
+      const int lj = local_col(occ_ptr, occ_pos, a_col[p], s0, s);
+      if (lj < 0 || lj > li) continue;
+      const int64_t pos = b.base + (int64_t)(ri.off + lj / kTile - ri.fc) * kTileElems +
+                          (int64_t)(li % kTile) * kTile + (lj % kTile);
+      tiles[pos] = (T)a_val[p];
+    }
+  }
+}
+
+void launch_block_envelope(const int32_t* starts, const int32_t* row0, int nblocks, const int32_t* inds,
+                           const int32_t* occ_ptr, const int32_t* occ_pos, const int32_t* a_ptr,
+                           const int32_t* a_col, int32_t* fc, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_block_envelope, dim3(nblocks), dim3(256), 0, st, starts, row0, inds, occ_ptr,
+                     occ_pos, a_ptr, a_col, fc);
+}
+
+template <typename T>
+void launch_scatter_blocks(T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+                           const int32_t* occ_ptr, const int32_t* occ_pos, const int32_t* a_ptr,
+                           const int32_t* a_col, const double* a_val, hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_scatter_blocks<T>, dim3(nblocks), dim3(256), 0, st, tiles, d, inds, occ_ptr,
+                     occ_pos, a_ptr, a_col, a_val);
+}
+template void launch_scatter_blocks<float>(float*, const BlockDesc*, int, const int32_t*, const int32_t*,
+                                           const int32_t*, const int32_t*, const int32_t*, const double*,
+                                           hipStream_t);
+template void launch_scatter_blocks<double>(double*, const BlockDesc*, int, const int32_t*, const int32_t*,
+                                            const int32_t*, const int32_t*, const int32_t*, const double*,
+                                            hipStream_t);
+
 // ------------------------------------------------------------------------
 // Symmetric Gauss-Seidel sweeps (SymGaussSeidel, stationary.F90:33-88):
 // x_i = (b_i - sum_{j != i} a_ij x_j) / a_ii in ascending column order, rows

@@ -43,6 +43,21 @@ struct BlockDesc {
 template <typename T>
 void launch_scatter_entries(T* tiles, const int64_t* dst, const int32_t* src, const double* val,
                             int64_t nentries, hipStream_t st);
+// Envelope and entries of index blocks from the device copies of A (CSR),
+// the blocks' unknown lists `inds` and each unknown's ascending occurrence
+// positions in them (occ_ptr/occ_pos).  launch_block_envelope lowers fc
+// (one entry per tile row, block b's rows from row0[b], holding the tile row
+// index on entry) to each tile row's first stored tile column;
+// launch_scatter_blocks writes the lower entries into zeroed tile storage.
+// A must have no duplicate (i, j) pair.
+void launch_block_envelope(const int32_t* starts, const int32_t* row0, int nblocks, const int32_t* inds,
+                           const int32_t* occ_ptr, const int32_t* occ_pos, const int32_t* a_ptr,
+                           const int32_t* a_col, int32_t* fc, hipStream_t st);
+template <typename T>
+void launch_scatter_blocks(T* tiles, const BlockDesc* d, int nblocks, const int32_t* inds,
+                           const int32_t* occ_ptr, const int32_t* occ_pos, const int32_t* a_ptr,
+                           const int32_t* a_col, const double* a_val, hipStream_t st);
+
 template <typename T>
 void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, hipStream_t st);
 // Left-looking tile Cholesky, step j: diagonal tiles (SYRK + POTRF + inverse)

@@ -240,14 +240,19 @@ int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
  * Galerkin coarse matrix on the GPU (same bits as the host build, which 0
  * selects); applies to smoothers = 1 at the next doug_setup.
  * DOUG_OPT_BATCH_TILES (1 or 2, default 2): off-diagonal tiles the batched
- * envelope solve stages per step (what doug_autotune sets). */
+ * envelope solve stages per step (what doug_autotune sets).
+ * DOUG_OPT_DEVICE_BLOCKS (default 1): take the subdomain factors' tile
+ * envelopes and entries on the GPU from the uploaded matrix (same tiles as
+ * the host scatter map, which 0 selects; a matrix holding a duplicate
+ * (i, j) pair always uses the host map); at the next doug_setup. */
 enum {
   DOUG_OPT_ITERATION_GRAPH = 1,
   DOUG_OPT_SPMV_SHAPE = 2,
   DOUG_OPT_TRSV_ORDER = 3,
   DOUG_OPT_LOCAL_ORDER = 4,
   DOUG_OPT_DEVICE_COARSE = 5,
-  DOUG_OPT_BATCH_TILES = 6
+  DOUG_OPT_BATCH_TILES = 6,
+  DOUG_OPT_DEVICE_BLOCKS = 7
 };
 int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 

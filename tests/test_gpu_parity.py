@@ -539,3 +539,46 @@ def test_virtual_ranks_against_oracle(case):
         x32, it32, _, _ = s.pcg(b, 1e-8, 1000)
         assert abs(it32 - ito) <= 2
         assert np.linalg.norm(x32 - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_block_build_matches_host_scatter(precision):
+    # the subdomain tiles taken on the GPU (envelope and entries through the
+    # occurrence lists) equal the host scatter map's, so every apply is
+    # bit-identical; also with virtual ranks' overlapping subdomains
+    n, i, j, v = P.diffusion3d(20, 1.0, 6)
+    r = np.random.default_rng(1).uniform(-1, 1, n)
+    print(f"device blocks precision={precision}", flush=True)
+    outs = []
+    for dev in (1, 0):
+        with D.Solver(0) as s:
+            s.set_params(**dict(CFG0_CTL, overlap=2))
+            s.set_matrix(n, i, j, v, 1)
+            s.set_partition(*P.block_partition((20, 20, 20), 10))
+            s.set_option(7, dev)
+            s.setup(precision)
+            a = s.prec(r)
+            s.set_partition(np.array([0], np.int32), np.array([], np.int32))
+            s.set_ranks(3)
+            s.setup(precision)
+            outs.append((a, s.prec(r), s.info()["factor_tiles"]))
+    for x, y in zip(outs[0], outs[1]):
+        if isinstance(x, np.ndarray):
+            assert _bits_equal(x, y)
+        else:
+            assert x == y
+
+
+def test_duplicate_entry_inside_a_subdomain_is_refused():
+    n, i, j, v = P.laplace2d(16)
+    d = np.flatnonzero(i == j)[5]
+    i2 = np.r_[i, i[d]]
+    j2 = np.r_[j, j[d]]
+    v2 = np.r_[v, 0.0]
+    print("duplicate entry", flush=True)
+    with D.Solver(0) as s:
+        s.set_params(**CFG0_CTL)
+        s.set_matrix(n, i2, j2, v2, 1)
+        s.set_partition(*P.block_partition((16, 16), 8))
+        with pytest.raises(RuntimeError, match="duplicate"):
+            s.setup(0)
