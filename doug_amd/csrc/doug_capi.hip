@@ -747,16 +747,20 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
   if (ctx->nranks > 0) {
     // each rank aggregates its own rows of the values as read
+    // the ranks also put their subdomains in the factor order of the device
+    // copy (reverse Cuthill-McKee over A's pattern), off the critical path
+    const Csr* rcm = (p.fine_method == 1 && ctx->local_order == 1 && ctx->device >= 0) ? &ctx->A : nullptr;
     if (rows_sorted_by_column(ctx->A)) {
       virtual_ranks(ctx->A, ctx->A_input_val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol,
-                    ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup);
+                    ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup,
+                    rcm);
     } else {
       Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
       virtual_ranks(As, As.val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains,
-                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup);
+                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup, rcm);
     }
-    ctx->fin = ctx->rsetup.inner;
-    ctx->ffull = ctx->rsetup.full;
+    ctx->fin = std::move(ctx->rsetup.inner);
+    ctx->ffull = std::move(ctx->rsetup.full);
   } else {
     aggregate(ctx->A, scaled, strong, radius1, min_asize1, max_asize1, strong_conn1, false, ctx->ps,
               ctx->fin, ctx->ffull);
@@ -974,8 +978,8 @@ static void phase_subdomains(doug_ctx* ctx) {
   ctx->cin = Aggregates{};
   ctx->cfull = Aggregates{};
   if (ctx->nranks > 0) {  // the ranks' subdomains, layers included
-    ctx->sstarts = ctx->rsetup.sstarts;
-    ctx->sinds = ctx->rsetup.sinds;
+    ctx->sstarts = std::move(ctx->rsetup.sstarts);
+    ctx->sinds = std::move(ctx->rsetup.sinds);
     ctx->nsub = (int)ctx->sstarts.size() - 1;
     ctx->info.t_subdomains = now() - t2;
     return;
@@ -1303,7 +1307,10 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   // tile envelope
   std::vector<int> dinds;
   if (p.fine_method == 1 && ctx->local_order == 1) {
-    block_rcm(A, ctx->sstarts, ctx->sinds, dinds);
+    if (ctx->nranks > 0 && ctx->rsetup.dinds.size() == ctx->sinds.size())
+      dinds = std::move(ctx->rsetup.dinds);  // ordered by the ranks' workers
+    else
+      block_rcm(A, ctx->sstarts, ctx->sinds, dinds);
   } else {
     dinds = ctx->sinds;
     if (p.fine_method == 2)  // Gauss-Seidel sweeps the rows in ascending order

@@ -999,6 +999,100 @@ int bfs_levels(const LocalGraph& g, int root, std::vector<int>& mark, int stamp,
 
 }  // namespace
 
+namespace {
+
+struct RcmBuf {
+  std::vector<int> mark, order, bfs, nbr;
+  std::vector<char> placed;
+  LocalGraph g;
+};
+
+// One block [s0, s0 + s) of `inds` into `out`; `floc` is -1 on entry and exit.
+void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::vector<int>& out,
+               std::vector<int>& floc, RcmBuf& buf) {
+  std::vector<int>& mark = buf.mark;
+  std::vector<int>& order = buf.order;
+  std::vector<int>& bfs = buf.bfs;
+  std::vector<int>& nbr = buf.nbr;
+  std::vector<char>& placed = buf.placed;
+  LocalGraph& g = buf.g;
+  if (s <= 1) {
+    if (s == 1) out[s0] = inds[s0];
+    return;
+  }
+  for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
+  g.ptr.assign((size_t)s + 1, 0);
+  g.adj.clear();
+  g.deg.assign((size_t)s, 0);
+  for (int li = 0; li < s; li++) {
+    int gi = inds[s0 + li];
+    for (int p = A.ptr[gi]; p < A.ptr[gi + 1]; p++) {
+      int lj = floc[A.col[p]];
+      if (lj >= 0 && lj != li) g.adj.push_back(lj);
+    }
+    g.ptr[li + 1] = (int)g.adj.size();
+    g.deg[li] = g.ptr[li + 1] - g.ptr[li];
+  }
+  for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
+  mark.assign((size_t)s, 0);
+  placed.assign((size_t)s, 0);
+  order.clear();
+  int stamp = 0, next = 0;
+  while ((int)order.size() < s) {
+    while (placed[next]) next++;
+    // minimum-degree unplaced node of this component
+    int root = next;
+    {
+      int lb, le;
+      bfs_levels(g, next, mark, ++stamp, bfs, lb, le);
+      for (int v : bfs)
+        if (g.deg[v] < g.deg[root] || (g.deg[v] == g.deg[root] && v < root)) root = v;
+    }
+    // pseudo-peripheral node: move to a minimum-degree node of the
+    // last level while the eccentricity grows
+    int lb, le;
+    int ecc = bfs_levels(g, root, mark, ++stamp, bfs, lb, le);
+    for (int round = 0; round < 8; round++) {
+      int cand = bfs[lb];
+      for (int q = lb; q < le; q++) {
+        int v = bfs[q];
+        if (g.deg[v] < g.deg[cand] || (g.deg[v] == g.deg[cand] && v < cand)) cand = v;
+      }
+      int clb, cle;
+      std::vector<int> cbfs;
+      int cecc = bfs_levels(g, cand, mark, ++stamp, cbfs, clb, cle);
+      if (cecc <= ecc) break;
+      root = cand;
+      ecc = cecc;
+      bfs.swap(cbfs);
+      lb = clb;
+      le = cle;
+    }
+    // Cuthill-McKee: neighbours in ascending degree (then index)
+    size_t head = order.size();
+    order.push_back(root);
+    placed[root] = 1;
+    while (head < order.size()) {
+      int v = order[head++];
+      nbr.clear();
+      for (int p = g.ptr[v]; p < g.ptr[v + 1]; p++) {
+        int u = g.adj[p];
+        if (!placed[u]) {
+          placed[u] = 1;
+          nbr.push_back(u);
+        }
+      }
+      std::sort(nbr.begin(), nbr.end(), [&](int x, int y) {
+        return g.deg[x] != g.deg[y] ? g.deg[x] < g.deg[y] : x < y;
+      });
+      order.insert(order.end(), nbr.begin(), nbr.end());
+    }
+  }
+  for (int q = 0; q < s; q++) out[s0 + q] = inds[s0 + order[s - 1 - q]];
+}
+
+}  // namespace
+
 void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
                std::vector<int>& out) {
   const int nb = (int)starts.size() - 1;
@@ -1007,84 +1101,19 @@ void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<i
   std::vector<std::thread> th;
   for (int t = 0; t < nth; t++)
     th.emplace_back([&, t]() {
-      std::vector<int> floc((size_t)A.nrows, -1), mark, order, bfs, nbr;
-      std::vector<char> placed;
-      LocalGraph g;
-      for (int b = t; b < nb; b += nth) {
-        const int s0 = starts[b], s = starts[b + 1] - s0;
-        if (s <= 1) continue;
-        for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
-        g.ptr.assign((size_t)s + 1, 0);
-        g.adj.clear();
-        g.deg.assign((size_t)s, 0);
-        for (int li = 0; li < s; li++) {
-          int gi = inds[s0 + li];
-          for (int p = A.ptr[gi]; p < A.ptr[gi + 1]; p++) {
-            int lj = floc[A.col[p]];
-            if (lj >= 0 && lj != li) g.adj.push_back(lj);
-          }
-          g.ptr[li + 1] = (int)g.adj.size();
-          g.deg[li] = g.ptr[li + 1] - g.ptr[li];
-        }
-        for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
-        mark.assign((size_t)s, 0);
-        placed.assign((size_t)s, 0);
-        order.clear();
-        int stamp = 0, next = 0;
-        while ((int)order.size() < s) {
-          while (placed[next]) next++;
-          // minimum-degree unplaced node of this component
-          int root = next;
-          {
-            int lb, le;
-            bfs_levels(g, next, mark, ++stamp, bfs, lb, le);
-            for (int v : bfs)
-              if (g.deg[v] < g.deg[root] || (g.deg[v] == g.deg[root] && v < root)) root = v;
-          }
-          // pseudo-peripheral node: move to a minimum-degree node of the
-          // last level while the eccentricity grows
-          int lb, le;
-          int ecc = bfs_levels(g, root, mark, ++stamp, bfs, lb, le);
-          for (int round = 0; round < 8; round++) {
-            int cand = bfs[lb];
-            for (int q = lb; q < le; q++) {
-              int v = bfs[q];
-              if (g.deg[v] < g.deg[cand] || (g.deg[v] == g.deg[cand] && v < cand)) cand = v;
-            }
-            int clb, cle;
-            std::vector<int> cbfs;
-            int cecc = bfs_levels(g, cand, mark, ++stamp, cbfs, clb, cle);
-            if (cecc <= ecc) break;
-            root = cand;
-            ecc = cecc;
-            bfs.swap(cbfs);
-            lb = clb;
-            le = cle;
-          }
-          // Cuthill-McKee: neighbours in ascending degree (then index)
-          size_t head = order.size();
-          order.push_back(root);
-          placed[root] = 1;
-          while (head < order.size()) {
-            int v = order[head++];
-            nbr.clear();
-            for (int p = g.ptr[v]; p < g.ptr[v + 1]; p++) {
-              int u = g.adj[p];
-              if (!placed[u]) {
-                placed[u] = 1;
-                nbr.push_back(u);
-              }
-            }
-            std::sort(nbr.begin(), nbr.end(), [&](int x, int y) {
-              return g.deg[x] != g.deg[y] ? g.deg[x] < g.deg[y] : x < y;
-            });
-            order.insert(order.end(), nbr.begin(), nbr.end());
-          }
-        }
-        for (int q = 0; q < s; q++) out[s0 + q] = inds[s0 + order[s - 1 - q]];
-      }
+      std::vector<int> floc((size_t)A.nrows, -1);
+      RcmBuf buf;
+      for (int b = t; b < nb; b += nth) rcm_block(A, inds, starts[b], starts[b + 1] - starts[b], out, floc, buf);
     });
   for (auto& x : th) x.join();
+}
+
+void block_rcm_serial(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
+                      std::vector<int>& out, std::vector<int>& floc) {
+  out.assign(inds.begin(), inds.end());
+  RcmBuf buf;
+  for (int b = 0; b + 1 < (int)starts.size(); b++)
+    rcm_block(A, inds, starts[b], starts[b + 1] - starts[b], out, floc, buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1343,29 +1372,10 @@ struct RankScratch {
   explicit RankScratch(int n) : layer((size_t)n, -1), stamp((size_t)n, 0), label((size_t)n, 0), gl((size_t)n, -1) {}
 };
 
-// Labels: bit 0 inner overlap of some neighbour, bit 1 sent to some neighbour.
-void rank_local(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn, int r,
-                const std::vector<int>& own, int ol, RankScratch& S, RankLocal& out) {
-  const int L = std::max(ol, 1), D = 2 * L;
-  std::vector<int> touched(own.begin(), own.end());
-  for (int v : own) S.layer[v] = 0;
-  // layers 1..D of outside nodes (build_ghost's first front)
-  std::vector<int> front, frontier(own.begin(), own.end()), next;
-  for (int l = 1; l <= D && !frontier.empty(); l++) {
-    next.clear();
-    for (int v : frontier)
-      for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
-        const int w = As.col[p];
-        if (S.layer[w] < 0) {
-          S.layer[w] = l;
-          next.push_back(w);
-          touched.push_back(w);
-        }
-      }
-    front.insert(front.end(), next.begin(), next.end());
-    frontier.swap(next);
-  }
-  // per neighbour rank: own nodes within L steps of its nodes on the front
+// Labels of own nodes, per neighbour rank: within L steps of that rank's
+// nodes on the front (inner overlap), exactly L steps away (sent).
+void rank_labels(const Csr& As, const std::vector<int>& eptn, const std::vector<int>& front, int L,
+                 RankScratch& S, std::vector<int>& frontier, std::vector<int>& next) {
   std::vector<int> nbr;
   for (int w : front) nbr.push_back(eptn[w]);
   std::sort(nbr.begin(), nbr.end());
@@ -1388,6 +1398,41 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
         }
       frontier.swap(next);
     }
+  }
+}
+
+// Labels: bit 0 inner overlap of some neighbour, bit 1 sent to some neighbour.
+void rank_local(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn, int r,
+                const std::vector<int>& own, int ol, RankScratch& S, RankLocal& out) {
+  const int L = std::max(ol, 1), D = 2 * L;
+  std::vector<int> touched(own.begin(), own.end());
+  for (int v : own) S.layer[v] = 0;
+  // layers 1..D of outside nodes (build_ghost's first front)
+  std::vector<int> front, frontier(own.begin(), own.end()), next;
+  for (int l = 1; l <= D && !frontier.empty(); l++) {
+    next.clear();
+    for (int v : frontier)
+      for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
+        const int w = As.col[p];
+        if (S.layer[w] < 0) {
+          S.layer[w] = l;
+          next.push_back(w);
+          touched.push_back(w);
+        }
+      }
+    front.insert(front.end(), next.begin(), next.end());
+    frontier.swap(next);
+  }
+  if (L == 1) {
+    // one step: an own node is sent (and in the inner overlap) when some
+    // front node's row holds it, whichever neighbour that node belongs to
+    for (int w : front)
+      for (int p = As.ptr[w]; p < As.ptr[w + 1]; p++) {
+        const int u = As.col[p];
+        if (S.layer[u] == 0) S.label[u] = 3;
+      }
+  } else {
+    rank_labels(As, eptn, front, L, S, frontier, next);
   }
   (void)r;
   // numbering: [sent | inner overlap | inner | outer overlap | received]
@@ -1453,13 +1498,13 @@ void list_aggregates(int nagr, const std::vector<int>& num, Aggregates& a) {
 void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
                    int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
-                   RankSetup& out) {
+                   RankSetup& out, const Csr* rcm_graph) {
   const int n = As.nrows;
   std::vector<std::vector<int>> own((size_t)nranks);
   for (int i = 0; i < n; i++) own[eptn[i]].push_back(i);
   out.ranks.assign((size_t)nranks, RankLocal{});
   std::vector<Aggregates> inner((size_t)nranks), full((size_t)nranks);
-  std::vector<std::vector<int>> sst((size_t)nranks), sin((size_t)nranks);
+  std::vector<std::vector<int>> sst((size_t)nranks), sin((size_t)nranks), dsin((size_t)nranks);
   const int nparts = std::max(nsub_per_rank, 1);
   // the ranks are independent processes in the reference: one worker each
   const int nw = std::max(1, std::min(nranks, setup_threads()));
@@ -1509,6 +1554,8 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
           if (pst[q + 1] > pst[q]) ps2.push_back(pst[q + 1]);
         add_layers_all(R.A, ps2, pnd, ol_sub, sst[r], sin[r]);
         for (int& v : sin[r]) v = R.lg[v];
+        if (rcm_graph) block_rcm_serial(*rcm_graph, sst[r], sin[r], dsin[r], S.gl);
+        R.A = Csr();  // only the numbering is kept
       }
       set_setup_thread_cap(0);
     });
@@ -1518,6 +1565,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
   int off_in = 0, off_full = 0;
   out.sstarts.assign(1, 0);
   out.sinds.clear();
+  out.dinds.clear();
   for (int r = 0; r < nranks; r++) {
     const RankLocal& R = out.ranks[r];
     for (int i = 0; i < R.ninner; i++) {
@@ -1529,6 +1577,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
     off_full += full[r].nagr;
     for (size_t q = 1; q < sst[r].size(); q++) out.sstarts.push_back((int)out.sinds.size() + sst[r][q]);
     out.sinds.insert(out.sinds.end(), sin[r].begin(), sin[r].end());
+    if (rcm_graph) out.dinds.insert(out.dinds.end(), dsin[r].begin(), dsin[r].end());
   }
   list_aggregates(off_in, num_in, out.inner);
   list_aggregates(off_full, num_full, out.full);

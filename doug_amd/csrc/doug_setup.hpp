@@ -113,6 +113,10 @@ void set_setup_thread_cap(int cap);
 // its input.  Independent of the thread count.
 void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
                std::vector<int>& out);
+// block_rcm on the calling thread with the caller's scratch `floc`
+// (A.nrows entries, all -1 on entry and on exit).
+void block_rcm_serial(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
+                      std::vector<int>& out, std::vector<int>& floc);
 
 // Project graph partitioner (stands in for METIS): recursive bisection of a
 // symmetric adjacency (0-based CSR, no self loops) into nparts parts of
@@ -150,18 +154,21 @@ struct RankLocal {
 // (Preconditioner_base.F90:96-123).  Aggregates are numbered rank by rank
 // (CoarseAllgathers.F90:531-540); subdomains are listed rank by rank.
 struct RankSetup {
-  std::vector<RankLocal> ranks;
+  std::vector<RankLocal> ranks;       // numbering only: each local matrix is released after use
   Aggregates inner, full;             // global numbering
   std::vector<int> sstarts, sinds;    // subdomains with their layers, global ids
+  std::vector<int> dinds;             // sinds, each subdomain in block_rcm order (if asked)
 };
 
 // `As`: the assembled matrix pattern with each row sorted by column
 // (stable); `val`: its values as read, in As's entry order.  `eptn[i]` in
 // [0, nranks).  `ol_dist` = max(overlap, smoothers) >= 1; `ol_sub` <= ol_dist.
+// With `rcm_graph`, each rank also orders its subdomains by block_rcm over
+// that graph (out.dinds), inside the rank's worker.
 void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
                    int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
-                   RankSetup& out);
+                   RankSetup& out, const Csr* rcm_graph = nullptr);
 
 // Stable per-row sort by column (SpMtx_arrange with sort, SpMtx_arrangement.F90:159-227),
 // carrying `val`.
