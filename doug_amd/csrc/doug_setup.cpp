@@ -1366,10 +1366,19 @@ bool rows_sorted_by_column(const Csr& A) {
 
 namespace {
 
+// Per-node state of one rank's pass, packed so that a lookup touches one
+// cache line: layer (-1 off the rank's local matrix), labels (bit 0 inner
+// overlap of some neighbour, bit 1 sent to some neighbour), local number,
+// visit stamp.
+struct RankNode {
+  int layer = -1, label = 0, gl = -1, stamp = 0;
+};
+
 struct RankScratch {
-  std::vector<int> layer, stamp, label, gl;
+  std::vector<RankNode> nd;
+  std::vector<int> floc;  // -1 between uses (block_rcm_serial scratch)
   int tick = 0;
-  explicit RankScratch(int n) : layer((size_t)n, -1), stamp((size_t)n, 0), label((size_t)n, 0), gl((size_t)n, -1) {}
+  explicit RankScratch(int n) : nd((size_t)n), floc((size_t)n, -1) {}
 };
 
 // Labels of own nodes, per neighbour rank: within L steps of that rank's
@@ -1385,16 +1394,16 @@ void rank_labels(const Csr& As, const std::vector<int>& eptn, const std::vector<
   for (size_t k = 0; k < nbr.size(); k++) {
     const int t = ++S.tick;
     frontier = by_rank[k];
-    for (int w : frontier) S.stamp[w] = t;
+    for (int w : frontier) S.nd[w].stamp = t;
     for (int d = 1; d <= L && !frontier.empty(); d++) {
       next.clear();
       for (int v : frontier)
         for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
           const int w = As.col[p];
-          if (S.layer[w] < 0 || S.stamp[w] == t) continue;  // off the rank's local matrix, or seen
-          S.stamp[w] = t;
+          if (S.nd[w].layer < 0 || S.nd[w].stamp == t) continue;  // off the rank's local matrix, or seen
+          S.nd[w].stamp = t;
           next.push_back(w);
-          if (S.layer[w] == 0) S.label[w] |= (d == L) ? 3 : 1;
+          if (S.nd[w].layer == 0) S.nd[w].label |= (d == L) ? 3 : 1;
         }
       frontier.swap(next);
     }
@@ -1406,7 +1415,7 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
                 const std::vector<int>& own, int ol, RankScratch& S, RankLocal& out) {
   const int L = std::max(ol, 1), D = 2 * L;
   std::vector<int> touched(own.begin(), own.end());
-  for (int v : own) S.layer[v] = 0;
+  for (int v : own) S.nd[v].layer = 0;
   // layers 1..D of outside nodes (build_ghost's first front)
   std::vector<int> front, frontier(own.begin(), own.end()), next;
   for (int l = 1; l <= D && !frontier.empty(); l++) {
@@ -1414,8 +1423,8 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
     for (int v : frontier)
       for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
         const int w = As.col[p];
-        if (S.layer[w] < 0) {
-          S.layer[w] = l;
+        if (S.nd[w].layer < 0) {
+          S.nd[w].layer = l;
           next.push_back(w);
           touched.push_back(w);
         }
@@ -1429,7 +1438,7 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
     for (int w : front)
       for (int p = As.ptr[w]; p < As.ptr[w + 1]; p++) {
         const int u = As.col[p];
-        if (S.layer[u] == 0) S.label[u] = 3;
+        if (S.nd[u].layer == 0) S.nd[u].label = 3;
       }
   } else {
     rank_labels(As, eptn, front, L, S, frontier, next);
@@ -1437,16 +1446,16 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
   (void)r;
   // numbering: [sent | inner overlap | inner | outer overlap | received]
   std::vector<int> grp[5];
-  for (int v : own) grp[(S.label[v] & 2) ? 0 : (S.label[v] & 1) ? 1 : 2].push_back(v);
+  for (int v : own) grp[(S.nd[v].label & 2) ? 0 : (S.nd[v].label & 1) ? 1 : 2].push_back(v);
   for (int w : front)
-    if (S.layer[w] <= L) grp[S.layer[w] == L ? 4 : 3].push_back(w);
+    if (S.nd[w].layer <= L) grp[S.nd[w].layer == L ? 4 : 3].push_back(w);
   std::sort(grp[3].begin(), grp[3].end());
   std::sort(grp[4].begin(), grp[4].end());
   out.lg.clear();
   for (auto& g : grp) out.lg.insert(out.lg.end(), g.begin(), g.end());
   out.ninner = (int)own.size();
   const int nl = (int)out.lg.size();
-  for (int k = 0; k < nl; k++) S.gl[out.lg[k]] = k;
+  for (int k = 0; k < nl; k++) S.nd[out.lg[k]].gl = k;
   // local matrix: own rows whole; rows of outside nodes nearer than L keep
   // the columns of own nodes and of outside nodes up to layer L; the outer
   // layer's rows are the ghost matrix, not part of it
@@ -1457,41 +1466,28 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
   A.val.clear();
   for (int k = 0; k < nl; k++) {
     const int v = out.lg[k];
-    if (S.layer[v] < L) {
-      const bool ownrow = S.layer[v] == 0;
+    if (S.nd[v].layer < L) {
+      const bool ownrow = S.nd[v].layer == 0;
       for (int pass = 0; pass < 2; pass++)
         for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
           const int w = As.col[p];
-          const int lw = S.layer[w];
+          const int lw = S.nd[w].layer;
           if (lw < 0 || (!ownrow && lw > L)) continue;
-          const bool sent = lw == 0 && (S.label[w] & 2);
+          const bool sent = lw == 0 && (S.nd[w].label & 2);
           if (sent != (pass == 0)) continue;
-          A.col.push_back(S.gl[w]);
+          A.col.push_back(S.nd[w].gl);
           A.val.push_back(val[p]);
         }
     }
     A.ptr[k + 1] = (int)A.col.size();
   }
   for (int v : touched) {
-    S.layer[v] = -1;
-    S.label[v] = 0;
-    S.gl[v] = -1;
+    S.nd[v].layer = -1;
+    S.nd[v].label = 0;
+    S.nd[v].gl = -1;
   }
 }
 
-// Aggregates listed per aggregate in ascending node order from `num`.
-void list_aggregates(int nagr, const std::vector<int>& num, Aggregates& a) {
-  a.nagr = nagr;
-  a.num = num;
-  a.starts.assign((size_t)nagr + 1, 0);
-  for (int x : num)
-    if (x > 0) a.starts[x]++;
-  for (int c = 0; c < nagr; c++) a.starts[c + 1] += a.starts[c];
-  a.nodes.assign((size_t)a.starts[nagr], 0);
-  std::vector<int> fill(a.starts.begin(), a.starts.end() - 1);
-  for (int i = 0; i < (int)num.size(); i++)
-    if (num[i] > 0) a.nodes[fill[num[i] - 1]++] = i;
-}
 
 }  // namespace
 
@@ -1554,33 +1550,85 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
           if (pst[q + 1] > pst[q]) ps2.push_back(pst[q + 1]);
         add_layers_all(R.A, ps2, pnd, ol_sub, sst[r], sin[r]);
         for (int& v : sin[r]) v = R.lg[v];
-        if (rcm_graph) block_rcm_serial(*rcm_graph, sst[r], sin[r], dsin[r], S.gl);
+        if (rcm_graph) block_rcm_serial(*rcm_graph, sst[r], sin[r], dsin[r], S.floc);
         R.A = Csr();  // only the numbering is kept
       }
       set_setup_thread_cap(0);
     });
   for (auto& x : th) x.join();
-  // rank-major global numbering of aggregates and subdomains
-  std::vector<int> num_in((size_t)n, 0), num_full((size_t)n, 0);
-  int off_in = 0, off_full = 0;
-  out.sstarts.assign(1, 0);
-  out.sinds.clear();
-  out.dinds.clear();
+  // rank-major global numbering of aggregates and subdomains: offsets by a
+  // scan over the ranks, then every rank fills its own slices in parallel
+  std::vector<int> off_in((size_t)nranks + 1, 0), off_full((size_t)nranks + 1, 0);
+  std::vector<int> off_sub((size_t)nranks + 1, 0), off_ent((size_t)nranks + 1, 0);
   for (int r = 0; r < nranks; r++) {
-    const RankLocal& R = out.ranks[r];
-    for (int i = 0; i < R.ninner; i++) {
-      const int v = R.lg[i];
-      if (inner[r].num[i] > 0) num_in[v] = inner[r].num[i] + off_in;
-      num_full[v] = full[r].num[i] + off_full;
-    }
-    off_in += inner[r].nagr;
-    off_full += full[r].nagr;
-    for (size_t q = 1; q < sst[r].size(); q++) out.sstarts.push_back((int)out.sinds.size() + sst[r][q]);
-    out.sinds.insert(out.sinds.end(), sin[r].begin(), sin[r].end());
-    if (rcm_graph) out.dinds.insert(out.dinds.end(), dsin[r].begin(), dsin[r].end());
+    off_in[r + 1] = off_in[r] + inner[r].nagr;
+    off_full[r + 1] = off_full[r] + full[r].nagr;
+    off_sub[r + 1] = off_sub[r] + (int)sst[r].size() - 1;
+    off_ent[r + 1] = off_ent[r] + (int)sin[r].size();
   }
-  list_aggregates(off_in, num_in, out.inner);
-  list_aggregates(off_full, num_full, out.full);
+  Aggregates* lists[2] = {&out.inner, &out.full};
+  const int nagr_tot[2] = {off_in[nranks], off_full[nranks]};
+  for (int w = 0; w < 2; w++) {
+    lists[w]->nagr = nagr_tot[w];
+    lists[w]->num.assign((size_t)n, 0);
+    lists[w]->starts.assign((size_t)nagr_tot[w] + 1, 0);
+    lists[w]->nodes.assign((size_t)n, 0);
+  }
+  out.sstarts.assign((size_t)off_sub[nranks] + 1, 0);
+  out.sinds.assign((size_t)off_ent[nranks], 0);
+  out.dinds.assign(rcm_graph ? (size_t)off_ent[nranks] : 0, 0);
+  // first listed node of each rank in either list
+  std::vector<int> node_off[2] = {std::vector<int>((size_t)nranks + 1, 0),
+                                  std::vector<int>((size_t)nranks + 1, 0)};
+  for (int r = 0; r < nranks; r++)
+    for (int w = 0; w < 2; w++) {
+      const Aggregates& a = w == 0 ? inner[r] : full[r];
+      int c = 0;
+      for (int i = 0; i < out.ranks[r].ninner; i++) c += a.num[i] > 0;
+      node_off[w][r + 1] = node_off[w][r] + c;
+    }
+  std::vector<std::thread> th2;
+  for (int t = 0; t < nw; t++)
+    th2.emplace_back([&, t]() {
+      std::vector<int> cnt;
+      for (int r = t; r < nranks; r += nw) {
+        const RankLocal& R = out.ranks[r];
+        // own nodes ascend in global number; listed per aggregate in that order
+        // (Form_Aggr lists each aggregate's nodes in node order)
+        for (int w = 0; w < 2; w++) {
+          const Aggregates& a = w == 0 ? inner[r] : full[r];
+          Aggregates& g = *lists[w];
+          const int off = w == 0 ? off_in[r] : off_full[r];
+          cnt.assign((size_t)a.nagr + 1, 0);
+          std::vector<std::pair<int, int>> vs;
+          vs.reserve((size_t)R.ninner);
+          for (int i = 0; i < R.ninner; i++)
+            if (a.num[i] > 0) vs.emplace_back(R.lg[i], a.num[i]);
+          std::sort(vs.begin(), vs.end());
+          for (auto& q : vs) cnt[q.second]++;
+          int base = node_off[w][r];
+          for (int c = 1; c <= a.nagr; c++) {
+            g.starts[off + c - 1] = base;
+            base += cnt[c];
+            cnt[c] = g.starts[off + c - 1];
+          }
+          for (auto& q : vs) {
+            g.num[q.first] = q.second + off;
+            g.nodes[cnt[q.second]++] = q.first;
+          }
+        }
+        for (size_t q = 1; q < sst[r].size(); q++) out.sstarts[off_sub[r] + q] = off_ent[r] + sst[r][q];
+        std::copy(sin[r].begin(), sin[r].end(), out.sinds.begin() + off_ent[r]);
+        if (rcm_graph) std::copy(dsin[r].begin(), dsin[r].end(), out.dinds.begin() + off_ent[r]);
+      }
+    });
+  for (auto& x : th2) x.join();
+  for (int w = 0; w < 2; w++) {
+    Aggregates& g = *lists[w];
+    const int tot = node_off[w][nranks];
+    g.nodes.resize((size_t)tot);
+    g.starts[g.nagr] = tot;
+  }
 }
 
 }  // namespace doug

@@ -500,27 +500,39 @@ def test_sgs_long_rows():
         assert _bits_equal(s.prec(r), o.prec(r))
 
 
-@pytest.mark.parametrize("case", ["cfg0-4-ranks", "3d-8-ranks-overlap2"])
+@pytest.mark.parametrize("case", ["cfg0-4-ranks", "3d-8-ranks-overlap2", "3d-box-ranks"])
 def test_virtual_ranks_against_oracle(case):
     # DOUG's decomposition over several processes (rank-local numbering,
     # aggregation and subdomains) on one GPU against the oracle's restatement
     if case == "cfg0-4-ranks":
         n, i, j, v = P.laplace2d(128)
         nr, ctl = 4, dict(CFG0_CTL, radius1=-2)
-    else:
+    elif case == "3d-8-ranks-overlap2":
         n, i, j, v = P.diffusion3d(24, 1.0, 5)
         nr, ctl = 8, dict(CFG0_CTL, radius1=-2, overlap=2, minasize1=20, maxasize1=60)
+    else:
+        # the benchmarked set-up at 32^3: one process per 16^3 box, one
+        # subdomain each, the 128^3 workload's control values
+        n, i, j, v = P.diffusion3d(32, 1.0, 1)
+        nr, ctl = 8, dict(radius1=4, minasize1=100, maxasize1=343, strong1=0.08, smoothers=1,
+                          overlap=1, symmstruct=1, symmnumeric=1)
     b = P.uniform_rhs(n, 2024)
     print(f"virtual ranks {case} n={n}", flush=True)
-    with D.Solver(-1) as h:  # the rank map of the project partitioner
-        h.set_params(**ctl)
-        h.set_matrix(n, i, j, v, 1)
-        h.set_ranks(nr)
-        h.setup_host()
+    if case == "3d-box-ranks":
+        st, nd = P.block_partition((32, 32, 32), 16)
         rank = np.zeros(n, np.int32)
         for r in range(nr):
-            ni, lg = h.rank_map(r)
-            rank[lg[:ni] - 1] = r + 1
+            rank[nd[st[r] - 1:st[r + 1] - 1] - 1] = r + 1
+    else:
+        with D.Solver(-1) as h:  # the rank map of the project partitioner
+            h.set_params(**ctl)
+            h.set_matrix(n, i, j, v, 1)
+            h.set_ranks(nr)
+            h.setup_host()
+            rank = np.zeros(n, np.int32)
+            for r in range(nr):
+                ni, lg = h.rank_map(r)
+                rank[lg[:ni] - 1] = r + 1
     o = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank)
     with D.Solver(0) as s:
         s.set_params(**ctl)
