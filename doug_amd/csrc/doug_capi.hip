@@ -1300,8 +1300,6 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const int n = A.nrows;
   double t3 = now();
   int rc;
-  if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
-  HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
   // device order of each subdomain's unknowns (the reported index sets keep
   // the reference's order): reverse Cuthill-McKee narrows the factor's
   // tile envelope
@@ -1470,6 +1468,9 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   std::vector<double>& scaled = ctx->scaled;
   std::vector<uint8_t>& strong = ctx->strong;
+  // non-positive pivot flags: [0] subdomains, [1] coarse matrix
+  if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
+  HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
   phase_values(ctx, scaled, strong);
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
@@ -1477,6 +1478,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     phase_coarse_host(ctx, strong);
     return DOUG_OK;
   };
+  bool coarse_done = false;
   if (ctx->have_partition) {
     // neither the matrix upload nor the subdomains depend on the
     // aggregation: a second host thread aggregates while this one uploads
@@ -1503,12 +1505,16 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     agg.join();
     if (rc) return rc;
     if ((rc = coarse_space())) return rc;
+    // the coarse factorisation runs on the GPU while the host prepares the
+    // subdomains (same stream: the factorisations still follow each other)
+    if ((rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+    coarse_done = true;
     phase_subdomains(ctx);
     if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;
   }
   fill_info(ctx);
   ctx->host_ready = true;
-  if ((rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+  if (!coarse_done && (rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
   int flag[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(flag, ctx->d_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -738,8 +738,18 @@ __global__ __launch_bounds__(256) void k_dot(const double* __restrict__ a,
                                              int mode, double* scal) {
   __shared__ double sh[4];
   double v = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    v = v + a[i] * b[i];
+  const int G = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // four strides loaded ahead, added in the same order as one at a time
+  for (; i + 3 * G < n; i += 4 * G) {
+    const double a0 = a[i], b0 = b[i], a1 = a[i + G], b1 = b[i + G];
+    const double a2 = a[i + 2 * G], b2 = b[i + 2 * G], a3 = a[i + 3 * G], b3 = b[i + 3 * G];
+    v = v + a0 * b0;
+    v = v + a1 * b1;
+    v = v + a2 * b2;
+    v = v + a3 * b3;
+  }
+  for (; i < n; i += G) v = v + a[i] * b[i];
   double s = block_sum(v, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
   __syncthreads();
@@ -764,7 +774,21 @@ __global__ __launch_bounds__(256) void k_xr_update(int n, double* __restrict__ x
   __shared__ double sh[4];
   const double alpha = scal[S_ALPHA];
   double v = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const int G = gridDim.x * blockDim.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // two strides loaded ahead, the sum in the same order as one at a time
+  for (; i + G < n; i += 2 * G) {
+    const double x0 = x[i], p0 = p[i], r0 = r[i], q0 = q[i];
+    const double x1 = x[i + G], p1 = p[i + G], r1 = r[i + G], q1 = q[i + G];
+    x[i] = x0 + alpha * p0;
+    x[i + G] = x1 + alpha * p1;
+    const double ri0 = r0 - alpha * q0, ri1 = r1 - alpha * q1;
+    r[i] = ri0;
+    r[i + G] = ri1;
+    v = v + ri0 * ri0;
+    v = v + ri1 * ri1;
+  }
+  for (; i < n; i += G) {
     x[i] = x[i] + alpha * p[i];
     double ri = r[i] - alpha * q[i];
     r[i] = ri;

@@ -594,3 +594,39 @@ def test_duplicate_entry_inside_a_subdomain_is_refused():
         s.set_partition(*P.block_partition((16, 16), 8))
         with pytest.raises(RuntimeError, match="duplicate"):
             s.setup(0)
+
+
+@pytest.mark.parametrize("variant", [
+    dict(),                        # the benchmarked options
+    dict(levels=1),                # one level
+    dict(fine_method=2),           # symmetric Gauss-Seidel on each rank's subdomain
+    dict(smoothers=0),             # unsmoothed aggregates
+    dict(smoothers=2, radius1=4),  # R = T (S2 S)^T
+    dict(overlap=2),               # two layers shipped and added
+])
+def test_rank_variants_against_oracle(variant):
+    # the benchmarked set-up (one process per box, one subdomain each) under
+    # the options on the path
+    n, i, j, v = P.diffusion3d(16, 1.0, 23)
+    st, nd = P.block_partition((16,) * 3, 8)
+    rank = np.zeros(n, np.int32)
+    for r in range(len(st) - 1):
+        rank[nd[st[r] - 1:st[r + 1] - 1] - 1] = r + 1
+    b = P.uniform_rhs(n, 24)
+    ctl = dict(CFG0_CTL, **variant)
+    print(f"rank variant {variant}", flush=True)
+    o = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank)
+    with D.Solver(0) as s:
+        s.set_params(**ctl)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_ranks(8, rank)
+        s.setup(0)
+        for a, c in zip(s.subdomains(), o.subdomains()):
+            np.testing.assert_array_equal(a, c)
+        r = np.random.default_rng(25).uniform(-1, 1, n)
+        z, zo = s.prec(r), o.prec(r)
+        assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+        x, it, cond, _ = s.pcg(b, 1e-8, 2000)
+        xo, ito, condo, _ = o.pcg(b, 1e-8, 2000)
+        assert it == ito, (it, ito)
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
