@@ -69,7 +69,7 @@ def cpu_baseline(size, block, sigma, seed, decomp):
     import oracle as O
     from doug_amd import problems as P
 
-    m = 64
+    m = 80  # 125 boxes of 16^3: 10-20 s of one core
     n, i, j, v = P.diffusion3d(m, sigma, seed)
     b = P.uniform_rhs(n, seed + 1)
     t0 = time.perf_counter()

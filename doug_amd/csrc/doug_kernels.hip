@@ -645,8 +645,19 @@ __global__ __launch_bounds__(TH) void k_spmv(const int32_t* __restrict__ rowblk,
     __syncthreads();
     const int row = r0 + t;
     if (row < r1) {
+      // in stored order; unrolled so that the LDS reads of long rows (the
+      // restriction's) are issued ahead of the dependent additions
       double acc = 0.0;
-      for (int e = ptr[row]; e < ptr[row + 1]; e++) acc = acc + prod[e - e0];
+      const int eb = ptr[row] - e0, ee = ptr[row + 1] - e0;
+      int e = eb;
+      for (; e + 8 <= ee; e += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = prod[e + q];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc = acc + v[q];
+      }
+      for (; e < ee; e++) acc = acc + prod[e];
       y[row] = acc;
     }
   } else {
