@@ -55,6 +55,7 @@ struct doug_ctx {
   std::vector<int> eptn;
   int ol_dist = 0;
   RankSetup rsetup;
+  RankWorkspaces rank_ws;  // the rank workers' scratch, reused by later setups
   ProcessState ps;
   Aggregates fin, ffull, cin, cfull;
   Csr R, Rt, AC;
@@ -349,6 +350,7 @@ int doug_set_ranks(doug_ctx* ctx, int32_t nranks, const int32_t* rank_of_node, i
     ctx->nranks = 0;
     ctx->eptn.clear();
     ctx->rsetup = RankSetup{};
+    ctx->rank_ws.clear();
     ctx->ready = ctx->host_ready = false;
     return DOUG_OK;
   }
@@ -747,17 +749,22 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
   if (ctx->nranks > 0) {
     // each rank aggregates its own rows of the values as read
-    // the ranks also put their subdomains in the factor order of the device
-    // copy (reverse Cuthill-McKee over A's pattern), off the critical path
-    const Csr* rcm = (p.fine_method == 1 && ctx->local_order == 1 && ctx->device >= 0) ? &ctx->A : nullptr;
+    // with at least one rank per worker thread the ranks also put their
+    // subdomains in the factor order of the device copy (reverse
+    // Cuthill-McKee over A's pattern); with fewer ranks the threaded
+    // block_rcm of the subdomain preparation keeps every thread busy
+    const Csr* rcm = (p.fine_method == 1 && ctx->local_order == 1 && ctx->device >= 0 &&
+                      ctx->nranks >= setup_threads())
+                         ? &ctx->A
+                         : nullptr;
     if (rows_sorted_by_column(ctx->A)) {
       virtual_ranks(ctx->A, ctx->A_input_val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol,
                     ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup,
-                    rcm);
+                    rcm, &ctx->rank_ws);
     } else {
       Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
       virtual_ranks(As, As.val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains,
-                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup, rcm);
+                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup, rcm, &ctx->rank_ws);
     }
     ctx->fin = std::move(ctx->rsetup.inner);
     ctx->ffull = std::move(ctx->rsetup.full);

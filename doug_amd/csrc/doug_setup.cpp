@@ -1364,8 +1364,6 @@ bool rows_sorted_by_column(const Csr& A) {
   return true;
 }
 
-namespace {
-
 // Per-node state of one rank's pass, packed so that a lookup touches one
 // cache line: layer (-1 off the rank's local matrix), labels (bit 0 inner
 // overlap of some neighbour, bit 1 sent to some neighbour), local number,
@@ -1375,11 +1373,20 @@ struct RankNode {
 };
 
 struct RankScratch {
+  int n = 0;
   std::vector<RankNode> nd;
   std::vector<int> floc;  // -1 between uses (block_rcm_serial scratch)
   int tick = 0;
-  explicit RankScratch(int n) : nd((size_t)n), floc((size_t)n, -1) {}
+  explicit RankScratch(int n_) : n(n_), nd((size_t)n_), floc((size_t)n_, -1) {}
 };
+
+RankWorkspaces::RankWorkspaces() = default;
+RankWorkspaces::~RankWorkspaces() = default;
+void RankWorkspaces::clear() { w.clear(); }
+
+namespace {
+
+
 
 // Labels of own nodes, per neighbour rank: within L steps of that rank's
 // nodes on the front (inner overlap), exactly L steps away (sent).
@@ -1494,7 +1501,7 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
 void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
                    int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
-                   RankSetup& out, const Csr* rcm_graph) {
+                   RankSetup& out, const Csr* rcm_graph, RankWorkspaces* ws) {
   const int n = As.nrows;
   std::vector<std::vector<int>> own((size_t)nranks);
   for (int i = 0; i < n; i++) own[eptn[i]].push_back(i);
@@ -1504,11 +1511,21 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
   const int nparts = std::max(nsub_per_rank, 1);
   // the ranks are independent processes in the reference: one worker each
   const int nw = std::max(1, std::min(nranks, setup_threads()));
+  if (ws) {  // scratch of earlier calls is reused when it fits this matrix
+    if ((int)ws->w.size() < nw) ws->w.resize((size_t)nw);
+    std::vector<std::thread> mk;
+    for (int t = 0; t < nw; t++)
+      if (!ws->w[t] || ws->w[t]->n != n)
+        mk.emplace_back([ws, t, n]() { ws->w[t] = std::make_unique<RankScratch>(n); });
+    for (auto& x : mk) x.join();
+  }
   std::vector<std::thread> th;
   for (int t = 0; t < nw; t++)
     th.emplace_back([&, t]() {
       set_setup_thread_cap(1);
-      RankScratch S(n);
+      std::unique_ptr<RankScratch> mine;
+      RankScratch* sp = ws ? ws->w[t].get() : (mine = std::make_unique<RankScratch>(n)).get();
+      RankScratch& S = *sp;
       for (int r = t; r < nranks; r += nw) {
         RankLocal& R = out.ranks[r];
         rank_local(As, val, eptn, r, own[r], ol_dist, S, R);

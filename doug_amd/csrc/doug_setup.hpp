@@ -6,6 +6,7 @@
 #pragma once
 #include <climits>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -163,12 +164,22 @@ struct RankSetup {
 // `As`: the assembled matrix pattern with each row sorted by column
 // (stable); `val`: its values as read, in As's entry order.  `eptn[i]` in
 // [0, nranks).  `ol_dist` = max(overlap, smoothers) >= 1; `ol_sub` <= ol_dist.
+// Per-worker scratch of virtual_ranks (node-indexed arrays of the whole
+// matrix), kept by the caller so that later set-ups reuse it.
+struct RankScratch;
+struct RankWorkspaces {
+  std::vector<std::unique_ptr<RankScratch>> w;
+  RankWorkspaces();
+  ~RankWorkspaces();
+  void clear();
+};
+
 // With `rcm_graph`, each rank also orders its subdomains by block_rcm over
 // that graph (out.dinds), inside the rank's worker.
 void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
                    int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
-                   RankSetup& out, const Csr* rcm_graph = nullptr);
+                   RankSetup& out, const Csr* rcm_graph = nullptr, RankWorkspaces* ws = nullptr);
 
 // Stable per-row sort by column (SpMtx_arrange with sort, SpMtx_arrangement.F90:159-227),
 // carrying `val`.

@@ -245,3 +245,22 @@ def test_rank_errors():
         s.set_partition(st, np.arange(1, n + 1, dtype=np.int32))
         with pytest.raises(RuntimeError, match="clear the partition"):
             s.setup_host()
+
+
+def test_repeated_setups_reuse_scratch_cleanly():
+    # the rank workers' scratch is kept by the context between set-ups:
+    # later set-ups (other options, other rank maps) equal fresh ones
+    n, i, j, v, rank, nr, ctl = _case("2d-field-overlap2")
+    runs = [(dict(ctl, overlap=1), rank, nr), (dict(ctl, overlap=2), rank, nr),
+            (dict(ctl, overlap=1), ((np.arange(n) // 11) % 5 + 1).astype(np.int32), 5)]
+    with D.Solver(-1) as s:
+        s.set_matrix(n, i, j, v, 1)
+        for c, rk, k in runs:
+            s.set_params(**c)
+            s.set_ranks(k, rk)
+            s.setup_host()
+            with _host(n, i, j, v, k, rk, **c) as fresh:
+                for a, b in zip(s.fine_aggregates(), fresh.fine_aggregates()):
+                    np.testing.assert_array_equal(a, b)
+                for a, b in zip(s.subdomains(), fresh.subdomains()):
+                    np.testing.assert_array_equal(a, b)
