@@ -1049,10 +1049,13 @@ static int check_params(doug_ctx* ctx) {
   // method 5 ("multiplicative on the fine level, additive with the coarse")
   // is method 1 with one process: the coarse correction is restricted from
   // the same residual and added to the same sum (pcg.F90:145-150,205-208).
-  // Methods 2-4 restrict a residual vector that the one-process preconditioner
-  // never computes (pcg.F90:146-147,253-255), so they have no defined result.
-  if (p.method != 0 && p.method != 1 && p.method != 5)
-    return fail(ctx, DOUG_E_UNSUPPORTED, "method must be 0 (none), 1 or 5 (additive Schwarz)");
+  // With one process methods 2-4 restrict a residual vector that the
+  // preconditioner never computes (pcg.F90:146-147,253-255), so they have no
+  // defined result; with several processes the restriction takes r
+  // (pcg.F90:130-141) and they are defined (enqueue_prec).
+  if (p.method < 0 || p.method > 5 || (ctx->nranks <= 0 && p.method >= 2 && p.method <= 4))
+    return fail(ctx, DOUG_E_UNSUPPORTED,
+                "method must be 0 (none), 1 or 5 (additive Schwarz); 2-4 need virtual ranks");
   if (ctx->nranks > 0) {
     // several processes: a negative radius1 selects the graph partition of
     // the matrix and is negated (Distribution_assm.F90:127-134); the
@@ -1637,19 +1640,24 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
     launch_copy(z, r, ctx->n, ctx->stream);
     return;
   }
-  const bool coarse = ctx->has_coarse;
+  // prec2Level_solve (pcg.F90:191-208) with several processes, where the
+  // restriction has taken r: method 4 adds no coarse correction, method 2
+  // (overlap > 0) keeps the coarse correction alone, 1, 3 and 5 add both
+  const bool coarse = ctx->has_coarse && p.method != 4;
+  const bool fine = !(ctx->has_coarse && p.method == 2);
   if (coarse) {
     launch_spmv(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, ctx->d_crhs, ctx->stream);
     launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_crhs, ctx->d_cu, ctx->d_cpart, ctx->stream);
     launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cu, ctx->d_csol, ctx->d_cpart, ctx->stream);
   }
   const double* fine_src = r;
-  if (p.fine_method >= 1) {
+  if (fine && p.fine_method >= 1) {
     enqueue_subdomain_solves(ctx, r);
     fine_src = ctx->d_subsol;
   }
-  launch_overlap_sum(ctx->n, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
-                     ctx->d_Rtcol, ctx->d_Rtval, ctx->d_csol, z, ctx->stream);
+  launch_overlap_sum(ctx->n, fine ? ctx->d_occ_ptr : nullptr, ctx->d_occ_pos, fine_src,
+                     coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, ctx->d_csol, z,
+                     ctx->stream);
 }
 
 int doug_spmv_device(doug_ctx* ctx, const double* dx, double* dy) {
@@ -1890,6 +1898,8 @@ int doug_prepare_batch(doug_ctx* ctx) {
   if (ctx->batch_ready) return DOUG_OK;
   if (ctx->p.fine_method == 2)
     return fail(ctx, DOUG_E_UNSUPPORTED, "the batched solve has no Gauss-Seidel fine level (fine_method 2)");
+  if (ctx->p.method >= 2 && ctx->p.method <= 4)
+    return fail(ctx, DOUG_E_UNSUPPORTED, "the batched solve covers methods 0, 1 and 5");
   const int n = ctx->n, NR = kMaxRhs;
   hipStream_t st = ctx->stream;
   int rc;

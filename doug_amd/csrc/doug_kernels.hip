@@ -683,7 +683,8 @@ __global__ void k_overlap_sum(int n, const int32_t* __restrict__ occ_ptr,
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = 0.0;
-  for (int p = occ_ptr[i]; p < occ_ptr[i + 1]; p++) acc = acc + sol[occ_pos[p]];
+  if (occ_ptr)  // no fine sum: the coarse correction alone
+    for (int p = occ_ptr[i]; p < occ_ptr[i + 1]; p++) acc = acc + sol[occ_pos[p]];
   if (rt_ptr) {
     double tmp = 0.0;
     for (int p = rt_ptr[i]; p < rt_ptr[i + 1]; p++) tmp = tmp + rt_val[p] * csol[rt_col[p]];

@@ -1366,6 +1366,13 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
   A->symmstruct = p->symmstruct;
   A->symmnumeric = p->symmstruct && p->symmnumeric;
   s->A = A;
+  if (p->method >= 2 && p->method <= 4) {
+    /* pcg.F90:146-147,253-255: with one process these restrict a residual
+     * array that nothing has written */
+    seterr("method %d is undefined with one process", p->method);
+    state_free(s);
+    return NULL;
+  }
 
   /* aggr.F90:131: with one process the aggregates, the coarse matrix and the
    * coarse-aggregate subdomains are built for levels 1 and 2 alike; only the
@@ -2117,7 +2124,17 @@ void or_prec_apply(or_state *s, const double *r, double *z) {
     env_solve(s->cfact, crhs + 1, csol + 1, w);
     double *tmp = s->w1;
     om_Ax(s->R, csol, tmp, 1, n);
-    for (int i = 0; i < n; i++) z[i] = z[i] + tmp[i + 1];
+    /* prec2Level_solve (pcg.F90:191-208) as written; with several processes
+     * (the only case where methods 2-4 are accepted) the restriction has taken
+     * r, and the overlap ol = max(overlap, smoothers) is positive */
+    int ol = p->overlap > s->smoothers ? p->overlap : s->smoothers;
+    if (p->method == 1 || p->method == 3) {
+      for (int i = 0; i < n; i++) z[i] = z[i] + tmp[i + 1];
+    } else if (p->method == 2 && ol > 0) {
+      for (int i = 0; i < n; i++) z[i] = tmp[i + 1];
+    }
+    if ((ol == 0 && p->method == 2) || p->method == 5)
+      for (int i = 0; i < n; i++) z[i] = z[i] + tmp[i + 1];
     free(crhs); free(csol); free(w);
   }
 }

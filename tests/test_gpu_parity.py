@@ -603,6 +603,9 @@ def test_duplicate_entry_inside_a_subdomain_is_refused():
     dict(smoothers=0),             # unsmoothed aggregates
     dict(smoothers=2, radius1=4),  # R = T (S2 S)^T
     dict(overlap=2),               # two layers shipped and added
+    dict(method=2),                # several processes: the coarse correction alone
+    dict(method=3),                # the same sum as method 1
+    dict(method=4),                # the fine level alone
 ])
 def test_rank_variants_against_oracle(variant):
     # the benchmarked set-up (one process per box, one subdomain each) under

@@ -264,3 +264,38 @@ def test_repeated_setups_reuse_scratch_cleanly():
                     np.testing.assert_array_equal(a, b)
                 for a, b in zip(s.subdomains(), fresh.subdomains()):
                     np.testing.assert_array_equal(a, b)
+
+
+def test_methods_2_to_4_need_several_processes():
+    # with one process methods 2-4 restrict a residual nothing has written
+    # (pcg.F90:146-147,253-255): refused by the library and the oracle
+    n, i, j, v = P.laplace2d(8)
+    for m in (2, 3, 4):
+        ctl = dict(BASE, radius1=2, method=m)
+        with pytest.raises(RuntimeError, match="undefined with one process"):
+            O.Oracle(n, i, j, v, O.params(**ctl))
+        with D.Solver(-1) as s:
+            s.set_params(**ctl)
+            s.set_matrix(n, i, j, v, 1)
+            with pytest.raises(RuntimeError, match="need virtual ranks"):
+                s.setup_host()
+            s.set_ranks(2)
+            s.setup_host()  # accepted with several processes
+
+
+def test_methods_2_to_4_with_ranks_follow_prec2level():
+    # prec2Level_solve with several processes: 3 adds the coarse correction
+    # like 1, 4 keeps the fine sum alone, 2 (overlap > 0) the coarse
+    # correction alone; checked on the oracle against its method 1 and
+    # one-level applies
+    n, i, j, v = P.diffusion2d(24, 1.0, 2)
+    rank = ((np.arange(n) // 96) + 1).astype(np.int32)
+    r = np.random.default_rng(3).uniform(-1, 1, n)
+    ctl = dict(BASE, radius1=2)
+    z = {m: O.Oracle.ranks(n, i, j, v, O.params(**dict(ctl, method=m)), rank).prec(r) for m in (1, 2, 3, 4)}
+    fine = O.Oracle.ranks(n, i, j, v, O.params(**dict(ctl, levels=1)), rank).prec(r)
+    np.testing.assert_array_equal(z[3], z[1])
+    np.testing.assert_array_equal(z[4], fine)
+    coarse = z[1] - fine
+    np.testing.assert_allclose(z[2], coarse, rtol=0, atol=1e-12 * np.abs(z[1]).max())
+    assert np.abs(z[2]).max() > 0
