@@ -1056,6 +1056,11 @@ static int check_params(doug_ctx* ctx) {
   if (p.method < 0 || p.method > 5 || (ctx->nranks <= 0 && p.method >= 2 && p.method <= 4))
     return fail(ctx, DOUG_E_UNSUPPORTED,
                 "method must be 0 (none), 1 or 5 (additive Schwarz); 2-4 need virtual ranks");
+  // a partition or rank map given for an earlier matrix
+  if (ctx->have_partition && (int64_t)ctx->part_nodes.size() != ctx->n)
+    return fail(ctx, DOUG_E_STATE, "the partition was set for another matrix: set it again");
+  if (ctx->nranks > 0 && (int64_t)ctx->eptn.size() != ctx->n)
+    return fail(ctx, DOUG_E_STATE, "the rank map was set for another matrix: call doug_set_ranks again");
   if (ctx->nranks > 0) {
     // several processes: a negative radius1 selects the graph partition of
     // the matrix and is negated (Distribution_assm.F90:127-134); the

@@ -299,3 +299,21 @@ def test_methods_2_to_4_with_ranks_follow_prec2level():
     coarse = z[1] - fine
     np.testing.assert_allclose(z[2], coarse, rtol=0, atol=1e-12 * np.abs(z[1]).max())
     assert np.abs(z[2]).max() > 0
+
+
+def test_stale_rank_map_or_partition_is_refused():
+    n, i, j, v = P.laplace2d(8)
+    n2, i2, j2, v2 = P.laplace2d(10)
+    with D.Solver(-1) as s:
+        s.set_params(**dict(BASE, radius1=2))
+        s.set_matrix(n, i, j, v, 1)
+        s.set_ranks(2)
+        s.set_matrix(n2, i2, j2, v2, 1)
+        with pytest.raises(RuntimeError, match="rank map was set for another matrix"):
+            s.setup_host()
+        s.set_ranks(0)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_partition(np.array([1, n + 1], np.int32), np.arange(1, n + 1, dtype=np.int32))
+        s.set_matrix(n2, i2, j2, v2, 1)
+        with pytest.raises(RuntimeError, match="partition was set for another matrix"):
+            s.setup_host()
