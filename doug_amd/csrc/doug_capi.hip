@@ -53,6 +53,10 @@ struct doug_ctx {
   // decomposition of the last setup
   int nranks = 0;
   std::vector<int> eptn;
+  // caller aggregates (doug_set_aggregates) and subdomains (doug_set_subdomains)
+  int inj_nagr = 0;
+  std::vector<int> inj_num;
+  std::vector<int> inj_sstarts, inj_sinds;
   int ol_dist = 0;
   RankSetup rsetup;
   RankWorkspaces rank_ws;  // the rank workers' scratch, reused by later setups
@@ -341,6 +345,58 @@ int doug_set_num_subdomains(doug_ctx* ctx, int32_t nsub) {
   if (nsub < 0) return fail(ctx, DOUG_E_ARG, "number of subdomains must be >= 0");
   ctx->num_subdomains = nsub;
   ctx->ready = false;
+  return DOUG_OK;
+}
+
+int doug_set_aggregates(doug_ctx* ctx, int32_t nagr, const int32_t* num) {
+  if (!ctx) return DOUG_E_ARG;
+  ctx->ready = ctx->host_ready = false;
+  if (nagr <= 0) {
+    ctx->inj_nagr = 0;
+    ctx->inj_num.clear();
+    return DOUG_OK;
+  }
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "doug_set_aggregates before doug_set_matrix_coo");
+  if (!num) return fail(ctx, DOUG_E_ARG, "num is NULL");
+  std::vector<int> cnt((size_t)nagr + 1, 0);
+  for (int i = 0; i < ctx->n; i++) {
+    if (num[i] < 0 || num[i] > nagr) return fail(ctx, DOUG_E_ARG, "aggregate number out of range");
+    cnt[num[i]]++;
+  }
+  for (int a = 1; a <= nagr; a++)
+    if (cnt[a] == 0) return fail(ctx, DOUG_E_ARG, "aggregate " + std::to_string(a) + " holds no unknown");
+  ctx->inj_num.assign(num, num + ctx->n);
+  ctx->inj_nagr = nagr;
+  return DOUG_OK;
+}
+
+int doug_set_subdomains(doug_ctx* ctx, int32_t nsub, const int32_t* starts, const int32_t* inds,
+                        int32_t base) {
+  if (!ctx) return DOUG_E_ARG;
+  ctx->ready = ctx->host_ready = false;
+  if (nsub <= 0) {
+    ctx->inj_sstarts.clear();
+    ctx->inj_sinds.clear();
+    return DOUG_OK;
+  }
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "doug_set_subdomains before doug_set_matrix_coo");
+  if (!starts || !inds || starts[0] != base || (base != 0 && base != 1))
+    return fail(ctx, DOUG_E_ARG, "bad subdomain starts");
+  std::vector<int> st((size_t)nsub + 1), in;
+  for (int q = 0; q <= nsub; q++) st[q] = starts[q] - base;
+  std::vector<int> seen((size_t)ctx->n, -1);
+  for (int q = 0; q < nsub; q++) {
+    if (st[q + 1] <= st[q]) return fail(ctx, DOUG_E_ARG, "subdomain " + std::to_string(q + base) + " is empty");
+    for (int k = st[q]; k < st[q + 1]; k++) {
+      const int v = inds[k] - base;
+      if (v < 0 || v >= ctx->n) return fail(ctx, DOUG_E_ARG, "subdomain index out of range");
+      if (seen[v] == q) return fail(ctx, DOUG_E_ARG, "an unknown appears twice in one subdomain");
+      seen[v] = q;
+      in.push_back(v);
+    }
+  }
+  ctx->inj_sstarts.swap(st);
+  ctx->inj_sinds.swap(in);
   return DOUG_OK;
 }
 
@@ -747,7 +803,29 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
   const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
   const int min_asize1 = p.minasize1 > 0 ? p.minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
   const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
-  if (ctx->nranks > 0) {
+  if (ctx->inj_nagr > 0) {
+    // the caller's aggregates; the full ones add the unaggregated unknowns
+    // one by one in node order (SpMtx_aggregation.F90:562-575)
+    const int n = ctx->A.nrows;
+    std::vector<int> full(ctx->inj_num);
+    int nf = ctx->inj_nagr;
+    for (int i = 0; i < n; i++)
+      if (full[i] == 0) full[i] = ++nf;
+    auto form = [n](int nagr, const std::vector<int>& num, Aggregates& a) {
+      a.nagr = nagr;
+      a.num = num;
+      a.starts.assign((size_t)nagr + 1, 0);
+      for (int x : num)
+        if (x > 0) a.starts[x]++;
+      for (int c = 0; c < nagr; c++) a.starts[c + 1] += a.starts[c];
+      a.nodes.assign((size_t)a.starts[nagr], 0);
+      std::vector<int> fill(a.starts.begin(), a.starts.end() - 1);
+      for (int i = 0; i < n; i++)
+        if (num[i] > 0) a.nodes[fill[num[i] - 1]++] = i;
+    };
+    form(ctx->inj_nagr, ctx->inj_num, ctx->fin);
+    form(nf, full, ctx->ffull);
+  } else if (ctx->nranks > 0) {
     // each rank aggregates its own rows of the values as read
     // with at least one rank per worker thread the ranks also put their
     // subdomains in the factor order of the device copy (reverse
@@ -984,6 +1062,13 @@ static void phase_subdomains(doug_ctx* ctx) {
   std::vector<int> pstarts, pnodes;
   ctx->cin = Aggregates{};
   ctx->cfull = Aggregates{};
+  if (!ctx->inj_sstarts.empty()) {  // the caller's index sets, overlap included
+    ctx->sstarts = ctx->inj_sstarts;
+    ctx->sinds = ctx->inj_sinds;
+    ctx->nsub = (int)ctx->sstarts.size() - 1;
+    ctx->info.t_subdomains = now() - t2;
+    return;
+  }
   if (ctx->nranks > 0) {  // the ranks' subdomains, layers included
     ctx->sstarts = std::move(ctx->rsetup.sstarts);
     ctx->sinds = std::move(ctx->rsetup.sinds);
@@ -1061,6 +1146,15 @@ static int check_params(doug_ctx* ctx) {
     return fail(ctx, DOUG_E_STATE, "the partition was set for another matrix: set it again");
   if (ctx->nranks > 0 && (int64_t)ctx->eptn.size() != ctx->n)
     return fail(ctx, DOUG_E_STATE, "the rank map was set for another matrix: call doug_set_ranks again");
+  if (ctx->inj_nagr > 0 && (int64_t)ctx->inj_num.size() != ctx->n)
+    return fail(ctx, DOUG_E_STATE, "the aggregates were set for another matrix: set them again");
+  if (!ctx->inj_sinds.empty()) {
+    for (int v : ctx->inj_sinds)
+      if (v >= ctx->n)
+        return fail(ctx, DOUG_E_STATE, "the subdomains were set for another matrix: set them again");
+  }
+  if (ctx->nranks > 0 && (ctx->inj_nagr > 0 || !ctx->inj_sstarts.empty()))
+    return fail(ctx, DOUG_E_ARG, "virtual ranks make their own aggregates and subdomains");
   if (ctx->nranks > 0) {
     // several processes: a negative radius1 selects the graph partition of
     // the matrix and is negated (Distribution_assm.F90:127-134); the

@@ -109,6 +109,22 @@ class Solver:
         """doug_set_num_subdomains: subdomains from parts of the fine-aggregate graph."""
         self._check(self._L.doug_set_num_subdomains(self.h, int(nsub)))
 
+    def set_aggregates(self, num):
+        """doug_set_aggregates: the fine aggregate (1-based, 0 = none) of
+        each unknown; None restores the aggregation."""
+        if num is None:
+            self._check(self._L.doug_set_aggregates(self.h, 0, None))
+            return
+        a = np.ascontiguousarray(num, np.int32)
+        self._check(self._L.doug_set_aggregates(self.h, int(a.max()), _lib.ptr(a)))
+
+    def set_subdomains(self, starts, inds, base=1):
+        """doug_set_subdomains: final index sets (overlap included)."""
+        st = np.ascontiguousarray(starts, np.int32)
+        ix = np.ascontiguousarray(inds, np.int32)
+        self._check(self._L.doug_set_subdomains(self.h, len(st) - 1, _lib.ptr(st), _lib.ptr(ix),
+                                                int(base)))
+
     def set_ranks(self, nranks: int, rank_of_node=None, base=1):
         """doug_set_ranks: the decomposition of a DOUG run on `nranks`
         processes (rank_of_node: owning rank of each unknown, or None for the

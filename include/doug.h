@@ -113,6 +113,26 @@ int doug_set_partition(doug_ctx *ctx, int32_t nparts, const int32_t *starts,
  * 0 or 1 restores the one-process rule. */
 int doug_set_num_subdomains(doug_ctx *ctx, int32_t nsub);
 
+/* Fine aggregates given by the caller in place of Partitionings_aggr_InitFine's
+ * (src/components/Partitioning_aggr.F90:33-117), e.g. the aggregates a DOUG
+ * run lists with `verbose` 5 or more (src/datatypes/Aggregate_mod.F90:115-120):
+ * num[i] is the aggregate (1..nagr) of
+ * unknown i, or 0 for an unknown no aggregate holds; every aggregate must
+ * hold an unknown.  The full aggregates add the others one by one in node
+ * order (src/datatypes/SpMtx/SpMtx_aggregation.F90:562-575).  The smoothed
+ * restriction, the coarse matrix and (without other subdomains) the coarse
+ * aggregation follow from them.  nagr <= 0 restores the aggregation. */
+int doug_set_aggregates(doug_ctx *ctx, int32_t nagr, const int32_t *num);
+
+/* Final subdomain index sets, overlap included, in place of
+ * FinePreconditioner_Init's (src/components/Preconditioner_base.F90:96-123),
+ * e.g. Decomposition%subd(:)%inds of a DOUG run: starts has nsub+1 entries
+ * (starts[0] == index_base), inds lists each set (non-empty, an unknown at
+ * most once per set; sets may overlap).  Takes precedence over
+ * doug_set_partition and doug_set_num_subdomains.  nsub <= 0 restores them. */
+int doug_set_subdomains(doug_ctx *ctx, int32_t nsub, const int32_t *starts, const int32_t *inds,
+                        int32_t index_base);
+
 /* Virtual ranks: the decomposition of a DOUG run on `nranks` MPI processes,
  * restated on one host and solved on one GPU.  rank_of_node[i] (index_base
  * 0 or 1) is the rank that owns unknown i, M%eptnmap of

@@ -633,3 +633,24 @@ def test_rank_variants_against_oracle(variant):
         xo, ito, condo, _ = o.pcg(b, 1e-8, 2000)
         assert it == ito, (it, ito)
         assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_given_aggregates_and_subdomains_against_oracle():
+    # DOUG's own setup products handed over through doug_set_aggregates and
+    # doug_set_subdomains: the solve equals the oracle's on them
+    n, i, j, v = P.diffusion3d(16, 1.0, 31)
+    b = P.uniform_rhs(n, 32)
+    o = O.Oracle(n, i, j, v, O.params(**CFG0_CTL))
+    num, _, _ = o.fine_aggregates()
+    st, inds = o.subdomains()
+    print("given aggregates and subdomains", flush=True)
+    with D.Solver(0) as s:
+        s.set_params(**CFG0_CTL)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_aggregates(num)
+        s.set_subdomains(st, inds)
+        s.setup(0)
+        x, it, cond, _ = s.pcg(b, 1e-8, 1000)
+    xo, ito, condo, _ = o.pcg(b, 1e-8, 1000)
+    assert it == ito
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
