@@ -317,3 +317,29 @@ def test_stale_rank_map_or_partition_is_refused():
         s.set_matrix(n2, i2, j2, v2, 1)
         with pytest.raises(RuntimeError, match="partition was set for another matrix"):
             s.setup_host()
+
+
+@pytest.mark.parametrize("case", ["node-per-rank", "node-per-rank-overlap2", "checkerboard",
+                                  "disconnected-matrix"])
+def test_edge_rank_maps_match_oracle(case):
+    ctl = dict(BASE, radius1=2)
+    n, i, j, v = P.laplace2d(6)
+    g = np.arange(n)
+    if case == "node-per-rank":
+        rank = g + 1
+    elif case == "node-per-rank-overlap2":
+        rank, ctl = g + 1, dict(ctl, overlap=2)
+    elif case == "checkerboard":  # every rank in many disconnected pieces
+        rank = (g // 6 + g % 6) % 2 + 1
+    else:  # two independent grids, ranks crossing both
+        n1, i1, j1, v1 = P.laplace2d(5)
+        n, i, j, v = 2 * n1, np.r_[i1, i1 + n1], np.r_[j1, j1 + n1], np.r_[v1, v1]
+        rank = (np.arange(n) // 7) % 3 + 1
+    rank = rank.astype(np.int32)
+    nr = int(rank.max())
+    o = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank)
+    with _host(n, i, j, v, nr, rank, **ctl) as s:
+        for a, b in zip(s.fine_aggregates(), o.fine_aggregates()):
+            np.testing.assert_array_equal(a, b)
+        for a, b in zip(s.subdomains(), o.subdomains()):
+            np.testing.assert_array_equal(a, b)
