@@ -35,6 +35,7 @@ double now() {
 struct doug_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // subdomain factorisation beside the coarse one
   std::string err;
   Params p;
   doug_info info{};
@@ -271,8 +272,12 @@ int doug_ctx_create(int device, doug_ctx** out) {
   doug_ctx* ctx = new doug_ctx();
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scalm, kMaxRhs * 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+    for (hipStream_t st : {ctx->stream, ctx->stream2})
+      if (st) (void)hipStreamDestroy(st);
     delete ctx;
     return DOUG_E_HIP;
   }
@@ -287,11 +292,13 @@ int doug_ctx_destroy(doug_ctx* ctx) {
     return DOUG_OK;
   }
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (hipStream_t st : {ctx->stream, ctx->stream2})
+    if (st) (void)hipStreamSynchronize(st);
   free_device(ctx);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
   if (ctx->h_scalm) (void)hipHostFree(ctx->h_scalm);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  for (hipStream_t st : {ctx->stream, ctx->stream2})
+    if (st) (void)hipStreamDestroy(st);
   delete ctx;
   return DOUG_OK;
 }
@@ -1573,7 +1580,9 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   HIPCHK(hipSetDevice(ctx->device));
   free_device(ctx);
   double t0 = now();
-  hipEvent_t ev[4];
+  // [0, 1] subdomain factorisation, [2, 3] coarse factorisation, [4] the
+  // input of the subdomain factorisation, [5] its end
+  hipEvent_t ev[6];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   std::vector<double>& scaled = ctx->scaled;
   std::vector<uint8_t>& strong = ctx->strong;
@@ -1615,11 +1624,21 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     if (rc) return rc;
     if ((rc = coarse_space())) return rc;
     // the coarse factorisation runs on the GPU while the host prepares the
-    // subdomains (same stream: the factorisations still follow each other)
+    // subdomains; the subdomain factorisation then goes to the second stream
+    // beside it (the coarse one is a chain of small right-looking steps that
+    // leaves most of the GPU idle).  It starts after the matrix upload, and
+    // the first stream waits for its end before anything else.
+    HIPCHK(hipEventRecord(ev[4], ctx->stream));
     if ((rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
     coarse_done = true;
     phase_subdomains(ctx);
-    if ((rc = device_fine(ctx, ev[0], ev[1]))) return rc;
+    HIPCHK(hipStreamWaitEvent(ctx->stream2, ev[4], 0));
+    std::swap(ctx->stream, ctx->stream2);
+    rc = device_fine(ctx, ev[0], ev[1]);
+    std::swap(ctx->stream, ctx->stream2);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ev[5], ctx->stream2));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ev[5], 0));
   }
   fill_info(ctx);
   ctx->host_ready = true;

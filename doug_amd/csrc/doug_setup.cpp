@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <queue>
 #include <tuple>
 #include <thread>
@@ -1504,10 +1505,19 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
                    RankSetup& out, const Csr* rcm_graph, RankWorkspaces* ws) {
   const int n = As.nrows;
   std::vector<std::vector<int>> own((size_t)nranks);
+  {
+    std::vector<int> cnt((size_t)nranks, 0);
+    for (int i = 0; i < n; i++) cnt[eptn[i]]++;
+    for (int r = 0; r < nranks; r++) own[r].reserve((size_t)cnt[r]);
+  }
   for (int i = 0; i < n; i++) own[eptn[i]].push_back(i);
   out.ranks.assign((size_t)nranks, RankLocal{});
   std::vector<Aggregates> inner((size_t)nranks), full((size_t)nranks);
   std::vector<std::vector<int>> sst((size_t)nranks), sin((size_t)nranks), dsin((size_t)nranks);
+  // own local indices in ascending global order, and the own nodes listed in
+  // the inner and full aggregates of each rank
+  std::vector<std::vector<int>> by_global((size_t)nranks);
+  std::vector<int> nlisted[2] = {std::vector<int>((size_t)nranks, 0), std::vector<int>((size_t)nranks, 0)};
   const int nparts = std::max(nsub_per_rank, 1);
   // the ranks are independent processes in the reference: one worker each
   const int nw = std::max(1, std::min(nranks, setup_threads()));
@@ -1542,6 +1552,21 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
         std::vector<uint8_t> strong = find_strong(LA, scaled, diag, strong1);
         ProcessState ps;
         aggregate(LA, scaled, strong, radius1, minasize1, maxasize1, strong1, false, ps, inner[r], full[r]);
+        for (int i = 0; i < R.ninner; i++) {
+          nlisted[0][r] += inner[r].num[i] > 0;
+          nlisted[1][r] += full[r].num[i] > 0;
+        }
+        // the own part of the numbering is a few ascending runs (sent, inner
+        // overlap, inner): merged run by run into global order
+        std::vector<int>& ord = by_global[r];
+        ord.resize((size_t)R.ninner);
+        std::iota(ord.begin(), ord.end(), 0);
+        const auto by_gl = [&R](int a, int b) { return R.lg[a] < R.lg[b]; };
+        for (int i = 1, run = 0; i <= R.ninner; i++)
+          if (i == R.ninner || R.lg[i] < R.lg[i - 1]) {
+            if (run > 0) std::inplace_merge(ord.begin(), ord.begin() + run, ord.begin() + i, by_gl);
+            run = i;
+          }
         // Partitionings_metis_InitCoarse: inner nodes in local order, grouped
         // by the part of their aggregate (one part: all of them)
         std::vector<int> part;
@@ -1598,12 +1623,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
   std::vector<int> node_off[2] = {std::vector<int>((size_t)nranks + 1, 0),
                                   std::vector<int>((size_t)nranks + 1, 0)};
   for (int r = 0; r < nranks; r++)
-    for (int w = 0; w < 2; w++) {
-      const Aggregates& a = w == 0 ? inner[r] : full[r];
-      int c = 0;
-      for (int i = 0; i < out.ranks[r].ninner; i++) c += a.num[i] > 0;
-      node_off[w][r + 1] = node_off[w][r] + c;
-    }
+    for (int w = 0; w < 2; w++) node_off[w][r + 1] = node_off[w][r] + nlisted[w][r];
   std::vector<std::thread> th2;
   for (int t = 0; t < nw; t++)
     th2.emplace_back([&, t]() {
@@ -1617,21 +1637,20 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
           Aggregates& g = *lists[w];
           const int off = w == 0 ? off_in[r] : off_full[r];
           cnt.assign((size_t)a.nagr + 1, 0);
-          std::vector<std::pair<int, int>> vs;
-          vs.reserve((size_t)R.ninner);
           for (int i = 0; i < R.ninner; i++)
-            if (a.num[i] > 0) vs.emplace_back(R.lg[i], a.num[i]);
-          std::sort(vs.begin(), vs.end());
-          for (auto& q : vs) cnt[q.second]++;
+            if (a.num[i] > 0) cnt[a.num[i]]++;
           int base = node_off[w][r];
           for (int c = 1; c <= a.nagr; c++) {
             g.starts[off + c - 1] = base;
             base += cnt[c];
             cnt[c] = g.starts[off + c - 1];
           }
-          for (auto& q : vs) {
-            g.num[q.first] = q.second + off;
-            g.nodes[cnt[q.second]++] = q.first;
+          for (int i : by_global[r]) {
+            const int c = a.num[i];
+            if (c <= 0) continue;
+            const int gi = R.lg[i];
+            g.num[gi] = c + off;
+            g.nodes[cnt[c]++] = gi;
           }
         }
         for (size_t q = 1; q < sst[r].size(); q++) out.sstarts[off_sub[r] + q] = off_ent[r] + sst[r][q];
