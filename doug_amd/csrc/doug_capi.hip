@@ -36,6 +36,8 @@ struct doug_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // subdomain factorisation beside the coarse one
+  hipStream_t stream3 = nullptr;  // coarse inverse one step behind the coarse factorisation
+  std::vector<hipEvent_t> cevents;  // coarse factorisation step ends, then the inverse's end
   std::string err;
   Params p;
   doug_info info{};
@@ -273,10 +275,11 @@ int doug_ctx_create(int device, doug_ctx** out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scalm, kMaxRhs * 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
-    for (hipStream_t st : {ctx->stream, ctx->stream2})
+    for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
       if (st) (void)hipStreamDestroy(st);
     delete ctx;
     return DOUG_E_HIP;
@@ -292,12 +295,13 @@ int doug_ctx_destroy(doug_ctx* ctx) {
     return DOUG_OK;
   }
   (void)hipSetDevice(ctx->device);
-  for (hipStream_t st : {ctx->stream, ctx->stream2})
+  for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
     if (st) (void)hipStreamSynchronize(st);
   free_device(ctx);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
   if (ctx->h_scalm) (void)hipHostFree(ctx->h_scalm);
-  for (hipStream_t st : {ctx->stream, ctx->stream2})
+  for (hipEvent_t e : ctx->cevents) (void)hipEventDestroy(e);
+  for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
     if (st) (void)hipStreamDestroy(st);
   delete ctx;
   return DOUG_OK;
@@ -1543,14 +1547,28 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
     // trailing updates of a step run side by side
     ctx->cnt = cdesc[0].nt;
     const int cnt = ctx->cnt;
+    // the explicit inverse's step m follows the factorisation's step m on
+    // the third stream, beside the factorisation's later steps (each a short
+    // chain of small launches that leaves most of the GPU idle)
+    while ((int)ctx->cevents.size() < cnt + 2) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ctx->cevents.push_back(e);
+    }
+    if ((rc = dalloc(ctx, &ctx->d_cW, (size_t)celems))) return rc;
+    HIPCHK(hipEventRecord(ctx->cevents[cnt + 1], ctx->stream));  // the coarse tiles are in place
+    HIPCHK(hipStreamWaitEvent(ctx->stream3, ctx->cevents[cnt + 1], 0));
+    launch_inv_init(ctx->d_cW, ctx->d_cdesc, cnt, ctx->stream3);
     for (int j = 0; j < cnt; j++) {
       launch_chol_diag(ctx->d_ctiles, ctx->d_cdesc, 1, j, ctx->d_flag + 1, ctx->stream, false);
       launch_chol_offdiag(ctx->d_ctiles, ctx->d_cdesc, 1, j, cnt - 1 - j, ctx->stream, false);
+      HIPCHK(hipEventRecord(ctx->cevents[j], ctx->stream));
+      HIPCHK(hipStreamWaitEvent(ctx->stream3, ctx->cevents[j], 0));
+      launch_inv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, j, ctx->stream3);
       launch_rl_update(ctx->d_ctiles, ctx->d_cdesc, cnt, j, ctx->stream);
     }
-    if ((rc = dalloc(ctx, &ctx->d_cW, (size_t)celems))) return rc;
-    launch_inv_init(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, ctx->stream);
-    for (int m = 0; m < cnt; m++) launch_inv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, m, ctx->stream);
+    HIPCHK(hipEventRecord(ctx->cevents[cnt], ctx->stream3));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->cevents[cnt], 0));
     HIPCHK(hipGetLastError());
     size_t cpad = (size_t)ctx->cnt * kTile;
     if ((rc = dalloc(ctx, &ctx->d_cpart, cpad * gemv_chunks(ctx->cnt)))) return rc;
@@ -1589,6 +1607,16 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   // non-positive pivot flags: [0] subdomains, [1] coarse matrix
   if ((rc = dalloc(ctx, &ctx->d_flag, 2))) return rc;
   HIPCHK(hipMemsetAsync(ctx->d_flag, 0, 2 * sizeof(int), ctx->stream));
+  // several processes aggregate their own rows of the values as read (the
+  // pattern of A and A_input_val, which the scale round trip leaves alone):
+  // they start before it, the round trip and the upload take half of the
+  // hardware threads beside them
+  std::thread agg;
+  const bool early_agg = ctx->nranks > 0 && ctx->inj_nagr == 0 && !ctx->have_partition;
+  if (early_agg) {
+    agg = std::thread([&]() { phase_aggregate(ctx, scaled, strong); });
+    set_setup_thread_cap(std::max(1, setup_threads() / 2));
+  }
   phase_values(ctx, scaled, strong);
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
@@ -1604,7 +1632,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     // two touch disjoint context members and read A, scaled and strong only.
     // The aggregation is sequential and the longest setup step, so this
     // thread's parallel loops leave it half of the hardware threads.
-    std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
+    agg = std::thread([&]() { phase_aggregate(ctx, scaled, strong); });
     set_setup_thread_cap(std::max(1, setup_threads() / 2 - 1));
     rc = upload_matrix(ctx);
     if (!rc) rc = upload_strong(ctx, strong);
@@ -1617,9 +1645,10 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     if (rc) return rc;
     if ((rc = coarse_space())) return rc;
   } else {
-    std::thread agg([&]() { phase_aggregate(ctx, scaled, strong); });
+    if (!early_agg) agg = std::thread([&]() { phase_aggregate(ctx, scaled, strong); });
     rc = upload_matrix(ctx);
     if (!rc) rc = upload_strong(ctx, strong);
+    set_setup_thread_cap(0);
     agg.join();
     if (rc) return rc;
     if ((rc = coarse_space())) return rc;

@@ -326,8 +326,11 @@ __global__ __launch_bounds__(256) void k_rl_update(double* __restrict__ tiles,
 }
 
 // Inverse W = L^{-1} of a dense factor by right-looking substitution on
-// L W = I (W starts as I with inv(L_mm) on the diagonal tiles).  Step m:
-// finish block row m, W_mc = inv(L_mm) W_mc for c < m ...
+// L W = I (W starts with zero tiles below the diagonal).  Step m reads only
+// block column m of L, final once the factorisation's step m has solved it,
+// so step m may run as soon as that step is done.  Step m: W_mm = inv(L_mm)
+// (held by the factor's diagonal tile), and block row m is finished,
+// W_mc = inv(L_mm) W_mc for c < m (grid: m + 1) ...
 __global__ __launch_bounds__(256) void k_inv_finish(const double* __restrict__ L,
                                                     double* __restrict__ W,
                                                     const BlockDesc* __restrict__ d, int m) {
@@ -335,6 +338,11 @@ __global__ __launch_bounds__(256) void k_inv_finish(const double* __restrict__ L
   __shared__ double Bs[kTile * LD];
   const BlockDesc b = d[0];
   const int c = blockIdx.x;
+  if (c == m) {
+    const int64_t o = b.base + toff(b, m, m);
+    for (int e = threadIdx.x; e < kTileElems; e += blockDim.x) W[o + e] = L[o + e];
+    return;
+  }
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
   tile_to_lds(As, L + b.base + toff(b, m, m));  // inv(L_mm)
   tile_to_lds(Bs, W + b.base + toff(b, m, c));
@@ -380,14 +388,13 @@ __global__ __launch_bounds__(256) void k_inv_update(const double* __restrict__ L
       }
 }
 
-// W's diagonal tiles = the factor's (inv(L_ii)), its other tiles zero
-__global__ void k_inv_init(const double* __restrict__ L, double* __restrict__ W,
-                           const BlockDesc* __restrict__ d) {
+// W's tiles below the diagonal zero (its diagonal tiles come with each step)
+__global__ void k_inv_init(double* __restrict__ W, const BlockDesc* __restrict__ d) {
   const BlockDesc b = d[0];
   const int i = blockIdx.y, k = blockIdx.x;
-  if (k > i) return;
+  if (k >= i) return;
   const int64_t o = b.base + toff(b, i, k);
-  for (int e = threadIdx.x; e < kTileElems; e += blockDim.x) W[o + e] = i == k ? L[o + e] : 0.0;
+  for (int e = threadIdx.x; e < kTileElems; e += blockDim.x) W[o + e] = 0.0;
 }
 
 // ------------------------------------------------------------------------
@@ -1511,14 +1518,14 @@ void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, hipStrea
   hipLaunchKernelGGL(k_rl_update, dim3(m * (m + 1) / 2), dim3(256), 0, st, tiles, d, j);
 }
 
-void launch_inv_init(const double* L, double* W, const BlockDesc* d, int nt, hipStream_t st) {
+void launch_inv_init(double* W, const BlockDesc* d, int nt, hipStream_t st) {
   if (nt <= 0) return;
-  hipLaunchKernelGGL(k_inv_init, dim3(nt, nt), dim3(256), 0, st, L, W, d);
+  hipLaunchKernelGGL(k_inv_init, dim3(nt, nt), dim3(256), 0, st, W, d);
 }
 
 void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m,
                      hipStream_t st) {
-  if (m > 0) hipLaunchKernelGGL(k_inv_finish, dim3(m), dim3(256), 0, st, L, W, d, m);
+  hipLaunchKernelGGL(k_inv_finish, dim3(m + 1), dim3(256), 0, st, L, W, d, m);
   if (nt - m - 1 > 0)
     hipLaunchKernelGGL(k_inv_update, dim3(m + 1, nt - m - 1), dim3(256), 0, st, L, W, d, m);
 }

@@ -76,8 +76,10 @@ void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int m
 // A_ik -= L_ij L_kj^T for j < k <= i.
 void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, hipStream_t st);
 // W = L^{-1} of a dense factor (diagonal tiles inv(L_ii)): launch_inv_init,
-// then launch_inv_step for m = 0 .. nt-1 (right-looking substitution).
-void launch_inv_init(const double* L, double* W, const BlockDesc* d, int nt, hipStream_t st);
+// then launch_inv_step for m = 0 .. nt-1 (right-looking substitution).  Step
+// m reads block column m of L alone: it may start once the factorisation's
+// step m has solved that column, while the later steps go on elsewhere.
+void launch_inv_init(double* W, const BlockDesc* d, int nt, hipStream_t st);
 void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m,
                      hipStream_t st);
 
