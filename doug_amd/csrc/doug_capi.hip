@@ -38,6 +38,9 @@ struct doug_ctx {
   hipStream_t stream2 = nullptr;  // subdomain factorisation beside the coarse one
   hipStream_t stream3 = nullptr;  // coarse inverse one step behind the coarse factorisation
   std::vector<hipEvent_t> cevents;  // coarse factorisation step ends, then the inverse's end
+  // temporaries of the device coarse build, kept between set-ups
+  std::vector<std::pair<void*, size_t>> scratch_slots;
+  std::thread reaper;  // frees the buffers of the previous set-up
   std::string err;
   Params p;
   doug_info info{};
@@ -193,10 +196,28 @@ static void drop_iteration_graph(doug_ctx* ctx) {
   }
 }
 
-static void free_device(doug_ctx* ctx) {
+static void join_reaper(doug_ctx* ctx) {
+  if (ctx->reaper.joinable()) ctx->reaper.join();
+}
+
+// Drops every device buffer of the last set-up.  In the background (a new
+// set-up) a host thread frees them while the set-up's host work goes on:
+// hipFree waits for the device and unmaps, milliseconds for the factors.
+static void free_device(doug_ctx* ctx, bool background = false) {
   drop_iteration_graph(ctx);
-  for (void* p : ctx->allocs) (void)hipFree(p);
-  ctx->allocs.clear();
+  join_reaper(ctx);
+  if (background && !ctx->allocs.empty()) {
+    std::vector<void*> old;
+    old.swap(ctx->allocs);
+    const int dev = ctx->device;
+    ctx->reaper = std::thread([dev, old = std::move(old)]() {
+      (void)hipSetDevice(dev);
+      for (void* p : old) (void)hipFree(p);
+    });
+  } else {
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    ctx->allocs.clear();
+  }
   ctx->ready = false;
   ctx->batch_ready = false;
 }
@@ -298,6 +319,8 @@ int doug_ctx_destroy(doug_ctx* ctx) {
   for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
     if (st) (void)hipStreamSynchronize(st);
   free_device(ctx);
+  for (auto& q : ctx->scratch_slots)
+    if (q.first) (void)hipFree(q.first);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
   if (ctx->h_scalm) (void)hipHostFree(ctx->h_scalm);
   for (hipEvent_t e : ctx->cevents) (void)hipEventDestroy(e);
@@ -881,16 +904,32 @@ static void phase_coarse_host(doug_ctx* ctx, const std::vector<uint8_t>& strong)
 // R^T's entries on their aggregate, T' = A R^T and A_c = R T' by
 // first-touch products.  R and R^T stay on the device for the
 // preconditioner; R and A_c are copied to the host (getters, coarse factor).
+// Temporaries of one device build, in slots that the context keeps between
+// set-ups: the k-th request takes slot k, grown when too small, so that a
+// repeated set-up of the same size neither allocates nor frees (hipFree
+// waits for the device).  The set-up ends with the device idle, so the next
+// one may overwrite them.
 struct DevTemp {
-  std::vector<void*> ptrs;
-  ~DevTemp() {
-    for (void* q : ptrs) (void)hipFree(q);
-  }
+  std::vector<std::pair<void*, size_t>>& slots;
+  size_t next = 0;
+  explicit DevTemp(std::vector<std::pair<void*, size_t>>& s) : slots(s) {}
   template <typename T>
   hipError_t alloc(T** q, size_t count) {
-    hipError_t e = hipMalloc((void**)q, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) ptrs.push_back((void*)*q);
-    return e;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (next == slots.size()) slots.emplace_back(nullptr, 0);
+    std::pair<void*, size_t>& sl = slots[next++];
+    if (sl.second < bytes) {
+      if (sl.first) (void)hipFree(sl.first);
+      sl = {nullptr, 0};
+      hipError_t e = hipMalloc(&sl.first, bytes);
+      if (e != hipSuccess) {
+        sl.first = nullptr;
+        return e;
+      }
+      sl.second = bytes;
+    }
+    *q = (T*)sl.first;
+    return hipSuccess;
   }
 };
 
@@ -900,7 +939,7 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   const int n = A.nrows, nagr = ctx->fin.nagr;
   const int64_t nnz = A.nnz();
   hipStream_t st = ctx->stream;
-  DevTemp tmp;
+  DevTemp tmp(ctx->scratch_slots);
   int rc;
   void* scan_tmp = nullptr;
   size_t scan_cap = 0;
@@ -1596,7 +1635,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   if (rc) return rc;
   ctx->precision = precision;
   HIPCHK(hipSetDevice(ctx->device));
-  free_device(ctx);
+  free_device(ctx, true);
   double t0 = now();
   // [0, 1] subdomain factorisation, [2, 3] coarse factorisation, [4] the
   // input of the subdomain factorisation, [5] its end
@@ -1675,6 +1714,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   int flag[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(flag, ctx->d_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  join_reaper(ctx);
   float ms_f = 0.f, ms_c = 0.f;
   HIPCHK(hipEventElapsedTime(&ms_f, ev[0], ev[1]));
   HIPCHK(hipEventElapsedTime(&ms_c, ev[2], ev[3]));
