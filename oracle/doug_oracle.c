@@ -29,6 +29,21 @@
  */
 #include "doug_oracle.h"
 
+/* OR_MUTANT (tests only): builds of this file with -DOR_MUTANT=k plant one
+ * known mistake each in the aggregation, so that the tests can show that
+ * their hand-traced cases fail on it.  0 (the default) is the restatement.
+ *   1 rounding test `mind > allmind` written as `>=`
+ *   2 `allmind` not saved between lets_colour calls
+ *   3 `allmind` reset at the start of a coarse-level call
+ *   4 `beta` reset at the start of a coarse-level call
+ *   5 repeated next-good-starts not preferred (always the first)
+ *   6 merge test `maxconnweightsum >= alpha` written as `>`
+ *   7 node hand-over compares |a_ij| with alpha instead of beta
+ *   8 rounding fit check `nneigs+ninnodes+1 > maxasize` written as `>=` */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+
 #include <limits.h>
 #include <math.h>
 #include <stdarg.h>
@@ -42,7 +57,9 @@
 
 enum { ARR_NO = 0, ARR_ROWS = 1, ARR_COLS = 2 };
 
-static char g_err[512];
+/* Per thread: the parallel loops below (ranks, subdomain factorisations and
+ * solves) copy a worker's message to the calling thread. */
+static _Thread_local char g_err[512];
 static void seterr(const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -65,8 +82,9 @@ static void *xcalloc(size_t n, size_t sz) {
 /* ------------------------------------------------------------------ */
 /* Saved state of one process                                           */
 /* ------------------------------------------------------------------ */
-static int g_allmind = D_MAXINT;   /* Aggregate_mod.F90:348 `save` */
-static double g_beta = 1.0E-5;     /* SpMtx_aggregation.F90:83 initialised => saved */
+/* One process per thread: a rank restated on a worker thread has its own. */
+static _Thread_local int g_allmind = D_MAXINT;   /* Aggregate_mod.F90:348 `save` */
+static _Thread_local double g_beta = 1.0E-5;     /* SpMtx_aggregation.F90:83 initialised => saved */
 
 void or_reset_process(void) {
   g_allmind = D_MAXINT;
@@ -428,11 +446,12 @@ static int can_add_roundlayer(int layer, int *ninnodes, Lbuf *b, int neighood,
       if (d > maxd) maxd = d;
     }
     if (mind < g_allmind) g_allmind = mind;
-    if (mind < maxd || mind > g_allmind) {
+    if (mind < maxd || (OR_MUTANT == 1 ? mind >= g_allmind : mind > g_allmind)) {
       for (int k = 1; k <= ntoadd; k++) {
         if (stat[newlayer[k]] == -maxd) {
           nrounders++;
-          if (*nneigs + *ninnodes + 1 > maxasize) {
+          if (OR_MUTANT == 8 ? *nneigs + *ninnodes + 1 >= maxasize
+                             : *nneigs + *ninnodes + 1 > maxasize) {
             layerfits = 0;
             break;
           }
@@ -510,6 +529,7 @@ static int lets_colour(int innode, int neighood, int minasize, int maxasize,
   *nneigs = 0;
   stat[innode] = 1;
   distance[innode] = 0;
+  if (OR_MUTANT == 2) g_allmind = D_MAXINT;
   int ninnodes = 1;
   b->inlayer[1] = innode;
   return can_add_roundlayer(0, &ninnodes, b, neighood, minasize, maxasize,
@@ -534,6 +554,8 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
     else
       maxasizelargest = maxasize + 4 * (2 * neighood);
     if (!has_fine) g_beta = alpha / 4.0;
+    if (has_fine && OR_MUTANT == 3) g_allmind = D_MAXINT;
+    if (has_fine && OR_MUTANT == 4) g_beta = 1.0E-5;
     int ngoodstarts = 0;
     int *nextgoodstart = IALLOC(n);
     int firstwocol = 1;
@@ -574,7 +596,7 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
          * per-node stamp, which selects the same entry in linear time */
         startnode = 0;
         stamp_id++;
-        for (int i = 1; i <= ngoodstarts; i++) {
+        for (int i = 1; i <= ngoodstarts && OR_MUTANT != 5; i++) {
           int v = nextgoodstart[i];
           if (stamp[v] == stamp_id) {
             startnode = v;
@@ -726,7 +748,7 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
           maxcw = connw[j];
           eater = colsaround[j];
         }
-      if (maxcw >= alpha || (has_fine && maxcw >= g_beta)) {
+      if ((OR_MUTANT == 6 ? maxcw > alpha : maxcw >= alpha) || (has_fine && maxcw >= g_beta)) {
         for (int j = 1; j <= aggrsize[i]; j++) {
           aggrsize[eater]++;
           if (aggrsize[eater] > maxstructlen) {
@@ -750,7 +772,8 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
               int cn = A->indj[j];
               if (cn > n) continue;
               int colr = aggrnum[cn];
-              if (colr != i && aggrsize[colr] < maxasizelargest && fabs(A->val[j]) >= g_beta) {
+              if (colr != i && aggrsize[colr] < maxasizelargest &&
+                  fabs(A->val[j]) >= (OR_MUTANT == 7 ? alpha : g_beta)) {
                 aggrsize[colr]++;
                 if (aggrsize[colr] > maxstructlen) {
                   fprintf(stderr, "doug_oracle: structnodes overflow\n");
@@ -1143,6 +1166,35 @@ int or_add_layers(const int *bounds, const int *adj, int n, int *nodes,
   return on;
 }
 
+/* SpMtx_find_strong + SpMtx_aggregate on a given matrix inside the current
+ * process state (allmind, beta are neither reset nor restored), as a
+ * fine-level (has_fine == 0) or coarse-level call.  num_inner/num_full
+ * receive n aggregate numbers each; returns the inner aggregate count. */
+int or_aggregate(int n, long nnz, const int *indi, const int *indj, const double *val,
+                 int symm, int radius, int minasize, int maxasize, double alpha, int has_fine,
+                 int *num_inner, int *num_full) {
+  OM *A = om_new(nnz, n, n);
+  for (long i = 0; i < nnz; i++) {
+    A->indi[i + 1] = indi[i];
+    A->indj[i + 1] = indj[i];
+    A->val[i + 1] = val[i];
+  }
+  A->symmstruct = symm;
+  A->symmnumeric = symm;
+  Aggrs in = {0}, full = {0};
+  om_find_strong(A, alpha);
+  om_aggregate(A, &in, &full, radius, minasize, maxasize, alpha, has_fine);
+  for (int i = 1; i <= n; i++) {
+    num_inner[i - 1] = in.num[i];
+    num_full[i - 1] = full.num[i];
+  }
+  int nagr = in.nagr;
+  aggrs_free(&in);
+  aggrs_free(&full);
+  om_free(A);
+  return nagr;
+}
+
 /* ------------------------------------------------------------------ */
 /* Exact sparse SPD solves: envelope Cholesky after reverse Cuthill-McKee */
 /* (stands in for UMFPACK's exact solve, src/solvers/Fact.F90:64-281)   */
@@ -1496,18 +1548,33 @@ static or_state *setup_finish(or_state *s) {
   int n = s->n, np = s->nsub;
   /* factorisations (Factorise_subdomains, subsolvers.F90:78-102) */
   if (p->fine_method == 1) {
+    /* the subdomains are independent: factorised side by side, each with
+     * the same arithmetic as one after another */
     s->fact = (Env **)xcalloc((size_t)np, sizeof(Env *));
-    int *floc = IALLOC(n);
-    for (int ip = 1; ip <= np; ip++) {
-      int st = s->sstarts[ip], nn = s->sstarts[ip + 1] - st;
-      s->fact[ip - 1] = factor_subdomain(A, s->sinds + st - 1, nn, floc);
-      if (!s->fact[ip - 1]) {
-        free(floc);
-        state_free(s);
-        return NULL;
+    int failed = 0;
+    char msg[512] = "";
+#pragma omp parallel
+    {
+      int *floc = IALLOC(n);
+#pragma omp for schedule(dynamic, 1)
+      for (int ip = 1; ip <= np; ip++) {
+        int st = s->sstarts[ip], nn = s->sstarts[ip + 1] - st;
+        s->fact[ip - 1] = factor_subdomain(A, s->sinds + st - 1, nn, floc);
+        if (!s->fact[ip - 1]) {
+#pragma omp critical(or_err)
+          if (!failed) {
+            failed = 1;
+            memcpy(msg, g_err, sizeof msg);
+          }
+        }
       }
+      free(floc);
     }
-    free(floc);
+    if (failed) {
+      memcpy(g_err, msg, sizeof msg);
+      state_free(s);
+      return NULL;
+    }
   }
   if (p->fine_method == 2) {
     /* FinePreconditioner_sgs_Init (FinePreconditioner_sgs.F90:32-64):
@@ -1882,6 +1949,14 @@ or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, cons
   long cap = 0;
   int **sub = (int **)xcalloc((size_t)nranks + 1, sizeof(int *));
   int *subn = IALLOC(nranks);
+  /* each rank is a process of its own: restated side by side on worker
+   * threads, each with its own saved aggregation state, and merged in rank
+   * order below */
+  Aggrs *rin = (Aggrs *)xcalloc((size_t)nranks + 1, sizeof(Aggrs));
+  Aggrs *rfull = (Aggrs *)xcalloc((size_t)nranks + 1, sizeof(Aggrs));
+  int **rlg = (int **)xcalloc((size_t)nranks + 1, sizeof(int *));
+  int *rninner = IALLOC(nranks);
+#pragma omp parallel for schedule(dynamic, 1)
   for (int r = 1; r <= nranks; r++) {
     RankView rv;
     rank_view(Ag, eptn, nranks, r, ol_dist, &rv);
@@ -1929,20 +2004,27 @@ or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, cons
     sub[r] = IALLOC(on);
     for (int q = 1; q <= on; q++) sub[r][q] = rv.lg[nodes[q]];
     subn[r] = on;
-    cap += on;
     free(nodes);
-    for (int i = 1; i <= rv.ninner; i++) {
-      int g = rv.lg[i];
-      if (i <= in.n && in.num[i] > 0) num_in[g] = in.num[i] + off_in;
-      if (i <= full.n) num_full[g] = full.num[i] + off_full;
-    }
-    off_in += in.nagr;
-    off_full += full.nagr;
-    aggrs_free(&in);
-    aggrs_free(&full);
+    rin[r] = in;
+    rfull[r] = full;
+    rlg[r] = rv.lg;
+    rninner[r] = rv.ninner;
     om_free(rv.A);
-    free(rv.lg);
   }
+  for (int r = 1; r <= nranks; r++) {
+    cap += subn[r];
+    for (int i = 1; i <= rninner[r]; i++) {
+      int g = rlg[r][i];
+      if (i <= rin[r].n && rin[r].num[i] > 0) num_in[g] = rin[r].num[i] + off_in;
+      if (i <= rfull[r].n) num_full[g] = rfull[r].num[i] + off_full;
+    }
+    off_in += rin[r].nagr;
+    off_full += rfull[r].nagr;
+    aggrs_free(&rin[r]);
+    aggrs_free(&rfull[r]);
+    free(rlg[r]);
+  }
+  free(rin); free(rfull); free(rlg); free(rninner);
   om_free(Ag);
   form_aggr(&s->fin, off_in, n, num_in);
   form_aggr(&s->ffull, off_full, n, num_full);
@@ -2108,14 +2190,28 @@ void or_prec_apply(or_state *s, const double *r, double *z) {
       int m = s->sstarts[ip + 1] - s->sstarts[ip];
       if (m > maxs) maxs = m;
     }
-    double *sr = DALLOC(maxs), *sx = DALLOC(maxs), *w = DALLOC(maxs);
+    /* Solve_subdomains (subsolvers.F90:54-76): the solves side by side, each
+     * into its own slot, then added to z in subdomain order as the
+     * sequential loop adds them */
+    long tot = s->sstarts[s->nsub + 1] - 1;
+    double *sxall = DALLOC(tot);
+#pragma omp parallel
+    {
+      double *sr = DALLOC(maxs), *w = DALLOC(maxs);
+#pragma omp for schedule(dynamic, 1)
+      for (int ip = 1; ip <= s->nsub; ip++) {
+        int st = s->sstarts[ip], m = s->sstarts[ip + 1] - st;
+        for (int k = 0; k < m; k++) sr[k] = r[s->sinds[st + k] - 1];
+        env_solve(s->fact[ip - 1], sr, sxall + st, w);
+      }
+      free(sr); free(w);
+    }
     for (int ip = 1; ip <= s->nsub; ip++) {
       int st = s->sstarts[ip], m = s->sstarts[ip + 1] - st;
-      for (int k = 0; k < m; k++) sr[k] = r[s->sinds[st + k] - 1];
-      env_solve(s->fact[ip - 1], sr, sx, w);
+      const double *sx = sxall + st;
       for (int k = 0; k < m; k++) z[s->sinds[st + k] - 1] = z[s->sinds[st + k] - 1] + sx[k];
     }
-    free(sr); free(sx); free(w);
+    free(sxall);
   }
   if (p->levels > 1 && s->has_coarse) {
     int nc = s->AC->nrows;

@@ -96,6 +96,14 @@ int or_pcg(or_state *s, const double *b, double *x, double tol, int maxit,
            double *cond, double *res_hist);
 
 /* stand-alone pieces used for pinning */
+/* SpMtx_find_strong + SpMtx_aggregate (SpMtx_aggregation.F90:49-641,829-899)
+ * on a given 1-based COO matrix, inside the current process state (see
+ * or_reset_process): has_fine != 0 makes it a coarse-level call.  symm
+ * selects symmetric scaling.  num_inner/num_full receive the n aggregate
+ * numbers (0 = none); returns the inner aggregate count. */
+int or_aggregate(int n, long nnz, const int *indi, const int *indj, const double *val,
+                 int symm, int radius, int minasize, int maxasize, double alpha, int has_fine,
+                 int *num_inner, int *num_full);
 /* CSR SpMV summing each row in stored order, y_i = ((0 + a1*x1) + a2*x2) ... */
 void or_spmv_csr(int n, const int *rowptr0, const int *col0, const double *val,
                  const double *x, double *y);

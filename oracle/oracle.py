@@ -47,44 +47,52 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
-        L.or_params_default.argtypes = [C.POINTER(Params)]
-        L.or_reset_process.argtypes = []
-        L.or_setup.restype = C.c_void_p
-        L.or_setup.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
-                               C.c_int, C.c_void_p, C.c_void_p]
-        L.or_free.argtypes = [C.c_void_p]
-        L.or_free_ptr.argtypes = [C.c_void_p]
-        L.or_setup_ranks.restype = C.c_void_p
-        L.or_setup_ranks.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
-                                     C.c_int, _i32p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.or_last_error.restype = C.c_char_p
-        for name in ("or_n", "or_nagr_fine", "or_nagr_coarse", "or_nsub", "or_overlap",
-                     "or_smoothers"):
-            getattr(L, name).argtypes = [C.c_void_p]
-            getattr(L, name).restype = C.c_int
-        for name in ("or_sub_total", "or_R_nnz", "or_AC_nnz", "or_A_nnz"):
-            getattr(L, name).argtypes = [C.c_void_p]
-            getattr(L, name).restype = C.c_long
-        L.or_get_fine_aggr.argtypes = [C.c_void_p, _i32p, _i32p, _i32p]
-        L.or_get_coarse_aggr.argtypes = [C.c_void_p, _i32p, _i32p, _i32p]
-        L.or_get_subdomains.argtypes = [C.c_void_p, _i32p, _i32p]
-        for name in ("or_get_R", "or_get_AC", "or_get_A"):
-            getattr(L, name).argtypes = [C.c_void_p, _i32p, _i32p, _f64p]
-        L.or_spmv.argtypes = [C.c_void_p, _f64p, _f64p]
-        L.or_prec_apply.argtypes = [C.c_void_p, _f64p, _f64p]
-        L.or_pcg.restype = C.c_int
-        L.or_pcg.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_int,
-                             C.POINTER(C.c_double), _f64p]
-        L.or_spmv_csr.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
-        L.or_lanczos_eigs.restype = C.c_int
-        L.or_lanczos_eigs.argtypes = [C.c_int, _f64p, _f64p, _f64p]
-        L.or_tql1.restype = C.c_int
-        L.or_tql1.argtypes = [C.c_int, _f64p, _f64p]
-        L.or_add_layers.restype = C.c_int
-        L.or_add_layers.argtypes = [_i32p, _i32p, C.c_int, _i32p, C.c_int, C.c_int]
-        _lib = L
+        _lib = bind(C.CDLL(_LIB))
     return _lib
+
+
+def bind(L):
+    """Declares the oracle's entry points on a loaded library (also used for
+    the tests' mutant builds of doug_oracle.c)."""
+    L.or_params_default.argtypes = [C.POINTER(Params)]
+    L.or_reset_process.argtypes = []
+    L.or_setup.restype = C.c_void_p
+    L.or_setup.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
+                           C.c_int, C.c_void_p, C.c_void_p]
+    L.or_free.argtypes = [C.c_void_p]
+    L.or_free_ptr.argtypes = [C.c_void_p]
+    L.or_setup_ranks.restype = C.c_void_p
+    L.or_setup_ranks.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
+                                 C.c_int, _i32p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_last_error.restype = C.c_char_p
+    for name in ("or_n", "or_nagr_fine", "or_nagr_coarse", "or_nsub", "or_overlap",
+                 "or_smoothers"):
+        getattr(L, name).argtypes = [C.c_void_p]
+        getattr(L, name).restype = C.c_int
+    for name in ("or_sub_total", "or_R_nnz", "or_AC_nnz", "or_A_nnz"):
+        getattr(L, name).argtypes = [C.c_void_p]
+        getattr(L, name).restype = C.c_long
+    L.or_get_fine_aggr.argtypes = [C.c_void_p, _i32p, _i32p, _i32p]
+    L.or_get_coarse_aggr.argtypes = [C.c_void_p, _i32p, _i32p, _i32p]
+    L.or_get_subdomains.argtypes = [C.c_void_p, _i32p, _i32p]
+    for name in ("or_get_R", "or_get_AC", "or_get_A"):
+        getattr(L, name).argtypes = [C.c_void_p, _i32p, _i32p, _f64p]
+    L.or_spmv.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.or_prec_apply.argtypes = [C.c_void_p, _f64p, _f64p]
+    L.or_pcg.restype = C.c_int
+    L.or_pcg.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double, C.c_int,
+                         C.POINTER(C.c_double), _f64p]
+    L.or_spmv_csr.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+    L.or_lanczos_eigs.restype = C.c_int
+    L.or_lanczos_eigs.argtypes = [C.c_int, _f64p, _f64p, _f64p]
+    L.or_tql1.restype = C.c_int
+    L.or_tql1.argtypes = [C.c_int, _f64p, _f64p]
+    L.or_add_layers.restype = C.c_int
+    L.or_add_layers.argtypes = [_i32p, _i32p, C.c_int, _i32p, C.c_int, C.c_int]
+    L.or_aggregate.restype = C.c_int
+    L.or_aggregate.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_double, C.c_int, _i32p, _i32p]
+    return L
 
 
 def params(**kw) -> Params:
@@ -241,6 +249,25 @@ def spmv_csr(rowptr, col, val, x):
                       np.ascontiguousarray(val, np.float64),
                       np.ascontiguousarray(x, np.float64), y)
     return y
+
+
+def aggregate(n, indi, indj, val, radius, minasize, maxasize, alpha, coarse=False, symm=True,
+              L=None):
+    """SpMtx_find_strong + SpMtx_aggregate on one matrix inside the current
+    process state (allmind, beta carry over from earlier calls until
+    reset_process()).  Returns (nagr, inner numbers, full numbers)."""
+    L = L or lib()
+    ni = np.zeros(n, np.int32)
+    nf = np.zeros(n, np.int32)
+    na = L.or_aggregate(n, len(val), np.ascontiguousarray(indi, dtype=np.int32),
+                        np.ascontiguousarray(indj, dtype=np.int32),
+                        np.ascontiguousarray(val, dtype=np.float64), int(symm), radius, minasize,
+                        maxasize, alpha, int(coarse), ni, nf)
+    return na, ni, nf
+
+
+def reset_process(L=None):
+    (L or lib()).or_reset_process()
 
 
 def lanczos_eigs(alpha, beta):
