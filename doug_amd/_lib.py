@@ -44,7 +44,7 @@ class DougInfo(C.Structure):
         ("nnz", _i64), ("sub_total", _i64), ("factor_bytes", _i64), ("coarse_factor_bytes", _i64),
         ("t_aggregate", _dbl), ("t_coarse_build", _dbl), ("t_subdomains", _dbl),
         ("t_upload", _dbl), ("t_factor", _dbl), ("t_coarse_factor", _dbl), ("t_total", _dbl),
-        ("factor_tiles", _i64), ("factor_flops", _dbl),
+        ("factor_tiles", _i64), ("factor_flops", _dbl), ("temp_bytes", _i64),
     ]
 
     def as_dict(self):

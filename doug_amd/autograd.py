@@ -23,7 +23,11 @@ from .solver import Solver
 
 
 class PatternSolver:
-    """A Solver bound to one COO pattern; the values change per forward."""
+    """A Solver bound to one COO pattern; the values change per forward.
+
+    `version` counts the value sets taken: a backward pass whose forward ran
+    on other values than the solver now holds sets its own values again
+    before the adjoint solve."""
 
     def __init__(self, n, indi, indj, params: dict, partition=None, precision=0, tol=1e-10,
                  maxit=1000, device=0):
@@ -37,13 +41,16 @@ class PatternSolver:
         self.tol = tol
         self.maxit = maxit
         self.iterations = []
+        self.version = 0
 
-    def set_values(self, values: torch.Tensor):
+    def set_values(self, values: torch.Tensor) -> int:
         v = values.detach().to("cpu", torch.float64).numpy()
         self.solver.set_matrix(self.n, self.indi, self.indj, v, 1)
         if self.partition is not None:
             self.solver.set_partition(*self.partition)
         self.solver.setup(self.precision)
+        self.version += 1
+        return self.version
 
     def solve(self, b: torch.Tensor) -> torch.Tensor:
         b = b.detach().contiguous()
@@ -60,16 +67,20 @@ class PatternSolver:
 class DougSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values: torch.Tensor, b: torch.Tensor, ps: PatternSolver):
-        ps.set_values(values)
+        ctx.version = ps.set_values(values)
         x = ps.solve(b)
         ctx.ps = ps
-        ctx.save_for_backward(x)
+        ctx.save_for_backward(x, values.detach())
         return x
 
     @staticmethod
     def backward(ctx, gx: torch.Tensor):
         ps: PatternSolver = ctx.ps
-        (x,) = ctx.saved_tensors
+        x, values = ctx.saved_tensors
+        if ps.version != ctx.version:
+            # another forward has set other values since: the adjoint needs
+            # this forward's matrix
+            ctx.version = ps.set_values(values)
         lam = ps.solve(gx.contiguous())  # adjoint solve, A^T = A
         gv = torch.empty(len(ps.indi), dtype=torch.float64, device=x.device)
         torch.cuda.synchronize()

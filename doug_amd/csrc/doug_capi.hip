@@ -78,6 +78,7 @@ struct doug_ctx {
   uint8_t* d_strong = nullptr;     // strength mask, uploaded while the aggregation runs
   bool use_device_coarse = true;   // DOUG_OPT_DEVICE_COARSE
   bool use_device_blocks = true;   // DOUG_OPT_DEVICE_BLOCKS
+  bool keep_setup_temps = true;    // DOUG_OPT_KEEP_SETUP_TEMPS
 
   // device: matrix
   std::vector<void*> allocs;
@@ -161,11 +162,26 @@ static int fail(doug_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+static void join_reaper(doug_ctx* ctx);
+
+// hipMalloc; when the device is full while the previous set-up's buffers are
+// still being freed on the reaper thread, waits for that thread and tries
+// once more.
+static hipError_t device_malloc(doug_ctx* ctx, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && ctx->reaper.joinable()) {
+    (void)hipGetLastError();
+    join_reaper(ctx);
+    e = hipMalloc(p, bytes);
+  }
+  return e;
+}
+
 template <class T>
 static int dalloc(doug_ctx* ctx, T** out, size_t count) {
   void* p = nullptr;
   size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&p, bytes);
+  hipError_t e = device_malloc(ctx, &p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     ctx->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
@@ -198,6 +214,20 @@ static void drop_iteration_graph(doug_ctx* ctx) {
 
 static void join_reaper(doug_ctx* ctx) {
   if (ctx->reaper.joinable()) ctx->reaper.join();
+}
+
+// The coarse build's device temporaries (kept between set-ups unless
+// DOUG_OPT_KEEP_SETUP_TEMPS is 0).
+static void free_scratch_slots(doug_ctx* ctx) {
+  for (auto& q : ctx->scratch_slots)
+    if (q.first) (void)hipFree(q.first);
+  ctx->scratch_slots.clear();
+}
+
+static int64_t scratch_slot_bytes(const doug_ctx* ctx) {
+  int64_t b = 0;
+  for (const auto& q : ctx->scratch_slots) b += (int64_t)q.second;
+  return b;
 }
 
 // Drops every device buffer of the last set-up.  In the background (a new
@@ -319,8 +349,7 @@ int doug_ctx_destroy(doug_ctx* ctx) {
   for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
     if (st) (void)hipStreamSynchronize(st);
   free_device(ctx);
-  for (auto& q : ctx->scratch_slots)
-    if (q.first) (void)hipFree(q.first);
+  free_scratch_slots(ctx);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
   if (ctx->h_scalm) (void)hipHostFree(ctx->h_scalm);
   for (hipEvent_t e : ctx->cevents) (void)hipEventDestroy(e);
@@ -910,9 +939,10 @@ static void phase_coarse_host(doug_ctx* ctx, const std::vector<uint8_t>& strong)
 // waits for the device).  The set-up ends with the device idle, so the next
 // one may overwrite them.
 struct DevTemp {
+  doug_ctx* ctx;
   std::vector<std::pair<void*, size_t>>& slots;
   size_t next = 0;
-  explicit DevTemp(std::vector<std::pair<void*, size_t>>& s) : slots(s) {}
+  explicit DevTemp(doug_ctx* c) : ctx(c), slots(c->scratch_slots) {}
   template <typename T>
   hipError_t alloc(T** q, size_t count) {
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
@@ -921,7 +951,7 @@ struct DevTemp {
     if (sl.second < bytes) {
       if (sl.first) (void)hipFree(sl.first);
       sl = {nullptr, 0};
-      hipError_t e = hipMalloc(&sl.first, bytes);
+      hipError_t e = device_malloc(ctx, &sl.first, bytes);
       if (e != hipSuccess) {
         sl.first = nullptr;
         return e;
@@ -939,7 +969,7 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   const int n = A.nrows, nagr = ctx->fin.nagr;
   const int64_t nnz = A.nnz();
   hipStream_t st = ctx->stream;
-  DevTemp tmp(ctx->scratch_slots);
+  DevTemp tmp(ctx);
   int rc;
   void* scan_tmp = nullptr;
   size_t scan_cap = 0;
@@ -1269,7 +1299,10 @@ static int set_trsv_order(doug_ctx* ctx, int order) {
   std::vector<BlockDesc> sorted(ctx->h_sub);
   std::stable_sort(sorted.begin(), sorted.end(),
                    [](const BlockDesc& a, const BlockDesc& b) { return a.nt > b.nt; });
-  return upload(ctx, &ctx->d_sub_trsv, sorted);
+  int rc = upload(ctx, &ctx->d_sub_trsv, sorted);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // `sorted` goes out of scope
+  return DOUG_OK;
 }
 
 // Matrix, vectors, subdomain data and the subdomain factorisation (enqueued).
@@ -1653,8 +1686,15 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   std::thread agg;
   const bool early_agg = ctx->nranks > 0 && ctx->inj_nagr == 0 && !ctx->have_partition;
   if (early_agg) {
-    agg = std::thread([&]() { phase_aggregate(ctx, scaled, strong); });
-    set_setup_thread_cap(std::max(1, setup_threads() / 2));
+    // the host threads are split between the two: the aggregation (the
+    // longer) keeps all but `side` of them, this thread's round trip and
+    // upload take `side`
+    const int all = setup_threads(), side = std::max(1, all / 8);
+    agg = std::thread([&ctx, &scaled, &strong, all, side]() {
+      set_setup_thread_cap(std::max(1, all - side));
+      phase_aggregate(ctx, scaled, strong);
+    });
+    set_setup_thread_cap(side);
   }
   phase_values(ctx, scaled, strong);
   // R = T S^T and A_c on the device for the smoothed prolongator
@@ -1721,6 +1761,8 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   for (auto& e : ev) (void)hipEventDestroy(e);
   ctx->info.t_factor = ms_f * 1e-3;
   ctx->info.t_coarse_factor = ms_c * 1e-3;
+  if (!ctx->keep_setup_temps) free_scratch_slots(ctx);
+  ctx->info.temp_bytes = scratch_slot_bytes(ctx);
   ctx->info.t_total = now() - t0;
   if (flag[0])
     return fail(ctx, DOUG_E_NUMERIC, "subdomain " + std::to_string(flag[0] - 1) + " is not positive definite");
@@ -2321,6 +2363,14 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
     case DOUG_OPT_DEVICE_COARSE:
       ctx->use_device_coarse = value != 0;
       ctx->ready = false;  // takes effect at the next doug_setup
+      return DOUG_OK;
+    case DOUG_OPT_KEEP_SETUP_TEMPS:
+      ctx->keep_setup_temps = value != 0;
+      if (!ctx->keep_setup_temps && ctx->device >= 0) {
+        (void)hipSetDevice(ctx->device);
+        free_scratch_slots(ctx);
+        ctx->info.temp_bytes = 0;
+      }
       return DOUG_OK;
     case DOUG_OPT_TRSV_ORDER:
       if (value != 0 && value != 1) return fail(ctx, DOUG_E_ARG, "subdomain order must be 0 or 1");

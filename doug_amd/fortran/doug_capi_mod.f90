@@ -22,6 +22,7 @@ module doug_capi
     real(c_double) :: t_coarse_factor, t_total
     integer(c_int64_t) :: factor_tiles
     real(c_double) :: factor_flops
+    integer(c_int64_t) :: temp_bytes
   end type doug_info
 
   type, bind(C) :: doug_tuning

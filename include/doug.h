@@ -73,6 +73,9 @@ typedef struct {
   double t_total;
   int64_t factor_tiles;
   double factor_flops;
+  /* device bytes the context keeps after the set-up for the next one (the
+   * coarse build's temporaries; 0 with DOUG_OPT_KEEP_SETUP_TEMPS 0) */
+  int64_t temp_bytes;
 } doug_info;
 
 /* Context.  Replaces the process-wide state DOUG_Init sets up
@@ -264,7 +267,11 @@ int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
  * DOUG_OPT_DEVICE_BLOCKS (default 1): take the subdomain factors' tile
  * envelopes and entries on the GPU from the uploaded matrix (same tiles as
  * the host scatter map, which 0 selects; a matrix holding a duplicate
- * (i, j) pair always uses the host map); at the next doug_setup. */
+ * (i, j) pair always uses the host map); at the next doug_setup.
+ * DOUG_OPT_KEEP_SETUP_TEMPS (default 1): keep the coarse build's device
+ * temporaries between set-ups (doug_info.temp_bytes), so that a repeated
+ * set-up of the same size neither allocates nor frees them; 0 frees them
+ * now and at the end of every set-up. */
 enum {
   DOUG_OPT_ITERATION_GRAPH = 1,
   DOUG_OPT_SPMV_SHAPE = 2,
@@ -272,7 +279,8 @@ enum {
   DOUG_OPT_LOCAL_ORDER = 4,
   DOUG_OPT_DEVICE_COARSE = 5,
   DOUG_OPT_BATCH_TILES = 6,
-  DOUG_OPT_DEVICE_BLOCKS = 7
+  DOUG_OPT_DEVICE_BLOCKS = 7,
+  DOUG_OPT_KEEP_SETUP_TEMPS = 8
 };
 int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 
@@ -285,10 +293,13 @@ int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 int doug_value_grad_device(doug_ctx *ctx, const double *dlam, const double *dx, double *dgrad);
 
 /* Several right-hand sides in one pass.  After doug_setup, doug_prepare_batch
- * forms the explicit inverse factors W_i = L_i^{-1} of every subdomain on the
- * matrix cores; each preconditioner apply is then one gather, two batched
- * tile GEMMs (W_i, W_i^T) and one overlap sum for all columns, reading each
- * factor tile once per apply whatever the number of columns.  Multi-vectors
+ * allocates the multi-vector workspaces; the subdomain factors are the
+ * single-column path's tile envelopes (L_i below the diagonal, inv(L_ii) on
+ * it).  Each preconditioner apply is then one multi-vector restriction, two
+ * chunked GEMMs on the coarse inverse factor, one launch of the batched
+ * forward/backward envelope solve for every subdomain and all columns (tile x
+ * panel products on the matrix cores, each factor tile read once per apply
+ * whatever the number of columns) and one overlap sum.  Multi-vectors
  * are node-major: entry (i, r) at i*nrhs + r; 1 <= nrhs <= 32.  Every column
  * runs pcg_weigs's recurrence (pcg.F90:387-435) and stops on its own test;
  * its[r] receives the iteration count of column r, res_hist (optional,

@@ -66,6 +66,32 @@ def test_gradient_matches_dense_reference():
     ps.close()
 
 
+def test_two_solves_in_one_graph_match_dense_reference():
+    # two forwards on one PatternSolver before one backward: each adjoint
+    # solve runs on its own forward's matrix
+    N = 16
+    torch, ps, theta, theta_t, b, n, indi, indj = _setup(N)
+    print(f"two solves in one graph n={n}", flush=True)
+    from doug_amd.autograd import DougSolve, diffusion3d_values
+
+    with torch.no_grad():
+        x_target = DougSolve.apply(diffusion3d_values(theta_t, N), b, ps)
+    th = theta.clone().requires_grad_(True)
+    loss = _loss_doug(torch, ps, th, b, x_target, N) + \
+        _loss_doug(torch, ps, 0.5 * th, b, x_target, N)
+    loss.backward()
+    g = th.grad.cpu()
+    th_c = theta.cpu().clone().requires_grad_(True)
+    loss_d = _loss_dense(torch, th_c, b.cpu(), x_target.cpu(), N, indi, indj) + \
+        _loss_dense(torch, 0.5 * th_c, b.cpu(), x_target.cpu(), N, indi, indj)
+    loss_d.backward()
+    gd = th_c.grad
+    rel = float(torch.linalg.norm(g - gd) / torch.linalg.norm(gd))
+    print(f"relative gradient error vs dense FP64: {rel:.2e}")
+    assert rel <= 1e-6
+    ps.close()
+
+
 def test_gradient_matches_central_differences():
     N = 16
     torch, ps, theta, theta_t, b, n, indi, indj = _setup(N)
