@@ -901,11 +901,12 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
     if (rows_sorted_by_column(ctx->A)) {
       virtual_ranks(ctx->A, ctx->A_input_val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol,
                     ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup,
-                    rcm, &ctx->rank_ws);
+                    rcm, &ctx->rank_ws, true);
     } else {
       Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
       virtual_ranks(As, As.val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains,
-                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup, rcm, &ctx->rank_ws);
+                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup, rcm, &ctx->rank_ws, true,
+                    &As.src);
     }
     ctx->fin = std::move(ctx->rsetup.inner);
     ctx->ffull = std::move(ctx->rsetup.full);
@@ -915,6 +916,25 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
   }
   ctx->info.t_aggregate = now() - t0;
 }
+
+// several processes: each aggregate's row of R as its owner builds it and
+// A_c as the sum of the processes' parts (ranks_coarse_space), from the
+// local matrices the rank workers kept
+static void phase_coarse_ranks(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
+  double t1 = now();
+  ranks_coarse_space(ctx->rsetup, ctx->A.nrows, ctx->A.val, ctx->A_input_val, strong,
+                     ctx->smoothers, ctx->p.omega, ctx->R, ctx->Rt, ctx->AC);
+  for (RankLocal& L : ctx->rsetup.ranks) {
+    L.A = Csr();
+    L.agg = Aggregates();
+  }
+  ctx->coarse_on_device = false;
+  ctx->info.t_coarse_build = now() - t1;
+}
+
+// whether the last aggregation was the virtual ranks' (their local matrices
+// are kept for the coarse space)
+static bool rank_coarse(const doug_ctx* ctx) { return ctx->nranks > 0 && ctx->inj_nagr == 0; }
 
 // host build of R, R^T and A_c (the host-only setup and smoothers = 0)
 static void phase_coarse_host(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
@@ -1282,7 +1302,10 @@ static int setup_host(doug_ctx* ctx) {
   std::vector<uint8_t> strong;
   phase_values(ctx, scaled, strong);
   phase_aggregate(ctx, scaled, strong);
-  phase_coarse_host(ctx, strong);
+  if (rank_coarse(ctx))
+    phase_coarse_ranks(ctx, strong);
+  else
+    phase_coarse_host(ctx, strong);
   phase_subdomains(ctx);
   fill_info(ctx);
   ctx->info.t_total = now() - t0;
@@ -1699,6 +1722,10 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   phase_values(ctx, scaled, strong);
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
+    if (rank_coarse(ctx)) {
+      phase_coarse_ranks(ctx, strong);
+      return DOUG_OK;
+    }
     if (ctx->smoothers == 1 && ctx->use_device_coarse) return coarse_build_device(ctx, strong);
     phase_coarse_host(ctx, strong);
     return DOUG_OK;

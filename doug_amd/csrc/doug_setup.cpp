@@ -717,15 +717,17 @@ Csr transpose(const Csr& A) {
 // T ((SS SS)(SS SS)) for 8, every product in first-touch order.  Rows of the
 // result are returned sorted by column, as the later products use them.
 static Csr restriction_powers(const Csr& A, const std::vector<uint8_t>& strong,
-                              const Aggregates& ag, int smoothers, double omega) {
+                              const Aggregates& ag, int smoothers, double omega,
+                              const std::vector<uint8_t>* ghost) {
   const int n = A.nrows;
   std::vector<double> d1((size_t)n, 0.0), d2((size_t)n, 0.0);
   parallel_rows(n, [&](int i) {
+    const bool gh = ghost && (*ghost)[i];
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
       if (A.col[p] == i) {
-        d1[i] = d1[i] + A.val[p];
+        d1[i] = gh ? A.val[p] : d1[i] + A.val[p];
         d2[i] = A.val[p];
-      } else if (!strong[p]) {
+      } else if (!strong[p] && !gh) {
         d1[i] = d1[i] + A.val[p];
         d2[i] = d2[i] + A.val[p];
       }
@@ -739,9 +741,10 @@ static Csr restriction_powers(const Csr& A, const std::vector<uint8_t>& strong,
     S.val.resize(A.val.size());
     const double one_minus = 1.0 - omega;
     parallel_rows(n, [&](int i) {
+      const bool gh = ghost && (*ghost)[i];
       for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
         if (A.col[p] == i)
-          S.val[p] = one_minus;
+          S.val[p] = gh ? 1.0 - omega / diag[i] * A.val[p] : one_minus;
         else if (strong[p])
           S.val[p] = -omega / diag[i] * A.val[p];
         else
@@ -777,7 +780,7 @@ static Csr restriction_powers(const Csr& A, const std::vector<uint8_t>& strong,
 }
 
 Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregates& ag,
-                int smoothers, double omega) {
+                int smoothers, double omega, const std::vector<uint8_t>* ghost) {
   const int n = A.nrows, nagr = ag.nagr;
   Csr R;
   R.nrows = nagr;
@@ -788,12 +791,20 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
     R.val.assign(R.col.size(), 1.0);
     return R;
   }
-  if (smoothers >= 2) return restriction_powers(A, strong, ag, smoothers, omega);
-  // filtered diagonal and the Jacobi smoother S (CoarseMtx.F90:152-233)
+  if (smoothers >= 2) return restriction_powers(A, strong, ag, smoothers, omega, ghost);
+  // filtered diagonal and the Jacobi smoother S (CoarseMtx.F90:152-233); an
+  // A_ghost row takes its diagonal entry (:164-170) and keeps a_ii/a_ii in
+  // its diagonal smoother entry (:200-205)
   std::vector<double> diag((size_t)n, 0.0);
   parallel_rows(n, [&](int i) {
-    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
-      if (A.col[p] == i || !strong[p]) diag[i] = diag[i] + A.val[p];
+    const bool gh = ghost && (*ghost)[i];
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
+      if (gh) {
+        if (A.col[p] == i) diag[i] = A.val[p];
+      } else if (A.col[p] == i || !strong[p]) {
+        diag[i] = diag[i] + A.val[p];
+      }
+    }
   });
   Csr S;
   S.nrows = S.ncols = n;
@@ -802,9 +813,10 @@ Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregat
   S.val.resize(A.val.size());
   const double one_minus = 1.0 - omega;
   parallel_rows(n, [&](int i) {
+    const bool gh = ghost && (*ghost)[i];
     for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) {
       if (A.col[p] == i)
-        S.val[p] = one_minus;
+        S.val[p] = gh ? 1.0 - omega / diag[i] * A.val[p] : one_minus;
       else if (strong[p])
         S.val[p] = -omega / diag[i] * A.val[p];
       else
@@ -1329,22 +1341,27 @@ Csr sort_rows_by_column(const Csr& A, const std::vector<double>& val) {
   S.ptr = A.ptr;
   S.col = A.col;
   S.val = val;
+  S.src.resize(A.col.size());
   // rows are short: a stable insertion sort in place (strict <), nothing to
-  // do for rows already in column order
+  // do for rows already in column order; src follows each entry
   parallel_rows(A.nrows, [&](int i) {
     int* c = S.col.data();
     double* x = S.val.data();
+    int* o = S.src.data();
+    for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++) o[p] = p;
     for (int p = A.ptr[i] + 1; p < A.ptr[i + 1]; p++) {
       if (c[p - 1] <= c[p]) continue;
-      const int ck = c[p];
+      const int ck = c[p], ok = o[p];
       const double xk = x[p];
       int q = p;
       for (; q > A.ptr[i] && c[q - 1] > ck; q--) {
         c[q] = c[q - 1];
         x[q] = x[q - 1];
+        o[q] = o[q - 1];
       }
       c[q] = ck;
       x[q] = xk;
+      o[q] = ok;
     }
   });
   return S;
@@ -1420,7 +1437,8 @@ void rank_labels(const Csr& As, const std::vector<int>& eptn, const std::vector<
 
 // Labels: bit 0 inner overlap of some neighbour, bit 1 sent to some neighbour.
 void rank_local(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn, int r,
-                const std::vector<int>& own, int ol, RankScratch& S, RankLocal& out) {
+                const std::vector<int>& own, int ol, RankScratch& S, RankLocal& out, bool ghost,
+                const int* pos) {
   const int L = std::max(ol, 1), D = 2 * L;
   std::vector<int> touched(own.begin(), own.end());
   for (int v : own) S.nd[v].layer = 0;
@@ -1472,6 +1490,7 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
   A.ptr.assign((size_t)nl + 1, 0);
   A.col.clear();
   A.val.clear();
+  A.src.clear();
   for (int k = 0; k < nl; k++) {
     const int v = out.lg[k];
     if (S.nd[v].layer < L) {
@@ -1485,9 +1504,33 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
           if (sent != (pass == 0)) continue;
           A.col.push_back(S.nd[w].gl);
           A.val.push_back(val[p]);
+          A.src.push_back(pos ? pos[p] : p);
         }
     }
     A.ptr[k + 1] = (int)A.col.size();
+  }
+  // A_ghost (SpMtx_build_ghost, SpMtx_arrangement.F90:1590-1612): the
+  // received nodes' rows, entries to nodes up to layer L, values as read;
+  // the entry order inside a ghost row decides no result
+  out.nrecv = (int)grp[4].size();
+  out.G = Csr();
+  if (ghost) {
+    Csr& G = out.G;
+    G.nrows = out.nrecv;
+    G.ncols = nl;
+    G.ptr.assign((size_t)out.nrecv + 1, 0);
+    for (int k = 0; k < out.nrecv; k++) {
+      const int v = grp[4][k];
+      for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
+        const int w = As.col[p];
+        const int lw = S.nd[w].layer;
+        if (lw < 0 || lw > L) continue;
+        G.col.push_back(S.nd[w].gl);
+        G.val.push_back(val[p]);
+        G.src.push_back(pos ? pos[p] : p);
+      }
+      G.ptr[k + 1] = (int)G.col.size();
+    }
   }
   for (int v : touched) {
     S.nd[v].layer = -1;
@@ -1502,7 +1545,8 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
 void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
                    int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
-                   RankSetup& out, const Csr* rcm_graph, RankWorkspaces* ws) {
+                   RankSetup& out, const Csr* rcm_graph, RankWorkspaces* ws, bool keep_local,
+                   const std::vector<int>* pos) {
   const int n = As.nrows;
   std::vector<std::vector<int>> own((size_t)nranks);
   {
@@ -1538,7 +1582,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
       RankScratch& S = *sp;
       for (int r = t; r < nranks; r += nw) {
         RankLocal& R = out.ranks[r];
-        rank_local(As, val, eptn, r, own[r], ol_dist, S, R);
+        rank_local(As, val, eptn, r, own[r], ol_dist, S, R, keep_local, pos ? pos->data() : nullptr);
         // getLocal (Aggregate_utils.f95:33-51): the own rows; a new matrix
         // carries no symmetry flags, so the strength test scales by rows
         Csr LA;
@@ -1593,7 +1637,19 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
         add_layers_all(R.A, ps2, pnd, ol_sub, sst[r], sin[r]);
         for (int& v : sin[r]) v = R.lg[v];
         if (rcm_graph) block_rcm_serial(*rcm_graph, sst[r], sin[r], dsin[r], S.floc);
-        R.A = Csr();  // only the numbering is kept
+        if (keep_local) {
+          // the outer layer's rows are the last ones: A_ghost fills them
+          const int first = R.A.nrows - R.nrecv;
+          const int base = R.A.ptr[first];
+          R.A.col.insert(R.A.col.end(), R.G.col.begin(), R.G.col.end());
+          R.A.val.insert(R.A.val.end(), R.G.val.begin(), R.G.val.end());
+          R.A.src.insert(R.A.src.end(), R.G.src.begin(), R.G.src.end());
+          for (int k = 0; k < R.nrecv; k++) R.A.ptr[first + k + 1] = base + R.G.ptr[k + 1];
+          R.G = Csr();
+          R.agg = inner[r];
+        } else {
+          R.A = Csr();  // only the numbering is kept
+        }
       }
       set_setup_thread_cap(0);
     });
@@ -1665,6 +1721,232 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
     g.nodes.resize((size_t)tot);
     g.starts[g.nagr] = tot;
   }
+}
+
+void ranks_coarse_space(const RankSetup& rs, int n, const std::vector<double>& val_rt,
+                        const std::vector<double>& val_in, const std::vector<uint8_t>& strong,
+                        int smoothers, double omega, Csr& R, Csr& Rt, Csr& AC) {
+  const int nranks = (int)rs.ranks.size();
+  std::vector<int> off((size_t)nranks + 1, 0);
+  for (int r = 0; r < nranks; r++) off[r + 1] = off[r] + rs.ranks[r].agg.nagr;
+  const int nagr = off[nranks];
+  const int nw = std::max(1, std::min(nranks, setup_threads()));
+  // (1) every rank's rows of R: IntRestBuild on its matrix with A_ghost's
+  // rows, for its own aggregates, in its local numbering
+  std::vector<Csr> Rp((size_t)nranks);
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nw; t++)
+      th.emplace_back([&, t]() {
+        set_setup_thread_cap(1);
+        for (int r = t; r < nranks; r += nw) {
+          const RankLocal& L = rs.ranks[r];
+          const int nl = L.A.nrows, first_ghost = nl - L.nrecv;
+          Csr M;
+          M.nrows = M.ncols = nl;
+          M.ptr = L.A.ptr;
+          M.col = L.A.col;
+          M.val.resize(L.A.col.size());
+          std::vector<uint8_t> st(L.A.col.size()), gh((size_t)nl, 0);
+          for (int i = 0; i < nl; i++) {
+            const bool g = i >= first_ghost;
+            gh[i] = g;
+            for (int p = M.ptr[i]; p < M.ptr[i + 1]; p++) {
+              const int e = L.A.src[p];
+              M.val[p] = g ? val_in[e] : val_rt[e];
+              st[p] = strong[e];
+            }
+          }
+          Aggregates ag = L.agg;
+          ag.num.resize((size_t)nl, 0);
+          Rp[r] = restriction(M, st, ag, smoothers, omega, &gh);
+        }
+        set_setup_thread_cap(0);
+      });
+    for (auto& x : th) x.join();
+  }
+  // (2) the global R: rank r's rows are aggregates off[r].., columns mapped
+  // to global numbers and sorted
+  R = Csr();
+  R.nrows = nagr;
+  R.ncols = n;
+  R.ptr.assign((size_t)nagr + 1, 0);
+  for (int r = 0; r < nranks; r++)
+    for (int a = 0; a < Rp[r].nrows; a++) R.ptr[off[r] + a + 1] = Rp[r].ptr[a + 1] - Rp[r].ptr[a];
+  for (int a = 0; a < nagr; a++) R.ptr[a + 1] += R.ptr[a];
+  R.col.resize((size_t)R.ptr[nagr]);
+  R.val.resize((size_t)R.ptr[nagr]);
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nw; t++)
+      th.emplace_back([&, t]() {
+        std::vector<std::pair<int, double>> row;
+        for (int r = t; r < nranks; r += nw) {
+          const Csr& P = Rp[r];
+          const std::vector<int>& lg = rs.ranks[r].lg;
+          for (int a = 0; a < P.nrows; a++) {
+            row.clear();
+            for (int p = P.ptr[a]; p < P.ptr[a + 1]; p++) row.push_back({lg[P.col[p]], P.val[p]});
+            std::sort(row.begin(), row.end(),
+                      [](const auto& x, const auto& y) { return x.first < y.first; });
+            int q = R.ptr[off[r] + a];
+            for (const auto& e : row) {
+              R.col[q] = e.first;
+              R.val[q] = e.second;
+              q++;
+            }
+          }
+          Rp[r] = Csr();
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  Rt = transpose(R);
+  // (3) each rank's part of A_c: T = A R^T on its own rows (its entry
+  // order), then R(:, own) T with the own nodes in local order
+  struct Part {
+    std::vector<int> row, col;
+    std::vector<double> val;
+  };
+  std::vector<Part> parts((size_t)nranks);
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nw; t++)
+      th.emplace_back([&, t]() {
+        std::vector<int> mark((size_t)nagr, -1), pos((size_t)nagr, 0);
+        std::vector<int> tptr, tcol, cnt, rtp, rti;
+        std::vector<double> tval, rtv;
+        for (int r = t; r < nranks; r += nw) {
+          const RankLocal& L = rs.ranks[r];
+          const int ni = L.ninner;
+          // T rows of the own nodes (local order); first touch assigns
+          tptr.assign((size_t)ni + 1, 0);
+          tcol.clear();
+          tval.clear();
+          for (int i = 0; i < ni; i++) {
+            const int start = (int)tcol.size();
+            for (int p = L.A.ptr[i]; p < L.A.ptr[i + 1]; p++) {
+              const double aij = val_rt[L.A.src[p]];
+              const int j = L.lg[L.A.col[p]];
+              for (int q = Rt.ptr[j]; q < Rt.ptr[j + 1]; q++) {
+                const int b = Rt.col[q];
+                const double prod = aij * Rt.val[q];
+                if (mark[b] != i || pos[b] < start) {
+                  mark[b] = i;
+                  pos[b] = (int)tcol.size();
+                  tcol.push_back(b);
+                  tval.push_back(prod);
+                } else {
+                  tval[pos[b]] = tval[pos[b]] + prod;
+                }
+              }
+            }
+            tptr[i + 1] = (int)tcol.size();
+          }
+          for (int i = 0; i < ni; i++)
+            for (int q = tptr[i]; q < tptr[i + 1]; q++) mark[tcol[q]] = -1;
+          // R(:, own): the coarse rows reaching own nodes, each row's nodes
+          // in local order
+          std::vector<int> rows;
+          for (int i = 0; i < ni; i++) {
+            const int g = L.lg[i];
+            for (int q = Rt.ptr[g]; q < Rt.ptr[g + 1]; q++)
+              if (mark[Rt.col[q]] == -1) {
+                mark[Rt.col[q]] = -2;
+                rows.push_back(Rt.col[q]);
+              }
+          }
+          for (size_t k = 0; k < rows.size(); k++) pos[rows[k]] = (int)k;
+          cnt.assign(rows.size() + 1, 0);
+          for (int i = 0; i < ni; i++) {
+            const int g = L.lg[i];
+            for (int q = Rt.ptr[g]; q < Rt.ptr[g + 1]; q++) cnt[pos[Rt.col[q]] + 1]++;
+          }
+          for (size_t k = 0; k < rows.size(); k++) cnt[k + 1] += cnt[k];
+          rtp.assign(cnt.begin(), cnt.end());
+          rti.assign((size_t)cnt.back(), 0);
+          rtv.assign((size_t)cnt.back(), 0.0);
+          for (int i = 0; i < ni; i++) {
+            const int g = L.lg[i];
+            for (int q = Rt.ptr[g]; q < Rt.ptr[g + 1]; q++) {
+              const int k = pos[Rt.col[q]];
+              rti[cnt[k]] = i;
+              rtv[cnt[k]] = Rt.val[q];
+              cnt[k]++;
+            }
+          }
+          for (int a : rows) mark[a] = -1;
+          // LAC = R(:, own) T, rows in the order found, columns first touch
+          Part& out = parts[r];
+          for (size_t k = 0; k < rows.size(); k++) {
+            const int a = rows[k];
+            const int start = (int)out.col.size();
+            for (int p = rtp[k]; p < rtp[k + 1]; p++) {
+              const int i = rti[p];
+              const double rai = rtv[p];
+              for (int q = tptr[i]; q < tptr[i + 1]; q++) {
+                const int b = tcol[q];
+                const double prod = rai * tval[q];
+                if (mark[b] != (int)k || pos[b] < start) {
+                  mark[b] = (int)k;
+                  pos[b] = (int)out.col.size();
+                  out.row.push_back(a);
+                  out.col.push_back(b);
+                  out.val.push_back(prod);
+                } else {
+                  out.val[pos[b]] = out.val[pos[b]] + prod;
+                }
+              }
+            }
+            for (size_t q = start; q < out.col.size(); q++) mark[out.col[q]] = -1;
+          }
+        }
+      });
+    for (auto& x : th) x.join();
+  }
+  // (4) gathered in rank order and consolidated: entries of a row sorted by
+  // column (stable), equal ones added in rank order
+  AC = Csr();
+  AC.nrows = AC.ncols = nagr;
+  std::vector<int> rcnt((size_t)nagr + 1, 0);
+  for (const Part& P : parts)
+    for (int a : P.row) rcnt[a + 1]++;
+  for (int a = 0; a < nagr; a++) rcnt[a + 1] += rcnt[a];
+  std::vector<int> gcol((size_t)rcnt[nagr]);
+  std::vector<double> gval((size_t)rcnt[nagr]);
+  {
+    std::vector<int> fill(rcnt.begin(), rcnt.end() - 1);
+    for (const Part& P : parts)
+      for (size_t k = 0; k < P.row.size(); k++) {
+        const int q = fill[P.row[k]]++;
+        gcol[q] = P.col[k];
+        gval[q] = P.val[k];
+      }
+  }
+  parts.clear();
+  std::vector<CsrPiece> pieces((size_t)nw);
+  parallel_chunks(nagr, nw, [&](int t, int b, int e) {
+    CsrPiece& out = pieces[t];
+    std::vector<std::pair<int, double>> row;
+    for (int a = b; a < e; a++) {
+      row.clear();
+      for (int q = rcnt[a]; q < rcnt[a + 1]; q++) row.push_back({gcol[q], gval[q]});
+      std::stable_sort(row.begin(), row.end(),
+                       [](const auto& x, const auto& y) { return x.first < y.first; });
+      int len = 0;
+      for (size_t q = 0; q < row.size();) {
+        double v = row[q].second;
+        size_t u = q + 1;
+        for (; u < row.size() && row[u].first == row[q].first; u++) v = v + row[u].second;
+        out.col.push_back(row[q].first);
+        out.val.push_back(v);
+        len++;
+        q = u;
+      }
+      out.rowlen.push_back(len);
+    }
+  });
+  join_pieces(pieces, nagr, AC);
 }
 
 }  // namespace doug

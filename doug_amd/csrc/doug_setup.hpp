@@ -78,8 +78,10 @@ void aggregate(const Csr& A, const std::vector<double>& scaled, const std::vecto
 // reference falls back to 2).  Returned as CSR with each row sorted by
 // column: CoarseMtxBuild arranges R by columns in place, and every later
 // use of R (the Galerkin product, prec2Level) sees that order.
+// ghost (optional, per row): rows of a process's A_ghost, smoothed with
+// their diagonal entry in place of the filtered diagonal (CoarseMtx.F90:164-220).
 Csr restriction(const Csr& A, const std::vector<uint8_t>& strong, const Aggregates& inner,
-                int smoothers, double omega);
+                int smoothers, double omega, const std::vector<uint8_t>* ghost = nullptr);
 
 // CoarseMtxBuild (CoarseMtx.F90:476-516): A_c = R (A R^T), rows in the
 // reference's first-touch column order; Rt = transpose(R).
@@ -144,7 +146,14 @@ void aggregate_graph(const Csr& A, const Aggregates& inner, const Aggregates& fu
 struct RankLocal {
   int ninner = 0;
   std::vector<int> lg;  // local -> global (0-based)
-  Csr A;                // nrows = ncols = lg.size(); rows of the outer layer are empty
+  // nrows = ncols = lg.size(); rows of the outer layer are empty.  Kept by
+  // virtual_ranks with `keep_local` only: then the outer layer's rows hold
+  // A_ghost's entries (values as read) and A.src gives every entry's
+  // position in the assembled matrix the caller arranged.
+  Csr A;
+  int nrecv = 0;        // the outer layer: the last nrecv local nodes
+  Csr G;                // A_ghost rows during the set-up
+  Aggregates agg;       // own aggregates in local numbering (keep_local)
 };
 
 // Per-rank results of the decomposition: each rank aggregates its own rows
@@ -175,14 +184,39 @@ struct RankWorkspaces {
 };
 
 // With `rcm_graph`, each rank also orders its subdomains by block_rcm over
-// that graph (out.dinds), inside the rank's worker.
+// that graph (out.dinds), inside the rank's worker.  With `keep_local` every
+// rank keeps its local matrix with A_ghost's rows and its own aggregates
+// (ranks_coarse_space's input); `pos` (optional) gives the position of each
+// of As's entries in the caller's matrix, recorded in RankLocal::A.src (As's
+// own positions without it), e.g. sort_rows_by_column's src.
 void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vector<int>& eptn,
                    int nranks, int ol_dist, int ol_sub,
                    int nsub_per_rank, int radius1, int minasize1, int maxasize1, double strong1,
-                   RankSetup& out, const Csr* rcm_graph = nullptr, RankWorkspaces* ws = nullptr);
+                   RankSetup& out, const Csr* rcm_graph = nullptr, RankWorkspaces* ws = nullptr,
+                   bool keep_local = false, const std::vector<int>* pos = nullptr);
+
+// The coarse space of a DOUG run on several processes
+// (CoarsePreconditioner_smooth_Init, CoarsePreconditioner_smooth.F90:47-89,
+// and the coarse matrix exchange, pcg.F90:78-93, CoarseAllgathers.F90:
+// 175-257), from virtual_ranks' kept local matrices:
+//  - each aggregate's row of R as its owner builds it (IntRestBuild with
+//    A_ghost, CoarseMtx.F90:50-474): rows of the rank's matrix smooth with
+//    the filtered diagonal summed in the rank's entry order and the values
+//    after the scaling round trip, A_ghost's rows with their diagonal entry
+//    and the values as read; the nodes of an aggregate in the owner's local
+//    order;
+//  - A_c as the sum over the ranks, in rank order, of R(:, own) (A R^T)(own, :)
+//    (CoarseMtxBuild, CoarseMtx.F90:476-516, then SpMtx_consolidate),
+//    each rank's products in its entry orders; rows sorted by column.
+// val_rt / val_in: the caller's matrix values after the round trip and as
+// read, strong: its strength flags (symmetric scaling), all indexed by the
+// positions in RankLocal::A.src.  R rows ascending in column, Rt = R^T.
+void ranks_coarse_space(const RankSetup& rs, int n, const std::vector<double>& val_rt,
+                        const std::vector<double>& val_in, const std::vector<uint8_t>& strong,
+                        int smoothers, double omega, Csr& R, Csr& Rt, Csr& AC);
 
 // Stable per-row sort by column (SpMtx_arrange with sort, SpMtx_arrangement.F90:159-227),
-// carrying `val`.
+// carrying `val`; src receives each entry's position in A.
 Csr sort_rows_by_column(const Csr& A, const std::vector<double>& val);
 // Whether every row of A already lists its columns in non-decreasing order.
 bool rows_sorted_by_column(const Csr& A);

@@ -39,13 +39,18 @@
  *   5 repeated next-good-starts not preferred (always the first)
  *   6 merge test `maxconnweightsum >= alpha` written as `>`
  *   7 node hand-over compares |a_ij| with alpha instead of beta
- *   8 rounding fit check `nneigs+ninnodes+1 > maxasize` written as `>=` */
+ *   8 rounding fit check `nneigs+ninnodes+1 > maxasize` written as `>=`
+ *   9 several processes: A_ghost's rows smoothed with the filtered
+ *     diagonal of the rank's own rows
+ *  10 several processes: the gathered coarse matrix consolidated with its
+ *     duplicates dropped instead of added */
 #ifndef OR_MUTANT
 #define OR_MUTANT 0
 #endif
 
 #include <limits.h>
 #include <math.h>
+#include <omp.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -67,6 +72,7 @@ static void seterr(const char *fmt, ...) {
   va_end(ap);
 }
 const char *or_last_error(void) { return g_err; }
+int or_num_threads(void) { return omp_get_max_threads(); }
 
 static void *xcalloc(size_t n, size_t sz) {
   void *p = calloc(n ? n : 1, sz);
@@ -969,11 +975,14 @@ static void om_keep_columns_upto(OM *A, int ninner) {
 }
 
 /* IntRestBuild (CoarseMtx.F90:50-474) for smoothers 0 to 8 (above 8 the
- * reference warns and builds 2). */
-static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
+ * reference warns and builds 2).  G: the process's A_ghost (NULL with one
+ * process): its rows take the diagonal entry as their smoothing diagonal
+ * and enter S and S2 after A's entries (CoarseMtx.F90:164-220,246-306). */
+static OM *int_rest_build(OM *A, OM *G, const Aggrs *ag, int smoothers, double omega) {
   int nagr = ag->nagr;
   long nz = ag->starts[nagr + 1] - 1;
   int Tnrows = A->nrows;
+  if (G && G->nrows > Tnrows) Tnrows = G->nrows;
   OM *T = om_new(nz, nagr, Tnrows);
   for (int i = 1; i <= nagr; i++)
     for (int j = ag->starts[i]; j <= ag->starts[i + 1] - 1; j++) T->indi[j] = i;
@@ -988,6 +997,11 @@ static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
   if (smoothers == 0) return T;
 
   int nd = A->nrows > A->ncols ? A->nrows : A->ncols;
+  if (G) {
+    if (G->nrows > nd) nd = G->nrows;
+    if (G->ncols > nd) nd = G->ncols;
+  }
+  long gnz = G ? G->nnz : 0;
   double *diag = DALLOC(nd);
   for (long i = 1; i <= A->nnz; i++) {
     if (A->indi[i] == A->indj[i])
@@ -995,7 +1009,14 @@ static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
     else if (!A->strong[i])
       diag[A->indi[i]] = diag[A->indi[i]] + A->val[i];
   }
-  OM *S = om_new(A->nnz, Tnrows, Tnrows);
+  for (long i = 1; i <= gnz; i++) {
+    if (OR_MUTANT == 9) {
+      if (G->indi[i] == G->indj[i] || !G->strong[i]) diag[G->indi[i]] = diag[G->indi[i]] + G->val[i];
+    } else if (G->indi[i] == G->indj[i]) {
+      diag[G->indi[i]] = G->val[i];
+    }
+  }
+  OM *S = om_new(A->nnz + gnz, Tnrows, Tnrows);
   double one_minus = 1.0 - omega;
   for (long i = 1; i <= A->nnz; i++) {
     S->indi[i] = A->indi[i];
@@ -1006,6 +1027,17 @@ static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
       S->val[i] = -omega / diag[A->indi[i]] * A->val[i];
     else
       S->val[i] = 0.0;
+  }
+  for (long i = 1; i <= gnz; i++) {
+    long t = A->nnz + i;
+    S->indi[t] = G->indi[i];
+    S->indj[t] = G->indj[i];
+    if (G->indi[i] == G->indj[i])
+      S->val[t] = 1.0 - omega / diag[G->indi[i]] * G->val[i];
+    else if (G->strong[i])
+      S->val[t] = -omega / diag[G->indi[i]] * G->val[i];
+    else
+      S->val[t] = 0.0;
   }
   S->arr = ARR_NO;
   free(diag);
@@ -1025,7 +1057,9 @@ static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
     else if (!A->strong[i])
       diag2[A->indi[i]] = diag2[A->indi[i]] + A->val[i];
   }
-  OM *S2 = om_new(A->nnz, Tnrows, Tnrows);
+  for (long i = 1; i <= gnz; i++)
+    if (G->indi[i] == G->indj[i]) diag2[G->indi[i]] = G->val[i];
+  OM *S2 = om_new(A->nnz + gnz, Tnrows, Tnrows);
   for (long i = 1; i <= A->nnz; i++) {
     S2->indi[i] = A->indi[i];
     S2->indj[i] = A->indj[i];
@@ -1035,6 +1069,17 @@ static OM *int_rest_build(OM *A, const Aggrs *ag, int smoothers, double omega) {
       S2->val[i] = -omega / diag2[A->indi[i]] * A->val[i];
     else
       S2->val[i] = 0.0;
+  }
+  for (long i = 1; i <= gnz; i++) {
+    long t = A->nnz + i;
+    S2->indi[t] = G->indi[i];
+    S2->indj[t] = G->indj[i];
+    if (G->indi[i] == G->indj[i])
+      S2->val[t] = 1.0 - omega / diag2[G->indi[i]] * G->val[i];
+    else if (G->strong[i])
+      S2->val[t] = -omega / diag2[G->indi[i]] * G->val[i];
+    else
+      S2->val[t] = 0.0;
   }
   S2->arr = ARR_NO;
   free(diag2);
@@ -1450,7 +1495,7 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
       state_free(s);
       return NULL;
     }
-    s->R = int_rest_build(A, &s->fin, smoothers, p->omega);
+    s->R = int_rest_build(A, NULL, &s->fin, smoothers, p->omega);
     s->AC = coarse_mtx_build(A, s->R, n);
     s->has_coarse = 1;
 
@@ -1666,6 +1711,7 @@ typedef struct {
   int ninner, nlf;
   int *lg;       /* 1..nlf, global numbers */
   OM *A;         /* local matrix in local numbering, entries in block order */
+  OM *G;         /* A_ghost: rows of the outer layer, local numbering, values as read */
 } RankView;
 
 /* One rank `clr` (1-based): SpMtx_distributeWithOverlap (SpMtx_arrangement.
@@ -1893,12 +1939,208 @@ static void rank_view(const OM *Ag, const int *eptn, int nprocs, int clr, int ol
     L->val[k] = tv[k];
   }
   rv->A = L;
+  /* A_ghost (SpMtx_build_ghost, SpMtx_arrangement.F90:1590-1612 with ol > 0):
+   * per neighbour, the rows of its nodes on the outer layer, entries to
+   * nodes on layers up to lastlayer, values as distributed */
+  {
+    long gsz = 0;
+    for (int pass = 0; pass < 2; pass++) {
+      if (pass == 1) rv->G = om_new(gsz, nlf, nlf);
+      long g = 0;
+      for (int k = 1; k <= nnghbrs; k++)
+        for (int q = cst[k]; q <= cst[k + 1] - 1; q++) {
+          int node = corder[q];
+          if (onfront[node] % hl != lastlayer) continue;
+          for (int j = A->bound[node]; j <= A->bound[node + 1] - 1; j++) {
+            int neigh = A->indj[j];
+            if (onfront[neigh] % hl <= lastlayer) {
+              g++;
+              if (pass == 1) {
+                rv->G->indi[g] = gl[node];
+                rv->G->indj[g] = gl[neigh];
+                rv->G->val[g] = A->val[j];
+              }
+            }
+          }
+        }
+      gsz = g;
+    }
+  }
   free(ti); free(tj); free(tv); free(gl);
   free(sendnodes); free(mark); free(olin_flag); free(olout_flag); free(recv_flag);
   free(bfront); free(bfs); free(bfe);
   free(ccount); free(cst); free(corder); free(nghbrs);
   free(front); free(onfront); free(fs); free(fe); free(nnodesonclrol); free(neighmap);
   om_free(A);
+}
+
+/* Position of entry (i, j) in Ag (arranged by rows, columns ascending), or 0. */
+static long ag_find(const OM *Ag, int i, int j) {
+  int lo = Ag->bound[i], hi = Ag->bound[i + 1] - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) / 2;
+    if (Ag->indj[mid] == j) return mid;
+    if (Ag->indj[mid] < j) lo = mid + 1; else hi = mid - 1;
+  }
+  return 0;
+}
+
+/* CoarsePreconditioner_smooth_Init with several processes
+ * (CoarsePreconditioner_smooth.F90:47-89) and the coarse matrix exchange of
+ * the first preconditioner call (pcg.F90:78-93, CoarseAllgathers.F90:175-257),
+ * each rank restated in turn:
+ *  - SpMtx_find_strong on the rank's matrix, scaled with the diagonal of
+ *    its rows and of A_ghost's (SpMtx_arrangement.F90:298-351,
+ *    SpMtx_aggregation.F90:829-900); A_ghost takes the flags its rows' owner
+ *    computed (SpMtx_exchange_strong, :643-824); the values return from the
+ *    scaling (SpMtx_unscale);
+ *  - IntRestBuild with A_ghost for the rank's own aggregates;
+ *  - CoarseSpace_Expand (CoarsePreconditioner_smooth.F90:137-294): every
+ *    rank ends with the rows of R that reach its nodes, each row as its
+ *    aggregate's owner built it, sorted by row and column
+ *    (SpMtx_add = SpMtx_addAll + SpMtx_consolidate, SpMtx_operation.F90:
+ *    728-768); the rows of other ranks' aggregates are dropped from the
+ *    rank's R at the end (KeepGivenRowIndeces, :85-87), so the global R is
+ *    the union of the owners' rows;
+ *  - CoarseMtxBuild with A_ghost (CoarseMtx.F90:476-516): the rank's part
+ *    R(:, own) (A R^T)(own, :);
+ *  - the parts gathered in rank order and consolidated with duplicates
+ *    added (SpMtx_consolidate, SpMtx_arrangement.F90:258-292).
+ * Ag: the input arranged by rows, columns ascending; rvs/rin: each rank's
+ * view and own aggregates (local numbering); rank_off: aggregates of the
+ * ranks before.  Sets s->R (arranged by columns) and s->AC. */
+static void ranks_coarse_space(or_state *s, const OM *Ag, const int *eptn, RankView *rvs,
+                               const Aggrs *rin, const int *rank_off, int nranks, int nagr,
+                               double alpha, int smoothers, double omega, int symm) {
+  (void)eptn;
+  unsigned char *gstrong = (unsigned char *)xcalloc((size_t)Ag->nnz + 1, 1);
+  /* strength of each rank's matrix, and the values' scaling round trip */
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 1; r <= nranks; r++) {
+    RankView *rv = &rvs[r];
+    OM *A = rv->A, *G = rv->G;
+    A->symmstruct = symm;
+    A->symmnumeric = symm;
+    int nd = rv->nlf;
+    A->diag = DALLOC(nd);
+    double *sv = DALLOC(nd);
+    for (long i = 1; i <= A->nnz; i++)
+      if (A->indi[i] == A->indj[i]) A->diag[A->indi[i]] = A->val[i];
+    for (long i = 1; i <= G->nnz; i++)
+      if (G->indi[i] == G->indj[i]) A->diag[G->indi[i]] = G->val[i];
+    if (symm) {
+      for (int i = 1; i <= nd; i++) sv[i] = sqrt(fabs(A->diag[i]));
+      for (long i = 1; i <= A->nnz; i++) {
+        A->val[i] = A->val[i] / sv[A->indi[i]];
+        A->val[i] = A->val[i] / sv[A->indj[i]];
+      }
+    } else {
+      for (int i = 1; i <= nd; i++) sv[i] = fabs(A->diag[i]);
+      for (long i = 1; i <= A->nnz; i++) A->val[i] = A->val[i] / sv[A->indi[i]];
+    }
+    free(sv);
+    A->scaled = 1;
+    om_find_strong(A, alpha);
+    for (long k = 1; k <= A->nnz; k++)
+      if (A->indi[k] <= rv->ninner) {
+        long e = ag_find(Ag, rv->lg[A->indi[k]], rv->lg[A->indj[k]]);
+        gstrong[e] = A->strong[k];
+      }
+    om_unscale(A);
+  }
+  /* A_ghost's flags from the owners; IntRestBuild for the own aggregates */
+  OM **Rp = (OM **)xcalloc((size_t)nranks + 1, sizeof(OM *));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 1; r <= nranks; r++) {
+    RankView *rv = &rvs[r];
+    OM *G = rv->G;
+    G->strong = (unsigned char *)xcalloc((size_t)G->nnz + 1, 1);
+    for (long k = 1; k <= G->nnz; k++)
+      G->strong[k] = gstrong[ag_find(Ag, rv->lg[G->indi[k]], rv->lg[G->indj[k]])];
+    Rp[r] = int_rest_build(rv->A, G, &rin[r], smoothers, omega);
+  }
+  free(gstrong);
+  /* the global R: each aggregate's row from its owner */
+  long rnz = 0;
+  for (int r = 1; r <= nranks; r++) rnz += Rp[r]->nnz;
+  OM *R = om_new(rnz, nagr, s->n);
+  long t = 0;
+  for (int r = 1; r <= nranks; r++) {
+    for (long k = 1; k <= Rp[r]->nnz; k++) {
+      t++;
+      R->indi[t] = Rp[r]->indi[k] + rank_off[r];
+      R->indj[t] = rvs[r].lg[Rp[r]->indj[k]];
+      R->val[t] = Rp[r]->val[k];
+    }
+    om_free(Rp[r]);
+  }
+  free(Rp);
+  om_arrange(R, ARR_COLS, 1, nagr, s->n);
+  /* each rank's part of the coarse matrix */
+  OM **lac = (OM **)xcalloc((size_t)nranks + 1, sizeof(OM *));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int r = 1; r <= nranks; r++) {
+    RankView *rv = &rvs[r];
+    long cnt = 0;
+    for (int j = 1; j <= rv->nlf; j++) cnt += R->bound[rv->lg[j] + 1] - R->bound[rv->lg[j]];
+    OM *Re = om_new(cnt, nagr, rv->nlf);
+    long c = 0;
+    for (int j = 1; j <= rv->nlf; j++)
+      for (int k = R->bound[rv->lg[j]]; k <= R->bound[rv->lg[j] + 1] - 1; k++) {
+        c++;
+        Re->indi[c] = R->indi[k];
+        Re->indj[c] = j;
+        Re->val[c] = R->val[k];
+      }
+    om_arrange(Re, ARR_ROWS, 1, nagr, rv->nlf);
+    OM *TT = om_new(rv->A->nnz + rv->G->nnz, rv->nlf, rv->nlf);
+    memcpy(TT->indi + 1, rv->A->indi + 1, sizeof(int) * (size_t)rv->A->nnz);
+    memcpy(TT->indj + 1, rv->A->indj + 1, sizeof(int) * (size_t)rv->A->nnz);
+    memcpy(TT->val + 1, rv->A->val + 1, sizeof(double) * (size_t)rv->A->nnz);
+    memcpy(TT->indi + 1 + rv->A->nnz, rv->G->indi + 1, sizeof(int) * (size_t)rv->G->nnz);
+    memcpy(TT->indj + 1 + rv->A->nnz, rv->G->indj + 1, sizeof(int) * (size_t)rv->G->nnz);
+    memcpy(TT->val + 1 + rv->A->nnz, rv->G->val + 1, sizeof(double) * (size_t)rv->G->nnz);
+    OM *T = om_AB(TT, Re, 0, 1);
+    om_free(TT);
+    OM *RT = om_copy(Re);
+    om_keep_columns_upto(RT, rv->ninner);
+    lac[r] = om_AB(RT, T, 0, 0);
+    om_free(RT);
+    om_free(T);
+    om_free(Re);
+  }
+  /* gathered in rank order, then consolidated (duplicates added) */
+  long anz = 0;
+  for (int r = 1; r <= nranks; r++) anz += lac[r]->nnz;
+  OM *AC = om_new(anz, nagr, nagr);
+  t = 0;
+  for (int r = 1; r <= nranks; r++) {
+    memcpy(AC->indi + t + 1, lac[r]->indi + 1, sizeof(int) * (size_t)lac[r]->nnz);
+    memcpy(AC->indj + t + 1, lac[r]->indj + 1, sizeof(int) * (size_t)lac[r]->nnz);
+    memcpy(AC->val + t + 1, lac[r]->val + 1, sizeof(double) * (size_t)lac[r]->nnz);
+    t += lac[r]->nnz;
+    om_free(lac[r]);
+  }
+  free(lac);
+  om_arrange(AC, ARR_ROWS, 1, nagr, nagr);
+  long k = 1;
+  for (long i = 2; i <= AC->nnz; i++) {
+    if (AC->indi[i] == AC->indi[k] && AC->indj[i] == AC->indj[k]) {
+      if (OR_MUTANT != 10) AC->val[k] = AC->val[k] + AC->val[i];
+    } else {
+      k++;
+      AC->indi[k] = AC->indi[i];
+      AC->indj[k] = AC->indj[i];
+      AC->val[k] = AC->val[i];
+    }
+  }
+  if (AC->nnz > 0) AC->nnz = k;
+  free(AC->bound);
+  AC->bound = NULL;
+  AC->nbound = 0;
+  AC->arr = ARR_NO;
+  s->R = R;
+  s->AC = AC;
 }
 
 or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, const double *val,
@@ -1954,8 +2196,7 @@ or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, cons
    * order below */
   Aggrs *rin = (Aggrs *)xcalloc((size_t)nranks + 1, sizeof(Aggrs));
   Aggrs *rfull = (Aggrs *)xcalloc((size_t)nranks + 1, sizeof(Aggrs));
-  int **rlg = (int **)xcalloc((size_t)nranks + 1, sizeof(int *));
-  int *rninner = IALLOC(nranks);
+  RankView *rvs = (RankView *)xcalloc((size_t)nranks + 1, sizeof(RankView));
 #pragma omp parallel for schedule(dynamic, 1)
   for (int r = 1; r <= nranks; r++) {
     RankView rv;
@@ -2007,28 +2248,26 @@ or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, cons
     free(nodes);
     rin[r] = in;
     rfull[r] = full;
-    rlg[r] = rv.lg;
-    rninner[r] = rv.ninner;
-    om_free(rv.A);
+    rvs[r] = rv;
   }
+  int *rank_off = IALLOC(nranks + 1);  /* aggregates of the ranks before */
   for (int r = 1; r <= nranks; r++) {
     cap += subn[r];
-    for (int i = 1; i <= rninner[r]; i++) {
-      int g = rlg[r][i];
+    for (int i = 1; i <= rvs[r].ninner; i++) {
+      int g = rvs[r].lg[i];
       if (i <= rin[r].n && rin[r].num[i] > 0) num_in[g] = rin[r].num[i] + off_in;
       if (i <= rfull[r].n) num_full[g] = rfull[r].num[i] + off_full;
     }
+    rank_off[r] = off_in;
     off_in += rin[r].nagr;
     off_full += rfull[r].nagr;
-    aggrs_free(&rin[r]);
     aggrs_free(&rfull[r]);
-    free(rlg[r]);
   }
-  free(rin); free(rfull); free(rlg); free(rninner);
-  om_free(Ag);
+  free(rfull);
   form_aggr(&s->fin, off_in, n, num_in);
   form_aggr(&s->ffull, off_full, n, num_full);
-  free(num_in); free(num_full); free(eptn);
+  free(num_in); free(num_full);
+  int *eptn_keep = eptn;
   s->nsub = nranks;
   s->sstarts = IALLOC(nranks + 1);
   s->sinds = IALLOC(cap);
@@ -2061,8 +2300,16 @@ or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, cons
     state_free(s);
     return NULL;
   }
-  s->R = int_rest_build(A, &s->fin, smoothers, p->omega);
-  s->AC = coarse_mtx_build(A, s->R, n);
+  ranks_coarse_space(s, Ag, eptn_keep, rvs, rin, rank_off, nranks, off_in, strong_conn1,
+                     smoothers, p->omega, p->symmstruct && p->symmnumeric);
+  for (int r = 1; r <= nranks; r++) {
+    om_free(rvs[r].A);
+    om_free(rvs[r].G);
+    free(rvs[r].lg);
+    aggrs_free(&rin[r]);
+  }
+  free(rvs); free(rin); free(rank_off); free(eptn_keep);
+  om_free(Ag);
   s->has_coarse = 1;
   s->ol = ol_sub;
   om_arrange(A, ARR_ROWS, 0, 0, 0);

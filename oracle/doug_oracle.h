@@ -64,6 +64,9 @@ or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, cons
 void or_free(or_state *s);
 void or_free_ptr(void *q);
 const char *or_last_error(void);
+/* worker threads of the parallel loops (ranks, subdomain factorisations and
+ * solves; OMP_NUM_THREADS) */
+int or_num_threads(void);
 
 /* sizes */
 int or_n(const or_state *s);

@@ -65,6 +65,7 @@ def bind(L):
     L.or_setup_ranks.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
                                  C.c_int, _i32p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_last_error.restype = C.c_char_p
+    L.or_num_threads.restype = C.c_int
     for name in ("or_n", "or_nagr_fine", "or_nagr_coarse", "or_nsub", "or_overlap",
                  "or_smoothers"):
         getattr(L, name).argtypes = [C.c_void_p]
@@ -110,7 +111,7 @@ class Oracle:
 
     def __init__(self, n, indi, indj, val, p: Params | None = None, partition=None,
                  fresh_process=True):
-        L = lib()
+        L = self.L = lib()
         if fresh_process:
             L.or_reset_process()
         self.p = p if p is not None else params()
@@ -131,12 +132,14 @@ class Oracle:
         self.n = n
 
     @classmethod
-    def ranks(cls, n, indi, indj, val, p: Params, rank_of_node):
+    def ranks(cls, n, indi, indj, val, p: Params, rank_of_node, L=None):
         """One DOUG run on several processes restated on the CPU:
         rank_of_node[i] in 1..nranks.  `self.rank_maps` holds each rank's
-        (ninner, local-to-global map), 1-based."""
-        L = lib()
+        (ninner, local-to-global map), 1-based.  L: another build of the
+        oracle (the tests' planted-mistake builds)."""
+        L = L or lib()
         self = cls.__new__(cls)
+        self.L = L
         self.p = p
         self.n = n
         r = np.ascontiguousarray(rank_of_node, dtype=np.int32)
@@ -159,13 +162,13 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_free(self.h)
+            self.L.or_free(self.h)
             self.h = None
 
     # ---- setup products ----
     def fine_aggregates(self):
         """fAggr%inner as (num, starts, nodes), 1-based."""
-        L = lib()
+        L = self.L
         na = L.or_nagr_fine(self.h)
         num = np.zeros(self.n, np.int32)
         st = np.zeros(na + 1, np.int32)
@@ -175,7 +178,7 @@ class Oracle:
 
     def coarse_aggregates(self):
         """cAggr%full over the coarse unknowns, 1-based."""
-        L = lib()
+        L = self.L
         nc = L.or_nagr_fine(self.h)
         na = L.or_nagr_coarse(self.h)
         num = np.zeros(nc, np.int32)
@@ -185,7 +188,7 @@ class Oracle:
         return num, st, nodes[: st[-1] - 1].copy()
 
     def subdomains(self):
-        L = lib()
+        L = self.L
         ns = L.or_nsub(self.h)
         tot = L.or_sub_total(self.h)
         st = np.zeros(ns + 1, np.int32)
@@ -201,34 +204,34 @@ class Oracle:
         return i[:nnz], j[:nnz], v[:nnz]
 
     def R(self):
-        L = lib()
+        L = self.L
         return self._coo(L.or_get_R, L.or_R_nnz(self.h))
 
     def AC(self):
-        L = lib()
+        L = self.L
         return self._coo(L.or_get_AC, L.or_AC_nnz(self.h))
 
     def A(self):
-        L = lib()
+        L = self.L
         return self._coo(L.or_get_A, L.or_A_nnz(self.h))
 
     @property
     def overlap(self):
-        return lib().or_overlap(self.h)
+        return self.L.or_overlap(self.h)
 
     @property
     def smoothers(self):
-        return lib().or_smoothers(self.h)
+        return self.L.or_smoothers(self.h)
 
     # ---- operators ----
     def spmv(self, x):
         y = np.zeros(self.n)
-        lib().or_spmv(self.h, np.ascontiguousarray(x, np.float64), y)
+        self.L.or_spmv(self.h, np.ascontiguousarray(x, np.float64), y)
         return y
 
     def prec(self, r):
         z = np.zeros(self.n)
-        lib().or_prec_apply(self.h, np.ascontiguousarray(r, np.float64), z)
+        self.L.or_prec_apply(self.h, np.ascontiguousarray(r, np.float64), z)
         return z
 
     def pcg(self, b, tol, maxit, x0=None):
@@ -236,7 +239,7 @@ class Oracle:
         cap = maxit if maxit > 0 else 100000
         hist = np.zeros(cap)
         cond = C.c_double(0.0)
-        it = lib().or_pcg(self.h, np.ascontiguousarray(b, np.float64), x, tol, maxit,
+        it = self.L.or_pcg(self.h, np.ascontiguousarray(b, np.float64), x, tol, maxit,
                           C.byref(cond), hist)
         return x, it, cond.value, hist[:it].copy()
 
@@ -264,6 +267,11 @@ def aggregate(n, indi, indj, val, radius, minasize, maxasize, alpha, coarse=Fals
                         np.ascontiguousarray(val, dtype=np.float64), int(symm), radius, minasize,
                         maxasize, alpha, int(coarse), ni, nf)
     return na, ni, nf
+
+
+def num_threads() -> int:
+    """Threads of the oracle's parallel loops."""
+    return lib().or_num_threads()
 
 
 def reset_process(L=None):

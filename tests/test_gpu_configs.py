@@ -139,21 +139,6 @@ def test_cfg2_64cube_error_against_sparse_direct():
     assert abs(errs[1, 1e-10] - errs[0, 1e-10]) <= 2e-9
 
 
-def graded_partition(pts, k=16, ratio=1.05):
-    """k x k cells of the unit square with widths in geometric ratio: on the
-    locally refined mesh the parts hold 1.7k-7.7k unknowns.  Returns 1-based
-    (starts, nodes) for doug_set_partition."""
-    w = ratio ** np.arange(k)
-    e = np.concatenate([[0.0], np.cumsum(w / w.sum())])
-    cx = np.clip(np.searchsorted(e, pts[:, 0], side="right") - 1, 0, k - 1)
-    cy = np.clip(np.searchsorted(e, pts[:, 1], side="right") - 1, 0, k - 1)
-    part = cy * k + cx
-    order = np.argsort(part, kind="stable")
-    counts = np.bincount(part, minlength=k * k)
-    starts = np.concatenate([[0], np.cumsum(counts)]) + 1
-    return starts.astype(np.int32), (order + 1).astype(np.int32)
-
-
 def test_cfg3_fem_1m_nodes_32_rhs_uneven_subdomains():
     import scipy.sparse as sps
     torch = pytest.importorskip("torch")
@@ -161,7 +146,7 @@ def test_cfg3_fem_1m_nodes_32_rhs_uneven_subdomains():
     A = sps.csr_matrix((v, (i - 1, j - 1)), shape=(n, n))
     nrhs = 32
     B = np.stack([P.uniform_rhs(n, 100 + k) for k in range(nrhs)], axis=1)
-    part = graded_partition(pts)
+    part = P.graded_partition(pts)
     psz = np.diff(part[0])
     print(f"FEM: n={n} nnz={len(v)}, partition sizes {psz.min()}-{psz.max()}", flush=True)
     dev = torch.device("cuda:0")

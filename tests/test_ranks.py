@@ -71,7 +71,15 @@ CASES = {
     # (SpMtx_distributeWithOverlap arranges with sort)
     "2d-shuffled-input": lambda: (_shuffled(P.diffusion2d(24, 1.0, 9)), "stripes", 3,
                                   dict(BASE, radius1=-2)),
+    # iid lognormal coefficients (sigma 2): weak connections at the rank
+    # boundaries, where A_ghost's rows smooth differently from own rows
+    "3d-contrast-boxes": lambda: (_contrast(12, 7), "boxes", 8, dict(BASE, radius1=2, strong1=0.25)),
 }
+
+
+def _contrast(N, seed):
+    kappa = np.exp(2.0 * np.random.default_rng(seed).standard_normal((N, N, N)))
+    return P.diffusion3d(N, kappa=kappa)
 
 
 def _shuffled(m):
@@ -84,6 +92,12 @@ def _case(name):
     (n, i, j, v), kind, nr, ctl = CASES[name]()
     if kind == "stripes":
         rank = ((np.arange(n) // 7) % nr + 1).astype(np.int32)
+    elif kind == "boxes":
+        N = round(n ** (1 / 3))
+        st, nd = P.block_partition((N,) * 3, N // 2)
+        rank = np.zeros(n, np.int32)
+        for r in range(len(st) - 1):
+            rank[nd[st[r] - 1:st[r + 1] - 1] - 1] = r + 1
     elif kind == "random":
         rank = np.random.default_rng(5).integers(1, nr + 1, n).astype(np.int32)
     else:
@@ -106,6 +120,132 @@ def test_library_matches_oracle_bit_for_bit(name):
         for a, b in zip(s.subdomains(), o.subdomains()):
             np.testing.assert_array_equal(a, b)
         assert s.info()["nsub"] == nr
+        # the coarse space of the several-process run: entries and bits
+        assert _same_entries(s.restriction(), o.R(), True)
+        assert _same_entries(s.coarse_matrix(), o.AC(), False)
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a).view(np.int64), np.asarray(b).view(np.int64))
+
+
+def _same_entries(csr, coo, sort_coo):
+    """The library's 0-based CSR against the oracle's 1-based COO (R: sorted
+    first, the oracle keeps it arranged by columns)."""
+    rp, c, v = csr
+    oi, oj, ov = coo
+    if sort_coo:
+        order = np.lexsort((oj, oi))
+        oi, oj, ov = oi[order], oj[order], ov[order]
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return np.array_equal(rows, oi - 1) and np.array_equal(c, oj - 1) and _bits_equal(v, ov)
+
+
+def _owner_view_R(n, i, j, v, rank, num, ctl):
+    """R = T S^T of DOUG's several-process run, from its definition and not
+    from either restatement (smoothers 1): R(a, j) sums S(j, k) over the
+    nodes k of aggregate a, with row j of S as a's owner sees it.  A row of
+    a node the owner holds (its own rows, and with overlap >= 2 the first
+    outside layer) is smoothed with the filtered diagonal a_jj + the weak
+    a_jk; a row of the outside layer the owner receives (overlap 1) is
+    A_ghost's: its diagonal entry alone.  Strength: |a_ij| > alpha max|a_ik|
+    on the scaled matrix (symmetric scaling when the matrix is flagged
+    symmetric, else by rows, which leaves the test unscaled)."""
+    om = 0.33333333333333
+    A = sp.coo_matrix((v, (i - 1, j - 1)), shape=(n, n)).tocsr()
+    A.sum_duplicates()
+    C = A.tocoo()
+    d = A.diagonal()
+    if ctl.get("symmstruct") and ctl.get("symmnumeric"):
+        sc = C.data / np.sqrt(abs(d[C.row])) / np.sqrt(abs(d[C.col]))
+    else:
+        sc = C.data
+    off = C.row != C.col
+    mx = np.zeros(n)
+    np.maximum.at(mx, C.row[off], abs(sc[off]))
+    strong = np.zeros(len(sc), bool)
+    strong[off] = abs(sc[off]) > ctl["strong1"] * mx[C.row[off]]
+    weak = off & ~strong
+    dfilt = d.copy()
+    np.add.at(dfilt, C.row[weak], C.data[weak])
+    L = max(ctl.get("overlap", 1), ctl.get("smoothers", 1), 1)
+    owner = np.zeros(num.max() + 1, int)
+    owner[num[num > 0]] = rank[num > 0]
+    keep = num[C.col] > 0
+    a = num[C.col][keep] - 1
+    jr, kk, val, st = C.row[keep], C.col[keep], C.data[keep], strong[keep]
+    ghost = (L == 1) & (rank[jr] != owner[a + 1])
+    dj = np.where(ghost, d[jr], dfilt[jr])
+    s_ = np.where(jr == kk, np.where(ghost, 1 - om / dj * val, 1 - om),
+                  np.where(st, -om / dj * val, 0.0))
+    return sp.csr_matrix((s_, (a, jr)), shape=(num.max(), n))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_coarse_space_of_several_processes(name):
+    # the oracle's restatement of the several-process coarse space against
+    # what its definition fixes: A_c = R A R^T over the ranks' parts, and
+    # (smoothers 1) R from the owner-view formula
+    n, i, j, v, rank, nr, ctl = _case(name)
+    o = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank)
+    _check_coarse_space(o, n, i, j, v, rank, ctl)
+
+
+def _check_coarse_space(o, n, i, j, v, rank, ctl):
+    ri, rj, rv = o.R()
+    nc = int(ri.max())
+    R = sp.csr_matrix((rv, (ri - 1, rj - 1)), shape=(nc, n))
+    ai, aj, av = o.A()
+    A = sp.csr_matrix((av, (ai - 1, aj - 1)), shape=(n, n))
+    ci, cj, cv = o.AC()
+    AC = sp.csr_matrix((cv, (ci - 1, cj - 1)), shape=(nc, nc))
+    G = (R @ A @ R.T).tocsr()
+    assert abs(AC - G).max() <= 1e-13 * abs(G).max()
+    if ctl.get("smoothers", 1) == 1:
+        num, _, _ = o.fine_aggregates()
+        Ri = _owner_view_R(n, i, j, v, rank, num, ctl)
+        assert abs(R - Ri).max() <= 1e-14 * abs(Ri).max()
+
+
+def test_one_rank_coarse_space_is_the_one_process_one():
+    n, i, j, v = P.diffusion3d(12, 1.0, 4)
+    ctl = dict(BASE, radius1=2)
+    o1 = O.Oracle.ranks(n, i, j, v, O.params(**ctl), np.ones(n, np.int32))
+    num, _, _ = o1.fine_aggregates()
+    # the same aggregates given to one process: the same R and A_c bits
+    with D.Solver(-1) as s:
+        s.set_params(**ctl)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_aggregates(num)
+        s.set_partition(*P.block_partition((12,) * 3, 6))
+        s.setup_host()
+        assert _same_entries(s.restriction(), o1.R(), True)
+        cp, cc, cv = s.coarse_matrix()
+    ci, cj, cv1 = o1.AC()  # sorted by row and column
+    rows = np.repeat(np.arange(len(cp) - 1), np.diff(cp))
+    order = np.lexsort((cc, rows))
+    np.testing.assert_array_equal(rows[order], ci - 1)
+    np.testing.assert_array_equal(cc[order], cj - 1)
+    assert _bits_equal(cv[order], cv1)
+
+
+def test_planted_coarse_space_mistakes_are_caught(tmp_path):
+    import ctypes as C
+    import subprocess
+    import os
+    src = os.path.join(os.path.dirname(O.__file__), "doug_oracle.c")
+    n, i, j, v, rank, nr, ctl = _case("3d-contrast-boxes")
+    for k in (9, 10):
+        out = str(tmp_path / f"liboracle_m{k}.so")
+        subprocess.run(["gcc", "-O1", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off",
+                        f"-DOR_MUTANT={k}", "-shared", "-o", out, src, "-lm"], check=True)
+        L = O.bind(C.CDLL(out))
+        # ghost rows with the filtered diagonal (9) fail the owner-view
+        # formula; a coarse matrix missing the other ranks' parts (10) fails
+        # R A R^T or is not even positive definite
+        with pytest.raises((AssertionError, RuntimeError)):
+            om = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank, L=L)
+            _check_coarse_space(om, n, i, j, v, rank, ctl)
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
