@@ -76,6 +76,11 @@ struct doug_ctx {
   int64_t nnz_R = 0;               // entries of R; with coarse_on_device R's columns and
                                    // values stay on the device until a getter asks
   uint8_t* d_strong = nullptr;     // strength mask, uploaded while the aggregation runs
+  double* d_valin = nullptr;       // values as read (several processes, overlap 1)
+  // several processes: place of each node in process order, the inverse,
+  // and whether its owner sends it (staging of the device coarse build)
+  std::vector<int32_t> mp_proc, mp_order;
+  std::vector<uint8_t> mp_sent;
   bool use_device_coarse = true;   // DOUG_OPT_DEVICE_COARSE
   bool use_device_blocks = true;   // DOUG_OPT_DEVICE_BLOCKS
   bool keep_setup_temps = true;    // DOUG_OPT_KEEP_SETUP_TEMPS
@@ -235,6 +240,7 @@ static int64_t scratch_slot_bytes(const doug_ctx* ctx) {
 // hipFree waits for the device and unmaps, milliseconds for the factors.
 static void free_device(doug_ctx* ctx, bool background = false) {
   drop_iteration_graph(ctx);
+  ctx->d_valin = nullptr;
   join_reaper(ctx);
   if (background && !ctx->allocs.empty()) {
     std::vector<void*> old;
@@ -983,7 +989,12 @@ struct DevTemp {
   }
 };
 
-static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
+// With `ranks` (virtual ranks, overlap 1): DOUG's several-process coarse
+// space -- the same bits as ranks_coarse_space, see doug_coarse.hip.  Returns
+// DOUG_E_UNSUPPORTED without side effects on the results when a row of A_c
+// has more columns than the wavefront kernel holds (the caller then builds
+// on the host).
+static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong, bool ranks = false) {
   double t1 = now();
   const Csr& A = ctx->A;
   const int n = A.nrows, nagr = ctx->fin.nagr;
@@ -1036,8 +1047,47 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   HIPCHK(tmp.alloc(&s_val, (size_t)nnz));
   HIPCHK(tmp.alloc(&cnt, (size_t)n + 1));
   HIPCHK(hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), st));
-  launch_rt_rows(n, ctx->d_ptr, ctx->d_col, ctx->d_val, d_strong, d_num, ctx->p.omega, s_col, s_val,
-                 cnt, st);
+  // several processes: the process layout, each row in its owner's order
+  int32_t *d_owner = nullptr, *d_proc = nullptr, *d_order = nullptr, *a_col = nullptr;
+  uint8_t* d_sent = nullptr;
+  double *a_val = nullptr, *d_dfilt = nullptr, *d_dghost = nullptr;
+  if (ranks) {
+    const RankSetup& rs = ctx->rsetup;
+    std::vector<int32_t>& proc = ctx->mp_proc;
+    std::vector<int32_t>& order = ctx->mp_order;
+    std::vector<uint8_t>& sent = ctx->mp_sent;
+    proc.assign((size_t)n, 0);
+    order.assign((size_t)n, 0);
+    sent.assign((size_t)n, 0);
+    int off = 0;
+    for (const RankLocal& L : rs.ranks) {
+      for (int k = 0; k < L.ninner; k++) {
+        proc[L.lg[k]] = off + k;
+        order[off + k] = L.lg[k];
+        sent[L.lg[k]] = k < L.nsent;
+      }
+      off += L.ninner;
+    }
+    HIPCHK(tmp.alloc(&d_owner, (size_t)n));
+    HIPCHK(tmp.alloc(&d_proc, (size_t)n));
+    HIPCHK(tmp.alloc(&d_order, (size_t)n));
+    HIPCHK(tmp.alloc(&d_sent, (size_t)n));
+    HIPCHK(tmp.alloc(&a_col, (size_t)nnz));
+    HIPCHK(tmp.alloc(&a_val, (size_t)nnz));
+    HIPCHK(tmp.alloc(&d_dfilt, (size_t)n));
+    HIPCHK(tmp.alloc(&d_dghost, (size_t)n));
+    HIPCHK(hipMemcpyAsync(d_owner, ctx->eptn.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_proc, proc.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_order, order.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_sent, sent.data(), (size_t)n, hipMemcpyHostToDevice, st));
+    launch_mp_own_rows(n, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_valin, d_strong, d_owner, d_sent,
+                       a_col, a_val, d_dfilt, d_dghost, st);
+    launch_mp_rt_rows(n, d_order, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_valin, d_strong, d_num,
+                      d_owner, d_proc, d_dfilt, d_dghost, ctx->p.omega, s_col, s_val, cnt, st);
+  } else {
+    launch_rt_rows(n, ctx->d_ptr, ctx->d_col, ctx->d_val, d_strong, d_num, ctx->p.omega, s_col, s_val,
+                   cnt, st);
+  }
   if ((rc = dalloc(ctx, &ctx->d_Rtptr, (size_t)n + 1)) || (rc = scan32(cnt, ctx->d_Rtptr, n + 1)))
     return rc;
   int32_t nnz_rt = 0;
@@ -1116,12 +1166,72 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   int32_t *tp_ptr, *tp_col, *ac_ptr, *ac_col;
   double *tp_val, *ac_val;
   int32_t nnz_tp = 0, nnz_ac = 0;
-  if ((rc = product(n, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Rtptr, ctx->d_Rtcol, ctx->d_Rtval,
-                    &tp_ptr, &tp_col, &tp_val, &nnz_tp, false, false)))
-    return rc;
-  if ((rc = product(nagr, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, tp_ptr, tp_col, tp_val, &ac_ptr,
-                    &ac_col, &ac_val, &nnz_ac, false, true)))
-    return rc;
+  if (ranks) {
+    // R with each row in process order: R^T's entries listed row by row in
+    // process order, stably sorted on their aggregate
+    int32_t *len, *poff, *ids, *pkeys, *pkeys_out, *porder, *rp_col;
+    double* rp_val;
+    HIPCHK(tmp.alloc(&len, (size_t)n + 1));
+    HIPCHK(tmp.alloc(&poff, (size_t)n + 1));
+    HIPCHK(tmp.alloc(&ids, (size_t)nnz_rt));
+    HIPCHK(tmp.alloc(&pkeys, (size_t)nnz_rt));
+    HIPCHK(tmp.alloc(&pkeys_out, (size_t)nnz_rt));
+    HIPCHK(tmp.alloc(&porder, (size_t)nnz_rt));
+    HIPCHK(tmp.alloc(&rp_col, (size_t)nnz_rt));
+    HIPCHK(tmp.alloc(&rp_val, (size_t)nnz_rt));
+    HIPCHK(hipMemsetAsync(len, 0, ((size_t)n + 1) * sizeof(int32_t), st));
+    launch_mp_row_lengths(n, d_order, ctx->d_Rtptr, len, st);
+    if ((rc = scan32(len, poff, n + 1))) return rc;
+    launch_mp_proc_entries(n, d_order, ctx->d_Rtptr, ctx->d_Rtcol, poff, ids, pkeys, st);
+    {
+      size_t bytes = 0;
+      HIPCHK(stable_sort_pairs(nullptr, bytes, pkeys, pkeys_out, ids, porder, nnz_rt, end_bit, st));
+      if ((rc = tmp_bytes(bytes))) return rc;
+      HIPCHK(stable_sort_pairs(scan_tmp, bytes, pkeys, pkeys_out, ids, porder, nnz_rt, end_bit, st));
+    }
+    launch_gather_entries(nnz_rt, porder, rowid, ctx->d_Rtval, rp_col, rp_val, st);
+    // T = A R^T with each row in its owner's entry order
+    if ((rc = product(n, ctx->d_ptr, a_col, a_val, ctx->d_Rtptr, ctx->d_Rtcol, ctx->d_Rtval, &tp_ptr,
+                      &tp_col, &tp_val, &nnz_tp, false, false)))
+      return rc;
+    // A_c: each process's part, then their sum in process order
+    int64_t *ub, *off;
+    int32_t *pc, *ccount;
+    double* pv;
+    HIPCHK(tmp.alloc(&ub, (size_t)nagr + 1));
+    HIPCHK(tmp.alloc(&off, (size_t)nagr + 1));
+    HIPCHK(hipMemsetAsync(ub, 0, ((size_t)nagr + 1) * sizeof(int64_t), st));
+    launch_product_bound(nagr, ctx->d_Rptr, rp_col, tp_ptr, ub, st);
+    if ((rc = scan64(ub, off, nagr + 1))) return rc;
+    int64_t total = 0;
+    if ((rc = read_back(off + nagr, total))) return rc;
+    HIPCHK(tmp.alloc(&pc, (size_t)total));
+    HIPCHK(tmp.alloc(&pv, (size_t)total));
+    HIPCHK(tmp.alloc(&ccount, (size_t)nagr + 1));
+    HIPCHK(hipMemsetAsync(ccount, 0, ((size_t)nagr + 1) * sizeof(int32_t), st));
+    int* ovf;
+    HIPCHK(tmp.alloc(&ovf, 1));
+    HIPCHK(hipMemsetAsync(ovf, 0, sizeof(int), st));
+    launch_mp_galerkin_wave(nagr, ctx->d_Rptr, rp_col, rp_val, d_owner, tp_ptr, tp_col, tp_val, off,
+                            pc, pv, ccount, ovf, st);
+    int o = 0;
+    if ((rc = read_back(ovf, o))) return rc;
+    if (o) return fail(ctx, DOUG_E_UNSUPPORTED, "a coarse matrix row above 128 entries");
+    HIPCHK(tmp.alloc(&ac_ptr, (size_t)nagr + 1));
+    if ((rc = scan32(ccount, ac_ptr, nagr + 1))) return rc;
+    if ((rc = read_back(ac_ptr + nagr, nnz_ac))) return rc;
+    HIPCHK(tmp.alloc(&ac_col, (size_t)nnz_ac));
+    HIPCHK(tmp.alloc(&ac_val, (size_t)nnz_ac));
+    launch_compact(nagr, off, ac_ptr, pc, pv, ac_col, ac_val, st);
+    HIPCHK(hipGetLastError());
+  } else {
+    if ((rc = product(n, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Rtptr, ctx->d_Rtcol, ctx->d_Rtval,
+                      &tp_ptr, &tp_col, &tp_val, &nnz_tp, false, false)))
+      return rc;
+    if ((rc = product(nagr, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, tp_ptr, tp_col, tp_val, &ac_ptr,
+                      &ac_col, &ac_val, &nnz_ac, false, true)))
+      return rc;
+  }
   // host copies: R's row pointers (row blocks; its entries are fetched by
   // doug_get_restriction) and A_c (coarse factor)
   Csr& R = ctx->R;
@@ -1337,6 +1447,11 @@ static int upload_matrix(doug_ctx* ctx) {
   int rc;
   if ((rc = upload(ctx, &ctx->d_ptr, A.ptr)) || (rc = upload(ctx, &ctx->d_col, A.col)) ||
       (rc = upload(ctx, &ctx->d_val, A.val)) || (rc = upload(ctx, &ctx->d_src, A.src)))
+    return rc;
+  // several processes with overlap 1: A_ghost's rows smooth with the
+  // values as read (the device coarse build)
+  if (rank_coarse(ctx) && ctx->ol_dist == 1 && ctx->smoothers == 1 && ctx->use_device_coarse &&
+      (rc = upload(ctx, &ctx->d_valin, ctx->A_input_val)))
     return rc;
   std::vector<int32_t> rb = row_blocks(A);
   ctx->nrb = (int)rb.size() - 1;
@@ -1723,6 +1838,17 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
     if (rank_coarse(ctx)) {
+      if (ctx->ol_dist == 1 && ctx->smoothers == 1 && ctx->use_device_coarse && ctx->d_valin) {
+        int r = coarse_build_device(ctx, strong, true);
+        if (r != DOUG_E_UNSUPPORTED) {
+          for (RankLocal& L : ctx->rsetup.ranks) {
+            L.A = Csr();
+            L.agg = Aggregates();
+          }
+          return r;
+        }
+        (void)hipStreamSynchronize(ctx->stream);
+      }
       phase_coarse_ranks(ctx, strong);
       return DOUG_OK;
     }

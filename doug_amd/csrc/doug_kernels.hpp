@@ -204,6 +204,28 @@ void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double*
 // Rows of R^T = S T^T into slots at ptr[j] (capacity = row length of A),
 // cnt[j] = entries written.  strong: per entry of A; num: 1-based aggregate
 // of each node (0 = none).
+// the coarse space of a DOUG run on several processes with overlap 1
+// (doug_coarse.hip): rows in their owner's entry order with the filtered and
+// the A_ghost diagonals; R^T rows in process order; entry ids in process
+// order for the sort that gives R in process order; A_c with each process's
+// part summed first, rows sorted by column
+void launch_mp_own_rows(int n, const int32_t* ptr, const int32_t* col, const double* val,
+                        const double* valin, const uint8_t* strong, const int32_t* owner,
+                        const uint8_t* sent, int32_t* ocol, double* oval, double* dfilt,
+                        double* dghost, hipStream_t st);
+void launch_mp_rt_rows(int n, const int32_t* order, const int32_t* ptr, const int32_t* col,
+                       const double* val, const double* valin, const uint8_t* strong,
+                       const int32_t* num, const int32_t* owner, const int32_t* proc,
+                       const double* dfilt, const double* dghost, double omega, int32_t* slot_col,
+                       double* slot_val, int32_t* cnt, hipStream_t st);
+void launch_mp_row_lengths(int n, const int32_t* order, const int32_t* rtptr, int32_t* len,
+                           hipStream_t st);
+void launch_mp_proc_entries(int n, const int32_t* order, const int32_t* rtptr, const int32_t* rtcol,
+                            const int32_t* off, int32_t* ids, int32_t* keys, hipStream_t st);
+void launch_mp_galerkin_wave(int nrows, const int32_t* xptr, const int32_t* xcol, const double* xval,
+                             const int32_t* owner, const int32_t* yptr, const int32_t* ycol,
+                             const double* yval, const int64_t* slot_off, int32_t* slot_col,
+                             double* slot_val, int32_t* cnt, int* overflow, hipStream_t st);
 void launch_rt_rows(int n, const int32_t* ptr, const int32_t* col, const double* val,
                     const uint8_t* strong, const int32_t* num, double omega, int32_t* slot_col,
                     double* slot_val, int32_t* cnt, hipStream_t st);

@@ -1512,6 +1512,7 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
   // A_ghost (SpMtx_build_ghost, SpMtx_arrangement.F90:1590-1612): the
   // received nodes' rows, entries to nodes up to layer L, values as read;
   // the entry order inside a ghost row decides no result
+  out.nsent = (int)grp[0].size();
   out.nrecv = (int)grp[4].size();
   out.G = Csr();
   if (ghost) {

@@ -151,6 +151,7 @@ struct RankLocal {
   // A_ghost's entries (values as read) and A.src gives every entry's
   // position in the assembled matrix the caller arranged.
   Csr A;
+  int nsent = 0;        // own nodes sent to a neighbour: the first nsent local nodes
   int nrecv = 0;        // the outer layer: the last nrecv local nodes
   Csr G;                // A_ghost rows during the set-up
   Aggregates agg;       // own aggregates in local numbering (keep_local)
