@@ -96,6 +96,7 @@ SIGNATURES = {
     "doug_spmv_device": (_i32, [_vp, _vp, _vp]),
     "doug_prec_apply": (_i32, [_vp, _vp, _vp]),
     "doug_prec_apply_device": (_i32, [_vp, _vp, _vp]),
+    "doug_prec_apply_device_noalloc": (_i32, [_vp, _vp, _vp]),
     "doug_pcg": (_i32, [_vp, _vp, _vp, _dbl, _i32, C.POINTER(_i32), C.POINTER(_dbl), _vp]),
     "doug_pcg_device": (_i32, [_vp, _vp, _vp, _dbl, _i32, C.POINTER(_i32), C.POINTER(_dbl), _vp]),
     "doug_time_operator": (_i32, [_vp, _i32, _i32, C.POINTER(_dbl)]),

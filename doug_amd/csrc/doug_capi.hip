@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1999,24 +2000,38 @@ static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
               ctx->spmv_shape);
 }
 
-static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
+// Work vectors of one preconditioner apply: the subdomain vectors and the
+// coarse vectors with the chunked GEMVs' partial sums.
+struct PrecWork {
+  double *subsol, *crhs, *cu, *csol, *cpart;
+};
+
+static PrecWork ctx_work(doug_ctx* ctx) {
+  return PrecWork{ctx->d_subsol, ctx->d_crhs, ctx->d_cu, ctx->d_csol, ctx->d_cpart};
+}
+
+static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r, double* subsol) {
   if (ctx->p.fine_method == 2) {  // FinePreconditioner_sgs_Apply
-    launch_gather_positions((int64_t)ctx->sinds.size(), ctx->d_inds, r, ctx->d_subsol, ctx->stream);
+    launch_gather_positions((int64_t)ctx->sinds.size(), ctx->d_inds, r, subsol, ctx->stream);
     launch_sgs(ctx->d_sgs, ctx->nsub, ctx->sgs_mmax, ctx->d_sgs_levptr, ctx->d_sgs_lrows,
                ctx->d_sgs_rbeg, ctx->d_sgs_rlen, ctx->d_sgs_col, ctx->d_sgs_val, ctx->d_sgs_diag,
-               ctx->d_subsol, ctx->sgs_iters, ctx->stream);
+               subsol, ctx->sgs_iters, ctx->stream);
     return;
   }
   if (ctx->precision == DOUG_PREC_FP32_FACTORS)
-    launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, ctx->d_subsol,
+    launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, subsol,
                 ctx->stream);
   else
-    launch_trsv(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, ctx->d_subsol,
+    launch_trsv(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, subsol,
                 ctx->stream);
 }
 
+static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
+  enqueue_subdomain_solves(ctx, r, ctx->d_subsol);
+}
+
 // preconditioner (pcg.F90:216-270): fine solves, then coarse correction added
-static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
+static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWork& w) {
   const Params& p = ctx->p;
   if (p.method == 0) {
     launch_copy(z, r, ctx->n, ctx->stream);
@@ -2028,18 +2043,22 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
   const bool coarse = ctx->has_coarse && p.method != 4;
   const bool fine = !(ctx->has_coarse && p.method == 2);
   if (coarse) {
-    launch_spmv(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, ctx->d_crhs, ctx->stream);
-    launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_crhs, ctx->d_cu, ctx->d_cpart, ctx->stream);
-    launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cu, ctx->d_csol, ctx->d_cpart, ctx->stream);
+    launch_spmv(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, w.crhs, ctx->stream);
+    launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, w.crhs, w.cu, w.cpart, ctx->stream);
+    launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, w.cu, w.csol, w.cpart, ctx->stream);
   }
   const double* fine_src = r;
   if (fine && p.fine_method >= 1) {
-    enqueue_subdomain_solves(ctx, r);
-    fine_src = ctx->d_subsol;
+    enqueue_subdomain_solves(ctx, r, w.subsol);
+    fine_src = w.subsol;
   }
   launch_overlap_sum(ctx->n, fine ? ctx->d_occ_ptr : nullptr, ctx->d_occ_pos, fine_src,
-                     coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, ctx->d_csol, z,
+                     coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, w.csol, z,
                      ctx->stream);
+}
+
+static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
+  enqueue_prec(ctx, r, z, ctx_work(ctx));
 }
 
 int doug_spmv_device(doug_ctx* ctx, const double* dx, double* dy) {
@@ -2066,6 +2085,65 @@ int doug_prec_apply_device(doug_ctx* ctx, const double* dr, double* dz) {
   if (!ctx) return DOUG_E_ARG;
   if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_prec_apply before doug_setup");
   enqueue_prec(ctx, dr, dz);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return DOUG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free preconditioner apply: the work vectors of an apply live in
+// one file-scope device array per GPU, of kNoAllocDoubles doubles (32 MiB),
+// which the loaded code object holds until the process ends.  Host code
+// never passes the array's name to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr int64_t kNoAllocDoubles = 4194304;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // per device ordinal
+
+static int scratch_address(doug_ctx* ctx, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= ctx->device) g_scratch_ptr.resize((size_t)ctx->device + 1, nullptr);
+  if (!g_scratch_ptr[ctx->device]) {
+    double* p = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(ctx, DOUG_E_HIP, std::string("hipGetSymbolAddress of the scratch array: ") +
+                                       hipGetErrorString(e));
+    }
+    g_scratch_ptr[ctx->device] = p;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return DOUG_OK;
+}
+
+extern "C" {
+
+int doug_prec_apply_device_noalloc(doug_ctx* ctx, const double* dr, double* dz) {
+  if (!ctx || !dr || !dz) return DOUG_E_ARG;
+  if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_prec_apply before doug_setup");
+  // layout: subdomain vectors, then the three coarse vectors and the partial
+  // sums, each part starting on a 256-byte boundary
+  auto up = [](int64_t v) { return (v + 31) / 32 * 32; };
+  const int64_t nsubv = up((int64_t)ctx->sinds.size());
+  const bool coarse = ctx->has_coarse && ctx->p.method != 4 && ctx->p.method != 0;
+  const int64_t cpad = coarse ? up((int64_t)ctx->cnt * kTile) : 0;
+  const int64_t npart = coarse ? up((int64_t)ctx->cnt * kTile * gemv_chunks(ctx->cnt)) : 0;
+  const int64_t need = nsubv + 3 * cpad + npart;
+  if (need > kNoAllocDoubles)
+    return fail(ctx, DOUG_E_ARG, "the allocation-free apply needs " + std::to_string(need) +
+                                     " doubles of work space; its scratch array holds " +
+                                     std::to_string(kNoAllocDoubles));
+  HIPCHK(hipSetDevice(ctx->device));
+  double* base = nullptr;
+  if (int rc = scratch_address(ctx, &base)) return rc;
+  PrecWork w{base, base + nsubv, base + nsubv + cpad, base + nsubv + 2 * cpad, base + nsubv + 3 * cpad};
+  // the coarse vectors' padding beyond n_c enters the tile GEMVs: start at zero
+  if (coarse) launch_zero(w.crhs, 3 * cpad, ctx->stream);
+  enqueue_prec(ctx, dr, dz, w);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return DOUG_OK;

@@ -824,6 +824,11 @@ __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src,
   if (i < n) dst[i] = src[i];
 }
 
+__global__ void k_zero(double* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = 0.0;
+}
+
 __global__ void k_residual(int n, const double* __restrict__ b, const double* __restrict__ ax,
                            double* __restrict__ r) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1793,6 +1798,11 @@ void launch_xr_update(int n, double* x, double* r, const double* p, const double
 
 void launch_copy(double* dst, const double* src, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), dim3(256), 0, st, dst, src, n);
+}
+
+void launch_zero(double* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n);
 }
 
 void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st) {

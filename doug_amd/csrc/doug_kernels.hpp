@@ -154,6 +154,8 @@ void launch_p_update(int n, const double* z, double* p, const double* scal, int 
 void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
                       double* partial, unsigned* counter, double* scal, hipStream_t st);
 void launch_copy(double* dst, const double* src, int n, hipStream_t st);
+// dst[0..n) = 0 (a kernel, so that captured graphs hold no memset node)
+void launch_zero(double* dst, int64_t n, hipStream_t st);
 void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st);
 
 // ---- several right-hand sides at once ----

@@ -237,6 +237,12 @@ class Solver:
         _fence()
         self._check(self._L.doug_prec_apply_device(self.h, C.c_void_p(dr), C.c_void_p(dz)))
 
+    def prec_device_noalloc(self, dr: int, dz: int):
+        """doug_prec_apply_device_noalloc: the apply with its work vectors in
+        the library's static per-GPU scratch array (no device allocation)."""
+        _fence()
+        self._check(self._L.doug_prec_apply_device_noalloc(self.h, C.c_void_p(dr), C.c_void_p(dz)))
+
     def pcg_device(self, db: int, dx: int, tol=1e-8, maxit=-1):
         _fence()
         cap = maxit if maxit > 0 else 100000

@@ -209,6 +209,18 @@ int doug_spmv_device(doug_ctx *ctx, const double *dx, double *dy);
 int doug_prec_apply(doug_ctx *ctx, const double *r, double *z);
 int doug_prec_apply_device(doug_ctx *ctx, const double *dr, double *dz);
 
+/* doug_prec_apply_device without any device allocation at or after the
+ * call: the apply's work vectors (the subdomain vectors, the coarse vectors
+ * and their partial sums) live in one static device array per GPU of
+ * 4194304 doubles (32 MiB), held until the process ends; a set-up needing
+ * more is refused with DOUG_E_ARG.  The results equal
+ * doug_prec_apply_device's bit for bit.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (several threads, or contexts on
+ * the same device) race.  No counterpart in the reference: it serves callers
+ * that cannot allocate (code under HIP graph capture, allocation-free
+ * serving loops). */
+int doug_prec_apply_device_noalloc(doug_ctx *ctx, const double *dr, double *dz);
+
 /* pcg_weigs (src/solvers/pcg.F90:289-462): x holds x0 on entry (DOUG's
  * driver passes zero) and the solution on return.  Stops when
  * ||r_k||^2 / ||r_0||^2 <= tol^2 or after maxit iterations (maxit == -1 means
