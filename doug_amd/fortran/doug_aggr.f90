@@ -64,6 +64,7 @@ program doug_aggr
   write(*, '(a)') ' Parsing DOUG control file: '//trim(ctlfile)
   call ctl_read(ctlfile, c, ok)
   if (.not. ok) call fatal('Unable to open DOUG control file: '//trim(ctlfile))
+  call ctl_print(c)
   if (c%solver /= 2) call fatal('[DOUG main] : Wrong solution method specified')
   if (c%input_type /= 1 .and. c%input_type /= 2 .and. c%input_type /= -1) &
     call fatal('[DOUG main] : Unrecognised input type.')
@@ -100,10 +101,19 @@ program doug_aggr
   else
     ! matrix and right-hand side (Distribution_assm.F90:65-76)
     write(*, '(a,a)') ' ##### Assembled input file: ##### ', trim(c%assembled_mtx_file)
+    if (c%assembled_mtx_format > 2) &
+      call fatal('[ReadInSparseAssembledHeader] Data format not recognized. 0=text, 1=binary, 2=XDR')
     call read_assembled(c%assembled_mtx_file, max(c%assembled_mtx_format, 0), n, nnz, &
                         indi, indj, val, ok)
     if (.not. ok) call fatal('Unable to read assembled file: '//trim(c%assembled_mtx_file))
     write(*, *) 'n,nnz:', n, nnz
+    ! the matrix as read, in the assembled text form (main/geom.F90:105-110
+    ! dumps D%A whatever the input type)
+    if (c%dump_matrix_only) then
+      call dump_matrix(c%dump_matrix_file, n, nnz, indi, indj, val, ok)
+      if (.not. ok) call fatal('[SpMtx_dumpMatrix] : Could not open file.')
+      stop
+    end if
     allocate(b(n), x(n))
     if (len_trim(c%assembled_rhs_file) > 0) then
       write(*, '(a,a)') ' ##### Assembled RHS file: ##### ', trim(c%assembled_rhs_file)

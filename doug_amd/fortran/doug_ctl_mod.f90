@@ -229,4 +229,65 @@ contains
     ok = .true.
   end subroutine ctl_read
 
+  !> CtrlData_print (DOUG_utils.F90:1155-1400): every value on the path as
+  !> `name = value`, integers (i5), reals (1p,e12.3), logicals (l5), so the
+  !> values the (i10)/(f10.5)/(l1) reads produced can be checked.
+  subroutine ctl_print(c)
+    type(ctl_values), intent(in) :: c
+    write(*, *)
+    write(*, *) 'Shared control parameters:'
+    call pi('solver', c%solver)
+    call pi('method', c%method)
+    call pi('fine_method', c%fine_method)
+    call pi('num_iters', c%num_iters)
+    call pi('coarse_method', c%coarse_method)
+    call pi('levels', c%levels)
+    call pi('overlap', c%overlap)
+    call pi('smoothers', c%smoothers)
+    call pi('input_type', c%input_type)
+    call pi('radius1', c%radius1)
+    call pi('radius2', c%radius2)
+    call pi('minasize1', c%minasize1)
+    call pi('minasize2', c%minasize2)
+    call pi('maxasize1', c%maxasize1)
+    call pi('maxasize2', c%maxasize2)
+    call pr('strong1', c%strong1)
+    call pr('strong2', c%strong2)
+    call pi('solve_maxiters', c%solve_maxiters)
+    call pr('solve_tolerance', c%solve_tolerance)
+    call pl('symmstruct', c%symmstruct)
+    call pl('symmnumeric', c%symmnumeric)
+    call pi('num_subdomains', c%num_subdomains)
+    call pi('virtual_ranks', c%virtual_ranks)
+    call pi('factor_precision', c%factor_precision)
+    write(*, *) 'Master control parameters:'
+    call pi('assembled_mtx_format', c%assembled_mtx_format)
+    call ps('assembled_mtx_file', c%assembled_mtx_file)
+    call pi('assembled_rhs_format', c%assembled_rhs_format)
+    call ps('assembled_rhs_file', c%assembled_rhs_file)
+    call pi('solution_format', c%solution_format)
+    call ps('solution_file', c%solution_file)
+    call pl('dump_matrix_only', c%dump_matrix_only)
+  contains
+    subroutine pi(w, v)
+      character(*), intent(in) :: w
+      integer, intent(in) :: v
+      write(*, '(a20,'' = '',i5)') w, v
+    end subroutine pi
+    subroutine pr(w, v)
+      character(*), intent(in) :: w
+      real(rk), intent(in) :: v
+      write(*, '(a20,'' = '',1p,e12.3)') w, v
+    end subroutine pr
+    subroutine pl(w, v)
+      character(*), intent(in) :: w
+      logical, intent(in) :: v
+      write(*, '(a20,'' = '',l5)') w, v
+    end subroutine pl
+    subroutine ps(w, v)
+      character(*), intent(in) :: w, v
+      write(*, '(a20,'' = '',a)') w, trim(v)
+    end subroutine ps
+  end subroutine ctl_print
+
 end module doug_ctl

@@ -49,8 +49,10 @@ contains
   end function little_endian
 
   !> Assembled matrix: text (0) `n nnz` then `i j v`, binary (1) two
-  !> unformatted records.  n and nnz are swapped when n > nnz; 0-based
-  !> indices are detected from the first entry.
+  !> unformatted records (n nnz; nnz triples of int32 i, int32 j, real8 v),
+  !> XDR (2) big-endian int32 n, nnz then int32 i, int32 j, double v per
+  !> entry.  n and nnz are swapped when n > nnz; 0-based indices are
+  !> detected from the first entry.
   subroutine read_assembled(fname, fmt, n, nnz, indi, indj, val, ok)
     character(*), intent(in) :: fname
     integer, intent(in) :: fmt
@@ -59,12 +61,22 @@ contains
     real(c_double), allocatable, intent(out) :: val(:)
     logical, intent(out) :: ok
     integer :: u, ios, k, t
+    integer(c_int32_t) :: h(2), ij(2)
+    real(c_double) :: v
     type(triple), allocatable :: tr(:)
     ok = .false.
     if (fmt == 1) then
       open(newunit=u, file=trim(fname), status='old', form='unformatted', iostat=ios)
       if (ios /= 0) return
       read(u, iostat=ios) n, nnz
+    else if (fmt == 2) then
+      open(newunit=u, file=trim(fname), status='old', access='stream', form='unformatted', &
+           iostat=ios)
+      if (ios /= 0) return
+      read(u, iostat=ios) h
+      if (little_endian()) h = [swap4(h(1)), swap4(h(2))]
+      n = h(1)
+      nnz = h(2)
     else
       open(newunit=u, file=trim(fname), status='old', form='formatted', iostat=ios)
       if (ios /= 0) return
@@ -85,6 +97,18 @@ contains
         indi(k) = tr(k)%i
         indj(k) = tr(k)%j
         val(k) = tr(k)%val
+      end do
+    else if (fmt == 2) then
+      do k = 1, nnz
+        read(u, iostat=ios) ij, v
+        if (ios /= 0) return
+        if (little_endian()) then
+          ij = [swap4(ij(1)), swap4(ij(2))]
+          v = swap8(v)
+        end if
+        indi(k) = ij(1)
+        indj(k) = ij(2)
+        val(k) = v
       end do
     else
       do k = 1, nnz

@@ -93,3 +93,223 @@ def test_virtual_ranks_run_matches_binding(tmp_path):
         s.setup(0)
         xb, it, _, _ = s.pcg(b, 1e-12, 300)
     np.testing.assert_array_equal(x.view(np.int64), xb.view(np.int64))
+
+
+# ---- DOUG's file formats: control values, matrix, right-hand side, solution
+
+def _ctl_value(out, key):
+    for line in out.splitlines():
+        if line.strip().startswith(key + " ="):
+            return line.split("=", 1)[1].strip()
+    raise AssertionError(f"{key} not printed")
+
+
+def test_control_reals_read_with_implied_decimals(tmp_path):
+    # reals are read with (f10.5) (DOUG_utils.F90:1095-1099): a value without
+    # a decimal point has five implied decimals, one with a point or an
+    # exponent is taken as written; integers (i10), logicals (l1)
+    (tmp_path / "DOUG.ctl").write_text(
+        "solver 2\nsolve_tolerance 1\nstrong1 0.67e0\nstrong2 25\nradius1 3\n"
+        "symmstruct T\nsymmnumeric F\nassembled_mtx_file none.txt\n")
+    p = _run(tmp_path, "-f", "DOUG.ctl")
+    assert p.returncode != 0 and "Unable to read assembled file" in p.stdout
+    assert float(_ctl_value(p.stdout, "solve_tolerance")) == 1e-5
+    assert float(_ctl_value(p.stdout, "strong1")) == 0.67
+    assert float(_ctl_value(p.stdout, "strong2")) == 2.5e-4
+    assert _ctl_value(p.stdout, "radius1") == "3"
+    assert _ctl_value(p.stdout, "symmstruct") == "T"
+    assert _ctl_value(p.stdout, "symmnumeric") == "F"
+    # the lap4x4 control file's own values
+    shutil.copy(os.path.join(GOLDEN, "lap4x4", "DOUG.ctl"), tmp_path / "lap.ctl")
+    p = _run(tmp_path, "-f", "lap.ctl")
+    assert float(_ctl_value(p.stdout, "solve_tolerance")) == 1e-12
+    assert float(_ctl_value(p.stdout, "strong2")) == 0.67
+    assert _ctl_value(p.stdout, "assembled_rhs_format") == "1"
+    assert _ctl_value(p.stdout, "solution_format") == "1"
+
+
+def _fortran_records(*records):
+    """Fortran unformatted sequential records: 4-byte length markers."""
+    out = b""
+    for r in records:
+        out += np.int32(len(r)).tobytes() + r + np.int32(len(r)).tobytes()
+    return out
+
+
+def _write_matrix(path, fmt, n, i, j, v, header=None):
+    nnz = len(v)
+    h = header or (n, nnz)
+    if fmt == 0:
+        with open(path, "w") as f:
+            f.write(f"{h[0]} {h[1]}\n")
+            for a, b, c in zip(i, j, v):
+                f.write(f"{a} {b} {c:.17g}\n")
+    elif fmt == 1:
+        tr = np.zeros(nnz, dtype=[("i", "<i4"), ("j", "<i4"), ("v", "<f8")])
+        tr["i"], tr["j"], tr["v"] = i, j, v
+        open(path, "wb").write(_fortran_records(np.array(h, "<i4").tobytes(), tr.tobytes()))
+    else:
+        tr = np.zeros(nnz, dtype=[("i", ">i4"), ("j", ">i4"), ("v", ">f8")])
+        tr["i"], tr["j"], tr["v"] = i, j, v
+        open(path, "wb").write(np.array(h, ">i4").tobytes() + tr.tobytes())
+
+
+def _dumped(path):
+    with open(path) as f:
+        n, nnz = map(int, f.readline().split())
+        d = np.loadtxt(f, ndmin=2)
+    return n, nnz, d[:, 0].astype(int), d[:, 1].astype(int), d[:, 2]
+
+
+@pytest.mark.parametrize("fmt,base,swap", [(0, 1, False), (0, 0, True), (1, 1, False),
+                                           (1, 0, True), (2, 1, False), (2, 0, False)])
+def test_matrix_formats_read_as_the_text_file(tmp_path, fmt, base, swap):
+    # ReadInSparseAssembled (SpMtx_class.F90:410-597): text (0), binary (1:
+    # a record `n nnz` then one record of nnz (int, int, real8) triples) and
+    # XDR (2); n and nnz swapped in the header are put back, 0-based indices
+    # are detected from the first entry.  dump_matrix_only writes the matrix
+    # as read (main/geom.F90:105-110) and stops before any GPU work.
+    from conftest import read_assembled_text
+    n, i, j, v = read_assembled_text(os.path.join(GOLDEN, "lap4x4", "Lap4x4.txt"))
+    rng = np.random.default_rng(fmt)
+    v = v * (1 + rng.uniform(-0.3, 0.3, len(v)))  # values that need every bit
+    ext = {0: "txt", 1: "bin", 2: "xdr"}[fmt]
+    _write_matrix(tmp_path / f"A.{ext}", fmt, n, i - 1 + base, j - 1 + base, v,
+                  header=(len(v), n) if swap else None)
+    (tmp_path / "DOUG.ctl").write_text(
+        f"solver 2\ninput_type 2\nassembled_mtx_file A.{ext}\nassembled_mtx_format {fmt}\n"
+        "dump_matrix_only T\ndump_matrix_file dump.txt\n")
+    p = _run(tmp_path, "-f", "DOUG.ctl")
+    assert p.returncode == 0, p.stdout + p.stderr
+    n2, nnz2, i2, j2, v2 = _dumped(tmp_path / "dump.txt")
+    assert (n2, nnz2) == (n, len(v))
+    order = np.argsort(i, kind="stable")
+    np.testing.assert_array_equal(i2, i[order])
+    np.testing.assert_array_equal(j2, j[order])
+    np.testing.assert_allclose(v2, v[order], rtol=1e-15)  # dump is (E24.16)
+
+
+def test_unknown_matrix_format_aborts(tmp_path):
+    shutil.copy(os.path.join(GOLDEN, "lap4x4", "Lap4x4.txt"), tmp_path / "A.txt")
+    (tmp_path / "DOUG.ctl").write_text("solver 2\nassembled_mtx_file A.txt\nassembled_mtx_format 3\n")
+    p = _run(tmp_path, "-f", "DOUG.ctl")
+    assert p.returncode != 0 and "Data format not recognized" in p.stdout
+
+
+def test_truncated_binary_matrix_aborts(tmp_path):
+    n, nnz = 9, 33
+    tr = np.zeros(nnz - 1, dtype=[("i", "<i4"), ("j", "<i4"), ("v", "<f8")])
+    open(tmp_path / "A.bin", "wb").write(
+        _fortran_records(np.array([n, nnz], "<i4").tobytes(), tr.tobytes()))
+    (tmp_path / "DOUG.ctl").write_text("solver 2\nassembled_mtx_file A.bin\nassembled_mtx_format 1\n"
+                                       "dump_matrix_only T\ndump_matrix_file d.txt\n")
+    p = _run(tmp_path, "-f", "DOUG.ctl")
+    assert p.returncode != 0 and "Unable to read assembled file" in p.stdout
+
+
+def _write_vector(path, fmt, x):
+    if fmt == 0:
+        # Vect_ReadFromFile_Text (Vect.F90:880-905): (i9) count, (e23.16) values
+        with open(path, "w") as f:
+            f.write(f"{len(x):9d}\n")
+            for t in x:
+                f.write(f"{t:23.16E}\n")
+    elif fmt == 1:
+        open(path, "wb").write(_fortran_records(np.asarray(x, "<f8").tobytes()))
+    else:
+        open(path, "wb").write(np.array([len(x)], ">i4").tobytes() + np.asarray(x, ">f8").tobytes())
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_rhs_formats_are_read(tmp_path, fmt):
+    # Vect_ReadFromFile (Vect.F90:840-951): the right-hand side is read
+    # before any GPU work, so without a GPU the run gets as far as creating
+    # the context; a file of the wrong length is refused in every format
+    src = os.path.join(GOLDEN, "lap4x4")
+    shutil.copy(os.path.join(src, "Lap4x4.txt"), tmp_path / "A.txt")
+    b = np.linspace(-1.0, 1.0, 9) / 3.0
+    ext = {0: "txt", 1: "bin", 2: "xdr"}[fmt]
+    for name, x in (("good", b), ("short", b[:8])):
+        _write_vector(tmp_path / f"{name}.{ext}", fmt, x)
+        (tmp_path / "DOUG.ctl").write_text(
+            f"solver 2\nassembled_mtx_file A.txt\nassembled_mtx_format 0\n"
+            f"assembled_rhs_file {name}.{ext}\nassembled_rhs_format {fmt}\n"
+            f"solution_format 0\nsolution_file x.txt\n")
+        p = _run(tmp_path, "-f", "DOUG.ctl")
+        if name == "good":
+            assert "Unable to read vector file" not in p.stdout
+            assert p.returncode == 0 or "no GPU available" in p.stdout, p.stdout
+        else:
+            # text and XDR carry the length; a binary record shorter than n
+            # ends before the last value
+            assert p.returncode != 0 and "Unable to read vector file" in p.stdout
+
+
+def _read_solution(path, fmt):
+    # WriteSolutionToFile (DOUG_utils.F90:1598-1700): (i6) + (e21.14) text,
+    # one unformatted record, or XDR
+    from conftest import read_fortran_record
+    if fmt == 0:
+        lines = open(path).read().split()
+        m = int(lines[0])
+        return np.array([float(t) for t in lines[1:1 + m]])
+    if fmt == 1:
+        return read_fortran_record(path)
+    return read_xdr_vector(path)
+
+
+@pytest.mark.gpu
+def test_lap4x4_known_answer_run(tmp_path):
+    # tests/assembled/lap4x4 end to end with its own DOUG.ctl: text matrix,
+    # binary right-hand side, binary solution against solution_correct.dat
+    from conftest import read_fortran_record
+    src = os.path.join(GOLDEN, "lap4x4")
+    for f in ("Lap4x4.txt", "rhs.dat", "DOUG.ctl"):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    print("driver lap4x4 DOUG.ctl", flush=True)
+    p = _run(tmp_path, "-f", "DOUG.ctl")
+    print(p.stdout[-1500:])
+    assert p.returncode == 0, p.stdout + p.stderr
+    x = read_fortran_record(tmp_path / "solution.dat")
+    xr = read_fortran_record(os.path.join(src, "solution_correct.dat"))
+    assert len(x) == 9
+    assert np.linalg.norm(x - xr) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_every_input_and_solution_format_gives_the_same_answer(tmp_path):
+    # the lap4x4 system read as text/binary/XDR matrix and text/binary/XDR
+    # right-hand side, the solution written in each format: binary and XDR
+    # carry every bit, text carries (e23.16) in and (e21.14) out
+    from conftest import read_assembled_text, read_fortran_record
+    src = os.path.join(GOLDEN, "lap4x4")
+    n, i, j, v = read_assembled_text(os.path.join(src, "Lap4x4.txt"))
+    b = read_fortran_record(os.path.join(src, "rhs.dat"))
+    xr = read_fortran_record(os.path.join(src, "solution_correct.dat"))
+    ctl = open(os.path.join(src, "DOUG.ctl")).read()
+    for k in ("assembled_mtx_file", "assembled_mtx_format", "assembled_rhs_format",
+              "assembled_rhs_file", "solution_format", "solution_file"):
+        ctl = "\n".join(l for l in ctl.splitlines() if not l.startswith(k)) + "\n"
+    ext = {0: "txt", 1: "bin", 2: "xdr"}
+    sols = {}
+    for mf, rf, sf in [(0, 1, 1), (1, 1, 2), (2, 2, 2), (0, 0, 0), (2, 0, 1)]:
+        d = tmp_path / f"m{mf}r{rf}s{sf}"
+        d.mkdir()
+        _write_matrix(d / f"A.{ext[mf]}", mf, n, i, j, v)
+        _write_vector(d / f"b.{ext[rf]}", rf, b)
+        (d / "DOUG.ctl").write_text(
+            f"assembled_mtx_file A.{ext[mf]}\nassembled_mtx_format {mf}\n"
+            f"assembled_rhs_file b.{ext[rf]}\nassembled_rhs_format {rf}\n"
+            f"solution_file x.{ext[sf]}\nsolution_format {sf}\n" + ctl)
+        print("driver lap4x4 formats", mf, rf, sf, flush=True)
+        p = _run(d, "-f", "DOUG.ctl")
+        assert p.returncode == 0, p.stdout + p.stderr
+        x = _read_solution(d / f"x.{ext[sf]}", sf)
+        assert len(x) == n
+        assert np.linalg.norm(x - xr) <= 1e-6
+        sols[(mf, rf, sf)] = x
+    exact = [sols[k] for k in sols if k[1] != 0 and k[2] != 0]
+    for x in exact[1:]:
+        np.testing.assert_array_equal(x.view(np.int64), exact[0].view(np.int64))
+    for k in sols:
+        np.testing.assert_allclose(sols[k], exact[0], rtol=1e-12, atol=1e-15)
