@@ -78,6 +78,8 @@ SIGNATURES = {
     "doug_set_partition": (_i32, [_vp, _i32, _vp, _vp, _i32]),
     "doug_set_num_subdomains": (_i32, [_vp, _i32]),
     "doug_set_aggregates": (_i32, [_vp, _i32, _vp]),
+    "doug_set_strong_mask": (_i32, [_vp, _i64, _vp]),
+    "doug_set_restriction_coo": (_i32, [_vp, _i32, _i64, _vp, _vp, _vp, _i32]),
     "doug_set_subdomains": (_i32, [_vp, _i32, _vp, _vp, _i32]),
     "doug_set_ranks": (_i32, [_vp, _i32, _vp, _i32]),
     "doug_get_rank_map": (_i32, [_vp, _i32, C.POINTER(_i32), C.POINTER(_i32), _vp]),

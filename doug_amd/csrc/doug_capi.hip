@@ -62,6 +62,10 @@ struct doug_ctx {
   std::vector<int> eptn;
   // caller aggregates (doug_set_aggregates) and subdomains (doug_set_subdomains)
   int inj_nagr = 0;
+  // caller's strength flags (per input entry, input order) and restriction
+  std::vector<uint8_t> inj_strong;
+  Csr inj_R;
+  bool have_inj_R = false;
   std::vector<int> inj_num;
   std::vector<int> inj_sstarts, inj_sinds;
   int ol_dist = 0;
@@ -440,6 +444,65 @@ int doug_set_aggregates(doug_ctx* ctx, int32_t nagr, const int32_t* num) {
   return DOUG_OK;
 }
 
+int doug_set_strong_mask(doug_ctx* ctx, int64_t nnz, const uint8_t* strong) {
+  if (!ctx) return DOUG_E_ARG;
+  ctx->ready = ctx->host_ready = false;
+  if (!strong) {
+    ctx->inj_strong.clear();
+    return DOUG_OK;
+  }
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "doug_set_strong_mask before doug_set_matrix_coo");
+  if (nnz != (int64_t)ctx->A.src.size())
+    return fail(ctx, DOUG_E_ARG, "the strength mask needs one flag per matrix entry (" +
+                                     std::to_string(ctx->A.src.size()) + ")");
+  ctx->inj_strong.assign(strong, strong + nnz);
+  return DOUG_OK;
+}
+
+int doug_set_restriction_coo(doug_ctx* ctx, int32_t nagr, int64_t nnz, const int32_t* ri,
+                             const int32_t* rj, const double* rv, int32_t base) {
+  if (!ctx) return DOUG_E_ARG;
+  ctx->ready = ctx->host_ready = false;
+  if (nagr <= 0) {
+    ctx->have_inj_R = false;
+    ctx->inj_R = Csr();
+    return DOUG_OK;
+  }
+  if (!ctx->have_matrix) return fail(ctx, DOUG_E_STATE, "doug_set_restriction_coo before doug_set_matrix_coo");
+  if (nnz < 0 || (nnz > 0 && (!ri || !rj || !rv)) || (base != 0 && base != 1))
+    return fail(ctx, DOUG_E_ARG, "bad restriction triples");
+  // sorted by (row, column), a repeated pair summed in input order
+  // (SpMtx_consolidate, SpMtx_arrangement.F90:258-292)
+  std::vector<int64_t> ord((size_t)nnz);
+  for (int64_t t = 0; t < nnz; t++) {
+    const int64_t r = (int64_t)ri[t] - base, c = (int64_t)rj[t] - base;
+    if (r < 0 || r >= nagr || c < 0 || c >= ctx->n)
+      return fail(ctx, DOUG_E_ARG, "restriction entry " + std::to_string(t) + " out of range");
+    ord[(size_t)t] = t;
+  }
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+    return ri[a] != ri[b] ? ri[a] < ri[b] : rj[a] < rj[b];
+  });
+  Csr R;
+  R.nrows = nagr;
+  R.ncols = ctx->n;
+  R.ptr.assign((size_t)nagr + 1, 0);
+  for (int64_t t : ord) {
+    const int r = ri[t] - base, c = rj[t] - base;
+    if (!R.col.empty() && R.ptr[r + 1] > 0 && R.col.back() == c) {
+      R.val.back() = R.val.back() + rv[t];
+      continue;
+    }
+    R.col.push_back(c);
+    R.val.push_back(rv[t]);
+    R.ptr[r + 1]++;
+  }
+  for (int r = 0; r < nagr; r++) R.ptr[r + 1] += R.ptr[r];
+  ctx->inj_R = std::move(R);
+  ctx->have_inj_R = true;
+  return DOUG_OK;
+}
+
 int doug_set_subdomains(doug_ctx* ctx, int32_t nsub, const int32_t* starts, const int32_t* inds,
                         int32_t base) {
   if (!ctx) return DOUG_E_ARG;
@@ -675,40 +738,64 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
     }
     *chol_flops = f;
   }
-  // scatter map: lower-triangle entries of M(inds, inds), blocks in parallel
-  std::vector<std::vector<int64_t>> dsts((size_t)nth);
-  std::vector<std::vector<int32_t>> srcs((size_t)nth);
-  std::vector<int> dup((size_t)nth, 0);
+  // scatter map: lower-triangle entries of M(inds, inds), blocks in parallel.
+  // A pair (i, j) that M stores several times is summed, as the reference's
+  // factorisation does (umf4triplet2col in Fact.F90:102-110 adds duplicate
+  // triplets): its first occurrence stays in the plain scatter and the group
+  // of all its occurrences, in stored order, goes to the summing pass.
+  std::vector<std::vector<int64_t>> dsts((size_t)nth), gdsts((size_t)nth), gcnts((size_t)nth);
+  std::vector<std::vector<int32_t>> srcs((size_t)nth), gsrcs((size_t)nth);
   for_blocks([&](int t, int b, std::vector<int>& floc) {
     std::vector<int64_t> chk;
+    std::vector<char> rep;
     const TileRow* tr = rows.data() + row0[b];
     const int s0 = starts[b], s = starts[b + 1] - s0;
     for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
     for (int li = 0; li < s; li++) {
       int g = inds[s0 + li];
-      size_t row_begin = dsts[t].size();
+      const size_t r0 = dsts[t].size();
       for (int p = M.ptr[g]; p < M.ptr[g + 1]; p++) {
         int lj = floc[M.col[p]];
         if (lj < 0 || lj > li) continue;
         dsts[t].push_back(desc[b].base + tile_pos(tr, li, lj));
         srcs[t].push_back(p);
       }
-      // duplicate (i,j) pairs would race in the scatter
-      if (dsts[t].size() - row_begin > 1) {
-        chk.assign(dsts[t].begin() + row_begin, dsts[t].end());
-        std::sort(chk.begin(), chk.end());
-        if (std::adjacent_find(chk.begin(), chk.end()) != chk.end()) dup[t] = 1;
+      const size_t r1 = dsts[t].size();
+      if (r1 - r0 < 2) continue;
+      chk.assign(dsts[t].begin() + r0, dsts[t].end());
+      std::sort(chk.begin(), chk.end());
+      if (std::adjacent_find(chk.begin(), chk.end()) == chk.end()) continue;
+      // this row holds a repeated pair: group the occurrences, keep firsts
+      rep.assign(r1 - r0, 0);
+      size_t w = r0;
+      for (size_t a = r0; a < r1; a++) {
+        if (rep[a - r0]) continue;
+        int64_t cnt = 1;
+        for (size_t c = a + 1; c < r1; c++)
+          if (dsts[t][c] == dsts[t][a]) { rep[c - r0] = 1; cnt++; }
+        if (cnt > 1) {
+          gdsts[t].push_back(dsts[t][a]);
+          gcnts[t].push_back(cnt);
+          for (size_t c = a; c < r1; c++)
+            if (dsts[t][c] == dsts[t][a]) gsrcs[t].push_back(srcs[t][c]);
+        }
+        dsts[t][w] = dsts[t][a];
+        srcs[t][w] = srcs[t][a];
+        w++;
       }
+      dsts[t].resize(w);
+      srcs[t].resize(w);
     }
     for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
   });
-  for (int t = 0; t < nth; t++)
-    if (dup[t]) return fail(ctx, DOUG_E_UNSUPPORTED, "duplicate matrix entries inside a subdomain are not supported");
-  std::vector<int64_t> dst;
-  std::vector<int32_t> src;
+  std::vector<int64_t> dst, gdst, gseg(1, 0);
+  std::vector<int32_t> src, gsrc;
   for (int t = 0; t < nth; t++) {
     dst.insert(dst.end(), dsts[t].begin(), dsts[t].end());
     src.insert(src.end(), srcs[t].begin(), srcs[t].end());
+    gdst.insert(gdst.end(), gdsts[t].begin(), gdsts[t].end());
+    gsrc.insert(gsrc.end(), gsrcs[t].begin(), gsrcs[t].end());
+    for (int64_t c : gcnts[t]) gseg.push_back(gseg.back() + c);
   }
   if ((rc = dalloc(ctx, d_tiles, (size_t)base))) return rc;
   if ((rc = upload(ctx, d_desc, desc))) return rc;
@@ -721,6 +808,14 @@ static int build_blocks(doug_ctx* ctx, const Csr& M, const double* d_vals,
   for (auto& d : desc) maxpad = std::max(maxpad, d.nt * kTile - d.s);
   launch_pad_identity(*d_tiles, *d_desc, nb, maxpad, ctx->stream);
   launch_scatter_entries(*d_tiles, d_dst, d_src, d_vals, (int64_t)dst.size(), ctx->stream);
+  if (!gdst.empty()) {
+    int64_t *d_gdst, *d_gseg;
+    int32_t* d_gsrc;
+    if ((rc = upload(ctx, &d_gdst, gdst)) || (rc = upload(ctx, &d_gseg, gseg)) ||
+        (rc = upload(ctx, &d_gsrc, gsrc)))
+      return rc;
+    launch_scatter_sums(*d_tiles, d_gdst, d_gseg, d_gsrc, d_vals, (int64_t)gdst.size(), ctx->stream);
+  }
   // the host copies above must stay alive until the stream consumed them
   HIPCHK(hipStreamSynchronize(ctx->stream));
   HIPCHK(hipGetLastError());
@@ -861,6 +956,8 @@ static void phase_values(doug_ctx* ctx, std::vector<double>& scaled, std::vector
   // the reference continues with the values after the scale/unscale round trip
   scale_round_trip(A, ctx->A_input_val, symm, scaled, diag, A.val);
   strong = find_strong(A, scaled, diag, strong_conn1);
+  if (!ctx->inj_strong.empty())  // the caller's flags, moved to the row order
+    for (size_t q = 0; q < strong.size(); q++) strong[q] = ctx->inj_strong[(size_t)A.src[q]] != 0;
 }
 
 // (2) fine aggregates, smoothed restriction and Galerkin coarse matrix
@@ -937,6 +1034,24 @@ static void phase_coarse_ranks(doug_ctx* ctx, const std::vector<uint8_t>& strong
   }
   ctx->coarse_on_device = false;
   ctx->info.t_coarse_build = now() - t1;
+}
+
+// the caller's R (doug_set_restriction_coo): R^T and A_c = R A R^T on the
+// host, as phase_coarse_host forms them
+static int phase_coarse_given(doug_ctx* ctx) {
+  double t1 = now();
+  const bool coarse_parts = ctx->inj_sstarts.empty() && !ctx->have_partition && ctx->num_subdomains <= 1;
+  if (coarse_parts && ctx->inj_R.nrows != ctx->fin.nagr)
+    return fail(ctx, DOUG_E_ARG, "the restriction has " + std::to_string(ctx->inj_R.nrows) +
+                                     " rows and the fine aggregation " + std::to_string(ctx->fin.nagr) +
+                                     " aggregates: the subdomains from the coarse aggregation need them "
+                                     "equal (or give a partition)");
+  ctx->R = ctx->inj_R;
+  ctx->Rt = transpose(ctx->R);
+  ctx->AC = galerkin(ctx->A, ctx->R, ctx->Rt);
+  ctx->coarse_on_device = false;
+  ctx->info.t_coarse_build = now() - t1;
+  return DOUG_OK;
 }
 
 // whether the last aggregation was the virtual ranks' (their local matrices
@@ -1366,6 +1481,12 @@ static int check_params(doug_ctx* ctx) {
   }
   if (ctx->nranks > 0 && (ctx->inj_nagr > 0 || !ctx->inj_sstarts.empty()))
     return fail(ctx, DOUG_E_ARG, "virtual ranks make their own aggregates and subdomains");
+  if (ctx->nranks > 0 && (ctx->have_inj_R || !ctx->inj_strong.empty()))
+    return fail(ctx, DOUG_E_ARG, "virtual ranks make their own strength flags and restriction");
+  if (!ctx->inj_strong.empty() && ctx->inj_strong.size() != ctx->A.src.size())
+    return fail(ctx, DOUG_E_STATE, "the strength mask was set for another matrix: set it again");
+  if (ctx->have_inj_R && ctx->inj_R.ncols != ctx->n)
+    return fail(ctx, DOUG_E_STATE, "the restriction was set for another matrix: set it again");
   if (ctx->nranks > 0) {
     // several processes: a negative radius1 selects the graph partition of
     // the matrix and is negated (Distribution_assm.F90:127-134); the
@@ -1415,8 +1536,11 @@ static int setup_host(doug_ctx* ctx) {
   phase_aggregate(ctx, scaled, strong);
   if (rank_coarse(ctx))
     phase_coarse_ranks(ctx, strong);
+  else if (ctx->have_inj_R)
+    rc = phase_coarse_given(ctx);
   else
     phase_coarse_host(ctx, strong);
+  if (rc) return rc;
   phase_subdomains(ctx);
   fill_info(ctx);
   ctx->info.t_total = now() - t0;
@@ -1853,6 +1977,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
       phase_coarse_ranks(ctx, strong);
       return DOUG_OK;
     }
+    if (ctx->have_inj_R) return phase_coarse_given(ctx);
     if (ctx->smoothers == 1 && ctx->use_device_coarse) return coarse_build_device(ctx, strong);
     phase_coarse_host(ctx, strong);
     return DOUG_OK;

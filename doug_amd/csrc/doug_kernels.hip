@@ -140,6 +140,20 @@ __global__ void k_scatter(T* __restrict__ tiles, const int64_t* __restrict__ dst
   if (e < ne) tiles[dst[e]] = (T)val[src[e]];
 }
 
+// Entries that one (i, j) pair of M receives more than once: group g's
+// values val[src[seg[g]..seg[g+1])] summed in stored order, in FP64, into
+// tiles[dst[g]] (the first occurrence's plain store is overwritten).
+template <typename T>
+__global__ void k_scatter_sum(T* __restrict__ tiles, const int64_t* __restrict__ dst,
+                              const int64_t* __restrict__ seg, const int32_t* __restrict__ src,
+                              const double* __restrict__ val, int64_t ngroups) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  double s = 0.0;
+  for (int64_t k = seg[g]; k < seg[g + 1]; k++) s += val[src[k]];
+  tiles[dst[g]] = (T)s;
+}
+
 template <typename T>
 __global__ void k_pad_identity(T* __restrict__ tiles, const BlockDesc* __restrict__ d,
                                int maxpad) {
@@ -1497,6 +1511,14 @@ void launch_scatter_entries(T* tiles, const int64_t* dst, const int32_t* src, co
 }
 
 template <typename T>
+void launch_scatter_sums(T* tiles, const int64_t* dst, const int64_t* seg, const int32_t* src,
+                         const double* val, int64_t ngroups, hipStream_t st) {
+  if (ngroups <= 0) return;
+  hipLaunchKernelGGL(k_scatter_sum<T>, dim3(cdiv(ngroups, 256)), dim3(256), 0, st, tiles, dst, seg,
+                     src, val, ngroups);
+}
+
+template <typename T>
 void launch_pad_identity(T* tiles, const BlockDesc* d, int nblocks, int maxpad, hipStream_t st) {
   if (maxpad <= 0 || nblocks <= 0) return;
   hipLaunchKernelGGL(k_pad_identity<T>, dim3(cdiv(maxpad, 64), nblocks), dim3(64), 0, st, tiles,
@@ -1554,6 +1576,8 @@ void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, con
 #define DOUG_INSTANTIATE(T)                                                                    \
   template void launch_scatter_entries<T>(T*, const int64_t*, const int32_t*, const double*,   \
                                           int64_t, hipStream_t);                               \
+  template void launch_scatter_sums<T>(T*, const int64_t*, const int64_t*, const int32_t*,     \
+                                       const double*, int64_t, hipStream_t);                   \
   template void launch_pad_identity<T>(T*, const BlockDesc*, int, int, hipStream_t);           \
   template void launch_chol_diag<T>(T*, const BlockDesc*, int, int, int*, hipStream_t, bool);  \
   template void launch_chol_offdiag<T>(T*, const BlockDesc*, int, int, int, hipStream_t, bool); \

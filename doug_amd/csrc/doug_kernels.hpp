@@ -43,6 +43,12 @@ struct BlockDesc {
 template <typename T>
 void launch_scatter_entries(T* tiles, const int64_t* dst, const int32_t* src, const double* val,
                             int64_t nentries, hipStream_t st);
+// Positions that receive several entries (duplicate (i, j) pairs): group g
+// stores the FP64 sum of val[src[seg[g]..seg[g+1])], in that order, at
+// tiles[dst[g]].  Runs after launch_scatter_entries on the same stream.
+template <typename T>
+void launch_scatter_sums(T* tiles, const int64_t* dst, const int64_t* seg, const int32_t* src,
+                         const double* val, int64_t ngroups, hipStream_t st);
 // Envelope and entries of index blocks from the device copies of A (CSR),
 // the blocks' unknown lists `inds` and each unknown's ascending occurrence
 // positions in them (occ_ptr/occ_pos).  launch_block_envelope lowers fc

@@ -118,6 +118,27 @@ class Solver:
         a = np.ascontiguousarray(num, np.int32)
         self._check(self._L.doug_set_aggregates(self.h, int(a.max()), _lib.ptr(a)))
 
+    def set_strong_mask(self, strong):
+        """doug_set_strong_mask: one strength flag per matrix entry, in the
+        input order of set_matrix; None restores the computed flags."""
+        if strong is None:
+            self._check(self._L.doug_set_strong_mask(self.h, 0, None))
+            return
+        m = np.ascontiguousarray(strong, np.uint8)
+        self._check(self._L.doug_set_strong_mask(self.h, len(m), _lib.ptr(m)))
+
+    def set_restriction(self, nagr, ri, rj, rv, base=1):
+        """doug_set_restriction_coo: R (nagr x n) as COO triples; nagr 0
+        restores the smoothed aggregation's R."""
+        if nagr <= 0:
+            self._check(self._L.doug_set_restriction_coo(self.h, 0, 0, None, None, None, 1))
+            return
+        i = np.ascontiguousarray(ri, np.int32)
+        j = np.ascontiguousarray(rj, np.int32)
+        v = np.ascontiguousarray(rv, np.float64)
+        self._check(self._L.doug_set_restriction_coo(self.h, int(nagr), len(v), _lib.ptr(i),
+                                                     _lib.ptr(j), _lib.ptr(v), int(base)))
+
     def set_subdomains(self, starts, inds, base=1):
         """doug_set_subdomains: final index sets (overlap included)."""
         st = np.ascontiguousarray(starts, np.int32)

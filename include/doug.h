@@ -127,6 +127,28 @@ int doug_set_num_subdomains(doug_ctx *ctx, int32_t nsub);
  * aggregation follow from them.  nagr <= 0 restores the aggregation. */
 int doug_set_aggregates(doug_ctx *ctx, int32_t nagr, const int32_t *num);
 
+/* Strength flags of the fine matrix given by the caller in place of
+ * SpMtx_find_strong's (src/datatypes/SpMtx/SpMtx_aggregation.F90:829-899,
+ * the A%strong a DOUG run holds after it): one flag per COO entry of
+ * doug_set_matrix_coo, in its input order, nonzero = strong.  They select
+ * the aggregation's strong graph and the entries the prolongator smoother
+ * keeps (src/coarse/CoarseMtx.F90:164-220).  Not with doug_set_ranks, whose
+ * ranks compute their own.  strong == NULL restores the computed flags. */
+int doug_set_strong_mask(doug_ctx *ctx, int64_t nnz, const uint8_t *strong);
+
+/* Restriction R (nagr x n) given by the caller as COO triples in place of the
+ * smoothed aggregation's (IntRestBuild,
+ * src/coarse/CoarseMtx.F90:50-474): sorted by (row, column) with a repeated
+ * pair summed in input order, as SpMtx_consolidate does
+ * (src/datatypes/SpMtx/SpMtx_arrangement.F90:258-292).  The coarse matrix
+ * is R A R^T (CoarseMtxBuild, src/coarse/CoarseMtx.F90:476-516).  Subdomains
+ * from the coarse aggregation need nagr equal to the fine aggregate count
+ * (its coarse unknowns are the fine aggregates); with a partition, given
+ * subdomains or doug_set_num_subdomains any nagr works.  Not with
+ * doug_set_ranks.  nagr <= 0 restores the built restriction. */
+int doug_set_restriction_coo(doug_ctx *ctx, int32_t nagr, int64_t nnz, const int32_t *ri,
+                             const int32_t *rj, const double *rv, int32_t index_base);
+
 /* Final subdomain index sets, overlap included, in place of
  * FinePreconditioner_Init's (src/components/Preconditioner_base.F90:96-123),
  * e.g. Decomposition%subd(:)%inds of a DOUG run: starts has nsub+1 entries

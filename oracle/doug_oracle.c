@@ -92,6 +92,38 @@ static void *xcalloc(size_t n, size_t sz) {
 static _Thread_local int g_allmind = D_MAXINT;   /* Aggregate_mod.F90:348 `save` */
 static _Thread_local double g_beta = 1.0E-5;     /* SpMtx_aggregation.F90:83 initialised => saved */
 
+/* Caller-given strength flags and restriction for the next or_setup (the
+ * checker of the library's doug_set_strong_mask / doug_set_restriction_coo;
+ * no counterpart in the reference, which always computes both). */
+typedef struct {
+  long nnz;
+  const unsigned char *strong;   /* per input entry, input order */
+  int rnagr;
+  long rnnz;
+  const int *ri, *rj;            /* 1-based */
+  const double *rv;
+} InjSet;
+static _Thread_local InjSet g_inj;
+
+typedef struct { int i, j; long t; } RKey;
+static int rkey_cmp(const void *x, const void *y) {
+  const RKey *a = (const RKey *)x, *b = (const RKey *)y;
+  if (a->i != b->i) return a->i < b->i ? -1 : 1;
+  if (a->j != b->j) return a->j < b->j ? -1 : 1;
+  return a->t < b->t ? -1 : (a->t > b->t);
+}
+
+void or_set_injection(long nnz, const unsigned char *strong, int rnagr, long rnnz,
+                      const int *ri, const int *rj, const double *rv) {
+  g_inj.nnz = nnz;
+  g_inj.strong = strong;
+  g_inj.rnagr = rnagr;
+  g_inj.rnnz = rnnz;
+  g_inj.ri = ri;
+  g_inj.rj = rj;
+  g_inj.rv = rv;
+}
+
 void or_reset_process(void) {
   g_allmind = D_MAXINT;
   g_beta = 1.0E-5;
@@ -1392,6 +1424,8 @@ struct or_state {
   Env *cfact;
   int has_coarse;
   double *w1, *w2, *w3, *w4;
+  unsigned char *strong_in;  /* fine strength flags per input entry (one process) */
+  long nnz_in;
 };
 
 static void state_free(or_state *s) {
@@ -1404,9 +1438,14 @@ static void state_free(or_state *s) {
   free(s->sstarts); free(s->sinds);
   free(s->sgs_start); free(s->sgs_i); free(s->sgs_j); free(s->sgs_v);
   free(s->w1); free(s->w2); free(s->w3); free(s->w4);
+  free(s->strong_in);
   free(s);
 }
 void or_free(or_state *s) { state_free(s); }
+long or_get_strong(const or_state *s, unsigned char *strong) {
+  if (strong && s->strong_in) memcpy(strong, s->strong_in, (size_t)s->nnz_in);
+  return s->strong_in ? s->nnz_in : 0;
+}
 void or_free_ptr(void *q) { free(q); }
 
 /* Factorise_subdomain (subsolvers.F90:104-163): entries of A, in A's order,
@@ -1444,6 +1483,8 @@ static or_state *setup_finish(or_state *s);
 or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
                    const double *val, const or_params *pp, int nparts,
                    const int *pstarts, const int *pnodes) {
+  InjSet inj = g_inj;  /* consumed by this call whatever its outcome */
+  memset(&g_inj, 0, sizeof g_inj);
   or_state *s = (or_state *)xcalloc(1, sizeof(or_state));
   s->n = n;
   s->p = *pp;
@@ -1483,6 +1524,28 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
     int min_asize1 = p->minasize1 > 0 ? p->minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
     int max_asize1 = p->maxasize1 > 0 ? p->maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
     om_find_strong(A, strong_conn1);
+    if (inj.strong) {
+      /* the given flags, moved to the row arrangement om_find_strong made
+       * (a stable counting sort of the input entries on their row) */
+      if (inj.nnz != nnz) {
+        seterr("strength mask has %ld flags, the matrix %ld entries", inj.nnz, nnz);
+        state_free(s);
+        return NULL;
+      }
+      int *fill = IALLOC(n + 1);
+      for (int i = 1; i <= n; i++) fill[i] = A->bound[i];
+      for (long t = 0; t < nnz; t++) A->strong[fill[indi[t]]++] = inj.strong[t] != 0;
+      free(fill);
+    }
+    {
+      /* the flags in input order, for or_get_strong */
+      s->nnz_in = nnz;
+      s->strong_in = (unsigned char *)xcalloc((size_t)nnz + 1, 1);
+      int *fill = IALLOC(n + 1);
+      for (int i = 1; i <= n; i++) fill[i] = A->bound[i];
+      for (long t = 0; t < nnz; t++) s->strong_in[t] = A->strong[fill[indi[t]]++];
+      free(fill);
+    }
     om_aggregate(A, &s->fin, &s->ffull, radius1, min_asize1, max_asize1, strong_conn1, 0);
     om_unscale(A);
 
@@ -1495,7 +1558,40 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
       state_free(s);
       return NULL;
     }
-    s->R = int_rest_build(A, NULL, &s->fin, smoothers, p->omega);
+    if (inj.rnagr > 0) {
+      /* the given R, sorted by (row, column) with repeated pairs summed in
+       * input order (SpMtx_consolidate, SpMtx_arrangement.F90:258-292) */
+      if (nparts <= 0 && inj.rnagr != s->fin.nagr) {
+        seterr("restriction has %d rows, the fine aggregation %d aggregates", inj.rnagr, s->fin.nagr);
+        memset(&g_inj, 0, sizeof g_inj);
+        state_free(s);
+        return NULL;
+      }
+      long rn = inj.rnnz;
+      RKey *ord = (RKey *)xcalloc((size_t)rn + 1, sizeof(RKey));
+      for (long t = 0; t < rn; t++) ord[t] = (RKey){inj.ri[t], inj.rj[t], t};
+      qsort(ord, (size_t)rn, sizeof(RKey), rkey_cmp);  /* position breaks ties: stable */
+      OM *R = om_new(rn, inj.rnagr, n);
+      long w = 0;
+      for (long a = 0; a < rn; a++) {
+        long t = ord[a].t;
+        if (w > 0 && R->indi[w] == inj.ri[t] && R->indj[w] == inj.rj[t]) {
+          R->val[w] = R->val[w] + inj.rv[t];
+        } else {
+          w++;
+          R->indi[w] = inj.ri[t];
+          R->indj[w] = inj.rj[t];
+          R->val[w] = inj.rv[t];
+        }
+      }
+      R->nnz = w;
+      R->arr = ARR_NO;
+      free(ord);
+      s->R = R;
+    } else {
+      s->R = int_rest_build(A, NULL, &s->fin, smoothers, p->omega);
+    }
+    memset(&g_inj, 0, sizeof g_inj);
     s->AC = coarse_mtx_build(A, s->R, n);
     s->has_coarse = 1;
 

@@ -61,7 +61,19 @@ or_state *or_setup(int n, long nnz, const int *indi, const int *indj,
 or_state *or_setup_ranks(int n, long nnz, const int *indi, const int *indj, const double *val,
                          const or_params *p, int nranks, const int *rank_of_node,
                          int nsub_per_rank, int **lg_out, int *nlf_out, int *ninner_out);
+/* For the next or_setup of this thread only: strength flags per input entry
+ * (nnz of them, NULL = computed) replacing SpMtx_find_strong's, and a
+ * restriction given as rnnz 1-based triples with rnagr rows (rnagr = 0 =
+ * built) replacing the smoothed aggregation's.  No counterpart in the
+ * reference; it checks the library's injection entry points.  The arrays
+ * must stay alive until or_setup returns. */
+void or_set_injection(long nnz, const unsigned char *strong, int rnagr, long rnnz,
+                      const int *ri, const int *rj, const double *rv);
 void or_free(or_state *s);
+/* The fine strength flags of a one-process or_setup, one per input entry in
+ * input order (SpMtx_find_strong's A%strong, or the injected flags); returns
+ * their count (0 for a several-process state).  strong may be NULL. */
+long or_get_strong(const or_state *s, unsigned char *strong);
 void or_free_ptr(void *q);
 const char *or_last_error(void);
 /* worker threads of the parallel loops (ranks, subdomain factorisations and

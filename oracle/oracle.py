@@ -59,7 +59,11 @@ def bind(L):
     L.or_setup.restype = C.c_void_p
     L.or_setup.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
                            C.c_int, C.c_void_p, C.c_void_p]
+    L.or_set_injection.argtypes = [C.c_long, C.c_void_p, C.c_int, C.c_long, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
     L.or_free.argtypes = [C.c_void_p]
+    L.or_get_strong.restype = C.c_long
+    L.or_get_strong.argtypes = [C.c_void_p, C.c_void_p]
     L.or_free_ptr.argtypes = [C.c_void_p]
     L.or_setup_ranks.restype = C.c_void_p
     L.or_setup_ranks.argtypes = [C.c_int, C.c_long, _i32p, _i32p, _f64p, C.POINTER(Params),
@@ -110,7 +114,10 @@ class Oracle:
     """One doug_aggr process (numprocs == 1) restated on the CPU."""
 
     def __init__(self, n, indi, indj, val, p: Params | None = None, partition=None,
-                 fresh_process=True):
+                 fresh_process=True, strong=None, restriction=None):
+        """strong: flags per input entry replacing the computed strength;
+        restriction: (nagr, ri, rj, rv), 1-based triples replacing the
+        smoothed aggregation's R."""
         L = self.L = lib()
         if fresh_process:
             L.or_reset_process()
@@ -118,6 +125,20 @@ class Oracle:
         indi = np.ascontiguousarray(indi, dtype=np.int32)
         indj = np.ascontiguousarray(indj, dtype=np.int32)
         val = np.ascontiguousarray(val, dtype=np.float64)
+        keep = []
+        if strong is not None or restriction is not None:
+            sm = None if strong is None else np.ascontiguousarray(strong, dtype=np.uint8)
+            if restriction is not None:
+                nagr, ri, rj, rv = restriction
+                ri = np.ascontiguousarray(ri, dtype=np.int32)
+                rj = np.ascontiguousarray(rj, dtype=np.int32)
+                rv = np.ascontiguousarray(rv, dtype=np.float64)
+                keep += [ri, rj, rv]
+                L.or_set_injection(len(val), None if sm is None else sm.ctypes.data, int(nagr),
+                                   len(rv), ri.ctypes.data, rj.ctypes.data, rv.ctypes.data)
+            else:
+                L.or_set_injection(len(val), sm.ctypes.data, 0, 0, None, None, None)
+            keep.append(sm)
         if partition is not None:
             ps, pn = partition
             self._ps = np.ascontiguousarray(ps, dtype=np.int32)
@@ -166,6 +187,13 @@ class Oracle:
             self.h = None
 
     # ---- setup products ----
+    def strong(self):
+        """The fine strength flags per input entry (one process)."""
+        m = self.L.or_get_strong(self.h, None)
+        out = np.zeros(m, np.uint8)
+        self.L.or_get_strong(self.h, out.ctypes.data)
+        return out
+
     def fine_aggregates(self):
         """fAggr%inner as (num, starts, nodes), 1-based."""
         L = self.L

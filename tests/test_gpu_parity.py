@@ -595,19 +595,102 @@ def test_device_block_build_matches_host_scatter(precision):
             assert x == y
 
 
-def test_duplicate_entry_inside_a_subdomain_is_refused():
+def _split_entries(n, i, j, v):
+    """The same matrix with some entries given as several COO triples: two
+    diagonal entries split in two and three parts, two symmetric off-diagonal
+    pairs split in two."""
+    dg = np.flatnonzero(i == j)[[5, 40]]
+    off = np.flatnonzero(i != j)[[10, 100]]
+    v0 = v.copy()
+    I, J, V = [i], [j], [v0]
+    v0[dg[0]] = v[dg[0]] - 1.0
+    I.append([i[dg[0]]]); J.append([j[dg[0]]]); V.append([1.0])
+    v0[dg[1]] = v[dg[1]] - 2.0
+    I.append([i[dg[1]]] * 2); J.append([j[dg[1]]] * 2); V.append([1.5, 0.5])
+    for k in off:
+        a, b = i[k], j[k]
+        kt = np.flatnonzero((i == b) & (j == a))[0]
+        v0[k] = 0.25 * v[k]
+        v0[kt] = 0.25 * v[kt]
+        I.append([a, b]); J.append([b, a]); V.append([0.75 * v[k], 0.75 * v[kt]])
+    return (np.concatenate(I).astype(np.int32), np.concatenate(J).astype(np.int32),
+            np.concatenate([np.asarray(x, float) for x in V]))
+
+
+@pytest.mark.parametrize("partition", [True, False])
+def test_duplicate_entries_inside_subdomains_match_oracle(partition):
+    # repeated (i, j) triples are summed where the subdomain matrices are
+    # factorised, as umf4triplet2col does (Fact.F90:102-110); the SpMV adds
+    # them too, so the system is the consolidated one
     n, i, j, v = P.laplace2d(16)
-    d = np.flatnonzero(i == j)[5]
-    i2 = np.r_[i, i[d]]
-    j2 = np.r_[j, j[d]]
-    v2 = np.r_[v, 0.0]
-    print("duplicate entry", flush=True)
+    i2, j2, v2 = _split_entries(n, i, j, v)
+    part = P.block_partition((16, 16), 8) if partition else None
+    ctl = dict(CFG0_CTL)
+    b = P.uniform_rhs(n, 31)
+    print("duplicate entries, partition", partition, flush=True)
+    o = O.Oracle(n, i2, j2, v2, O.params(**ctl), partition=part)
     with D.Solver(0) as s:
-        s.set_params(**CFG0_CTL)
+        s.set_params(**ctl)
         s.set_matrix(n, i2, j2, v2, 1)
-        s.set_partition(*P.block_partition((16, 16), 8))
-        with pytest.raises(RuntimeError, match="duplicate"):
-            s.setup(0)
+        if part is not None:
+            s.set_partition(*part)
+        s.setup(0)
+        for a, c in zip(o.subdomains(), s.subdomains()):
+            np.testing.assert_array_equal(a, c)
+        r = np.random.default_rng(32).uniform(-1, 1, n)
+        z, zo = s.prec(r), o.prec(r)
+        assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+        x, it, _, _ = s.pcg(b, 1e-10, 500)
+        xo, ito, _, _ = o.pcg(b, 1e-10, 500)
+        assert it == ito, (it, ito)
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        A = np.zeros((n, n))
+        np.add.at(A, (i - 1, j - 1), v)
+        assert np.linalg.norm(A @ x - b) <= 1e-9 * np.linalg.norm(b)
+        # FP32 factors: the sums are formed in FP64 and rounded once
+        s.setup(1)
+        x32, it32, _, _ = s.pcg(b, 1e-10, 500)
+        assert np.linalg.norm(x32 - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("what", ["strong", "restriction"])
+def test_given_strength_flags_and_restriction_against_oracle(what):
+    # doug_set_strong_mask / doug_set_restriction_coo through the device
+    # set-up: R and A_c bit for bit, the preconditioner and PCG against the
+    # oracle given the same flags or R
+    n, i, j, v = P.diffusion3d(12, 1.0, 41)
+    ctl = dict(CFG0_CTL)
+    b = P.uniform_rhs(n, 42)
+    o0 = O.Oracle(n, i, j, v, O.params(**ctl))
+    if what == "strong":
+        flags = o0.strong().copy()
+        rng = np.random.default_rng(43)
+        flags[(i != j) & (rng.uniform(size=len(v)) < 0.3)] = 0
+        kw = dict(strong=flags)
+    else:
+        o1 = O.Oracle(n, i, j, v, O.params(**dict(ctl, smoothers=0)))
+        kw = dict(restriction=(o1.L.or_nagr_fine(o1.h), *o1.R()))
+    print("given", what, flush=True)
+    o = O.Oracle(n, i, j, v, O.params(**ctl), **kw)
+    with D.Solver(0) as s:
+        s.set_params(**ctl)
+        s.set_matrix(n, i, j, v, 1)
+        if what == "strong":
+            s.set_strong_mask(flags)
+        else:
+            s.set_restriction(*kw["restriction"])
+        s.setup(0)
+        assert _same_entries(s.restriction(), o.R(), True)
+        assert _same_entries(s.coarse_matrix(), o.AC(), False)
+        for a, c in zip(o.subdomains(), s.subdomains()):
+            np.testing.assert_array_equal(a, c)
+        r = np.random.default_rng(44).uniform(-1, 1, n)
+        z, zo = s.prec(r), o.prec(r)
+        assert np.linalg.norm(z - zo) <= 1e-11 * np.linalg.norm(zo)
+        x, it, _, _ = s.pcg(b, 1e-10, 500)
+        xo, ito, _, _ = o.pcg(b, 1e-10, 500)
+        assert it == ito, (it, ito)
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
 @pytest.mark.parametrize("variant", [
