@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <thread>
 #include <vector>
@@ -86,6 +87,10 @@ struct doug_ctx {
   // and whether its owner sends it (staging of the device coarse build)
   std::vector<int32_t> mp_proc, mp_order;
   std::vector<uint8_t> mp_sent;
+  // whether the host's A.val and strong hold the round-trip values and the
+  // strength flags (a several-process set-up forms them on the device and
+  // copies them back only when host code needs them)
+  bool host_values = true;
   bool use_device_coarse = true;   // DOUG_OPT_DEVICE_COARSE
   bool use_device_blocks = true;   // DOUG_OPT_DEVICE_BLOCKS
   bool keep_setup_temps = true;    // DOUG_OPT_KEEP_SETUP_TEMPS
@@ -129,6 +134,10 @@ struct doug_ctx {
   int nc = 0, cnt = 0;
   BlockDesc* d_cdesc = nullptr;
   double *d_ctiles = nullptr, *d_cW = nullptr;
+  // coarse unknown at each position of the coarse factor's order (reverse
+  // Cuthill-McKee of A_c), padded with the identity to cnt * 64 entries
+  int32_t* d_cperm = nullptr;
+  float* d_cW32 = nullptr;  // the coarse inverse rounded to FP32 (FP32-factor form)
   double* d_cpart = nullptr;  // partial sums of the chunked coarse GEMVs
   double* d_cPm = nullptr;    // partial panels of the chunked coarse products (batch)
   double *d_crhs = nullptr, *d_cu = nullptr, *d_csol = nullptr;
@@ -246,6 +255,8 @@ static int64_t scratch_slot_bytes(const doug_ctx* ctx) {
 static void free_device(doug_ctx* ctx, bool background = false) {
   drop_iteration_graph(ctx);
   ctx->d_valin = nullptr;
+  ctx->d_cperm = nullptr;
+  ctx->d_cW32 = nullptr;
   join_reaper(ctx);
   if (background && !ctx->allocs.empty()) {
     std::vector<void*> old;
@@ -1566,12 +1577,14 @@ static int set_trsv_order(doug_ctx* ctx, int order) {
 // Matrix, vectors, subdomain data and the subdomain factorisation (enqueued).
 // The matrix (values after the scale/unscale round trip) and its SpMV row
 // blocks on the device.
-static int upload_matrix(doug_ctx* ctx) {
+// Without `values` d_val is only allocated (the device forms the values).
+static int upload_matrix(doug_ctx* ctx, bool values = true) {
   const Csr& A = ctx->A;
   double t = now();
   int rc;
+  ctx->d_src = nullptr;  // uploaded by the first doug_value_grad_device
   if ((rc = upload(ctx, &ctx->d_ptr, A.ptr)) || (rc = upload(ctx, &ctx->d_col, A.col)) ||
-      (rc = upload(ctx, &ctx->d_val, A.val)) || (rc = upload(ctx, &ctx->d_src, A.src)))
+      (rc = values ? upload(ctx, &ctx->d_val, A.val) : dalloc(ctx, &ctx->d_val, A.col.size())))
     return rc;
   // several processes with overlap 1: A_ghost's rows smooth with the
   // values as read (the device coarse build)
@@ -1748,6 +1761,35 @@ static int upload_strong(doug_ctx* ctx, const std::vector<uint8_t>& strong) {
   return upload(ctx, &ctx->d_strong, strong);
 }
 
+// phase_values on the device from the uploaded values as read (d_valin):
+// d_val and d_strong; the host copies follow on demand (host_values_now)
+static int device_values(doug_ctx* ctx) {
+  const Params& p = ctx->p;
+  const int n = ctx->A.nrows;
+  double *d_diag, *d_sv;
+  int rc;
+  if ((rc = dalloc(ctx, &d_diag, (size_t)n)) || (rc = dalloc(ctx, &d_sv, (size_t)n)) ||
+      (rc = dalloc(ctx, &ctx->d_strong, ctx->A.col.size())))
+    return rc;
+  const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
+  launch_values_strong(n, ctx->d_ptr, ctx->d_col, ctx->d_valin, (p.symmstruct && p.symmnumeric) ? 1 : 0,
+                       strong_conn1, d_diag, d_sv, ctx->d_val, ctx->d_strong, ctx->stream);
+  HIPCHK(hipGetLastError());
+  ctx->host_values = false;
+  return DOUG_OK;
+}
+
+// the device's round-trip values and strength flags copied to the host
+static int host_values_now(doug_ctx* ctx) {
+  if (ctx->host_values) return DOUG_OK;
+  ctx->strong.resize(ctx->A.col.size());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(ctx->A.val.data(), ctx->d_val, ctx->A.val.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ctx->strong.data(), ctx->d_strong, ctx->strong.size(), hipMemcpyDeviceToHost));
+  ctx->host_values = true;
+  return DOUG_OK;
+}
+
 static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   const Params& p = ctx->p;
   doug_info& info = ctx->info;
@@ -1792,7 +1834,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
   for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
     if ((rc = dalloc(ctx, v, (size_t)n))) return rc;
-  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) || (rc = dalloc(ctx, &ctx->d_partial, kRedBlocks)) ||
+  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) || (rc = dalloc(ctx, &ctx->d_partial, (size_t)vec_partials(n))) ||
       (rc = dalloc(ctx, &ctx->d_counter, 1)))
     return rc;
   HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned), ctx->stream));
@@ -1800,7 +1842,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   HIPCHK(hipStreamSynchronize(ctx->stream));  // host staging vectors go out of scope
   info.t_upload += now() - t3;
 
-  if (p.fine_method == 2 && (rc = build_sgs(ctx, dinds))) return rc;
+  if (p.fine_method == 2 && ((rc = host_values_now(ctx)) || (rc = build_sgs(ctx, dinds)))) return rc;
   // ---- Factorise_subdomains (subsolvers.F90:78-163) ----
   int64_t elems = 0;
   size_t esize = ctx->precision == DOUG_PREC_FP32_FACTORS ? sizeof(float) : sizeof(double);
@@ -1869,8 +1911,13 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
     if ((rc = upload(ctx, &ctx->d_Rrowblk, rbR))) return rc;
     double* d_acval;
     if ((rc = upload(ctx, &d_acval, ctx->AC.val))) return rc;
-    std::vector<int> cst{0, ctx->nc}, cinds((size_t)ctx->nc);
-    for (int i = 0; i < ctx->nc; i++) cinds[i] = i;
+    // the factor holds A_c in reverse Cuthill-McKee order: its Cholesky
+    // factor then stays inside a narrow envelope, and the factorisation and
+    // the inverse skip the tiles outside it
+    const int nc = ctx->nc;
+    std::vector<int> cst{0, nc}, ident((size_t)nc), cinds;
+    std::iota(ident.begin(), ident.end(), 0);
+    block_rcm(ctx->AC, cst, ident, cinds);
     std::vector<BlockDesc> cdesc;
     std::vector<int> coffcount;
     int64_t celems = 0;
@@ -1882,6 +1929,34 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
     // trailing updates of a step run side by side
     ctx->cnt = cdesc[0].nt;
     const int cnt = ctx->cnt;
+    // envelope: first tile column of each tile row's lower part, made
+    // non-decreasing (no fill left of it), and the last tile row reaching
+    // each column
+    std::vector<int> pos((size_t)nc), fc((size_t)cnt), last((size_t)cnt);
+    for (int k = 0; k < nc; k++) pos[cinds[k]] = k;
+    for (int i = 0; i < cnt; i++) fc[i] = i;
+    for (int k = 0; k < nc; k++) {
+      const int g = cinds[k];
+      for (int q = ctx->AC.ptr[g]; q < ctx->AC.ptr[g + 1]; q++) {
+        const int l = pos[ctx->AC.col[q]];
+        if (l < k) fc[k / kTile] = std::min(fc[k / kTile], l / kTile);
+      }
+    }
+    for (int i = cnt - 2; i >= 0; i--) fc[i] = std::min(fc[i], fc[i + 1]);
+    for (int j = 0, i = 0; j < cnt; j++) {
+      while (i + 1 < cnt && fc[i + 1] <= j) i++;
+      last[j] = std::max(i, j);
+    }
+    {
+      std::vector<int32_t> perm((size_t)cnt * kTile), iperm((size_t)nc);
+      for (int k = 0; k < cnt * kTile; k++) perm[k] = k < nc ? cinds[k] : k;
+      for (int k = 0; k < nc; k++) iperm[cinds[k]] = k;
+      int32_t* d_iperm;
+      if ((rc = upload(ctx, &ctx->d_cperm, perm)) || (rc = upload(ctx, &d_iperm, iperm))) return rc;
+      // the prolongation reads the coarse solution in the factor's order
+      launch_map_index(ctx->d_Rtcol, doug_restriction_nnz(ctx), d_iperm, ctx->stream);
+      HIPCHK(hipStreamSynchronize(ctx->stream));  // perm and iperm go out of scope
+    }
     // the explicit inverse's step m follows the factorisation's step m on
     // the third stream, beside the factorisation's later steps (each a short
     // chain of small launches that leaves most of the GPU idle)
@@ -1896,14 +1971,20 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
     launch_inv_init(ctx->d_cW, ctx->d_cdesc, cnt, ctx->stream3);
     for (int j = 0; j < cnt; j++) {
       launch_chol_diag(ctx->d_ctiles, ctx->d_cdesc, 1, j, ctx->d_flag + 1, ctx->stream, false);
-      launch_chol_offdiag(ctx->d_ctiles, ctx->d_cdesc, 1, j, cnt - 1 - j, ctx->stream, false);
+      launch_chol_offdiag(ctx->d_ctiles, ctx->d_cdesc, 1, j, last[j] - j, ctx->stream, false);
       HIPCHK(hipEventRecord(ctx->cevents[j], ctx->stream));
       HIPCHK(hipStreamWaitEvent(ctx->stream3, ctx->cevents[j], 0));
-      launch_inv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, j, ctx->stream3);
-      launch_rl_update(ctx->d_ctiles, ctx->d_cdesc, cnt, j, ctx->stream);
+      launch_inv_step(ctx->d_ctiles, ctx->d_cW, ctx->d_cdesc, cnt, j, last[j], ctx->stream3);
+      launch_rl_update(ctx->d_ctiles, ctx->d_cdesc, cnt, j, last[j], ctx->stream);
     }
     HIPCHK(hipEventRecord(ctx->cevents[cnt], ctx->stream3));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->cevents[cnt], 0));
+    if (ctx->precision == DOUG_PREC_FP32_FACTORS) {
+      // the FP32-factor form applies the coarse inverse in FP32 too (half
+      // the bytes per apply; its products and sums stay in FP64)
+      if ((rc = dalloc(ctx, &ctx->d_cW32, (size_t)celems))) return rc;
+      launch_to_float(ctx->d_cW, ctx->d_cW32, celems, ctx->stream);
+    }
     HIPCHK(hipGetLastError());
     size_t cpad = (size_t)ctx->cnt * kTile;
     if ((rc = dalloc(ctx, &ctx->d_cpart, cpad * gemv_chunks(ctx->cnt)))) return rc;
@@ -1911,7 +1992,8 @@ static int device_coarse(doug_ctx* ctx, hipEvent_t ev2, hipEvent_t ev3) {
       if ((rc = dalloc(ctx, v, cpad))) return rc;
       HIPCHK(hipMemsetAsync(*v, 0, cpad * sizeof(double), ctx->stream));
     }
-    ctx->info.coarse_factor_bytes = 2 * celems * (int64_t)sizeof(double);
+    ctx->info.coarse_factor_bytes = 2 * celems * (int64_t)sizeof(double) +
+                                    (ctx->d_cW32 ? celems * (int64_t)sizeof(float) : 0);
   }
   HIPCHK(hipEventRecord(ev3, ctx->stream));
   return DOUG_OK;
@@ -1948,6 +2030,12 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   // hardware threads beside them
   std::thread agg;
   const bool early_agg = ctx->nranks > 0 && ctx->inj_nagr == 0 && !ctx->have_partition;
+  // several processes with the device coarse build: the ranks aggregate
+  // their own values as read, and the global round trip and strength go to
+  // the device
+  const bool dev_values = early_agg && ctx->ol_dist == 1 && ctx->smoothers == 1 &&
+                          ctx->use_device_coarse && ctx->has_coarse;
+  ctx->host_values = true;
   if (early_agg) {
     // the host threads are split between the two: the aggregation (the
     // longer) keeps all but `side` of them, this thread's round trip and
@@ -1959,7 +2047,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     });
     set_setup_thread_cap(side);
   }
-  phase_values(ctx, scaled, strong);
+  if (!dev_values) phase_values(ctx, scaled, strong);
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
     if (rank_coarse(ctx)) {
@@ -1974,6 +2062,7 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
         }
         (void)hipStreamSynchronize(ctx->stream);
       }
+      if (int r = host_values_now(ctx)) return r;
       phase_coarse_ranks(ctx, strong);
       return DOUG_OK;
     }
@@ -2004,27 +2093,36 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     if ((rc = coarse_space())) return rc;
   } else {
     if (!early_agg) agg = std::thread([&]() { phase_aggregate(ctx, scaled, strong); });
-    rc = upload_matrix(ctx);
-    if (!rc) rc = upload_strong(ctx, strong);
+    rc = upload_matrix(ctx, !dev_values);
+    if (!rc) rc = dev_values ? device_values(ctx) : upload_strong(ctx, strong);
     set_setup_thread_cap(0);
     agg.join();
     if (rc) return rc;
-    if ((rc = coarse_space())) return rc;
-    // the coarse factorisation runs on the GPU while the host prepares the
-    // subdomains; the subdomain factorisation then goes to the second stream
-    // beside it (the coarse one is a chain of small right-looking steps that
-    // leaves most of the GPU idle).  It starts after the matrix upload, and
-    // the first stream waits for its end before anything else.
+    // the subdomain factorisation goes to the second stream, after the
+    // matrix upload; the first stream waits for its end before anything
+    // else.  The several-process subdomains are known once the ranks have
+    // aggregated, so their factorisation runs beside the coarse space's
+    // construction and factorisation; the one-process subdomains come from
+    // the coarse aggregation, so they follow the coarse space, and their
+    // factorisation runs beside the coarse one (a chain of small steps that
+    // leaves most of the GPU idle).
     HIPCHK(hipEventRecord(ev[4], ctx->stream));
-    if ((rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+    auto fine = [&]() -> int {
+      phase_subdomains(ctx);
+      HIPCHK(hipStreamWaitEvent(ctx->stream2, ev[4], 0));
+      std::swap(ctx->stream, ctx->stream2);
+      int r = device_fine(ctx, ev[0], ev[1]);
+      std::swap(ctx->stream, ctx->stream2);
+      if (r) return r;
+      HIPCHK(hipEventRecord(ev[5], ctx->stream2));
+      return DOUG_OK;
+    };
+    if (rank_coarse(ctx)) {
+      if ((rc = fine()) || (rc = coarse_space()) || (rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+    } else {
+      if ((rc = coarse_space()) || (rc = device_coarse(ctx, ev[2], ev[3])) || (rc = fine())) return rc;
+    }
     coarse_done = true;
-    phase_subdomains(ctx);
-    HIPCHK(hipStreamWaitEvent(ctx->stream2, ev[4], 0));
-    std::swap(ctx->stream, ctx->stream2);
-    rc = device_fine(ctx, ev[0], ev[1]);
-    std::swap(ctx->stream, ctx->stream2);
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ev[5], ctx->stream2));
     HIPCHK(hipStreamWaitEvent(ctx->stream, ev[5], 0));
   }
   fill_info(ctx);
@@ -2155,11 +2253,14 @@ static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
   enqueue_subdomain_solves(ctx, r, ctx->d_subsol);
 }
 
-// preconditioner (pcg.F90:216-270): fine solves, then coarse correction added
-static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWork& w) {
+// preconditioner (pcg.F90:216-270): fine solves, then coarse correction added.
+// With `dot`, PCG's r.z (rho and beta in d_scal) is formed with z.
+static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWork& w,
+                         bool dot = false) {
   const Params& p = ctx->p;
   if (p.method == 0) {
     launch_copy(z, r, ctx->n, ctx->stream);
+    if (dot) launch_dot(r, z, ctx->n, ctx->d_partial, nullptr, 1, ctx->d_scal, ctx->stream);
     return;
   }
   // prec2Level_solve (pcg.F90:191-208) with several processes, where the
@@ -2168,22 +2269,32 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWo
   const bool coarse = ctx->has_coarse && p.method != 4;
   const bool fine = !(ctx->has_coarse && p.method == 2);
   if (coarse) {
-    launch_spmv(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, w.crhs, ctx->stream);
-    launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, w.crhs, w.cu, w.cpart, ctx->stream);
-    launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, w.cu, w.csol, w.cpart, ctx->stream);
+    launch_spmv_wave_rows(ctx->nc, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, w.crhs, ctx->stream);
+    if (ctx->d_cW32) {
+      launch_gemv_w(ctx->d_cW32, ctx->d_cdesc, ctx->cnt, w.crhs, w.cu, w.cpart, ctx->stream, ctx->d_cperm);
+      launch_gemv_wt(ctx->d_cW32, ctx->d_cdesc, ctx->cnt, w.cu, w.csol, w.cpart, ctx->stream);
+    } else {
+      launch_gemv_w(ctx->d_cW, ctx->d_cdesc, ctx->cnt, w.crhs, w.cu, w.cpart, ctx->stream, ctx->d_cperm);
+      launch_gemv_wt(ctx->d_cW, ctx->d_cdesc, ctx->cnt, w.cu, w.csol, w.cpart, ctx->stream);
+    }
   }
   const double* fine_src = r;
   if (fine && p.fine_method >= 1) {
     enqueue_subdomain_solves(ctx, r, w.subsol);
     fine_src = w.subsol;
   }
-  launch_overlap_sum(ctx->n, fine ? ctx->d_occ_ptr : nullptr, ctx->d_occ_pos, fine_src,
-                     coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, w.csol, z,
-                     ctx->stream);
+  if (dot)
+    launch_overlap_sum_dot(ctx->n, fine ? ctx->d_occ_ptr : nullptr, ctx->d_occ_pos, fine_src,
+                           coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, w.csol, z, r,
+                           ctx->d_partial, ctx->d_scal, ctx->stream);
+  else
+    launch_overlap_sum(ctx->n, fine ? ctx->d_occ_ptr : nullptr, ctx->d_occ_pos, fine_src,
+                       coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, w.csol, z,
+                       ctx->stream);
 }
 
-static void enqueue_prec(doug_ctx* ctx, const double* r, double* z) {
-  enqueue_prec(ctx, r, z, ctx_work(ctx));
+static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, bool dot = false) {
+  enqueue_prec(ctx, r, z, ctx_work(ctx), dot);
 }
 
 int doug_spmv_device(doug_ctx* ctx, const double* dx, double* dy) {
@@ -2390,7 +2501,7 @@ static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, doubl
   // r = b - A x
   enqueue_spmv(ctx, ctx->d_x, ctx->d_tmp);
   launch_residual(n, ctx->d_b, ctx->d_tmp, ctx->d_r, st);
-  launch_dot(ctx->d_r, ctx->d_r, n, ctx->d_partial, ctx->d_counter, nullptr, 3, ctx->d_scal, st);
+  launch_dot(ctx->d_r, ctx->d_r, n, ctx->d_partial, nullptr, 3, ctx->d_scal, st);
   HIPCHK(hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   double init_norm = scal[S_RES];
@@ -2401,12 +2512,11 @@ static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, doubl
   // iteration body (pcg.F90:387-435); from the second iteration on it is
   // replayed from a captured HIP graph
   auto enqueue_iteration = [&](bool first) {
-    enqueue_prec(ctx, ctx->d_r, ctx->d_z);
-    launch_dot(ctx->d_r, ctx->d_z, n, ctx->d_partial, ctx->d_counter, nullptr, 1, ctx->d_scal, st);
+    enqueue_prec(ctx, ctx->d_r, ctx->d_z, true);  // with rho = r.z and beta
     launch_p_update(n, ctx->d_z, ctx->d_p, ctx->d_scal, first ? 1 : 0, st);
     enqueue_spmv(ctx, ctx->d_p, ctx->d_q);
-    launch_dot(ctx->d_p, ctx->d_q, n, ctx->d_partial, ctx->d_counter, nullptr, 2, ctx->d_scal, st);
-    launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_counter, ctx->d_scal, st);
+    launch_dot(ctx->d_p, ctx->d_q, n, ctx->d_partial, nullptr, 2, ctx->d_scal, st);
+    launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_scal, st);
     (void)hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st);
   };
   while (ratio > tol * tol && it < maxit) {
@@ -2538,7 +2648,8 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
     else
       launch_spmv_multi(ctx->nrbR, ctx->d_Rrowblk, ctx->d_Rptr, ctx->d_Rcol, ctx->d_Rval, r, nrp,
                         ctx->d_cG, NR, nrp, st);
-    launch_wgemm_fwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cG, ctx->d_cU, ctx->d_cPm, NR, st);
+    launch_wgemm_fwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->d_cG, ctx->d_cU, ctx->d_cPm, NR, st,
+                     ctx->d_cperm);
     launch_wgemm_bwd(ctx->d_cW, ctx->d_cdesc, ctx->cnt, ctx->cnt * kTile, ctx->d_cU, ctx->d_cS, NR, nrp,
                      ctx->d_cPm, NR, st);
   }
@@ -2744,6 +2855,10 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
 int doug_value_grad_device(doug_ctx* ctx, const double* dlam, const double* dx, double* dgrad) {
   if (!ctx || !dlam || !dx || !dgrad) return DOUG_E_ARG;
   if (!ctx->ready) return fail(ctx, DOUG_E_STATE, "doug_value_grad before doug_setup");
+  if (!ctx->d_src) {  // each entry's input position, needed by the gradient alone
+    if (int rc = upload(ctx, &ctx->d_src, ctx->A.src)) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   launch_value_grad(ctx->n, ctx->d_ptr, ctx->d_col, ctx->d_src, dlam, dx, dgrad, ctx->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));

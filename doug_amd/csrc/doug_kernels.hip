@@ -8,6 +8,7 @@
 // restriction, overlap sum, vector updates) keep products and sums as
 // separate roundings.
 #include "doug_kernels.hpp"
+#include <algorithm>
 
 #pragma clang fp contract(off)
 
@@ -571,9 +572,11 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
 // kGemvChunk tiles so that the whole chip streams W; each chunk writes a
 // partial 64-vector, summed afterwards chunk by chunk in a fixed order.
 // part_i(ch) = sum_{k in chunk ch, k <= i} W_ik c_k   (grid: chunk, tile row)
-__global__ __launch_bounds__(256) void k_gemv_w_part(const double* __restrict__ W,
+template <typename TW>
+__global__ __launch_bounds__(256) void k_gemv_w_part(const TW* __restrict__ W,
                                                      const BlockDesc* __restrict__ d,
                                                      const double* __restrict__ c,
+                                                     const int32_t* __restrict__ perm,
                                                      double* __restrict__ part, int nch) {
   const BlockDesc b = d[0];
   const int ch = blockIdx.x, i = blockIdx.y;
@@ -585,17 +588,19 @@ __global__ __launch_bounds__(256) void k_gemv_w_part(const double* __restrict__ 
 #pragma unroll
   for (int q = 0; q < 16; q++) acc[q] = 0.0;
   for (int k = k0; k < k1; k++) {
-    const double* Tk = W + b.base + toff(b, i, k) + (16 * w) * kTile + lane;
-    const double ck = c[k * kTile + lane];
+    const TW* Tk = W + b.base + toff(b, i, k) + (16 * w) * kTile + lane;
+    const int ci = k * kTile + lane;
+    const double ck = c[perm ? perm[ci] : ci];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] += Tk[q * kTile] * ck;
+    for (int q = 0; q < 16; q++) acc[q] += (double)Tk[q * kTile] * ck;
   }
   double s = reduce_scatter16(acc, lane);
   if ((lane & 3) == 0) part[((int64_t)i * nch + ch) * kTile + 16 * w + (lane >> 2)] = s;
 }
 
 // part_j(ch) = sum_{i in chunk ch of [j, nt)} W_ij^T u_i   (grid: chunk, tile column)
-__global__ __launch_bounds__(256) void k_gemv_wt_part(const double* __restrict__ W,
+template <typename TW>
+__global__ __launch_bounds__(256) void k_gemv_wt_part(const TW* __restrict__ W,
                                                       const BlockDesc* __restrict__ d,
                                                       const double* __restrict__ u,
                                                       double* __restrict__ part, int nch) {
@@ -608,10 +613,10 @@ __global__ __launch_bounds__(256) void k_gemv_wt_part(const double* __restrict__
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double acc = 0.0;
   for (int i = i0; i < i1; i++) {
-    const double* Ti = W + b.base + toff(b, i, j) + (16 * w) * kTile + lane;
+    const TW* Ti = W + b.base + toff(b, i, j) + (16 * w) * kTile + lane;
     const double* ui = u + i * kTile + 16 * w;
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc += Ti[q * kTile] * ui[q];
+    for (int q = 0; q < 16; q++) acc += (double)Ti[q * kTile] * ui[q];
   }
   red[w * kTile + lane] = acc;
   __syncthreads();
@@ -715,8 +720,9 @@ __global__ void k_overlap_sum(int n, const int32_t* __restrict__ occ_ptr,
 }
 
 // ------------------------------------------------------------------------
-// Deterministic reductions: fixed grid, per-block tree, last block sums the
-// partials in a fixed tree and applies the scalar recurrence.
+// Deterministic reductions: a fixed tree per workgroup, then the partials
+// summed in a fixed order (k_reduce_finish, or the last workgroup of the
+// several-column kernels) before the scalar recurrence.
 // ------------------------------------------------------------------------
 enum { S_RHO = 0, S_RHO_PREV = 1, S_BETA = 2, S_PQ = 3, S_ALPHA = 4, S_RES = 5 };
 
@@ -731,24 +737,25 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-__device__ void finish(double* partial, unsigned* counter, double* out, int mode, double* scal,
-                       double* sh) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // partial[] was written by other blocks: atomic loads read it at device
-  // scope instead of a possibly stale L1 line
+
+// The second stage of the single-vector reductions: one workgroup sums the
+// `count` workgroup partials (thread t takes t, t + 1024, ... in order, each
+// wavefront's by a butterfly, the 16 wavefronts' in order) and applies the
+// scalar recurrence of `mode`: 0 the sum to out,
+// 1 rho = r.z with beta, 2 p.q with alpha, 3 the squared residual norm.
+__global__ __launch_bounds__(1024) void k_reduce_finish(const double* __restrict__ partial, int count,
+                                                        double* out, int mode, double* scal) {
+  __shared__ double sh[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   double v = 0.0;
-  for (int p = threadIdx.x; p < (int)gridDim.x; p += blockDim.x)
-    v += __hip_atomic_load(&partial[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  double s = block_sum(v, sh);
-  if (threadIdx.x == 0) {
+  for (int q = t; q < count; q += 1024) v += partial[q];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (t == 0) {
+    double s = 0.0;
+    for (int k = 0; k < 16; k++) s += sh[k];
     if (mode == 0) {
       out[0] = s;
     } else if (mode == 1) {  // rho = r.z
@@ -761,76 +768,184 @@ __device__ void finish(double* partial, unsigned* counter, double* out, int mode
     } else {
       scal[S_RES] = s;
     }
-    *counter = 0u;
   }
 }
 
-__global__ __launch_bounds__(256) void k_dot(const double* __restrict__ a,
-                                             const double* __restrict__ b, int n,
-                                             double* partial, unsigned* counter, double* out,
-                                             int mode, double* scal) {
+// PCG vector kernels over chunks of kVecChunk consecutive entries, one
+// chunk per 256-thread workgroup, each thread taking entries t, t+256, ...
+// of its chunk (every load coalesced, four loads per array in flight per
+// thread).  The reductions sum each thread's entries in ascending order,
+// the workgroup by the fixed tree of block_sum, and k_reduce_finish the
+// workgroups' partials: the bits depend on n alone.
+constexpr int kVecPer = kVecChunk / 256;
+
+__global__ __launch_bounds__(256) void k_dot_chunks(const double* __restrict__ a,
+                                                    const double* __restrict__ b, int n,
+                                                    double* partial) {
   __shared__ double sh[4];
-  double v = 0.0;
-  const int G = gridDim.x * blockDim.x;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  // four strides loaded ahead, added in the same order as one at a time
-  for (; i + 3 * G < n; i += 4 * G) {
-    const double a0 = a[i], b0 = b[i], a1 = a[i + G], b1 = b[i + G];
-    const double a2 = a[i + 2 * G], b2 = b[i + 2 * G], a3 = a[i + 3 * G], b3 = b[i + 3 * G];
-    v = v + a0 * b0;
-    v = v + a1 * b1;
-    v = v + a2 * b2;
-    v = v + a3 * b3;
+  const int i0 = blockIdx.x * kVecChunk + threadIdx.x;
+  double av[kVecPer], bv[kVecPer];
+#pragma unroll
+  for (int k = 0; k < kVecPer; k++) {
+    const int i = i0 + 256 * k;
+    av[k] = i < n ? a[i] : 0.0;
+    bv[k] = i < n ? b[i] : 0.0;
   }
-  for (; i < n; i += G) v = v + a[i] * b[i];
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < kVecPer; k++) v = v + av[k] * bv[k];
   double s = block_sum(v, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  __syncthreads();
-  finish(partial, counter, out, mode, scal, sh);
 }
 
-__global__ void k_p_update(int n, const double* __restrict__ z, double* __restrict__ p,
-                           const double* __restrict__ scal, int first) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (first)
-    p[i] = z[i];
-  else
-    p[i] = z[i] + scal[S_BETA] * p[i];
-}
-
-__global__ __launch_bounds__(256) void k_xr_update(int n, double* __restrict__ x,
-                                                   double* __restrict__ r,
-                                                   const double* __restrict__ p,
-                                                   const double* __restrict__ q, double* partial,
-                                                   unsigned* counter, double* scal) {
-  __shared__ double sh[4];
-  const double alpha = scal[S_ALPHA];
-  double v = 0.0;
-  const int G = gridDim.x * blockDim.x;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  // two strides loaded ahead, the sum in the same order as one at a time
-  for (; i + G < n; i += 2 * G) {
-    const double x0 = x[i], p0 = p[i], r0 = r[i], q0 = q[i];
-    const double x1 = x[i + G], p1 = p[i + G], r1 = r[i + G], q1 = q[i + G];
-    x[i] = x0 + alpha * p0;
-    x[i + G] = x1 + alpha * p1;
-    const double ri0 = r0 - alpha * q0, ri1 = r1 - alpha * q1;
-    r[i] = ri0;
-    r[i + G] = ri1;
-    v = v + ri0 * ri0;
-    v = v + ri1 * ri1;
+__global__ __launch_bounds__(256) void k_p_update_chunks(int n, const double* __restrict__ z,
+                                                         double* __restrict__ p,
+                                                         const double* __restrict__ scal, int first) {
+  const int i0 = blockIdx.x * kVecChunk + threadIdx.x;
+  const double beta = first ? 0.0 : scal[S_BETA];
+  double zv[kVecPer], pv[kVecPer];
+#pragma unroll
+  for (int k = 0; k < kVecPer; k++) {
+    const int i = i0 + 256 * k;
+    zv[k] = i < n ? z[i] : 0.0;
+    pv[k] = (i < n && !first) ? p[i] : 0.0;
   }
-  for (; i < n; i += G) {
-    x[i] = x[i] + alpha * p[i];
-    double ri = r[i] - alpha * q[i];
-    r[i] = ri;
+#pragma unroll
+  for (int k = 0; k < kVecPer; k++) {
+    const int i = i0 + 256 * k;
+    if (i < n) p[i] = first ? zv[k] : zv[k] + beta * pv[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_xr_update_chunks(int n, double* __restrict__ x,
+                                                          double* __restrict__ r,
+                                                          const double* __restrict__ p,
+                                                          const double* __restrict__ q,
+                                                          const double* __restrict__ alpha_p,
+                                                          double* partial) {
+  __shared__ double sh[4];
+  const double alpha = *alpha_p;
+  const int i0 = blockIdx.x * kVecChunk + threadIdx.x;
+  double xv[kVecPer], rv[kVecPer], pv[kVecPer], qv[kVecPer];
+#pragma unroll
+  for (int k = 0; k < kVecPer; k++) {
+    const int i = i0 + 256 * k;
+    const bool in = i < n;
+    xv[k] = in ? x[i] : 0.0;
+    rv[k] = in ? r[i] : 0.0;
+    pv[k] = in ? p[i] : 0.0;
+    qv[k] = in ? q[i] : 0.0;
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < kVecPer; k++) {
+    const int i = i0 + 256 * k;
+    const double ri = rv[k] - alpha * qv[k];
+    if (i < n) {
+      x[i] = xv[k] + alpha * pv[k];
+      r[i] = ri;
+    }
     v = v + ri * ri;
   }
   double s = block_sum(v, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  __syncthreads();
-  finish(partial, counter, nullptr, 3, scal, sh);
+}
+
+// k_overlap_sum with the preconditioned residual's dot r.z (PCG's rho and
+// beta) folded in: one row per thread, the workgroup's products summed by
+// block_sum into one partial per workgroup for k_reduce_finish.
+__global__ __launch_bounds__(256) void k_overlap_sum_dot(
+    int n, const int32_t* __restrict__ occ_ptr, const int32_t* __restrict__ occ_pos,
+    const double* __restrict__ sol, const int32_t* __restrict__ rt_ptr,
+    const int32_t* __restrict__ rt_col, const double* __restrict__ rt_val,
+    const double* __restrict__ csol, double* __restrict__ z, const double* __restrict__ r,
+    double* partial) {
+  __shared__ double sh[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    double acc = 0.0;
+    if (occ_ptr)
+      for (int p = occ_ptr[i]; p < occ_ptr[i + 1]; p++) acc = acc + sol[occ_pos[p]];
+    if (rt_ptr) {
+      double tmp = 0.0;
+      for (int p = rt_ptr[i]; p < rt_ptr[i + 1]; p++) tmp = tmp + rt_val[p] * csol[rt_col[p]];
+      acc = acc + tmp;
+    }
+    z[i] = acc;
+    v = r[i] * acc;
+  }
+  double s = block_sum(v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// SpMtx_scale / SpMtx_find_strong / SpMtx_unscale on the device, one row
+// per thread, the same operations in the same order as the host's
+// scale_round_trip and find_strong: sv_i = sqrt|a_ii| (symmetric scaling) or
+// |a_ii| (the last diagonal entry of the row), a~_ij = (a_ij / sv_i) / sv_j,
+// the round-trip value (a~_ij * sv_i) * sv_j, and the strength flag
+// |a~_ij| > alpha * max_{k != i} |a~_ik| (the diagonal entry: a_ii > that bound).
+__global__ void k_row_scale(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ vin, int symm, double* __restrict__ diag,
+                            double* __restrict__ sv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int p = ptr[i]; p < ptr[i + 1]; p++)
+    if (col[p] == i) d = vin[p];
+  diag[i] = d;
+  sv[i] = symm ? sqrt(fabs(d)) : fabs(d);
+}
+
+__global__ void k_round_trip_strong(int n, const int32_t* __restrict__ ptr,
+                                    const int32_t* __restrict__ col, const double* __restrict__ vin,
+                                    const double* __restrict__ diag, const double* __restrict__ sv,
+                                    int symm, double alpha, double* __restrict__ val,
+                                    uint8_t* __restrict__ strong) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double si = sv[i];
+  double maxnd = (double)(-1.0e15f);
+  for (int p = ptr[i]; p < ptr[i + 1]; p++) {
+    const int c = col[p];
+    const double v = vin[p] / si;
+    const double s = symm ? v / sv[c] : v;
+    const double a = fabs(s);
+    if (c != i && maxnd < a) maxnd = a;
+  }
+  maxnd = maxnd * alpha;
+  for (int p = ptr[i]; p < ptr[i + 1]; p++) {
+    const int c = col[p];
+    const double sj = sv[c];
+    double v = vin[p] / si;
+    const double s = symm ? v / sj : v;
+    v = s * si;
+    val[p] = symm ? v * sj : v;
+    strong[p] = c != i ? (fabs(s) > maxnd) : (diag[i] > maxnd);
+  }
+}
+
+
+__global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+// y = A x for matrices with long rows (the restriction: hundreds of entries
+// per row): one wavefront per row, lane l summing entries l, l+64, ... in
+// order, then a fixed butterfly over the lanes.
+__global__ __launch_bounds__(256) void k_spmv_wave_rows(int nrows, const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  double acc = 0.0;
+  for (int e = ptr[row] + lane; e < ptr[row + 1]; e += 64) acc = acc + val[e] * x[col[e]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) y[row] = acc;
 }
 
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
@@ -937,6 +1052,7 @@ template <int NR>
 __global__ __launch_bounds__(256) void k_wgemm_fwd_part(const double* __restrict__ W,
                                                         const BlockDesc* __restrict__ d,
                                                         const double* __restrict__ G,
+                                                        const int32_t* __restrict__ perm,
                                                         double* __restrict__ part, int nch) {
   __shared__ double As[kTile * LD];
   __shared__ double Bs[kTile * (NR + 1)];
@@ -952,7 +1068,11 @@ __global__ __launch_bounds__(256) void k_wgemm_fwd_part(const double* __restrict
   for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
   for (int k = k0; k < k1; k++) {
     tile_to_lds(As, W + b.base + toff(b, i, k));
-    panel_to_lds<double, NR>(Bs, G + (int64_t)k * kTile * NR);
+    if (perm)  // rows of G in the factor's order
+      for (int e = threadIdx.x; e < kTile * NR; e += 256)
+        Bs[(e / NR) * (NR + 1) + (e % NR)] = G[(int64_t)perm[k * kTile + e / NR] * NR + (e % NR)];
+    else
+      panel_to_lds<double, NR>(Bs, G + (int64_t)k * kTile * NR);
     __syncthreads();
     panel_mma<double, NR, false>(acc, As, Bs, w, li, lk);
     __syncthreads();
@@ -1539,8 +1659,8 @@ void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int m
                      left ? 1 : 0);
 }
 
-void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, hipStream_t st) {
-  const int m = nt - 1 - j;
+void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, int last, hipStream_t st) {
+  const int m = std::min(last, nt - 1) - j;
   if (m <= 0) return;
   hipLaunchKernelGGL(k_rl_update, dim3(m * (m + 1) / 2), dim3(256), 0, st, tiles, d, j);
 }
@@ -1550,11 +1670,30 @@ void launch_inv_init(double* W, const BlockDesc* d, int nt, hipStream_t st) {
   hipLaunchKernelGGL(k_inv_init, dim3(nt, nt), dim3(256), 0, st, W, d);
 }
 
-void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m,
+void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m, int last,
                      hipStream_t st) {
   hipLaunchKernelGGL(k_inv_finish, dim3(m + 1), dim3(256), 0, st, L, W, d, m);
-  if (nt - m - 1 > 0)
-    hipLaunchKernelGGL(k_inv_update, dim3(m + 1, nt - m - 1), dim3(256), 0, st, L, W, d, m);
+  const int rows = std::min(last, nt - 1) - m;
+  if (rows > 0)
+    hipLaunchKernelGGL(k_inv_update, dim3(m + 1, rows), dim3(256), 0, st, L, W, d, m);
+}
+
+__global__ void k_map_index(int32_t* __restrict__ idx, int64_t n, const int32_t* __restrict__ map) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) idx[p] = map[idx[p]];
+}
+
+void launch_values_strong(int n, const int32_t* ptr, const int32_t* col, const double* vin, int symm,
+                          double alpha, double* diag, double* sv, double* val, uint8_t* strong,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_row_scale, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ptr, col, vin, symm, diag, sv);
+  hipLaunchKernelGGL(k_round_trip_strong, dim3(cdiv(n, 256)), dim3(256), 0, st, n, ptr, col, vin, diag,
+                     sv, symm, alpha, val, strong);
+}
+
+void launch_map_index(int32_t* idx, int64_t n, const int32_t* map, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_map_index, dim3(cdiv(n, 256)), dim3(256), 0, st, idx, n, map);
 }
 
 template <typename T>
@@ -1782,18 +1921,39 @@ void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32
   }
 }
 
-void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
-                   double* part, hipStream_t st) {
+template <typename TW>
+void launch_gemv_w(const TW* W, const BlockDesc* d, int nt, const double* c, double* u,
+                   double* part, hipStream_t st, const int32_t* perm) {
   const int nch = gemv_chunks(nt);
-  hipLaunchKernelGGL(k_gemv_w_part, dim3(nch, nt), dim3(256), 0, st, W, d, c, part, nch);
+  hipLaunchKernelGGL(k_gemv_w_part<TW>, dim3(nch, nt), dim3(256), 0, st, W, d, c, perm, part, nch);
   hipLaunchKernelGGL(k_gemv_sum, dim3(cdiv(nt * kTile, 256)), dim3(256), 0, st, part, nt, nch, 0, u);
 }
 
-void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u, double* c,
+template <typename TW>
+void launch_gemv_wt(const TW* W, const BlockDesc* d, int nt, const double* u, double* c,
                     double* part, hipStream_t st) {
   const int nch = gemv_chunks(nt);
-  hipLaunchKernelGGL(k_gemv_wt_part, dim3(nch, nt), dim3(256), 0, st, W, d, u, part, nch);
+  hipLaunchKernelGGL(k_gemv_wt_part<TW>, dim3(nch, nt), dim3(256), 0, st, W, d, u, part, nch);
   hipLaunchKernelGGL(k_gemv_sum, dim3(cdiv(nt * kTile, 256)), dim3(256), 0, st, part, nt, nch, 1, c);
+}
+template void launch_gemv_w<double>(const double*, const BlockDesc*, int, const double*, double*,
+                                    double*, hipStream_t, const int32_t*);
+template void launch_gemv_w<float>(const float*, const BlockDesc*, int, const double*, double*,
+                                   double*, hipStream_t, const int32_t*);
+template void launch_gemv_wt<double>(const double*, const BlockDesc*, int, const double*, double*,
+                                     double*, hipStream_t);
+template void launch_gemv_wt<float>(const float*, const BlockDesc*, int, const double*, double*,
+                                    double*, hipStream_t);
+
+void launch_to_float(const double* src, float* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_float, dim3(cdiv(n, 256)), dim3(256), 0, st, src, dst, n);
+}
+
+void launch_spmv_wave_rows(int nrows, const int32_t* ptr, const int32_t* col, const double* val,
+                           const double* x, double* y, hipStream_t st) {
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_spmv_wave_rows, dim3(cdiv(nrows, 4)), dim3(256), 0, st, nrows, ptr, col, val, x, y);
 }
 
 void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
@@ -1803,21 +1963,35 @@ void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, c
                      sol, rt_ptr, rt_col, rt_val, csol, z);
 }
 
-void launch_dot(const double* a, const double* b, int n, double* partial, unsigned* counter,
-                double* out, int mode, double* scal, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(256), 0, st, a, b, n, partial, counter, out,
-                     mode, scal);
+void launch_dot(const double* a, const double* b, int n, double* partial, double* out, int mode,
+                double* scal, hipStream_t st) {
+  const int nb = (int)cdiv(n, kVecChunk);
+  hipLaunchKernelGGL(k_dot_chunks, dim3(nb), dim3(256), 0, st, a, b, n, partial);
+  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, st, partial, nb, out, mode, scal);
 }
 
 void launch_p_update(int n, const double* z, double* p, const double* scal, int first,
                      hipStream_t st) {
-  hipLaunchKernelGGL(k_p_update, dim3(cdiv(n, 256)), dim3(256), 0, st, n, z, p, scal, first);
+  hipLaunchKernelGGL(k_p_update_chunks, dim3(cdiv(n, kVecChunk)), dim3(256), 0, st, n, z, p, scal,
+                     first);
 }
 
 void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
-                      double* partial, unsigned* counter, double* scal, hipStream_t st) {
-  hipLaunchKernelGGL(k_xr_update, dim3(kRedBlocks), dim3(256), 0, st, n, x, r, p, q, partial,
-                     counter, scal);
+                      double* partial, double* scal, hipStream_t st) {
+  const int nb = (int)cdiv(n, kVecChunk);
+  hipLaunchKernelGGL(k_xr_update_chunks, dim3(nb), dim3(256), 0, st, n, x, r, p, q,
+                     scal + S_ALPHA, partial);
+  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, st, partial, nb, nullptr, 3, scal);
+}
+
+void launch_overlap_sum_dot(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
+                            const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,
+                            const double* csol, double* z, const double* r, double* partial,
+                            double* scal, hipStream_t st) {
+  const int nb = (int)cdiv(n, 256);
+  hipLaunchKernelGGL(k_overlap_sum_dot, dim3(nb), dim3(256), 0, st, n, occ_ptr, occ_pos, sol, rt_ptr,
+                     rt_col, rt_val, csol, z, r, partial);
+  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, st, partial, nb, nullptr, 1, scal);
 }
 
 void launch_copy(double* dst, const double* src, int n, hipStream_t st) {
@@ -1850,14 +2024,14 @@ void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_
 }
 
 void launch_wgemm_fwd(const double* W, const BlockDesc* d, int nt, const double* G, double* U,
-                      double* part, int NR, hipStream_t st) {
+                      double* part, int NR, hipStream_t st, const int32_t* perm) {
   const int nch = gemv_chunks(nt);
   const dim3 g(nch, nt), gs(cdiv((int64_t)nt * kTile * NR, 256));
   if (NR == 16) {
-    hipLaunchKernelGGL(k_wgemm_fwd_part<16>, g, dim3(256), 0, st, W, d, G, part, nch);
+    hipLaunchKernelGGL(k_wgemm_fwd_part<16>, g, dim3(256), 0, st, W, d, G, perm, part, nch);
     hipLaunchKernelGGL(k_panel_sum<16>, gs, dim3(256), 0, st, part, nt, nch, 0, U, NR, nt * kTile, NR);
   } else {
-    hipLaunchKernelGGL(k_wgemm_fwd_part<32>, g, dim3(256), 0, st, W, d, G, part, nch);
+    hipLaunchKernelGGL(k_wgemm_fwd_part<32>, g, dim3(256), 0, st, W, d, G, perm, part, nch);
     hipLaunchKernelGGL(k_panel_sum<32>, gs, dim3(256), 0, st, part, nt, nch, 0, U, NR, nt * kTile, NR);
   }
 }

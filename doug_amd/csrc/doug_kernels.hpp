@@ -11,6 +11,9 @@ constexpr int kSpmvThreads = 256;      // CSR-stream block
 constexpr int kSpmvChunk = 2048;       // products staged in LDS per block
 constexpr int kMaxTrsvTiles = 256;     // subdomain size limit of the one-block solve (16384)
 constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reductions
+constexpr int kVecChunk = 1024;        // entries per workgroup of the PCG vector kernels
+// partial sums the single-vector reductions of an n-vector need
+constexpr int64_t vec_partials(int64_t n) { return (n + 255) / 256 + kRedBlocks; }
 constexpr int kMaxRhs = 32;            // right-hand sides per batched solve
 
 // Tile row i of a block's lower envelope: tiles (i, fc..i) are stored, the
@@ -79,15 +82,26 @@ template <typename T>
 void launch_chol_offdiag(T* tiles, const BlockDesc* d, int nblocks, int j, int maxcount,
                          hipStream_t st, bool left = true);
 // Dense single block d[0]: right-looking trailing update after step j,
-// A_ik -= L_ij L_kj^T for j < k <= i.
-void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, hipStream_t st);
+// A_ik -= L_ij L_kj^T for j < k <= i <= last, where `last` is the last tile
+// row whose envelope reaches column j (L_ij = 0 below it; nt - 1: all).
+void launch_rl_update(double* tiles, const BlockDesc* d, int nt, int j, int last, hipStream_t st);
 // W = L^{-1} of a dense factor (diagonal tiles inv(L_ii)): launch_inv_init,
-// then launch_inv_step for m = 0 .. nt-1 (right-looking substitution).  Step
-// m reads block column m of L alone: it may start once the factorisation's
-// step m has solved that column, while the later steps go on elsewhere.
+// then launch_inv_step for m = 0 .. nt-1 (right-looking substitution), with
+// `last` the last tile row of L's column m inside the envelope.  Step m reads
+// block column m of L alone: it may start once the factorisation's step m
+// has solved that column, while the later steps go on elsewhere.
 void launch_inv_init(double* W, const BlockDesc* d, int nt, hipStream_t st);
-void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m,
+void launch_inv_step(const double* L, double* W, const BlockDesc* d, int nt, int m, int last,
                      hipStream_t st);
+// The values after the scale/unscale round trip (val) and the strength
+// flags of the symmetric- (symm) or row-scaled matrix, from the values as
+// read; diag and sv are n-entry work arrays.  Bit-identical to the host's
+// scale_round_trip + find_strong.
+void launch_values_strong(int n, const int32_t* ptr, const int32_t* col, const double* vin, int symm,
+                          double alpha, double* diag, double* sv, double* val, uint8_t* strong,
+                          hipStream_t st);
+// idx[p] = map[idx[p]] (renumbering of column indices)
+void launch_map_index(int32_t* idx, int64_t n, const int32_t* map, hipStream_t st);
 
 // ---- per-iteration operators ----
 // y = A x, CSR-stream: each row summed in stored order, products rounded
@@ -115,10 +129,21 @@ void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, con
 // nt * gemv_chunks(nt) * 64 doubles of partial sums.
 constexpr int kGemvChunk = 8;  // tiles per workgroup
 constexpr int gemv_chunks(int nt) { return (nt + kGemvChunk - 1) / kGemvChunk; }
-void launch_gemv_w(const double* W, const BlockDesc* d, int nt, const double* c, double* u,
-                   double* part, hipStream_t st);
-void launch_gemv_wt(const double* W, const BlockDesc* d, int nt, const double* u, double* c,
+// perm (nt*64 entries, or null): u = W c(perm), W's rows and columns in the
+// factor's order, c in the original one.  W in FP64 or FP32 (products and
+// sums in FP64).
+template <typename TW>
+void launch_gemv_w(const TW* W, const BlockDesc* d, int nt, const double* c, double* u,
+                   double* part, hipStream_t st, const int32_t* perm = nullptr);
+template <typename TW>
+void launch_gemv_wt(const TW* W, const BlockDesc* d, int nt, const double* u, double* c,
                     double* part, hipStream_t st);
+// dst[0..n) = (float)src
+void launch_to_float(const double* src, float* dst, int64_t n, hipStream_t st);
+// y = A x with one wavefront per row (rows of hundreds of entries), each row
+// summed in a fixed order
+void launch_spmv_wave_rows(int nrows, const int32_t* ptr, const int32_t* col, const double* val,
+                           const double* x, double* y, hipStream_t st);
 // z[i] = sum over occurrences (ascending subdomain) of sol, then
 //        + sum_a Rt(i,a) csol[a]  (coarse prolongation), fine part first.
 // Symmetric Gauss-Seidel sweeps over each subdomain's own matrix (the
@@ -151,14 +176,22 @@ void launch_overlap_sum(int n, const int32_t* occ_ptr, const int32_t* occ_pos, c
                         const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,
                         const double* csol, double* z, hipStream_t st);
 
+// The same z, with r.z reduced into scal (rho and beta, as launch_dot's mode
+// 1); `partial` holds vec_partials(n) doubles.
+void launch_overlap_sum_dot(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,
+                            const int32_t* rt_ptr, const int32_t* rt_col, const double* rt_val,
+                            const double* csol, double* z, const double* r, double* partial,
+                            double* scal, hipStream_t st);
+
 // ---- PCG vector kernels with deterministic two-level reductions ----
-// scal layout (device doubles): see doug_capi.hip
-void launch_dot(const double* a, const double* b, int n, double* partial, unsigned* counter,
-                double* out, int mode, double* scal, hipStream_t st);
+// scal layout (device doubles): see doug_capi.hip.  `partial` holds
+// vec_partials(n) doubles.
+void launch_dot(const double* a, const double* b, int n, double* partial, double* out, int mode,
+                double* scal, hipStream_t st);
 void launch_p_update(int n, const double* z, double* p, const double* scal, int first,
                      hipStream_t st);
 void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
-                      double* partial, unsigned* counter, double* scal, hipStream_t st);
+                      double* partial, double* scal, hipStream_t st);
 void launch_copy(double* dst, const double* src, int n, hipStream_t st);
 // dst[0..n) = 0 (a kernel, so that captured graphs hold no memset node)
 void launch_zero(double* dst, int64_t n, hipStream_t st);
@@ -183,7 +216,7 @@ void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_
 // U_i = sum_{k<=i} W_ik G_k (all nt*64 rows), and
 // out(p, c) = (sum_{i>=j} W_ij^T U_i)(p, c) for p < rows, c < nrhs (leading dimension ldo).
 void launch_wgemm_fwd(const double* W, const BlockDesc* d, int nt, const double* G, double* U,
-                      double* part, int NR, hipStream_t st);
+                      double* part, int NR, hipStream_t st, const int32_t* perm = nullptr);
 void launch_wgemm_bwd(const double* W, const BlockDesc* d, int nt, int rows, const double* U,
                       double* out, int ldo, int nrhs, double* part, int NR, hipStream_t st);
 // y(:, r) = A x(:, r) for r < nrhs, each row summed in stored order

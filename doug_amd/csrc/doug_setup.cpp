@@ -3,6 +3,7 @@
 #include "doug_setup.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
 #include <queue>
@@ -1574,6 +1575,9 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
         mk.emplace_back([ws, t, n]() { ws->w[t] = std::make_unique<RankScratch>(n); });
     for (auto& x : mk) x.join();
   }
+  // ranks differ in cost (boundary boxes are smaller): each worker takes the
+  // next rank not yet taken; a rank's results do not depend on its worker
+  std::atomic<int> next_rank{0};
   std::vector<std::thread> th;
   for (int t = 0; t < nw; t++)
     th.emplace_back([&, t]() {
@@ -1581,7 +1585,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
       std::unique_ptr<RankScratch> mine;
       RankScratch* sp = ws ? ws->w[t].get() : (mine = std::make_unique<RankScratch>(n)).get();
       RankScratch& S = *sp;
-      for (int r = t; r < nranks; r += nw) {
+      for (int r = next_rank++; r < nranks; r = next_rank++) {
         RankLocal& R = out.ranks[r];
         rank_local(As, val, eptn, r, own[r], ol_dist, S, R, keep_local, pos ? pos->data() : nullptr);
         // getLocal (Aggregate_utils.f95:33-51): the own rows; a new matrix

@@ -441,10 +441,13 @@ def test_option_variants_against_oracle(variant):
         x, it, cond, _ = s.pcg(b, 1e-8, 2000)
         xo, ito, condo, _ = o.pcg(b, 1e-8, 2000)
         # dot products are fixed-order trees, not DOUG's sequential sums: over
-        # hundreds of weakly preconditioned iterations the stop may drift
+        # hundreds of weakly preconditioned iterations (plain CG takes ~265
+        # here) the iterates and the stop drift by a few iterations; both
+        # solutions then meet the stopping test, checked with the oracle's SpMV
         long_run = ito > 100
-        assert abs(it - ito) <= (max(1, ito // 100) if long_run else 0), (it, ito)
-        assert np.linalg.norm(x - xo) <= (1e-7 if long_run else 1e-8) * np.linalg.norm(xo)
+        assert abs(it - ito) <= (max(2, ito // 50) if long_run else 0), (it, ito)
+        assert np.linalg.norm(b - o.spmv(x)) <= 1.01e-8 * np.linalg.norm(b)
+        assert np.linalg.norm(x - xo) <= (1e-6 if long_run else 1e-8) * np.linalg.norm(xo)
         assert abs(cond - condo) <= (1e-3 if long_run else 1e-6) * condo
 
 
