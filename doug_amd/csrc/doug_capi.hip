@@ -1813,24 +1813,36 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
         std::sort(dinds.begin() + ctx->sstarts[b], dinds.begin() + ctx->sstarts[b + 1]);
   }
   // occurrence lists for the deterministic overlap sum (a node's
-  // occurrences in ascending subdomain order)
-  std::vector<int32_t> occ_ptr((size_t)n + 1, 0), occ_pos(dinds.size());
+  // occurrences in ascending subdomain order), formed on the device: counts
+  // per node, their prefix sum, and the positions stably sorted on their node
+  if ((rc = upload(ctx, &ctx->d_inds, dinds))) return rc;
+  const int64_t total = (int64_t)dinds.size();
+  if ((rc = dalloc(ctx, &ctx->d_occ_ptr, (size_t)n + 1))) return rc;
+  hipStream_t st = ctx->stream;
   if (p.fine_method >= 1) {
-    for (int v : dinds) occ_ptr[v + 1]++;
-    for (int i = 0; i < n; i++) occ_ptr[i + 1] += occ_ptr[i];
-    std::vector<int32_t> fill(occ_ptr.begin(), occ_ptr.end() - 1);
-    for (size_t q = 0; q < dinds.size(); q++) occ_pos[fill[dinds[q]]++] = (int32_t)q;
+    int32_t *cnt, *keys_out, *iota;
+    if ((rc = dalloc(ctx, &cnt, (size_t)n + 1)) || (rc = dalloc(ctx, &ctx->d_occ_pos, (size_t)total)) ||
+        (rc = dalloc(ctx, &keys_out, (size_t)total)) || (rc = dalloc(ctx, &iota, (size_t)total)))
+      return rc;
+    HIPCHK(hipMemsetAsync(cnt, 0, ((size_t)n + 1) * sizeof(int32_t), st));
+    launch_key_histogram(total, ctx->d_inds, cnt, st);
+    launch_iota(total, iota, st);
+    int end_bit = 1;
+    while (end_bit < 31 && (1 << end_bit) <= n) end_bit++;
+    size_t scan_bytes = 0, sort_bytes = 0;
+    HIPCHK(exclusive_scan(nullptr, scan_bytes, cnt, ctx->d_occ_ptr, n + 1, st));
+    HIPCHK(stable_sort_pairs(nullptr, sort_bytes, ctx->d_inds, keys_out, iota, ctx->d_occ_pos, (int)total,
+                             end_bit, st));
+    char* tmp;
+    if ((rc = dalloc(ctx, &tmp, std::max(scan_bytes, sort_bytes)))) return rc;
+    HIPCHK(exclusive_scan(tmp, scan_bytes, cnt, ctx->d_occ_ptr, n + 1, st));
+    HIPCHK(stable_sort_pairs(tmp, sort_bytes, ctx->d_inds, keys_out, iota, ctx->d_occ_pos, (int)total,
+                             end_bit, st));
   } else {  // no fine solves: z starts as r (FinePreconditioner_Apply, type NONE)
-    occ_pos.resize((size_t)n);
-    for (int i = 0; i < n; i++) {
-      occ_ptr[i + 1] = i + 1;
-      occ_pos[i] = i;
-    }
+    if ((rc = dalloc(ctx, &ctx->d_occ_pos, (size_t)n))) return rc;
+    launch_iota((int64_t)n + 1, ctx->d_occ_ptr, st);
+    launch_iota(n, ctx->d_occ_pos, st);
   }
-  if ((rc = upload(ctx, &ctx->d_occ_ptr, occ_ptr)) || (rc = upload(ctx, &ctx->d_occ_pos, occ_pos)))
-    return rc;
-  std::vector<int32_t> inds(dinds.begin(), dinds.end());
-  if ((rc = upload(ctx, &ctx->d_inds, inds))) return rc;
   if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
   for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
     if ((rc = dalloc(ctx, v, (size_t)n))) return rc;
@@ -2039,8 +2051,8 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
   if (early_agg) {
     // the host threads are split between the two: the aggregation (the
     // longer) keeps all but `side` of them, this thread's round trip and
-    // upload take `side`
-    const int all = setup_threads(), side = std::max(1, all / 8);
+    // upload take `side` (one thread when the device forms the values)
+    const int all = setup_threads(), side = dev_values ? 1 : std::max(1, all / 8);
     agg = std::thread([&ctx, &scaled, &strong, all, side]() {
       set_setup_thread_cap(std::max(1, all - side));
       phase_aggregate(ctx, scaled, strong);

@@ -572,53 +572,126 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
 // kGemvChunk tiles so that the whole chip streams W; each chunk writes a
 // partial 64-vector, summed afterwards chunk by chunk in a fixed order.
 // part_i(ch) = sum_{k in chunk ch, k <= i} W_ik c_k   (grid: chunk, tile row)
+// Vector loads of W's tiles: VEC entries (16 bytes) per lane, LPR lanes per
+// 64-entry tile row, RPI tile rows per wavefront load.
+template <typename TW>
+struct GemvVec {
+  static constexpr int VEC = 16 / (int)sizeof(TW), LPR = kTile / VEC, RPI = 64 / LPR, NI = 16 / RPI;
+};
+template <typename TW>
+struct Vec16Of;
+template <>
+struct Vec16Of<double> {
+  typedef double2 type;
+};
+template <>
+struct Vec16Of<float> {
+  typedef float4 type;
+};
+template <typename V>
+__device__ __forceinline__ double vget(const V& v, int e);
+template <>
+__device__ __forceinline__ double vget<double2>(const double2& v, int e) { return e == 0 ? v.x : v.y; }
+template <>
+__device__ __forceinline__ double vget<float4>(const float4& v, int e) {
+  return (double)(e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w);
+}
+
+// part_i(ch) = sum_{k in chunk ch, k <= i} W_ik c_k   (grid: chunk, tile row).
+// Wavefront w takes tile rows 16w..16w+15; lane (rs, cg) reads columns
+// cg*VEC.. of rows 16w + q*RPI + rs.  Each lane sums its VEC products in
+// column order, then its tiles in order; a butterfly over the LPR lanes of a
+// row finishes the row.
 template <typename TW>
 __global__ __launch_bounds__(256) void k_gemv_w_part(const TW* __restrict__ W,
                                                      const BlockDesc* __restrict__ d,
                                                      const double* __restrict__ c,
                                                      const int32_t* __restrict__ perm,
                                                      double* __restrict__ part, int nch) {
+  typedef GemvVec<TW> G;
+  typedef typename Vec16Of<TW>::type V;
+  __shared__ double cs[kGemvChunk * kTile];
   const BlockDesc b = d[0];
   const int ch = blockIdx.x, i = blockIdx.y;
   const int k0 = ch * kGemvChunk;
   if (i >= b.nt || k0 > i) return;
   const int k1 = min(i + 1, k0 + kGemvChunk);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  double acc[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) acc[q] = 0.0;
-  for (int k = k0; k < k1; k++) {
-    const TW* Tk = W + b.base + toff(b, i, k) + (16 * w) * kTile + lane;
-    const int ci = k * kTile + lane;
-    const double ck = c[perm ? perm[ci] : ci];
-#pragma unroll
-    for (int q = 0; q < 16; q++) acc[q] += (double)Tk[q * kTile] * ck;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, cg = lane % G::LPR, rs = lane / G::LPR;
+  // the chunk's slice of c, gathered through perm once
+  for (int e = t; e < (k1 - k0) * kTile; e += 256) {
+    const int ci = k0 * kTile + e;
+    cs[e] = c[perm ? perm[ci] : ci];
   }
-  double s = reduce_scatter16(acc, lane);
-  if ((lane & 3) == 0) part[((int64_t)i * nch + ch) * kTile + 16 * w + (lane >> 2)] = s;
+  __syncthreads();
+  double acc[G::NI];
+#pragma unroll
+  for (int q = 0; q < G::NI; q++) acc[q] = 0.0;
+  for (int k = k0; k < k1; k++) {
+    const TW* Tb = W + b.base + toff(b, i, k) + (16 * w + rs) * kTile + cg * G::VEC;
+    V wv[G::NI];
+#pragma unroll
+    for (int q = 0; q < G::NI; q++) wv[q] = *reinterpret_cast<const V*>(Tb + q * G::RPI * kTile);
+    double cv[G::VEC];
+#pragma unroll
+    for (int e = 0; e < G::VEC; e++) cv[e] = cs[(k - k0) * kTile + cg * G::VEC + e];
+#pragma unroll
+    for (int q = 0; q < G::NI; q++) {
+      double sdot = 0.0;
+#pragma unroll
+      for (int e = 0; e < G::VEC; e++) sdot = sdot + vget(wv[q], e) * cv[e];
+      acc[q] = acc[q] + sdot;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < G::NI; q++)
+#pragma unroll
+    for (int o = G::LPR / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+  if (cg == 0)
+#pragma unroll
+    for (int q = 0; q < G::NI; q++)
+      part[((int64_t)i * nch + ch) * kTile + 16 * w + q * G::RPI + rs] = acc[q];
 }
 
-// part_j(ch) = sum_{i in chunk ch of [j, nt)} W_ij^T u_i   (grid: chunk, tile column)
+// part_j(ch) = sum_{i in chunk ch of [j, nt)} W_ij^T u_i   (grid: chunk, tile
+// column): lane (rs, cg) accumulates its VEC columns over its rows in order,
+// a butterfly over the RPI lanes of a column, then the four wavefronts in order.
 template <typename TW>
 __global__ __launch_bounds__(256) void k_gemv_wt_part(const TW* __restrict__ W,
                                                       const BlockDesc* __restrict__ d,
                                                       const double* __restrict__ u,
                                                       double* __restrict__ part, int nch) {
+  typedef GemvVec<TW> G;
+  typedef typename Vec16Of<TW>::type V;
   __shared__ double red[4 * kTile];
   const BlockDesc b = d[0];
   const int ch = blockIdx.x, j = blockIdx.y;
   const int i0 = j + ch * kGemvChunk;
   if (j >= b.nt || i0 >= b.nt) return;
   const int i1 = min(b.nt, i0 + kGemvChunk);
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  double acc = 0.0;
-  for (int i = i0; i < i1; i++) {
-    const TW* Ti = W + b.base + toff(b, i, j) + (16 * w) * kTile + lane;
-    const double* ui = u + i * kTile + 16 * w;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, cg = lane % G::LPR, rs = lane / G::LPR;
+  double acc[G::VEC];
 #pragma unroll
-    for (int q = 0; q < 16; q++) acc += (double)Ti[q * kTile] * ui[q];
+  for (int e = 0; e < G::VEC; e++) acc[e] = 0.0;
+  for (int i = i0; i < i1; i++) {
+    const TW* Tb = W + b.base + toff(b, i, j) + (16 * w + rs) * kTile + cg * G::VEC;
+    V wv[G::NI];
+#pragma unroll
+    for (int q = 0; q < G::NI; q++) wv[q] = *reinterpret_cast<const V*>(Tb + q * G::RPI * kTile);
+    const double* ui = u + i * kTile + 16 * w + rs;
+#pragma unroll
+    for (int q = 0; q < G::NI; q++) {
+      const double uq = ui[q * G::RPI];
+#pragma unroll
+      for (int e = 0; e < G::VEC; e++) acc[e] = acc[e] + vget(wv[q], e) * uq;
+    }
   }
-  red[w * kTile + lane] = acc;
+#pragma unroll
+  for (int e = 0; e < G::VEC; e++)
+#pragma unroll
+    for (int o = G::LPR; o < 64; o <<= 1) acc[e] += __shfl_xor(acc[e], o);
+  if (rs == 0)
+#pragma unroll
+    for (int e = 0; e < G::VEC; e++) red[w * kTile + cg * G::VEC + e] = acc[e];
   __syncthreads();
   if (w == 0)
     part[((int64_t)j * nch + ch) * kTile + lane] =
@@ -942,7 +1015,19 @@ __global__ __launch_bounds__(256) void k_spmv_wave_rows(int nrows, const int32_t
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= nrows) return;
   double acc = 0.0;
-  for (int e = ptr[row] + lane; e < ptr[row + 1]; e += 64) acc = acc + val[e] * x[col[e]];
+  int e = ptr[row] + lane;
+  const int end = ptr[row + 1];
+  // four strides loaded ahead, added in the same order as one at a time
+  for (; e + 192 < end; e += 256) {
+    const int c0 = col[e], c1 = col[e + 64], c2 = col[e + 128], c3 = col[e + 192];
+    const double v0 = val[e], v1 = val[e + 64], v2 = val[e + 128], v3 = val[e + 192];
+    const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    acc = acc + v0 * x0;
+    acc = acc + v1 * x1;
+    acc = acc + v2 * x2;
+    acc = acc + v3 * x3;
+  }
+  for (; e < end; e += 64) acc = acc + val[e] * x[col[e]];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
   if (lane == 0) y[row] = acc;

@@ -1492,6 +1492,18 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
   A.col.clear();
   A.val.clear();
   A.src.clear();
+  {
+    // room for the rows kept here and (with `ghost`) the ghost rows that
+    // the caller appends, so that neither grows the arrays again
+    size_t cap = 0;
+    for (int k = 0; k < nl; k++) {
+      const int v = out.lg[k];
+      if (S.nd[v].layer < L || (ghost && S.nd[v].layer == L)) cap += (size_t)(As.ptr[v + 1] - As.ptr[v]);
+    }
+    A.col.reserve(cap);
+    A.val.reserve(cap);
+    A.src.reserve(cap);
+  }
   for (int k = 0; k < nl; k++) {
     const int v = out.lg[k];
     if (S.nd[v].layer < L) {
@@ -1588,14 +1600,12 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
       for (int r = next_rank++; r < nranks; r = next_rank++) {
         RankLocal& R = out.ranks[r];
         rank_local(As, val, eptn, r, own[r], ol_dist, S, R, keep_local, pos ? pos->data() : nullptr);
-        // getLocal (Aggregate_utils.f95:33-51): the own rows; a new matrix
+        // getLocal (Aggregate_utils.f95:33-51): the own rows, which lead the
+        // local matrix (these steps read rows < nrows alone); a new matrix
         // carries no symmetry flags, so the strength test scales by rows
-        Csr LA;
+        Csr& LA = R.A;
+        const int local_rows = LA.nrows;
         LA.nrows = R.ninner;
-        LA.ncols = R.A.ncols;
-        LA.ptr.assign(R.A.ptr.begin(), R.A.ptr.begin() + R.ninner + 1);
-        LA.col.assign(R.A.col.begin(), R.A.col.begin() + R.A.ptr[R.ninner]);
-        LA.val.assign(R.A.val.begin(), R.A.val.begin() + R.A.ptr[R.ninner]);
         std::vector<double> scaled, diag;
         scale_values(LA, false, scaled, diag);
         std::vector<uint8_t> strong = find_strong(LA, scaled, diag, strong1);
@@ -1624,6 +1634,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
           aggregate_graph(LA, inner[r], inner[r], xadj, adjncy);
           partition_graph(inner[r].nagr, xadj.data(), adjncy.data(), nullptr, nparts, part);
         }
+        LA.nrows = local_rows;
         std::vector<int> pst((size_t)nparts + 1, 0), pnd;
         for (int i = 0; i < R.ninner; i++) {
           const int a = inner[r].num[i];
