@@ -55,7 +55,7 @@ class DougTuning(C.Structure):
     _fields_ = [
         ("spmv_shape", _i32), ("spmv_rows", _i32), ("spmv_xcd_order", _i32), ("trsv_order", _i32),
         ("spmv_ms", _dbl * 6), ("trsv_ms", _dbl * 2),
-        ("trsm_tiles_per_step", _i32), ("pad", _i32), ("trsm_ms", _dbl * 2),
+        ("trsm_tiles_per_step", _i32), ("trsm_col_groups", _i32), ("trsm_ms", _dbl * 2),
     ]
 
     def as_dict(self):
@@ -63,7 +63,6 @@ class DougTuning(C.Structure):
         d["spmv_ms"] = list(self.spmv_ms)
         d["trsv_ms"] = list(self.trsv_ms)
         d["trsm_ms"] = list(self.trsm_ms)
-        del d["pad"]
         return d
 
 

@@ -15,6 +15,8 @@ chain rule to theta is ordinary autograd.
 """
 from __future__ import annotations
 
+import time
+
 import numpy as np
 import torch
 
@@ -41,14 +43,18 @@ class PatternSolver:
         self.tol = tol
         self.maxit = maxit
         self.iterations = []
+        self.solve_times = []  # seconds per PCG (forward or adjoint)
+        self.setup_times = []  # seconds per set_values (host copy + doug_setup)
         self.version = 0
 
     def set_values(self, values: torch.Tensor) -> int:
+        t0 = time.perf_counter()
         v = values.detach().to("cpu", torch.float64).numpy()
         self.solver.set_matrix(self.n, self.indi, self.indj, v, 1)
         if self.partition is not None:
             self.solver.set_partition(*self.partition)
         self.solver.setup(self.precision)
+        self.setup_times.append(time.perf_counter() - t0)
         self.version += 1
         return self.version
 
@@ -56,7 +62,9 @@ class PatternSolver:
         b = b.detach().contiguous()
         x = torch.zeros_like(b)
         torch.cuda.synchronize()
+        t0 = time.perf_counter()
         it, _, _ = self.solver.pcg_device(b.data_ptr(), x.data_ptr(), self.tol, self.maxit)
+        self.solve_times.append(time.perf_counter() - t0)
         self.iterations.append(it)
         return x
 

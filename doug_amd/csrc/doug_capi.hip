@@ -116,6 +116,7 @@ struct doug_ctx {
   std::vector<BlockDesc> h_sub;
   int trsv_order = 0;               // 1: largest subdomain dispatched first
   int trsm_kb = 2;                  // tiles per step of the batched envelope solve
+  int trsm_cg = 1;                  // column groups (workgroups per block) of that solve
   int local_order = 1;              // 1: reverse Cuthill-McKee inside each subdomain
   BlockDesc* d_sub_trsv = nullptr;  // descriptors in dispatch order
   doug_tuning tuning{};
@@ -2669,10 +2670,10 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
   if (p.fine_method == 1) {
     if (ctx->precision == DOUG_PREC_FP32_FACTORS)
       launch_trsm_multi(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, nrp,
-                        (float*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, st);
+                        (float*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, ctx->trsm_cg, st);
     else
       launch_trsm_multi(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->d_inds, r, nrp,
-                        (double*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, st);
+                        (double*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, ctx->trsm_cg, st);
     fine_src = ctx->d_subsolm;
   }
   launch_overlap_sum_multi(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
@@ -2798,14 +2799,18 @@ int doug_pcg_batch(doug_ctx* ctx, int32_t nrhs, const double* B, double* X, doub
   if (!ctx->batch_ready) return fail(ctx, DOUG_E_STATE, "doug_pcg_batch before doug_prepare_batch");
   const int n = ctx->n, nrp = next_pow2(nrhs);
   hipStream_t st = ctx->stream;
-  const size_t w = nrhs * sizeof(double), wp = nrp * sizeof(double);
-  HIPCHK(hipMemsetAsync(ctx->d_Bm, 0, (size_t)n * wp, st));
-  HIPCHK(hipMemsetAsync(ctx->d_Xm, 0, (size_t)n * wp, st));
-  HIPCHK(hipMemcpy2DAsync(ctx->d_Bm, wp, B, w, w, n, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpy2DAsync(ctx->d_Xm, wp, X, w, w, n, hipMemcpyHostToDevice, st));
+  // host rows of nrhs columns travel as one block through the workspace
+  // d_Tm and are spread to (or gathered from) rows of nrp columns on the
+  // device: a strided copy from pageable memory would go row by row
+  const size_t bytes = (size_t)n * nrhs * sizeof(double);
+  HIPCHK(hipMemcpyAsync(ctx->d_Tm, B, bytes, hipMemcpyHostToDevice, st));
+  launch_repack(ctx->d_Bm, nrp, ctx->d_Tm, nrhs, n, nrhs, st);
+  HIPCHK(hipMemcpyAsync(ctx->d_Tm, X, bytes, hipMemcpyHostToDevice, st));
+  launch_repack(ctx->d_Xm, nrp, ctx->d_Tm, nrhs, n, nrhs, st);
   int rc = pcg_batch_core(ctx, nrhs, tol, maxit, its, res_hist);
   if (rc) return rc;
-  HIPCHK(hipMemcpy2DAsync(X, w, ctx->d_Xm, wp, w, n, hipMemcpyDeviceToHost, st));
+  launch_repack(ctx->d_Tm, nrhs, ctx->d_Xm, nrp, n, nrhs, st);
+  HIPCHK(hipMemcpyAsync(X, ctx->d_Tm, bytes, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return DOUG_OK;
 }
@@ -2834,6 +2839,11 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
       if (value != 1 && value != 2) return fail(ctx, DOUG_E_ARG, "tiles per step must be 1 or 2");
       drop_iteration_graph(ctx);
       ctx->trsm_kb = (int)value;
+      return DOUG_OK;
+    case DOUG_OPT_BATCH_COLUMN_GROUPS:
+      if (value != 1 && value != 2) return fail(ctx, DOUG_E_ARG, "column groups must be 1 or 2");
+      drop_iteration_graph(ctx);
+      ctx->trsm_cg = (int)value;
       return DOUG_OK;
     case DOUG_OPT_DEVICE_BLOCKS:
       ctx->use_device_blocks = value != 0;
@@ -2984,18 +2994,29 @@ int doug_autotune(doug_ctx* ctx, int32_t reps, doug_tuning* out) {
     if ((rc = set_trsv_order(ctx, order))) return rc;
   }
   // batched envelope solve: off-diagonal tiles staged per step
-  int kb = ctx->trsm_kb;
+  int kb = ctx->trsm_kb, cg = ctx->trsm_cg;
   if (ctx->batch_ready && ctx->p.fine_method == 1 && ctx->nsub > 0) {
     const int rt = std::max(2, reps / 8);
-    for (int q = 1; q <= 2; q++) {
-      ctx->trsm_kb = q;
-      if ((rc = time_launches(ctx, rt, [&]() { enqueue_prec_multi(ctx, kMaxRhs, ctx->d_Rm, ctx->d_Zm); },
-                              &tu.trsm_ms[q - 1])))
-        return rc;
-    }
-    kb = tu.trsm_ms[0] < tu.trsm_ms[1] ? 1 : 2;
+    double best_ms = 0.0, ms[2][2];
+    for (int g = 1; g <= 2; g++)
+      for (int q = 1; q <= 2; q++) {
+        ctx->trsm_kb = q;
+        ctx->trsm_cg = g;
+        if ((rc = time_launches(ctx, rt, [&]() { enqueue_prec_multi(ctx, kMaxRhs, ctx->d_Rm, ctx->d_Zm); },
+                                &ms[g - 1][q - 1])))
+          return rc;
+        if (best_ms == 0.0 || ms[g - 1][q - 1] < best_ms) {
+          best_ms = ms[g - 1][q - 1];
+          kb = q;
+          cg = g;
+        }
+      }
+    tu.trsm_ms[0] = ms[cg - 1][0];
+    tu.trsm_ms[1] = ms[cg - 1][1];
   }
   ctx->trsm_kb = kb;
+  ctx->trsm_cg = cg;
+  tu.trsm_col_groups = cg;
   HIPCHK(hipStreamSynchronize(ctx->stream));
   tu.trsm_tiles_per_step = kb;
   tu.spmv_shape = best;
@@ -3016,6 +3037,7 @@ int doug_get_tuning(const doug_ctx* ctx, doug_tuning* out) {
   out->spmv_xcd_order = ctx->spmv_shape & 1;
   out->trsv_order = ctx->trsv_order;
   out->trsm_tiles_per_step = ctx->trsm_kb;
+  out->trsm_col_groups = ctx->trsm_cg;
   return DOUG_OK;
 }
 

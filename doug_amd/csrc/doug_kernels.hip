@@ -1033,6 +1033,17 @@ __global__ __launch_bounds__(256) void k_spmv_wave_rows(int nrows, const int32_t
   if (lane == 0) y[row] = acc;
 }
 
+// dst(i, c) = src(i, c) for c < ncols and 0 for ncols <= c < ldd: a matrix
+// of n rows between leading dimensions lds and ldd
+__global__ void k_repack(double* __restrict__ dst, int ldd, const double* __restrict__ src, int lds,
+                         int64_t n, int ncols) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * ldd) return;
+  const int64_t i = e / ldd;
+  const int c = (int)(e - i * ldd);
+  dst[e] = c < ncols ? src[i * lds + c] : 0.0;
+}
+
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -1225,9 +1236,9 @@ __global__ void k_panel_sum(const double* __restrict__ part, int nt, int nch, in
 // stages KB off-diagonal tiles with their panels, and the first step of a
 // tile row also the diagonal tile, with all loads issued before any LDS
 // store.
-template <typename T, int NR, int KB>
+template <typename T, int NR, int KB, int CG>
 __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
-                                                    const BlockDesc* __restrict__ d,
+                                                    const BlockDesc* __restrict__ d, int nblocks,
                                                     const int32_t* __restrict__ inds,
                                                     const double* __restrict__ rhs, int ldr,
                                                     T* __restrict__ U, double* __restrict__ out,
@@ -1236,10 +1247,24 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
   __shared__ T Ds[kTile * LD];
   __shared__ T Bs[KB][kTile * (NR + 1)];
   typedef typename Mma<T>::v4 V;
-  const BlockDesc b = d[blockIdx.x];
+  // CG column groups of NR columns each per subdomain; with two, the groups
+  // of one subdomain are workgroups bid and bid + 8, so that they run on the
+  // same XCD (workgroups go to the XCDs round-robin) and share its L2 copy of
+  // the factor tiles
+  int sub = blockIdx.x, grp = 0;
+  if (CG == 2) {
+    sub = (int)(blockIdx.x / 16) * 8 + (int)(blockIdx.x % 8);
+    grp = (int)(blockIdx.x / 8) & 1;
+    if (sub >= nblocks) return;
+  }
+  const BlockDesc b = d[sub];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
   const T* Lb = L + b.base;
-  T* Ub = U + b.woff * NR;
+  T* Ub = U + b.woff * (NR * CG) + (int64_t)grp * b.nt * kTile * NR;
+  const int c0 = grp * NR;  // first right-hand side of this group
+  rhs += c0;
+  out += c0;
+  nrhs -= c0;
   V acc[NR / 16];
   // forward: Y_i = inv(L_ii) (R_i - sum_{k=fc_i}^{i-1} L_ik Y_k)
   for (int i = 0; i < b.nt; i++) {
@@ -2083,6 +2108,12 @@ void launch_copy(double* dst, const double* src, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy, dim3(cdiv(n, 256)), dim3(256), 0, st, dst, src, n);
 }
 
+void launch_repack(double* dst, int ldd, const double* src, int lds, int64_t n, int ncols,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_repack, dim3(cdiv(n * ldd, 256)), dim3(256), 0, st, dst, ldd, src, lds, n, ncols);
+}
+
 void launch_zero(double* dst, int64_t n, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n);
@@ -2095,17 +2126,26 @@ void launch_residual(int n, const double* b, const double* ax, double* r, hipStr
 template <typename T>
 void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
                        const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
-                       int kb, hipStream_t st) {
+                       int kb, int cg, hipStream_t st) {
   if (nblocks <= 0) return;
-  const dim3 g(nblocks), b(256);
+  const dim3 b(256);
+  if (NR == 32 && cg == 2) {  // two groups of 16 columns
+    const dim3 g(cdiv(nblocks, 8) * 16);
+    if (kb == 1)
+      hipLaunchKernelGGL((k_trsm_multi<T, 16, 1, 2>), g, b, 0, st, L, d, nblocks, inds, rhs, ldr, U, out, ldo, nrhs);
+    else
+      hipLaunchKernelGGL((k_trsm_multi<T, 16, 2, 2>), g, b, 0, st, L, d, nblocks, inds, rhs, ldr, U, out, ldo, nrhs);
+    return;
+  }
+  const dim3 g(nblocks);
   if (NR == 16 && kb == 1)
-    hipLaunchKernelGGL((k_trsm_multi<T, 16, 1>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 16, 1, 1>), g, b, 0, st, L, d, nblocks, inds, rhs, ldr, U, out, ldo, nrhs);
   else if (NR == 16)
-    hipLaunchKernelGGL((k_trsm_multi<T, 16, 2>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 16, 2, 1>), g, b, 0, st, L, d, nblocks, inds, rhs, ldr, U, out, ldo, nrhs);
   else if (kb == 1)
-    hipLaunchKernelGGL((k_trsm_multi<T, 32, 1>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 32, 1, 1>), g, b, 0, st, L, d, nblocks, inds, rhs, ldr, U, out, ldo, nrhs);
   else
-    hipLaunchKernelGGL((k_trsm_multi<T, 32, 2>), g, b, 0, st, L, d, inds, rhs, ldr, U, out, ldo, nrhs);
+    hipLaunchKernelGGL((k_trsm_multi<T, 32, 2, 1>), g, b, 0, st, L, d, nblocks, inds, rhs, ldr, U, out, ldo, nrhs);
 }
 
 void launch_wgemm_fwd(const double* W, const BlockDesc* d, int nt, const double* G, double* U,
@@ -2136,7 +2176,7 @@ void launch_wgemm_bwd(const double* W, const BlockDesc* d, int nt, int rows, con
 
 #define DOUG_INSTANTIATE_MULTI(T)                                                              \
   template void launch_trsm_multi<T>(const T*, const BlockDesc*, int, const int32_t*,          \
-                                     const double*, int, T*, double*, int, int, int, int,       \
+                                     const double*, int, T*, double*, int, int, int, int, int,  \
                                      hipStream_t);
 DOUG_INSTANTIATE_MULTI(double)
 DOUG_INSTANTIATE_MULTI(float)

@@ -193,6 +193,10 @@ void launch_p_update(int n, const double* z, double* p, const double* scal, int 
 void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
                       double* partial, double* scal, hipStream_t st);
 void launch_copy(double* dst, const double* src, int n, hipStream_t st);
+// dst(i, c) = src(i, c) for c < ncols, 0 for ncols <= c < ldd (n rows;
+// leading dimensions ldd and lds)
+void launch_repack(double* dst, int ldd, const double* src, int lds, int64_t n, int ncols,
+                   hipStream_t st);
 // dst[0..n) = 0 (a kernel, so that captured graphs hold no memset node)
 void launch_zero(double* dst, int64_t n, hipStream_t st);
 void launch_residual(int n, const double* b, const double* ax, double* r, hipStream_t st);
@@ -206,11 +210,13 @@ void launch_residual(int n, const double* b, const double* ax, double* r, hipStr
 // products on the matrix cores: U(block rows) <- L^{-T} L^{-1} rhs(inds, :),
 // then out[(vec_off + p) * ldo + r] = U for p < s, r < nrhs.  rhs has
 // leading dimension ldr; U is T workspace of nt*64 rows of NR from woff.
-// kb: off-diagonal tiles staged per step (1 or 2).
+// kb: off-diagonal tiles staged per step (1 or 2).  cg = 2 with NR = 32:
+// two workgroups per block, 16 columns each (U then holds the two groups'
+// nt*64 x 16 panels one after the other in the block's 32-column space).
 template <typename T>
 void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_t* inds,
                        const double* rhs, int ldr, T* U, double* out, int ldo, int nrhs, int NR,
-                       int kb, hipStream_t st);
+                       int kb, int cg, hipStream_t st);
 // Dense coarse inverse (block d[0], nt tiles) times NR columns, chunked with
 // partial panels in `part` (nt * gemv_chunks(nt) * 64 * NR doubles):
 // U_i = sum_{k<=i} W_ik G_k (all nt*64 rows), and

@@ -28,7 +28,7 @@ module doug_capi
   type, bind(C) :: doug_tuning
     integer(c_int32_t) :: spmv_shape, spmv_rows, spmv_xcd_order, trsv_order
     real(c_double) :: spmv_ms(6), trsv_ms(2)
-    integer(c_int32_t) :: trsm_tiles_per_step, pad
+    integer(c_int32_t) :: trsm_tiles_per_step, trsm_col_groups
     real(c_double) :: trsm_ms(2)
   end type doug_tuning
 

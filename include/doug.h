@@ -278,9 +278,10 @@ typedef struct doug_tuning {
   double spmv_ms[6];      /* measured average per shape */
   double trsv_ms[2];      /* measured per order; 0 where not timed */
   int32_t trsm_tiles_per_step; /* batched envelope solve: 1 or 2 tiles staged per step */
-  int32_t pad;
-  double trsm_ms[2];      /* 32-column preconditioner apply per choice; 0 where not timed
-                             (timed only after doug_prepare_batch) */
+  int32_t trsm_col_groups;     /* its workgroups per subdomain for 32 columns: 1 or 2 */
+  double trsm_ms[2];      /* 32-column preconditioner apply per tiles-per-step choice, at
+                             the chosen column groups; 0 where not timed (timed only
+                             after doug_prepare_batch) */
 } doug_tuning;
 int doug_autotune(doug_ctx *ctx, int32_t reps, doug_tuning *out);
 int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
@@ -305,7 +306,10 @@ int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
  * DOUG_OPT_KEEP_SETUP_TEMPS (default 1): keep the coarse build's device
  * temporaries between set-ups (doug_info.temp_bytes), so that a repeated
  * set-up of the same size neither allocates nor frees them; 0 frees them
- * now and at the end of every set-up. */
+ * now and at the end of every set-up.
+ * DOUG_OPT_BATCH_COLUMN_GROUPS (1 or 2, default 1): workgroups per subdomain
+ * of the 32-column batched envelope solve, 16 columns each with 2 (what
+ * doug_autotune sets); the bits are the same either way. */
 enum {
   DOUG_OPT_ITERATION_GRAPH = 1,
   DOUG_OPT_SPMV_SHAPE = 2,
@@ -314,7 +318,8 @@ enum {
   DOUG_OPT_DEVICE_COARSE = 5,
   DOUG_OPT_BATCH_TILES = 6,
   DOUG_OPT_DEVICE_BLOCKS = 7,
-  DOUG_OPT_KEEP_SETUP_TEMPS = 8
+  DOUG_OPT_KEEP_SETUP_TEMPS = 8,
+  DOUG_OPT_BATCH_COLUMN_GROUPS = 9
 };
 int doug_set_option(doug_ctx *ctx, int32_t option, int64_t value);
 
