@@ -4,6 +4,10 @@ module doug_capi
   implicit none
 
   integer(c_int), parameter :: DOUG_OK = 0, DOUG_PREC_FP64 = 0, DOUG_PREC_FP32_FACTORS = 1
+  integer(c_int32_t), parameter :: DOUG_OPT_ITERATION_GRAPH = 1, DOUG_OPT_SPMV_SHAPE = 2, &
+    DOUG_OPT_TRSV_ORDER = 3, DOUG_OPT_LOCAL_ORDER = 4, DOUG_OPT_DEVICE_COARSE = 5, &
+    DOUG_OPT_BATCH_TILES = 6, DOUG_OPT_DEVICE_BLOCKS = 7, DOUG_OPT_KEEP_SETUP_TEMPS = 8, &
+    DOUG_OPT_BATCH_COLUMN_GROUPS = 9
 
   !> doug_params (field order and meaning as in include/doug.h)
   type, bind(C) :: doug_params
@@ -130,6 +134,232 @@ module doug_capi
       integer(c_int32_t), value :: maxit
       integer(c_int32_t) :: it
       real(c_double) :: cond
+      integer(c_int) :: rc
+    end function
+
+    function doug_set_aggregates(ctx, nagr, num) bind(C, name="doug_set_aggregates") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nagr
+      integer(c_int32_t) :: num(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_set_strong_mask(ctx, nnz, strong) bind(C, name="doug_set_strong_mask") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: nnz
+      type(c_ptr), value :: strong        ! uint8_t flags, c_null_ptr clears them
+      integer(c_int) :: rc
+    end function
+
+    function doug_set_restriction_coo(ctx, nagr, nnz, ri, rj, rv, base) &
+        bind(C, name="doug_set_restriction_coo") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nagr, base
+      integer(c_int64_t), value :: nnz
+      integer(c_int32_t) :: ri(*), rj(*)
+      real(c_double) :: rv(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_set_subdomains(ctx, nsub, starts, inds, base) &
+        bind(C, name="doug_set_subdomains") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nsub, base
+      integer(c_int32_t) :: starts(*), inds(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_rank_map(ctx, rank, ninner, nlocal, lg) bind(C, name="doug_get_rank_map") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: rank
+      integer(c_int32_t) :: ninner, nlocal
+      type(c_ptr), value :: lg            ! int32_t(nlocal) or c_null_ptr to ask the size
+      integer(c_int) :: rc
+    end function
+
+    function doug_partition_graph(n, xadj, adjncy, vwgt, nparts, base, part) &
+        bind(C, name="doug_partition_graph") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      integer(c_int32_t), value :: n, nparts, base
+      integer(c_int32_t) :: xadj(*), adjncy(*), part(*)
+      type(c_ptr), value :: vwgt          ! int32_t(n) or c_null_ptr
+      integer(c_int) :: rc
+    end function
+
+    function doug_setup_host(ctx) bind(C, name="doug_setup_host") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_fine_aggregates(ctx, num, starts, nodes) &
+        bind(C, name="doug_get_fine_aggregates") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: num(*), starts(*), nodes(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_coarse_aggregates(ctx, num, starts, nodes) &
+        bind(C, name="doug_get_coarse_aggregates") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: num(*), starts(*), nodes(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_subdomains(ctx, starts, inds) bind(C, name="doug_get_subdomains") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: starts(*), inds(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_restriction(ctx, rowptr, cols, vals) bind(C, name="doug_get_restriction") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: rowptr(*), cols(*)
+      real(c_double) :: vals(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_coarse_matrix(ctx, rowptr, cols, vals) &
+        bind(C, name="doug_get_coarse_matrix") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t) :: rowptr(*), cols(*)
+      real(c_double) :: vals(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_restriction_nnz(ctx) bind(C, name="doug_restriction_nnz") result(nnz)
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t) :: nnz
+    end function
+
+    function doug_coarse_matrix_nnz(ctx) bind(C, name="doug_coarse_matrix_nnz") result(nnz)
+      import :: c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t) :: nnz
+    end function
+
+    function doug_spmv(ctx, x, y) bind(C, name="doug_spmv") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(*), y(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_spmv_device(ctx, dx, dy) bind(C, name="doug_spmv_device") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, dx, dy
+      integer(c_int) :: rc
+    end function
+
+    function doug_prec_apply(ctx, r, z) bind(C, name="doug_prec_apply") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: r(*), z(*)
+      integer(c_int) :: rc
+    end function
+
+    function doug_prec_apply_device(ctx, dr, dz) bind(C, name="doug_prec_apply_device") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, dr, dz
+      integer(c_int) :: rc
+    end function
+
+    function doug_prec_apply_device_noalloc(ctx, dr, dz) &
+        bind(C, name="doug_prec_apply_device_noalloc") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, dr, dz
+      integer(c_int) :: rc
+    end function
+
+    function doug_pcg_device(ctx, db, dx, tol, maxit, it, cond, res_hist) &
+        bind(C, name="doug_pcg_device") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, db, dx
+      real(c_double), value :: tol
+      integer(c_int32_t), value :: maxit
+      integer(c_int32_t) :: it
+      real(c_double) :: cond
+      type(c_ptr), value :: res_hist      ! host double(maxit) or c_null_ptr
+      integer(c_int) :: rc
+    end function
+
+    function doug_time_operator(ctx, which, reps, ms) bind(C, name="doug_time_operator") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: which, reps
+      real(c_double) :: ms
+      integer(c_int) :: rc
+    end function
+
+    function doug_get_tuning(ctx, tu) bind(C, name="doug_get_tuning") result(rc)
+      import :: c_int, c_ptr, doug_tuning
+      type(c_ptr), value :: ctx
+      type(doug_tuning) :: tu
+      integer(c_int) :: rc
+    end function
+
+    function doug_set_option(ctx, option, val) bind(C, name="doug_set_option") result(rc)
+      import :: c_int, c_int32_t, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: option
+      integer(c_int64_t), value :: val
+      integer(c_int) :: rc
+    end function
+
+    function doug_value_grad_device(ctx, dlam, dx, dgrad) &
+        bind(C, name="doug_value_grad_device") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, dlam, dx, dgrad
+      integer(c_int) :: rc
+    end function
+
+    function doug_prepare_batch(ctx) bind(C, name="doug_prepare_batch") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+
+    function doug_prec_apply_batch_device(ctx, nrhs, dr, dz) &
+        bind(C, name="doug_prec_apply_batch_device") result(rc)
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx, dr, dz
+      integer(c_int32_t), value :: nrhs
+      integer(c_int) :: rc
+    end function
+
+    !> B and X node-major: entry (i, r) of the n x nrhs multi-vector at
+    !> position i*nrhs + r (0-based), i.e. a Fortran array(nrhs, n)
+    function doug_pcg_batch(ctx, nrhs, b, x, tol, maxit, its, res_hist) &
+        bind(C, name="doug_pcg_batch") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nrhs, maxit
+      real(c_double) :: b(*), x(*)
+      real(c_double), value :: tol
+      integer(c_int32_t) :: its(*)
+      type(c_ptr), value :: res_hist      ! host double(maxit*nrhs) or c_null_ptr
+      integer(c_int) :: rc
+    end function
+
+    function doug_pcg_batch_device(ctx, nrhs, db, dx, tol, maxit, its, res_hist) &
+        bind(C, name="doug_pcg_batch_device") result(rc)
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx, db, dx
+      integer(c_int32_t), value :: nrhs, maxit
+      real(c_double), value :: tol
+      integer(c_int32_t) :: its(*)
+      type(c_ptr), value :: res_hist
       integer(c_int) :: rc
     end function
   end interface

@@ -27,3 +27,11 @@ def test_library_exports_every_declared_symbol():
 
 def test_binding_covers_header():
     assert set(_declared()) == set(_lib.SIGNATURES), set(_declared()) ^ set(_lib.SIGNATURES)
+
+
+def test_fortran_module_binds_every_entry_point():
+    # doug_capi_mod.f90 is the ISO_C_BINDING module a Fortran caller (DOUG's
+    # driver) uses: one bind(C) interface per header entry point
+    text = open(os.path.join(ROOT, "doug_amd", "fortran", "doug_capi_mod.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(doug_[a-z_0-9]+)"\)', text))
+    assert bound == set(_declared()), bound ^ set(_declared())
