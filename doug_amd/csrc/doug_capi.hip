@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -32,11 +33,31 @@ double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A host thread that enqueues one part of a set-up on another of the
+// context's streams names it here; every `ctx->stream` read on that thread
+// then yields it (the set-up's subdomain work runs on a second host thread
+// beside the coarse space's construction).
+thread_local hipStream_t t_stream_override = nullptr;
+
+struct StreamRef {
+  hipStream_t s = nullptr;
+  operator hipStream_t() const { return t_stream_override ? t_stream_override : s; }
+};
+
+struct StreamOverride {
+  explicit StreamOverride(hipStream_t st) { t_stream_override = st; }
+  ~StreamOverride() { t_stream_override = nullptr; }
+  StreamOverride(const StreamOverride&) = delete;
+  StreamOverride& operator=(const StreamOverride&) = delete;
+};
+
 }  // namespace
 
 struct doug_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  StreamRef stream;  // the main stream (or this thread's override)
+  // guards allocs, err and the reaper while two host threads set up
+  std::mutex mu;
   hipStream_t stream2 = nullptr;  // subdomain factorisation beside the coarse one
   hipStream_t stream3 = nullptr;  // coarse inverse one step behind the coarse factorisation
   std::vector<hipEvent_t> cevents;  // coarse factorisation step ends, then the inverse's end
@@ -83,10 +104,10 @@ struct doug_ctx {
                                    // values stay on the device until a getter asks
   uint8_t* d_strong = nullptr;     // strength mask, uploaded while the aggregation runs
   double* d_valin = nullptr;       // values as read (several processes, overlap 1)
+  int32_t* d_eptn = nullptr;       // owner of each unknown (with d_valin)
   // several processes: place of each node in process order, the inverse,
   // and whether its owner sends it (staging of the device coarse build)
-  std::vector<int32_t> mp_proc, mp_order;
-  std::vector<uint8_t> mp_sent;
+  std::vector<int32_t> mp_nsent;  // nodes each rank sends (its first own nodes)
   // whether the host's A.val and strong hold the round-trip values and the
   // strength flags (a several-process set-up forms them on the device and
   // copies them back only when host code needs them)
@@ -172,12 +193,14 @@ struct doug_ctx {
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
     if (e_ != hipSuccess) {                                                       \
+      std::lock_guard<std::mutex> g_(ctx->mu);                                    \
       ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
       return DOUG_E_HIP;                                                          \
     }                                                                             \
   } while (0)
 
 static int fail(doug_ctx* ctx, int code, const std::string& msg) {
+  std::lock_guard<std::mutex> g(ctx->mu);
   ctx->err = msg;
   return code;
 }
@@ -189,10 +212,13 @@ static void join_reaper(doug_ctx* ctx);
 // once more.
 static hipError_t device_malloc(doug_ctx* ctx, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess && ctx->reaper.joinable()) {
-    (void)hipGetLastError();
-    join_reaper(ctx);
-    e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->reaper.joinable()) {
+      (void)hipGetLastError();
+      join_reaper(ctx);
+      e = hipMalloc(p, bytes);
+    }
   }
   return e;
 }
@@ -202,6 +228,7 @@ static int dalloc(doug_ctx* ctx, T** out, size_t count) {
   void* p = nullptr;
   size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
   hipError_t e = device_malloc(ctx, &p, bytes);
+  std::lock_guard<std::mutex> g(ctx->mu);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     ctx->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e);
@@ -256,6 +283,7 @@ static int64_t scratch_slot_bytes(const doug_ctx* ctx) {
 static void free_device(doug_ctx* ctx, bool background = false) {
   drop_iteration_graph(ctx);
   ctx->d_valin = nullptr;
+  ctx->d_eptn = nullptr;
   ctx->d_cperm = nullptr;
   ctx->d_cW32 = nullptr;
   join_reaper(ctx);
@@ -347,13 +375,13 @@ int doug_ctx_create(int device, doug_ctx** out) {
   if (device < 0 || device >= ndev) return DOUG_E_ARG;
   doug_ctx* ctx = new doug_ctx();
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream.s, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scalm, kMaxRhs * 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
-    for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
+    for (hipStream_t st : {ctx->stream.s, ctx->stream2, ctx->stream3})
       if (st) (void)hipStreamDestroy(st);
     delete ctx;
     return DOUG_E_HIP;
@@ -369,14 +397,14 @@ int doug_ctx_destroy(doug_ctx* ctx) {
     return DOUG_OK;
   }
   (void)hipSetDevice(ctx->device);
-  for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
+  for (hipStream_t st : {ctx->stream.s, ctx->stream2, ctx->stream3})
     if (st) (void)hipStreamSynchronize(st);
   free_device(ctx);
   free_scratch_slots(ctx);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
   if (ctx->h_scalm) (void)hipHostFree(ctx->h_scalm);
   for (hipEvent_t e : ctx->cevents) (void)hipEventDestroy(e);
-  for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
+  for (hipStream_t st : {ctx->stream.s, ctx->stream2, ctx->stream3})
     if (st) (void)hipStreamDestroy(st);
   delete ctx;
   return DOUG_OK;
@@ -1180,34 +1208,35 @@ static int coarse_build_device(doug_ctx* ctx, const std::vector<uint8_t>& strong
   uint8_t* d_sent = nullptr;
   double *a_val = nullptr, *d_dfilt = nullptr, *d_dghost = nullptr;
   if (ranks) {
+    // the ranks' workers listed their own nodes in local order, rank after
+    // rank (the process-major order); the position of each node in it and
+    // whether its owner sends it are formed on the device
     const RankSetup& rs = ctx->rsetup;
-    std::vector<int32_t>& proc = ctx->mp_proc;
-    std::vector<int32_t>& order = ctx->mp_order;
-    std::vector<uint8_t>& sent = ctx->mp_sent;
-    proc.assign((size_t)n, 0);
-    order.assign((size_t)n, 0);
-    sent.assign((size_t)n, 0);
-    int off = 0;
-    for (const RankLocal& L : rs.ranks) {
-      for (int k = 0; k < L.ninner; k++) {
-        proc[L.lg[k]] = off + k;
-        order[off + k] = L.lg[k];
-        sent[L.lg[k]] = k < L.nsent;
-      }
-      off += L.ninner;
-    }
-    HIPCHK(tmp.alloc(&d_owner, (size_t)n));
+    const int nr = (int)rs.ranks.size();
+    if ((int)rs.order.size() != n || (int)rs.own_off.size() != nr + 1 || rs.own_off[nr] != n)
+      return fail(ctx, DOUG_E_STATE, "the ranks' node order is missing");
+    ctx->mp_nsent.resize((size_t)nr);
+    for (int r = 0; r < nr; r++) ctx->mp_nsent[r] = rs.ranks[r].nsent;
+    int32_t *d_off, *d_nsent;
     HIPCHK(tmp.alloc(&d_proc, (size_t)n));
     HIPCHK(tmp.alloc(&d_order, (size_t)n));
     HIPCHK(tmp.alloc(&d_sent, (size_t)n));
+    HIPCHK(tmp.alloc(&d_off, (size_t)nr + 1));
+    HIPCHK(tmp.alloc(&d_nsent, (size_t)nr));
     HIPCHK(tmp.alloc(&a_col, (size_t)nnz));
     HIPCHK(tmp.alloc(&a_val, (size_t)nnz));
     HIPCHK(tmp.alloc(&d_dfilt, (size_t)n));
     HIPCHK(tmp.alloc(&d_dghost, (size_t)n));
-    HIPCHK(hipMemcpyAsync(d_owner, ctx->eptn.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_proc, proc.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_order, order.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_sent, sent.data(), (size_t)n, hipMemcpyHostToDevice, st));
+    if (ctx->d_eptn) {
+      d_owner = ctx->d_eptn;  // uploaded with the matrix
+    } else {
+      HIPCHK(tmp.alloc(&d_owner, (size_t)n));
+      HIPCHK(hipMemcpyAsync(d_owner, ctx->eptn.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(d_order, rs.order.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_off, rs.own_off.data(), ((size_t)nr + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_nsent, ctx->mp_nsent.data(), (size_t)nr * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    launch_mp_layout(n, d_order, d_owner, d_off, d_nsent, d_proc, d_sent, st);
     launch_mp_own_rows(n, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_valin, d_strong, d_owner, d_sent,
                        a_col, a_val, d_dfilt, d_dghost, st);
     launch_mp_rt_rows(n, d_order, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_valin, d_strong, d_num,
@@ -1590,7 +1619,7 @@ static int upload_matrix(doug_ctx* ctx, bool values = true) {
   // several processes with overlap 1: A_ghost's rows smooth with the
   // values as read (the device coarse build)
   if (rank_coarse(ctx) && ctx->ol_dist == 1 && ctx->smoothers == 1 && ctx->use_device_coarse &&
-      (rc = upload(ctx, &ctx->d_valin, ctx->A_input_val)))
+      ((rc = upload(ctx, &ctx->d_valin, ctx->A_input_val)) || (rc = upload(ctx, &ctx->d_eptn, ctx->eptn))))
     return rc;
   std::vector<int32_t> rb = row_blocks(A);
   ctx->nrb = (int)rb.size() - 1;
@@ -2061,6 +2090,8 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     set_setup_thread_cap(side);
   }
   if (!dev_values) phase_values(ctx, scaled, strong);
+  // waits for the host thread that enqueues the subdomain work, if any
+  std::function<int()> join_fine;
   // R = T S^T and A_c on the device for the smoothed prolongator
   auto coarse_space = [&]() -> int {
     if (rank_coarse(ctx)) {
@@ -2075,6 +2106,8 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
         }
         (void)hipStreamSynchronize(ctx->stream);
       }
+      if (join_fine)
+        if (int r = join_fine()) return r;
       if (int r = host_values_now(ctx)) return r;
       phase_coarse_ranks(ctx, strong);
       return DOUG_OK;
@@ -2123,15 +2156,41 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
     auto fine = [&]() -> int {
       phase_subdomains(ctx);
       HIPCHK(hipStreamWaitEvent(ctx->stream2, ev[4], 0));
-      std::swap(ctx->stream, ctx->stream2);
-      int r = device_fine(ctx, ev[0], ev[1]);
-      std::swap(ctx->stream, ctx->stream2);
+      int r;
+      {
+        StreamOverride on2(ctx->stream2);
+        r = device_fine(ctx, ev[0], ev[1]);
+      }
       if (r) return r;
       HIPCHK(hipEventRecord(ev[5], ctx->stream2));
       return DOUG_OK;
     };
     if (rank_coarse(ctx)) {
-      if ((rc = fine()) || (rc = coarse_space()) || (rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+      // With exact subdomain solves and the device coarse build, a second
+      // host thread enqueues the subdomain work (index sets, occurrence
+      // lists, envelopes, factorisation) while this one builds the coarse
+      // space: both spend most of their time waiting on the GPU.  They touch
+      // disjoint context members; allocations and errors go through ctx->mu.
+      // The host coarse build (the fallback) reads the host values, so it
+      // waits for that thread first.
+      const bool beside = ctx->p.fine_method == 1 && ctx->ol_dist == 1 && ctx->smoothers == 1 &&
+                          ctx->use_device_coarse && ctx->d_valin;
+      int rc_fine = DOUG_OK;
+      std::thread fth;
+      if (beside)
+        fth = std::thread([&]() {
+          rc_fine = hipSetDevice(ctx->device) == hipSuccess ? fine() : DOUG_E_HIP;
+        });
+      else if ((rc = fine()))
+        return rc;
+      join_fine = [&]() {
+        if (fth.joinable()) fth.join();
+        return rc_fine;
+      };
+      rc = coarse_space();
+      const int rf = join_fine();
+      join_fine = nullptr;
+      if (rc || (rc = rf) || (rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
     } else {
       if ((rc = coarse_space()) || (rc = device_coarse(ctx, ev[2], ev[3])) || (rc = fine())) return rc;
     }

@@ -285,6 +285,21 @@ __global__ void k_gather_entries(int64_t n, const int32_t* __restrict__ order,
 // owner; A_c is the sum, in process order, of each process's part
 // R(:, own) (A R^T)(own, :).
 
+// The process layout from the process-major node order `order` (every
+// rank's own nodes in its local order, rank after rank, rank r from
+// off[r]): proc[v] = v's position in it; sent[v] = v is one of the first
+// nsent[owner] own nodes of its owner (the nodes a rank sends lead its local
+// order).  `order` is a permutation of [0, n).
+__global__ void k_mp_layout(int n, const int32_t* __restrict__ order, const int32_t* __restrict__ owner,
+                            const int32_t* __restrict__ off, const int32_t* __restrict__ nsent,
+                            int32_t* __restrict__ proc, uint8_t* __restrict__ sent) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int v = order[e], r = owner[v];
+  proc[v] = e;
+  sent[v] = (uint8_t)(e - off[r] < nsent[r]);
+}
+
 // Row j in its owner's entry order (group: the column is a node its owner
 // sends; then ascending column, then stored position), with the values
 // after the round trip; the filtered diagonal summed in that order; and the
@@ -533,6 +548,12 @@ void launch_mp_own_rows(int n, const int32_t* ptr, const int32_t* col, const dou
   if (n <= 0) return;
   hipLaunchKernelGGL(k_mp_own_rows, dim3(cdiv64(n, 256)), dim3(256), 0, st, n, ptr, col, val, valin,
                      strong, owner, sent, ocol, oval, dfilt, dghost);
+}
+
+void launch_mp_layout(int n, const int32_t* order, const int32_t* owner, const int32_t* off,
+                      const int32_t* nsent, int32_t* proc, uint8_t* sent, hipStream_t st) {
+  hipLaunchKernelGGL(k_mp_layout, dim3(cdiv64(n, 256)), dim3(256), 0, st, n, order, owner, off, nsent, proc,
+                     sent);
 }
 
 void launch_mp_rt_rows(int n, const int32_t* order, const int32_t* ptr, const int32_t* col,

@@ -166,12 +166,183 @@ __global__ void k_pad_identity(T* __restrict__ tiles, const BlockDesc* __restric
   tiles[b.base + toff(b, b.nt - 1, b.nt - 1) + lr * kTile + lr] = (T)1;
 }
 
+// ---- Cholesky factor and inverse of one 64x64 tile, in 16x16 blocks ----
+//
+// A 16x16 block inside one wavefront: lane l holds row l%16, columns
+// 4(l/16)..4(l/16)+3 in v[0..3].
+
+// Lower Cholesky factor of the block in v, right-looking: with
+// s_c = rsqrt(D_cc), L_cc = D_cc s_c, L_rc = D_rc s_c and
+// D_rq -= L_rc L_qc for c < q <= r; zeros above the diagonal on return, and
+// s_r (= 1 / L_rr) in rinv of every lane of row r.  A non-positive pivot is
+// replaced by 1 and reported through *flag when `report` is set.
+template <typename T>
+__device__ __forceinline__ void potrf16(T (&v)[4], T& rinv, int lane, bool report, int* flag) {
+  const int r = lane & 15, g = lane >> 4;
+  rinv = (T)0;
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    const int cg = c >> 2, cq = c & 3;
+    T piv = __shfl(v[cq], c + 16 * cg);
+    if (!(piv > (T)0)) {
+      if (report && lane == 0) atomicCAS(flag, 0, (int)blockIdx.x + 1);
+      piv = (T)1;
+    }
+    const T sc = rsqrt(piv);
+    if (r == c) rinv = sc;
+    if (g == cg) v[cq] = r > c ? v[cq] * sc : (r == c ? piv * sc : (T)0);
+    const T lrc = __shfl(v[cq], r + 16 * cg);  // L_rc
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int col = 4 * g + q;
+      const T lqc = __shfl(v[cq], col + 16 * cg);  // L_col,c
+      if (col > c && col <= r) v[q] = v[q] - lrc * lqc;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (4 * g + q > r) v[q] = (T)0;
+}
+
+// acc += A B^T and acc += A B for 16x16 blocks in LDS (row strides la, lb)
+template <typename T>
+__device__ __forceinline__ void mma16_abt(typename Mma<T>::v4& acc, const T* a, int la, const T* b, int lb,
+                                          int lane) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) acc = Mma<T>::op(a[li * la + 4 * kk + lk], b[li * lb + 4 * kk + lk], acc);
+}
+template <typename T>
+__device__ __forceinline__ void mma16_ab(typename Mma<T>::v4& acc, const T* a, int la, const T* b, int lb,
+                                         int lane) {
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < 4; kk++) acc = Mma<T>::op(a[li * la + 4 * kk + lk], b[(4 * kk + lk) * lb + li], acc);
+}
+
+// orders this wavefront's LDS writes before its later LDS reads of them
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one wavefront's staging: a 16x16 block (stride 17), then 16 reciprocals
+constexpr int kWs = 16 * 17 + 16;
+
+// The symmetric 64x64 tile C in As (lower part read) becomes W = L^-1 in
+// Cg (row-major, zeros above the diagonal), C = L L^T.  Blocked by 16: each
+// diagonal block is factored by every wavefront in registers (the same
+// values in each, so no barrier is needed before the panel); each
+// wavefront solves one panel block below it, X L_kk^T = C_Ik, one row per
+// lane with L_kk from its own staging copy; the trailing update runs on the
+// matrix cores.  Then W: each wavefront inverts one diagonal block, one
+// column per lane (forward substitution, no lane exchange), and the
+// off-diagonal blocks follow by distance from the diagonal,
+// W_IJ = -W_II sum_{J<=K<I} L_IK W_KJ.  As ends with L, Bs with W; Ws holds
+// 4 x kWs elements and rv 64 reciprocals of L's diagonal.  256 threads.
+template <typename T>
+__device__ __forceinline__ void factor_invert_tile(T* As, T* Bs, T* Ws, T* rv, T* __restrict__ Cg, int* flag) {
+  typedef typename Mma<T>::v4 V;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  T* ws = Ws + w * kWs;
+  for (int kb = 0; kb < 4; kb++) {
+    T l[4], rinv;
+#pragma unroll
+    for (int q = 0; q < 4; q++) l[q] = As[(16 * kb + r) * LD + 16 * kb + 4 * g + q];
+    potrf16(l, rinv, lane, w == 0, flag);
+    const int I = kb + 1 + w;  // this wavefront's panel block
+    if (I < 4) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) ws[r * 17 + 4 * g + q] = l[q];
+      if (g == 0) ws[16 * 17 + r] = rinv;
+      wave_lds_sync();
+      if (lane < 16) {  // row 16 I + lane of the panel: X L_kk^T = C, column by column
+        T* row = As + (16 * I + lane) * LD + 16 * kb;
+        T x[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++) x[c] = row[c];
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+          x[c] = x[c] * ws[16 * 17 + c];
+#pragma unroll
+          for (int c2 = c + 1; c2 < 16; c2++) x[c2] = x[c2] - x[c] * ws[c2 * 17 + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 16; c++) row[c] = x[c];
+      }
+    }
+    __syncthreads();  // every wavefront has read C_kk: wavefront 3 may overwrite it
+    if (w == 3) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) As[(16 * kb + r) * LD + 16 * kb + 4 * g + q] = l[q];
+      if (g == 0) rv[16 * kb + r] = rinv;
+    }
+    // trailing update of the blocks (I, J), kb < J <= I < 4
+    const int m = 3 - kb;
+    for (int idx = w; idx < m * (m + 1) / 2; idx += 4) {
+      int ii = 0;
+      while ((ii + 1) * (ii + 2) / 2 <= idx) ii++;
+      const int I2 = kb + 1 + ii, J2 = kb + 1 + (idx - ii * (ii + 1) / 2);
+      V acc = V{0, 0, 0, 0};
+      mma16_abt<T>(acc, As + 16 * I2 * LD + 16 * kb, LD, As + 16 * J2 * LD + 16 * kb, LD, lane);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        T* c = As + (16 * I2 + Mma<T>::row(lane, q)) * LD + 16 * J2 + (lane & 15);
+        *c = *c - acc[q];
+      }
+    }
+    __syncthreads();
+  }
+  // W_ww = L_ww^-1 by columns: lane j solves L_ww x = e_j
+  if (lane < 16) {
+    const T* L = As + 16 * w * LD + 16 * w;
+    T x[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) x[i] = i == lane ? (T)1 : (T)0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      x[i] = x[i] * rv[16 * w + i];
+#pragma unroll
+      for (int i2 = i + 1; i2 < 16; i2++) x[i2] = x[i2] - L[i2 * LD + i] * x[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) Bs[(16 * w + i) * LD + 16 * w + lane] = i >= lane ? x[i] : (T)0;
+  }
+  __syncthreads();
+  // off-diagonal blocks of W by distance from the diagonal
+  for (int dd = 1; dd < 4; dd++) {
+    const int J = w, I = w + dd;
+    if (I < 4) {
+      V acc = V{0, 0, 0, 0};
+      for (int K = J; K < I; K++) mma16_ab<T>(acc, As + 16 * I * LD + 16 * K, LD, Bs + 16 * K * LD + 16 * J, LD, lane);
+#pragma unroll
+      for (int q = 0; q < 4; q++) ws[Mma<T>::row(lane, q) * 17 + (lane & 15)] = acc[q];
+      wave_lds_sync();
+      V acc2 = V{0, 0, 0, 0};
+      mma16_ab<T>(acc2, Bs + 16 * I * LD + 16 * I, LD, ws, 17, lane);
+#pragma unroll
+      for (int q = 0; q < 4; q++) Bs[(16 * I + Mma<T>::row(lane, q)) * LD + 16 * J + (lane & 15)] = -acc2[q];
+    }
+    __syncthreads();
+  }
+  // W to global memory, zeros above the diagonal blocks
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int e = t + 256 * q, row = e >> 6, col = e & 63;
+    Cg[e] = (col >> 4) <= (row >> 4) ? Bs[row * LD + col] : (T)0;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
                                                    const BlockDesc* __restrict__ d, int j,
                                                    int* __restrict__ flag, int left) {
   __shared__ T As[kTile * LD];
   __shared__ T Bs[kTile * LD];
+  __shared__ T Ws[4 * kWs];
+  __shared__ T rv[kTile];
   typedef typename Mma<T>::v4 V;
   const BlockDesc b = d[blockIdx.x];
   if (j >= b.nt) return;
@@ -195,67 +366,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(T* __restrict__ tiles,
         As[row * LD + col] = Cg[row * kTile + col] - acc[bi][bj][r];
       }
   __syncthreads();
-  // Unblocked Cholesky of the 64x64 tile in registers: lane r of wave g
-  // holds row r, columns 16g..16g+15.  Step c: the wave owning column c
-  // takes the pivot from lane c, scales the column and publishes it
-  // (column-major in Bs, which ends up holding L); then every lane updates
-  // its entries right of c.  One barrier per step: each step writes its own
-  // column of Bs.  The same operations, in the same order, as the textbook
-  // right-looking loop A_rq -= L_rc L_qc, L_rc = A_rc / sqrt(A_cc).
-  T a[16];
-#pragma unroll
-  for (int jj = 0; jj < 16; jj++) a[jj] = As[lane * LD + 16 * w + jj];
-  for (int cb = 0; cb < 4; cb++) {
-#pragma unroll
-    for (int cc = 0; cc < 16; cc++) {
-      const int c = 16 * cb + cc;
-      if (w == cb) {
-        T dv = __shfl(a[cc], c);
-        if (!(dv > (T)0)) {
-          if (lane == 0) atomicCAS(flag, 0, (int)blockIdx.x + 1);
-          dv = (T)1;
-        }
-        const T piv = sqrt(dv);
-        Bs[c * LD + lane] = lane > c ? a[cc] / piv : (lane == c ? piv : (T)0);
-      }
-      __syncthreads();
-      const T lrc = Bs[c * LD + lane];
-      if (w > cb) {
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) a[jj] = a[jj] - lrc * Bs[c * LD + 16 * w + jj];
-      } else if (w == cb) {
-#pragma unroll
-        for (int jj = cc + 1; jj < 16; jj++) a[jj] = a[jj] - lrc * Bs[c * LD + 16 * w + jj];
-      }
-    }
-  }
-  // W = inv(L) by forward substitution, now with lane q of wave g holding
-  // column q of rows 16g..16g+15: W_kq = (delta_kq - sum_{m<k} L_km W_mq) / L_kk,
-  // the sum taken in ascending m.  Row k is divided by the wave owning it and
-  // published in As (row-major); every later row then subtracts L_rk W_kq.
-#pragma unroll
-  for (int jj = 0; jj < 16; jj++) a[jj] = (16 * w + jj == lane) ? (T)1 : (T)0;
-  for (int kb = 0; kb < 4; kb++) {
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) {
-      const int k = 16 * kb + kk;
-      if (w == kb) {
-        a[kk] = a[kk] / Bs[k * LD + k];
-        As[k * LD + lane] = a[kk];
-      }
-      __syncthreads();
-      const T wkq = As[k * LD + lane];
-      if (w > kb) {
-#pragma unroll
-        for (int jj = 0; jj < 16; jj++) a[jj] = a[jj] - Bs[k * LD + 16 * w + jj] * wkq;
-      } else if (w == kb) {
-#pragma unroll
-        for (int jj = kk + 1; jj < 16; jj++) a[jj] = a[jj] - Bs[k * LD + 16 * w + jj] * wkq;
-      }
-    }
-  }
-#pragma unroll
-  for (int jj = 0; jj < 16; jj++) Cg[(16 * w + jj) * kTile + lane] = a[jj];
+  factor_invert_tile<T>(As, Bs, Ws, rv, Cg, flag);
 }
 
 template <typename T>

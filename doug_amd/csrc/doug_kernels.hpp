@@ -256,6 +256,8 @@ void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double*
 // the A_ghost diagonals; R^T rows in process order; entry ids in process
 // order for the sort that gives R in process order; A_c with each process's
 // part summed first, rows sorted by column
+void launch_mp_layout(int n, const int32_t* order, const int32_t* owner, const int32_t* off,
+                      const int32_t* nsent, int32_t* proc, uint8_t* sent, hipStream_t st);
 void launch_mp_own_rows(int n, const int32_t* ptr, const int32_t* col, const double* val,
                         const double* valin, const uint8_t* strong, const int32_t* owner,
                         const uint8_t* sent, int32_t* ocol, double* oval, double* dfilt,

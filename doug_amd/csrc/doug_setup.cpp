@@ -1570,6 +1570,9 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
   }
   for (int i = 0; i < n; i++) own[eptn[i]].push_back(i);
   out.ranks.assign((size_t)nranks, RankLocal{});
+  out.own_off.assign((size_t)nranks + 1, 0);
+  for (int r = 0; r < nranks; r++) out.own_off[r + 1] = out.own_off[r] + (int)own[r].size();
+  out.order.resize((size_t)n);
   std::vector<Aggregates> inner((size_t)nranks), full((size_t)nranks);
   std::vector<std::vector<int>> sst((size_t)nranks), sin((size_t)nranks), dsin((size_t)nranks);
   // own local indices in ascending global order, and the own nodes listed in
@@ -1600,6 +1603,7 @@ void virtual_ranks(const Csr& As, const std::vector<double>& val, const std::vec
       for (int r = next_rank++; r < nranks; r = next_rank++) {
         RankLocal& R = out.ranks[r];
         rank_local(As, val, eptn, r, own[r], ol_dist, S, R, keep_local, pos ? pos->data() : nullptr);
+        std::copy(R.lg.begin(), R.lg.begin() + R.ninner, out.order.begin() + out.own_off[r]);
         // getLocal (Aggregate_utils.f95:33-51): the own rows, which lead the
         // local matrix (these steps read rows < nrows alone); a new matrix
         // carries no symmetry flags, so the strength test scales by rows

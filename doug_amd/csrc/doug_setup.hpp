@@ -169,6 +169,9 @@ struct RankSetup {
   Aggregates inner, full;             // global numbering
   std::vector<int> sstarts, sinds;    // subdomains with their layers, global ids
   std::vector<int> dinds;             // sinds, each subdomain in block_rcm order (if asked)
+  // every rank's own nodes in its local order, rank after rank (own_off[r]
+  // is rank r's first position): the process-major node order
+  std::vector<int> order, own_off;
 };
 
 // `As`: the assembled matrix pattern with each row sorted by column
