@@ -1037,9 +1037,14 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
     // with at least one rank per worker thread the ranks also put their
     // subdomains in the factor order of the device copy (reverse
     // Cuthill-McKee over A's pattern); with fewer ranks the threaded
-    // block_rcm of the subdomain preparation keeps every thread busy
+    // block_rcm of the subdomain preparation keeps every thread busy.
+    // When the subdomain work is prepared beside the coarse build (doug_setup)
+    // that order is left to it too: only the aggregates are needed before
+    // the coarse space can start.
+    const bool beside = p.fine_method == 1 && ctx->ol_dist == 1 && ctx->smoothers == 1 &&
+                        ctx->use_device_coarse && ctx->inj_nagr == 0 && ctx->has_coarse;
     const Csr* rcm = (p.fine_method == 1 && ctx->local_order == 1 && ctx->device >= 0 &&
-                      ctx->nranks >= setup_threads())
+                      ctx->nranks >= setup_threads() && !beside)
                          ? &ctx->A
                          : nullptr;
     if (rows_sorted_by_column(ctx->A)) {
@@ -2188,9 +2193,10 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
         return rc_fine;
       };
       rc = coarse_space();
+      if (!rc) rc = device_coarse(ctx, ev[2], ev[3]);
       const int rf = join_fine();
       join_fine = nullptr;
-      if (rc || (rc = rf) || (rc = device_coarse(ctx, ev[2], ev[3]))) return rc;
+      if (rc || (rc = rf)) return rc;
     } else {
       if ((rc = coarse_space()) || (rc = device_coarse(ctx, ev[2], ev[3])) || (rc = fine())) return rc;
     }

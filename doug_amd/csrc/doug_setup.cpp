@@ -11,6 +11,7 @@
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <stdexcept>
 
 namespace doug {
@@ -180,6 +181,53 @@ std::vector<uint8_t> find_strong(const Csr& A, const std::vector<double>& scaled
   return strong;
 }
 
+// Compare-exchange of slots A < B: (key, pos) in descending key, ascending
+// pos among equal keys; written with selects rather than branches, and
+// with constant slot numbers, so that the eight slots stay in registers.
+template <int A, int B>
+static inline void cx_desc(double (&key)[8], int (&pos)[8]) {
+  // the exchange as bit masks: a compiler may turn selects back into
+  // branches, which mispredict on these data
+  const bool sw = (key[B] > key[A]) | ((key[B] == key[A]) & (pos[B] < pos[A]));
+  const uint64_t m = 0 - (uint64_t)sw;
+  uint64_t ka, kb;
+  std::memcpy(&ka, &key[A], 8);
+  std::memcpy(&kb, &key[B], 8);
+  const uint64_t na = (kb & m) | (ka & ~m), nb = (ka & m) | (kb & ~m);
+  std::memcpy(&key[A], &na, 8);
+  std::memcpy(&key[B], &nb, 8);
+  const uint32_t m32 = (uint32_t)m, pa = (uint32_t)pos[A], pb = (uint32_t)pos[B];
+  pos[A] = (int)((pb & m32) | (pa & ~m32));
+  pos[B] = (int)((pa & m32) | (pb & ~m32));
+}
+
+template <int A, int B>
+static inline void cx_asc(int64_t (&key)[8]) {
+  const int64_t a = key[A], b = key[B];
+  key[A] = std::min(a, b);
+  key[B] = std::max(a, b);
+}
+
+// Eight slots through Batcher's odd-even merge network (19 exchanges).
+static inline void sort8_desc(double (&k)[8], int (&p)[8]) {
+  cx_desc<0, 1>(k, p), cx_desc<2, 3>(k, p), cx_desc<4, 5>(k, p), cx_desc<6, 7>(k, p);
+  cx_desc<0, 2>(k, p), cx_desc<1, 3>(k, p), cx_desc<4, 6>(k, p), cx_desc<5, 7>(k, p);
+  cx_desc<1, 2>(k, p), cx_desc<5, 6>(k, p);
+  cx_desc<0, 4>(k, p), cx_desc<1, 5>(k, p), cx_desc<2, 6>(k, p), cx_desc<3, 7>(k, p);
+  cx_desc<2, 4>(k, p), cx_desc<3, 5>(k, p);
+  cx_desc<1, 2>(k, p), cx_desc<3, 4>(k, p), cx_desc<5, 6>(k, p);
+}
+
+// Eight distinct keys, ascending, by the same network.
+static inline void sort8_asc(int64_t (&k)[8]) {
+  cx_asc<0, 1>(k), cx_asc<2, 3>(k), cx_asc<4, 5>(k), cx_asc<6, 7>(k);
+  cx_asc<0, 2>(k), cx_asc<1, 3>(k), cx_asc<4, 6>(k), cx_asc<5, 7>(k);
+  cx_asc<1, 2>(k), cx_asc<5, 6>(k);
+  cx_asc<0, 4>(k), cx_asc<1, 5>(k), cx_asc<2, 6>(k), cx_asc<3, 7>(k);
+  cx_asc<2, 4>(k), cx_asc<3, 5>(k);
+  cx_asc<1, 2>(k), cx_asc<3, 4>(k), cx_asc<5, 6>(k);
+}
+
 // SpMtx_build_refs_symm with sortdown (SpMtx_arrangement.F90:461-568):
 // strong off-diagonal neighbours per row, |scaled| descending, ties keep
 // their row order.
@@ -199,10 +247,40 @@ static void strong_refs(const Csr& A, const std::vector<double>& scaled,
   parallel_chunks(n, setup_threads(), [&](int, int b, int e) {
     std::vector<int> tmp;
     for (int i = b; i < e; i++) {
+      const int cnt = srow[i + 1] - srow[i];
+      const int p0 = A.ptr[i], len = A.ptr[i + 1] - p0;
+      if (len <= 8) {
+        // |scaled| descending, ties in row order: a strict order on
+        // (|scaled|, position), so a sorting network over the row's (at most
+        // eight) entries gives the stable order; entries that are not strong
+        // off-diagonal neighbours get the key -1 and sort after all others.
+        // Slots are filled by constant index (no unpredictable branches and
+        // no stores at computed offsets).
+        double key[8];
+        int pos[8];
+#define DOUG_SLOT(e)                                                              \
+  {                                                                               \
+    const int q = e < len ? p0 + e : p0;                                          \
+    const int c = A.col[q];                                                       \
+    const bool ok = (e < len) & (strong[q] != 0) & (c != i) & (c < n);            \
+    key[e] = ok ? std::fabs(scaled[q]) : -1.0;                                    \
+    pos[e] = q;                                                                   \
+  }
+        DOUG_SLOT(0) DOUG_SLOT(1) DOUG_SLOT(2) DOUG_SLOT(3)
+        DOUG_SLOT(4) DOUG_SLOT(5) DOUG_SLOT(6) DOUG_SLOT(7)
+#undef DOUG_SLOT
+        sort8_desc(key, pos);
+        // read back by constant index, so that key and pos stay in registers
+        const int res[8] = {pos[0], pos[1], pos[2], pos[3], pos[4], pos[5], pos[6], pos[7]};
+        int* out = scol.data() + srow[i];
+        const int* col = A.col.data();
+        for (int q = 0; q < cnt; q++) out[q] = col[res[q]];
+        continue;
+      }
       tmp.clear();
       for (int p = A.ptr[i]; p < A.ptr[i + 1]; p++)
         if (strong[p] && A.col[p] != i && A.col[p] < n) tmp.push_back(p);
-      // stable insertion sort, |scaled| descending (rows are short)
+      // stable insertion sort, |scaled| descending
       for (size_t q = 1; q < tmp.size(); q++) {
         const int e = tmp[q];
         const double ve = std::fabs(scaled[e]);
@@ -986,26 +1064,34 @@ struct LocalGraph {
 // number of levels and leaves the last level in [last_b, last_e) of `order`.
 int bfs_levels(const LocalGraph& g, int root, std::vector<int>& mark, int stamp,
                std::vector<int>& order, int& last_b, int& last_e) {
-  order.clear();
-  order.push_back(root);
-  mark[root] = stamp;
-  int lb = 0, le = 1, levels = 1;
+  // `order` needs room for every node (each enters once); the scan appends
+  // without branches: a node is written at the end every time and kept
+  // (the end advanced) only when it was unmarked
+  if (order.size() < g.deg.size() + 1) order.resize(g.deg.size() + 1);
+  int* ord = order.data();
+  int* mk = mark.data();
+  const int* ptr = g.ptr.data();
+  const int* adj = g.adj.data();
+  ord[0] = root;
+  mk[root] = stamp;
+  int lb = 0, le = 1, levels = 1, top = 1;
   while (true) {
     for (int q = lb; q < le; q++) {
-      int v = order[q];
-      for (int p = g.ptr[v]; p < g.ptr[v + 1]; p++) {
-        int u = g.adj[p];
-        if (mark[u] != stamp) {
-          mark[u] = stamp;
-          order.push_back(u);
-        }
+      const int v = ord[q];
+      for (int p = ptr[v]; p < ptr[v + 1]; p++) {
+        const int u = adj[p];
+        const int fresh = mk[u] != stamp;
+        ord[top] = u;
+        top += fresh;
+        mk[u] = stamp;
       }
     }
-    if ((int)order.size() == le) break;
+    if (top == le) break;
     lb = le;
-    le = (int)order.size();
+    le = top;
     levels++;
   }
+  order.resize((size_t)top);
   last_b = lb;
   last_e = le;
   return levels;
@@ -1016,7 +1102,7 @@ int bfs_levels(const LocalGraph& g, int root, std::vector<int>& mark, int stamp,
 namespace {
 
 struct RcmBuf {
-  std::vector<int> mark, order, bfs, nbr;
+  std::vector<int> mark, order, bfs, cbfs, nbr;
   std::vector<char> placed;
   LocalGraph g;
 };
@@ -1036,16 +1122,27 @@ void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::v
   }
   for (int li = 0; li < s; li++) floc[inds[s0 + li]] = li;
   g.ptr.assign((size_t)s + 1, 0);
-  g.adj.clear();
   g.deg.assign((size_t)s, 0);
-  for (int li = 0; li < s; li++) {
-    int gi = inds[s0 + li];
-    for (int p = A.ptr[gi]; p < A.ptr[gi + 1]; p++) {
-      int lj = floc[A.col[p]];
-      if (lj >= 0 && lj != li) g.adj.push_back(lj);
+  {
+    // room for every entry of the block's rows; an entry is written at the
+    // end every time and kept when its column lies in the block (no branch)
+    size_t cap = 0;
+    for (int li = 0; li < s; li++) cap += (size_t)(A.ptr[inds[s0 + li] + 1] - A.ptr[inds[s0 + li]]);
+    if (g.adj.size() < cap + 1) g.adj.resize(cap + 1);
+    int* adj = g.adj.data();
+    const int* fl = floc.data();
+    const int* acol = A.col.data();
+    int top = 0;
+    for (int li = 0; li < s; li++) {
+      const int gi = inds[s0 + li];
+      for (int p = A.ptr[gi]; p < A.ptr[gi + 1]; p++) {
+        const int lj = fl[acol[p]];
+        adj[top] = lj;
+        top += (lj >= 0) & (lj != li);
+      }
+      g.ptr[li + 1] = top;
+      g.deg[li] = g.ptr[li + 1] - g.ptr[li];
     }
-    g.ptr[li + 1] = (int)g.adj.size();
-    g.deg[li] = g.ptr[li + 1] - g.ptr[li];
   }
   for (int li = 0; li < s; li++) floc[inds[s0 + li]] = -1;
   mark.assign((size_t)s, 0);
@@ -1054,18 +1151,29 @@ void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::v
   int stamp = 0, next = 0;
   while ((int)order.size() < s) {
     while (placed[next]) next++;
-    // minimum-degree unplaced node of this component
-    int root = next;
-    {
-      int lb, le;
+    // minimum-degree unplaced node of this component (ties: lowest index).
+    // The first component is usually the whole block: then it is the
+    // block's minimum-degree node, and the search from it shows that it
+    // reached every node
+    int root = next, lb, le, ecc = -1;
+    if (order.empty()) {
+      int gmin = 0;
+      for (int v = 1; v < s; v++)
+        if (g.deg[v] < g.deg[gmin]) gmin = v;
+      ecc = bfs_levels(g, gmin, mark, ++stamp, bfs, lb, le);
+      if ((int)bfs.size() == s)
+        root = gmin;
+      else
+        ecc = -1;
+    }
+    if (ecc < 0) {
       bfs_levels(g, next, mark, ++stamp, bfs, lb, le);
       for (int v : bfs)
         if (g.deg[v] < g.deg[root] || (g.deg[v] == g.deg[root] && v < root)) root = v;
+      ecc = bfs_levels(g, root, mark, ++stamp, bfs, lb, le);
     }
-    // pseudo-peripheral node: move to a minimum-degree node of the
-    // last level while the eccentricity grows
-    int lb, le;
-    int ecc = bfs_levels(g, root, mark, ++stamp, bfs, lb, le);
+    // pseudo-peripheral node: move to a minimum-degree node of the last
+    // level while the eccentricity grows
     for (int round = 0; round < 8; round++) {
       int cand = bfs[lb];
       for (int q = lb; q < le; q++) {
@@ -1073,7 +1181,7 @@ void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::v
         if (g.deg[v] < g.deg[cand] || (g.deg[v] == g.deg[cand] && v < cand)) cand = v;
       }
       int clb, cle;
-      std::vector<int> cbfs;
+      std::vector<int>& cbfs = buf.cbfs;
       int cecc = bfs_levels(g, cand, mark, ++stamp, cbfs, clb, cle);
       if (cecc <= ecc) break;
       root = cand;
@@ -1088,18 +1196,33 @@ void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::v
     placed[root] = 1;
     while (head < order.size()) {
       int v = order[head++];
-      nbr.clear();
+      nbr.resize((size_t)(g.ptr[v + 1] - g.ptr[v]) + 1);
+      int m = 0;
       for (int p = g.ptr[v]; p < g.ptr[v + 1]; p++) {
-        int u = g.adj[p];
-        if (!placed[u]) {
-          placed[u] = 1;
-          nbr.push_back(u);
-        }
+        const int u = g.adj[p];
+        nbr[m] = u;
+        m += !placed[u];
+        placed[u] = 1;
       }
-      std::sort(nbr.begin(), nbr.end(), [&](int x, int y) {
-        return g.deg[x] != g.deg[y] ? g.deg[x] < g.deg[y] : x < y;
-      });
-      order.insert(order.end(), nbr.begin(), nbr.end());
+      nbr.resize((size_t)m);
+      if (nbr.size() <= 8) {
+        // (degree, index) packed into one ordered key; eight slots through
+        // the sorting network, unused ones last
+        int64_t kb[8];
+        const int m = (int)nbr.size();
+        for (int q = 0; q < 8; q++)
+          kb[q] = q < m ? ((int64_t)g.deg[nbr[q]] << 32) | (uint32_t)nbr[q] : INT64_MAX;
+        int64_t key[8] = {kb[0], kb[1], kb[2], kb[3], kb[4], kb[5], kb[6], kb[7]};
+        sort8_asc(key);
+        kb[0] = key[0], kb[1] = key[1], kb[2] = key[2], kb[3] = key[3];
+        kb[4] = key[4], kb[5] = key[5], kb[6] = key[6], kb[7] = key[7];
+        for (int q = 0; q < m; q++) order.push_back((int)(uint32_t)kb[q]);
+      } else {
+        std::sort(nbr.begin(), nbr.end(), [&](int x, int y) {
+          return g.deg[x] != g.deg[y] ? g.deg[x] < g.deg[y] : x < y;
+        });
+        order.insert(order.end(), nbr.begin(), nbr.end());
+      }
     }
   }
   for (int q = 0; q < s; q++) out[s0 + q] = inds[s0 + order[s - 1 - q]];
