@@ -108,6 +108,7 @@ struct doug_ctx {
   // several processes: place of each node in process order, the inverse,
   // and whether its owner sends it (staging of the device coarse build)
   std::vector<int32_t> mp_nsent;  // nodes each rank sends (its first own nodes)
+  bool ranks_kept_local = true;    // rsetup.ranks hold the local matrices
   // whether the host's A.val and strong hold the round-trip values and the
   // strength flags (a several-process set-up forms them on the device and
   // copies them back only when host code needs them)
@@ -1000,6 +1001,25 @@ static void phase_values(doug_ctx* ctx, std::vector<double>& scaled, std::vector
     for (size_t q = 0; q < strong.size(); q++) strong[q] = ctx->inj_strong[(size_t)A.src[q]] != 0;
 }
 
+// virtual_ranks for the context's decomposition and aggregation parameters
+static void run_virtual_ranks(doug_ctx* ctx, RankSetup& out, const Csr* rcm, bool keep_local) {
+  const Params& p = ctx->p;
+  const double strong_conn1 = p.strong1 != 0.0 ? p.strong1 : 0.67;
+  const int radius1 = p.radius1 > 0 ? p.radius1 : 2;
+  const int min_asize1 = p.minasize1 > 0 ? p.minasize1 : (int)(0.5 * (double)((2 * radius1 + 1) * (2 * radius1 + 1)));
+  const int max_asize1 = p.maxasize1 > 0 ? p.maxasize1 : (2 * radius1 + 1) * (2 * radius1 + 1);
+  if (rows_sorted_by_column(ctx->A)) {
+    virtual_ranks(ctx->A, ctx->A_input_val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol,
+                  ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, out, rcm,
+                  &ctx->rank_ws, keep_local);
+  } else {
+    Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
+    virtual_ranks(As, As.val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains,
+                  radius1, min_asize1, max_asize1, strong_conn1, out, rcm, &ctx->rank_ws, keep_local,
+                  &As.src);
+  }
+}
+
 // (2) fine aggregates, smoothed restriction and Galerkin coarse matrix
 //     (Partitioning_aggr.F90:33-117, CoarsePreconditioner_smooth.F90:47-89)
 static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
@@ -1047,16 +1067,11 @@ static void phase_aggregate(doug_ctx* ctx, const std::vector<double>& scaled,
                       ctx->nranks >= setup_threads() && !beside)
                          ? &ctx->A
                          : nullptr;
-    if (rows_sorted_by_column(ctx->A)) {
-      virtual_ranks(ctx->A, ctx->A_input_val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol,
-                    ctx->num_subdomains, radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup,
-                    rcm, &ctx->rank_ws, true);
-    } else {
-      Csr As = sort_rows_by_column(ctx->A, ctx->A_input_val);
-      virtual_ranks(As, As.val, ctx->eptn, ctx->nranks, ctx->ol_dist, ctx->ol, ctx->num_subdomains,
-                    radius1, min_asize1, max_asize1, strong_conn1, ctx->rsetup, rcm, &ctx->rank_ws, true,
-                    &As.src);
-    }
+    // the device coarse build forms the several-process coarse space from
+    // the whole matrix: the ranks' local matrices (and their ghost rows) are
+    // then not kept; the host build, should it be needed, forms them again
+    ctx->ranks_kept_local = !beside || ctx->device < 0;
+    run_virtual_ranks(ctx, ctx->rsetup, rcm, ctx->ranks_kept_local);
     ctx->fin = std::move(ctx->rsetup.inner);
     ctx->ffull = std::move(ctx->rsetup.full);
   } else {
@@ -1840,7 +1855,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
     if (ctx->nranks > 0 && ctx->rsetup.dinds.size() == ctx->sinds.size())
       dinds = std::move(ctx->rsetup.dinds);  // ordered by the ranks' workers
     else
-      block_rcm(A, ctx->sstarts, ctx->sinds, dinds);
+      block_rcm(A, ctx->sstarts, ctx->sinds, dinds, &ctx->rank_ws);
   } else {
     dinds = ctx->sinds;
     if (p.fine_method == 2)  // Gauss-Seidel sweeps the rows in ascending order
@@ -2114,6 +2129,14 @@ int doug_setup(doug_ctx* ctx, int32_t precision) {
       if (join_fine)
         if (int r = join_fine()) return r;
       if (int r = host_values_now(ctx)) return r;
+      if (!ctx->ranks_kept_local) {
+        // the same decomposition again, now keeping each rank's local
+        // matrix (the aggregation is deterministic: the same results)
+        RankSetup again;
+        run_virtual_ranks(ctx, again, nullptr, true);
+        ctx->rsetup.ranks = std::move(again.ranks);
+        ctx->ranks_kept_local = true;
+      }
       phase_coarse_ranks(ctx, strong);
       return DOUG_OK;
     }

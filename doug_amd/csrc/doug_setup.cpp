@@ -1230,15 +1230,36 @@ void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::v
 
 }  // namespace
 
+// node-indexed scratch of one virtual_ranks worker (kept between set-ups)
+struct RankNode {
+  int layer = -1, label = 0, gl = -1, stamp = 0;
+};
+
+struct RankScratch {
+  int n = 0;
+  std::vector<RankNode> nd;
+  std::vector<int> floc;  // -1 between uses (block_rcm_serial scratch)
+  int tick = 0;
+  explicit RankScratch(int n_) : n(n_), nd((size_t)n_), floc((size_t)n_, -1) {}
+};
+
 void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
-               std::vector<int>& out) {
+               std::vector<int>& out, RankWorkspaces* ws) {
   const int nb = (int)starts.size() - 1;
   out.assign(inds.begin(), inds.end());
-  const int nth = std::max(1, std::min(setup_threads(), nb));
+  int nth = std::max(1, std::min(setup_threads(), nb));
+  // as many threads as lent scratch arrays (a fresh one costs more than a
+  // thread gains)
+  if (ws && !ws->w.empty() && ws->w[0] && ws->w[0]->n == A.nrows) nth = std::min(nth, (int)ws->w.size());
   std::vector<std::thread> th;
   for (int t = 0; t < nth; t++)
     th.emplace_back([&, t]() {
-      std::vector<int> floc((size_t)A.nrows, -1);
+      // a node-indexed scratch of the whole matrix: the rank workers' one
+      // when it fits (-1 between uses), else a fresh one
+      RankScratch* rs = ws && t < (int)ws->w.size() && ws->w[t] && ws->w[t]->n == A.nrows ? ws->w[t].get() : nullptr;
+      std::vector<int> own;
+      if (!rs) own.assign((size_t)A.nrows, -1);
+      std::vector<int>& floc = rs ? rs->floc : own;
       RcmBuf buf;
       for (int b = t; b < nb; b += nth) rcm_block(A, inds, starts[b], starts[b + 1] - starts[b], out, floc, buf);
     });
@@ -1510,17 +1531,6 @@ bool rows_sorted_by_column(const Csr& A) {
 // cache line: layer (-1 off the rank's local matrix), labels (bit 0 inner
 // overlap of some neighbour, bit 1 sent to some neighbour), local number,
 // visit stamp.
-struct RankNode {
-  int layer = -1, label = 0, gl = -1, stamp = 0;
-};
-
-struct RankScratch {
-  int n = 0;
-  std::vector<RankNode> nd;
-  std::vector<int> floc;  // -1 between uses (block_rcm_serial scratch)
-  int tick = 0;
-  explicit RankScratch(int n_) : n(n_), nd((size_t)n_), floc((size_t)n_, -1) {}
-};
 
 RankWorkspaces::RankWorkspaces() = default;
 RankWorkspaces::~RankWorkspaces() = default;
@@ -1626,24 +1636,51 @@ void rank_local(const Csr& As, const std::vector<double>& val, const std::vector
     A.col.reserve(cap);
     A.val.reserve(cap);
     A.src.reserve(cap);
+    // the kept rows are written into [0, cap) below, one slot of slack
+    A.col.resize(cap + 1);
+    A.val.resize(cap + 1);
+    A.src.resize(cap + 1);
   }
-  for (int k = 0; k < nl; k++) {
-    const int v = out.lg[k];
-    if (S.nd[v].layer < L) {
-      const bool ownrow = S.nd[v].layer == 0;
-      for (int pass = 0; pass < 2; pass++)
-        for (int p = As.ptr[v]; p < As.ptr[v + 1]; p++) {
-          const int w = As.col[p];
-          const int lw = S.nd[w].layer;
-          if (lw < 0 || (!ownrow && lw > L)) continue;
-          const bool sent = lw == 0 && (S.nd[w].label & 2);
-          if (sent != (pass == 0)) continue;
-          A.col.push_back(S.nd[w].gl);
-          A.val.push_back(val[p]);
-          A.src.push_back(pos ? pos[p] : p);
+  {
+    // each kept row: the entries to nodes the rank sends first, then the
+    // others, each group in stored order.  A first scan counts the first
+    // group; the second writes every entry at its group's cursor and
+    // advances that cursor only for an entry that is kept (no branches)
+    int* acol = A.col.data();
+    double* aval = A.val.data();
+    int* asrc = A.src.data();
+    const RankNode* nd = S.nd.data();
+    int top = 0;
+    for (int k = 0; k < nl; k++) {
+      const int v = out.lg[k];
+      if (nd[v].layer < L) {
+        const bool ownrow = nd[v].layer == 0;
+        const int p0 = As.ptr[v], p1 = As.ptr[v + 1];
+        int ns = 0;
+        for (int p = p0; p < p1; p++) {
+          const RankNode& w = nd[As.col[p]];
+          const bool keep = (w.layer >= 0) & (ownrow | (w.layer <= L));
+          ns += keep & (w.layer == 0) & ((w.label & 2) != 0);
         }
+        int a = top, b = top + ns;
+        for (int p = p0; p < p1; p++) {
+          const RankNode& w = nd[As.col[p]];
+          const bool keep = (w.layer >= 0) & (ownrow | (w.layer <= L));
+          const bool sent = keep & (w.layer == 0) & ((w.label & 2) != 0);
+          const int at = sent ? a : b;
+          acol[at] = w.gl;
+          aval[at] = val[p];
+          asrc[at] = pos ? pos[p] : p;
+          a += sent;
+          b += keep & !sent;
+        }
+        top = b;
+      }
+      A.ptr[k + 1] = top;
     }
-    A.ptr[k + 1] = (int)A.col.size();
+    A.col.resize((size_t)top);
+    A.val.resize((size_t)top);
+    A.src.resize((size_t)top);
   }
   // A_ghost (SpMtx_build_ghost, SpMtx_arrangement.F90:1590-1612): the
   // received nodes' rows, entries to nodes up to layer L, values as read;

@@ -114,8 +114,10 @@ void set_setup_thread_cap(int cap);
 // Reverse Cuthill-McKee order of each block of `inds` over the graph of A
 // restricted to the block: out holds the same blocks, each a permutation of
 // its input.  Independent of the thread count.
+// `ws`, when given, lends its workers' node-indexed scratch.
+struct RankWorkspaces;
 void block_rcm(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,
-               std::vector<int>& out);
+               std::vector<int>& out, RankWorkspaces* ws = nullptr);
 // block_rcm on the calling thread with the caller's scratch `floc`
 // (A.nrows entries, all -1 on entry and on exit).
 void block_rcm_serial(const Csr& A, const std::vector<int>& starts, const std::vector<int>& inds,

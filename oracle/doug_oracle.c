@@ -686,6 +686,14 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
             if (stat[node] == next_start_layer) {
               if (distance[node] < mindistance) mindistance = distance[node];
               nouters++;
+              /* the reference parks them at the tail of its nextgoodstart(n)
+               * while the live entries grow from the front: once the two
+               * meet its result is undefined (the library keeps an unbounded
+               * list), so the restatement stops there */
+              if (ngoodstarts + nouters > n) {
+                fprintf(stderr, "doug_oracle: nextgoodstart exceeds n\n");
+                abort();
+              }
               nextgoodstart[n - nouters + 1] = node;
             }
           }
@@ -693,6 +701,13 @@ static void om_aggregate(OM *A, Aggrs *inner, Aggrs *full, int neighood,
         for (int j = 1; j <= nouters; j++)
           if (distance[nextgoodstart[n - j + 1]] == mindistance) {
             ngoodstarts++;
+            /* an append that lands on a parked entry not yet read (or past
+             * n) leaves the reference's list differing from an unbounded
+             * one: undefined there, so the restatement stops */
+            if (ngoodstarts > n || (ngoodstarts >= n - nouters + 1 && ngoodstarts < n - j + 1)) {
+              fprintf(stderr, "doug_oracle: nextgoodstart exceeds n\n");
+              abort();
+            }
             nextgoodstart[ngoodstarts] = nextgoodstart[n - j + 1];
           }
         for (int j = 1; j <= nneigs; j++) {

@@ -1,6 +1,7 @@
 """The HIP path (libdoug_hip.so on the GPU) against the oracle."""
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import doug_amd as D
 import oracle as O
@@ -568,6 +569,57 @@ def test_virtual_ranks_against_oracle(case):
         x32, it32, _, _ = s.pcg(b, 1e-8, 1000)
         assert abs(it32 - ito) <= 2
         assert np.linalg.norm(x32 - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_virtual_ranks_wide_coarse_rows_use_the_host_build():
+    # coarse rows wider than the device Galerkin kernel holds (128 entries):
+    # the set-up falls back to the host build of the several-process coarse
+    # space, which first forms the ranks' local matrices again (the device
+    # path does not keep them); the results are the oracle's.  A 27-point
+    # operator (tridiag(-1, 3, -1) in each direction, symmetrically scaled)
+    # with aggregates of a few nodes couples each aggregate to ~170 others
+    m = 10
+    T = sp.diags([-1.0, 3.0, -1.0], [-1, 0, 1], shape=(m, m))
+    d = sp.diags(np.random.default_rng(1).uniform(0.5, 2.0, m ** 3))
+    A = (d @ sp.kron(T, sp.kron(T, T)) @ d).tocoo()
+    n = m ** 3
+    order = np.lexsort((A.col, A.row))
+    i = (A.row[order] + 1).astype(np.int32)
+    j = (A.col[order] + 1).astype(np.int32)
+    v = A.data[order]
+    ctl = dict(radius1=1, minasize1=2, maxasize1=8, strong1=0.0, smoothers=1, overlap=1,
+               symmstruct=1, symmnumeric=1)
+    nr = 4
+    with D.Solver(-1) as h:
+        h.set_params(**ctl)
+        h.set_matrix(n, i, j, v, 1)
+        h.set_ranks(nr)
+        h.setup_host()
+        rank = np.zeros(n, np.int32)
+        for r in range(nr):
+            ni, lg = h.rank_map(r)
+            rank[lg[:ni] - 1] = r + 1
+        cp, _, _ = h.coarse_matrix()
+        assert np.diff(cp).max() > 128
+    print("wide coarse rows with ranks", flush=True)
+    o = O.Oracle.ranks(n, i, j, v, O.params(**ctl), rank)
+    b = P.uniform_rhs(n, 77)
+    with D.Solver(0) as s:
+        s.set_params(**ctl)
+        s.set_matrix(n, i, j, v, 1)
+        s.set_ranks(nr, rank)
+        for rep in range(2):  # a repeated set-up takes the same path
+            s.setup(0)
+            for a, c in zip(s.fine_aggregates(), o.fine_aggregates()):
+                np.testing.assert_array_equal(a, c)
+            for a, c in zip(s.subdomains(), o.subdomains()):
+                np.testing.assert_array_equal(a, c)
+            assert _same_entries(s.restriction(), o.R(), True)
+            assert _same_entries(s.coarse_matrix(), o.AC(), False)
+            x, it, _, _ = s.pcg(b, 1e-8, 1000)
+            xo, ito, _, _ = o.pcg(b, 1e-8, 1000)
+            assert it == ito
+            assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
