@@ -1344,9 +1344,10 @@ __global__ void k_panel_sum(const double* __restrict__ part, int nt, int nch, in
 // block.  Wave w owns rows 16w..16w+15 of each 64 x NR panel; panels of
 // finished tile rows live in U (global, re-read through L2); the workgroup's
 // barriers order its own global writes before its later reads.  Each step
-// stages KB off-diagonal tiles with their panels, and the first step of a
-// tile row also the diagonal tile, with all loads issued before any LDS
-// store.
+// stages KB off-diagonal tiles with their panels (a tile row's diagonal
+// tile with its first step); the next step's loads are issued into
+// registers before this step's products, so that their latency overlaps
+// the matrix-core work.
 template <typename T, int NR, int KB, int CG>
 __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
                                                     const BlockDesc* __restrict__ d, int nblocks,
@@ -1382,30 +1383,40 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
     const int fc = b.tr[i].fc;
-    bool diag_staged = false;
-    for (int k = fc; k < i || !diag_staged; k += KB) {
+    {
+      // steps of KB off-diagonal tiles; the next step's tiles and panels are
+      // loaded into registers while this step's products run
       TileRegs<T> ra[KB], rd;
       PanelRegs<T, NR> rb[KB];
-      if (!diag_staged) rd.load(Lb + toff(b, i, i));
+      rd.load(Lb + toff(b, i, i));
 #pragma unroll
       for (int u = 0; u < KB; u++)
-        if (k + u < i) {
-          ra[u].load(Lb + toff(b, i, k + u));
-          rb[u].load(Ub + (int64_t)(k + u) * kTile * NR);
+        if (fc + u < i) {
+          ra[u].load(Lb + toff(b, i, fc + u));
+          rb[u].load(Ub + (int64_t)(fc + u) * kTile * NR);
         }
-      if (!diag_staged) rd.store(Ds);
+      rd.store(Ds);
+      for (int k = fc;; k += KB) {
 #pragma unroll
-      for (int u = 0; u < KB; u++)
-        if (k + u < i) {
-          ra[u].store(As[u]);
-          rb[u].store(Bs[u]);
-        }
-      diag_staged = true;
-      __syncthreads();
+        for (int u = 0; u < KB; u++)
+          if (k + u < i) {
+            ra[u].store(As[u]);
+            rb[u].store(Bs[u]);
+          }
+        __syncthreads();
+        const int kn = k + KB;
 #pragma unroll
-      for (int u = 0; u < KB; u++)
-        if (k + u < i) panel_mma<T, NR, false>(acc, As[u], Bs[u], w, li, lk);
-      __syncthreads();
+        for (int u = 0; u < KB; u++)
+          if (kn + u < i) {
+            ra[u].load(Lb + toff(b, i, kn + u));
+            rb[u].load(Ub + (int64_t)(kn + u) * kTile * NR);
+          }
+#pragma unroll
+        for (int u = 0; u < KB; u++)
+          if (k + u < i) panel_mma<T, NR, false>(acc, As[u], Bs[u], w, li, lk);
+        __syncthreads();
+        if (kn >= i) break;
+      }
     }
 #pragma unroll
     for (int c = 0; c < NR / 16; c++)
@@ -1433,30 +1444,38 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
 #pragma unroll
     for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
     const int lr = b.tr[j].lr;
-    bool diag_staged = false;
-    for (int i = j + 1; i <= lr || !diag_staged; i += KB) {
+    {
       TileRegs<T> ra[KB], rd;
       PanelRegs<T, NR> rb[KB];
-      if (!diag_staged) rd.load(Lb + toff(b, j, j));
+      rd.load(Lb + toff(b, j, j));
 #pragma unroll
       for (int u = 0; u < KB; u++)
-        if (i + u <= lr) {
-          ra[u].load(Lb + toff(b, i + u, j));
-          rb[u].load(Ub + (int64_t)(i + u) * kTile * NR);
+        if (j + 1 + u <= lr) {
+          ra[u].load(Lb + toff(b, j + 1 + u, j));
+          rb[u].load(Ub + (int64_t)(j + 1 + u) * kTile * NR);
         }
-      if (!diag_staged) rd.store(Ds);
+      rd.store(Ds);
+      for (int i = j + 1;; i += KB) {
 #pragma unroll
-      for (int u = 0; u < KB; u++)
-        if (i + u <= lr) {
-          ra[u].store(As[u]);
-          rb[u].store(Bs[u]);
-        }
-      diag_staged = true;
-      __syncthreads();
+        for (int u = 0; u < KB; u++)
+          if (i + u <= lr) {
+            ra[u].store(As[u]);
+            rb[u].store(Bs[u]);
+          }
+        __syncthreads();
+        const int in = i + KB;
 #pragma unroll
-      for (int u = 0; u < KB; u++)
-        if (i + u <= lr) panel_mma<T, NR, true>(acc, As[u], Bs[u], w, li, lk);
-      __syncthreads();
+        for (int u = 0; u < KB; u++)
+          if (in + u <= lr) {
+            ra[u].load(Lb + toff(b, in + u, j));
+            rb[u].load(Ub + (int64_t)(in + u) * kTile * NR);
+          }
+#pragma unroll
+        for (int u = 0; u < KB; u++)
+          if (i + u <= lr) panel_mma<T, NR, true>(acc, As[u], Bs[u], w, li, lk);
+        __syncthreads();
+        if (in > lr) break;
+      }
     }
     T* Uj = Ub + (int64_t)j * kTile * NR;
 #pragma unroll
