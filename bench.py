@@ -601,6 +601,7 @@ def main():
                "iterations_min_max": [int(its_b.min()), int(its_b.max())],
                "prec_apply_32rhs_ms": round(prec32_ms, 3),
                "tiles_per_step": tune_batch["trsm_tiles_per_step"],
+               "column_layout": tune_batch["trsm_col_groups"],
                "prec_apply_ms_per_tiles_per_step": [round(t, 3) for t in tune_batch["trsm_ms"]],
                "rel_residual_col0": float(torch.linalg.norm(Bm[:, 0] - rr) / torch.linalg.norm(Bm[:, 0])),
                "speedup_per_rhs_vs_fp64_single": round(fp64["time_to_solve_s"] /

@@ -138,7 +138,7 @@ struct doug_ctx {
   std::vector<BlockDesc> h_sub;
   int trsv_order = 0;               // 1: largest subdomain dispatched first
   int trsm_kb = 2;                  // tiles per step of the batched envelope solve
-  int trsm_cg = 1;                  // column groups (workgroups per block) of that solve
+  int trsm_cg = 1;                  // column layout of that solve (1..3, DOUG_OPT_BATCH_COLUMN_GROUPS)
   int local_order = 1;              // 1: reverse Cuthill-McKee inside each subdomain
   BlockDesc* d_sub_trsv = nullptr;  // descriptors in dispatch order
   doug_tuning tuning{};
@@ -180,13 +180,16 @@ struct doug_ctx {
   void* d_U = nullptr;
   double *d_cG = nullptr, *d_cU = nullptr, *d_cS = nullptr, *d_subsolm = nullptr;
   double *d_Xm = nullptr, *d_Rm = nullptr, *d_Zm = nullptr, *d_Pm = nullptr, *d_Qm = nullptr;
+  double* d_Pm2 = nullptr;  // second p buffer of the fused even-column iteration
   double *d_Bm = nullptr, *d_Tm = nullptr, *d_scalm = nullptr, *d_partialm = nullptr;
   unsigned* d_counterm = nullptr;
   double* h_scalm = nullptr;  // pinned, kMaxRhs * 8
   // one PCG iteration captured as a HIP graph (iterations >= 2)
   bool use_graph = true;
   hipGraphExec_t iter_graph = nullptr;
-  hipGraphExec_t batch_graph = nullptr;  // batched iteration, for batch_graph_nrp columns
+  // batched iteration, for batch_graph_nrp columns; two graphs when the
+  // iteration alternates between the two p buffers
+  hipGraphExec_t batch_graph[2] = {nullptr, nullptr};
   int batch_graph_nrp = 0;
 };
 
@@ -253,11 +256,12 @@ static void drop_iteration_graph(doug_ctx* ctx) {
     (void)hipGraphExecDestroy(ctx->iter_graph);
     ctx->iter_graph = nullptr;
   }
-  if (ctx->batch_graph) {
-    (void)hipGraphExecDestroy(ctx->batch_graph);
-    ctx->batch_graph = nullptr;
-    ctx->batch_graph_nrp = 0;
-  }
+  for (hipGraphExec_t& g : ctx->batch_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  ctx->batch_graph_nrp = 0;
 }
 
 static void join_reaper(doug_ctx* ctx) {
@@ -2721,10 +2725,13 @@ int doug_prepare_batch(doug_ctx* ctx) {
     }
   }
   if ((rc = dalloc(ctx, &ctx->d_subsolm, ctx->sinds.size() * (size_t)NR))) return rc;
-  for (double** v : {&ctx->d_Xm, &ctx->d_Rm, &ctx->d_Zm, &ctx->d_Pm, &ctx->d_Qm, &ctx->d_Bm, &ctx->d_Tm})
+  for (double** v : {&ctx->d_Xm, &ctx->d_Rm, &ctx->d_Zm, &ctx->d_Pm, &ctx->d_Pm2, &ctx->d_Qm, &ctx->d_Bm,
+                     &ctx->d_Tm})
     if ((rc = dalloc(ctx, v, (size_t)n * NR))) return rc;
+  // block partials: the column-pair reductions take kRedBlocksWide slots,
+  // the fused SpMV one per row block
   if ((rc = dalloc(ctx, &ctx->d_scalm, (size_t)kMaxRhs * 8)) ||
-      (rc = dalloc(ctx, &ctx->d_partialm, (size_t)kRedBlocks * kMaxRhs)) ||
+      (rc = dalloc(ctx, &ctx->d_partialm, (size_t)std::max(kRedBlocksWide, ctx->nrb) * kMaxRhs)) ||
       (rc = dalloc(ctx, &ctx->d_counterm, 1)))
     return rc;
   HIPCHK(hipMemsetAsync(ctx->d_counterm, 0, sizeof(unsigned), st));
@@ -2733,8 +2740,9 @@ int doug_prepare_batch(doug_ctx* ctx) {
   return DOUG_OK;
 }
 
-// z = M^{-1} r for nrp columns (node-major, leading dimension nrp)
-static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* z) {
+// z = M^{-1} r for nrp columns (node-major, leading dimension nrp); with
+// rz_dot (nrp even) the overlap sum also forms rho = r.z per column
+static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* z, bool rz_dot = false) {
   const Params& p = ctx->p;
   hipStream_t st = ctx->stream;
   const int n = ctx->n, NR = nrp <= 16 ? 16 : 32;
@@ -2764,8 +2772,13 @@ static void enqueue_prec_multi(doug_ctx* ctx, int nrp, const double* r, double* 
                         (double*)ctx->d_U, ctx->d_subsolm, nrp, nrp, NR, ctx->trsm_kb, ctx->trsm_cg, st);
     fine_src = ctx->d_subsolm;
   }
-  launch_overlap_sum_multi(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
-                           ctx->d_Rtcol, ctx->d_Rtval, ctx->d_cS, NR, z, st);
+  if (rz_dot)
+    launch_overlap_sum_dot_multi2(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src,
+                                  coarse ? ctx->d_Rtptr : nullptr, ctx->d_Rtcol, ctx->d_Rtval, ctx->d_cS, NR,
+                                  z, r, ctx->d_partialm, ctx->d_counterm, ctx->d_scalm, st);
+  else
+    launch_overlap_sum_multi(n, nrp, ctx->d_occ_ptr, ctx->d_occ_pos, fine_src, coarse ? ctx->d_Rtptr : nullptr,
+                             ctx->d_Rtcol, ctx->d_Rtval, ctx->d_cS, NR, z, st);
 }
 
 int doug_prec_apply_batch_device(doug_ctx* ctx, int32_t nrhs, const double* dr, double* dz) {
@@ -2797,6 +2810,7 @@ static int pcg_batch_core(doug_ctx* ctx, int nrhs, double tol, int maxit, int32_
                     ctx->d_Tm, nrp, nrp, st);
   launch_residual(n * nrp, ctx->d_Bm, ctx->d_Tm, ctx->d_Rm, st);
   HIPCHK(hipMemsetAsync(ctx->d_Pm, 0, (size_t)n * nrp * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(ctx->d_Pm2, 0, (size_t)n * nrp * sizeof(double), st));
   for (int k = 0; k < kMaxRhs * 8; k++) hs[k] = 0.0;
   for (int r = 0; r < nrhs; r++) hs[r * 8 + SM_ACTIVE] = 1.0;
   HIPCHK(hipMemcpyAsync(ctx->d_scalm, hs, kMaxRhs * 8 * sizeof(double), hipMemcpyHostToDevice, st));
@@ -2809,39 +2823,57 @@ static int pcg_batch_core(doug_ctx* ctx, int nrhs, double tol, int maxit, int32_
     its[r] = 0;
   }
   int it = 0, nactive = nrhs;
+  // With an even column count the iteration runs fused: the overlap sum
+  // forms rho = r.z, and one SpMV pass forms p_new = z + beta p_old, q and
+  // p.q, reading p from one buffer and writing the other (parity par of the
+  // iteration).  Iterates are the same bits as the unfused sequence below;
+  // only the summation order of p.q differs.
+  const bool fused = nrp % 2 == 0;
+  double* const P[2] = {ctx->d_Pm, ctx->d_Pm2};
   // iteration body; from the second iteration on it is replayed from a
-  // captured HIP graph (one per padded column count)
-  auto enqueue_iteration = [&](bool first) {
-    enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm);
-    launch_dot_multi(ctx->d_Rm, ctx->d_Zm, n, nrp, ctx->d_partialm, ctx->d_counterm, 1, ctx->d_scalm, st);
-    launch_p_update_multi(n, nrp, ctx->d_Zm, ctx->d_Pm, ctx->d_scalm, first ? 1 : 0, st);
-    launch_spmv_multi(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Pm, nrp,
-                      ctx->d_Qm, nrp, nrp, st);
-    launch_dot_multi(ctx->d_Pm, ctx->d_Qm, n, nrp, ctx->d_partialm, ctx->d_counterm, 2, ctx->d_scalm, st);
-    launch_xr_update_multi(n, nrp, ctx->d_Xm, ctx->d_Rm, ctx->d_Pm, ctx->d_Qm, ctx->d_partialm,
-                           ctx->d_counterm, ctx->d_scalm, st);
+  // captured HIP graph (one per padded column count and p parity)
+  auto enqueue_iteration = [&](bool first, int par) {
+    double* p = P[par ^ 1];
+    if (fused) {
+      enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm, true);
+      launch_spmv_p_multi2(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Zm, P[par], p,
+                           ctx->d_Qm, nrp, first ? 1 : 0, ctx->d_partialm, ctx->d_counterm, ctx->d_scalm, st);
+    } else {
+      p = P[0];
+      enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm);
+      launch_dot_multi(ctx->d_Rm, ctx->d_Zm, n, nrp, ctx->d_partialm, ctx->d_counterm, 1, ctx->d_scalm, st);
+      launch_p_update_multi(n, nrp, ctx->d_Zm, p, ctx->d_scalm, first ? 1 : 0, st);
+      launch_spmv_multi(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, p, nrp, ctx->d_Qm, nrp,
+                        nrp, st);
+      launch_dot_multi(p, ctx->d_Qm, n, nrp, ctx->d_partialm, ctx->d_counterm, 2, ctx->d_scalm, st);
+    }
+    launch_xr_update_multi(n, nrp, ctx->d_Xm, ctx->d_Rm, p, ctx->d_Qm, ctx->d_partialm, ctx->d_counterm,
+                           ctx->d_scalm, st);
     (void)hipMemcpyAsync(hs, ctx->d_scalm, kMaxRhs * 8 * sizeof(double), hipMemcpyDeviceToHost, st);
   };
   while (nactive > 0 && it < maxit) {
     it++;
+    const int par = fused ? (it - 1) & 1 : 0;
     if (it >= 2 && ctx->use_graph) {
-      if (ctx->batch_graph && ctx->batch_graph_nrp != nrp) {
-        (void)hipGraphExecDestroy(ctx->batch_graph);
-        ctx->batch_graph = nullptr;
-      }
-      if (!ctx->batch_graph) {
+      if (ctx->batch_graph_nrp != nrp)
+        for (hipGraphExec_t& g : ctx->batch_graph)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+      ctx->batch_graph_nrp = nrp;
+      if (!ctx->batch_graph[par]) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        enqueue_iteration(false);
+        enqueue_iteration(false, par);
         HIPCHK(hipStreamEndCapture(st, &g));
-        hipError_t ie = hipGraphInstantiate(&ctx->batch_graph, g, nullptr, nullptr, 0);
+        hipError_t ie = hipGraphInstantiate(&ctx->batch_graph[par], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HIPCHK(ie);
-        ctx->batch_graph_nrp = nrp;
       }
-      HIPCHK(hipGraphLaunch(ctx->batch_graph, st));
+      HIPCHK(hipGraphLaunch(ctx->batch_graph[par], st));
     } else {
-      enqueue_iteration(it == 1);
+      enqueue_iteration(it == 1, par);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
@@ -2929,7 +2961,7 @@ int doug_set_option(doug_ctx* ctx, int32_t option, int64_t value) {
       ctx->trsm_kb = (int)value;
       return DOUG_OK;
     case DOUG_OPT_BATCH_COLUMN_GROUPS:
-      if (value != 1 && value != 2) return fail(ctx, DOUG_E_ARG, "column groups must be 1 or 2");
+      if (value < 1 || value > 3) return fail(ctx, DOUG_E_ARG, "column groups must be 1, 2 or 3");
       drop_iteration_graph(ctx);
       ctx->trsm_cg = (int)value;
       return DOUG_OK;
@@ -3085,8 +3117,8 @@ int doug_autotune(doug_ctx* ctx, int32_t reps, doug_tuning* out) {
   int kb = ctx->trsm_kb, cg = ctx->trsm_cg;
   if (ctx->batch_ready && ctx->p.fine_method == 1 && ctx->nsub > 0) {
     const int rt = std::max(2, reps / 8);
-    double best_ms = 0.0, ms[2][2];
-    for (int g = 1; g <= 2; g++)
+    double best_ms = 0.0, ms[3][2];
+    for (int g = 1; g <= 3; g++)
       for (int q = 1; q <= 2; q++) {
         ctx->trsm_kb = q;
         ctx->trsm_cg = g;

@@ -1197,19 +1197,19 @@ struct Vec16<double> {
   static constexpr int n = 2;
 };
 
-template <typename T>
+template <typename T, int NT = 256>
 struct TileRegs {
-  static constexpr int V = Vec16<T>::n, N = kTileElems / (256 * V);
+  static constexpr int V = Vec16<T>::n, N = kTileElems / (NT * V);
   typename Vec16<T>::type r[N];
   __device__ __forceinline__ void load(const T* __restrict__ g) {
 #pragma unroll
     for (int q = 0; q < N; q++)
-      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + 256 * q];
+      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + NT * q];
   }
   __device__ __forceinline__ void store(T* __restrict__ S) const {
 #pragma unroll
     for (int q = 0; q < N; q++) {
-      const int e = (threadIdx.x + 256 * q) * V, row = e >> 6, col = e & 63;
+      const int e = (threadIdx.x + NT * q) * V, row = e >> 6, col = e & 63;
       const T* v = reinterpret_cast<const T*>(&r[q]);
 #pragma unroll
       for (int c = 0; c < V; c++) S[row * LD + col + c] = v[c];
@@ -1217,19 +1217,20 @@ struct TileRegs {
   }
 };
 
-template <typename T, int NR>
+template <typename T, int NR, int NT = 256>
 struct PanelRegs {
-  static constexpr int V = Vec16<T>::n, N = kTile * NR / (256 * V);
+  static constexpr int V = Vec16<T>::n, N = kTile * NR / (NT * V);
+  static_assert(N >= 1, "panel smaller than one load per thread");
   typename Vec16<T>::type r[N];
   __device__ __forceinline__ void load(const T* __restrict__ g) {
 #pragma unroll
     for (int q = 0; q < N; q++)
-      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + 256 * q];
+      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + NT * q];
   }
   __device__ __forceinline__ void store(T* __restrict__ S) const {
 #pragma unroll
     for (int q = 0; q < N; q++) {
-      const int e = (threadIdx.x + 256 * q) * V, row = e / NR, col = e % NR;
+      const int e = (threadIdx.x + NT * q) * V, row = e / NR, col = e % NR;
       const T* v = reinterpret_cast<const T*>(&r[q]);
 #pragma unroll
       for (int c = 0; c < V; c++) S[row * (NR + 1) + col + c] = v[c];
@@ -1237,16 +1238,17 @@ struct PanelRegs {
   }
 };
 
-// acc[c] += A(16w.., :) * B(:, 16c..) from LDS; TR: use A^T
-template <typename T, int NR, bool TR>
-__device__ __forceinline__ void panel_mma(typename Mma<T>::v4 (&acc)[NR / 16], const T* As,
-                                          const T* Bs, int w, int li, int lk) {
+// acc[c] += A(16w.., :) * B(:, cb + 16c..) from LDS, for the CPW columns
+// from cb of an NR-column panel; TR: use A^T
+template <typename T, int NR, bool TR, int CPW = NR>
+__device__ __forceinline__ void panel_mma(typename Mma<T>::v4 (&acc)[CPW / 16], const T* As,
+                                          const T* Bs, int w, int li, int lk, int cb = 0) {
 #pragma unroll 4
   for (int ks = 0; ks < 16; ks++) {
     const int kk = 4 * ks + lk;
     const T a = TR ? As[kk * LD + 16 * w + li] : As[(16 * w + li) * LD + kk];
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + c * 16 + li], acc[c]);
+    for (int c = 0; c < CPW / 16; c++) acc[c] = Mma<T>::op(a, Bs[kk * (NR + 1) + cb + c * 16 + li], acc[c]);
   }
 }
 
@@ -1340,6 +1342,26 @@ __global__ void k_panel_sum(const double* __restrict__ part, int nt, int nch, in
   out[(int64_t)p * ldo + c] = s;
 }
 
+// The tiles a tile row's solve starts with: its diagonal tile and the
+// off-diagonal tiles of its first step (forward: row i, tiles (i, fc_i..);
+// backward, TR: column j, tiles (j+1.., j)).
+template <typename T, int KB, bool TR, int NT>
+__device__ __forceinline__ void load_row_tiles(const T* Lb, const BlockDesc& b, int i, TileRegs<T, NT>& rd,
+                                               TileRegs<T, NT> (&ra)[KB]) {
+  rd.load(Lb + toff(b, i, i));
+  if (!TR) {
+    const int fc = b.tr[i].fc;
+#pragma unroll
+    for (int u = 0; u < KB; u++)
+      if (fc + u < i) ra[u].load(Lb + toff(b, i, fc + u));
+  } else {
+    const int lr = b.tr[i].lr;
+#pragma unroll
+    for (int u = 0; u < KB; u++)
+      if (i + 1 + u <= lr) ra[u].load(Lb + toff(b, i + 1 + u, i));
+  }
+}
+
 // Envelope forward/backward substitution for NR columns, one workgroup per
 // block.  Wave w owns rows 16w..16w+15 of each 64 x NR panel; panels of
 // finished tile rows live in U (global, re-read through L2); the workgroup's
@@ -1347,9 +1369,11 @@ __global__ void k_panel_sum(const double* __restrict__ part, int nt, int nch, in
 // stages KB off-diagonal tiles with their panels (a tile row's diagonal
 // tile with its first step); the next step's loads are issued into
 // registers before this step's products, so that their latency overlaps
-// the matrix-core work.
-template <typename T, int NR, int KB, int CG>
-__global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
+// the matrix-core work.  The last step of a tile row loads the next row's
+// first tiles in the same way, and a row's right-hand sides (forward) or
+// Y_j (backward) are read at its start, ahead of the steps.
+template <typename T, int NR, int KB, int CG, int WG = 4>
+__global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
                                                     const BlockDesc* __restrict__ d, int nblocks,
                                                     const int32_t* __restrict__ inds,
                                                     const double* __restrict__ rhs, int ldr,
@@ -1370,30 +1394,47 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
     if (sub >= nblocks) return;
   }
   const BlockDesc b = d[sub];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6, li = lane & 15, lk = lane >> 4;
+  // WG = 8: wavefronts w and w + 4 take the same rows, columns 0..NR/2-1 and
+  // NR/2..NR-1, so that one staged tile serves the whole panel
+  constexpr int NT = 64 * WG, CPW = NR * 4 / WG;
+  static_assert(WG == 4 || (WG == 8 && CG == 1 && NR == 32), "eight waves take two 16-column halves");
+  const int t = threadIdx.x, lane = t & 63, w = (t >> 6) & 3, li = lane & 15, lk = lane >> 4;
+  const int cb = (t >> 8) * CPW;  // first column of this wavefront
   const T* Lb = L + b.base;
   T* Ub = U + b.woff * (NR * CG) + (int64_t)grp * b.nt * kTile * NR;
   const int c0 = grp * NR;  // first right-hand side of this group
   rhs += c0;
   out += c0;
   nrhs -= c0;
-  V acc[NR / 16];
+  V acc[CPW / 16];
+  T rv[CPW / 16][4];  // right-hand side (forward) or Y_j (backward) of the row
+  // tiles of the next tile row (its diagonal tile and first step), loaded
+  // while the current row's last step runs
+  TileRegs<T, NT> rdn, ran[KB];
+  load_row_tiles<T, KB, false, NT>(Lb, b, 0, rdn, ran);
   // forward: Y_i = inv(L_ii) (R_i - sum_{k=fc_i}^{i-1} L_ik Y_k)
   for (int i = 0; i < b.nt; i++) {
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+    for (int c = 0; c < CPW / 16; c++) acc[c] = V{0, 0, 0, 0};
     const int fc = b.tr[i].fc;
     {
       // steps of KB off-diagonal tiles; the next step's tiles and panels are
       // loaded into registers while this step's products run
-      TileRegs<T> ra[KB], rd;
-      PanelRegs<T, NR> rb[KB];
-      rd.load(Lb + toff(b, i, i));
+      TileRegs<T, NT> ra[KB], rd = rdn;
+      PanelRegs<T, NR, NT> rb[KB];
 #pragma unroll
-      for (int u = 0; u < KB; u++)
-        if (fc + u < i) {
-          ra[u].load(Lb + toff(b, i, fc + u));
-          rb[u].load(Ub + (int64_t)(fc + u) * kTile * NR);
+      for (int u = 0; u < KB; u++) {
+        ra[u] = ran[u];
+        if (fc + u < i) rb[u].load(Ub + (int64_t)(fc + u) * kTile * NR);
+      }
+      // this row's right-hand sides, gathered while the steps run
+#pragma unroll
+      for (int c = 0; c < CPW / 16; c++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = 16 * w + Mma<T>::row(lane, r), col = cb + c * 16 + li;
+          const int p = i * kTile + row;
+          rv[c][r] = p < b.s && col < nrhs ? (T)rhs[(int64_t)inds[b.vec_off + p] * ldr + col] : (T)0;
         }
       rd.store(Ds);
       for (int k = fc;; k += KB) {
@@ -1405,55 +1446,61 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
           }
         __syncthreads();
         const int kn = k + KB;
+        if (kn < i) {
+#pragma unroll
+          for (int u = 0; u < KB; u++)
+            if (kn + u < i) {
+              ra[u].load(Lb + toff(b, i, kn + u));
+              rb[u].load(Ub + (int64_t)(kn + u) * kTile * NR);
+            }
+        } else if (i + 1 < b.nt) {
+          load_row_tiles<T, KB, false, NT>(Lb, b, i + 1, rdn, ran);
+        }
 #pragma unroll
         for (int u = 0; u < KB; u++)
-          if (kn + u < i) {
-            ra[u].load(Lb + toff(b, i, kn + u));
-            rb[u].load(Ub + (int64_t)(kn + u) * kTile * NR);
-          }
-#pragma unroll
-        for (int u = 0; u < KB; u++)
-          if (k + u < i) panel_mma<T, NR, false>(acc, As[u], Bs[u], w, li, lk);
+          if (k + u < i) panel_mma<T, NR, false, CPW>(acc, As[u], Bs[u], w, li, lk, cb);
         __syncthreads();
         if (kn >= i) break;
       }
     }
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++)
+    for (int c = 0; c < CPW / 16; c++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
-        const int p = i * kTile + row;
-        T v = (T)0;
-        if (p < b.s && col < nrhs) v = (T)rhs[(int64_t)inds[b.vec_off + p] * ldr + col];
-        Bs[0][row * (NR + 1) + col] = v - acc[c][r];
+        const int row = 16 * w + Mma<T>::row(lane, r), col = cb + c * 16 + li;
+        Bs[0][row * (NR + 1) + col] = rv[c][r] - acc[c][r];
       }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-    panel_mma<T, NR, false>(acc, Ds, Bs[0], w, li, lk);
+    for (int c = 0; c < CPW / 16; c++) acc[c] = V{0, 0, 0, 0};
+    panel_mma<T, NR, false, CPW>(acc, Ds, Bs[0], w, li, lk, cb);
     T* Ui = Ub + (int64_t)i * kTile * NR;
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++)
+    for (int c = 0; c < CPW / 16; c++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) Ui[(16 * w + Mma<T>::row(lane, r)) * NR + c * 16 + li] = acc[c][r];
+      for (int r = 0; r < 4; r++) Ui[(16 * w + Mma<T>::row(lane, r)) * NR + cb + c * 16 + li] = acc[c][r];
     __syncthreads();
   }
   // backward: X_j = inv(L_jj)^T (Y_j - sum_{i=j+1}^{lr_j} L_ij^T X_i), X_j over Y_j
+  load_row_tiles<T, KB, true, NT>(Lb, b, b.nt - 1, rdn, ran);
   for (int j = b.nt - 1; j >= 0; j--) {
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
+    for (int c = 0; c < CPW / 16; c++) acc[c] = V{0, 0, 0, 0};
     const int lr = b.tr[j].lr;
+    T* Uj = Ub + (int64_t)j * kTile * NR;
     {
-      TileRegs<T> ra[KB], rd;
-      PanelRegs<T, NR> rb[KB];
-      rd.load(Lb + toff(b, j, j));
+      TileRegs<T, NT> ra[KB], rd = rdn;
+      PanelRegs<T, NR, NT> rb[KB];
 #pragma unroll
-      for (int u = 0; u < KB; u++)
-        if (j + 1 + u <= lr) {
-          ra[u].load(Lb + toff(b, j + 1 + u, j));
-          rb[u].load(Ub + (int64_t)(j + 1 + u) * kTile * NR);
-        }
+      for (int u = 0; u < KB; u++) {
+        ra[u] = ran[u];
+        if (j + 1 + u <= lr) rb[u].load(Ub + (int64_t)(j + 1 + u) * kTile * NR);
+      }
+      // Y_j, read while the steps run (the steps read X_i, i > j, only)
+#pragma unroll
+      for (int c = 0; c < CPW / 16; c++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) rv[c][r] = Uj[(16 * w + Mma<T>::row(lane, r)) * NR + cb + c * 16 + li];
       rd.store(Ds);
       for (int i = j + 1;; i += KB) {
 #pragma unroll
@@ -1464,36 +1511,39 @@ __global__ __launch_bounds__(256) void k_trsm_multi(const T* __restrict__ L,
           }
         __syncthreads();
         const int in = i + KB;
+        if (in <= lr) {
+#pragma unroll
+          for (int u = 0; u < KB; u++)
+            if (in + u <= lr) {
+              ra[u].load(Lb + toff(b, in + u, j));
+              rb[u].load(Ub + (int64_t)(in + u) * kTile * NR);
+            }
+        } else if (j > 0) {
+          load_row_tiles<T, KB, true, NT>(Lb, b, j - 1, rdn, ran);
+        }
 #pragma unroll
         for (int u = 0; u < KB; u++)
-          if (in + u <= lr) {
-            ra[u].load(Lb + toff(b, in + u, j));
-            rb[u].load(Ub + (int64_t)(in + u) * kTile * NR);
-          }
-#pragma unroll
-        for (int u = 0; u < KB; u++)
-          if (i + u <= lr) panel_mma<T, NR, true>(acc, As[u], Bs[u], w, li, lk);
+          if (i + u <= lr) panel_mma<T, NR, true, CPW>(acc, As[u], Bs[u], w, li, lk, cb);
         __syncthreads();
         if (in > lr) break;
       }
     }
-    T* Uj = Ub + (int64_t)j * kTile * NR;
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++)
+    for (int c = 0; c < CPW / 16; c++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
-        Bs[0][row * (NR + 1) + col] = Uj[row * NR + col] - acc[c][r];
+        const int row = 16 * w + Mma<T>::row(lane, r), col = cb + c * 16 + li;
+        Bs[0][row * (NR + 1) + col] = rv[c][r] - acc[c][r];
       }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++) acc[c] = V{0, 0, 0, 0};
-    panel_mma<T, NR, true>(acc, Ds, Bs[0], w, li, lk);  // inv(L_jj)^T
+    for (int c = 0; c < CPW / 16; c++) acc[c] = V{0, 0, 0, 0};
+    panel_mma<T, NR, true, CPW>(acc, Ds, Bs[0], w, li, lk, cb);  // inv(L_jj)^T
 #pragma unroll
-    for (int c = 0; c < NR / 16; c++)
+    for (int c = 0; c < CPW / 16; c++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = 16 * w + Mma<T>::row(lane, r), col = c * 16 + li;
+        const int row = 16 * w + Mma<T>::row(lane, r), col = cb + c * 16 + li;
         Uj[row * NR + col] = acc[c][r];
         const int p = j * kTile + row;
         if (p < b.s && col < nrhs) out[(b.vec_off + p) * ldo + col] = (double)acc[c][r];
@@ -1661,6 +1711,22 @@ __global__ void k_overlap_sum_multi2(int n, int nrhs, const int32_t* __restrict_
 constexpr int kMultiMax = kMaxRhs;
 enum { SM_ACTIVE = 6 };
 
+// mode 1: v = r.z (beta, rho); mode 2: v = p.q (alpha; 0 for a frozen
+// column); mode 3: v = r.r
+__device__ __forceinline__ void apply_colsum(double* sc, double v, int mode) {
+  const bool active = sc[SM_ACTIVE] != 0.0;
+  if (mode == 1) {
+    sc[S_BETA] = v / sc[S_RHO];
+    sc[S_RHO_PREV] = sc[S_RHO];
+    sc[S_RHO] = v;
+  } else if (mode == 2) {
+    sc[S_PQ] = v;
+    sc[S_ALPHA] = active ? sc[S_RHO] / v : 0.0;
+  } else {
+    sc[S_RES] = v;
+  }
+}
+
 __device__ void finish_multi(double* partial, unsigned* counter, int nrhs, int mode, double* scal,
                              double* sh) {
   __shared__ bool last;
@@ -1683,22 +1749,36 @@ __device__ void finish_multi(double* partial, unsigned* counter, int nrhs, int m
     if (sub < o) sh[t] = sh[t] + sh[t + o];
     __syncthreads();
   }
-  if (sub == 0) {
-    v = sh[t];
-    double* sc = scal + r * 8;
-    const bool active = sc[SM_ACTIVE] != 0.0;
-    if (mode == 1) {
-      sc[S_BETA] = v / sc[S_RHO];
-      sc[S_RHO_PREV] = sc[S_RHO];
-      sc[S_RHO] = v;
-    } else if (mode == 2) {
-      sc[S_PQ] = v;
-      sc[S_ALPHA] = active ? sc[S_RHO] / v : 0.0;
-    } else {
-      sc[S_RES] = v;
-    }
-  }
+  if (sub == 0) apply_colsum(scal + r * 8, sh[t], mode);
   if (threadIdx.x == 0) *counter = 0u;
+}
+
+// Column sums of per-block partials (partial[column * nb + b], nb blocks)
+// by a kernel of their own, one workgroup per column: thread t adds blocks
+// t, t + 256, ... in order, then a fixed tree.  Used after the
+// wide column-pair kernels, whose nb is too large for finish_multi's single
+// workgroup.
+__global__ __launch_bounds__(256) void k_finish_cols(const double* __restrict__ partial, int nb, int mode,
+                                                     double* scal) {
+  __shared__ double sh[256];
+  const int r = blockIdx.x, t = threadIdx.x;
+  double v = 0.0;
+  int b = t;
+  for (; b + 3 * 256 < nb; b += 4 * 256) {
+    double a[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) a[u] = partial[(int64_t)r * nb + b + u * 256];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v = v + a[u];
+  }
+  for (; b < nb; b += 256) v = v + partial[(int64_t)r * nb + b];
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) sh[t] = sh[t] + sh[t + o];
+    __syncthreads();
+  }
+  if (t == 0) apply_colsum(scal + r * 8, sh[0], mode);
 }
 
 __device__ __forceinline__ void block_colsums(double v, int nrhs, double* sh, double* partial) {
@@ -1738,15 +1818,14 @@ __device__ __forceinline__ void block_colsums2(double v0, double v1, int nrhs, d
     const double* h = sh + (t & 1) * 256;
     double s = 0.0;
     for (int q = t >> 1; q < 256; q += nr2) s += h[q];
-    partial[blockIdx.x * kMultiMax + t] = s;
+    partial[(int64_t)t * gridDim.x + blockIdx.x] = s;  // column-major, finished by k_finish_cols
   }
   __syncthreads();
 }
 
 __global__ __launch_bounds__(256) void k_dot_multi2(const double2* __restrict__ a,
                                                     const double2* __restrict__ b, int64_t total2,
-                                                    int nrhs, double* partial, unsigned* counter,
-                                                    int mode, double* scal) {
+                                                    int nrhs, double* partial) {
   __shared__ double sh[512];
   const int nr2 = nrhs >> 1;
   const int64_t stride = ((int64_t)gridDim.x * 256 / nr2) * nr2;
@@ -1759,7 +1838,6 @@ __global__ __launch_bounds__(256) void k_dot_multi2(const double2* __restrict__ 
       v1 = v1 + x.y * y.y;
     }
   block_colsums2(v0, v1, nrhs, sh, partial);
-  finish_multi(partial, counter, nrhs, mode, scal, sh);
 }
 
 __global__ void k_p_update_multi(int64_t total, int nrhs, const double* __restrict__ z,
@@ -1809,8 +1887,7 @@ __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nr
                                                           double2* __restrict__ r,
                                                           const double2* __restrict__ p,
                                                           const double2* __restrict__ q,
-                                                          double* partial, unsigned* counter,
-                                                          double* scal) {
+                                                          double* partial, const double* scal) {
   __shared__ double sh[512];
   const int nr2 = nrhs >> 1;
   const int64_t stride = ((int64_t)gridDim.x * 256 / nr2) * nr2;
@@ -1824,7 +1901,42 @@ __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nr
     const double a0 = s0[SM_ACTIVE] != 0.0 ? s0[S_ALPHA] : 0.0;
     const double a1 = s1[SM_ACTIVE] != 0.0 ? s1[S_ALPHA] : 0.0;
     const bool any = s0[SM_ACTIVE] != 0.0 || s1[SM_ACTIVE] != 0.0;
-    for (int64_t e = start; e < total2; e += stride) {
+    const bool f0 = s0[SM_ACTIVE] != 0.0, f1 = s1[SM_ACTIVE] != 0.0;
+    int64_t e = start;
+    // four items per round with every load issued first; the same element
+    // updates and the same order of the sums as one item at a time
+    for (; e + 3 * stride < total2; e += 4 * stride) {
+      double2 ri[4], pe[4], qe[4], xe[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) ri[u] = r[e + u * stride];
+      if (any) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          pe[u] = p[e + u * stride];
+          qe[u] = q[e + u * stride];
+          xe[u] = x[e + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (f0) {
+            xe[u].x = xe[u].x + a0 * pe[u].x;
+            ri[u].x = ri[u].x - a0 * qe[u].x;
+          }
+          if (f1) {
+            xe[u].y = xe[u].y + a1 * pe[u].y;
+            ri[u].y = ri[u].y - a1 * qe[u].y;
+          }
+          x[e + u * stride] = xe[u];
+          r[e + u * stride] = ri[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        v0 = v0 + ri[u].x * ri[u].x;
+        v1 = v1 + ri[u].y * ri[u].y;
+      }
+    }
+    for (; e < total2; e += stride) {
       double2 ri = r[e];
       if (any) {
         const double2 pe = p[e], qe = q[e];
@@ -1845,7 +1957,183 @@ __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nr
     }
   }
   block_colsums2(v0, v1, nrhs, sh, partial);
-  finish_multi(partial, counter, nrhs, 3, scal, sh);
+}
+
+// Fused forms of the batched iteration (column pairs, nrhs even).
+//
+// z = overlap sum of the subdomain solutions (+ coarse correction), and
+// rho = r.z per column: the elements are visited in the partition of
+// k_dot_multi2 (thread t of the grid takes e = t, t + stride, ...) and the
+// partials finished by k_finish_cols, so z and rho are the same bits as
+// k_overlap_sum_multi2 followed by k_dot_multi2 on the same grid.
+__global__ __launch_bounds__(256) void k_overlap_sum_dot_multi2(
+    int n, int nrhs, const int32_t* __restrict__ occ_ptr, const int32_t* __restrict__ occ_pos,
+    const double* __restrict__ sol, const int32_t* __restrict__ rt_ptr,
+    const int32_t* __restrict__ rt_col, const double* __restrict__ rt_val,
+    const double* __restrict__ csol, int ldc, double* __restrict__ z, const double* __restrict__ r,
+    double* partial) {
+  __shared__ double sh[512];
+  const int np = nrhs >> 1;
+  const int64_t total2 = (int64_t)n * np;
+  const int64_t stride = ((int64_t)gridDim.x * 256 / np) * np;
+  const int64_t start = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  double v0 = 0.0, v1 = 0.0;
+  if (start < stride) {
+    const int c = 2 * (int)(start % np), di = (int)(stride / np);
+    int i = (int)(start / np);
+    // kU nodes per round in lockstep, so that the dependent loads (list
+    // bounds, then positions, then values) of the kU nodes overlap; each
+    // node's sums keep their order and the dot keeps the order of the items
+    constexpr int kU = 2;
+    for (int64_t e = start; e < total2; e += kU * stride, i += kU * di) {
+      int p0[kU], p1[kU], q0[kU], q1[kU];
+      bool in[kU];
+      double2 acc[kU], tmp[kU], re[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        in[u] = e + u * stride < total2;
+        const int iu = in[u] ? i + u * di : i;
+        p0[u] = occ_ptr[iu];
+        p1[u] = in[u] ? occ_ptr[iu + 1] : p0[u];
+        q0[u] = rt_ptr ? rt_ptr[iu] : 0;
+        q1[u] = rt_ptr && in[u] ? rt_ptr[iu + 1] : q0[u];
+        re[u] = in[u] ? reinterpret_cast<const double2*>(r)[e + u * stride] : make_double2(0.0, 0.0);
+        acc[u] = make_double2(0.0, 0.0);
+        tmp[u] = make_double2(0.0, 0.0);
+      }
+      for (int j = 0;; j++) {
+        bool g[kU], any = false;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+          g[u] = p0[u] + j < p1[u];
+          any = any || g[u];
+        }
+        if (!any) break;
+        int pos[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          if (g[u]) pos[u] = occ_pos[p0[u] + j];
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          if (g[u]) {
+            const double2 sv = *reinterpret_cast<const double2*>(sol + (int64_t)pos[u] * nrhs + c);
+            acc[u].x = acc[u].x + sv.x;
+            acc[u].y = acc[u].y + sv.y;
+          }
+      }
+      for (int j = 0;; j++) {
+        bool g[kU], any = false;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+          g[u] = q0[u] + j < q1[u];
+          any = any || g[u];
+        }
+        if (!any) break;
+        int cc[kU];
+        double w[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          if (g[u]) {
+            cc[u] = rt_col[q0[u] + j];
+            w[u] = rt_val[q0[u] + j];
+          }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          if (g[u]) {
+            const double2 cs = *reinterpret_cast<const double2*>(csol + (int64_t)cc[u] * ldc + c);
+            tmp[u].x = tmp[u].x + w[u] * cs.x;
+            tmp[u].y = tmp[u].y + w[u] * cs.y;
+          }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++)
+        if (in[u]) {
+          if (rt_ptr) {
+            acc[u].x = acc[u].x + tmp[u].x;
+            acc[u].y = acc[u].y + tmp[u].y;
+          }
+          reinterpret_cast<double2*>(z)[e + u * stride] = acc[u];
+          v0 = v0 + re[u].x * acc[u].x;
+          v1 = v1 + re[u].y * acc[u].y;
+        }
+    }
+  }
+  block_colsums2(v0, v1, nrhs, sh, partial);
+}
+
+// p_new = z + beta p_old (p_new = z on the first iteration); a frozen column
+// keeps p_old
+__device__ __forceinline__ double2 p_next(double2 zv, double2 pv, double b0, double b1, bool a0,
+                                          bool a1, bool first) {
+  double2 o;
+  o.x = a0 ? (first ? zv.x : zv.x + b0 * pv.x) : pv.x;
+  o.y = a1 ? (first ? zv.y : zv.y + b1 * pv.y) : pv.y;
+  return o;
+}
+
+// q = A p_new with p_new formed on the fly from z and p_old at every
+// gathered node (the same expression as k_p_update_multi, so q is the same
+// bits as the unfused sequence), p_new written for the block's own rows,
+// and p.q per column from fixed-order block partials (one per row block).  p_old and p_new are
+// separate buffers: rows of other blocks still read p_old.  Entries of a
+// row are taken eight at a time, loads first, then added in entry order;
+// the np threads of a row read its entries from global memory together
+// (one request per wavefront), which leaves LDS free for occupancy.
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv_p_multi2(
+    const int32_t* __restrict__ rowblk, const int32_t* __restrict__ ptr,
+    const int32_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ z,
+    const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q, int nrhs,
+    int first, double* partial, const double* scal) {
+  __shared__ double sh[512];
+  const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int t = threadIdx.x, np = nrhs >> 1;
+  // np divides the block size, so the column pair of a thread is fixed
+  const int pc = t % np;
+  const double* s0 = scal + 2 * pc * 8;
+  const double* s1 = s0 + 8;
+  const bool a0 = s0[SM_ACTIVE] != 0.0, a1 = s1[SM_ACTIVE] != 0.0, fst = first != 0;
+  const double b0 = s0[S_BETA], b1 = s1[S_BETA];
+  const double2* z2 = reinterpret_cast<const double2*>(z);
+  const double2* po2 = reinterpret_cast<const double2*>(pold);
+  const int work = (r1 - r0) * np;
+  double v0 = 0.0, v1 = 0.0;
+  for (int idx = t; idx < work; idx += kSpmvThreads) {
+    const int row = r0 + idx / np;
+    const int eb = ptr[row], ee = ptr[row + 1];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int e = eb; e < ee; e += 8) {
+      const int m = min(8, ee - e);
+      int k[8];
+      double a[8];
+      double2 zv[8], pv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < m) {
+          k[u] = col[e + u];
+          a[u] = val[e + u];
+        }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < m) {
+          zv[u] = z2[(int64_t)k[u] * np + pc];
+          pv[u] = po2[(int64_t)k[u] * np + pc];
+        }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (u < m) {
+          const double2 pk = p_next(zv[u], pv[u], b0, b1, a0, a1, fst);
+          acc.x = acc.x + a[u] * pk.x;
+          acc.y = acc.y + a[u] * pk.y;
+        }
+    }
+    const int64_t o = (int64_t)row * np + pc;
+    const double2 pr = p_next(z2[o], po2[o], b0, b1, a0, a1, fst);
+    reinterpret_cast<double2*>(pnew)[o] = pr;
+    reinterpret_cast<double2*>(q)[o] = acc;
+    v0 = v0 + pr.x * acc.x;
+    v1 = v1 + pr.y * acc.y;
+  }
+  block_colsums2(v0, v1, nrhs, sh, partial);
 }
 
 // d loss / d a_ij = -lambda_i x_j for every stored entry, written to the
@@ -2259,6 +2547,15 @@ void launch_trsm_multi(const T* L, const BlockDesc* d, int nblocks, const int32_
                        int kb, int cg, hipStream_t st) {
   if (nblocks <= 0) return;
   const dim3 b(256);
+  if (NR == 32 && cg == 3) {  // one eight-wave workgroup, two halves of 16 columns
+    if (kb == 1)
+      hipLaunchKernelGGL((k_trsm_multi<T, 32, 1, 1, 8>), dim3(nblocks), dim3(512), 0, st, L, d, nblocks, inds, rhs,
+                         ldr, U, out, ldo, nrhs);
+    else
+      hipLaunchKernelGGL((k_trsm_multi<T, 32, 2, 1, 8>), dim3(nblocks), dim3(512), 0, st, L, d, nblocks, inds, rhs,
+                         ldr, U, out, ldo, nrhs);
+    return;
+  }
   if (NR == 32 && cg == 2) {  // two groups of 16 columns
     const dim3 g(cdiv(nblocks, 8) * 16);
     if (kb == 1)
@@ -2345,11 +2642,12 @@ void launch_overlap_sum_multi(int n, int nrhs, const int32_t* occ_ptr, const int
 
 void launch_dot_multi(const double* a, const double* b, int n, int nrhs, double* partial,
                       unsigned* counter, int mode, double* scal, hipStream_t st) {
-  if (nrhs % 2 == 0)
-    hipLaunchKernelGGL(k_dot_multi2, dim3(kRedBlocks), dim3(256), 0, st,
+  if (nrhs % 2 == 0) {
+    hipLaunchKernelGGL(k_dot_multi2, dim3(kRedBlocksWide), dim3(256), 0, st,
                        reinterpret_cast<const double2*>(a), reinterpret_cast<const double2*>(b),
-                       (int64_t)n * nrhs / 2, nrhs, partial, counter, mode, scal);
-  else
+                       (int64_t)n * nrhs / 2, nrhs, partial);
+    hipLaunchKernelGGL(k_finish_cols, dim3(nrhs), dim3(256), 0, st, partial, kRedBlocksWide, mode, scal);
+  } else
     hipLaunchKernelGGL(k_dot_multi, dim3(kRedBlocks), dim3(256), 0, st, a, b, (int64_t)n * nrhs,
                        nrhs, partial, counter, mode, scal);
 }
@@ -2364,14 +2662,34 @@ void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const do
 void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double* p,
                             const double* q, double* partial, unsigned* counter, double* scal,
                             hipStream_t st) {
-  if (nrhs % 2 == 0)
-    hipLaunchKernelGGL(k_xr_update_multi2, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs / 2,
+  if (nrhs % 2 == 0) {
+    hipLaunchKernelGGL(k_xr_update_multi2, dim3(kRedBlocksWide), dim3(256), 0, st, (int64_t)n * nrhs / 2,
                        nrhs, reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
                        reinterpret_cast<const double2*>(p), reinterpret_cast<const double2*>(q), partial,
-                       counter, scal);
-  else
+                       scal);
+    hipLaunchKernelGGL(k_finish_cols, dim3(nrhs), dim3(256), 0, st, partial, kRedBlocksWide, 3, scal);
+  } else
     hipLaunchKernelGGL(k_xr_update_multi, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs, nrhs,
                        x, r, p, q, partial, counter, scal);
+}
+
+void launch_overlap_sum_dot_multi2(int n, int nrhs, const int32_t* occ_ptr, const int32_t* occ_pos,
+                                   const double* sol, const int32_t* rt_ptr, const int32_t* rt_col,
+                                   const double* rt_val, const double* csol, int ldc, double* z,
+                                   const double* r, double* partial, unsigned* counter, double* scal,
+                                   hipStream_t st) {
+  hipLaunchKernelGGL(k_overlap_sum_dot_multi2, dim3(kRedBlocksWide), dim3(256), 0, st, n, nrhs, occ_ptr,
+                     occ_pos, sol, rt_ptr, rt_col, rt_val, csol, ldc, z, r, partial);
+  hipLaunchKernelGGL(k_finish_cols, dim3(nrhs), dim3(256), 0, st, partial, kRedBlocksWide, 1, scal);
+}
+
+void launch_spmv_p_multi2(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
+                          const double* val, const double* z, const double* pold, double* pnew, double* q,
+                          int nrhs, int first, double* partial, unsigned* counter, double* scal,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_spmv_p_multi2, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, z, pold,
+                     pnew, q, nrhs, first, partial, scal);
+  hipLaunchKernelGGL(k_finish_cols, dim3(nrhs), dim3(256), 0, st, partial, nrb, 2, scal);
 }
 
 void launch_value_grad(int n, const int32_t* ptr, const int32_t* col, const int32_t* src,

@@ -11,6 +11,7 @@ constexpr int kSpmvThreads = 256;      // CSR-stream block
 constexpr int kSpmvChunk = 2048;       // products staged in LDS per block
 constexpr int kMaxTrsvTiles = 256;     // subdomain size limit of the one-block solve (16384)
 constexpr int kRedBlocks = 1024;       // fixed partial count of the grid reductions
+constexpr int kRedBlocksWide = 8192;   // grid of the column-pair reductions (partials per column)
 constexpr int kVecChunk = 1024;        // entries per workgroup of the PCG vector kernels
 // partial sums the single-vector reductions of an n-vector need
 constexpr int64_t vec_partials(int64_t n) { return (n + 255) / 256 + kRedBlocks; }
@@ -246,6 +247,18 @@ void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const do
 void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double* p,
                             const double* q, double* partial, unsigned* counter, double* scal,
                             hipStream_t st);
+// fused forms (nrhs even): overlap sum with rho = r.z (mode 1), and
+// p_new = z + beta p_old, q = A p_new with p.q (mode 2).  Column-pair
+// reductions take max(kRedBlocksWide, nrb) partial slots of kMaxRhs.
+void launch_overlap_sum_dot_multi2(int n, int nrhs, const int32_t* occ_ptr, const int32_t* occ_pos,
+                                   const double* sol, const int32_t* rt_ptr, const int32_t* rt_col,
+                                   const double* rt_val, const double* csol, int ldc, double* z,
+                                   const double* r, double* partial, unsigned* counter, double* scal,
+                                   hipStream_t st);
+void launch_spmv_p_multi2(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
+                          const double* val, const double* z, const double* pold, double* pnew, double* q,
+                          int nrhs, int first, double* partial, unsigned* counter, double* scal,
+                          hipStream_t st);
 
 // ---- coarse space on the device (setup; bit-identical to the host build) ----
 // Rows of R^T = S T^T into slots at ptr[j] (capacity = row length of A),

@@ -307,9 +307,11 @@ int doug_get_tuning(const doug_ctx *ctx, doug_tuning *out);
  * temporaries between set-ups (doug_info.temp_bytes), so that a repeated
  * set-up of the same size neither allocates nor frees them; 0 frees them
  * now and at the end of every set-up.
- * DOUG_OPT_BATCH_COLUMN_GROUPS (1 or 2, default 1): workgroups per subdomain
- * of the 32-column batched envelope solve, 16 columns each with 2 (what
- * doug_autotune sets); the bits are the same either way. */
+ * DOUG_OPT_BATCH_COLUMN_GROUPS (1, 2 or 3, default 1): layout of the
+ * 32-column batched envelope solve: 1, one workgroup of four wavefronts per
+ * subdomain; 2, two such workgroups of 16 columns each; 3, one workgroup of
+ * eight wavefronts whose two halves take 16 columns each of one staged tile
+ * (doug_autotune picks the fastest); the bits are the same in every layout. */
 enum {
   DOUG_OPT_ITERATION_GRAPH = 1,
   DOUG_OPT_SPMV_SHAPE = 2,
