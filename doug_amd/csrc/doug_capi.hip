@@ -27,7 +27,9 @@ using namespace doug;
 
 namespace {
 
-enum { S_RHO = 0, S_RHO_PREV = 1, S_BETA = 2, S_PQ = 3, S_ALPHA = 4, S_RES = 5, S_N = 8 };
+enum { S_RHO = 0, S_RHO_PREV = 1, S_BETA = 2, S_PQ = 3, S_ALPHA = 4, S_RES = 5 };
+// the device's PCG convergence test (as in the kernels), then the slot count
+enum { S_DONE = 8, S_ITER = 9, S_THR = 10, S_INIT = 11, S_MAXIT = 12, S_HCAP = 13, S_N = 16 };
 
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -170,7 +172,9 @@ struct doug_ctx {
   double *d_b = nullptr, *d_scal = nullptr, *d_partial = nullptr, *d_tmp = nullptr;
   unsigned* d_counter = nullptr;
   int* d_flag = nullptr;
-  double* h_scal = nullptr;  // pinned
+  double* h_scal = nullptr;  // pinned, 2 * S_N: the states of two iterations in flight
+  double* d_hist = nullptr;  // PCG (alpha, beta, residual) per iteration, hist_cap entries
+  int64_t hist_cap = 0;
   // several right-hand sides (doug_prepare_batch): W = L^{-1} per subdomain,
   // padded workspaces, node-major multi-vectors of kMaxRhs columns
   bool batch_ready = false;
@@ -291,6 +295,8 @@ static void free_device(doug_ctx* ctx, bool background = false) {
   ctx->d_eptn = nullptr;
   ctx->d_cperm = nullptr;
   ctx->d_cW32 = nullptr;
+  ctx->d_hist = nullptr;
+  ctx->hist_cap = 0;
   join_reaper(ctx);
   if (background && !ctx->allocs.empty()) {
     std::vector<void*> old;
@@ -383,7 +389,7 @@ int doug_ctx_create(int device, doug_ctx** out) {
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream.s, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->h_scal, S_N * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_scal, 2 * S_N * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&ctx->h_scalm, kMaxRhs * 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
     for (hipStream_t st : {ctx->stream.s, ctx->stream2, ctx->stream3})
@@ -2338,7 +2344,8 @@ static PrecWork ctx_work(doug_ctx* ctx) {
   return PrecWork{ctx->d_subsol, ctx->d_crhs, ctx->d_cu, ctx->d_csol, ctx->d_cpart};
 }
 
-static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r, double* subsol) {
+static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r, double* subsol,
+                                     const double* skip = nullptr) {
   if (ctx->p.fine_method == 2) {  // FinePreconditioner_sgs_Apply
     launch_gather_positions((int64_t)ctx->sinds.size(), ctx->d_inds, r, subsol, ctx->stream);
     launch_sgs(ctx->d_sgs, ctx->nsub, ctx->sgs_mmax, ctx->d_sgs_levptr, ctx->d_sgs_lrows,
@@ -2348,10 +2355,10 @@ static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r, double* sub
   }
   if (ctx->precision == DOUG_PREC_FP32_FACTORS)
     launch_trsv(ctx->d_tiles32, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, subsol,
-                ctx->stream);
+                ctx->stream, skip);
   else
     launch_trsv(ctx->d_tiles, ctx->d_sub_trsv, ctx->nsub, ctx->maxnt, ctx->d_inds, r, subsol,
-                ctx->stream);
+                ctx->stream, skip);
 }
 
 static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
@@ -2359,9 +2366,10 @@ static void enqueue_subdomain_solves(doug_ctx* ctx, const double* r) {
 }
 
 // preconditioner (pcg.F90:216-270): fine solves, then coarse correction added.
-// With `dot`, PCG's r.z (rho and beta in d_scal) is formed with z.
+// With `dot`, PCG's r.z (rho and beta in d_scal) is formed with z; with
+// `skip`, the subdomain solves do nothing once *skip is set.
 static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWork& w,
-                         bool dot = false) {
+                         bool dot = false, const double* skip = nullptr) {
   const Params& p = ctx->p;
   if (p.method == 0) {
     launch_copy(z, r, ctx->n, ctx->stream);
@@ -2385,7 +2393,7 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWo
   }
   const double* fine_src = r;
   if (fine && p.fine_method >= 1) {
-    enqueue_subdomain_solves(ctx, r, w.subsol);
+    enqueue_subdomain_solves(ctx, r, w.subsol, skip);
     fine_src = w.subsol;
   }
   if (dot)
@@ -2398,8 +2406,9 @@ static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, const PrecWo
                        ctx->stream);
 }
 
-static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, bool dot = false) {
-  enqueue_prec(ctx, r, z, ctx_work(ctx), dot);
+static void enqueue_prec(doug_ctx* ctx, const double* r, double* z, bool dot = false,
+                         const double* skip = nullptr) {
+  enqueue_prec(ctx, r, z, ctx_work(ctx), dot, skip);
 }
 
 int doug_spmv_device(doug_ctx* ctx, const double* dx, double* dy) {
@@ -2612,21 +2621,48 @@ static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, doubl
   double init_norm = scal[S_RES];
   if (init_norm == 0.0) init_norm = 1.0;
   std::vector<double> alpha((size_t)std::min(maxit, 1 << 20) + 2, 0.0), beta(alpha.size(), 0.0);
-  int it = 0;
-  double ratio = 1.0;
+  // The convergence test runs on the device (k_reduce_finish): the host
+  // keeps the next iteration queued while it waits for the current one, so
+  // the GPU does not idle between iterations.  An iteration queued after
+  // the test passed leaves x and r unchanged and skips the subdomain solves.
+  const int64_t cap = (int64_t)alpha.size() - 2;
+  if (cap > ctx->hist_cap) {
+    int rc = dalloc(ctx, &ctx->d_hist, (size_t)cap * 3);
+    if (rc) return rc;
+    ctx->hist_cap = cap;
+  }
+  scal[S_DONE] = 0.0;
+  scal[S_ITER] = 0.0;
+  scal[S_THR] = tol * tol;
+  scal[S_INIT] = init_norm;
+  scal[S_MAXIT] = (double)maxit;
+  scal[S_HCAP] = (double)cap;
+  HIPCHK(hipMemcpyAsync(ctx->d_scal + S_DONE, scal + S_DONE, (S_HCAP - S_DONE + 1) * sizeof(double),
+                        hipMemcpyHostToDevice, st));
+  const double* done_flag = ctx->d_scal + S_DONE;
   // iteration body (pcg.F90:387-435); from the second iteration on it is
   // replayed from a captured HIP graph
   auto enqueue_iteration = [&](bool first) {
-    enqueue_prec(ctx, ctx->d_r, ctx->d_z, true);  // with rho = r.z and beta
+    enqueue_prec(ctx, ctx->d_r, ctx->d_z, true, done_flag);  // with rho = r.z and beta
     launch_p_update(n, ctx->d_z, ctx->d_p, ctx->d_scal, first ? 1 : 0, st);
     enqueue_spmv(ctx, ctx->d_p, ctx->d_q);
     launch_dot(ctx->d_p, ctx->d_q, n, ctx->d_partial, nullptr, 2, ctx->d_scal, st);
-    launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_scal, st);
-    (void)hipMemcpyAsync(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st);
+    launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_scal, st, ctx->d_hist);
   };
-  while (ratio > tol * tol && it < maxit) {
-    it++;
-    if (it >= 2 && ctx->use_graph) {
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (hipEvent_t& e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t* e;
+    ~EventGuard() {
+      for (int k = 0; k < 2; k++)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    }
+  } guard{ev};
+  int issued = 0;
+  // queue iteration issued + 1 and a copy of the scalar state behind it
+  auto issue = [&]() -> int {
+    issued++;
+    if (issued >= 2 && ctx->use_graph) {
       if (!ctx->iter_graph) {
         hipGraph_t g;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2638,17 +2674,37 @@ static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, doubl
       }
       HIPCHK(hipGraphLaunch(ctx->iter_graph, st));
     } else {
-      enqueue_iteration(it == 1);
+      enqueue_iteration(issued == 1);
     }
+    double* hs = scal + (issued & 1) * S_N;
+    HIPCHK(hipMemcpyAsync(hs, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ev[issued & 1], st));
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    if ((size_t)it < alpha.size()) {
-      alpha[it] = scal[S_ALPHA];
-      beta[it] = scal[S_BETA];
+    return DOUG_OK;
+  };
+  // no iteration at all where the loop test fails at the start (ratio 1)
+  bool done = maxit <= 0 || !(1.0 > tol * tol);
+  int rc = done ? DOUG_OK : issue();
+  if (rc) return rc;
+  for (int checked = 0; !done;) {
+    if (issued < maxit && (rc = issue())) return rc;
+    checked++;
+    HIPCHK(hipEventSynchronize(ev[checked & 1]));
+    done = scal[(checked & 1) * S_N + S_DONE] != 0.0;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  int it = 0;
+  if (issued > 0) {
+    HIPCHK(hipMemcpy(scal, ctx->d_scal, S_N * sizeof(double), hipMemcpyDeviceToHost));
+    it = (int)scal[S_ITER];
+    const int nh = (int)std::min<int64_t>(it, cap);
+    std::vector<double> h((size_t)nh * 3);
+    if (nh > 0) HIPCHK(hipMemcpy(h.data(), ctx->d_hist, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 1; k <= nh; k++) {
+      alpha[k] = h[3 * (k - 1)];
+      beta[k] = h[3 * (k - 1) + 1];
+      if (res_hist) res_hist[k - 1] = std::sqrt(h[3 * (k - 1) + 2]);
     }
-    double res = scal[S_RES];
-    ratio = res / init_norm;
-    if (res_hist) res_hist[it - 1] = std::sqrt(res);
   }
   if (it_out) *it_out = it;
   if (cond) *cond = lanczos_cond(std::min<int>(it, (int)alpha.size() - 2), alpha, beta);

@@ -564,13 +564,15 @@ __device__ __forceinline__ T reduce_scatter16(T (&v)[16], int lane) {
 // ------------------------------------------------------------------------
 // Forward then backward substitution for one subdomain per block, in the
 // factor's precision T; the right-hand side is gathered from (and the
-// solution written to) FP64 vectors.
+// solution written to) FP64 vectors.  Nothing when *skip is set (a PCG
+// iteration queued after the device's convergence test passed).
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
                                               const BlockDesc* __restrict__ d,
                                               const int32_t* __restrict__ inds,
                                               const double* __restrict__ rhs,
-                                              double* __restrict__ sol) {
+                                              double* __restrict__ sol, const double* skip) {
+  if (skip && *skip != 0.0) return;
   // the subdomain vector: nt*64 entries in dynamic LDS sized for the largest block
   extern __shared__ __align__(16) unsigned char trsv_dyn[];
   T* y = reinterpret_cast<T*>(trsv_dyn);
@@ -909,6 +911,8 @@ __global__ void k_overlap_sum(int n, const int32_t* __restrict__ occ_ptr,
 // several-column kernels) before the scalar recurrence.
 // ------------------------------------------------------------------------
 enum { S_RHO = 0, S_RHO_PREV = 1, S_BETA = 2, S_PQ = 3, S_ALPHA = 4, S_RES = 5 };
+// the single-vector PCG's convergence test on the device (see k_reduce_finish)
+enum { S_DONE = 8, S_ITER = 9, S_THR = 10, S_INIT = 11, S_MAXIT = 12, S_HCAP = 13 };
 
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -927,6 +931,11 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // wavefront's by a butterfly, the 16 wavefronts' in order) and applies the
 // scalar recurrence of `mode`: 0 the sum to out,
 // 1 rho = r.z with beta, 2 p.q with alpha, 3 the squared residual norm.
+// Mode 3 with out (a PCG iteration's residual) also keeps the iteration on
+// the device: unless S_DONE is set, it counts the iteration in S_ITER,
+// writes (alpha, beta, residual) to out[3(it-1)..] for it <= S_HCAP, and
+// sets S_DONE once !(res / S_INIT > S_THR) or it >= S_MAXIT, the host's
+// test; after that the iteration's other kernels leave x and r alone.
 __global__ __launch_bounds__(1024) void k_reduce_finish(const double* __restrict__ partial, int count,
                                                         double* out, int mode, double* scal) {
   __shared__ double sh[16];
@@ -949,8 +958,18 @@ __global__ __launch_bounds__(1024) void k_reduce_finish(const double* __restrict
     } else if (mode == 2) {  // p.q
       scal[S_PQ] = s;
       scal[S_ALPHA] = scal[S_RHO] / s;
-    } else {
+    } else if (!out) {
       scal[S_RES] = s;
+    } else if (scal[S_DONE] == 0.0) {
+      scal[S_RES] = s;
+      const int it = (int)scal[S_ITER] + 1;
+      scal[S_ITER] = it;
+      if (it <= (int)scal[S_HCAP]) {
+        out[3 * (it - 1)] = scal[S_ALPHA];
+        out[3 * (it - 1) + 1] = scal[S_BETA];
+        out[3 * (it - 1) + 2] = s;
+      }
+      if (!(s / scal[S_INIT] > scal[S_THR]) || it >= (int)scal[S_MAXIT]) scal[S_DONE] = 1.0;
     }
   }
 }
@@ -1001,14 +1020,17 @@ __global__ __launch_bounds__(256) void k_p_update_chunks(int n, const double* __
   }
 }
 
+// x += alpha p, r -= alpha q and r.r partials; nothing once the device's
+// convergence test has passed (S_DONE).
 __global__ __launch_bounds__(256) void k_xr_update_chunks(int n, double* __restrict__ x,
                                                           double* __restrict__ r,
                                                           const double* __restrict__ p,
                                                           const double* __restrict__ q,
-                                                          const double* __restrict__ alpha_p,
+                                                          const double* __restrict__ scal,
                                                           double* partial) {
   __shared__ double sh[4];
-  const double alpha = *alpha_p;
+  if (scal[S_DONE] != 0.0) return;
+  const double alpha = scal[S_ALPHA];
   const int i0 = blockIdx.x * kVecChunk + threadIdx.x;
   double xv[kVecPer], rv[kVecPer], pv[kVecPer], qv[kVecPer];
 #pragma unroll
@@ -2234,10 +2256,10 @@ void prepare_trsv(int maxnt) {
 
 template <typename T>
 void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
-                 const double* r, double* sol, hipStream_t st) {
+                 const double* r, double* sol, hipStream_t st, const double* skip) {
   if (nblocks <= 0) return;
   const size_t bytes = (size_t)maxnt * kTile * sizeof(T);
-  hipLaunchKernelGGL(k_trsv<T>, dim3(nblocks), dim3(256), bytes, st, tiles, d, inds, r, sol);
+  hipLaunchKernelGGL(k_trsv<T>, dim3(nblocks), dim3(256), bytes, st, tiles, d, inds, r, sol, skip);
 }
 
 #define DOUG_INSTANTIATE(T)                                                                    \
@@ -2249,7 +2271,7 @@ void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, con
   template void launch_chol_diag<T>(T*, const BlockDesc*, int, int, int*, hipStream_t, bool);  \
   template void launch_chol_offdiag<T>(T*, const BlockDesc*, int, int, int, hipStream_t, bool); \
   template void launch_trsv<T>(const T*, const BlockDesc*, int, int, const int32_t*,          \
-                               const double*, double*, hipStream_t);                          \
+                               const double*, double*, hipStream_t, const double*);           \
   template void prepare_trsv<T>(int);
 DOUG_INSTANTIATE(double)
 DOUG_INSTANTIATE(float)
@@ -2505,11 +2527,10 @@ void launch_p_update(int n, const double* z, double* p, const double* scal, int 
 }
 
 void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
-                      double* partial, double* scal, hipStream_t st) {
+                      double* partial, double* scal, hipStream_t st, double* hist) {
   const int nb = (int)cdiv(n, kVecChunk);
-  hipLaunchKernelGGL(k_xr_update_chunks, dim3(nb), dim3(256), 0, st, n, x, r, p, q,
-                     scal + S_ALPHA, partial);
-  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, st, partial, nb, nullptr, 3, scal);
+  hipLaunchKernelGGL(k_xr_update_chunks, dim3(nb), dim3(256), 0, st, n, x, r, p, q, scal, partial);
+  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, st, partial, nb, hist, 3, scal);
 }
 
 void launch_overlap_sum_dot(int n, const int32_t* occ_ptr, const int32_t* occ_pos, const double* sol,

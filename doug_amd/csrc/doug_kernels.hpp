@@ -124,8 +124,9 @@ void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, co
 template <typename T>
 void prepare_trsv(int maxnt);
 template <typename T>
+// skip (device, optional): the launch does nothing when *skip != 0.
 void launch_trsv(const T* tiles, const BlockDesc* d, int nblocks, int maxnt, const int32_t* inds,
-                 const double* r, double* sol, hipStream_t st);
+                 const double* r, double* sol, hipStream_t st, const double* skip = nullptr);
 // u = W c and c' = W^T u for the coarse inverse factor; `part` holds
 // nt * gemv_chunks(nt) * 64 doubles of partial sums.
 constexpr int kGemvChunk = 8;  // tiles per workgroup
@@ -191,8 +192,11 @@ void launch_dot(const double* a, const double* b, int n, double* partial, double
                 double* scal, hipStream_t st);
 void launch_p_update(int n, const double* z, double* p, const double* scal, int first,
                      hipStream_t st);
+// With hist, the residual's finish keeps the iteration count, the
+// (alpha, beta, residual) history and the convergence flag on the device
+// (scal slots S_DONE.. of the kernels); x and r stay unchanged once set.
 void launch_xr_update(int n, double* x, double* r, const double* p, const double* q,
-                      double* partial, double* scal, hipStream_t st);
+                      double* partial, double* scal, hipStream_t st, double* hist = nullptr);
 void launch_copy(double* dst, const double* src, int n, hipStream_t st);
 // dst(i, c) = src(i, c) for c < ncols, 0 for ncols <= c < ldd (n rows;
 // leading dimensions ldd and lds)
