@@ -1906,9 +1906,7 @@ static int device_fine(doug_ctx* ctx, hipEvent_t ev0, hipEvent_t ev1) {
   if ((rc = dalloc(ctx, &ctx->d_subsol, ctx->sinds.size()))) return rc;
   for (double** v : {&ctx->d_x, &ctx->d_r, &ctx->d_z, &ctx->d_p, &ctx->d_q, &ctx->d_b, &ctx->d_tmp})
     if ((rc = dalloc(ctx, v, (size_t)n))) return rc;
-  // partials: the vector reductions', or one per SpMV row block (p.q)
-  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) ||
-      (rc = dalloc(ctx, &ctx->d_partial, (size_t)std::max<int64_t>(vec_partials(n), n + 1))) ||
+  if ((rc = dalloc(ctx, &ctx->d_scal, S_N)) || (rc = dalloc(ctx, &ctx->d_partial, (size_t)vec_partials(n))) ||
       (rc = dalloc(ctx, &ctx->d_counter, 1)))
     return rc;
   HIPCHK(hipMemsetAsync(ctx->d_counter, 0, sizeof(unsigned), ctx->stream));
@@ -2331,10 +2329,9 @@ int doug_get_coarse_matrix(const doug_ctx* ctx, int32_t* rowptr, int32_t* cols, 
 }
 
 // ---------------------------------------------------------------------------
-// y = A x; with pq, PCG's p.q = x.y and alpha as well (d_scal)
-static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y, bool pq = false) {
+static void enqueue_spmv(doug_ctx* ctx, const double* x, double* y) {
   launch_spmv(ctx->nrb_a, ctx->d_rowblk_a, ctx->d_ptr, ctx->d_col, ctx->d_val, x, y, ctx->stream,
-              ctx->spmv_shape, pq ? ctx->d_partial : nullptr, pq ? ctx->d_scal : nullptr);
+              ctx->spmv_shape);
 }
 
 // Work vectors of one preconditioner apply: the subdomain vectors and the
@@ -2648,7 +2645,8 @@ static int pcg_core(doug_ctx* ctx, double tol, int maxit, int32_t* it_out, doubl
   auto enqueue_iteration = [&](bool first) {
     enqueue_prec(ctx, ctx->d_r, ctx->d_z, true, done_flag);  // with rho = r.z and beta
     launch_p_update(n, ctx->d_z, ctx->d_p, ctx->d_scal, first ? 1 : 0, st);
-    enqueue_spmv(ctx, ctx->d_p, ctx->d_q, true);  // with p.q and alpha
+    enqueue_spmv(ctx, ctx->d_p, ctx->d_q);
+    launch_dot(ctx->d_p, ctx->d_q, n, ctx->d_partial, nullptr, 2, ctx->d_scal, st);
     launch_xr_update(n, ctx->d_x, ctx->d_r, ctx->d_p, ctx->d_q, ctx->d_partial, ctx->d_scal, st, ctx->d_hist);
   };
   hipEvent_t ev[2] = {nullptr, nullptr};

@@ -838,21 +838,16 @@ __device__ __forceinline__ int xcd_swizzle(int b, int nb) {
 
 // CSR-stream SpMV: the block's products val*x[col] are formed with coalesced
 // loads into LDS (8 per thread), then each thread sums one row in entry
-// order, so every shape gives the same bits.  TH = rows per block.  With
-// DOT, partial[b] = sum over the block's rows of x[row] y[row] (PCG's p.q),
-// each wavefront's by a butterfly and the wavefronts' in order.
-template <int TH, bool SWZ, bool DOT = false>
+// order, so every shape gives the same bits.  TH = rows per block.
+template <int TH, bool SWZ>
 __global__ __launch_bounds__(TH) void k_spmv(const int32_t* __restrict__ rowblk,
                                              const int32_t* __restrict__ ptr,
                                              const int32_t* __restrict__ col,
                                              const double* __restrict__ val,
                                              const double* __restrict__ x,
-                                             double* __restrict__ y, int nb,
-                                             double* __restrict__ partial = nullptr) {
+                                             double* __restrict__ y, int nb) {
   constexpr int CH = 8 * TH;
   __shared__ double prod[CH];
-  __shared__ double wsum[TH / 64];
-  double xy = 0.0;  // this thread's row's x y (DOT)
   const int b = SWZ ? xcd_swizzle(blockIdx.x, nb) : (int)blockIdx.x;
   const int r0 = rowblk[b], r1 = rowblk[b + 1];
   const int e0 = ptr[r0], e1 = ptr[r1];
@@ -876,7 +871,6 @@ __global__ __launch_bounds__(TH) void k_spmv(const int32_t* __restrict__ rowblk,
       }
       for (; e < ee; e++) acc = acc + prod[e];
       y[row] = acc;
-      if (DOT) xy = x[row] * acc;
     }
   } else {
     // a single long row, streamed through LDS chunk by chunk
@@ -889,22 +883,7 @@ __global__ __launch_bounds__(TH) void k_spmv(const int32_t* __restrict__ rowblk,
         for (int e = c0; e < c1; e++) acc = acc + prod[e - c0];
       __syncthreads();
     }
-    if (t == 0) {
-      y[r0] = acc;
-      if (DOT) xy = x[r0] * acc;
-    }
-  }
-  if (DOT) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) xy += __shfl_xor(xy, o);
-    if ((t & 63) == 0) wsum[t >> 6] = xy;
-    __syncthreads();
-    if (t == 0) {
-      double s = 0.0;
-#pragma unroll
-      for (int k = 0; k < TH / 64; k++) s += wsum[k];
-      partial[b] = s;
-    }
+    if (t == 0) y[r0] = acc;
   }
 }
 
@@ -2479,22 +2458,9 @@ void launch_sgs(const SgsDesc* d, int nsub, int mmax, const int32_t* levptr, con
 }
 
 void launch_spmv(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
-                 const double* val, const double* x, double* y, hipStream_t st, int shape,
-                 double* partial, double* scal) {
+                 const double* val, const double* x, double* y, hipStream_t st, int shape) {
   if (nrb <= 0) return;
   const dim3 g(nrb);
-  if (partial) {  // with p.q: block partials, then the recurrence's alpha
-    switch (shape) {
-      case 0: hipLaunchKernelGGL((k_spmv<128, false, true>), g, dim3(128), 0, st, rowblk, ptr, col, val, x, y, nrb, partial); break;
-      case 1: hipLaunchKernelGGL((k_spmv<128, true, true>), g, dim3(128), 0, st, rowblk, ptr, col, val, x, y, nrb, partial); break;
-      case 3: hipLaunchKernelGGL((k_spmv<256, true, true>), g, dim3(256), 0, st, rowblk, ptr, col, val, x, y, nrb, partial); break;
-      case 4: hipLaunchKernelGGL((k_spmv<512, false, true>), g, dim3(512), 0, st, rowblk, ptr, col, val, x, y, nrb, partial); break;
-      case 5: hipLaunchKernelGGL((k_spmv<512, true, true>), g, dim3(512), 0, st, rowblk, ptr, col, val, x, y, nrb, partial); break;
-      default: hipLaunchKernelGGL((k_spmv<256, false, true>), g, dim3(256), 0, st, rowblk, ptr, col, val, x, y, nrb, partial); break;
-    }
-    hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(1024), 0, st, partial, nrb, nullptr, 2, scal);
-    return;
-  }
   switch (shape) {
     case 0: hipLaunchKernelGGL((k_spmv<128, false>), g, dim3(128), 0, st, rowblk, ptr, col, val, x, y, nrb); break;
     case 1: hipLaunchKernelGGL((k_spmv<128, true>), g, dim3(128), 0, st, rowblk, ptr, col, val, x, y, nrb); break;

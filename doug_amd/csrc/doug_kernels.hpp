@@ -113,12 +113,9 @@ void launch_map_index(int32_t* idx, int64_t n, const int32_t* map, hipStream_t s
 constexpr int kSpmvShapes = 6;
 constexpr int kSpmvDefaultShape = 2;
 constexpr int spmv_shape_rows(int shape) { return 128 << (shape >> 1); }
-// With partial (nblocks_rows entries) and scal: also PCG's p.q = x.y from
-// fixed-order block partials, and alpha = rho / p.q in scal (mode 2 of the
-// single-vector reductions).
 void launch_spmv(int nblocks_rows, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                  const double* val, const double* x, double* y, hipStream_t st,
-                 int shape = kSpmvDefaultShape, double* partial = nullptr, double* scal = nullptr);
+                 int shape = kSpmvDefaultShape);
 // Batched forward/backward substitution, one block per subdomain: gathers
 // r(inds), solves L L^T u = r in the factor's precision, writes u to sol
 // (FP64, concatenated layout).
