@@ -247,7 +247,9 @@ int doug_prec_apply_device_noalloc(doug_ctx *ctx, const double *dr, double *dz);
  * driver passes zero) and the solution on return.  Stops when
  * ||r_k||^2 / ||r_0||^2 <= tol^2 or after maxit iterations (maxit == -1 means
  * 100000).  cond receives the Lanczos condition estimate; res_hist (optional,
- * maxit entries) the residual norm of every iteration. */
+ * maxit entries) the residual norm of every iteration, for the first 2^20
+ * iterations.  The stopping test runs on the GPU; an iteration queued behind
+ * it after it passed leaves x unchanged. */
 int doug_pcg(doug_ctx *ctx, const double *b, double *x, double tol, int32_t maxit, int32_t *it,
              double *cond, double *res_hist);
 int doug_pcg_device(doug_ctx *ctx, const double *db, double *dx, double tol, int32_t maxit,
