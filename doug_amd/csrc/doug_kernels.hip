@@ -2097,18 +2097,28 @@ __device__ __forceinline__ double2 p_next(double2 zv, double2 pv, double b0, dou
 // gathered node (the same expression as k_p_update_multi, so q is the same
 // bits as the unfused sequence), p_new written for the block's own rows,
 // and p.q per column from fixed-order block partials (one per row block).  p_old and p_new are
-// separate buffers: rows of other blocks still read p_old.  Entries of a
-// row are taken eight at a time, loads first, then added in entry order;
-// the np threads of a row read its entries from global memory together
-// (one request per wavefront), which leaves LDS free for occupancy.
+// separate buffers: rows of other blocks still read p_old.  The block's
+// entries are staged in LDS (as in k_spmv_multi2); a row's are taken four
+// at a time, loads first, then added in entry order.
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv_p_multi2(
     const int32_t* __restrict__ rowblk, const int32_t* __restrict__ ptr,
     const int32_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ z,
     const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q, int nrhs,
     int first, double* partial, const double* scal) {
+  __shared__ double sval[kSpmvChunk];
+  __shared__ int32_t scol[kSpmvChunk];
   __shared__ double sh[512];
   const int r0 = rowblk[blockIdx.x], r1 = rowblk[blockIdx.x + 1];
+  const int e0 = ptr[r0], e1 = ptr[r1];
   const int t = threadIdx.x, np = nrhs >> 1;
+  const bool staged = e1 - e0 <= kSpmvChunk;
+  if (staged) {
+    for (int e = e0 + t; e < e1; e += kSpmvThreads) {
+      sval[e - e0] = val[e];
+      scol[e - e0] = col[e];
+    }
+    __syncthreads();
+  }
   // np divides the block size, so the column pair of a thread is fixed
   const int pc = t % np;
   const double* s0 = scal + 2 * pc * 8;
@@ -2123,30 +2133,34 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv_p_multi2(
     const int row = r0 + idx / np;
     const int eb = ptr[row], ee = ptr[row + 1];
     double2 acc = make_double2(0.0, 0.0);
-    for (int e = eb; e < ee; e += 8) {
-      const int m = min(8, ee - e);
-      int k[8];
-      double a[8];
-      double2 zv[8], pv[8];
+    int e = eb;
+    for (; e + 4 <= ee; e += 4) {
+      int k[4];
+      double a[4];
+      double2 zv[4], pv[4];
 #pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (u < m) {
-          k[u] = col[e + u];
-          a[u] = val[e + u];
-        }
+      for (int u = 0; u < 4; u++) {
+        k[u] = staged ? scol[e + u - e0] : col[e + u];
+        a[u] = staged ? sval[e + u - e0] : val[e + u];
+      }
 #pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (u < m) {
-          zv[u] = z2[(int64_t)k[u] * np + pc];
-          pv[u] = po2[(int64_t)k[u] * np + pc];
-        }
+      for (int u = 0; u < 4; u++) {
+        zv[u] = z2[(int64_t)k[u] * np + pc];
+        pv[u] = po2[(int64_t)k[u] * np + pc];
+      }
 #pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (u < m) {
-          const double2 pk = p_next(zv[u], pv[u], b0, b1, a0, a1, fst);
-          acc.x = acc.x + a[u] * pk.x;
-          acc.y = acc.y + a[u] * pk.y;
-        }
+      for (int u = 0; u < 4; u++) {
+        const double2 pk = p_next(zv[u], pv[u], b0, b1, a0, a1, fst);
+        acc.x = acc.x + a[u] * pk.x;
+        acc.y = acc.y + a[u] * pk.y;
+      }
+    }
+    for (; e < ee; e++) {
+      const int k = staged ? scol[e - e0] : col[e];
+      const double a = staged ? sval[e - e0] : val[e];
+      const double2 pk = p_next(z2[(int64_t)k * np + pc], po2[(int64_t)k * np + pc], b0, b1, a0, a1, fst);
+      acc.x = acc.x + a * pk.x;
+      acc.y = acc.y + a * pk.y;
     }
     const int64_t o = (int64_t)row * np + pc;
     const double2 pr = p_next(z2[o], po2[o], b0, b1, a0, a1, fst);
