@@ -1190,40 +1190,53 @@ void rcm_block(const Csr& A, const std::vector<int>& inds, int s0, int s, std::v
       lb = clb;
       le = cle;
     }
-    // Cuthill-McKee: neighbours in ascending degree (then index)
-    size_t head = order.size();
-    order.push_back(root);
-    placed[root] = 1;
-    while (head < order.size()) {
-      int v = order[head++];
-      nbr.resize((size_t)(g.ptr[v + 1] - g.ptr[v]) + 1);
-      int m = 0;
-      for (int p = g.ptr[v]; p < g.ptr[v + 1]; p++) {
-        const int u = g.adj[p];
-        nbr[m] = u;
-        m += !placed[u];
-        placed[u] = 1;
-      }
-      nbr.resize((size_t)m);
-      if (nbr.size() <= 8) {
-        // (degree, index) packed into one ordered key; eight slots through
-        // the sorting network, unused ones last
-        int64_t kb[8];
-        const int m = (int)nbr.size();
-        for (int q = 0; q < 8; q++)
-          kb[q] = q < m ? ((int64_t)g.deg[nbr[q]] << 32) | (uint32_t)nbr[q] : INT64_MAX;
-        int64_t key[8] = {kb[0], kb[1], kb[2], kb[3], kb[4], kb[5], kb[6], kb[7]};
+    // Cuthill-McKee: neighbours in ascending degree (then index), appended
+    // to `order` (sized for the block) through raw pointers
+    int head = (int)order.size(), tail = head;
+    order.resize((size_t)s);
+    int* ord = order.data();
+    char* pl = placed.data();
+    const int* adj = g.adj.data();
+    const int* gp = g.ptr.data();
+    const int* dg = g.deg.data();
+    ord[tail++] = root;
+    pl[root] = 1;
+    while (head < tail) {
+      const int v = ord[head++], p0 = gp[v], d = gp[v + 1] - p0;
+      if (d <= 8) {
+        // (degree, index) packed into one ordered key, an entry already
+        // placed (or a repeat of one in this row) keyed last; eight slots
+        // through the sorting network
+        int64_t key[8];
+        int m = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int u = q < d ? adj[p0 + q] : 0;
+          const bool fresh = q < d && !pl[u];
+          key[q] = fresh ? ((int64_t)dg[u] << 32) | (uint32_t)u : INT64_MAX;
+          if (q < d) pl[u] = 1;
+          m += fresh;
+        }
         sort8_asc(key);
-        kb[0] = key[0], kb[1] = key[1], kb[2] = key[2], kb[3] = key[3];
-        kb[4] = key[4], kb[5] = key[5], kb[6] = key[6], kb[7] = key[7];
-        for (int q = 0; q < m; q++) order.push_back((int)(uint32_t)kb[q]);
+        for (int q = 0; q < m; q++) ord[tail + q] = (int)(uint32_t)key[q];
+        tail += m;
       } else {
-        std::sort(nbr.begin(), nbr.end(), [&](int x, int y) {
-          return g.deg[x] != g.deg[y] ? g.deg[x] < g.deg[y] : x < y;
+        nbr.resize((size_t)d);
+        int m = 0;
+        for (int p = p0; p < p0 + d; p++) {
+          const int u = adj[p];
+          nbr[m] = u;
+          m += !pl[u];
+          pl[u] = 1;
+        }
+        std::sort(nbr.begin(), nbr.begin() + m, [&](int x, int y) {
+          return dg[x] != dg[y] ? dg[x] < dg[y] : x < y;
         });
-        order.insert(order.end(), nbr.begin(), nbr.end());
+        for (int q = 0; q < m; q++) ord[tail + q] = nbr[q];
+        tail += m;
       }
     }
+    order.resize((size_t)tail);
   }
   for (int q = 0; q < s; q++) out[s0 + q] = inds[s0 + order[s - 1 - q]];
 }
