@@ -2882,8 +2882,9 @@ static int pcg_batch_core(doug_ctx* ctx, int nrhs, double tol, int maxit, int32_
   // With an even column count the iteration runs fused: the overlap sum
   // forms rho = r.z, and one SpMV pass forms p_new = z + beta p_old, q and
   // p.q, reading p from one buffer and writing the other (parity par of the
-  // iteration).  Iterates are the same bits as the unfused sequence below;
-  // only the summation order of p.q differs.
+  // iteration); that pass also adds the previous iteration's alpha p to x,
+  // and the last one is added after the loop.  Iterates are the same bits as
+  // the unfused sequence below; only the summation order of p.q differs.
   const bool fused = nrp % 2 == 0;
   double* const P[2] = {ctx->d_Pm, ctx->d_Pm2};
   // iteration body; from the second iteration on it is replayed from a
@@ -2893,7 +2894,8 @@ static int pcg_batch_core(doug_ctx* ctx, int nrhs, double tol, int maxit, int32_
     if (fused) {
       enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm, true);
       launch_spmv_p_multi2(ctx->nrb, ctx->d_rowblk, ctx->d_ptr, ctx->d_col, ctx->d_val, ctx->d_Zm, P[par], p,
-                           ctx->d_Qm, nrp, first ? 1 : 0, ctx->d_partialm, ctx->d_counterm, ctx->d_scalm, st);
+                           ctx->d_Qm, nrp, first ? 1 : 0, ctx->d_partialm, ctx->d_counterm, ctx->d_scalm,
+                           ctx->d_Xm, st);
     } else {
       p = P[0];
       enqueue_prec_multi(ctx, nrp, ctx->d_Rm, ctx->d_Zm);
@@ -2947,6 +2949,11 @@ static int pcg_batch_core(doug_ctx* ctx, int nrhs, double tol, int maxit, int32_
     }
     if (changed)
       HIPCHK(hipMemcpyAsync(ctx->d_scalm, hs, kMaxRhs * 8 * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  // the last iteration's x += alpha p (alpha is 0 for columns frozen before it)
+  if (fused && it > 0) {
+    launch_x_final_multi2(n, nrp, ctx->d_Xm, P[((it - 1) & 1) ^ 1], ctx->d_scalm, st);
+    HIPCHK(hipGetLastError());
   }
   return DOUG_OK;
 }

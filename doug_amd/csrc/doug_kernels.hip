@@ -1904,10 +1904,10 @@ __global__ __launch_bounds__(256) void k_xr_update_multi(int64_t total, int nrhs
   finish_multi(partial, counter, nrhs, 3, scal, sh);
 }
 
+// r -= alpha q per active column, with the r.r partials; x's update waits
+// for the next k_spmv_p_multi2 (or k_x_final_multi2), which reads p anyway.
 __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nrhs,
-                                                          double2* __restrict__ x,
                                                           double2* __restrict__ r,
-                                                          const double2* __restrict__ p,
                                                           const double2* __restrict__ q,
                                                           double* partial, const double* scal) {
   __shared__ double sh[512];
@@ -1919,36 +1919,23 @@ __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nr
     const int c0 = 2 * (int)(start % nr2);
     const double* s0 = scal + c0 * 8;
     const double* s1 = s0 + 8;
-    // a frozen column keeps x and r (alpha 0 leaves them unchanged)
-    const double a0 = s0[SM_ACTIVE] != 0.0 ? s0[S_ALPHA] : 0.0;
-    const double a1 = s1[SM_ACTIVE] != 0.0 ? s1[S_ALPHA] : 0.0;
-    const bool any = s0[SM_ACTIVE] != 0.0 || s1[SM_ACTIVE] != 0.0;
-    const bool f0 = s0[SM_ACTIVE] != 0.0, f1 = s1[SM_ACTIVE] != 0.0;
+    // a frozen column keeps r
+    const bool f0 = s0[SM_ACTIVE] != 0.0, f1 = s1[SM_ACTIVE] != 0.0, any = f0 || f1;
+    const double a0 = f0 ? s0[S_ALPHA] : 0.0, a1 = f1 ? s1[S_ALPHA] : 0.0;
     int64_t e = start;
     // four items per round with every load issued first; the same element
     // updates and the same order of the sums as one item at a time
     for (; e + 3 * stride < total2; e += 4 * stride) {
-      double2 ri[4], pe[4], qe[4], xe[4];
+      double2 ri[4], qe[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) ri[u] = r[e + u * stride];
       if (any) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          pe[u] = p[e + u * stride];
-          qe[u] = q[e + u * stride];
-          xe[u] = x[e + u * stride];
-        }
+        for (int u = 0; u < 4; u++) qe[u] = q[e + u * stride];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          if (f0) {
-            xe[u].x = xe[u].x + a0 * pe[u].x;
-            ri[u].x = ri[u].x - a0 * qe[u].x;
-          }
-          if (f1) {
-            xe[u].y = xe[u].y + a1 * pe[u].y;
-            ri[u].y = ri[u].y - a1 * qe[u].y;
-          }
-          x[e + u * stride] = xe[u];
+          if (f0) ri[u].x = ri[u].x - a0 * qe[u].x;
+          if (f1) ri[u].y = ri[u].y - a1 * qe[u].y;
           r[e + u * stride] = ri[u];
         }
       }
@@ -1961,17 +1948,9 @@ __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nr
     for (; e < total2; e += stride) {
       double2 ri = r[e];
       if (any) {
-        const double2 pe = p[e], qe = q[e];
-        double2 xe = x[e];
-        if (s0[SM_ACTIVE] != 0.0) {
-          xe.x = xe.x + a0 * pe.x;
-          ri.x = ri.x - a0 * qe.x;
-        }
-        if (s1[SM_ACTIVE] != 0.0) {
-          xe.y = xe.y + a1 * pe.y;
-          ri.y = ri.y - a1 * qe.y;
-        }
-        x[e] = xe;
+        const double2 qe = q[e];
+        if (f0) ri.x = ri.x - a0 * qe.x;
+        if (f1) ri.y = ri.y - a1 * qe.y;
         r[e] = ri;
       }
       v0 = v0 + ri.x * ri.x;
@@ -1979,6 +1958,21 @@ __global__ __launch_bounds__(256) void k_xr_update_multi2(int64_t total2, int nr
     }
   }
   block_colsums2(v0, v1, nrhs, sh, partial);
+}
+
+// x += alpha p, the last iteration's deferred update (alpha is 0 for the
+// columns frozen before it)
+__global__ void k_x_final_multi2(int64_t total2, int nrhs, double2* __restrict__ x,
+                                 const double2* __restrict__ p, const double* __restrict__ scal) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total2) return;
+  const int c0 = 2 * (int)(e % (nrhs >> 1));
+  const double al0 = scal[c0 * 8 + S_ALPHA], al1 = scal[(c0 + 1) * 8 + S_ALPHA];
+  double2 xe = x[e];
+  const double2 pe = p[e];
+  xe.x = xe.x + al0 * pe.x;
+  xe.y = xe.y + al1 * pe.y;
+  x[e] = xe;
 }
 
 // Fused forms of the batched iteration (column pairs, nrhs even).
@@ -2096,7 +2090,9 @@ __device__ __forceinline__ double2 p_next(double2 zv, double2 pv, double b0, dou
 // q = A p_new with p_new formed on the fly from z and p_old at every
 // gathered node (the same expression as k_p_update_multi, so q is the same
 // bits as the unfused sequence), p_new written for the block's own rows,
-// and p.q per column from fixed-order block partials (one per row block).  p_old and p_new are
+// and p.q per column from fixed-order block partials (one per row block).
+// The own rows also take the previous iteration's x += alpha p_old, which
+// k_xr_update_multi2 leaves to this pass (k_x_final_multi2 adds the last).  p_old and p_new are
 // separate buffers: rows of other blocks still read p_old.  The block's
 // entries are staged in LDS (as in k_spmv_multi2); a row's are taken four
 // at a time, loads first, then added in entry order.
@@ -2104,7 +2100,7 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv_p_multi2(
     const int32_t* __restrict__ rowblk, const int32_t* __restrict__ ptr,
     const int32_t* __restrict__ col, const double* __restrict__ val, const double* __restrict__ z,
     const double* __restrict__ pold, double* __restrict__ pnew, double* __restrict__ q, int nrhs,
-    int first, double* partial, const double* scal) {
+    int first, double* partial, const double* scal, double* __restrict__ x) {
   __shared__ double sval[kSpmvChunk];
   __shared__ int32_t scol[kSpmvChunk];
   __shared__ double sh[512];
@@ -2125,6 +2121,9 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv_p_multi2(
   const double* s1 = s0 + 8;
   const bool a0 = s0[SM_ACTIVE] != 0.0, a1 = s1[SM_ACTIVE] != 0.0, fst = first != 0;
   const double b0 = s0[S_BETA], b1 = s1[S_BETA];
+  // the previous iteration's x += alpha p, deferred to this pass (p_old is
+  // that p; a column frozen since has alpha 0 from its next p.q)
+  const double al0 = s0[S_ALPHA], al1 = s1[S_ALPHA];
   const double2* z2 = reinterpret_cast<const double2*>(z);
   const double2* po2 = reinterpret_cast<const double2*>(pold);
   const int work = (r1 - r0) * np;
@@ -2163,8 +2162,15 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv_p_multi2(
       acc.y = acc.y + a * pk.y;
     }
     const int64_t o = (int64_t)row * np + pc;
-    const double2 pr = p_next(z2[o], po2[o], b0, b1, a0, a1, fst);
+    const double2 pk = po2[o];
+    const double2 pr = p_next(z2[o], pk, b0, b1, a0, a1, fst);
     reinterpret_cast<double2*>(pnew)[o] = pr;
+    if (!fst) {
+      double2 xe = reinterpret_cast<double2*>(x)[o];
+      xe.x = xe.x + al0 * pk.x;
+      xe.y = xe.y + al1 * pk.y;
+      reinterpret_cast<double2*>(x)[o] = xe;
+    }
     reinterpret_cast<double2*>(q)[o] = acc;
     v0 = v0 + pr.x * acc.x;
     v1 = v1 + pr.y * acc.y;
@@ -2697,11 +2703,9 @@ void launch_p_update_multi(int n, int nrhs, const double* z, double* p, const do
 void launch_xr_update_multi(int n, int nrhs, double* x, double* r, const double* p,
                             const double* q, double* partial, unsigned* counter, double* scal,
                             hipStream_t st) {
-  if (nrhs % 2 == 0) {
+  if (nrhs % 2 == 0) {  // x's update is deferred to launch_spmv_p_multi2 / launch_x_final_multi2
     hipLaunchKernelGGL(k_xr_update_multi2, dim3(kRedBlocksWide), dim3(256), 0, st, (int64_t)n * nrhs / 2,
-                       nrhs, reinterpret_cast<double2*>(x), reinterpret_cast<double2*>(r),
-                       reinterpret_cast<const double2*>(p), reinterpret_cast<const double2*>(q), partial,
-                       scal);
+                       nrhs, reinterpret_cast<double2*>(r), reinterpret_cast<const double2*>(q), partial, scal);
     hipLaunchKernelGGL(k_finish_cols, dim3(nrhs), dim3(256), 0, st, partial, kRedBlocksWide, 3, scal);
   } else
     hipLaunchKernelGGL(k_xr_update_multi, dim3(kRedBlocks), dim3(256), 0, st, (int64_t)n * nrhs, nrhs,
@@ -2720,11 +2724,17 @@ void launch_overlap_sum_dot_multi2(int n, int nrhs, const int32_t* occ_ptr, cons
 
 void launch_spmv_p_multi2(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                           const double* val, const double* z, const double* pold, double* pnew, double* q,
-                          int nrhs, int first, double* partial, unsigned* counter, double* scal,
+                          int nrhs, int first, double* partial, unsigned* counter, double* scal, double* x,
                           hipStream_t st) {
   hipLaunchKernelGGL(k_spmv_p_multi2, dim3(nrb), dim3(kSpmvThreads), 0, st, rowblk, ptr, col, val, z, pold,
-                     pnew, q, nrhs, first, partial, scal);
+                     pnew, q, nrhs, first, partial, scal, x);
   hipLaunchKernelGGL(k_finish_cols, dim3(nrhs), dim3(256), 0, st, partial, nrb, 2, scal);
+}
+
+void launch_x_final_multi2(int n, int nrhs, double* x, const double* p, const double* scal, hipStream_t st) {
+  const int64_t total2 = (int64_t)n * nrhs / 2;
+  hipLaunchKernelGGL(k_x_final_multi2, dim3(cdiv(total2, 256)), dim3(256), 0, st, total2, nrhs,
+                     reinterpret_cast<double2*>(x), reinterpret_cast<const double2*>(p), scal);
 }
 
 void launch_value_grad(int n, const int32_t* ptr, const int32_t* col, const int32_t* src,

@@ -259,10 +259,14 @@ void launch_overlap_sum_dot_multi2(int n, int nrhs, const int32_t* occ_ptr, cons
                                    const double* rt_val, const double* csol, int ldc, double* z,
                                    const double* r, double* partial, unsigned* counter, double* scal,
                                    hipStream_t st);
+// The SpMV form also applies the previous iteration's x += alpha p_old (not
+// on the first), which the column-pair x/r update leaves to it;
+// launch_x_final_multi2 adds the last iteration's after the loop.
 void launch_spmv_p_multi2(int nrb, const int32_t* rowblk, const int32_t* ptr, const int32_t* col,
                           const double* val, const double* z, const double* pold, double* pnew, double* q,
-                          int nrhs, int first, double* partial, unsigned* counter, double* scal,
+                          int nrhs, int first, double* partial, unsigned* counter, double* scal, double* x,
                           hipStream_t st);
+void launch_x_final_multi2(int n, int nrhs, double* x, const double* p, const double* scal, hipStream_t st);
 
 // ---- coarse space on the device (setup; bit-identical to the host build) ----
 // Rows of R^T = S T^T into slots at ptr[j] (capacity = row length of A),
