@@ -1364,23 +1364,48 @@ __global__ void k_panel_sum(const double* __restrict__ part, int nt, int nch, in
   out[(int64_t)p * ldo + c] = s;
 }
 
+// A block's tile-row table packed into LDS, one word per tile row: off in
+// the high 16 bits (a block has at most 256 * 257 / 2 tiles), fc and lr in
+// a byte each (at most 256 tile rows).  Reading it from LDS keeps the
+// solve's address arithmetic off the vector-memory counter, so that the
+// waits for a step's tiles do not also wait for later loads.
+struct PackedRows {
+  const uint32_t* w;
+  __device__ __forceinline__ int fc(int i) const { return __builtin_amdgcn_readfirstlane((int)((w[i] >> 8) & 255u)); }
+  __device__ __forceinline__ int lr(int i) const { return __builtin_amdgcn_readfirstlane((int)(w[i] & 255u)); }
+  // element offset of tile (i, k) from the block's base
+  __device__ __forceinline__ int64_t toff(int i, int k) const {
+    const uint32_t v = w[i];
+    return (int64_t)(__builtin_amdgcn_readfirstlane((int)(v >> 16)) + k -
+                     __builtin_amdgcn_readfirstlane((int)((v >> 8) & 255u))) *
+           kTileElems;
+  }
+};
+
+__device__ __forceinline__ void pack_rows(uint32_t* w, const BlockDesc& b, int nthreads) {
+  for (int i = threadIdx.x; i < b.nt; i += nthreads) {
+    const TileRow r = b.tr[i];
+    w[i] = ((uint32_t)r.off << 16) | ((uint32_t)r.fc << 8) | (uint32_t)r.lr;
+  }
+}
+
 // The tiles a tile row's solve starts with: its diagonal tile and the
 // off-diagonal tiles of its first step (forward: row i, tiles (i, fc_i..);
 // backward, TR: column j, tiles (j+1.., j)).
 template <typename T, int KB, bool TR, int NT>
-__device__ __forceinline__ void load_row_tiles(const T* Lb, const BlockDesc& b, int i, TileRegs<T, NT>& rd,
+__device__ __forceinline__ void load_row_tiles(const T* Lb, const PackedRows& tr, int i, TileRegs<T, NT>& rd,
                                                TileRegs<T, NT> (&ra)[KB]) {
-  rd.load(Lb + toff(b, i, i));
+  rd.load(Lb + tr.toff(i, i));
   if (!TR) {
-    const int fc = b.tr[i].fc;
+    const int fc = tr.fc(i);
 #pragma unroll
     for (int u = 0; u < KB; u++)
-      if (fc + u < i) ra[u].load(Lb + toff(b, i, fc + u));
+      if (fc + u < i) ra[u].load(Lb + tr.toff(i, fc + u));
   } else {
-    const int lr = b.tr[i].lr;
+    const int lr = tr.lr(i);
 #pragma unroll
     for (int u = 0; u < KB; u++)
-      if (i + 1 + u <= lr) ra[u].load(Lb + toff(b, i + 1 + u, i));
+      if (i + 1 + u <= lr) ra[u].load(Lb + tr.toff(i + 1 + u, i));
   }
 }
 
@@ -1404,6 +1429,7 @@ __global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
   __shared__ T As[KB][kTile * LD];
   __shared__ T Ds[kTile * LD];
   __shared__ T Bs[KB][kTile * (NR + 1)];
+  __shared__ uint32_t trw[kMaxTrsvTiles];
   typedef typename Mma<T>::v4 V;
   // CG column groups of NR columns each per subdomain; with two, the groups
   // of one subdomain are workgroups bid and bid + 8, so that they run on the
@@ -1416,6 +1442,9 @@ __global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
     if (sub >= nblocks) return;
   }
   const BlockDesc b = d[sub];
+  pack_rows(trw, b, 64 * WG);
+  __syncthreads();
+  const PackedRows tr{trw};
   // WG = 8: wavefronts w and w + 4 take the same rows, columns 0..NR/2-1 and
   // NR/2..NR-1, so that one staged tile serves the whole panel
   constexpr int NT = 64 * WG, CPW = NR * 4 / WG;
@@ -1433,12 +1462,12 @@ __global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
   // tiles of the next tile row (its diagonal tile and first step), loaded
   // while the current row's last step runs
   TileRegs<T, NT> rdn, ran[KB];
-  load_row_tiles<T, KB, false, NT>(Lb, b, 0, rdn, ran);
+  load_row_tiles<T, KB, false, NT>(Lb, tr, 0, rdn, ran);
   // forward: Y_i = inv(L_ii) (R_i - sum_{k=fc_i}^{i-1} L_ik Y_k)
   for (int i = 0; i < b.nt; i++) {
 #pragma unroll
     for (int c = 0; c < CPW / 16; c++) acc[c] = V{0, 0, 0, 0};
-    const int fc = b.tr[i].fc;
+    const int fc = tr.fc(i);
     {
       // steps of KB off-diagonal tiles; the next step's tiles and panels are
       // loaded into registers while this step's products run
@@ -1472,11 +1501,11 @@ __global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
 #pragma unroll
           for (int u = 0; u < KB; u++)
             if (kn + u < i) {
-              ra[u].load(Lb + toff(b, i, kn + u));
+              ra[u].load(Lb + tr.toff(i, kn + u));
               rb[u].load(Ub + (int64_t)(kn + u) * kTile * NR);
             }
         } else if (i + 1 < b.nt) {
-          load_row_tiles<T, KB, false, NT>(Lb, b, i + 1, rdn, ran);
+          load_row_tiles<T, KB, false, NT>(Lb, tr, i + 1, rdn, ran);
         }
 #pragma unroll
         for (int u = 0; u < KB; u++)
@@ -1504,11 +1533,11 @@ __global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
     __syncthreads();
   }
   // backward: X_j = inv(L_jj)^T (Y_j - sum_{i=j+1}^{lr_j} L_ij^T X_i), X_j over Y_j
-  load_row_tiles<T, KB, true, NT>(Lb, b, b.nt - 1, rdn, ran);
+  load_row_tiles<T, KB, true, NT>(Lb, tr, b.nt - 1, rdn, ran);
   for (int j = b.nt - 1; j >= 0; j--) {
 #pragma unroll
     for (int c = 0; c < CPW / 16; c++) acc[c] = V{0, 0, 0, 0};
-    const int lr = b.tr[j].lr;
+    const int lr = tr.lr(j);
     T* Uj = Ub + (int64_t)j * kTile * NR;
     {
       TileRegs<T, NT> ra[KB], rd = rdn;
@@ -1537,11 +1566,11 @@ __global__ __launch_bounds__(64 * WG) void k_trsm_multi(const T* __restrict__ L,
 #pragma unroll
           for (int u = 0; u < KB; u++)
             if (in + u <= lr) {
-              ra[u].load(Lb + toff(b, in + u, j));
+              ra[u].load(Lb + tr.toff(in + u, j));
               rb[u].load(Ub + (int64_t)(in + u) * kTile * NR);
             }
         } else if (j > 0) {
-          load_row_tiles<T, KB, true, NT>(Lb, b, j - 1, rdn, ran);
+          load_row_tiles<T, KB, true, NT>(Lb, tr, j - 1, rdn, ran);
         }
 #pragma unroll
         for (int u = 0; u < KB; u++)
