@@ -411,6 +411,9 @@ def main():
                     help="kernels: one setup, then a few launches of the copy, SpMV and "
                          "subdomain-solve kernels only (for hardware-counter runs)")
     ap.add_argument("--precision", type=int, default=1, help="factor precision for --mode kernels")
+    ap.add_argument("--batch", action="store_true",
+                    help="--mode kernels: also two iterations of the 32-column batch (eight-wavefront "
+                         "envelope solve, one tile per step)")
     ap.add_argument("--fem-points", type=int, default=1_000_000,
                     help="points of the FEM batching workload (0: skip it)")
     ap.add_argument("--diff-size", type=int, default=64,
@@ -465,6 +468,16 @@ def main():
         for which, reps in ((3, 5), (0, 5), (1, 3)):
             ms = s.time_operator(which, reps)
             log(f"[bench] operator {which}: {ms:.4f} ms")
+        if args.batch:
+            s.prepare_batch()
+            s.set_option(9, 3)
+            s.set_option(6, 1)
+            Bm = torch.tensor(np.stack([P.uniform_rhs(n, args.seed + 1 + k) for k in range(32)], axis=1),
+                              dtype=torch.float64, device=dev).contiguous()
+            Xm = torch.zeros_like(Bm)
+            its_b = s.pcg_batch_device(32, Bm.data_ptr(), Xm.data_ptr(), 1e-8, 2)
+            torch.cuda.synchronize()
+            log(f"[bench] batch: {int(its_b.max())} iterations of 32 columns")
         inf = s.info()
         print(json.dumps({"mode": "kernels", "precision": args.precision, "n": n, "nnz": nnz,
                           "copy_bytes_per_launch": 16 * n, "spmv_bytes": spmv_bytes(n, nnz),

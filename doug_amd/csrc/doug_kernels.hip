@@ -1219,14 +1219,19 @@ struct Vec16<double> {
   static constexpr int n = 2;
 };
 
+// Factor tiles are streamed once per pass: their loads are non-temporal, so
+// that they do not evict the solved panels, which the next rows read again.
 template <typename T, int NT = 256>
 struct TileRegs {
   static constexpr int V = Vec16<T>::n, N = kTileElems / (NT * V);
   typename Vec16<T>::type r[N];
+  typedef T native __attribute__((ext_vector_type(V)));
   __device__ __forceinline__ void load(const T* __restrict__ g) {
 #pragma unroll
-    for (int q = 0; q < N; q++)
-      r[q] = reinterpret_cast<const typename Vec16<T>::type*>(g)[threadIdx.x + NT * q];
+    for (int q = 0; q < N; q++) {
+      const native v = __builtin_nontemporal_load(reinterpret_cast<const native*>(g) + threadIdx.x + NT * q);
+      __builtin_memcpy(&r[q], &v, sizeof(v));
+    }
   }
   __device__ __forceinline__ void store(T* __restrict__ S) const {
 #pragma unroll
