@@ -559,6 +559,14 @@ __device__ __forceinline__ T reduce_scatter16(T (&v)[16], int lane) {
   return v[0];
 }
 
+// A factor tile element, loaded non-temporally: the subdomain solve streams
+// every tile once per pass, and the hint keeps the 6 GB stream from
+// displacing what the iteration's other kernels read again (A, the vectors).
+template <typename T>
+__device__ __forceinline__ T ntl(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 // ------------------------------------------------------------------------
 // Per-iteration kernels
 // ------------------------------------------------------------------------
@@ -600,7 +608,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
     T dg[16], acc[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      dg[q] = Ri[(int64_t)i * kTileElems + q * kTile];
+      dg[q] = ntl(Ri + (int64_t)i * kTileElems + q * kTile);
       acc[q] = (T)0;
     }
     int k = rfc[i];
@@ -609,8 +617,8 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       T a0[16], a1[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
-        a0[q] = A0[q * kTile];
-        a1[q] = A0[kTileElems + q * kTile];
+        a0[q] = ntl(A0 + q * kTile);
+        a1[q] = ntl(A0 + kTileElems + q * kTile);
       }
       const T y0 = y[k * kTile + lane], y1 = y[(k + 1) * kTile + lane];
 #pragma unroll
@@ -620,7 +628,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       const T* A0 = Ri + (int64_t)k * kTileElems;
       const T y0 = y[k * kTile + lane];
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc[q] += A0[q * kTile] * y0;
+      for (int q = 0; q < 16; q++) acc[q] += ntl(A0 + q * kTile) * y0;
     }
     T s = reduce_scatter16(acc, lane);
     if ((lane & 3) == 0) {
@@ -639,7 +647,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
   for (int j = nt - 1; j >= 0; j--) {
     T dg[16];
 #pragma unroll
-    for (int q = 0; q < 16; q++) dg[q] = T0[((int64_t)rbase[j] + j) * kTileElems + q * kTile];
+    for (int q = 0; q < 16; q++) dg[q] = ntl(T0 + ((int64_t)rbase[j] + j) * kTileElems + q * kTile);
     T acc = (T)0;
     const int lr = rlr[j];
     int i = j + 1;
@@ -649,8 +657,8 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       T a0[16], a1[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
-        a0[q] = A0[q * kTile];
-        a1[q] = A1[q * kTile];
+        a0[q] = ntl(A0 + q * kTile);
+        a1[q] = ntl(A1 + q * kTile);
       }
       const T* x0 = y + i * kTile + 16 * w;
 #pragma unroll
@@ -660,7 +668,7 @@ __global__ __launch_bounds__(256) void k_trsv(const T* __restrict__ tiles,
       const T* A0 = T0 + ((int64_t)rbase[i] + j) * kTileElems;
       const T* x0 = y + i * kTile + 16 * w;
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc += A0[q * kTile] * x0[q];
+      for (int q = 0; q < 16; q++) acc += ntl(A0 + q * kTile) * x0[q];
     }
     red[w * kTile + lane] = acc;
     __syncthreads();
