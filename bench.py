@@ -39,8 +39,8 @@ MEASURED_TRAFFIC_128 = {
     # HBM bytes per launch (read + written) from the FETCH_SIZE / WRITE_SIZE runs of
     # profiles/hbm_traffic_pmc_128cube.md (FETCH_SIZE scaled by the copy-kernel factor);
     # the subdomain solve per decomposition of the workload
-    ("k_trsv", 1, "ranks"): 6_053_626_751 + 22_282_240,
-    ("k_trsv", 1, "boxes"): 6_053_640_466 + 22_282_240,
+    ("k_trsv", 1, "ranks"): 6_062_536_148 + 22_282_240,
+    ("k_trsv", 1, "boxes"): 6_062_693_646 + 22_282_240,
     ("k_spmv", 0): 244_787_743 + 16_777_216,
     ("k_spmv", 1): 213_542_137 + 16_777_216,
     ("k_spmv", 2): 222_963_734 + 16_777_216,
